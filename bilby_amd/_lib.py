@@ -1,0 +1,104 @@
+"""ctypes binding of libgwl.so (the C ABI declared in include/gwl.h).
+
+The library is built in-tree by ``python -m bilby_amd.build``.  There is no
+CPU fallback: if the library is missing, importing this module raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libgwl.so")
+
+from .abi import *  # noqa: F401,F403
+from .abi import NPARAM, PARAM_COLUMNS  # noqa: F401
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int32)
+
+
+class EventDesc(ctypes.Structure):
+    _fields_ = [
+        ("n_det", ctypes.c_int32), ("n_freq", ctypes.c_int32),
+        ("duration", ctypes.c_double), ("sampling_frequency", ctypes.c_double),
+        ("start_time", ctypes.c_double),
+        ("mask_lo", _ip), ("mask_hi", _ip),
+        ("strain", _dp), ("psd", _dp), ("detector_tensor", _dp), ("vertex", _dp),
+        ("approximant", ctypes.c_int32),
+        ("wf_minimum_frequency", ctypes.c_double), ("wf_maximum_frequency", ctypes.c_double),
+        ("reference_frequency", ctypes.c_double),
+        ("flags", ctypes.c_uint32),
+        ("time_prior_minimum", ctypes.c_double), ("time_prior_maximum", ctypes.c_double),
+        ("reference_distance", ctypes.c_double),
+    ]
+
+
+class GWLError(RuntimeError):
+    pass
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            "libgwl.so is not built (%s). Run `python -m bilby_amd.build` first; "
+            "there is no CPU fallback for the likelihood." % LIB_PATH)
+    lib = ctypes.CDLL(LIB_PATH)
+    vp = ctypes.c_void_p
+    sig = {
+        "gwl_last_error": ([], ctypes.c_char_p),
+        "gwl_ctx_create": ([ctypes.c_int32, ctypes.POINTER(vp)], ctypes.c_int),
+        "gwl_ctx_destroy": ([vp], ctypes.c_int),
+        "gwl_event_create": ([vp, ctypes.POINTER(EventDesc), ctypes.POINTER(vp),
+                              ctypes.POINTER(ctypes.c_int64)], ctypes.c_int),
+        "gwl_event_destroy": ([vp], ctypes.c_int),
+        "gwl_event_set_distance_spline": ([vp, ctypes.c_int32, ctypes.c_int32, _dp, _dp, _dp, _dp],
+                                          ctypes.c_int),
+        "gwl_event_reserve": ([vp, ctypes.c_int64], ctypes.c_int),
+        "gwl_eval": ([vp, vp, ctypes.c_int64, ctypes.c_int32, ctypes.c_int32, vp, vp, vp],
+                     ctypes.c_int),
+        "gwl_detector_response": ([vp, _dp, ctypes.c_int32, ctypes.c_int32, _dp], ctypes.c_int),
+        "gwl_distance_table": ([vp, _dp, _dp, ctypes.c_int32, ctypes.c_double, ctypes.c_int32,
+                                _dp, ctypes.c_int32, _dp, ctypes.c_int32, _dp], ctypes.c_int),
+        "gwl_spline2d_coefficients": ([ctypes.c_int32, ctypes.c_int32, _dp, _dp, _dp, _dp, _dp, _dp],
+                                      ctypes.c_int),
+        "gwl_spline2d_eval_host": ([ctypes.c_int32, ctypes.c_int32, _dp, _dp, _dp, ctypes.c_int64,
+                                    _dp, _dp, _dp], ctypes.c_int),
+        "gwl_fft_batch": ([vp, vp, vp, ctypes.c_int64, ctypes.c_int32, ctypes.c_int32, vp],
+                          ctypes.c_int),
+        "gwl_ln_i0": ([vp, vp, vp, ctypes.c_int64, vp], ctypes.c_int),
+        "gwl_noise_log_likelihood": ([vp, _dp], ctypes.c_int),
+    }
+    for name, (argt, rest) in sig.items():
+        fn = getattr(lib, name)
+        fn.argtypes = argt
+        fn.restype = rest
+    return lib
+
+
+LIB = _load()
+
+
+def check(rc, what="libgwl"):
+    if rc != 0:
+        msg = LIB.gwl_last_error().decode(errors="replace")
+        raise GWLError("%s failed (%d): %s" % (what, rc, msg))
+
+
+def dptr(a):
+    """ctypes double* of a C-contiguous float64 numpy array."""
+    assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(_dp)
+
+
+def iptr(a):
+    assert a.dtype == np.int32 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(_ip)
+
+
+EXPORTED_SYMBOLS = [
+    "gwl_ctx_create", "gwl_ctx_destroy", "gwl_last_error", "gwl_event_create",
+    "gwl_event_destroy", "gwl_event_set_distance_spline", "gwl_event_reserve", "gwl_eval",
+    "gwl_detector_response", "gwl_distance_table", "gwl_spline2d_coefficients",
+    "gwl_spline2d_eval_host", "gwl_fft_batch", "gwl_ln_i0", "gwl_noise_log_likelihood",
+]

@@ -1,0 +1,671 @@
+// C ABI of libgwl: context and event management, device residency of the
+// event data, the distance-marginalisation table kernel, the host spline
+// coefficient solve, and small operator entry points (batched LDS FFT,
+// ln I0) used by the likelihood and exposed for checking.
+#include "gwl_internal.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <vector>
+
+namespace gwl {
+
+static thread_local std::string g_last_error;
+
+void set_error(const std::string& msg) { g_last_error = msg; }
+
+int hip_fail(hipError_t e, const char* what) {
+  g_last_error = std::string("HIP error ") + hipGetErrorString(e) + " in " + what;
+  return GWL_E_HIP;
+}
+
+static int ilog2(long long x) {
+  int l = 0;
+  while ((1LL << l) < x) ++l;
+  return l;
+}
+
+// ------------------------------------------------------------------ distance table
+// base.py:994-1018: table[ii][jj] = logsumexp_k(f(d_jj s_k) - h_ii s_k^2 / 2,
+// b = prior_k * delta) - log_norm, s_k = d_ref / d_k, f = ln I0(|.|) with
+// phase marginalisation, identity otherwise.  One workgroup per d-axis
+// point; the per-distance terms of a chunk of k are staged in LDS and every
+// thread (one h-axis point) streams them with a running-max logsumexp.
+constexpr int kTabChunk = 2048;
+
+__global__ __launch_bounds__(256) void k_distance_table(const double* dist, const double* prior,
+                                                        int nk, double dref, int phase,
+                                                        double delta, const double* daxis, int nd,
+                                                        const double* haxis, int nh,
+                                                        double lognorm, double* out) {
+  __shared__ double dterm[kTabChunk];
+  __shared__ double s2[kTabChunk];
+  __shared__ double lb[kTabChunk];
+  const int jj = blockIdx.x;
+  const double x = daxis[jj];
+  constexpr int HPT = 4;  // h-axis points per thread (nh <= 1024)
+  double m[HPT], s[HPT], hv[HPT];
+#pragma unroll
+  for (int q = 0; q < HPT; ++q) {
+    int ii = threadIdx.x + q * 256;
+    hv[q] = ii < nh ? haxis[ii] : 0.0;
+    m[q] = -INFINITY;
+    s[q] = 0.0;
+  }
+  for (int k0 = 0; k0 < nk; k0 += kTabChunk) {
+    const int kn = min(kTabChunk, nk - k0);
+    __syncthreads();
+    for (int k = threadIdx.x; k < kn; k += 256) {
+      const double sc = dref / dist[k0 + k];
+      const double dv = x * sc;
+      dterm[k] = phase ? ln_i0(fabs(dv)) : dv;
+      s2[k] = sc * sc;
+      const double b = prior[k0 + k] * delta;
+      lb[k] = b;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < HPT; ++q) {
+      if (threadIdx.x + q * 256 >= nh) continue;
+      double mm = m[q], ss = s[q];
+      const double h = hv[q];
+      for (int k = 0; k < kn; ++k) {
+        const double b = lb[k];
+        if (b == 0.0) continue;
+        const double a = dterm[k] - h * s2[k] / 2.0;
+        if (a > mm) {
+          ss = ss * exp(mm - a) + b;
+          mm = a;
+        } else {
+          ss += b * exp(a - mm);
+        }
+      }
+      m[q] = mm;
+      s[q] = ss;
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < HPT; ++q) {
+    int ii = threadIdx.x + q * 256;
+    if (ii < nh) out[(size_t)ii * nd + jj] = (log(s[q]) + m[q]) - lognorm;
+  }
+}
+
+// ------------------------------------------------------------------ batched FFT operator
+template <typename R, int P>
+struct FftTile {
+  static constexpr int B = (P >= 4096 ? 1 : 4096 / P);
+};
+
+template <typename R, int P, int NT>
+__global__ __launch_bounds__(NT) void k_fft_batch(const cplx<R>* in, cplx<R>* out, long long batch) {
+  using C = cplx<R>;
+  constexpr int B = FftTile<R, P>::B;
+  constexpr int MAXBF8 = ((B * P) / 8 + NT - 1) / NT > 0 ? ((B * P) / 8 + NT - 1) / NT : 1;
+  extern __shared__ __align__(16) unsigned char smem[];
+  C* buf = reinterpret_cast<C*>(smem);
+  C* tw = buf + B * P;
+  init_twiddles<P, NT>(tw);
+  const long long b0 = (long long)blockIdx.x * B;
+  const int nb = (int)min((long long)B, batch - b0);
+  for (int i = threadIdx.x; i < B * P; i += NT)
+    buf[i] = (i < nb * P) ? in[b0 * P + i] : cmk<C>((R)0, (R)0);
+  __syncthreads();
+  fft_lds<NT, MAXBF8, P>(buf, tw, B);
+  for (int i = threadIdx.x; i < nb * P; i += NT) out[b0 * P + i] = buf[i];
+}
+
+template <typename R, int P>
+static int fft_launch(const gwl_ctx* ctx, const void* in, void* out, long long batch, hipStream_t st) {
+  constexpr int NT = 256;
+  constexpr int B = FftTile<R, P>::B;
+  const size_t lds = (size_t)(B * P + TwLayout<P>::kEntries) * sizeof(cplx<R>);
+  if (lds > ctx->lds_per_block) {
+    set_error("gwl_fft_batch: transform does not fit the LDS of a workgroup");
+    return GWL_E_UNSUPPORTED;
+  }
+  long long nblk = (batch + B - 1) / B;
+  hipLaunchKernelGGL((k_fft_batch<R, P, NT>), dim3((unsigned)nblk), dim3(NT), lds, st,
+                     reinterpret_cast<const cplx<R>*>(in), reinterpret_cast<cplx<R>*>(out), batch);
+  GWL_HIP_CHECK(hipGetLastError());
+  return GWL_OK;
+}
+
+template <typename R>
+static int fft_dispatch(const gwl_ctx* ctx, const void* in, void* out, long long batch, int P,
+                        hipStream_t st) {
+  switch (P) {
+    case 2: return fft_launch<R, 2>(ctx, in, out, batch, st);
+    case 4: return fft_launch<R, 4>(ctx, in, out, batch, st);
+    case 8: return fft_launch<R, 8>(ctx, in, out, batch, st);
+    case 16: return fft_launch<R, 16>(ctx, in, out, batch, st);
+    case 32: return fft_launch<R, 32>(ctx, in, out, batch, st);
+    case 64: return fft_launch<R, 64>(ctx, in, out, batch, st);
+    case 128: return fft_launch<R, 128>(ctx, in, out, batch, st);
+    case 256: return fft_launch<R, 256>(ctx, in, out, batch, st);
+    case 512: return fft_launch<R, 512>(ctx, in, out, batch, st);
+    case 1024: return fft_launch<R, 1024>(ctx, in, out, batch, st);
+    case 2048: return fft_launch<R, 2048>(ctx, in, out, batch, st);
+    case 4096: return fft_launch<R, 4096>(ctx, in, out, batch, st);
+    case 8192: return fft_launch<R, 8192>(ctx, in, out, batch, st);
+    case 16384: return fft_launch<R, 16384>(ctx, in, out, batch, st);
+  }
+  set_error("gwl_fft_batch: unsupported length");
+  return GWL_E_UNSUPPORTED;
+}
+
+// -sum_det |4/T sum |d|^2/S| / 2 over the masked bins (base.py:402-413);
+// |d|^2/S = |dw|^2 / w with the scaled arrays dw = d sigma / S, w = sigma^2 / S.
+__global__ __launch_bounds__(1024) void k_noise_lnl(EvDev ev, double* out) {
+  __shared__ double red[16];
+  double tot = 0.0;
+  for (int d = 0; d < ev.nd; ++d) {
+    double s = 0.0;
+    for (int i = threadIdx.x; i < ev.N; i += 1024) {
+      double2 v = ev.dw64[(size_t)d * ev.N + i];
+      double w = ev.w64[(size_t)d * ev.N + i];
+      if (w > 0.0) s += (v.x * v.x + v.y * v.y) / w;
+    }
+    if (threadIdx.x == 0 && ev.wN[d] > 0.0)
+      s += (ev.dwN[d][0] * ev.dwN[d][0] + ev.dwN[d][1] * ev.dwN[d][1]) / ev.wN[d];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double v = 0.0;
+      for (int w = 0; w < 16; ++w) v += red[w];
+      tot -= fabs(4.0 / ev.T * v / 2.0);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out = tot;
+}
+
+__global__ void k_ln_i0(const double* in, double* out, long long n) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = ln_i0(in[i]);
+}
+
+// ------------------------------------------------------------------ host spline solve
+// Not-a-knot cubic knots on the data abscissae (FITPACK's choice for s = 0):
+// t = {x0 x4, x2 .. x_{n-3}, x_{n-1} x4}.
+static void knots(const double* x, int n, double* t) {
+  for (int i = 0; i < 4; ++i) t[i] = x[0];
+  for (int i = 2; i <= n - 3; ++i) t[i + 2] = x[i];
+  for (int i = 0; i < 4; ++i) t[n + i] = x[n - 1];
+}
+
+// LU with partial pivoting of the n x n collocation matrix A[i][m] = B_m(x_i).
+struct LU {
+  int n;
+  std::vector<double> a;
+  std::vector<int> piv;
+  bool factor(const double* x, const double* t) {
+    a.assign((size_t)n * n, 0.0);
+    piv.resize(n);
+    for (int i = 0; i < n; ++i) {
+      int l = knot_interval(t, n, x[i]);
+      double h[4];
+      bspline3(t, l, x[i], h);
+      for (int k = 0; k < 4; ++k) a[(size_t)i * n + (l - 3 + k)] = h[k];
+    }
+    for (int k = 0; k < n; ++k) {
+      int p = k;
+      double best = std::fabs(a[(size_t)k * n + k]);
+      for (int i = k + 1; i < std::min(n, k + 8); ++i) {
+        double v = std::fabs(a[(size_t)i * n + k]);
+        if (v > best) { best = v; p = i; }
+      }
+      if (best == 0.0) return false;
+      piv[k] = p;
+      // interchange the not-yet-eliminated part only, so that the multipliers
+      // stay where the interleaved forward substitution in solve() reads them
+      if (p != k)
+        for (int j = k; j < std::min(n, k + 12); ++j) std::swap(a[(size_t)k * n + j], a[(size_t)p * n + j]);
+      const double inv = 1.0 / a[(size_t)k * n + k];
+      for (int i = k + 1; i < std::min(n, k + 8); ++i) {
+        double f = a[(size_t)i * n + k] * inv;
+        if (f == 0.0) continue;
+        a[(size_t)i * n + k] = f;
+        for (int j = k + 1; j < std::min(n, k + 12); ++j) a[(size_t)i * n + j] -= f * a[(size_t)k * n + j];
+      }
+    }
+    return true;
+  }
+  // solve A y = b in place (b has stride 'stride' between consecutive entries)
+  void solve(double* b, size_t stride) const {
+    for (int k = 0; k < n; ++k) {
+      int p = piv[k];
+      if (p != k) std::swap(b[k * stride], b[p * stride]);
+      for (int i = k + 1; i < std::min(n, k + 8); ++i) b[i * stride] -= a[(size_t)i * n + k] * b[k * stride];
+    }
+    for (int k = n - 1; k >= 0; --k) {
+      double s = b[k * stride];
+      for (int j = k + 1; j < std::min(n, k + 12); ++j) s -= a[(size_t)k * n + j] * b[j * stride];
+      b[k * stride] = s / a[(size_t)k * n + k];
+    }
+  }
+};
+
+}  // namespace gwl
+
+using namespace gwl;
+
+extern "C" {
+
+const char* gwl_last_error(void) { return g_last_error.c_str(); }
+
+int gwl_ctx_create(int32_t device, gwl_ctx** out) {
+  if (!out) { set_error("gwl_ctx_create: out is NULL"); return GWL_E_INVALID; }
+  int ndev = 0;
+  GWL_HIP_CHECK(hipGetDeviceCount(&ndev));
+  if (device < 0 || device >= ndev) { set_error("gwl_ctx_create: no such device"); return GWL_E_INVALID; }
+  GWL_HIP_CHECK(hipSetDevice(device));
+  hipDeviceProp_t prop;
+  GWL_HIP_CHECK(hipGetDeviceProperties(&prop, device));
+  gwl_ctx* c = new (std::nothrow) gwl_ctx;
+  if (!c) { set_error("out of host memory"); return GWL_E_NOMEM; }
+  c->device = device;
+  c->n_cu = prop.multiProcessorCount;
+  c->lds_per_block = prop.sharedMemPerBlock;
+  *out = c;
+  return GWL_OK;
+}
+
+int gwl_ctx_destroy(gwl_ctx* ctx) {
+  delete ctx;
+  return GWL_OK;
+}
+
+int gwl_event_create(gwl_ctx* ctx, const gwl_event_desc* d, gwl_event** out, int64_t* hbm_bytes) {
+  if (!ctx || !d || !out) { set_error("gwl_event_create: NULL argument"); return GWL_E_INVALID; }
+  if (d->n_det < 1 || d->n_det > GWL_MAX_DET) { set_error("n_det must be 1..4"); return GWL_E_INVALID; }
+  if (d->n_freq < 3) { set_error("n_freq too small"); return GWL_E_INVALID; }
+  if (d->approximant != GWL_SINE_GAUSSIAN && d->approximant != GWL_TAYLORF2) {
+    set_error("unknown approximant");
+    return GWL_E_INVALID;
+  }
+  GWL_HIP_CHECK(hipSetDevice(ctx->device));
+  const int nd = d->n_det, nf = d->n_freq, N = nf - 1;
+  EvDev ev{};
+  ev.nd = nd;
+  ev.nf = nf;
+  ev.N = N;
+  ev.T = d->duration;
+  ev.fs = d->sampling_frequency;
+  ev.df = (d->sampling_frequency / 2.0) / (double)(nf - 1);  // linspace step, series.py:134
+  ev.start = d->start_time;
+  ev.approx = d->approximant;
+  ev.wf_fmin = d->wf_minimum_frequency;
+  ev.wf_fmax = d->wf_maximum_frequency > 0 ? d->wf_maximum_frequency : ev.df * (double)N;
+  ev.fref = d->reference_frequency > 0 ? d->reference_frequency : 50.0;
+  ev.flags = d->flags;
+  ev.tmin = d->time_prior_minimum;
+  ev.tmax = d->time_prior_maximum;
+  ev.dref = d->reference_distance > 0 ? d->reference_distance : 1.0;
+  for (int k = 0; k < nd; ++k) {
+    ev.mlo[k] = std::max(0, d->mask_lo[k]);
+    ev.mhi[k] = std::min(N, d->mask_hi[k]);
+    if (ev.mlo[k] > ev.mhi[k]) { set_error("empty frequency mask"); return GWL_E_INVALID; }
+    for (int i = 0; i < 9; ++i) ev.tensor[k][i] = d->detector_tensor[9 * k + i];
+    for (int i = 0; i < 3; ++i) ev.vertex[k][i] = d->vertex[3 * k + i];
+  }
+  if ((ev.flags & GWL_DISTANCE_MARG) && !(d->reference_distance > 0)) {
+    set_error("distance marginalisation needs reference_distance > 0");
+    return GWL_E_INVALID;
+  }
+  // strain scale: sigma^2 = median PSD over detector 0's mask
+  {
+    std::vector<double> v;
+    for (int n = ev.mlo[0]; n <= ev.mhi[0]; ++n) {
+      double s = d->psd[n];
+      if (std::isfinite(s) && s > 0) v.push_back(s);
+    }
+    if (v.empty()) { set_error("PSD not finite and positive inside the mask"); return GWL_E_INVALID; }
+    std::nth_element(v.begin(), v.begin() + v.size() / 2, v.end());
+    ev.sigma = std::sqrt(v[v.size() / 2]);
+  }
+  // time marginalisation window and transform tile
+  ev.tstep = d->duration / (double)N;
+  ev.tdelta = 2.0 / d->sampling_frequency;
+  if (ev.flags & GWL_TIME_MARG) {
+    if (!(ev.tmax > ev.tmin)) { set_error("time prior must have maximum > minimum"); return GWL_E_INVALID; }
+    long long jlo = (long long)std::floor((ev.tmin - ev.start - ev.tdelta) / ev.tstep) - 3;
+    long long jhi = (long long)std::ceil((ev.tmax - ev.start + ev.tdelta) / ev.tstep) + 2;
+    jlo = std::max(0LL, jlo);
+    jhi = std::min((long long)N - 1, jhi);
+    if (jhi < jlo) { set_error("time prior outside the segment"); return GWL_E_INVALID; }
+    ev.jlo = (int)jlo;
+    ev.L = (int)(jhi - jlo + 1);
+    long long P = 512;
+    while (P < ev.L) P *= 2;
+    if (P > N) P = N;
+    if ((N % P) != 0 || (P & (P - 1)) != 0 || ev.L > P) {
+      set_error("time marginalisation needs N = n_freq - 1 divisible by a power of two >= the window");
+      return GWL_E_UNSUPPORTED;
+    }
+    if (P > 16384) {
+      set_error("time-prior window longer than 16384 samples is not supported");
+      return GWL_E_UNSUPPORTED;
+    }
+    ev.P = (int)P;
+    ev.Q = (int)(N / P);
+    ev.logP = ilog2(P);
+  } else {
+    ev.logP = ilog2(N);
+    ev.P = 1 << ev.logP;
+    ev.Q = 1;
+    ev.jlo = 0;
+    ev.L = 0;
+  }
+  // host staging in decimated order
+  const size_t n_el = (size_t)nd * N;
+  std::vector<float2> dw32(n_el);
+  std::vector<float> w32(n_el);
+  std::vector<double2> dw64(n_el);
+  std::vector<double> w64(n_el);
+  const double sg = ev.sigma, sg2 = ev.sigma * ev.sigma;
+  for (int k = 0; k < nd; ++k) {
+    for (int i = 0; i < N; ++i) {
+      const int n = (ev.Q == 1) ? i : ev.Q * (i & (ev.P - 1)) + (i >> ev.logP);
+      double re = 0, im = 0, w = 0;
+      if (n >= ev.mlo[k] && n <= ev.mhi[k]) {
+        const double s = d->psd[(size_t)k * nf + n];
+        const double* h = d->strain + 2 * ((size_t)k * nf + n);
+        re = h[0] / s * sg;
+        im = h[1] / s * sg;
+        w = sg2 / s;
+        if (!std::isfinite(re) || !std::isfinite(im) || !std::isfinite(w)) {
+          set_error("non-finite data or PSD inside the mask");
+          return GWL_E_INVALID;
+        }
+      }
+      const size_t o = (size_t)k * N + i;
+      dw32[o] = make_float2((float)re, (float)im);
+      w32[o] = (float)w;
+      dw64[o] = make_double2(re, im);
+      w64[o] = w;
+    }
+    const int n = N;
+    if (n >= ev.mlo[k] && n <= ev.mhi[k]) {
+      const double s = d->psd[(size_t)k * nf + n];
+      const double* h = d->strain + 2 * ((size_t)k * nf + n);
+      ev.dwN[k][0] = h[0] / s * sg;
+      ev.dwN[k][1] = h[1] / s * sg;
+      ev.wN[k] = sg2 / s;
+    }
+  }
+  gwl_event* e = new (std::nothrow) gwl_event;
+  if (!e) { set_error("out of host memory"); return GWL_E_NOMEM; }
+  e->ctx = ctx;
+  const size_t b32 = n_el * (sizeof(float2) + sizeof(float));
+  const size_t b64 = n_el * (sizeof(double2) + sizeof(double));
+  e->data_bytes = b32 + b64 + 1024;
+  if (hipMalloc(&e->data, e->data_bytes) != hipSuccess) {
+    delete e;
+    set_error("hipMalloc failed for event data");
+    return GWL_E_NOMEM;
+  }
+  char* p = reinterpret_cast<char*>(e->data);
+  ev.dw64 = reinterpret_cast<double2*>(p);
+  p += n_el * sizeof(double2);
+  ev.w64 = reinterpret_cast<double*>(p);
+  p += n_el * sizeof(double);
+  ev.dw32 = reinterpret_cast<float2*>(p);
+  p += n_el * sizeof(float2);
+  ev.w32 = reinterpret_cast<float*>(p);
+  hipError_t err = hipSuccess;
+  if (err == hipSuccess) err = hipMemcpy((void*)ev.dw64, dw64.data(), n_el * sizeof(double2), hipMemcpyHostToDevice);
+  if (err == hipSuccess) err = hipMemcpy((void*)ev.w64, w64.data(), n_el * sizeof(double), hipMemcpyHostToDevice);
+  if (err == hipSuccess) err = hipMemcpy((void*)ev.dw32, dw32.data(), n_el * sizeof(float2), hipMemcpyHostToDevice);
+  if (err == hipSuccess) err = hipMemcpy((void*)ev.w32, w32.data(), n_el * sizeof(float), hipMemcpyHostToDevice);
+  if (err != hipSuccess) {
+    (void)hipFree(e->data);
+    delete e;
+    return hip_fail(err, "gwl_event_create upload");
+  }
+  e->dev = ev;
+  *out = e;
+  if (hbm_bytes) *hbm_bytes = (int64_t)e->data_bytes;
+  return GWL_OK;
+}
+
+int gwl_event_destroy(gwl_event* e) {
+  if (!e) return GWL_OK;
+  (void)hipSetDevice(e->ctx->device);
+  if (e->data) (void)hipFree(e->data);
+  if (e->spline) (void)hipFree(e->spline);
+  if (e->ws) (void)hipFree(e->ws);
+  delete e;
+  return GWL_OK;
+}
+
+int gwl_event_set_distance_spline(gwl_event* e, int32_t nx, int32_t ny, const double* tx,
+                                  const double* ty, const double* coeffs, const double* bbox) {
+  if (!e || !tx || !ty || !coeffs || !bbox || nx < 4 || ny < 4) {
+    set_error("gwl_event_set_distance_spline: invalid argument");
+    return GWL_E_INVALID;
+  }
+  GWL_HIP_CHECK(hipSetDevice(e->ctx->device));
+  const size_t n = (size_t)(nx + 4) + (ny + 4) + (size_t)nx * ny;
+  if (e->spline) {
+    (void)hipFree(e->spline);
+    e->spline = nullptr;
+  }
+  GWL_HIP_CHECK(hipMalloc(&e->spline, n * sizeof(double)));
+  e->spline_bytes = n * sizeof(double);
+  GWL_HIP_CHECK(hipMemcpy(e->spline, tx, (nx + 4) * sizeof(double), hipMemcpyHostToDevice));
+  GWL_HIP_CHECK(hipMemcpy(e->spline + nx + 4, ty, (ny + 4) * sizeof(double), hipMemcpyHostToDevice));
+  GWL_HIP_CHECK(hipMemcpy(e->spline + nx + 4 + ny + 4, coeffs, (size_t)nx * ny * sizeof(double),
+                          hipMemcpyHostToDevice));
+  SplineDev sp;
+  sp.tx = e->spline;
+  sp.ty = e->spline + nx + 4;
+  sp.c = e->spline + nx + 4 + ny + 4;
+  sp.nx = nx;
+  sp.ny = ny;
+  sp.xmin = bbox[0];
+  sp.xmax = bbox[1];
+  sp.ymin = bbox[2];
+  sp.ymax = bbox[3];
+  e->dev.sp = sp;
+  e->has_spline = true;
+  return GWL_OK;
+}
+
+int gwl_event_reserve(gwl_event* e, int64_t rows) {
+  if (!e || rows < 1) { set_error("gwl_event_reserve: invalid argument"); return GWL_E_INVALID; }
+  size_t need = workspace_bytes(e, rows);
+  if (need <= e->ws_bytes) return GWL_OK;
+  GWL_HIP_CHECK(hipSetDevice(e->ctx->device));
+  if (e->ws) {
+    (void)hipFree(e->ws);
+    e->ws = nullptr;
+    e->ws_bytes = 0;
+  }
+  if (hipMalloc(&e->ws, need) != hipSuccess) {
+    set_error("hipMalloc failed for workspace");
+    return GWL_E_NOMEM;
+  }
+  e->ws_bytes = need;
+  e->ws_rows = rows;
+  return GWL_OK;
+}
+
+int gwl_noise_log_likelihood(gwl_event* e, double* out) {
+  if (!e || !out) { set_error("gwl_noise_log_likelihood: invalid argument"); return GWL_E_INVALID; }
+  GWL_HIP_CHECK(hipSetDevice(e->ctx->device));
+  double* d = nullptr;
+  GWL_HIP_CHECK(hipMalloc(&d, sizeof(double)));
+  hipLaunchKernelGGL(k_noise_lnl, dim3(1), dim3(1024), 0, nullptr, e->dev, d);
+  hipError_t err = hipGetLastError();
+  if (err == hipSuccess) err = hipMemcpy(out, d, sizeof(double), hipMemcpyDeviceToHost);
+  (void)hipFree(d);
+  if (err != hipSuccess) return hip_fail(err, "gwl_noise_log_likelihood");
+  return GWL_OK;
+}
+
+int gwl_eval(gwl_event* e, const double* params, int64_t n, int32_t ld, int32_t precision,
+             double* out, double* aux, void* stream) {
+  if (!e || !params || !out || n < 0 || ld < GWL_NPARAM) {
+    set_error("gwl_eval: invalid argument");
+    return GWL_E_INVALID;
+  }
+  if (n == 0) return GWL_OK;
+  if (precision != GWL_FP32 && precision != GWL_FP64) {
+    set_error("gwl_eval: unknown precision");
+    return GWL_E_INVALID;
+  }
+  if ((e->dev.flags & GWL_DISTANCE_MARG) && !e->has_spline) {
+    set_error("gwl_eval: distance marginalisation requested but no spline attached");
+    return GWL_E_INVALID;
+  }
+  int rc = gwl_event_reserve(e, n);
+  if (rc != GWL_OK) return rc;
+  return launch_eval(e, params, n, ld, precision, out, aux, reinterpret_cast<hipStream_t>(stream));
+}
+
+int gwl_detector_response(gwl_event* e, const double* params, int32_t det, int32_t precision,
+                          double* out) {
+  if (!e || !params || !out || det < 0 || det >= e->dev.nd) {
+    set_error("gwl_detector_response: invalid argument");
+    return GWL_E_INVALID;
+  }
+  GWL_HIP_CHECK(hipSetDevice(e->ctx->device));
+  double* dp = nullptr;
+  double* dout = nullptr;
+  GWL_HIP_CHECK(hipMalloc(&dp, GWL_NPARAM * sizeof(double)));
+  if (hipMalloc(&dout, (size_t)2 * e->dev.nf * sizeof(double)) != hipSuccess) {
+    (void)hipFree(dp);
+    set_error("hipMalloc failed");
+    return GWL_E_NOMEM;
+  }
+  int rc = GWL_OK;
+  hipError_t err = hipMemcpy(dp, params, GWL_NPARAM * sizeof(double), hipMemcpyHostToDevice);
+  if (err == hipSuccess) {
+    rc = launch_response(e, dp, det, precision, dout, nullptr);
+    if (rc == GWL_OK) {
+      err = hipMemcpy(out, dout, (size_t)2 * e->dev.nf * sizeof(double), hipMemcpyDeviceToHost);
+    }
+  }
+  (void)hipFree(dp);
+  (void)hipFree(dout);
+  if (err != hipSuccess) return hip_fail(err, "gwl_detector_response");
+  return rc;
+}
+
+int gwl_distance_table(gwl_ctx* ctx, const double* dist, const double* prior, int32_t nk,
+                       double dref, int32_t phase, const double* daxis, int32_t nd,
+                       const double* haxis, int32_t nh, double* out) {
+  if (!ctx || !dist || !prior || !daxis || !haxis || !out || nk < 2 || nd < 1 || nh < 1 ||
+      nh > 1024) {
+    set_error("gwl_distance_table: invalid argument");
+    return GWL_E_INVALID;
+  }
+  GWL_HIP_CHECK(hipSetDevice(ctx->device));
+  const double delta = dist[1] - dist[0];  // base.py:887-888
+  // log_norm = logsumexp(0 / d, b = prior * delta)  (base.py:1013-1016)
+  double lognorm;
+  {
+    double s = 0.0;
+    for (int k = 0; k < nk; ++k) s += prior[k] * delta;
+    lognorm = std::log(s);
+  }
+  double *d_dist = nullptr, *d_prior = nullptr, *d_dax = nullptr, *d_hax = nullptr, *d_out = nullptr;
+  size_t bytes = sizeof(double) * ((size_t)2 * nk + nd + nh + (size_t)nd * nh);
+  char* base = nullptr;
+  GWL_HIP_CHECK(hipMalloc(&base, bytes));
+  d_dist = reinterpret_cast<double*>(base);
+  d_prior = d_dist + nk;
+  d_dax = d_prior + nk;
+  d_hax = d_dax + nd;
+  d_out = d_hax + nh;
+  hipError_t err = hipMemcpy(d_dist, dist, nk * sizeof(double), hipMemcpyHostToDevice);
+  if (err == hipSuccess) err = hipMemcpy(d_prior, prior, nk * sizeof(double), hipMemcpyHostToDevice);
+  if (err == hipSuccess) err = hipMemcpy(d_dax, daxis, nd * sizeof(double), hipMemcpyHostToDevice);
+  if (err == hipSuccess) err = hipMemcpy(d_hax, haxis, nh * sizeof(double), hipMemcpyHostToDevice);
+  if (err == hipSuccess) {
+    hipLaunchKernelGGL(k_distance_table, dim3(nd), dim3(256), 0, nullptr, d_dist, d_prior, nk, dref,
+                       phase, delta, d_dax, nd, d_hax, nh, lognorm, d_out);
+    err = hipGetLastError();
+  }
+  if (err == hipSuccess) err = hipMemcpy(out, d_out, (size_t)nd * nh * sizeof(double), hipMemcpyDeviceToHost);
+  (void)hipFree(base);
+  if (err != hipSuccess) return hip_fail(err, "gwl_distance_table");
+  return GWL_OK;
+}
+
+int gwl_spline2d_coefficients(int32_t nx, int32_t ny, const double* x, const double* y,
+                              const double* z, double* tx, double* ty, double* c) {
+  if (nx < 4 || ny < 4 || !x || !y || !z || !tx || !ty || !c) {
+    set_error("gwl_spline2d_coefficients: invalid argument");
+    return GWL_E_INVALID;
+  }
+  for (int i = 1; i < nx; ++i)
+    if (!(x[i] > x[i - 1])) { set_error("x must be strictly increasing"); return GWL_E_INVALID; }
+  for (int i = 1; i < ny; ++i)
+    if (!(y[i] > y[i - 1])) { set_error("y must be strictly increasing"); return GWL_E_INVALID; }
+  knots(x, nx, tx);
+  knots(y, ny, ty);
+  LU lx{nx}, ly{ny};
+  if (!lx.factor(x, tx) || !ly.factor(y, ty)) {
+    set_error("singular collocation matrix");
+    return GWL_E_INVALID;
+  }
+  std::memcpy(c, z, sizeof(double) * (size_t)nx * ny);
+  // along x: columns of c (stride ny)
+  for (int j = 0; j < ny; ++j) lx.solve(c + j, (size_t)ny);
+  // along y: rows of c (stride 1)
+  for (int i = 0; i < nx; ++i) ly.solve(c + (size_t)i * ny, 1);
+  return GWL_OK;
+}
+
+int gwl_spline2d_eval_host(int32_t nx, int32_t ny, const double* tx, const double* ty,
+                           const double* c, int64_t m, const double* x, const double* y,
+                           double* out) {
+  if (nx < 4 || ny < 4 || !tx || !ty || !c || m < 0 || (m && (!x || !y || !out))) {
+    set_error("gwl_spline2d_eval_host: invalid argument");
+    return GWL_E_INVALID;
+  }
+  SplineDev sp;
+  sp.tx = tx;
+  sp.ty = ty;
+  sp.c = c;
+  sp.nx = nx;
+  sp.ny = ny;
+  sp.xmin = tx[0];
+  sp.xmax = tx[nx + 3];
+  sp.ymin = ty[0];
+  sp.ymax = ty[ny + 3];
+  for (int64_t i = 0; i < m; ++i) out[i] = spline2d(sp, x[i], y[i]);
+  return GWL_OK;
+}
+
+int gwl_fft_batch(gwl_ctx* ctx, const void* in, void* out, int64_t batch, int32_t P,
+                  int32_t precision, void* stream) {
+  if (!ctx || !in || !out || batch < 0) { set_error("gwl_fft_batch: invalid argument"); return GWL_E_INVALID; }
+  if (batch == 0) return GWL_OK;
+  GWL_HIP_CHECK(hipSetDevice(ctx->device));
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  (void)hipGetLastError();
+  if (precision == GWL_FP32) return fft_dispatch<float>(ctx, in, out, batch, P, st);
+  if (precision == GWL_FP64) return fft_dispatch<double>(ctx, in, out, batch, P, st);
+  set_error("gwl_fft_batch: unknown precision");
+  return GWL_E_INVALID;
+}
+
+int gwl_ln_i0(gwl_ctx* ctx, const double* in, double* out, int64_t n, void* stream) {
+  if (!ctx || !in || !out || n < 0) { set_error("gwl_ln_i0: invalid argument"); return GWL_E_INVALID; }
+  if (n == 0) return GWL_OK;
+  GWL_HIP_CHECK(hipSetDevice(ctx->device));
+  (void)hipGetLastError();
+  hipLaunchKernelGGL(k_ln_i0, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                     reinterpret_cast<hipStream_t>(stream), in, out, (long long)n);
+  GWL_HIP_CHECK(hipGetLastError());
+  return GWL_OK;
+}
+
+}  // extern "C"

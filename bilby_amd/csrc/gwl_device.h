@@ -1,0 +1,563 @@
+// Device-side building blocks of the compact-binary likelihood kernels:
+// complex arithmetic, the per-row FP64 prologue (sidereal time, antenna
+// patterns, time delays, source constants), per-bin waveform evaluation,
+// ln I0, the bicubic B-spline and the LDS radix FFT.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "../../include/gwl.h"
+
+namespace gwl {
+
+constexpr double kPi = 3.141592653589793238462643383279502884;
+constexpr double kTwoPi = 6.283185307179586476925286766559005768;
+// bilby/core/utils/constants.py:3-7
+constexpr double kSpeedOfLight = 299792458.0;
+constexpr double kParsec = 3.085677581491367e16;
+constexpr double kSolarMass = 1.988409870698050731911960804878414216e30;
+constexpr double kG = 6.6743e-11;
+constexpr double kEulerGamma = 0.5772156649015329;
+
+// ------------------------------------------------------------ complex
+template <typename R> struct cplx_t;
+template <> struct cplx_t<float> { using type = float2; };
+template <> struct cplx_t<double> { using type = double2; };
+template <typename R> using cplx = typename cplx_t<R>::type;
+
+template <typename C> __device__ __forceinline__ C cmk(decltype(C::x) a, decltype(C::x) b) {
+  C r; r.x = a; r.y = b; return r;
+}
+template <typename C> __device__ __forceinline__ C cadd(C a, C b) { return cmk<C>(a.x + b.x, a.y + b.y); }
+template <typename C> __device__ __forceinline__ C csub(C a, C b) { return cmk<C>(a.x - b.x, a.y - b.y); }
+template <typename C> __device__ __forceinline__ C cmul(C a, C b) {
+  return cmk<C>(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+// a * (-i)
+template <typename C> __device__ __forceinline__ C cmul_mi(C a) { return cmk<C>(a.y, -a.x); }
+
+// exp(-2 pi i r) for r in [0, 1): (cos 2 pi r, -sin 2 pi r)
+__device__ __forceinline__ float2 expm2pi(float r) {
+  // v_sin_f32 / v_cos_f32 take their argument in revolutions.
+  return make_float2(__builtin_amdgcn_cosf(r), -__builtin_amdgcn_sinf(r));
+}
+__device__ __forceinline__ double2 expm2pi(double r) {
+  double s, c;
+  sincospi(2.0 * r, &s, &c);
+  return make_double2(c, -s);
+}
+
+// ------------------------------------------------------------ event (device view)
+struct SplineDev {
+  const double* tx;
+  const double* ty;
+  const double* c;
+  int nx, ny;  // coefficient counts (knots: nx + 4, ny + 4)
+  double xmin, xmax, ymin, ymax;
+};
+
+struct EvDev {
+  int nd, nf, N, P, Q, logP;
+  double T, fs, df, start;
+  int mlo[GWL_MAX_DET], mhi[GWL_MAX_DET];
+  double tensor[GWL_MAX_DET][9];
+  double vertex[GWL_MAX_DET][3];
+  int approx;
+  double wf_fmin, wf_fmax, fref;
+  unsigned flags;
+  double tmin, tmax, dref, sigma;
+  // data in decimated order: element i <-> bin n = Q*(i % P) + i / P, i < N.
+  const float2* dw32;  // [nd][N]  (d/S) * sigma
+  const float* w32;    // [nd][N]  sigma^2 / S
+  const double2* dw64;
+  const double* w64;
+  // Nyquist bin (n = N), kept apart from the power-of-two FFT range
+  double dwN[GWL_MAX_DET][2];
+  double wN[GWL_MAX_DET];
+  // time marginalisation window: j in [jlo, jlo + L)
+  int jlo, L;
+  double tstep;  // spacing of the time grid, T / N = 2 / fs
+  double tdelta; // bilby's _delta_tc
+  SplineDev sp;
+};
+
+// ------------------------------------------------------------ sidereal time, geometry
+// bilby/gw/time.py:74-93
+__constant__ static const double kLeapSeconds[18] = {
+    46828800, 78364801, 109900802, 173059203, 252028804, 315187205, 346723206,
+    393984007, 425520008, 457056009, 504489610, 551750411, 599184012,
+    820108813, 914803214, 1025136015, 1119744016, 1167264017};
+
+// bilby/gw/time.py:114-192 (GPS -> UTC -> Julian day -> GMST in radians)
+__device__ inline double gmst_rad(double gps) {
+  double gps_int = floor(gps);
+  int nleap = 0;
+#pragma unroll
+  for (int k = 0; k < 18; ++k) nleap += (gps_int > kLeapSeconds[k]) ? 1 : 0;
+  double second = gps_int - (double)nleap;
+  // datetime(1980, 1, 6) julian_day: 367 y - 7 (y + (m + 9) // 12) // 4 + 275 m // 9 + d
+  double jd = (723231.0 + second / 86400.0) + 1721013.5;
+  double t_hi = (jd - 2451545.0) / 36525.0;
+  double t_lo = fmod(gps, 1.0) / (36525.0 * 86400.0);
+  double t = t_hi + t_lo;
+  double st = (0.0 + (-6.2e-6 * t + 0.093104) * (t * t)) + 67310.54841;
+  st += 8640184.812866 * t_lo;
+  st += 3155760000.0 * t_lo;
+  st += 8640184.812866 * t_hi;
+  st += 3155760000.0 * t_hi;
+  return st * 2.0 * kPi / 86400.0;
+}
+
+// Antenna patterns (bilby/gw/geometry.py:118-186, 261-279) and geocentre
+// time delay (geometry.py:282-343) for one detector.
+__device__ inline void antenna_and_delay(const double* tensor, const double* vertex, double ra,
+                                         double dec, double gps, double psi, double& fp,
+                                         double& fc, double& delay) {
+  double g = gmst_rad(gps);
+  g = g - kTwoPi * floor(g / kTwoPi);
+  double phi = ra - g;
+  double theta = 0.5 * kPi - dec;
+  double sphi, cphi, sth, cth, sps, cps;
+  sincos(phi, &sphi, &cphi);
+  sincos(theta, &sth, &cth);
+  sincos(psi, &sps, &cps);
+  double u[3] = {cphi * cth, cth * sphi, -sth};
+  double v[3] = {-sphi, cphi, 0.0};
+  double m[3], n[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    m[i] = -u[i] * sps - v[i] * cps;
+    n[i] = -u[i] * cps + v[i] * sps;
+  }
+  double ap = 0.0, ac = 0.0;
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      ap += tensor[3 * i + j] * (m[i] * m[j] - n[i] * n[j]);
+      ac += tensor[3 * i + j] * (m[i] * n[j] + n[i] * m[j]);
+    }
+  fp = ap;
+  fc = ac;
+  double om[3] = {sth * cphi, sth * sphi, cth};
+  delay = (om[0] * (0.0 - vertex[0]) + om[1] * (0.0 - vertex[1]) + om[2] * (0.0 - vertex[2])) /
+          kSpeedOfLight;
+}
+
+// ------------------------------------------------------------ per-row constants
+// Phase basis of TaylorF2 (per bin, float64):
+//   b = {f^-5/3, f^-1, f^-2/3, f^-1/3, 1, f^1/3, f^2/3, ln f, f^1/3 ln f}
+constexpr int kNB = 9;
+
+struct SampleConst {
+  double c[10];                 // TF2: basis coefficients (cycles); SG: see below
+  double dt[GWL_MAX_DET];       // total time shift of each detector [s]
+  double cre[GWL_MAX_DET];      // complex detector amplitude (TF2) / F+ A+ (SG)
+  double cim[GWL_MAX_DET];      //                                 / -Fx Ax (SG)
+  double cabs2[GWL_MAX_DET];
+  double hscale;                // d_L / d_ref (distance marginalisation) else 1
+  double jitter;
+  int kb, ke;                   // waveform support, bins [kb, ke]
+  int valid;
+  int pad;
+};
+
+// TaylorF2 3.5PN phase coefficients (non-spinning: Buonanno et al. 2009,
+// arXiv:0907.0700 Eq. 3.18; aligned spin: Khan et al. 2016,
+// arXiv:1508.07253 App. B).  phil5, phil6 multiply ln v.
+__device__ inline void tf2_phis(double m1, double m2, double chi1, double chi2, double* phi,
+                                double& phil5, double& phil6, double& eta) {
+  double M = m1 + m2;
+  eta = m1 * m2 / (M * M);
+  double delta = (m1 - m2) / M;
+  double chis = 0.5 * (chi1 + chi2), chia = 0.5 * (chi1 - chi2);
+  double pi = kPi, e2 = eta * eta, e3 = e2 * eta;
+  phi[0] = 1.0;
+  phi[1] = 0.0;
+  phi[2] = 3715.0 / 756.0 + 55.0 * eta / 9.0;
+  phi[3] = -16.0 * pi + 113.0 * delta * chia / 3.0 + (113.0 / 3.0 - 76.0 * eta / 3.0) * chis;
+  phi[4] = 15293365.0 / 508032.0 + 27145.0 * eta / 504.0 + 3085.0 * e2 / 72.0 +
+           (-405.0 / 8.0 + 200.0 * eta) * chia * chia - 405.0 / 4.0 * delta * chia * chis +
+           (-405.0 / 8.0 + 5.0 * eta / 2.0) * chis * chis;
+  double p5 = 38645.0 * pi / 756.0 - 65.0 * pi * eta / 9.0 +
+              delta * (-732985.0 / 2268.0 - 140.0 * eta / 9.0) * chia +
+              (-732985.0 / 2268.0 + 24260.0 * eta / 81.0 + 340.0 * e2 / 9.0) * chis;
+  phi[5] = p5;
+  phil5 = 3.0 * p5;
+  phi[6] = 11583231236531.0 / 4694215680.0 - 6848.0 * kEulerGamma / 21.0 - 640.0 * pi * pi / 3.0 +
+           (-15737765635.0 / 3048192.0 + 2255.0 * pi * pi / 12.0) * eta + 76055.0 * e2 / 1728.0 -
+           127825.0 * e3 / 1296.0 - 6848.0 / 21.0 * log(4.0) + 2270.0 * pi * delta * chia / 3.0 +
+           (2270.0 * pi / 3.0 - 520.0 * pi * eta) * chis;
+  phil6 = -6848.0 / 21.0;
+  phi[7] = 77096675.0 * pi / 254016.0 + 378515.0 * pi * eta / 1512.0 - 74045.0 * pi * e2 / 756.0 +
+           delta * (-25150083775.0 / 3048192.0 + 26804935.0 * eta / 6048.0 - 1985.0 * e2 / 48.0) * chia +
+           (-25150083775.0 / 3048192.0 + 10566655595.0 * eta / 762048.0 - 1042165.0 * e2 / 3024.0 +
+            5345.0 * e3 / 36.0) * chis;
+}
+
+__device__ inline void tf2_basis(double f, double* b) {
+  double f13 = cbrt(f);
+  double u = 1.0 / f13;
+  double u2 = u * u, u3 = u2 * u;
+  double lnf = log(f);
+  b[0] = u3 * u2;
+  b[1] = u3;
+  b[2] = u2;
+  b[3] = u;
+  b[4] = 1.0;
+  b[5] = f13;
+  b[6] = f13 * f13;
+  b[7] = lnf;
+  b[8] = f13 * lnf;
+}
+
+__device__ inline double tf2_phase_cycles(const double* c, const double* b) {
+  double s = c[4];
+  s = fma(c[0], b[0], s);
+  s = fma(c[1], b[1], s);
+  s = fma(c[2], b[2], s);
+  s = fma(c[3], b[3], s);
+  s = fma(c[5], b[5], s);
+  s = fma(c[6], b[6], s);
+  s = fma(c[7], b[7], s);
+  s = fma(c[8], b[8], s);
+  return s;
+}
+
+// First bin k in [lo, hi] with k*df >= f (returns hi + 1 if none).
+__device__ inline int first_bin_ge(double f, double df, int lo, int hi) {
+  double q = ceil(f / df);
+  long long k = (long long)fmax(q, (double)lo - 1.0);
+  if (k < lo) k = lo;
+  while (k > lo && (double)(k - 1) * df >= f) --k;
+  while (k <= hi && (double)k * df < f) ++k;
+  return (int)k;
+}
+// Last bin k in [lo, hi] with k*df <= f (returns lo - 1 if none).
+__device__ inline int last_bin_le(double f, double df, int lo, int hi) {
+  double q = floor(f / df);
+  long long k = (long long)fmin(q, (double)hi + 1.0);
+  if (k > hi) k = hi;
+  while (k < hi && (double)(k + 1) * df <= f) ++k;
+  while (k >= lo && (double)k * df > f) --k;
+  return (int)k;
+}
+
+// Per-row prologue: everything that depends on the parameter row only.
+__device__ inline void compute_sample_const(const EvDev& ev, const double* row, SampleConst& sc) {
+  double ra = row[GWL_P_RA], dec = row[GWL_P_DEC], psi = row[GWL_P_PSI];
+  double tc = row[GWL_P_GEOCENT_TIME], jit = row[GWL_P_TIME_JITTER];
+  double dl = row[GWL_P_LUMINOSITY_DISTANCE];
+  bool timem = (ev.flags & GWL_TIME_MARG) != 0;
+  bool jitter = timem && (ev.flags & GWL_JITTER_TIME);
+  // base.py:427-428: geocent_time += time_jitter before the detector loop
+  double gt = jitter ? (tc + jit) : tc;
+  sc.jitter = jitter ? jit : 0.0;
+  sc.hscale = (ev.flags & GWL_DISTANCE_MARG) ? dl / ev.dref : 1.0;
+  int lo = ev.mlo[0], hi = ev.mhi[0];
+  for (int d = 1; d < ev.nd; ++d) {
+    lo = min(lo, ev.mlo[d]);
+    hi = max(hi, ev.mhi[d]);
+  }
+  double fp[GWL_MAX_DET], fc[GWL_MAX_DET];
+  for (int d = 0; d < ev.nd; ++d) {
+    double delay;
+    antenna_and_delay(ev.tensor[d], ev.vertex[d], ra, dec, gt, psi, fp[d], fc[d], delay);
+    double dtg = gt - ev.start;  // interferometer.py:355-358: subtract GPS times first
+    sc.dt[d] = dtg + delay;
+  }
+  sc.valid = 1;
+  if (ev.approx == GWL_TAYLORF2) {
+    double m1 = row[GWL_P_MASS_1], m2 = row[GWL_P_MASS_2];
+    double chi1 = row[GWL_P_CHI_1], chi2 = row[GWL_P_CHI_2];
+    double iota = row[GWL_P_THETA_JN], phase = row[GWL_P_PHASE];
+    if (!(m1 > 0.0) || !(m2 > 0.0) || !(dl > 0.0) || !(fabs(chi1) < 1.0) || !(fabs(chi2) < 1.0) ||
+        !isfinite(m1 + m2 + dl + iota + phase)) {
+      sc.valid = 0;
+    }
+    double M = m1 + m2;
+    double mtsun = kG * kSolarMass / (kSpeedOfLight * kSpeedOfLight * kSpeedOfLight);
+    double msec = M * mtsun;
+    double phi[8], phil5, phil6, eta;
+    tf2_phis(m1, m2, chi1, chi2, phi, phil5, phil6, eta);
+    double piM = kPi * msec;
+    double p13 = cbrt(piM), ip13 = 1.0 / p13;
+    double lnpiM = log(piM);
+    double pre = 3.0 / (128.0 * eta);
+    double c[kNB];
+    c[0] = pre * phi[0] * ip13 * ip13 * ip13 * ip13 * ip13;
+    c[1] = pre * phi[2] * ip13 * ip13 * ip13;
+    c[2] = pre * phi[3] * ip13 * ip13;
+    c[3] = pre * phi[4] * ip13;
+    c[4] = pre * (phi[5] + phil5 * lnpiM / 3.0);
+    c[5] = pre * (phi[6] + phil6 * lnpiM / 3.0) * p13;
+    c[6] = pre * phi[7] * p13 * p13;
+    c[7] = pre * phil5 / 3.0;
+    c[8] = pre * phil6 / 3.0 * p13;
+    double bref[kNB];
+    tf2_basis(ev.fref, bref);
+    double psi_ref = tf2_phase_cycles(c, bref);  // radians here (c not yet scaled)
+    c[4] += -psi_ref - 2.0 * phase - 0.25 * kPi;
+    for (int k = 0; k < kNB; ++k) sc.c[k] = c[k] / kTwoPi;
+    sc.c[9] = 0.0;
+    double mc = pow(eta, 0.6) * msec;
+    double amp0 = sqrt(5.0 / 24.0) * pow(kPi, -2.0 / 3.0) * kSpeedOfLight / (dl * 1e6 * kParsec) *
+                  pow(mc, 5.0 / 6.0) / ev.sigma;
+    double ci = cos(iota);
+    double ap = amp0 * (1.0 + ci * ci) / 2.0, ax = amp0 * ci;
+    for (int d = 0; d < ev.nd; ++d) {
+      sc.cre[d] = fp[d] * ap;
+      sc.cim[d] = -fc[d] * ax;
+      sc.cabs2[d] = sc.cre[d] * sc.cre[d] + sc.cim[d] * sc.cim[d];
+    }
+    double fisco = 1.0 / (pow(6.0, 1.5) * kPi * msec);
+    double fmax = fmin(fisco, ev.wf_fmax);
+    int kb = ev.wf_fmin > 0.0 ? first_bin_ge(ev.wf_fmin, ev.df, lo, hi) : lo;
+    int ke = last_bin_le(fmax, ev.df, lo, hi);
+    sc.kb = max(kb, lo);
+    sc.ke = min(ke, hi);
+  } else {  // sine-Gaussian, source.py:1153-1209
+    double hrss = row[GWL_P_HRSS], Q = row[GWL_P_Q], f0 = row[GWL_P_FREQUENCY];
+    if (!(Q > 0.0) || !(f0 > 0.0) || !isfinite(hrss)) sc.valid = 0;
+    double tau = Q / (sqrt(2.0) * kPi * f0);
+    double temp = Q / (4.0 * sqrt(kPi) * f0);
+    double base = hrss * sqrt(kPi) * tau / 2.0;
+    double ap = base / sqrt(temp * (1.0 + exp(-Q * Q))) / ev.sigma;
+    double ax = base / sqrt(temp * (1.0 - exp(-Q * Q))) / ev.sigma;
+    // exponent: exp(-pi^2 tau^2 (f -+ f0)^2) = exp2(-c0 (f -+ f0)^2)
+    sc.c[0] = kPi * kPi * tau * tau * 1.4426950408889634074;
+    sc.c[1] = f0;
+    sc.c[2] = kPi * kPi * tau * tau;
+    for (int k = 3; k < 10; ++k) sc.c[k] = 0.0;
+    for (int d = 0; d < ev.nd; ++d) {
+      sc.cre[d] = fp[d] * ap;
+      sc.cim[d] = -fc[d] * ax;
+      sc.cabs2[d] = 0.0;
+    }
+    sc.kb = lo;
+    sc.ke = hi;
+  }
+  if (!sc.valid) {
+    sc.kb = 1;
+    sc.ke = 0;
+  }
+}
+
+// ------------------------------------------------------------ special functions
+// ln I0(x) (bilby/gw/utils.py:1006-1022 computes log(i0e(x)) + |x|).
+// Power series I0(x) = sum_k (x^2/4)^k / (k!)^2 for x < 30 (all terms
+// positive, no cancellation); for x >= 30 the large-argument expansion
+// I0(x) = e^x / sqrt(2 pi x) sum_k ((2k-1)!!)^2 / (k! (8x)^k)
+// (Abramowitz & Stegun 9.7.1), summed until float64 round-off (its terms
+// fall below 1e-17 long before they start to grow at k ~ 2x).
+__host__ __device__ inline double ln_i0(double x) {
+  x = fabs(x);
+  if (x < 30.0) {
+    double q = 0.25 * x * x;
+    double term = 1.0, sum = 1.0;
+    for (int k = 1; k < 120; ++k) {
+      term *= q / ((double)k * (double)k);
+      sum += term;
+      if (term < 1e-17 * sum) break;
+    }
+    return log(sum);
+  }
+  double y = 1.0 / (8.0 * x);
+  double term = 1.0, sum = 1.0;
+  for (int k = 1; k < 40; ++k) {
+    double a = 2.0 * k - 1.0;
+    double next = term * a * a * y / (double)k;
+    if (next >= term) break;
+    term = next;
+    sum += term;
+    if (term < 1e-17 * sum) break;
+  }
+  return x - 0.5 * log(kTwoPi * x) + log(sum);
+}
+
+// Largest l in [3, ncoef - 1] with t[l] <= x (FITPACK fpbisp interval rule).
+__host__ __device__ inline int knot_interval(const double* t, int ncoef, double x) {
+  int lo = 3, hi = ncoef - 1;
+  while (lo < hi) {
+    int mid = (lo + hi + 1) >> 1;
+    if (t[mid] <= x) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
+// Cubic B-spline values at x in interval l (Cox-de Boor, as FITPACK fpbspl).
+__host__ __device__ inline void bspline3(const double* t, int l, double x, double* h) {
+  double hh[3];
+  h[0] = 1.0;
+#pragma unroll
+  for (int j = 1; j <= 3; ++j) {
+#pragma unroll
+    for (int i = 0; i < j; ++i) hh[i] = h[i];
+    h[0] = 0.0;
+#pragma unroll
+    for (int i = 1; i <= j; ++i) {
+      int li = l + i, lj = li - j;
+      double f = hh[i - 1] / (t[li] - t[lj]);
+      h[i - 1] += f * (t[li] - x);
+      h[i] = f * (x - t[lj]);
+    }
+  }
+}
+
+// BoundedRectBivariateSpline (bilby/core/utils/calculus.py:221-262):
+// the bicubic interpolant, -inf outside the bounding box.
+__host__ __device__ inline double spline2d(const SplineDev& sp, double x, double y) {
+  if (!(x >= sp.xmin && x <= sp.xmax && y >= sp.ymin && y <= sp.ymax)) return -INFINITY;
+  int lx = knot_interval(sp.tx, sp.nx, x);
+  int ly = knot_interval(sp.ty, sp.ny, y);
+  double bx[4], by[4];
+  bspline3(sp.tx, lx, x, bx);
+  bspline3(sp.ty, ly, y, by);
+  double s = 0.0;
+  const double* c = sp.c + (size_t)(lx - 3) * sp.ny + (ly - 3);
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    double r = 0.0;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) r += by[j] * c[(size_t)i * sp.ny + j];
+    s += bx[i] * r;
+  }
+  return s;
+}
+
+// ------------------------------------------------------------ LDS radix FFT
+// Forward DFT X[k] = sum_n x[n] exp(-2 pi i n k / P) of B independent
+// P-point sequences held in LDS (buf[b * P + n]), Stockham autosort with
+// radix-8/4/2 passes; twiddles W_P^m from an LDS table tw[m], m < P.
+template <typename C>
+__device__ __forceinline__ void dft2(C& a, C& b) {
+  C t = a;
+  a = cadd(t, b);
+  b = csub(t, b);
+}
+template <typename C>
+__device__ __forceinline__ void dft4(C* x) {
+  C a0 = cadd(x[0], x[2]), a1 = csub(x[0], x[2]);
+  C a2 = cadd(x[1], x[3]), a3 = cmul_mi(csub(x[1], x[3]));
+  x[0] = cadd(a0, a2);
+  x[2] = csub(a0, a2);
+  x[1] = cadd(a1, a3);
+  x[3] = csub(a1, a3);
+}
+template <typename C>
+__device__ __forceinline__ void dft8(C* x) {
+  using R = decltype(C::x);
+  C e[4] = {x[0], x[2], x[4], x[6]};
+  C o[4] = {x[1], x[3], x[5], x[7]};
+  dft4(e);
+  dft4(o);
+  const R r = (R)0.70710678118654752440;
+  // W8^1 = (1 - i)/sqrt2, W8^2 = -i, W8^3 = (-1 - i)/sqrt2
+  C o1 = cmk<C>((o[1].x + o[1].y) * r, (o[1].y - o[1].x) * r);
+  C o2 = cmul_mi(o[2]);
+  C o3 = cmk<C>((-o[3].x + o[3].y) * r, (-o[3].y - o[3].x) * r);
+  x[0] = cadd(e[0], o[0]);
+  x[4] = csub(e[0], o[0]);
+  x[1] = cadd(e[1], o1);
+  x[5] = csub(e[1], o1);
+  x[2] = cadd(e[2], o2);
+  x[6] = csub(e[2], o2);
+  x[3] = cadd(e[3], o3);
+  x[7] = csub(e[3], o3);
+}
+
+template <int R, typename C>
+__device__ __forceinline__ void dftR(C* x) {
+  if constexpr (R == 8) dft8(x);
+  else if constexpr (R == 4) dft4(x);
+  else dft2(x[0], x[1]);
+}
+
+// Twiddle table W_P^m = exp(-2 pi i m / P), m < P.  Up to P = 1024 it is
+// stored whole; beyond, as two levels W_P^m = H[m >> 6] L[m & 63] with
+// H[q] = W_P^(64 q) and L[l] = W_P^l, so that LDS holds P/64 + 64 entries.
+template <int P>
+struct TwLayout {
+  static constexpr bool kTwo = P > 1024;
+  static constexpr int kHi = kTwo ? (P >> 6) : P;
+  static constexpr int kEntries = kTwo ? kHi + 64 : P;
+};
+
+template <int P, typename C>
+__device__ __forceinline__ C tw_get(const C* tw, int m) {
+  if constexpr (TwLayout<P>::kTwo) {
+    return cmul(tw[m >> 6], tw[TwLayout<P>::kHi + (m & 63)]);
+  } else {
+    return tw[m];
+  }
+}
+
+template <int P, int NT, typename C>
+__device__ inline void init_twiddles(C* tw) {
+  using R = decltype(C::x);
+  for (int m = threadIdx.x; m < TwLayout<P>::kEntries; m += NT) {
+    int e;
+    if constexpr (TwLayout<P>::kTwo) e = (m < TwLayout<P>::kHi) ? (m << 6) : (m - TwLayout<P>::kHi);
+    else e = m;
+    double s, c;
+    sincospi(-2.0 * (double)e / (double)P, &s, &c);
+    tw[m] = cmk<C>((R)c, (R)s);
+  }
+}
+
+// One Stockham pass of radix RAD over B sequences of length P, span p.
+// Each thread handles butterflies bf = tid, tid + NT, ...
+template <int RAD, int NT, int MAXBF, int P, typename C>
+__device__ __forceinline__ void stockham_pass(C* buf, const C* tw, int B, int p) {
+  constexpr int nb = P / RAD;    // butterflies per sequence
+  const int total = B * nb;
+  const int tid = threadIdx.x;
+  C x[MAXBF][RAD];
+  int base[MAXBF];
+#pragma unroll
+  for (int q = 0; q < MAXBF; ++q) {
+    int bf = tid + q * NT;
+    base[q] = -1;
+    if (bf < total) {
+      int b = bf / nb, i = bf - b * nb;
+      int k = i & (p - 1);
+      const C* src = buf + (size_t)b * P;
+#pragma unroll
+      for (int r = 0; r < RAD; ++r) x[q][r] = src[i + r * nb];
+      // twiddle W_{p RAD}^{r k} = W_P^{r k P/(p RAD)}
+      int step = (P / (p * RAD)) * k;
+#pragma unroll
+      for (int r = 1; r < RAD; ++r) x[q][r] = cmul(x[q][r], tw_get<P>(tw, (r * step) & (P - 1)));
+      dftR<RAD>(x[q]);
+      base[q] = b * P + (i - k) * RAD + k;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < MAXBF; ++q) {
+    if (base[q] >= 0) {
+#pragma unroll
+      for (int r = 0; r < RAD; ++r) buf[base[q] + r * p] = x[q][r];
+    }
+  }
+  __syncthreads();
+}
+
+// Full forward FFT of B sequences of length P (power of two, P >= 2).
+// MAXBF8 bounds the radix-8 butterflies one thread handles per pass.
+template <int NT, int MAXBF8, int P, typename C>
+__device__ inline void fft_lds(C* buf, const C* tw, int B) {
+  int p = 1;
+  int rem = P;
+  while (rem >= 8) {
+    stockham_pass<8, NT, MAXBF8, P>(buf, tw, B, p);
+    p *= 8;
+    rem /= 8;
+  }
+  if (rem == 4) {
+    stockham_pass<4, NT, 2 * MAXBF8, P>(buf, tw, B, p);
+  } else if (rem == 2) {
+    stockham_pass<2, NT, 4 * MAXBF8, P>(buf, tw, B, p);
+  }
+}
+
+}  // namespace gwl

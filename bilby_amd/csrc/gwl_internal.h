@@ -1,0 +1,48 @@
+// Host-side structures shared by the libgwl translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string>
+#include "../../include/gwl.h"
+#include "gwl_device.h"
+
+struct gwl_ctx {
+  int device;
+  int n_cu;
+  size_t lds_per_block;
+};
+
+// Tile of the time-marginalised kernel: S rows per workgroup, P-point LDS
+// transforms, G groups of the Q = N / P decimated sub-sequences.
+struct TimeTile {
+  int S, P, G;
+};
+
+struct gwl_event {
+  gwl_ctx* ctx;
+  gwl::EvDev dev;
+  void* data = nullptr;          // device: dw32, w32, dw64, w64
+  size_t data_bytes = 0;
+  double* spline = nullptr;      // device: tx, ty, coefficients
+  size_t spline_bytes = 0;
+  void* ws = nullptr;            // device workspace (partials)
+  size_t ws_bytes = 0;
+  int64_t ws_rows = 0;
+  bool has_spline = false;
+};
+
+namespace gwl {
+void set_error(const std::string& msg);
+int hip_fail(hipError_t e, const char* what);
+size_t workspace_bytes(const gwl_event* ev, int64_t rows);
+int launch_eval(gwl_event* ev, const double* params, int64_t n, int ld, int precision, double* out,
+                double* aux, hipStream_t stream);
+int launch_response(gwl_event* ev, const double* params_dev, int det, int precision, double* out_dev,
+                    hipStream_t stream);
+}  // namespace gwl
+
+#define GWL_HIP_CHECK(expr)                                   \
+  do {                                                        \
+    hipError_t _e = (expr);                                   \
+    if (_e != hipSuccess) return gwl::hip_fail(_e, #expr);    \
+  } while (0)

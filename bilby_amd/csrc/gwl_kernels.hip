@@ -1,0 +1,774 @@
+// Likelihood kernels for gfx950.
+//
+//  k_reduce : fused source model + detector projection + noise-weighted
+//             inner products <d|h>, <h|h> (no time marginalisation).
+//             bilby/gw/likelihood/base.py:260-354 (calculate_snrs) with
+//             interferometer.py:303-368 and gw/utils.py:118-138.
+//  k_time   : fused source model + projection + d_inner_h_array
+//             = 4/T FFT_N(h conj(d) / S) at the time-prior window only
+//             (base.py:325-330), as a decimated two-stage transform: P-point
+//             LDS radix FFTs of the Q = N/P sub-sequences x[Q n1 + n2],
+//             then X[j] = sum_n2 W_N^(n2 j) Y_n2[j mod P] for j in the window.
+//  k_epi_*  : float64 marginalisation epilogues (base.py:448-477, 775-820).
+//
+// Data layout in HBM (per detector, decimated order): element i holds bin
+// n = Q (i mod P) + i / P, so that one n2 step reads P consecutive elements.
+// Work is tiled S rows x (bins) per workgroup: the per-bin float64 phase
+// basis and the data are loaded once and reused by S parameter rows.
+#include "gwl_internal.h"
+#include <algorithm>
+#include <cfloat>
+
+namespace gwl {
+
+template <typename R>
+struct SampleFast {
+  R cre[GWL_MAX_DET], cim[GWL_MAX_DET], cabs2[GWL_MAX_DET];
+  R k2;  // sine-Gaussian exponent coefficient (log2 units)
+};
+
+template <typename R>
+__device__ __forceinline__ R exp2r(R x);
+template <>
+__device__ __forceinline__ float exp2r<float>(float x) { return __builtin_amdgcn_exp2f(x); }
+template <>
+__device__ __forceinline__ double exp2r<double>(double x) { return exp2(x); }
+
+template <typename R>
+__device__ __forceinline__ cplx<R> load_dw(const EvDev& ev, int d, int i);
+template <>
+__device__ __forceinline__ float2 load_dw<float>(const EvDev& ev, int d, int i) {
+  return ev.dw32[(size_t)d * ev.N + i];
+}
+template <>
+__device__ __forceinline__ double2 load_dw<double>(const EvDev& ev, int d, int i) {
+  return ev.dw64[(size_t)d * ev.N + i];
+}
+template <typename R>
+__device__ __forceinline__ R load_w(const EvDev& ev, int d, int i);
+template <>
+__device__ __forceinline__ float load_w<float>(const EvDev& ev, int d, int i) {
+  return ev.w32[(size_t)d * ev.N + i];
+}
+template <>
+__device__ __forceinline__ double load_w<double>(const EvDev& ev, int d, int i) {
+  return ev.w64[(size_t)d * ev.N + i];
+}
+
+// Row prologue for a tile of S rows (threads 0..S-1), into LDS.
+template <typename R, int S>
+__device__ inline void tile_prologue(const EvDev& ev, const double* params, int ld, long long n,
+                                     long long row0, SampleConst* sc, SampleFast<R>* sf,
+                                     int* band) {
+  int t = threadIdx.x;
+  if (t < S) {
+    long long r = row0 + t;
+    if (r < n) {
+      compute_sample_const(ev, params + r * (long long)ld, sc[t]);
+    } else {
+      sc[t].valid = 0;
+      sc[t].kb = 1;
+      sc[t].ke = 0;
+      sc[t].hscale = 1.0;
+      sc[t].jitter = 0.0;
+    }
+#pragma unroll
+    for (int d = 0; d < GWL_MAX_DET; ++d) {
+      sf[t].cre[d] = (R)(d < ev.nd ? sc[t].cre[d] : 0.0);
+      sf[t].cim[d] = (R)(d < ev.nd ? sc[t].cim[d] : 0.0);
+      sf[t].cabs2[d] = (R)(d < ev.nd ? sc[t].cabs2[d] : 0.0);
+    }
+    sf[t].k2 = (R)sc[t].c[0];
+  }
+  __syncthreads();
+  if (t == 0) {
+    int lo = 1 << 30, hi = -1;
+    for (int s = 0; s < S; ++s) {
+      if (sc[s].kb <= sc[s].ke) {
+        lo = min(lo, sc[s].kb);
+        hi = max(hi, sc[s].ke);
+      }
+    }
+    band[0] = lo;
+    band[1] = hi;
+  }
+  __syncthreads();
+}
+
+// Source model x projection x data for one frequency bin n and S rows:
+//   I[s]  = sum_d h_{s,d}(n) conj(dw_d(n))     (h conj(d) / S, base.py:327-329)
+//   hh[s] += sum_d |h_{s,d}(n)|^2 w_d(n)        (|h|^2 / S, gw/utils.py:137)
+// with h_{s,d} = H_{s,d}(f) exp(-2 pi i (Psi_s(f) / 2pi + f dt_{s,d})),
+// the phase argument in float64 (interferometer.py:357-362).
+template <typename R, int WAVE, int ND, int S>
+__device__ __forceinline__ void bin_rows(int n, double f, const cplx<R>* dw, const R* w,
+                                         const SampleConst* sc, const SampleFast<R>* sf,
+                                         cplx<R>* I, R* hh) {
+  using C = cplx<R>;
+  if constexpr (WAVE == GWL_TAYLORF2) {
+    double b[kNB];
+    tf2_basis(f, b);
+    const R amp = (R)(b[1] * sqrt(b[3]));  // f^-7/6
+    const R amp2 = amp * amp;
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+      if (n < sc[s].kb || n > sc[s].ke) {
+        I[s] = cmk<C>((R)0, (R)0);
+        continue;
+      }
+      const double psi = tf2_phase_cycles(sc[s].c, b);
+      R ar = 0, ai = 0, hw = 0;
+#pragma unroll
+      for (int d = 0; d < ND; ++d) {
+        const double cyc = fma(f, sc[s].dt[d], psi);
+        const C e = expm2pi((R)(cyc - floor(cyc)));
+        const R cr = sf[s].cre[d] * amp, ci = sf[s].cim[d] * amp;
+        const R hr = cr * e.x - ci * e.y, hi = cr * e.y + ci * e.x;
+        ar += hr * dw[d].x + hi * dw[d].y;
+        ai += hi * dw[d].x - hr * dw[d].y;
+        hw += sf[s].cabs2[d] * w[d];
+      }
+      I[s] = cmk<C>(ar, ai);
+      hh[s] += amp2 * hw;
+    }
+  } else {  // sine-Gaussian
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+      if (n < sc[s].kb || n > sc[s].ke) {
+        I[s] = cmk<C>((R)0, (R)0);
+        continue;
+      }
+      const R xm = (R)(f - sc[s].c[1]), xp = (R)(f + sc[s].c[1]);
+      const R k2 = sf[s].k2;
+      const R em = exp2r<R>(-k2 * xm * xm), ep = exp2r<R>(-k2 * xp * xp);
+      const R sp = em + ep, sm = em - ep;
+      R ar = 0, ai = 0, hw = 0;
+#pragma unroll
+      for (int d = 0; d < ND; ++d) {
+        const double cyc = f * sc[s].dt[d];
+        const C e = expm2pi((R)(cyc - floor(cyc)));
+        const R pr = sf[s].cre[d] * sp, pim = sf[s].cim[d] * sm;
+        const R hr = pr * e.x - pim * e.y, hi = pr * e.y + pim * e.x;
+        ar += hr * dw[d].x + hi * dw[d].y;
+        ai += hi * dw[d].x - hr * dw[d].y;
+        hw += (pr * pr + pim * pim) * w[d];
+      }
+      I[s] = cmk<C>(ar, ai);
+      hh[s] += hw;
+    }
+  }
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+// ------------------------------------------------------------------ k_reduce
+struct RedArgs {
+  EvDev ev;
+  const double* params;
+  int ld;
+  long long n;
+  int chunk;    // elements per chunk
+  int nchunk;
+  double* part; // [n][nchunk][3]
+};
+
+template <typename R, int WAVE, int ND, int S, int NT>
+__global__ __launch_bounds__(NT) void k_reduce(RedArgs a) {
+  using C = cplx<R>;
+  const EvDev& ev = a.ev;
+  __shared__ SampleConst sc[S];
+  __shared__ SampleFast<R> sf[S];
+  __shared__ int band[2];
+  __shared__ double red[NT / 64][S][3];
+  const int tid = threadIdx.x;
+  const long long row0 = (long long)blockIdx.y * S;
+  const int chunk = blockIdx.x;
+  tile_prologue<R, S>(ev, a.params, a.ld, a.n, row0, sc, sf, band);
+
+  R acc_r[S], acc_i[S], acc_h[S];
+#pragma unroll
+  for (int s = 0; s < S; ++s) acc_r[s] = acc_i[s] = acc_h[s] = 0;
+
+  const int i0 = chunk * a.chunk;
+  const int i1 = min(i0 + a.chunk, ev.N);
+  for (int i = i0 + tid; i < i1; i += NT) {
+    const int n = ev.Q * (i & (ev.P - 1)) + (i >> ev.logP);
+    if (n < band[0] || n > band[1]) continue;
+    C dw[ND];
+    R w[ND];
+#pragma unroll
+    for (int d = 0; d < ND; ++d) {
+      dw[d] = load_dw<R>(ev, d, i);
+      w[d] = load_w<R>(ev, d, i);
+    }
+    C I[S];
+    bin_rows<R, WAVE, ND, S>(n, (double)n * ev.df, dw, w, sc, sf, I, acc_h);
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+      acc_r[s] += I[s].x;
+      acc_i[s] += I[s].y;
+    }
+  }
+  double dr[S], di[S], dh[S];
+#pragma unroll
+  for (int s = 0; s < S; ++s) {
+    dr[s] = acc_r[s];
+    di[s] = acc_i[s];
+    dh[s] = acc_h[s];
+  }
+  // Nyquist bin n = N (inside the inner-product mask, outside the FFT range)
+  if (chunk == a.nchunk - 1 && tid == 0) {
+    const int n = ev.N;
+    if (n >= band[0] && n <= band[1]) {
+      C dw[ND];
+      R w[ND];
+#pragma unroll
+      for (int d = 0; d < ND; ++d) {
+        dw[d] = cmk<C>((R)ev.dwN[d][0], (R)ev.dwN[d][1]);
+        w[d] = (R)ev.wN[d];
+      }
+      C I[S];
+      R h[S];
+#pragma unroll
+      for (int s = 0; s < S; ++s) h[s] = 0;
+      bin_rows<R, WAVE, ND, S>(n, (double)n * ev.df, dw, w, sc, sf, I, h);
+#pragma unroll
+      for (int s = 0; s < S; ++s) {
+        dr[s] += I[s].x;
+        di[s] += I[s].y;
+        dh[s] += h[s];
+      }
+    }
+  }
+  const int lane = tid & 63, wid = tid >> 6;
+#pragma unroll
+  for (int s = 0; s < S; ++s) {
+    double x = wave_sum(dr[s]), y = wave_sum(di[s]), z = wave_sum(dh[s]);
+    if (lane == 0) {
+      red[wid][s][0] = x;
+      red[wid][s][1] = y;
+      red[wid][s][2] = z;
+    }
+  }
+  __syncthreads();
+  if (tid < S * 3) {
+    const int s = tid / 3, c = tid % 3;
+    const long long r = row0 + s;
+    if (r < a.n) {
+      double v = 0.0;
+      for (int w = 0; w < NT / 64; ++w) v += red[w][s][c];
+      a.part[(r * a.nchunk + chunk) * 3 + c] = v;
+    }
+  }
+}
+
+// ------------------------------------------------------------------ k_time
+struct TimeArgs {
+  EvDev ev;
+  const double* params;
+  int ld;
+  long long n;
+  int G;
+  double2* part;    // [n][G][L]
+  double* hhpart;   // [n][G]
+};
+
+template <typename R, int WAVE, int ND, int S, int P, int NT>
+__global__ __launch_bounds__(NT) void k_time(TimeArgs a) {
+  using C = cplx<R>;
+  constexpr int MAXBF8 = (S * P / 8 + NT - 1) / NT;
+  constexpr int NPT = (P + NT - 1) / NT;   // n1 values per thread per step
+  constexpr int JT = (P + NT - 1) / NT;    // window slots per thread
+  const EvDev& ev = a.ev;
+  extern __shared__ __align__(16) unsigned char smem[];
+  C* buf = reinterpret_cast<C*>(smem);       // [S][P]
+  C* tw = buf + S * P;                       // twiddles (TwLayout<P>)
+  __shared__ SampleConst sc[S];
+  __shared__ SampleFast<R> sf[S];
+  __shared__ int band[2];
+  __shared__ double red[NT / 64][S];
+  const int tid = threadIdx.x;
+  const int g = blockIdx.x;
+  const long long row0 = (long long)blockIdx.y * S;
+  init_twiddles<P, NT>(tw);
+  tile_prologue<R, S>(ev, a.params, a.ld, a.n, row0, sc, sf, band);
+
+  const int Q = ev.N / P;
+  const int n2b = (int)(((long long)g * Q) / a.G), n2e = (int)(((long long)(g + 1) * Q) / a.G);
+  double acc[S][JT][2];
+  double hacc[S];
+#pragma unroll
+  for (int s = 0; s < S; ++s) {
+    hacc[s] = 0.0;
+#pragma unroll
+    for (int jt = 0; jt < JT; ++jt) acc[s][jt][0] = acc[s][jt][1] = 0.0;
+  }
+  const unsigned nmask = (unsigned)ev.N - 1u;
+  const R invN = (R)1.0 / (R)ev.N;
+
+  for (int n2 = n2b; n2 < n2e; ++n2) {
+    // 1) integrand of this decimated sub-sequence, S rows, into LDS
+#pragma unroll
+    for (int q = 0; q < NPT; ++q) {
+      const int n1 = tid + q * NT;
+      if (n1 < P) {
+        const int n = Q * n1 + n2;
+        C I[S];
+        R h[S];
+#pragma unroll
+        for (int s = 0; s < S; ++s) h[s] = 0;
+        if (n >= band[0] && n <= band[1]) {
+          const int i = n2 * P + n1;
+          C dw[ND];
+          R w[ND];
+#pragma unroll
+          for (int d = 0; d < ND; ++d) {
+            dw[d] = load_dw<R>(ev, d, i);
+            w[d] = load_w<R>(ev, d, i);
+          }
+          bin_rows<R, WAVE, ND, S>(n, (double)n * ev.df, dw, w, sc, sf, I, h);
+        } else {
+#pragma unroll
+          for (int s = 0; s < S; ++s) I[s] = cmk<C>((R)0, (R)0);
+        }
+#pragma unroll
+        for (int s = 0; s < S; ++s) {
+          buf[s * P + n1] = I[s];
+          hacc[s] += (double)h[s];
+        }
+      }
+    }
+    __syncthreads();
+    // 2) P-point transforms in LDS
+    fft_lds<NT, MAXBF8, P>(buf, tw, S);
+    // 3) twiddle + accumulate the window bins j: X[j] += W_N^(n2 j) Y[j mod P]
+#pragma unroll
+    for (int jt = 0; jt < JT; ++jt) {
+      const int jj = tid + jt * NT;
+      if (jj < ev.L) {
+        const unsigned j = (unsigned)(ev.jlo + jj);
+        const unsigned idx = ((unsigned)n2 * j) & nmask;
+        const C t = expm2pi((R)idx * invN);
+        const int pj = (int)(j & (unsigned)(P - 1));
+#pragma unroll
+        for (int s = 0; s < S; ++s) {
+          const C y = buf[s * P + pj];
+          const C z = cmul(t, y);
+          acc[s][jt][0] += (double)z.x;
+          acc[s][jt][1] += (double)z.y;
+        }
+      }
+    }
+    __syncthreads();
+  }
+  // Nyquist bin: enters <h|h> only (the FFT drops it, base.py:327-329)
+  if (g == 0 && tid == 0 && ev.N >= band[0] && ev.N <= band[1]) {
+    C dw[ND];
+    R w[ND];
+#pragma unroll
+    for (int d = 0; d < ND; ++d) {
+      dw[d] = cmk<C>((R)ev.dwN[d][0], (R)ev.dwN[d][1]);
+      w[d] = (R)ev.wN[d];
+    }
+    C I[S];
+    R h[S];
+#pragma unroll
+    for (int s = 0; s < S; ++s) h[s] = 0;
+    bin_rows<R, WAVE, ND, S>(ev.N, (double)ev.N * ev.df, dw, w, sc, sf, I, h);
+#pragma unroll
+    for (int s = 0; s < S; ++s) hacc[s] += (double)h[s];
+  }
+  // write window partials
+#pragma unroll
+  for (int s = 0; s < S; ++s) {
+    const long long r = row0 + s;
+    if (r < a.n) {
+      double2* dst = a.part + ((size_t)r * a.G + g) * ev.L;
+#pragma unroll
+      for (int jt = 0; jt < JT; ++jt) {
+        const int jj = tid + jt * NT;
+        if (jj < ev.L) dst[jj] = make_double2(acc[s][jt][0], acc[s][jt][1]);
+      }
+    }
+  }
+  const int lane = tid & 63, wid = tid >> 6;
+#pragma unroll
+  for (int s = 0; s < S; ++s) {
+    double v = wave_sum(hacc[s]);
+    if (lane == 0) red[wid][s] = v;
+  }
+  __syncthreads();
+  if (tid < S) {
+    const long long r = row0 + tid;
+    if (r < a.n) {
+      double v = 0.0;
+      for (int w = 0; w < NT / 64; ++w) v += red[w][tid];
+      a.hhpart[(size_t)r * a.G + g] = v;
+    }
+  }
+}
+
+// ------------------------------------------------------------------ epilogues
+struct EpiArgs {
+  EvDev ev;
+  const double* params;
+  int ld;
+  long long n;
+  int nchunk;          // k_reduce chunks
+  const double* part3; // [n][nchunk][3]
+  int G;
+  const double2* part; // [n][G][L]
+  const double* hhpart;
+  double* out;
+  double* aux;
+};
+
+__device__ inline double row_valid_sentinel() { return -DBL_MAX; }  // base.py:424-425
+
+// lnL from the summed inner products (base.py:448-477, 775-792).
+__global__ void k_epi_plain(EpiArgs a) {
+  const long long r = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= a.n) return;
+  const EvDev& ev = a.ev;
+  SampleConst sc;
+  compute_sample_const(ev, a.params + r * a.ld, sc);
+  double dr = 0.0, di = 0.0, hh = 0.0;
+  const double* p = a.part3 + (size_t)r * a.nchunk * 3;
+  for (int c = 0; c < a.nchunk; ++c) {
+    dr += p[3 * c + 0];
+    di += p[3 * c + 1];
+    hh += p[3 * c + 2];
+  }
+  const double norm = 4.0 / ev.T;
+  // accumulated sum h conj(d)/S; <d|h> = sum conj(h) d / S is its conjugate
+  const double dhr = norm * dr, dhi = -norm * di;
+  hh *= norm;
+  if (a.aux) {
+    a.aux[3 * r + 0] = dhr;
+    a.aux[3 * r + 1] = dhi;
+    a.aux[3 * r + 2] = hh;
+  }
+  if (!sc.valid) {
+    a.out[r] = row_valid_sentinel();
+    return;
+  }
+  double ll;
+  if (ev.flags & GWL_DISTANCE_MARG) {
+    const double hs = sc.hscale;
+    double d = (ev.flags & GWL_PHASE_MARG) ? hypot(dhr, dhi) * hs : dhr * hs;
+    ll = spline2d(ev.sp, d, hh * hs * hs);
+  } else if (ev.flags & GWL_PHASE_MARG) {
+    ll = ln_i0(hypot(dhr, dhi)) - hh / 2.0;
+  } else {
+    ll = dhr - hh / 2.0;
+  }
+  a.out[r] = ll;
+}
+
+// Time-marginalised lnL: logsumexp over the prior window (base.py:794-820).
+template <int NT>
+__global__ __launch_bounds__(NT) void k_epi_time(EpiArgs a) {
+  const long long r = blockIdx.x;
+  if (r >= a.n) return;
+  const EvDev& ev = a.ev;
+  __shared__ double s_hh, s_jit, s_hs;
+  __shared__ int s_valid;
+  __shared__ double redv[NT / 64];
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  (void)lane;
+  if (tid == 0) {
+    SampleConst sc;
+    compute_sample_const(ev, a.params + r * a.ld, sc);
+    double hh = 0.0;
+    for (int g = 0; g < a.G; ++g) hh += a.hhpart[(size_t)r * a.G + g];
+    s_hh = hh * (4.0 / ev.T);
+    s_jit = sc.jitter;
+    s_hs = sc.hscale;
+    s_valid = sc.valid;
+  }
+  __syncthreads();
+  if (!s_valid) {
+    if (tid == 0) a.out[r] = row_valid_sentinel();
+    return;
+  }
+  const double norm = 4.0 / ev.T;
+  const double hh = s_hh, hs = s_hs, jit = s_jit;
+  // single-pass (running max, scaled sum) logsumexp per thread, then merged
+  double m = -INFINITY, s = 0.0;
+  for (int jj = tid; jj < ev.L; jj += NT) {
+    double xr = 0.0, xi = 0.0;
+    const double2* p = a.part + (size_t)r * a.G * ev.L + jj;
+    for (int g = 0; g < a.G; ++g) {
+      double2 v = p[(size_t)g * ev.L];
+      xr += v.x;
+      xi += v.y;
+    }
+    xr *= norm;
+    xi *= norm;
+    const int j = ev.jlo + jj;
+    const double t = (ev.start + (double)(j + 1) * ev.tstep) + jit;
+    if (!(t >= ev.tmin && t <= ev.tmax)) continue;
+    double v;
+    if (ev.flags & GWL_DISTANCE_MARG) {
+      double d = (ev.flags & GWL_PHASE_MARG) ? hypot(xr, xi) * hs : xr * hs;
+      v = spline2d(ev.sp, d, hh * hs * hs);
+    } else if (ev.flags & GWL_PHASE_MARG) {
+      v = ln_i0(hypot(xr, xi)) - hh / 2.0;
+    } else {
+      v = xr - hh / 2.0;
+    }
+    if (v > m) {
+      s = (m > -INFINITY ? s * exp(m - v) : 0.0) + 1.0;
+      m = v;
+    } else if (v > -INFINITY) {
+      s += exp(v - m);
+    }
+  }
+  // merge (m, s) across the block
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    double mo = __shfl_xor(m, off, 64), so = __shfl_xor(s, off, 64);
+    double mn = fmax(m, mo);
+    if (mn > -INFINITY) {
+      s = (m > -INFINITY ? s * exp(m - mn) : 0.0) + (mo > -INFINITY ? so * exp(mo - mn) : 0.0);
+    } else {
+      s = 0.0;
+    }
+    m = mn;
+  }
+  __shared__ double redm[NT / 64];
+  if (lane == 0) {
+    redm[wid] = m;
+    redv[wid] = s;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    double mm = -INFINITY;
+    for (int w = 0; w < NT / 64; ++w) mm = fmax(mm, redm[w]);
+    double tot = 0.0;
+    if (mm > -INFINITY)
+      for (int w = 0; w < NT / 64; ++w)
+        if (redm[w] > -INFINITY) tot += redv[w] * exp(redm[w] - mm);
+    const double b = (1.0 / (ev.tmax - ev.tmin)) * ev.tdelta;
+    a.out[r] = (mm > -INFINITY) ? log(b * tot) + mm : -INFINITY;
+  }
+}
+
+// ------------------------------------------------------------------ response
+// get_detector_response (interferometer.py:303-368) of one row, all bins.
+struct RespArgs {
+  EvDev ev;
+  const double* params;
+  int det;
+  double* out;  // [nf][2]
+};
+
+template <typename R, int WAVE>
+__global__ void k_response(RespArgs a) {
+  using C = cplx<R>;
+  const EvDev& ev = a.ev;
+  __shared__ SampleConst sc[1];
+  __shared__ SampleFast<R> sf[1];
+  __shared__ int band[2];
+  tile_prologue<R, 1>(ev, a.params, GWL_NPARAM, 1, 0, sc, sf, band);
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= ev.nf) return;
+  // unit "data" isolates h: I = h * conj(1)
+  double re = 0.0, im = 0.0;
+  const int d = a.det;
+  if (n >= ev.mlo[d] && n <= ev.mhi[d] && n >= sc[0].kb && n <= sc[0].ke) {
+    // single-detector projection: evaluate with only detector d's amplitude
+    SampleConst one = sc[0];
+    SampleFast<R> onef = sf[0];
+    one.dt[0] = sc[0].dt[d];
+    onef.cre[0] = sf[0].cre[d];
+    onef.cim[0] = sf[0].cim[d];
+    onef.cabs2[0] = sf[0].cabs2[d];
+    C dw[1] = {cmk<C>((R)1, (R)0)};
+    R w[1] = {(R)0};
+    C I[1];
+    R h[1] = {(R)0};
+    bin_rows<R, WAVE, 1, 1>(n, (double)n * ev.df, dw, w, &one, &onef, I, h);
+    re = (double)I[0].x * ev.sigma;
+    im = (double)I[0].y * ev.sigma;
+  }
+  a.out[2 * n] = re;
+  a.out[2 * n + 1] = im;
+}
+
+// ------------------------------------------------------------------ dispatch
+static int pick_nd(int nd) { return nd; }
+
+// rows per workgroup of the time-marginalised kernel (LDS: S P complex)
+static int time_tile_rows(int P, long long n) {
+  if (n < 64) return 1;
+  if (P == 512) return 8;
+  if (P == 1024) return 4;
+  if (P == 2048) return 2;
+  return 1;
+}
+
+// groups of sub-sequences per row: enough workgroups to fill the CUs
+static int time_tile_groups(const EvDev& ev, long long n, int S, int n_cu) {
+  const int Q = ev.N / ev.P;
+  const long long nblk = (n + S - 1) / S;
+  int G = 1;
+  while (G < Q && G < 64 && nblk * G < (long long)n_cu * 8) G *= 2;
+  return G;
+}
+
+template <typename R, int WAVE, int ND>
+static int eval_typed(gwl_event* e, const double* params, int64_t n, int ld, double* out,
+                      double* aux, hipStream_t st) {
+  const EvDev& ev = e->dev;
+  if (ev.flags & GWL_TIME_MARG) {
+    // tile: S rows x P-point transforms, G groups of sub-sequences
+    constexpr int NT = 512;
+    const int P = ev.P;
+    TimeArgs ta;
+    ta.ev = ev;
+    ta.params = params;
+    ta.ld = ld;
+    ta.n = n;
+    const int S = time_tile_rows(P, n);
+    const long long nblk = (n + S - 1) / S;
+    const int G = time_tile_groups(ev, n, S, e->ctx->n_cu);
+    ta.G = G;
+    ta.part = reinterpret_cast<double2*>(e->ws);
+    ta.hhpart = reinterpret_cast<double*>(reinterpret_cast<char*>(e->ws) +
+                                          (size_t)n * G * ev.L * sizeof(double2));
+    dim3 grid(G, (unsigned)nblk);
+#define GWL_LAUNCH_TIME(SS, PP)                                                                  \
+  do {                                                                                           \
+    const size_t lds = (size_t)(SS * PP + TwLayout<PP>::kEntries) * sizeof(cplx<R>);            \
+    if (lds > e->ctx->lds_per_block) {                                                           \
+      set_error("time-marginalisation tile exceeds the LDS of a workgroup");                     \
+      return GWL_E_UNSUPPORTED;                                                                  \
+    }                                                                                            \
+    hipLaunchKernelGGL((k_time<R, WAVE, ND, SS, PP, NT>), grid, dim3(NT), lds, st, ta);          \
+  } while (0)
+    if (P == 512 && S == 8) GWL_LAUNCH_TIME(8, 512);
+    else if (P == 512) GWL_LAUNCH_TIME(1, 512);
+    else if (P == 1024 && S == 4) GWL_LAUNCH_TIME(4, 1024);
+    else if (P == 1024) GWL_LAUNCH_TIME(1, 1024);
+    else if (P == 2048 && S == 2) GWL_LAUNCH_TIME(2, 2048);
+    else if (P == 2048) GWL_LAUNCH_TIME(1, 2048);
+    else if (P == 4096) GWL_LAUNCH_TIME(1, 4096);
+    else if (P == 8192) GWL_LAUNCH_TIME(1, 8192);
+    else if (P == 16384) GWL_LAUNCH_TIME(1, 16384);
+    else {
+      set_error("time-marginalisation transform size not supported");
+      return GWL_E_UNSUPPORTED;
+    }
+#undef GWL_LAUNCH_TIME
+    GWL_HIP_CHECK(hipGetLastError());
+    EpiArgs ea{};
+    ea.ev = ev;
+    ea.params = params;
+    ea.ld = ld;
+    ea.n = n;
+    ea.G = G;
+    ea.part = ta.part;
+    ea.hhpart = ta.hhpart;
+    ea.out = out;
+    ea.aux = aux;
+    hipLaunchKernelGGL(k_epi_time<256>, dim3((unsigned)n), dim3(256), 0, st, ea);
+    GWL_HIP_CHECK(hipGetLastError());
+    return GWL_OK;
+  }
+  constexpr int NT = 256;
+  constexpr int S = 8;
+  RedArgs ra;
+  ra.ev = ev;
+  ra.params = params;
+  ra.ld = ld;
+  ra.n = n;
+  // chunking fixed per event (independent of the batch) so that a row's
+  // reduction order does not depend on what it is batched with
+  ra.chunk = NT * 4;
+  ra.nchunk = (ev.N + ra.chunk - 1) / ra.chunk;
+  ra.part = reinterpret_cast<double*>(e->ws);
+  const long long nblk = (n + S - 1) / S;
+  hipLaunchKernelGGL((k_reduce<R, WAVE, ND, S, NT>), dim3(ra.nchunk, (unsigned)nblk), dim3(NT), 0,
+                     st, ra);
+  GWL_HIP_CHECK(hipGetLastError());
+  EpiArgs ea{};
+  ea.ev = ev;
+  ea.params = params;
+  ea.ld = ld;
+  ea.n = n;
+  ea.nchunk = ra.nchunk;
+  ea.part3 = ra.part;
+  ea.out = out;
+  ea.aux = aux;
+  hipLaunchKernelGGL(k_epi_plain, dim3((unsigned)((n + 127) / 128)), dim3(128), 0, st, ea);
+  GWL_HIP_CHECK(hipGetLastError());
+  return GWL_OK;
+}
+
+template <typename R, int WAVE>
+static int eval_nd(gwl_event* e, const double* params, int64_t n, int ld, double* out, double* aux,
+                   hipStream_t st) {
+  switch (pick_nd(e->dev.nd)) {
+    case 1: return eval_typed<R, WAVE, 1>(e, params, n, ld, out, aux, st);
+    case 2: return eval_typed<R, WAVE, 2>(e, params, n, ld, out, aux, st);
+    case 3: return eval_typed<R, WAVE, 3>(e, params, n, ld, out, aux, st);
+    case 4: return eval_typed<R, WAVE, 4>(e, params, n, ld, out, aux, st);
+  }
+  set_error("unsupported detector count");
+  return GWL_E_INVALID;
+}
+
+size_t workspace_bytes(const gwl_event* e, int64_t rows) {
+  const EvDev& ev = e->dev;
+  if (ev.flags & GWL_TIME_MARG) {
+    // partials [n][G][L] double2 + [n][G] double; n G <= max(n, 2 S 8 n_cu)
+    // for every n <= rows (G halves as the row blocks double)
+    const long long bound = std::max<long long>((long long)rows * time_tile_groups(ev, rows, time_tile_rows(ev.P, rows), e->ctx->n_cu),
+                                                2LL * 8 * 8 * e->ctx->n_cu);
+    return (size_t)bound * (ev.L * sizeof(double2) + sizeof(double)) + 4096;
+  }
+  const int nchunk = (ev.N + 1023) / 1024;
+  return (size_t)rows * nchunk * 3 * sizeof(double) + 4096;
+}
+
+int launch_eval(gwl_event* e, const double* params, int64_t n, int ld, int precision, double* out,
+                double* aux, hipStream_t st) {
+  (void)hipGetLastError();  // clear a stale error left by an unrelated call
+  const int wave = e->dev.approx;
+  if (precision == GWL_FP32) {
+    if (wave == GWL_TAYLORF2) return eval_nd<float, GWL_TAYLORF2>(e, params, n, ld, out, aux, st);
+    return eval_nd<float, GWL_SINE_GAUSSIAN>(e, params, n, ld, out, aux, st);
+  }
+  if (wave == GWL_TAYLORF2) return eval_nd<double, GWL_TAYLORF2>(e, params, n, ld, out, aux, st);
+  return eval_nd<double, GWL_SINE_GAUSSIAN>(e, params, n, ld, out, aux, st);
+}
+
+int launch_response(gwl_event* e, const double* params_dev, int det, int precision, double* out_dev,
+                    hipStream_t st) {
+  RespArgs ra;
+  ra.ev = e->dev;
+  ra.params = params_dev;
+  ra.det = det;
+  ra.out = out_dev;
+  dim3 grid((e->dev.nf + 255) / 256);
+  if (precision == GWL_FP32) {
+    if (e->dev.approx == GWL_TAYLORF2)
+      hipLaunchKernelGGL((k_response<float, GWL_TAYLORF2>), grid, dim3(256), 0, st, ra);
+    else
+      hipLaunchKernelGGL((k_response<float, GWL_SINE_GAUSSIAN>), grid, dim3(256), 0, st, ra);
+  } else {
+    if (e->dev.approx == GWL_TAYLORF2)
+      hipLaunchKernelGGL((k_response<double, GWL_TAYLORF2>), grid, dim3(256), 0, st, ra);
+    else
+      hipLaunchKernelGGL((k_response<double, GWL_SINE_GAUSSIAN>), grid, dim3(256), 0, st, ra);
+  }
+  GWL_HIP_CHECK(hipGetLastError());
+  return GWL_OK;
+}
+
+}  // namespace gwl

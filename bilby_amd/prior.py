@@ -1,0 +1,187 @@
+"""Priors the likelihood reads (setup only): the marginalised keys need
+``minimum``, ``maximum``, ``prob`` and ``rescale`` (bilby/core/prior).
+
+UniformComovingVolume uses an in-package flat Lambda-CDM cosmology
+(H0 = 67.74 km/s/Mpc, Omega_m = 0.3075, no radiation), because astropy is not
+available; bilby's default Planck15 (astropy) differs slightly in the
+distance-redshift relation, so the prior *shape* is not identical to bilby's.
+The likelihood only consumes the resulting prior array.
+"""
+import numpy as np
+
+
+class Prior:
+    def __init__(self, minimum=-np.inf, maximum=np.inf, name=None, latex_label=None, unit=None,
+                 boundary=None):
+        self.minimum = minimum
+        self.maximum = maximum
+        self.name = name
+        self.latex_label = latex_label
+        self.unit = unit
+        self.boundary = boundary
+
+    @property
+    def is_fixed(self):
+        return False
+
+    def sample(self, size=None, rng=None):
+        rng = rng if rng is not None else np.random.default_rng()
+        return self.rescale(rng.uniform(0, 1, size))
+
+
+class Uniform(Prior):
+    """bilby/core/prior/analytical.py:191-242."""
+
+    def rescale(self, val):
+        return self.minimum + np.asarray(val) * (self.maximum - self.minimum)
+
+    def prob(self, val):
+        val = np.asarray(val)
+        return ((val >= self.minimum) & (val <= self.maximum)) / (self.maximum - self.minimum)
+
+
+class DeltaFunction(Prior):
+    def __init__(self, peak, name=None, **kw):
+        super().__init__(minimum=peak, maximum=peak, name=name)
+        self.peak = peak
+
+    @property
+    def is_fixed(self):
+        return True
+
+    def rescale(self, val):
+        return self.peak * np.ones_like(np.asarray(val, float))
+
+    def prob(self, val):
+        return (np.asarray(val) == self.peak) * 1.0
+
+
+class PowerLaw(Prior):
+    """p(x) ~ x^alpha on [minimum, maximum] (bilby/core/prior/analytical.py)."""
+
+    def __init__(self, alpha, minimum, maximum, name=None, **kw):
+        super().__init__(minimum=minimum, maximum=maximum, name=name)
+        self.alpha = alpha
+
+    def rescale(self, val):
+        val = np.asarray(val, float)
+        if self.alpha == -1:
+            return self.minimum * np.exp(val * np.log(self.maximum / self.minimum))
+        a1 = self.alpha + 1
+        return (self.minimum ** a1 + val * (self.maximum ** a1 - self.minimum ** a1)) ** (1 / a1)
+
+    def prob(self, val):
+        val = np.asarray(val, float)
+        if self.alpha == -1:
+            norm = 1 / np.log(self.maximum / self.minimum)
+        else:
+            a1 = self.alpha + 1
+            norm = a1 / (self.maximum ** a1 - self.minimum ** a1)
+        inside = (val >= self.minimum) & (val <= self.maximum)
+        return np.where(inside, norm * np.abs(val) ** self.alpha, 0.0)
+
+
+def _trapezoid(y, x):
+    return float(np.sum((y[1:] + y[:-1]) * np.diff(x)) / 2)
+
+
+class Interped(Prior):
+    """bilby/core/prior/interpolated.py:10-177: linear interpolation of a
+    tabulated density, re-gridded on linspace(minimum, maximum, len(xx)),
+    normalised by the trapezoid rule; rescale by the inverse of the
+    cumulative trapezoid."""
+
+    def __init__(self, xx, yy, minimum=np.nan, maximum=np.nan, name=None, **kw):
+        xx = np.asarray(xx, float)
+        yy = np.asarray(yy, float)
+        minimum = float(np.nanmax(np.array((xx.min(), minimum))))
+        maximum = float(np.nanmin(np.array((xx.max(), maximum))))
+        super().__init__(minimum=minimum, maximum=maximum, name=name)
+        self._xx0, self._yy0 = xx, yy
+        self.xx = np.linspace(minimum, maximum, len(xx))
+        self.yy = self._interp_all(self.xx)
+        self.yy = self.yy / _trapezoid(self.yy, self.xx)
+        ydx = (self.yy[1:] + self.yy[:-1]) * np.diff(self.xx) / 2
+        self.YY = np.concatenate([[0.0], np.cumsum(ydx)])
+        self.YY[-1] = 1
+
+    def _interp_all(self, x):
+        out = np.interp(x, self._xx0, self._yy0)
+        out[(x < self._xx0[0]) | (x > self._xx0[-1])] = 0.0
+        return out
+
+    def prob(self, val):
+        val = np.asarray(val, float)
+        out = np.interp(val, self.xx, self.yy)
+        out = np.where((val < self.xx[0]) | (val > self.xx[-1]), 0.0, out)
+        return out[()] if out.ndim == 0 else out
+
+    def cdf(self, val):
+        return np.interp(val, self.xx, self.YY, left=0.0, right=1.0)
+
+    def rescale(self, val):
+        val = np.asarray(val, float)
+        if np.any((val < 0) | (val > 1)):
+            raise ValueError("rescale input outside [0, 1]")
+        out = np.interp(val, self.YY, self.xx)
+        return out[()] if out.ndim == 0 else out
+
+
+class FlatLambdaCDM:
+    """Flat Lambda-CDM distances (Mpc); radiation neglected."""
+
+    def __init__(self, H0=67.74, Om0=0.3075):
+        self.H0 = H0
+        self.Om0 = Om0
+        self.hubble_distance = 299792.458 / H0
+        z = np.linspace(0.0, 3.0, 30001)
+        ez = np.sqrt(Om0 * (1 + z) ** 3 + (1 - Om0))
+        dc = np.concatenate([[0.0], np.cumsum((1 / ez[1:] + 1 / ez[:-1]) / 2 * np.diff(z))])
+        self._z = z
+        self._dc = dc * self.hubble_distance
+
+    def comoving_distance(self, z):
+        return np.interp(z, self._z, self._dc)
+
+    def luminosity_distance(self, z):
+        return (1 + np.asarray(z)) * self.comoving_distance(z)
+
+    def z_at_luminosity_distance(self, dl):
+        return np.interp(dl, (1 + self._z) * self._dc, self._z)
+
+    def differential_comoving_volume(self, z):
+        z = np.asarray(z, float)
+        ez = np.sqrt(self.Om0 * (1 + z) ** 3 + (1 - self.Om0))
+        return self.hubble_distance * self.comoving_distance(z) ** 2 / ez
+
+
+class UniformComovingVolume(Interped):
+    """bilby/gw/prior.py:310-326 (p(z) ~ dVc/dz on z in [0.99 z_min,
+    1.01 z_max], mapped to luminosity distance, prior.py:277-282)."""
+
+    def __init__(self, minimum, maximum, name="luminosity_distance", cosmology=None, **kw):
+        if name != "luminosity_distance":
+            raise ValueError("only luminosity_distance is supported")
+        self.cosmology = cosmology or FlatLambdaCDM()
+        zmin = self.cosmology.z_at_luminosity_distance(minimum)
+        zmax = self.cosmology.z_at_luminosity_distance(maximum)
+        zs = np.linspace(zmin * 0.99, zmax * 1.01, 1000)
+        p_dz = self.cosmology.differential_comoving_volume(zs)
+        dl = self.cosmology.luminosity_distance(zs)
+        p_dl = p_dz / np.gradient(dl, zs)
+        super().__init__(dl, p_dl, minimum=minimum, maximum=maximum, name=name)
+
+
+class PriorDict(dict):
+    def copy(self):
+        return PriorDict(self)
+
+    def sample(self, size, rng=None):
+        rng = rng if rng is not None else np.random.default_rng()
+        out = {}
+        for k, p in self.items():
+            if isinstance(p, Prior):
+                out[k] = p.sample(size, rng)
+            else:
+                out[k] = np.full(size, float(p))
+        return out

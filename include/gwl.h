@@ -1,0 +1,177 @@
+/*
+ * gwl.h -- C ABI of the MI355X (gfx950) compact-binary likelihood library
+ * (libgwl.so, built from bilby_amd/csrc/).
+ *
+ * The reference exposes this path only as a duck-typed Python object,
+ * bilby.gw.likelihood.GravitationalWaveTransient
+ * (bilby/gw/likelihood/base.py:150-229 constructor, :419-446
+ * log_likelihood_ratio).  It has no FFI; these entry points are what a
+ * Python (ctypes) binding of that interface calls.  Each declaration names
+ * the reference code it replaces.
+ *
+ * Conventions
+ *  - Every function returns 0 on success and a negative GWL_E* code on
+ *    failure; gwl_last_error() then holds a message (thread-local).
+ *    No C++ exception crosses this boundary.
+ *  - "device pointer" arguments live on the context's device; "host
+ *    pointer" arguments are ordinary host memory.
+ *  - Parameter rows are GWL_NPARAM float64 values in the column order of
+ *    enum gwl_param.  Unused columns are ignored.
+ *  - stream is a hipStream_t passed as void* (NULL = the legacy stream).
+ */
+#ifndef GWL_H
+#define GWL_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GWL_NPARAM 16
+#define GWL_MAX_DET 4
+
+enum gwl_status {
+  GWL_OK = 0,
+  GWL_E_INVALID = -1,   /* bad argument / shape (bilby: ValueError, TypeError) */
+  GWL_E_HIP = -2,       /* HIP runtime error */
+  GWL_E_NOMEM = -3,     /* allocation failed or HBM budget exceeded */
+  GWL_E_UNSUPPORTED = -4
+};
+
+/* Column order of a parameter row (float64). */
+enum gwl_param {
+  GWL_P_RA = 0, GWL_P_DEC = 1, GWL_P_PSI = 2, GWL_P_GEOCENT_TIME = 3,
+  GWL_P_TIME_JITTER = 4, GWL_P_LUMINOSITY_DISTANCE = 5, GWL_P_THETA_JN = 6,
+  GWL_P_PHASE = 7, GWL_P_MASS_1 = 8, GWL_P_MASS_2 = 9, GWL_P_CHI_1 = 10,
+  GWL_P_CHI_2 = 11, GWL_P_HRSS = 12, GWL_P_Q = 13, GWL_P_FREQUENCY = 14
+};
+
+/* Source models: bilby/gw/source.py:1153-1209 (sinegaussian) and the
+ * TaylorF2 approximant reached through source.py:269-348, 552-690. */
+enum gwl_approximant { GWL_SINE_GAUSSIAN = 1, GWL_TAYLORF2 = 2 };
+
+/* Marginalisation flags: base.py:150-229 constructor arguments
+ * time_marginalization, distance_marginalization, phase_marginalization,
+ * jitter_time. */
+enum gwl_flags {
+  GWL_TIME_MARG = 1, GWL_DISTANCE_MARG = 2, GWL_PHASE_MARG = 4, GWL_JITTER_TIME = 8
+};
+
+/* Arithmetic of an evaluation: all-float64, or float32 bulk arithmetic with
+ * float64 phase arguments and float64 final accumulation. */
+enum gwl_precision { GWL_FP64 = 0, GWL_FP32 = 1 };
+
+typedef struct gwl_ctx gwl_ctx;
+typedef struct gwl_event gwl_event;
+
+/* One analysis segment ("event"): the state bilby's likelihood holds in its
+ * InterferometerList (bilby/gw/detector/networks.py:13-352) plus the
+ * constructor options of GravitationalWaveTransient (base.py:150-229). */
+typedef struct {
+  int32_t n_det;                 /* 1..GWL_MAX_DET */
+  int32_t n_freq;                /* N_f = round(T fs)/2 + 1 (series.py:115-134) */
+  double duration;               /* T [s] */
+  double sampling_frequency;     /* fs [Hz] */
+  double start_time;             /* GPS start of the segment */
+  const int32_t* mask_lo;        /* host [n_det]: first bin with f >= f_min (strain_data.py:142-164) */
+  const int32_t* mask_hi;        /* host [n_det]: last bin with f <= min(f_max, fs/2) */
+  const double* strain;          /* host [n_det][n_freq][2] frequency-domain data (complex128) */
+  const double* psd;             /* host [n_det][n_freq] one-sided PSD on the grid */
+  const double* detector_tensor; /* host [n_det][9] (detector/geometry.py:250-269) */
+  const double* vertex;          /* host [n_det][3] metres (detector/geometry.py:200-214) */
+  int32_t approximant;           /* enum gwl_approximant */
+  double wf_minimum_frequency;   /* waveform band (source.py:618-619); <= 0: none */
+  double wf_maximum_frequency;   /* <= 0: the grid's last frequency */
+  double reference_frequency;    /* TaylorF2 f_ref (source.py:338-343 default 50 Hz) */
+  uint32_t flags;                /* enum gwl_flags */
+  double time_prior_minimum;     /* Uniform geocent_time prior (base.py:794-820) */
+  double time_prior_maximum;
+  double reference_distance;     /* d_ref = prior.rescale(0.5) (base.py:216) */
+} gwl_event_desc;
+
+/* Context on one HIP device. */
+int gwl_ctx_create(int32_t device, gwl_ctx** out);
+int gwl_ctx_destroy(gwl_ctx* ctx);
+const char* gwl_last_error(void);
+
+/* Upload one event and make it resident in HBM.  Replaces the per-call
+ * re-reads of bilby's strain/PSD/mask accessors (strain_data.py:212-233,
+ * interferometer.py:551-564, psd.py:246-251).  *hbm_bytes receives the
+ * device bytes the event holds. */
+int gwl_event_create(gwl_ctx* ctx, const gwl_event_desc* desc, gwl_event** out,
+                     int64_t* hbm_bytes);
+int gwl_event_destroy(gwl_event* ev);
+
+/* Attach the distance-marginalisation interpolant: a bicubic tensor
+ * B-spline (knots tx[nx+4], ty[ny+4], coefficients c[nx][ny]) evaluated
+ * with value -inf outside bbox = {xmin, xmax, ymin, ymax}.  Replaces
+ * BoundedRectBivariateSpline (bilby/core/utils/calculus.py:221-262) built
+ * in base.py:916-934. */
+int gwl_event_set_distance_spline(gwl_event* ev, int32_t nx, int32_t ny, const double* tx,
+                                  const double* ty, const double* coeffs, const double* bbox);
+
+/* Pre-size the event's device workspace for batches up to max_rows (so
+ * that gwl_eval allocates nothing afterwards). */
+int gwl_event_reserve(gwl_event* ev, int64_t max_rows);
+
+/* Noise log-likelihood of the event's data, float64 on the device:
+ * -sum_det |4/T sum_mask |d|^2 / S| / 2 (base.py:402-413). */
+int gwl_noise_log_likelihood(gwl_event* ev, double* out);
+
+/* Batched log-likelihood ratio: out[i] = lnL ratio of params[i*ld ..].
+ * Replaces base.py:419-446 log_likelihood_ratio (one row per call there).
+ * params, out: device pointers.  aux (device, may be NULL) receives per row
+ * {Re<d|h>, Im<d|h>, <h|h>} summed over detectors (base.py:126-134) for
+ * evaluations without time marginalisation.  Rows whose waveform is
+ * undefined (non-physical masses, ...) get -1.7976931348623157e308 as
+ * base.py:424-425 does. */
+int gwl_eval(gwl_event* ev, const double* params, int64_t n_rows, int32_t ld,
+             int32_t precision, double* out, double* aux, void* stream);
+
+/* Detector response of one parameter row: bilby
+ * Interferometer.get_detector_response (interferometer.py:303-368) of the
+ * event's source model.  params: host [GWL_NPARAM]; out: host
+ * [n_freq][2] complex128. */
+int gwl_detector_response(gwl_event* ev, const double* params, int32_t det,
+                          int32_t precision, double* out);
+
+/* Distance-marginalisation lookup table on the GPU: base.py:994-1018
+ * (_create_lookup_table).  distance_array/prior_array: host [n_dist];
+ * d_axis: host [n_d] <d|h> axis, h_axis: host [n_h] <h|h> axis (the
+ * reference uses n_h = 400, n_d = 800 on the axes of base.py:890-914);
+ * out: host [n_h][n_d] float64, normalised by the prior as base.py:1013-1017. */
+int gwl_distance_table(gwl_ctx* ctx, const double* distance_array, const double* prior_array,
+                       int32_t n_dist, double reference_distance, int32_t phase_marginalization,
+                       const double* d_axis, int32_t n_d, const double* h_axis, int32_t n_h,
+                       double* out);
+
+/* Host-side coefficient solve of the interpolating (s = 0) bicubic spline on
+ * a rectangular grid x[nx], y[ny], z[nx][ny] with not-a-knot knots, the
+ * spline scipy's RectBivariateSpline(kx=3, ky=3, s=0) builds
+ * (calculus.py:221-237).  Outputs tx[nx+4], ty[ny+4], c[nx][ny]. */
+int gwl_spline2d_coefficients(int32_t nx, int32_t ny, const double* x, const double* y,
+                              const double* z, double* tx, double* ty, double* c);
+
+/* Host-side evaluation of such a spline at m points (for checks on hosts
+ * without a GPU; the device evaluates the same formula). */
+int gwl_spline2d_eval_host(int32_t nx, int32_t ny, const double* tx, const double* ty,
+                           const double* c, int64_t m, const double* x, const double* y,
+                           double* out);
+
+/* Batched complex FFT of length P (power of two, 2..16384) with the LDS
+ * radix kernel used by time marginalisation (numpy.fft.fft as called at
+ * base.py:326).  in/out: device [batch][P][2]; precision selects float32
+ * (GWL_FP32) or float64 data. */
+int gwl_fft_batch(gwl_ctx* ctx, const void* in, void* out, int64_t batch, int32_t P,
+                  int32_t precision, void* stream);
+
+/* Natural log of the modified Bessel function I0 (bilby/gw/utils.py:1006-1022
+ * ln_i0) evaluated on the device: in/out device float64 [n]. */
+int gwl_ln_i0(gwl_ctx* ctx, const double* in, double* out, int64_t n, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GWL_H */
