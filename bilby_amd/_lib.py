@@ -68,6 +68,9 @@ def _load():
                           ctypes.c_int),
         "gwl_ln_i0": ([vp, vp, vp, ctypes.c_int64, vp], ctypes.c_int),
         "gwl_noise_log_likelihood": ([vp, _dp], ctypes.c_int),
+        "gwl_event_timing": ([vp, ctypes.c_int32], ctypes.c_int),
+        "gwl_event_set_tile": ([vp, ctypes.c_int32, ctypes.c_int32], ctypes.c_int),
+        "gwl_event_timing_read": ([vp, _dp, ctypes.POINTER(ctypes.c_int64)], ctypes.c_int),
     }
     for name, (argt, rest) in sig.items():
         fn = getattr(lib, name)
@@ -101,4 +104,5 @@ EXPORTED_SYMBOLS = [
     "gwl_event_destroy", "gwl_event_set_distance_spline", "gwl_event_reserve", "gwl_eval",
     "gwl_detector_response", "gwl_distance_table", "gwl_spline2d_coefficients",
     "gwl_spline2d_eval_host", "gwl_fft_batch", "gwl_ln_i0", "gwl_noise_log_likelihood",
+    "gwl_event_timing", "gwl_event_timing_read", "gwl_event_set_tile",
 ]

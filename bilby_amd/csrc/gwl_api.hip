@@ -433,9 +433,41 @@ int gwl_event_create(gwl_ctx* ctx, const gwl_event_desc* d, gwl_event** out, int
   return GWL_OK;
 }
 
+int gwl_event_set_tile(gwl_event* e, int32_t rows, int32_t wg_per_cu) {
+  if (!e || rows < 0 || rows > 16 || wg_per_cu < 0 || wg_per_cu > 64) {
+    set_error("gwl_event_set_tile: invalid argument");
+    return GWL_E_INVALID;
+  }
+  e->tile_rows = rows;
+  e->tile_wg_per_cu = wg_per_cu;
+  return GWL_OK;
+}
+
+int gwl_event_timing(gwl_event* e, int32_t enable) {
+  if (!e) { set_error("gwl_event_timing: NULL event"); return GWL_E_INVALID; }
+  GWL_HIP_CHECK(hipSetDevice(e->ctx->device));
+  if (enable && !e->t0) {
+    GWL_HIP_CHECK(hipEventCreate(&e->t0));
+    GWL_HIP_CHECK(hipEventCreate(&e->t1));
+  }
+  e->timing = enable != 0;
+  return GWL_OK;
+}
+
+int gwl_event_timing_read(gwl_event* e, double* total_ms, int64_t* launches) {
+  if (!e || !total_ms || !launches) { set_error("gwl_event_timing_read: invalid argument"); return GWL_E_INVALID; }
+  *total_ms = e->timed_ms;
+  *launches = e->timed_launches;
+  e->timed_ms = 0.0;
+  e->timed_launches = 0;
+  return GWL_OK;
+}
+
 int gwl_event_destroy(gwl_event* e) {
   if (!e) return GWL_OK;
   (void)hipSetDevice(e->ctx->device);
+  if (e->t0) (void)hipEventDestroy(e->t0);
+  if (e->t1) (void)hipEventDestroy(e->t1);
   if (e->data) (void)hipFree(e->data);
   if (e->spline) (void)hipFree(e->spline);
   if (e->ws) (void)hipFree(e->ws);

@@ -242,6 +242,55 @@ __device__ inline int last_bin_le(double f, double df, int lo, int hi) {
   return (int)k;
 }
 
+// TaylorF2 per-row constants: phase-basis coefficients c[kNB] in cycles
+// (Psi(f) / 2pi = sum_k c_k b_k(f), reference-frequency phase and t_c = 0
+// folded in), the amplitude prefactor amp0 (divided by ``sigma``) of
+// A(f) = amp0 f^-7/6, and the ISCO frequency.  Returns 0 for rows whose
+// waveform is undefined (non-positive masses or distance, |chi| >= 1,
+// non-finite values).
+__device__ inline int tf2_row_constants(const double* row, double fref, double sigma, double* cout,
+                                        double& amp0, double& fisco) {
+  double m1 = row[GWL_P_MASS_1], m2 = row[GWL_P_MASS_2];
+  double chi1 = row[GWL_P_CHI_1], chi2 = row[GWL_P_CHI_2];
+  double dl = row[GWL_P_LUMINOSITY_DISTANCE];
+  double iota = row[GWL_P_THETA_JN], phase = row[GWL_P_PHASE];
+  int valid = 1;
+  if (!(m1 > 0.0) || !(m2 > 0.0) || !(dl > 0.0) || !(fabs(chi1) < 1.0) || !(fabs(chi2) < 1.0) ||
+      !isfinite(m1 + m2 + dl + iota + phase)) {
+    valid = 0;
+  }
+  double M = m1 + m2;
+  double mtsun = kG * kSolarMass / (kSpeedOfLight * kSpeedOfLight * kSpeedOfLight);
+  double msec = M * mtsun;
+  double phi[8], phil5, phil6, eta;
+  tf2_phis(m1, m2, chi1, chi2, phi, phil5, phil6, eta);
+  double piM = kPi * msec;
+  double p13 = cbrt(piM), ip13 = 1.0 / p13;
+  double lnpiM = log(piM);
+  double pre = 3.0 / (128.0 * eta);
+  double c[kNB];
+  c[0] = pre * phi[0] * ip13 * ip13 * ip13 * ip13 * ip13;
+  c[1] = pre * phi[2] * ip13 * ip13 * ip13;
+  c[2] = pre * phi[3] * ip13 * ip13;
+  c[3] = pre * phi[4] * ip13;
+  c[4] = pre * (phi[5] + phil5 * lnpiM / 3.0);
+  c[5] = pre * (phi[6] + phil6 * lnpiM / 3.0) * p13;
+  c[6] = pre * phi[7] * p13 * p13;
+  c[7] = pre * phil5 / 3.0;
+  c[8] = pre * phil6 / 3.0 * p13;
+  double bref[kNB];
+  tf2_basis(fref, bref);
+  double psi_ref = tf2_phase_cycles(c, bref);  // radians here (c not yet scaled)
+  c[4] += -psi_ref - 2.0 * phase - 0.25 * kPi;
+  for (int k = 0; k < kNB; ++k) cout[k] = c[k] / kTwoPi;
+  cout[9] = 0.0;
+  double mc = pow(eta, 0.6) * msec;
+  amp0 = sqrt(5.0 / 24.0) * pow(kPi, -2.0 / 3.0) * kSpeedOfLight / (dl * 1e6 * kParsec) *
+         pow(mc, 5.0 / 6.0) / sigma;
+  fisco = 1.0 / (pow(6.0, 1.5) * kPi * msec);
+  return valid;
+}
+
 // Per-row prologue: everything that depends on the parameter row only.
 __device__ inline void compute_sample_const(const EvDev& ev, const double* row, SampleConst& sc) {
   double ra = row[GWL_P_RA], dec = row[GWL_P_DEC], psi = row[GWL_P_PSI];
@@ -267,41 +316,9 @@ __device__ inline void compute_sample_const(const EvDev& ev, const double* row, 
   }
   sc.valid = 1;
   if (ev.approx == GWL_TAYLORF2) {
-    double m1 = row[GWL_P_MASS_1], m2 = row[GWL_P_MASS_2];
-    double chi1 = row[GWL_P_CHI_1], chi2 = row[GWL_P_CHI_2];
-    double iota = row[GWL_P_THETA_JN], phase = row[GWL_P_PHASE];
-    if (!(m1 > 0.0) || !(m2 > 0.0) || !(dl > 0.0) || !(fabs(chi1) < 1.0) || !(fabs(chi2) < 1.0) ||
-        !isfinite(m1 + m2 + dl + iota + phase)) {
-      sc.valid = 0;
-    }
-    double M = m1 + m2;
-    double mtsun = kG * kSolarMass / (kSpeedOfLight * kSpeedOfLight * kSpeedOfLight);
-    double msec = M * mtsun;
-    double phi[8], phil5, phil6, eta;
-    tf2_phis(m1, m2, chi1, chi2, phi, phil5, phil6, eta);
-    double piM = kPi * msec;
-    double p13 = cbrt(piM), ip13 = 1.0 / p13;
-    double lnpiM = log(piM);
-    double pre = 3.0 / (128.0 * eta);
-    double c[kNB];
-    c[0] = pre * phi[0] * ip13 * ip13 * ip13 * ip13 * ip13;
-    c[1] = pre * phi[2] * ip13 * ip13 * ip13;
-    c[2] = pre * phi[3] * ip13 * ip13;
-    c[3] = pre * phi[4] * ip13;
-    c[4] = pre * (phi[5] + phil5 * lnpiM / 3.0);
-    c[5] = pre * (phi[6] + phil6 * lnpiM / 3.0) * p13;
-    c[6] = pre * phi[7] * p13 * p13;
-    c[7] = pre * phil5 / 3.0;
-    c[8] = pre * phil6 / 3.0 * p13;
-    double bref[kNB];
-    tf2_basis(ev.fref, bref);
-    double psi_ref = tf2_phase_cycles(c, bref);  // radians here (c not yet scaled)
-    c[4] += -psi_ref - 2.0 * phase - 0.25 * kPi;
-    for (int k = 0; k < kNB; ++k) sc.c[k] = c[k] / kTwoPi;
-    sc.c[9] = 0.0;
-    double mc = pow(eta, 0.6) * msec;
-    double amp0 = sqrt(5.0 / 24.0) * pow(kPi, -2.0 / 3.0) * kSpeedOfLight / (dl * 1e6 * kParsec) *
-                  pow(mc, 5.0 / 6.0) / ev.sigma;
+    double iota = row[GWL_P_THETA_JN];
+    double amp0, fisco;
+    sc.valid = tf2_row_constants(row, ev.fref, ev.sigma, sc.c, amp0, fisco);
     double ci = cos(iota);
     double ap = amp0 * (1.0 + ci * ci) / 2.0, ax = amp0 * ci;
     for (int d = 0; d < ev.nd; ++d) {
@@ -309,7 +326,6 @@ __device__ inline void compute_sample_const(const EvDev& ev, const double* row, 
       sc.cim[d] = -fc[d] * ax;
       sc.cabs2[d] = sc.cre[d] * sc.cre[d] + sc.cim[d] * sc.cim[d];
     }
-    double fisco = 1.0 / (pow(6.0, 1.5) * kPi * msec);
     double fmax = fmin(fisco, ev.wf_fmax);
     int kb = ev.wf_fmin > 0.0 ? first_bin_ge(ev.wf_fmin, ev.df, lo, hi) : lo;
     int ke = last_bin_le(fmax, ev.df, lo, hi);

@@ -29,6 +29,14 @@ struct gwl_event {
   size_t ws_bytes = 0;
   int64_t ws_rows = 0;
   bool has_spline = false;
+  // tile overrides (0 = heuristic)
+  int tile_rows = 0;
+  int tile_wg_per_cu = 0;
+  // kernel timing (measurement only)
+  bool timing = false;
+  hipEvent_t t0 = nullptr, t1 = nullptr;
+  double timed_ms = 0.0;
+  int64_t timed_launches = 0;
 };
 
 namespace gwl {
@@ -37,6 +45,8 @@ int hip_fail(hipError_t e, const char* what);
 size_t workspace_bytes(const gwl_event* ev, int64_t rows);
 int launch_eval(gwl_event* ev, const double* params, int64_t n, int ld, int precision, double* out,
                 double* aux, hipStream_t stream);
+int timing_begin(gwl_event* ev, hipStream_t stream);
+int timing_end(gwl_event* ev, hipStream_t stream);
 int launch_response(gwl_event* ev, const double* params_dev, int det, int precision, double* out_dev,
                     hipStream_t stream);
 }  // namespace gwl

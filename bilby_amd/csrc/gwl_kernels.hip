@@ -429,11 +429,59 @@ struct EpiArgs {
 
 __device__ inline double row_valid_sentinel() { return -DBL_MAX; }  // base.py:424-425
 
-// lnL from the summed inner products (base.py:448-477, 775-792).
-__global__ void k_epi_plain(EpiArgs a) {
+// Knots of the distance spline staged in LDS: tx[nx + 4] then ty[ny + 4].
+__device__ inline void stage_knots(const EvDev& ev, double* lds) {
+  const int nx4 = ev.sp.nx + 4, ny4 = ev.sp.ny + 4;
+  for (int i = threadIdx.x; i < nx4; i += blockDim.x) lds[i] = ev.sp.tx[i];
+  for (int i = threadIdx.x; i < ny4; i += blockDim.x) lds[nx4 + i] = ev.sp.ty[i];
+}
+
+// y-part of the bicubic interpolant (fixed per row): interval and basis.
+struct SplineY {
+  int ly;
+  double by[4];
+  bool inside;
+};
+
+__device__ inline SplineY spline_y(const EvDev& ev, const double* ty, double y) {
+  SplineY r;
+  r.inside = (y >= ev.sp.ymin && y <= ev.sp.ymax);
+  r.ly = 3;
+  r.by[0] = r.by[1] = r.by[2] = r.by[3] = 0.0;
+  if (r.inside) {
+    r.ly = knot_interval(ty, ev.sp.ny, y);
+    bspline3(ty, r.ly, y, r.by);
+  }
+  return r;
+}
+
+// BoundedRectBivariateSpline value at (x, y) with the y-part precomputed;
+// -inf outside the bounding box (calculus.py:252-260).
+__device__ inline double spline_x(const EvDev& ev, const double* tx, const SplineY& yy, double x) {
+  if (!yy.inside || !(x >= ev.sp.xmin && x <= ev.sp.xmax)) return -INFINITY;
+  const int lx = knot_interval(tx, ev.sp.nx, x);
+  double bx[4];
+  bspline3(tx, lx, x, bx);
+  const double* c = ev.sp.c + (size_t)(lx - 3) * ev.sp.ny + (yy.ly - 3);
+  double s = 0.0;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const double* ci = c + (size_t)i * ev.sp.ny;
+    s += bx[i] * (yy.by[0] * ci[0] + yy.by[1] * ci[1] + yy.by[2] * ci[2] + yy.by[3] * ci[3]);
+  }
+  return s;
+}
+
+// lnL from the summed inner products (base.py:448-477, 775-792); one thread
+// per row, knots in LDS.
+__global__ __launch_bounds__(128) void k_epi_plain(EpiArgs a) {
+  extern __shared__ double knots[];
+  const EvDev& ev = a.ev;
+  const bool dist = (ev.flags & GWL_DISTANCE_MARG) != 0;
+  if (dist) stage_knots(ev, knots);
+  __syncthreads();
   const long long r = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= a.n) return;
-  const EvDev& ev = a.ev;
   SampleConst sc;
   compute_sample_const(ev, a.params + r * a.ld, sc);
   double dr = 0.0, di = 0.0, hh = 0.0;
@@ -457,10 +505,11 @@ __global__ void k_epi_plain(EpiArgs a) {
     return;
   }
   double ll;
-  if (ev.flags & GWL_DISTANCE_MARG) {
+  if (dist) {
     const double hs = sc.hscale;
-    double d = (ev.flags & GWL_PHASE_MARG) ? hypot(dhr, dhi) * hs : dhr * hs;
-    ll = spline2d(ev.sp, d, hh * hs * hs);
+    const double d = (ev.flags & GWL_PHASE_MARG) ? hypot(dhr, dhi) * hs : dhr * hs;
+    const SplineY yy = spline_y(ev, knots + ev.sp.nx + 4, hh * hs * hs);
+    ll = spline_x(ev, knots, yy, d);
   } else if (ev.flags & GWL_PHASE_MARG) {
     ll = ln_i0(hypot(dhr, dhi)) - hh / 2.0;
   } else {
@@ -470,52 +519,59 @@ __global__ void k_epi_plain(EpiArgs a) {
 }
 
 // Time-marginalised lnL: logsumexp over the prior window (base.py:794-820).
-template <int NT>
-__global__ __launch_bounds__(NT) void k_epi_time(EpiArgs a) {
-  const long long r = blockIdx.x;
-  if (r >= a.n) return;
+// One wavefront per row, kEpiRows rows per workgroup, knots in LDS.
+constexpr int kEpiRows = 4;
+
+__global__ __launch_bounds__(64 * kEpiRows) void k_epi_time(EpiArgs a) {
+  extern __shared__ double knots[];
   const EvDev& ev = a.ev;
-  __shared__ double s_hh, s_jit, s_hs;
-  __shared__ int s_valid;
-  __shared__ double redv[NT / 64];
-  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
-  (void)lane;
-  if (tid == 0) {
+  const bool dist = (ev.flags & GWL_DISTANCE_MARG) != 0;
+  if (dist) stage_knots(ev, knots);
+  __syncthreads();
+  const int lane = threadIdx.x & 63;
+  const long long r = (long long)blockIdx.x * kEpiRows + (threadIdx.x >> 6);
+  if (r >= a.n) return;
+  double hh = 0.0, jit = 0.0, hs = 1.0;
+  int valid = 1;
+  if (lane == 0) {
     SampleConst sc;
     compute_sample_const(ev, a.params + r * a.ld, sc);
-    double hh = 0.0;
     for (int g = 0; g < a.G; ++g) hh += a.hhpart[(size_t)r * a.G + g];
-    s_hh = hh * (4.0 / ev.T);
-    s_jit = sc.jitter;
-    s_hs = sc.hscale;
-    s_valid = sc.valid;
+    hh *= 4.0 / ev.T;
+    jit = sc.jitter;
+    hs = sc.hscale;
+    valid = sc.valid;
   }
-  __syncthreads();
-  if (!s_valid) {
-    if (tid == 0) a.out[r] = row_valid_sentinel();
+  hh = __shfl(hh, 0, 64);
+  jit = __shfl(jit, 0, 64);
+  hs = __shfl(hs, 0, 64);
+  valid = __shfl(valid, 0, 64);
+  if (!valid) {
+    if (lane == 0) a.out[r] = row_valid_sentinel();
     return;
   }
   const double norm = 4.0 / ev.T;
-  const double hh = s_hh, hs = s_hs, jit = s_jit;
-  // single-pass (running max, scaled sum) logsumexp per thread, then merged
+  SplineY yy;
+  if (dist) yy = spline_y(ev, knots + ev.sp.nx + 4, hh * hs * hs);
+  // single-pass (running max, scaled sum) logsumexp per lane, then merged
   double m = -INFINITY, s = 0.0;
-  for (int jj = tid; jj < ev.L; jj += NT) {
+  const double2* prow = a.part + (size_t)r * a.G * ev.L;
+  for (int jj = lane; jj < ev.L; jj += 64) {
+    const int j = ev.jlo + jj;
+    const double t = (ev.start + (double)(j + 1) * ev.tstep) + jit;
+    if (!(t >= ev.tmin && t <= ev.tmax)) continue;
     double xr = 0.0, xi = 0.0;
-    const double2* p = a.part + (size_t)r * a.G * ev.L + jj;
     for (int g = 0; g < a.G; ++g) {
-      double2 v = p[(size_t)g * ev.L];
+      const double2 v = prow[(size_t)g * ev.L + jj];
       xr += v.x;
       xi += v.y;
     }
     xr *= norm;
     xi *= norm;
-    const int j = ev.jlo + jj;
-    const double t = (ev.start + (double)(j + 1) * ev.tstep) + jit;
-    if (!(t >= ev.tmin && t <= ev.tmax)) continue;
     double v;
-    if (ev.flags & GWL_DISTANCE_MARG) {
-      double d = (ev.flags & GWL_PHASE_MARG) ? hypot(xr, xi) * hs : xr * hs;
-      v = spline2d(ev.sp, d, hh * hs * hs);
+    if (dist) {
+      const double d = (ev.flags & GWL_PHASE_MARG) ? hypot(xr, xi) * hs : xr * hs;
+      v = spline_x(ev, knots, yy, d);
     } else if (ev.flags & GWL_PHASE_MARG) {
       v = ln_i0(hypot(xr, xi)) - hh / 2.0;
     } else {
@@ -528,11 +584,10 @@ __global__ __launch_bounds__(NT) void k_epi_time(EpiArgs a) {
       s += exp(v - m);
     }
   }
-  // merge (m, s) across the block
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) {
-    double mo = __shfl_xor(m, off, 64), so = __shfl_xor(s, off, 64);
-    double mn = fmax(m, mo);
+    const double mo = __shfl_xor(m, off, 64), so = __shfl_xor(s, off, 64);
+    const double mn = fmax(m, mo);
     if (mn > -INFINITY) {
       s = (m > -INFINITY ? s * exp(m - mn) : 0.0) + (mo > -INFINITY ? so * exp(mo - mn) : 0.0);
     } else {
@@ -540,21 +595,9 @@ __global__ __launch_bounds__(NT) void k_epi_time(EpiArgs a) {
     }
     m = mn;
   }
-  __shared__ double redm[NT / 64];
   if (lane == 0) {
-    redm[wid] = m;
-    redv[wid] = s;
-  }
-  __syncthreads();
-  if (tid == 0) {
-    double mm = -INFINITY;
-    for (int w = 0; w < NT / 64; ++w) mm = fmax(mm, redm[w]);
-    double tot = 0.0;
-    if (mm > -INFINITY)
-      for (int w = 0; w < NT / 64; ++w)
-        if (redm[w] > -INFINITY) tot += redv[w] * exp(redm[w] - mm);
     const double b = (1.0 / (ev.tmax - ev.tmin)) * ev.tdelta;
-    a.out[r] = (mm > -INFINITY) ? log(b * tot) + mm : -INFINITY;
+    a.out[r] = (m > -INFINITY) ? log(b * s) + m : -INFINITY;
   }
 }
 
@@ -603,21 +646,26 @@ __global__ void k_response(RespArgs a) {
 // ------------------------------------------------------------------ dispatch
 static int pick_nd(int nd) { return nd; }
 
-// rows per workgroup of the time-marginalised kernel (LDS: S P complex)
-static int time_tile_rows(int P, long long n) {
+// rows per workgroup of the time-marginalised kernel (LDS: S P complex);
+// instantiated: P = 512: 1, 2, 4, 8; P = 1024: 1, 2, 4; P = 2048: 1, 2.
+static int time_tile_rows(int P, long long n, int want) {
+  const int maxS = P == 512 ? 8 : P == 1024 ? 4 : P == 2048 ? 2 : 1;
+  if (want > 0) {
+    int S = 1;
+    while (S * 2 <= want && S * 2 <= maxS) S *= 2;
+    return S;
+  }
   if (n < 64) return 1;
-  if (P == 512) return 8;
-  if (P == 1024) return 4;
-  if (P == 2048) return 2;
-  return 1;
+  return maxS;
 }
 
 // groups of sub-sequences per row: enough workgroups to fill the CUs
-static int time_tile_groups(const EvDev& ev, long long n, int S, int n_cu) {
+static int time_tile_groups(const EvDev& ev, long long n, int S, int n_cu, int wg_per_cu) {
   const int Q = ev.N / ev.P;
   const long long nblk = (n + S - 1) / S;
+  const long long target = (long long)n_cu * (wg_per_cu > 0 ? wg_per_cu : 8);
   int G = 1;
-  while (G < Q && G < 64 && nblk * G < (long long)n_cu * 8) G *= 2;
+  while (G < Q && G < 64 && nblk * G < target) G *= 2;
   return G;
 }
 
@@ -634,9 +682,9 @@ static int eval_typed(gwl_event* e, const double* params, int64_t n, int ld, dou
     ta.params = params;
     ta.ld = ld;
     ta.n = n;
-    const int S = time_tile_rows(P, n);
+    const int S = time_tile_rows(P, n, e->tile_rows);
     const long long nblk = (n + S - 1) / S;
-    const int G = time_tile_groups(ev, n, S, e->ctx->n_cu);
+    const int G = time_tile_groups(ev, n, S, e->ctx->n_cu, e->tile_wg_per_cu);
     ta.G = G;
     ta.part = reinterpret_cast<double2*>(e->ws);
     ta.hhpart = reinterpret_cast<double*>(reinterpret_cast<char*>(e->ws) +
@@ -651,9 +699,13 @@ static int eval_typed(gwl_event* e, const double* params, int64_t n, int ld, dou
     }                                                                                            \
     hipLaunchKernelGGL((k_time<R, WAVE, ND, SS, PP, NT>), grid, dim3(NT), lds, st, ta);          \
   } while (0)
+    if (int rc = timing_begin(e, st)) return rc;
     if (P == 512 && S == 8) GWL_LAUNCH_TIME(8, 512);
+    else if (P == 512 && S == 4) GWL_LAUNCH_TIME(4, 512);
+    else if (P == 512 && S == 2) GWL_LAUNCH_TIME(2, 512);
     else if (P == 512) GWL_LAUNCH_TIME(1, 512);
     else if (P == 1024 && S == 4) GWL_LAUNCH_TIME(4, 1024);
+    else if (P == 1024 && S == 2) GWL_LAUNCH_TIME(2, 1024);
     else if (P == 1024) GWL_LAUNCH_TIME(1, 1024);
     else if (P == 2048 && S == 2) GWL_LAUNCH_TIME(2, 2048);
     else if (P == 2048) GWL_LAUNCH_TIME(1, 2048);
@@ -666,6 +718,7 @@ static int eval_typed(gwl_event* e, const double* params, int64_t n, int ld, dou
     }
 #undef GWL_LAUNCH_TIME
     GWL_HIP_CHECK(hipGetLastError());
+    if (int rc = timing_end(e, st)) return rc;
     EpiArgs ea{};
     ea.ev = ev;
     ea.params = params;
@@ -676,12 +729,14 @@ static int eval_typed(gwl_event* e, const double* params, int64_t n, int ld, dou
     ea.hhpart = ta.hhpart;
     ea.out = out;
     ea.aux = aux;
-    hipLaunchKernelGGL(k_epi_time<256>, dim3((unsigned)n), dim3(256), 0, st, ea);
+    const size_t klds = (ev.flags & GWL_DISTANCE_MARG) ? (size_t)(ev.sp.nx + ev.sp.ny + 8) * sizeof(double) : 0;
+    hipLaunchKernelGGL(k_epi_time, dim3((unsigned)((n + kEpiRows - 1) / kEpiRows)),
+                       dim3(64 * kEpiRows), klds, st, ea);
     GWL_HIP_CHECK(hipGetLastError());
     return GWL_OK;
   }
   constexpr int NT = 256;
-  constexpr int S = 8;
+  const int S = e->tile_rows >= 16 ? 16 : e->tile_rows >= 8 || e->tile_rows <= 0 ? 8 : 4;
   RedArgs ra;
   ra.ev = ev;
   ra.params = params;
@@ -693,9 +748,18 @@ static int eval_typed(gwl_event* e, const double* params, int64_t n, int ld, dou
   ra.nchunk = (ev.N + ra.chunk - 1) / ra.chunk;
   ra.part = reinterpret_cast<double*>(e->ws);
   const long long nblk = (n + S - 1) / S;
-  hipLaunchKernelGGL((k_reduce<R, WAVE, ND, S, NT>), dim3(ra.nchunk, (unsigned)nblk), dim3(NT), 0,
-                     st, ra);
+  if (int rc = timing_begin(e, st)) return rc;
+  if (S == 16)
+    hipLaunchKernelGGL((k_reduce<R, WAVE, ND, 16, NT>), dim3(ra.nchunk, (unsigned)nblk), dim3(NT), 0,
+                       st, ra);
+  else if (S == 4)
+    hipLaunchKernelGGL((k_reduce<R, WAVE, ND, 4, NT>), dim3(ra.nchunk, (unsigned)nblk), dim3(NT), 0,
+                       st, ra);
+  else
+    hipLaunchKernelGGL((k_reduce<R, WAVE, ND, 8, NT>), dim3(ra.nchunk, (unsigned)nblk), dim3(NT), 0,
+                       st, ra);
   GWL_HIP_CHECK(hipGetLastError());
+  if (int rc = timing_end(e, st)) return rc;
   EpiArgs ea{};
   ea.ev = ev;
   ea.params = params;
@@ -705,7 +769,8 @@ static int eval_typed(gwl_event* e, const double* params, int64_t n, int ld, dou
   ea.part3 = ra.part;
   ea.out = out;
   ea.aux = aux;
-  hipLaunchKernelGGL(k_epi_plain, dim3((unsigned)((n + 127) / 128)), dim3(128), 0, st, ea);
+  const size_t klds = (ev.flags & GWL_DISTANCE_MARG) ? (size_t)(ev.sp.nx + ev.sp.ny + 8) * sizeof(double) : 0;
+  hipLaunchKernelGGL(k_epi_plain, dim3((unsigned)((n + 127) / 128)), dim3(128), klds, st, ea);
   GWL_HIP_CHECK(hipGetLastError());
   return GWL_OK;
 }
@@ -728,8 +793,11 @@ size_t workspace_bytes(const gwl_event* e, int64_t rows) {
   if (ev.flags & GWL_TIME_MARG) {
     // partials [n][G][L] double2 + [n][G] double; n G <= max(n, 2 S 8 n_cu)
     // for every n <= rows (G halves as the row blocks double)
-    const long long bound = std::max<long long>((long long)rows * time_tile_groups(ev, rows, time_tile_rows(ev.P, rows), e->ctx->n_cu),
-                                                2LL * 8 * 8 * e->ctx->n_cu);
+    const int S = time_tile_rows(ev.P, rows, e->tile_rows);
+    const int wpc = e->tile_wg_per_cu > 0 ? e->tile_wg_per_cu : 8;
+    const long long bound = std::max<long long>(
+        (long long)rows * time_tile_groups(ev, rows, S, e->ctx->n_cu, e->tile_wg_per_cu),
+        2LL * 8 * wpc * e->ctx->n_cu);
     return (size_t)bound * (ev.L * sizeof(double2) + sizeof(double)) + 4096;
   }
   const int nchunk = (ev.N + 1023) / 1024;
@@ -746,6 +814,23 @@ int launch_eval(gwl_event* e, const double* params, int64_t n, int ld, int preci
   }
   if (wave == GWL_TAYLORF2) return eval_nd<double, GWL_TAYLORF2>(e, params, n, ld, out, aux, st);
   return eval_nd<double, GWL_SINE_GAUSSIAN>(e, params, n, ld, out, aux, st);
+}
+
+int timing_begin(gwl_event* e, hipStream_t st) {
+  if (!e->timing) return GWL_OK;
+  GWL_HIP_CHECK(hipEventRecord(e->t0, st));
+  return GWL_OK;
+}
+
+int timing_end(gwl_event* e, hipStream_t st) {
+  if (!e->timing) return GWL_OK;
+  GWL_HIP_CHECK(hipEventRecord(e->t1, st));
+  GWL_HIP_CHECK(hipEventSynchronize(e->t1));
+  float ms = 0.f;
+  GWL_HIP_CHECK(hipEventElapsedTime(&ms, e->t0, e->t1));
+  e->timed_ms += ms;
+  e->timed_launches += 1;
+  return GWL_OK;
 }
 
 int launch_response(gwl_event* e, const double* params_dev, int det, int precision, double* out_dev,

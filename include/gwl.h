@@ -129,6 +129,21 @@ int gwl_noise_log_likelihood(gwl_event* ev, double* out);
 int gwl_eval(gwl_event* ev, const double* params, int64_t n_rows, int32_t ld,
              int32_t precision, double* out, double* aux, void* stream);
 
+/* Tile selection for this event's kernels (0 = built-in heuristic):
+ * rows_per_workgroup = parameter rows sharing one workgroup's bin loop
+ * (1, 2, 4, 8 or 16), workgroups_per_cu = target resident workgroups per CU
+ * used to split the time-marginalised transform into groups.  Used by the
+ * autotuner; no counterpart in the reference. */
+int gwl_event_set_tile(gwl_event* ev, int32_t rows_per_workgroup, int32_t workgroups_per_cu);
+
+/* Kernel timing for measurement: while enabled, every gwl_eval on this event
+ * brackets its dominant kernel (the fused inner-product or time-marginalised
+ * kernel) with HIP events on the launch stream and waits for them; the
+ * accumulated milliseconds and launch count are read (and reset) with
+ * gwl_event_timing_read.  No counterpart in the reference (measurement). */
+int gwl_event_timing(gwl_event* ev, int32_t enable);
+int gwl_event_timing_read(gwl_event* ev, double* total_ms, int64_t* launches);
+
 /* Detector response of one parameter row: bilby
  * Interferometer.get_detector_response (interferometer.py:303-368) of the
  * event's source model.  params: host [GWL_NPARAM]; out: host
@@ -169,6 +184,59 @@ int gwl_fft_batch(gwl_ctx* ctx, const void* in, void* out, int64_t batch, int32_
 /* Natural log of the modified Bessel function I0 (bilby/gw/utils.py:1006-1022
  * ln_i0) evaluated on the device: in/out device float64 [n]. */
 int gwl_ln_i0(gwl_ctx* ctx, const double* in, double* out, int64_t n, void* stream);
+
+/* ------------------------------------------------------------------ ROQ */
+/* Reduced-order-quadrature likelihood, bilby/gw/likelihood/roq.py
+ * (ROQGravitationalWaveTransient: calculate_snrs roq.py:467-549, time grid
+ * roq.py:320-331, five-sample interpolation roq.py:576-651). */
+typedef struct gwl_roq gwl_roq;
+
+enum gwl_roq_mode {
+  GWL_ROQ_FP64 = 0,    /* float64 FMA GEMM (reference) */
+  GWL_ROQ_FP32 = 1,    /* float32 MFMA GEMM */
+  GWL_ROQ_BF16X3 = 2   /* split-bfloat16 x3 MFMA GEMM, float32 accumulation */
+};
+
+typedef struct {
+  int32_t n_det;
+  int32_t n_linear;              /* L linear frequency nodes */
+  int32_t n_quadratic;           /* quadratic frequency nodes */
+  int32_t n_time_samples;        /* rows of each detector's linear weights */
+  double start_time;
+  double time_samples_start;     /* weights['time_samples'][0] (s from start_time) */
+  double time_samples_step;      /* their spacing */
+  const double* linear_nodes;    /* host [n_linear] Hz */
+  const double* quadratic_nodes; /* host [n_quadratic] Hz */
+  const double* linear_weights;  /* host [n_det][n_time_samples][n_linear][2] complex */
+  const double* quadratic_weights; /* host [n_det][n_quadratic][2] complex (real part used) */
+  const double* detector_tensor; /* host [n_det][9] */
+  const double* vertex;          /* host [n_det][3] */
+  double reference_frequency;    /* TaylorF2 f_ref */
+  double wf_minimum_frequency;
+  uint32_t flags;                /* enum gwl_flags */
+  double time_prior_minimum, time_prior_maximum;
+  double delta_tc;               /* requested time-grid spacing (roq.py:320-331) */
+  double reference_distance;
+} gwl_roq_desc;
+
+/* ROQGravitationalWaveTransient.__init__ (roq.py:99-300): weights resident
+ * in HBM in the layout the GEMM reads. */
+int gwl_roq_create(gwl_ctx* ctx, const gwl_roq_desc* desc, gwl_roq** out, int64_t* hbm_bytes);
+int gwl_roq_destroy(gwl_roq* roq);
+/* Distance interpolant, as gwl_event_set_distance_spline. */
+int gwl_roq_set_distance_spline(gwl_roq* roq, int32_t nx, int32_t ny, const double* tx,
+                                const double* ty, const double* coeffs, const double* bbox);
+/* Batched ROQ log-likelihood ratio (roq.py:463-465 -> base.py:419-446):
+ * params, out device pointers; mode enum gwl_roq_mode. */
+int gwl_roq_eval(gwl_roq* roq, const double* params, int64_t n_rows, int32_t ld, int32_t mode,
+                 double* out, void* stream);
+
+/* The ROQ contraction as a plain GEMM, C[M][N] = A[M][K] Bt[N][K]^T
+ * (M, N multiples of 128, K of 32), for checking the MFMA kernels:
+ * mode GWL_ROQ_FP64 (double), GWL_ROQ_FP32 (float), GWL_ROQ_BF16X3
+ * (A = A + A_lo, Bt = Bt + Bt_lo in bfloat16, C float).  Device pointers. */
+int gwl_gemm(gwl_ctx* ctx, int32_t mode, const void* A, const void* A_lo, const void* Bt,
+             const void* Bt_lo, void* C, int32_t M, int32_t N, int32_t K, void* stream);
 
 #ifdef __cplusplus
 }
