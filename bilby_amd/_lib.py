@@ -34,6 +34,21 @@ class EventDesc(ctypes.Structure):
     ]
 
 
+class RoqDesc(ctypes.Structure):
+    _fields_ = [
+        ("n_det", ctypes.c_int32), ("n_linear", ctypes.c_int32),
+        ("n_quadratic", ctypes.c_int32), ("n_time_samples", ctypes.c_int32),
+        ("start_time", ctypes.c_double), ("time_samples_start", ctypes.c_double),
+        ("time_samples_step", ctypes.c_double),
+        ("linear_nodes", _dp), ("quadratic_nodes", _dp), ("linear_weights", _dp),
+        ("quadratic_weights", _dp), ("detector_tensor", _dp), ("vertex", _dp),
+        ("reference_frequency", ctypes.c_double), ("wf_minimum_frequency", ctypes.c_double),
+        ("flags", ctypes.c_uint32),
+        ("time_prior_minimum", ctypes.c_double), ("time_prior_maximum", ctypes.c_double),
+        ("delta_tc", ctypes.c_double), ("reference_distance", ctypes.c_double),
+    ]
+
+
 class GWLError(RuntimeError):
     pass
 
@@ -70,6 +85,23 @@ def _load():
         "gwl_noise_log_likelihood": ([vp, _dp], ctypes.c_int),
         "gwl_event_timing": ([vp, ctypes.c_int32], ctypes.c_int),
         "gwl_event_set_tile": ([vp, ctypes.c_int32, ctypes.c_int32], ctypes.c_int),
+        "gwl_roq_create": ([vp, ctypes.POINTER(RoqDesc), ctypes.POINTER(vp),
+                            ctypes.POINTER(ctypes.c_int64)], ctypes.c_int),
+        "gwl_roq_destroy": ([vp], ctypes.c_int),
+        "gwl_roq_set_distance_spline": ([vp, ctypes.c_int32, ctypes.c_int32, _dp, _dp, _dp, _dp],
+                                        ctypes.c_int),
+        "gwl_roq_eval": ([vp, vp, ctypes.c_int64, ctypes.c_int32, ctypes.c_int32, vp, vp],
+                         ctypes.c_int),
+        "gwl_multi_create": ([vp, ctypes.c_int32, ctypes.POINTER(vp), ctypes.POINTER(vp),
+                              ctypes.POINTER(vp), ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(vp)],
+                             ctypes.c_int),
+        "gwl_multi_set_rows": ([vp, ctypes.POINTER(ctypes.c_int64), vp], ctypes.c_int),
+        "gwl_multi_set_tile": ([vp, ctypes.POINTER(ctypes.c_int32)], ctypes.c_int),
+        "gwl_multi_launch": ([vp, vp], ctypes.c_int),
+        "gwl_multi_capture": ([vp, vp], ctypes.c_int),
+        "gwl_multi_destroy": ([vp], ctypes.c_int),
+        "gwl_gemm": ([vp, ctypes.c_int32, vp, vp, vp, vp, vp, ctypes.c_int32, ctypes.c_int32,
+                      ctypes.c_int32, vp], ctypes.c_int),
         "gwl_event_timing_read": ([vp, _dp, ctypes.POINTER(ctypes.c_int64)], ctypes.c_int),
     }
     for name, (argt, rest) in sig.items():
@@ -105,4 +137,7 @@ EXPORTED_SYMBOLS = [
     "gwl_detector_response", "gwl_distance_table", "gwl_spline2d_coefficients",
     "gwl_spline2d_eval_host", "gwl_fft_batch", "gwl_ln_i0", "gwl_noise_log_likelihood",
     "gwl_event_timing", "gwl_event_timing_read", "gwl_event_set_tile",
+    "gwl_roq_create", "gwl_roq_destroy", "gwl_roq_set_distance_spline", "gwl_roq_eval",
+    "gwl_gemm", "gwl_multi_create", "gwl_multi_set_rows", "gwl_multi_set_tile",
+    "gwl_multi_launch", "gwl_multi_capture", "gwl_multi_destroy",
 ]

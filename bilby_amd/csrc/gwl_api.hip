@@ -630,6 +630,199 @@ int gwl_distance_table(gwl_ctx* ctx, const double* dist, const double* prior, in
   return GWL_OK;
 }
 
+int gwl_multi_create(gwl_ctx* ctx, int32_t n_events, gwl_event* const* events,
+                     const double* const* params_dev, double* const* out_dev,
+                     const int64_t* max_rows, gwl_multi** out) {
+  if (!ctx || n_events < 1 || !events || !params_dev || !out_dev || !max_rows || !out) {
+    set_error("gwl_multi_create: invalid argument");
+    return GWL_E_INVALID;
+  }
+  GWL_HIP_CHECK(hipSetDevice(ctx->device));
+  gwl_multi* m = new (std::nothrow) gwl_multi;
+  if (!m) { set_error("out of host memory"); return GWL_E_NOMEM; }
+  m->ctx = ctx;
+  m->n_events = n_events;
+  m->wave = events[0]->dev.approx;
+  size_t items = 0, blocks = 0;
+  for (int e = 0; e < n_events; ++e) {
+    const gwl_event* ev = events[e];
+    if (!ev || !(ev->dev.flags & GWL_TIME_MARG) || ev->dev.P != 512 || ev->dev.nd > 3 ||
+        ev->dev.approx != m->wave || max_rows[e] < 0 ||
+        ((ev->dev.flags & GWL_DISTANCE_MARG) && !ev->has_spline)) {
+      delete m;
+      set_error("gwl_multi_create: every event must be time-marginalised with a 512-sample "
+                "transform tile, <= 3 detectors, one source model (and a distance spline "
+                "when distance-marginalised)");
+      return GWL_E_UNSUPPORTED;
+    }
+    m->events.push_back(const_cast<gwl_event*>(ev));
+    m->max_rows.push_back(max_rows[e]);
+    m->rows.push_back(0);
+    m->params.push_back(params_dev[e]);
+    m->outs.push_back(out_dev[e]);
+    const int Q = ev->dev.N / 512;
+    int G = (Q + 31) / 32;  // ~32 sub-sequence steps per item
+    G = std::max(1, std::min(G, std::min(Q, 16)));
+    m->groups.push_back(G);
+    items += (size_t)((max_rows[e] + 7) / 8) * std::min(Q, 16);
+    blocks += (size_t)((max_rows[e] + 3) / 4);
+  }
+  m->max_items = std::max<size_t>(items, 1);
+  m->max_blocks = std::max<size_t>(blocks, 1);
+  // per-event partials for up to 16 groups
+  for (int e = 0; e < n_events; ++e) {
+    const size_t L = (size_t)m->events[e]->dev.L;
+    void* part = nullptr;
+    double* hh = nullptr;
+    const size_t rows = (size_t)std::max<int64_t>(1, m->max_rows[e]);
+    if (hipMalloc(&part, rows * 16 * L * sizeof(double2)) != hipSuccess ||
+        hipMalloc((void**)&hh, rows * 16 * sizeof(double)) != hipSuccess) {
+      if (part) (void)hipFree(part);
+      m->parts.push_back(nullptr);
+      m->hhs.push_back(nullptr);
+      gwl_multi_destroy(m);
+      set_error("hipMalloc failed for multi-event partials");
+      return GWL_E_NOMEM;
+    }
+    m->parts.push_back(part);
+    m->hhs.push_back(hh);
+  }
+  const size_t ne = (size_t)n_events;
+  const size_t bytes = ne * sizeof(EvDev) + 5 * ne * sizeof(void*) + ne * sizeof(long long) +
+                       ne * sizeof(int) + m->max_items * sizeof(MultiItem) +
+                       m->max_blocks * (sizeof(int) + sizeof(long long)) + 64 * 8 + 16 * 256;
+  if (hipMalloc(&m->dev, bytes) != hipSuccess) {
+    gwl_multi_destroy(m);
+    set_error("hipMalloc failed for multi-event tables");
+    return GWL_E_NOMEM;
+  }
+  m->dev_bytes = bytes;
+  char* p = reinterpret_cast<char*>(m->dev);
+  auto take = [&](size_t nb) {
+    char* q = p;
+    p += (nb + 255) / 256 * 256;
+    return q;
+  };
+  m->d_evs = reinterpret_cast<EvDev*>(take(ne * sizeof(EvDev)));
+  m->d_params = reinterpret_cast<const double**>(take(ne * sizeof(void*)));
+  m->d_outs = reinterpret_cast<double**>(take(ne * sizeof(void*)));
+  m->d_parts = reinterpret_cast<void**>(take(ne * sizeof(void*)));
+  m->d_hhs = reinterpret_cast<double**>(take(ne * sizeof(void*)));
+  m->d_rows = reinterpret_cast<long long*>(take(ne * sizeof(long long)));
+  m->d_groups = reinterpret_cast<int*>(take(ne * sizeof(int)));
+  m->d_items = reinterpret_cast<MultiItem*>(take(m->max_items * sizeof(MultiItem)));
+  m->d_block_event = reinterpret_cast<int*>(take(m->max_blocks * sizeof(int)));
+  m->d_block_row0 = reinterpret_cast<long long*>(take(m->max_blocks * sizeof(long long)));
+  m->d_n_items = reinterpret_cast<int*>(take(sizeof(int)));
+  m->d_counter = reinterpret_cast<int*>(take(sizeof(int)));
+  m->d_n_blocks = reinterpret_cast<int*>(take(sizeof(int)));
+  std::vector<EvDev> evs;
+  for (auto* ev : m->events) evs.push_back(ev->dev);
+  hipError_t err = hipMemcpy(m->d_evs, evs.data(), ne * sizeof(EvDev), hipMemcpyHostToDevice);
+  if (err == hipSuccess) err = hipMemcpy(m->d_params, m->params.data(), ne * sizeof(void*), hipMemcpyHostToDevice);
+  if (err == hipSuccess) err = hipMemcpy(m->d_outs, m->outs.data(), ne * sizeof(void*), hipMemcpyHostToDevice);
+  if (err == hipSuccess) err = hipMemcpy(m->d_parts, m->parts.data(), ne * sizeof(void*), hipMemcpyHostToDevice);
+  if (err == hipSuccess) err = hipMemcpy(m->d_hhs, m->hhs.data(), ne * sizeof(void*), hipMemcpyHostToDevice);
+  if (err == hipSuccess) err = hipMemcpy(m->d_groups, m->groups.data(), ne * sizeof(int), hipMemcpyHostToDevice);
+  if (err != hipSuccess) {
+    gwl_multi_destroy(m);
+    return hip_fail(err, "gwl_multi_create upload");
+  }
+  std::vector<int64_t> zero(ne, 0);
+  int rc = multi_set_rows(m, zero.data(), nullptr);
+  if (rc != GWL_OK) {
+    gwl_multi_destroy(m);
+    return rc;
+  }
+  if (multi_lds_bytes() > ctx->lds_per_block) {
+    gwl_multi_destroy(m);
+    set_error("multi-event tile exceeds the LDS of a workgroup");
+    return GWL_E_UNSUPPORTED;
+  }
+  *out = m;
+  return GWL_OK;
+}
+
+int gwl_multi_set_tile(gwl_multi* m, const int32_t* steps_per_item) {
+  if (!m || !steps_per_item) { set_error("gwl_multi_set_tile: invalid argument"); return GWL_E_INVALID; }
+  for (int e = 0; e < m->n_events; ++e) {
+    const int Q = m->events[e]->dev.N / 512;
+    int st = steps_per_item[e];
+    int G = st > 0 ? (Q + st - 1) / st : (Q + 31) / 32;
+    m->groups[e] = std::max(1, std::min(G, std::min(Q, 16)));
+  }
+  GWL_HIP_CHECK(hipSetDevice(m->ctx->device));
+  GWL_HIP_CHECK(hipMemcpy(m->d_groups, m->groups.data(), m->groups.size() * sizeof(int),
+                          hipMemcpyHostToDevice));
+  std::vector<int64_t> rows(m->rows.begin(), m->rows.end());
+  return multi_set_rows(m, rows.data(), nullptr);
+}
+
+int gwl_multi_set_rows(gwl_multi* m, const int64_t* rows, void* stream) {
+  if (!m || !rows) { set_error("gwl_multi_set_rows: invalid argument"); return GWL_E_INVALID; }
+  GWL_HIP_CHECK(hipSetDevice(m->ctx->device));
+  return multi_set_rows(m, rows, reinterpret_cast<hipStream_t>(stream));
+}
+
+int gwl_multi_launch(gwl_multi* m, void* stream) {
+  if (!m) { set_error("gwl_multi_launch: NULL"); return GWL_E_INVALID; }
+  GWL_HIP_CHECK(hipSetDevice(m->ctx->device));
+  (void)hipGetLastError();
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  if (m->captured) {
+    GWL_HIP_CHECK(hipGraphLaunch(m->exec, st));
+    return GWL_OK;
+  }
+  return multi_launch_raw(m, st);
+}
+
+int gwl_multi_capture(gwl_multi* m, void* stream) {
+  if (!m || !stream) {
+    set_error("gwl_multi_capture: needs a non-default stream");
+    return GWL_E_INVALID;
+  }
+  GWL_HIP_CHECK(hipSetDevice(m->ctx->device));
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  if (m->captured) {
+    (void)hipGraphExecDestroy(m->exec);
+    (void)hipGraphDestroy(m->graph);
+    m->captured = false;
+  }
+  GWL_HIP_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+  int rc = multi_launch_raw(m, st);
+  hipGraph_t g = nullptr;
+  hipError_t err = hipStreamEndCapture(st, &g);
+  if (rc != GWL_OK) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc;
+  }
+  if (err != hipSuccess) return hip_fail(err, "hipStreamEndCapture");
+  err = hipGraphInstantiate(&m->exec, g, nullptr, nullptr, 0);
+  if (err != hipSuccess) {
+    (void)hipGraphDestroy(g);
+    return hip_fail(err, "hipGraphInstantiate");
+  }
+  m->graph = g;
+  m->captured = true;
+  return GWL_OK;
+}
+
+int gwl_multi_destroy(gwl_multi* m) {
+  if (!m) return GWL_OK;
+  (void)hipSetDevice(m->ctx->device);
+  if (m->captured) {
+    (void)hipGraphExecDestroy(m->exec);
+    (void)hipGraphDestroy(m->graph);
+  }
+  for (void* p : m->parts)
+    if (p) (void)hipFree(p);
+  for (double* p : m->hhs)
+    if (p) (void)hipFree(p);
+  if (m->dev) (void)hipFree(m->dev);
+  delete m;
+  return GWL_OK;
+}
+
 int gwl_spline2d_coefficients(int32_t nx, int32_t ny, const double* x, const double* y,
                               const double* z, double* tx, double* ty, double* c) {
   if (nx < 4 || ny < 4 || !x || !y || !z || !tx || !ty || !c) {

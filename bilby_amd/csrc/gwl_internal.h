@@ -39,7 +39,57 @@ struct gwl_event {
   int64_t timed_launches = 0;
 };
 
+#include <vector>
+
 namespace gwl {
+// one item of the multi-event work list: rows [row0, row0 + 8) of event e,
+// group g of its decimated sub-sequences
+struct MultiItem {
+  int e;
+  int g;
+  long long row0;
+};
+}
+
+// Many events evaluated together: one persistent launch over a work list,
+// optionally replayed from a captured hipGraph.
+struct gwl_multi {
+  gwl_ctx* ctx = nullptr;
+  int n_events = 0;
+  int wave = 0;
+  std::vector<gwl_event*> events;
+  std::vector<int64_t> max_rows, rows;
+  std::vector<int> groups;              // G_e (groups of sub-sequences per row)
+  std::vector<const double*> params;    // caller-owned device buffers
+  std::vector<double*> outs;
+  std::vector<void*> parts;             // [max_rows][16][L] double2 per event
+  std::vector<double*> hhs;             // [max_rows][16] per event
+  void* dev = nullptr;                  // device tables (events, pointers, items, blocks)
+  size_t dev_bytes = 0;
+  gwl::EvDev* d_evs = nullptr;
+  const double** d_params = nullptr;
+  double** d_outs = nullptr;
+  long long* d_rows = nullptr;
+  int* d_groups = nullptr;
+  void** d_parts = nullptr;
+  double** d_hhs = nullptr;
+  gwl::MultiItem* d_items = nullptr;
+  int* d_n_items = nullptr;
+  int* d_counter = nullptr;
+  int* d_block_event = nullptr;
+  long long* d_block_row0 = nullptr;
+  int* d_n_blocks = nullptr;
+  size_t max_items = 0, max_blocks = 0;
+  int n_items = 0;
+  bool captured = false;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
+namespace gwl {
+int multi_launch_raw(gwl_multi* m, hipStream_t stream);
+int multi_set_rows(gwl_multi* m, const int64_t* rows, hipStream_t stream);
+size_t multi_lds_bytes();
 void set_error(const std::string& msg);
 int hip_fail(hipError_t e, const char* what);
 size_t workspace_bytes(const gwl_event* ev, int64_t rows);

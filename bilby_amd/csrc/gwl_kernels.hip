@@ -103,7 +103,7 @@ __device__ inline void tile_prologue(const EvDev& ev, const double* params, int 
 template <typename R, int WAVE, int ND, int S>
 __device__ __forceinline__ void bin_rows(int n, double f, const cplx<R>* dw, const R* w,
                                          const SampleConst* sc, const SampleFast<R>* sf,
-                                         cplx<R>* I, R* hh) {
+                                         cplx<R>* I, R* hh, int nd = ND) {
   using C = cplx<R>;
   if constexpr (WAVE == GWL_TAYLORF2) {
     double b[kNB];
@@ -120,6 +120,7 @@ __device__ __forceinline__ void bin_rows(int n, double f, const cplx<R>* dw, con
       R ar = 0, ai = 0, hw = 0;
 #pragma unroll
       for (int d = 0; d < ND; ++d) {
+        if (d >= nd) break;
         const double cyc = fma(f, sc[s].dt[d], psi);
         const C e = expm2pi((R)(cyc - floor(cyc)));
         const R cr = sf[s].cre[d] * amp, ci = sf[s].cim[d] * amp;
@@ -145,6 +146,7 @@ __device__ __forceinline__ void bin_rows(int n, double f, const cplx<R>* dw, con
       R ar = 0, ai = 0, hw = 0;
 #pragma unroll
       for (int d = 0; d < ND; ++d) {
+        if (d >= nd) break;
         const double cyc = f * sc[s].dt[d];
         const C e = expm2pi((R)(cyc - floor(cyc)));
         const R pr = sf[s].cre[d] * sp, pim = sf[s].cim[d] * sm;
@@ -277,28 +279,24 @@ struct TimeArgs {
   double* hhpart;   // [n][G]
 };
 
+// One tile of the time-marginalised transform: rows [row0, row0 + S) of an
+// event, decimated sub-sequences n2 in [n2b, n2e).  Writes the window
+// partials part[r][g][:] and hhpart[r][g].  LDS: buf [S][P], tw (twiddles,
+// initialised by the caller), sc/sf/band/red scratch.
 template <typename R, int WAVE, int ND, int S, int P, int NT>
-__global__ __launch_bounds__(NT) void k_time(TimeArgs a) {
+__device__ __forceinline__ void time_tile(const EvDev& ev, int nd, const double* params, int ld,
+                                          long long n, long long row0, int n2b, int n2e,
+                                          bool nyquist, int G, int g, double2* part,
+                                          double* hhpart, cplx<R>* buf, const cplx<R>* tw,
+                                          SampleConst* sc, SampleFast<R>* sf, int* band,
+                                          double (*red)[S]) {
   using C = cplx<R>;
   constexpr int MAXBF8 = (S * P / 8 + NT - 1) / NT;
   constexpr int NPT = (P + NT - 1) / NT;   // n1 values per thread per step
   constexpr int JT = (P + NT - 1) / NT;    // window slots per thread
-  const EvDev& ev = a.ev;
-  extern __shared__ __align__(16) unsigned char smem[];
-  C* buf = reinterpret_cast<C*>(smem);       // [S][P]
-  C* tw = buf + S * P;                       // twiddles (TwLayout<P>)
-  __shared__ SampleConst sc[S];
-  __shared__ SampleFast<R> sf[S];
-  __shared__ int band[2];
-  __shared__ double red[NT / 64][S];
   const int tid = threadIdx.x;
-  const int g = blockIdx.x;
-  const long long row0 = (long long)blockIdx.y * S;
-  init_twiddles<P, NT>(tw);
-  tile_prologue<R, S>(ev, a.params, a.ld, a.n, row0, sc, sf, band);
-
+  tile_prologue<R, S>(ev, params, ld, n, row0, sc, sf, band);
   const int Q = ev.N / P;
-  const int n2b = (int)(((long long)g * Q) / a.G), n2e = (int)(((long long)(g + 1) * Q) / a.G);
   double acc[S][JT][2];
   double hacc[S];
 #pragma unroll
@@ -316,21 +314,26 @@ __global__ __launch_bounds__(NT) void k_time(TimeArgs a) {
     for (int q = 0; q < NPT; ++q) {
       const int n1 = tid + q * NT;
       if (n1 < P) {
-        const int n = Q * n1 + n2;
+        const int nb = Q * n1 + n2;
         C I[S];
         R h[S];
 #pragma unroll
         for (int s = 0; s < S; ++s) h[s] = 0;
-        if (n >= band[0] && n <= band[1]) {
+        if (nb >= band[0] && nb <= band[1]) {
           const int i = n2 * P + n1;
           C dw[ND];
           R w[ND];
 #pragma unroll
           for (int d = 0; d < ND; ++d) {
-            dw[d] = load_dw<R>(ev, d, i);
-            w[d] = load_w<R>(ev, d, i);
+            if (d < nd) {
+              dw[d] = load_dw<R>(ev, d, i);
+              w[d] = load_w<R>(ev, d, i);
+            } else {
+              dw[d] = cmk<C>((R)0, (R)0);
+              w[d] = (R)0;
+            }
           }
-          bin_rows<R, WAVE, ND, S>(n, (double)n * ev.df, dw, w, sc, sf, I, h);
+          bin_rows<R, WAVE, ND, S>(nb, (double)nb * ev.df, dw, w, sc, sf, I, h, nd);
         } else {
 #pragma unroll
           for (int s = 0; s < S; ++s) I[s] = cmk<C>((R)0, (R)0);
@@ -366,7 +369,7 @@ __global__ __launch_bounds__(NT) void k_time(TimeArgs a) {
     __syncthreads();
   }
   // Nyquist bin: enters <h|h> only (the FFT drops it, base.py:327-329)
-  if (g == 0 && tid == 0 && ev.N >= band[0] && ev.N <= band[1]) {
+  if (nyquist && tid == 0 && ev.N >= band[0] && ev.N <= band[1]) {
     C dw[ND];
     R w[ND];
 #pragma unroll
@@ -378,7 +381,7 @@ __global__ __launch_bounds__(NT) void k_time(TimeArgs a) {
     R h[S];
 #pragma unroll
     for (int s = 0; s < S; ++s) h[s] = 0;
-    bin_rows<R, WAVE, ND, S>(ev.N, (double)ev.N * ev.df, dw, w, sc, sf, I, h);
+    bin_rows<R, WAVE, ND, S>(ev.N, (double)ev.N * ev.df, dw, w, sc, sf, I, h, nd);
 #pragma unroll
     for (int s = 0; s < S; ++s) hacc[s] += (double)h[s];
   }
@@ -386,8 +389,8 @@ __global__ __launch_bounds__(NT) void k_time(TimeArgs a) {
 #pragma unroll
   for (int s = 0; s < S; ++s) {
     const long long r = row0 + s;
-    if (r < a.n) {
-      double2* dst = a.part + ((size_t)r * a.G + g) * ev.L;
+    if (r < n) {
+      double2* dst = part + ((size_t)r * G + g) * ev.L;
 #pragma unroll
       for (int jt = 0; jt < JT; ++jt) {
         const int jj = tid + jt * NT;
@@ -404,12 +407,31 @@ __global__ __launch_bounds__(NT) void k_time(TimeArgs a) {
   __syncthreads();
   if (tid < S) {
     const long long r = row0 + tid;
-    if (r < a.n) {
+    if (r < n) {
       double v = 0.0;
       for (int w = 0; w < NT / 64; ++w) v += red[w][tid];
-      a.hhpart[(size_t)r * a.G + g] = v;
+      hhpart[(size_t)r * G + g] = v;
     }
   }
+}
+
+template <typename R, int WAVE, int ND, int S, int P, int NT>
+__global__ __launch_bounds__(NT) void k_time(TimeArgs a) {
+  using C = cplx<R>;
+  extern __shared__ __align__(16) unsigned char smem[];
+  C* buf = reinterpret_cast<C*>(smem);       // [S][P]
+  C* tw = buf + S * P;                       // twiddles (TwLayout<P>)
+  __shared__ SampleConst sc[S];
+  __shared__ SampleFast<R> sf[S];
+  __shared__ int band[2];
+  __shared__ double red[NT / 64][S];
+  const int g = blockIdx.x;
+  init_twiddles<P, NT>(tw);
+  const int Q = a.ev.N / P;
+  const int n2b = (int)(((long long)g * Q) / a.G), n2e = (int)(((long long)(g + 1) * Q) / a.G);
+  time_tile<R, WAVE, ND, S, P, NT>(a.ev, ND, a.params, a.ld, a.n, (long long)blockIdx.y * S, n2b,
+                                   n2e, g == 0, a.G, g, a.part, a.hhpart, buf, tw, sc, sf, band,
+                                   red);
 }
 
 // ------------------------------------------------------------------ epilogues
@@ -522,21 +544,20 @@ __global__ __launch_bounds__(128) void k_epi_plain(EpiArgs a) {
 // One wavefront per row, kEpiRows rows per workgroup, knots in LDS.
 constexpr int kEpiRows = 4;
 
-__global__ __launch_bounds__(64 * kEpiRows) void k_epi_time(EpiArgs a) {
-  extern __shared__ double knots[];
-  const EvDev& ev = a.ev;
-  const bool dist = (ev.flags & GWL_DISTANCE_MARG) != 0;
-  if (dist) stage_knots(ev, knots);
-  __syncthreads();
+// One wavefront finishes one row: sums the group partials, evaluates the
+// marginalised lnL at every window bin and reduces them with a running-max
+// logsumexp.  ``knots`` holds the spline knots in LDS (distance only).
+__device__ inline void epi_time_row(const EvDev& ev, const double* params, int ld, long long r,
+                                    int G, const double2* part, const double* hhpart, double* out,
+                                    const double* knots) {
   const int lane = threadIdx.x & 63;
-  const long long r = (long long)blockIdx.x * kEpiRows + (threadIdx.x >> 6);
-  if (r >= a.n) return;
+  const bool dist = (ev.flags & GWL_DISTANCE_MARG) != 0;
   double hh = 0.0, jit = 0.0, hs = 1.0;
   int valid = 1;
   if (lane == 0) {
     SampleConst sc;
-    compute_sample_const(ev, a.params + r * a.ld, sc);
-    for (int g = 0; g < a.G; ++g) hh += a.hhpart[(size_t)r * a.G + g];
+    compute_sample_const(ev, params + r * ld, sc);
+    for (int g = 0; g < G; ++g) hh += hhpart[(size_t)r * G + g];
     hh *= 4.0 / ev.T;
     jit = sc.jitter;
     hs = sc.hscale;
@@ -547,7 +568,7 @@ __global__ __launch_bounds__(64 * kEpiRows) void k_epi_time(EpiArgs a) {
   hs = __shfl(hs, 0, 64);
   valid = __shfl(valid, 0, 64);
   if (!valid) {
-    if (lane == 0) a.out[r] = row_valid_sentinel();
+    if (lane == 0) out[r] = row_valid_sentinel();
     return;
   }
   const double norm = 4.0 / ev.T;
@@ -555,13 +576,13 @@ __global__ __launch_bounds__(64 * kEpiRows) void k_epi_time(EpiArgs a) {
   if (dist) yy = spline_y(ev, knots + ev.sp.nx + 4, hh * hs * hs);
   // single-pass (running max, scaled sum) logsumexp per lane, then merged
   double m = -INFINITY, s = 0.0;
-  const double2* prow = a.part + (size_t)r * a.G * ev.L;
+  const double2* prow = part + (size_t)r * G * ev.L;
   for (int jj = lane; jj < ev.L; jj += 64) {
     const int j = ev.jlo + jj;
     const double t = (ev.start + (double)(j + 1) * ev.tstep) + jit;
     if (!(t >= ev.tmin && t <= ev.tmax)) continue;
     double xr = 0.0, xi = 0.0;
-    for (int g = 0; g < a.G; ++g) {
+    for (int g = 0; g < G; ++g) {
       const double2 v = prow[(size_t)g * ev.L + jj];
       xr += v.x;
       xi += v.y;
@@ -597,9 +618,105 @@ __global__ __launch_bounds__(64 * kEpiRows) void k_epi_time(EpiArgs a) {
   }
   if (lane == 0) {
     const double b = (1.0 / (ev.tmax - ev.tmin)) * ev.tdelta;
-    a.out[r] = (m > -INFINITY) ? log(b * s) + m : -INFINITY;
+    out[r] = (m > -INFINITY) ? log(b * s) + m : -INFINITY;
   }
 }
+
+__global__ __launch_bounds__(64 * kEpiRows) void k_epi_time(EpiArgs a) {
+  extern __shared__ double knots[];
+  if (a.ev.flags & GWL_DISTANCE_MARG) stage_knots(a.ev, knots);
+  __syncthreads();
+  const long long r = (long long)blockIdx.x * kEpiRows + (threadIdx.x >> 6);
+  if (r >= a.n) return;
+  epi_time_row(a.ev, a.params, a.ld, r, a.G, a.part, a.hhpart, a.out, knots);
+}
+
+// ------------------------------------------------------------------ multi-event
+// Uneven batches over many events in one persistent launch: a work list of
+// (event, row tile, group) items in longest-first order is drained through
+// an atomic counter; every workgroup exits once the counter passes the list.
+struct MultiArgs {
+  const EvDev* evs;
+  const double* const* params;
+  const long long* rows;
+  const int* groups;
+  double2* const* part;
+  double* const* hhpart;
+  const MultiItem* items;
+  const int* n_items;
+  int* counter;
+  int ld;
+};
+
+constexpr int kMultiS = 8, kMultiP = 512, kMultiNT = 512;
+
+template <int WAVE>
+__global__ __launch_bounds__(kMultiNT) void k_time_multi(MultiArgs a) {
+  using C = cplx<float>;
+  extern __shared__ __align__(16) unsigned char smem[];
+  C* buf = reinterpret_cast<C*>(smem);
+  C* tw = buf + kMultiS * kMultiP;
+  __shared__ SampleConst sc[kMultiS];
+  __shared__ SampleFast<float> sf[kMultiS];
+  __shared__ int band[2];
+  __shared__ double red[kMultiNT / 64][kMultiS];
+  __shared__ EvDev s_ev;
+  __shared__ int s_item;
+  init_twiddles<kMultiP, kMultiNT>(tw);
+  const int n_items = *a.n_items;
+  for (;;) {
+    __syncthreads();
+    if (threadIdx.x == 0) s_item = atomicAdd(a.counter, 1);
+    __syncthreads();
+    const int it = s_item;
+    if (it >= n_items) break;
+    const MultiItem m = a.items[it];
+    constexpr int nw = (int)(sizeof(EvDev) / sizeof(unsigned));
+    const unsigned* src = reinterpret_cast<const unsigned*>(a.evs + m.e);
+    unsigned* dst = reinterpret_cast<unsigned*>(&s_ev);
+    for (int i = threadIdx.x; i < nw; i += kMultiNT) dst[i] = src[i];
+    __syncthreads();
+    const int G = a.groups[m.e];
+    const int Q = s_ev.N / kMultiP;
+    const int n2b = (int)(((long long)m.g * Q) / G), n2e = (int)(((long long)(m.g + 1) * Q) / G);
+    time_tile<float, WAVE, 3, kMultiS, kMultiP, kMultiNT>(
+        s_ev, s_ev.nd, a.params[m.e], a.ld, a.rows[m.e], m.row0, n2b, n2e, m.g == 0, G, m.g,
+        a.part[m.e], a.hhpart[m.e], buf, tw, sc, sf, band, red);
+  }
+}
+
+struct MultiEpiArgs {
+  const EvDev* evs;
+  const double* const* params;
+  const long long* rows;
+  const int* groups;
+  double2* const* part;
+  double* const* hhpart;
+  double* const* out;
+  const int* block_event;       // [blocks] event of each epilogue block
+  const long long* block_row0;  // [blocks] first row of each block
+  const int* n_blocks;
+  int ld;
+};
+
+__global__ __launch_bounds__(64 * kEpiRows) void k_epi_time_multi(MultiEpiArgs a) {
+  extern __shared__ double knots[];
+  __shared__ EvDev s_ev;
+  if ((int)blockIdx.x >= *a.n_blocks) return;
+  const int e = a.block_event[blockIdx.x];
+  constexpr int nw = (int)(sizeof(EvDev) / sizeof(unsigned));
+  const unsigned* src = reinterpret_cast<const unsigned*>(a.evs + e);
+  unsigned* dst = reinterpret_cast<unsigned*>(&s_ev);
+  for (int i = threadIdx.x; i < nw; i += blockDim.x) dst[i] = src[i];
+  __syncthreads();
+  if (s_ev.flags & GWL_DISTANCE_MARG) stage_knots(s_ev, knots);
+  __syncthreads();
+  const long long r = a.block_row0[blockIdx.x] + (threadIdx.x >> 6);
+  if (r >= a.rows[e]) return;
+  epi_time_row(s_ev, a.params[e], a.ld, r, a.groups[e], a.part[e], a.hhpart[e], a.out[e], knots);
+}
+
+__global__ void k_zero_int(int* p) { *p = 0; }
 
 // ------------------------------------------------------------------ response
 // get_detector_response (interferometer.py:303-368) of one row, all bins.
@@ -814,6 +931,110 @@ int launch_eval(gwl_event* e, const double* params, int64_t n, int ld, int preci
   }
   if (wave == GWL_TAYLORF2) return eval_nd<double, GWL_TAYLORF2>(e, params, n, ld, out, aux, st);
   return eval_nd<double, GWL_SINE_GAUSSIAN>(e, params, n, ld, out, aux, st);
+}
+
+size_t multi_lds_bytes() {
+  return (size_t)(kMultiS * kMultiP + TwLayout<kMultiP>::kEntries) * sizeof(float2);
+}
+
+// Work list for the current row counts: items (event, group, row tile) in
+// decreasing cost (sub-sequence steps) order, and the epilogue blocks.
+int multi_set_rows(gwl_multi* m, const int64_t* rows, hipStream_t st) {
+  std::vector<MultiItem> items;
+  std::vector<int> bev;
+  std::vector<long long> brow;
+  std::vector<long long> cost;
+  for (int e = 0; e < m->n_events; ++e) {
+    if (rows[e] < 0 || rows[e] > m->max_rows[e]) {
+      set_error("gwl_multi_set_rows: rows outside [0, max_rows]");
+      return GWL_E_INVALID;
+    }
+    m->rows[e] = rows[e];
+    const EvDev& ev = m->events[e]->dev;
+    const int Q = ev.N / kMultiP;
+    const int G = m->groups[e];
+    for (long long r0 = 0; r0 < rows[e]; r0 += kMultiS) {
+      for (int g = 0; g < G; ++g) {
+        MultiItem it;
+        it.e = e;
+        it.g = g;
+        it.row0 = r0;
+        items.push_back(it);
+        cost.push_back(((long long)(g + 1) * Q) / G - ((long long)g * Q) / G);
+      }
+    }
+    for (long long r0 = 0; r0 < rows[e]; r0 += kEpiRows) {
+      bev.push_back(e);
+      brow.push_back(r0);
+    }
+  }
+  if (items.size() > m->max_items || bev.size() > m->max_blocks) {
+    set_error("gwl_multi_set_rows: work list larger than allocated");
+    return GWL_E_INVALID;
+  }
+  std::vector<size_t> order(items.size());
+  for (size_t i = 0; i < order.size(); ++i) order[i] = i;
+  std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) { return cost[a] > cost[b]; });
+  std::vector<MultiItem> sorted(items.size());
+  for (size_t i = 0; i < order.size(); ++i) sorted[i] = items[order[i]];
+  const int n_items = (int)sorted.size();
+  const int n_blocks = (int)bev.size();
+  std::vector<long long> rr(m->rows.begin(), m->rows.end());
+  GWL_HIP_CHECK(hipMemcpy(m->d_rows, rr.data(), rr.size() * sizeof(long long), hipMemcpyHostToDevice));
+  if (n_items)
+    GWL_HIP_CHECK(hipMemcpy(m->d_items, sorted.data(), n_items * sizeof(MultiItem), hipMemcpyHostToDevice));
+  GWL_HIP_CHECK(hipMemcpy(m->d_n_items, &n_items, sizeof(int), hipMemcpyHostToDevice));
+  if (n_blocks) {
+    GWL_HIP_CHECK(hipMemcpy(m->d_block_event, bev.data(), n_blocks * sizeof(int), hipMemcpyHostToDevice));
+    GWL_HIP_CHECK(hipMemcpy(m->d_block_row0, brow.data(), n_blocks * sizeof(long long), hipMemcpyHostToDevice));
+  }
+  GWL_HIP_CHECK(hipMemcpy(m->d_n_blocks, &n_blocks, sizeof(int), hipMemcpyHostToDevice));
+  m->n_items = n_items;
+  (void)st;
+  return GWL_OK;
+}
+
+// zero the queue counter (a kernel, not a memset node), persistent pass,
+// epilogue over every block the work list can hold.
+int multi_launch_raw(gwl_multi* m, hipStream_t st) {
+  hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(1), 0, st, m->d_counter);
+  GWL_HIP_CHECK(hipGetLastError());
+  MultiArgs a;
+  a.evs = m->d_evs;
+  a.params = m->d_params;
+  a.rows = m->d_rows;
+  a.groups = m->d_groups;
+  a.part = reinterpret_cast<double2* const*>(m->d_parts);
+  a.hhpart = m->d_hhs;
+  a.items = m->d_items;
+  a.n_items = m->d_n_items;
+  a.counter = m->d_counter;
+  a.ld = GWL_NPARAM;
+  const dim3 grid((unsigned)(m->ctx->n_cu * 2));
+  if (m->wave == GWL_TAYLORF2)
+    hipLaunchKernelGGL(k_time_multi<GWL_TAYLORF2>, grid, dim3(kMultiNT), multi_lds_bytes(), st, a);
+  else
+    hipLaunchKernelGGL(k_time_multi<GWL_SINE_GAUSSIAN>, grid, dim3(kMultiNT), multi_lds_bytes(), st, a);
+  GWL_HIP_CHECK(hipGetLastError());
+  MultiEpiArgs ea;
+  ea.evs = m->d_evs;
+  ea.params = m->d_params;
+  ea.rows = m->d_rows;
+  ea.groups = m->d_groups;
+  ea.part = reinterpret_cast<double2* const*>(m->d_parts);
+  ea.hhpart = m->d_hhs;
+  ea.out = m->d_outs;
+  ea.block_event = m->d_block_event;
+  ea.block_row0 = m->d_block_row0;
+  ea.n_blocks = m->d_n_blocks;
+  ea.ld = GWL_NPARAM;
+  size_t klds = 0;
+  for (auto* ev : m->events)
+    if (ev->dev.flags & GWL_DISTANCE_MARG)
+      klds = std::max(klds, (size_t)(ev->dev.sp.nx + ev->dev.sp.ny + 8) * sizeof(double));
+  hipLaunchKernelGGL(k_epi_time_multi, dim3((unsigned)m->max_blocks), dim3(64 * kEpiRows), klds, st, ea);
+  GWL_HIP_CHECK(hipGetLastError());
+  return GWL_OK;
 }
 
 int timing_begin(gwl_event* e, hipStream_t st) {

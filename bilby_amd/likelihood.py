@@ -20,6 +20,7 @@ from .detector import InterferometerList
 from .prior import Uniform
 
 _CTX = {}
+_INTERPOLANTS = {}  # distance interpolants by (grid and prior digest, phase flag)
 
 
 def get_context(device=0):
@@ -53,6 +54,77 @@ def d_inner_h_ref_axis(phase_marginalization, n=800):
 def optimal_snr_squared_ref_axis(n=400):
     """base.py:897-900."""
     return np.logspace(-5, 10, n)
+
+
+def _test_cached_lookup_table(loaded, distance_array, prior_array, ref_dist, phase):
+    """base.py:979-992."""
+    pairs = dict(distance_array=distance_array, prior_array=prior_array,
+                 reference_distance=ref_dist, phase_marginalization=phase)
+    for key, val in pairs.items():
+        if key not in loaded:
+            return False
+        if not np.allclose(np.atleast_1d(loaded[key]), np.atleast_1d(val), rtol=1e-15):
+            return False
+    return True
+
+
+def create_lookup_table(device, distance_array, prior_array, ref_dist, phase):
+    """base.py:994-1018 on the GPU (gwl_distance_table)."""
+    ctx = get_context(device)
+    x = d_inner_h_ref_axis(phase)
+    y = optimal_snr_squared_ref_axis()
+    out = np.zeros((len(y), len(x)))
+    dist = np.ascontiguousarray(distance_array, dtype=np.float64)
+    prior = np.ascontiguousarray(prior_array, dtype=np.float64)
+    check(LIB.gwl_distance_table(ctx, dptr(dist), dptr(prior), len(dist), float(ref_dist),
+                                 int(phase), dptr(x), len(x), dptr(y), len(y), dptr(out)),
+          "gwl_distance_table")
+    return out
+
+
+def distance_interpolant(device, distance_array, prior_array, ref_dist, phase, lookup_table=None):
+    """Lookup table (loaded from / cached to an .npz file when a name is
+    given, base.py:936-977) and the knots, coefficients and bounding box of
+    its bicubic interpolant (base.py:929-934)."""
+    import hashlib
+    key = None
+    if lookup_table is None:
+        h = hashlib.sha1()
+        h.update(np.ascontiguousarray(distance_array, dtype=np.float64).tobytes())
+        h.update(np.ascontiguousarray(prior_array, dtype=np.float64).tobytes())
+        h.update(np.float64(ref_dist).tobytes())
+        key = (h.hexdigest(), bool(phase))
+        if key in _INTERPOLANTS:
+            return _INTERPOLANTS[key]
+    table = None
+    filename = None
+    if isinstance(lookup_table, str):
+        filename = lookup_table if lookup_table.endswith(".npz") else lookup_table + ".npz"
+        if os.path.exists(filename):
+            loaded = dict(np.load(filename))
+            if _test_cached_lookup_table(loaded, distance_array, prior_array, ref_dist, phase):
+                table = loaded["lookup_table"]
+    elif isinstance(lookup_table, dict):
+        if _test_cached_lookup_table(lookup_table, distance_array, prior_array, ref_dist, phase):
+            table = lookup_table["lookup_table"]
+    if table is None:
+        table = create_lookup_table(device, distance_array, prior_array, ref_dist, phase)
+        if filename is not None:
+            np.savez(filename, distance_array=distance_array, prior_array=prior_array,
+                     lookup_table=table, reference_distance=ref_dist,
+                     phase_marginalization=phase)
+    x = d_inner_h_ref_axis(phase, table.shape[1])
+    y = optimal_snr_squared_ref_axis(table.shape[0])
+    z = np.ascontiguousarray(table.T)
+    tx = np.zeros(len(x) + 4)
+    ty = np.zeros(len(y) + 4)
+    c = np.zeros_like(z)
+    check(LIB.gwl_spline2d_coefficients(len(x), len(y), dptr(x), dptr(y), dptr(z), dptr(tx),
+                                        dptr(ty), dptr(c)), "gwl_spline2d_coefficients")
+    bbox = np.array([x.min(), x.max(), y.min(), y.max()])
+    if key is not None:
+        _INTERPOLANTS[key] = (table, (tx, ty, c, bbox))
+    return table, (tx, ty, c, bbox)
 
 
 class GravitationalWaveTransient:
@@ -206,62 +278,20 @@ class GravitationalWaveTransient:
         """base.py:916-992: lookup table (cached to an .npz file when a file
         name is given, validated like _test_cached_lookup_table) and its
         bicubic interpolant, attached to the device event."""
-        table = None
-        filename = None
-        if isinstance(lookup_table, str):
-            filename = lookup_table if lookup_table.endswith(".npz") else lookup_table + ".npz"
-            if os.path.exists(filename):
-                loaded = dict(np.load(filename))
-                if self._test_cached_lookup_table(loaded):
-                    table = loaded["lookup_table"]
-        elif isinstance(lookup_table, dict):
-            if self._test_cached_lookup_table(lookup_table):
-                table = lookup_table["lookup_table"]
-        if table is None:
-            table = self.create_lookup_table()
-            if filename is not None:
-                np.savez(filename, distance_array=self._distance_array,
-                         prior_array=self.distance_prior_array, lookup_table=table,
-                         reference_distance=self._ref_dist,
-                         phase_marginalization=self.phase_marginalization)
+        table, spline = distance_interpolant(self.device, self._distance_array,
+                                             self.distance_prior_array, self._ref_dist,
+                                             self.phase_marginalization, lookup_table)
         self._dist_margd_loglikelihood_array = table
-        x = d_inner_h_ref_axis(self.phase_marginalization, table.shape[1])
-        y = optimal_snr_squared_ref_axis(table.shape[0])
-        z = np.ascontiguousarray(table.T)
-        tx = np.zeros(len(x) + 4)
-        ty = np.zeros(len(y) + 4)
-        c = np.zeros_like(z)
-        check(LIB.gwl_spline2d_coefficients(len(x), len(y), dptr(x), dptr(y), dptr(z), dptr(tx),
-                                            dptr(ty), dptr(c)), "gwl_spline2d_coefficients")
-        self._spline = (tx, ty, c)
-        bbox = np.array([x.min(), x.max(), y.min(), y.max()])
-        check(LIB.gwl_event_set_distance_spline(self._event, len(x), len(y), dptr(tx), dptr(ty),
-                                                dptr(c), dptr(bbox)),
+        self._spline = spline
+        tx, ty, c, bbox = spline
+        check(LIB.gwl_event_set_distance_spline(self._event, len(tx) - 4, len(ty) - 4, dptr(tx),
+                                                dptr(ty), dptr(c), dptr(bbox)),
               "gwl_event_set_distance_spline")
-
-    def _test_cached_lookup_table(self, loaded):
-        pairs = dict(distance_array=self._distance_array, prior_array=self.distance_prior_array,
-                     reference_distance=self._ref_dist,
-                     phase_marginalization=self.phase_marginalization)
-        for key, val in pairs.items():
-            if key not in loaded:
-                return False
-            if not np.allclose(np.atleast_1d(loaded[key]), np.atleast_1d(val), rtol=1e-15):
-                return False
-        return True
 
     def create_lookup_table(self):
         """base.py:994-1018 on the GPU (gwl_distance_table)."""
-        ctx = get_context(self.device)
-        x = d_inner_h_ref_axis(self.phase_marginalization)
-        y = optimal_snr_squared_ref_axis()
-        out = np.zeros((len(y), len(x)))
-        dist = np.ascontiguousarray(self._distance_array, dtype=np.float64)
-        prior = np.ascontiguousarray(self.distance_prior_array, dtype=np.float64)
-        check(LIB.gwl_distance_table(ctx, dptr(dist), dptr(prior), len(dist), float(self._ref_dist),
-                                     int(self.phase_marginalization), dptr(x), len(x), dptr(y),
-                                     len(y), dptr(out)), "gwl_distance_table")
-        return out
+        return create_lookup_table(self.device, self._distance_array, self.distance_prior_array,
+                                   self._ref_dist, self.phase_marginalization)
 
     def __del__(self):
         try:

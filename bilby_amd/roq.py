@@ -1,0 +1,208 @@
+"""Drop-in ROQGravitationalWaveTransient on the MI355X.
+
+bilby/gw/likelihood/roq.py:14-300 constructor semantics for precomputed
+weights (a dict with 'time_samples', '<ifo>_linear' [T x L],
+'<ifo>_quadratic' [Q], 'frequency_nodes_linear', 'frequency_nodes_quadratic'),
+the TaylorF2 source model at the nodes, and time / phase / distance
+marginalisation.  The linear sums over the ROQ time samples are one complex
+GEMM on MFMA (float32, or split-bfloat16 x3), float64 for the reference.
+"""
+import copy
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import LIB, check, dptr
+from .detector import InterferometerList
+from .likelihood import distance_interpolant, get_context
+from .prior import Uniform
+
+MODES = {"fp64": 0, "fp32": 1, "bf16x3": 2}
+
+
+class ROQGravitationalWaveTransient:
+    def __init__(self, interferometers, waveform_generator, priors, weights=None,
+                 linear_matrix=None, quadratic_matrix=None, roq_params=None,
+                 roq_params_check=True, roq_scale_factor=1, distance_marginalization=False,
+                 phase_marginalization=False, time_marginalization=False, jitter_time=True,
+                 delta_tc=None, distance_marginalization_lookup_table=None,
+                 reference_frame="sky", time_reference="geocenter", parameter_conversion=None,
+                 precision="fp32", device=0):
+        if weights is None:
+            raise NotImplementedError(
+                "building ROQ weights from bases (linear_matrix / quadratic_matrix) is not "
+                "supported; pass precomputed weights")
+        if reference_frame != "sky" or "geocent" not in time_reference:
+            raise NotImplementedError("only the sky frame and geocenter time are supported")
+        if roq_scale_factor != 1:
+            raise NotImplementedError("roq_scale_factor != 1 is not supported")
+        self.interferometers = InterferometerList(interferometers)
+        self.waveform_generator = waveform_generator
+        self.weights = weights
+        self.priors = priors.copy()
+        self.time_marginalization = time_marginalization
+        self.phase_marginalization = phase_marginalization
+        self.distance_marginalization = distance_marginalization
+        self.jitter_time = jitter_time if time_marginalization else False
+        self.precision = MODES[precision] if isinstance(precision, str) else int(precision)
+        self.device = device
+        self._marginalized_parameters = []
+        ts = np.asarray(weights["time_samples"], float)
+        self.frequency_nodes_linear = np.asarray(weights["frequency_nodes_linear"], float)
+        self.frequency_nodes_quadratic = np.asarray(weights["frequency_nodes_quadratic"], float)
+        if time_marginalization:
+            if delta_tc is None:
+                delta_tc = ts[1] - ts[0]
+            tp = self.priors["geocent_time"]
+            if not isinstance(tp, Uniform):
+                raise NotImplementedError("time marginalisation supports a Uniform geocent_time prior")
+            n = int(np.ceil((tp.maximum - tp.minimum) / delta_tc))
+            self._delta_tc = (tp.maximum - tp.minimum) / n
+            self._times = tp.minimum + self._delta_tc / 2. + np.arange(n) * self._delta_tc
+            self._check_time_coverage(ts)
+            priors["geocent_time"] = float(self.interferometers.start_time)
+            if self.jitter_time:
+                priors["time_jitter"] = Uniform(-self._delta_tc / 2, self._delta_tc / 2,
+                                                name="time_jitter", boundary="periodic")
+            self._marginalized_parameters.append("geocent_time")
+        if phase_marginalization:
+            priors["phase"] = 0.0
+            self._marginalized_parameters.append("phase")
+        self._ref_dist = 1.0
+        if distance_marginalization:
+            prior = self.priors["luminosity_distance"]
+            self._distance_array = np.linspace(prior.minimum, prior.maximum, int(1e4))
+            self.distance_prior_array = np.array([prior.prob(d) for d in self._distance_array])
+            self._ref_dist = float(prior.rescale(0.5))
+            priors["luminosity_distance"] = float(self._ref_dist)
+            self._marginalized_parameters.append("luminosity_distance")
+        self._build(ts, delta_tc, distance_marginalization_lookup_table)
+
+    def _check_time_coverage(self, ts):
+        """Every time the interpolation reads (prior window, geocentre delay,
+        jitter, +-2 samples) must lie on the weights' time grid (the reference
+        checks the basis against the prior in perform_roq_params_check)."""
+        start = self.interferometers.start_time
+        rel = self._times - start
+        margin = 6371e3 * 1.01 / 299792458.0 + self._delta_tc
+        lo, hi = rel.min() - margin, rel.max() + margin
+        dts = ts[1] - ts[0]
+        if lo - 2 * dts < ts[0] or hi + 3 * dts > ts[-1]:
+            raise ValueError("ROQ time samples [%g, %g] do not cover the prior window [%g, %g]"
+                             % (ts[0], ts[-1], lo, hi))
+
+    def _build(self, ts, delta_tc, lookup_table):
+        ifos = self.interferometers
+        names = [ifo.name for ifo in ifos]
+        lin = np.stack([np.asarray(self.weights[n + "_linear"], complex) for n in names])
+        quad = np.stack([np.asarray(self.weights[n + "_quadratic"], complex) for n in names])
+        lin_f = np.ascontiguousarray(lin).view(np.float64).copy()
+        quad_f = np.ascontiguousarray(quad).view(np.float64).copy()
+        tens = np.ascontiguousarray(np.stack([i.geometry.detector_tensor.ravel() for i in ifos]))
+        vert = np.ascontiguousarray(np.stack([i.geometry.vertex for i in ifos]))
+        fl = np.ascontiguousarray(self.frequency_nodes_linear)
+        fq = np.ascontiguousarray(self.frequency_nodes_quadratic)
+        wfa = self.waveform_generator.waveform_arguments
+        d = _lib.RoqDesc()
+        d.n_det = len(ifos)
+        d.n_linear = len(fl)
+        d.n_quadratic = len(fq)
+        d.n_time_samples = len(ts)
+        d.start_time = float(ifos.start_time)
+        d.time_samples_start = float(ts[0])
+        d.time_samples_step = float(ts[1] - ts[0])
+        d.linear_nodes = dptr(fl)
+        d.quadratic_nodes = dptr(fq)
+        d.linear_weights = dptr(lin_f)
+        d.quadratic_weights = dptr(quad_f)
+        d.detector_tensor = dptr(tens)
+        d.vertex = dptr(vert)
+        d.reference_frequency = float(wfa.get("reference_frequency", 20.0))
+        d.wf_minimum_frequency = float(wfa.get("minimum_frequency", 20.0))
+        flags = 0
+        if self.time_marginalization:
+            flags |= _lib.TIME_MARG | (_lib.JITTER_TIME if self.jitter_time else 0)
+            tp = self.priors["geocent_time"]
+            d.time_prior_minimum = float(tp.minimum)
+            d.time_prior_maximum = float(tp.maximum)
+            d.delta_tc = float(delta_tc)
+        if self.phase_marginalization:
+            flags |= _lib.PHASE_MARG
+        if self.distance_marginalization:
+            flags |= _lib.DISTANCE_MARG
+        d.flags = flags
+        d.reference_distance = float(self._ref_dist)
+        ctx = get_context(self.device)
+        h = ctypes.c_void_p()
+        nbytes = ctypes.c_int64()
+        check(LIB.gwl_roq_create(ctx, ctypes.byref(d), ctypes.byref(h), ctypes.byref(nbytes)),
+              "gwl_roq_create")
+        self._roq = h
+        self.hbm_bytes = int(nbytes.value)
+        if self.distance_marginalization:
+            table, (tx, ty, c, bbox) = distance_interpolant(
+                self.device, self._distance_array, self.distance_prior_array, self._ref_dist,
+                self.phase_marginalization, lookup_table)
+            self._spline = (tx, ty, c, bbox)
+            check(LIB.gwl_roq_set_distance_spline(self._roq, len(tx) - 4, len(ty) - 4, dptr(tx),
+                                                  dptr(ty), dptr(c), dptr(bbox)),
+                  "gwl_roq_set_distance_spline")
+
+    def __del__(self):
+        try:
+            if getattr(self, "_roq", None) is not None:
+                LIB.gwl_roq_destroy(self._roq)
+                self._roq = None
+        except Exception:
+            pass
+
+    @property
+    def marginalized_parameters(self):
+        return self._marginalized_parameters
+
+    def _fill_defaults(self, p):
+        if self.time_marginalization:
+            p.setdefault("geocent_time", float(self.interferometers.start_time))
+            p.setdefault("time_jitter", 0.0)
+        if self.phase_marginalization:
+            p.setdefault("phase", 0.0)
+        if self.distance_marginalization:
+            p.setdefault("luminosity_distance", self._ref_dist)
+        return p
+
+    def rows_from_parameters(self, parameters):
+        p = self.waveform_generator.parameter_conversion(dict(parameters))
+        p = self._fill_defaults(p)
+        n = 1
+        for v in p.values():
+            if np.ndim(v) > 0:
+                n = len(v)
+                break
+        rows = np.zeros((n, _lib.NPARAM))
+        for key, col in _lib.PARAM_COLUMNS.items():
+            if key in p:
+                rows[:, col] = np.asarray(p[key], dtype=float)
+        return rows
+
+    def log_likelihood_ratio_batch(self, parameters, precision=None, stream=None):
+        mode = self.precision if precision is None else (
+            MODES[precision] if isinstance(precision, str) else int(precision))
+        on_device = isinstance(parameters, torch.Tensor) and parameters.is_cuda
+        if on_device:
+            rows_t = parameters.to(torch.float64).contiguous()
+        else:
+            rows = (self.rows_from_parameters(parameters) if isinstance(parameters, dict)
+                    else np.ascontiguousarray(parameters, dtype=np.float64))
+            rows_t = torch.from_numpy(rows).to("cuda:%d" % self.device)
+        n = rows_t.shape[0]
+        out = torch.empty(n, dtype=torch.float64, device=rows_t.device)
+        st = torch.cuda.current_stream(rows_t.device).cuda_stream if stream is None else stream
+        check(LIB.gwl_roq_eval(self._roq, ctypes.c_void_p(rows_t.data_ptr()), n, _lib.NPARAM, mode,
+                               ctypes.c_void_p(out.data_ptr()), ctypes.c_void_p(st)),
+              "gwl_roq_eval")
+        return out if on_device else out.cpu().numpy()
+
+    def log_likelihood_ratio(self, parameters):
+        return float(self.log_likelihood_ratio_batch(copy.copy(parameters))[0])

@@ -185,6 +185,32 @@ int gwl_fft_batch(gwl_ctx* ctx, const void* in, void* out, int64_t batch, int32_
  * ln_i0) evaluated on the device: in/out device float64 [n]. */
 int gwl_ln_i0(gwl_ctx* ctx, const double* in, double* out, int64_t n, void* stream);
 
+/* ------------------------------------------------------------------ many events */
+/* Uneven batches over many events (segments of different lengths, sampling
+ * rates and detector sets) in one persistent launch that drains a
+ * longest-first work list, followed by one epilogue launch; the pair can be
+ * captured once into a hipGraph and replayed.  The reference evaluates one
+ * likelihood object per process (bilby/core/sampler/base_sampler.py:772-800);
+ * this has no direct counterpart there.  Every event must be
+ * time-marginalised with a 512-sample transform tile (time-prior windows up
+ * to 512 samples), <= 3 detectors and one source model. */
+typedef struct gwl_multi gwl_multi;
+/* params_dev[e]: device [max_rows[e]][GWL_NPARAM]; out_dev[e]: device
+ * [max_rows[e]]; both caller-owned and fixed for the object's lifetime. */
+int gwl_multi_create(gwl_ctx* ctx, int32_t n_events, gwl_event* const* events,
+                     const double* const* params_dev, double* const* out_dev,
+                     const int64_t* max_rows, gwl_multi** out);
+/* Rows to evaluate per event in the next launches (<= max_rows). */
+int gwl_multi_set_rows(gwl_multi* m, const int64_t* rows, void* stream);
+/* Work-item size per event in decimated sub-sequence steps (0 = default,
+ * ~32); the autotuner sets it per bin-count bucket. */
+int gwl_multi_set_tile(gwl_multi* m, const int32_t* steps_per_item);
+/* Launch (replays the captured graph if gwl_multi_capture was called). */
+int gwl_multi_launch(gwl_multi* m, void* stream);
+/* Capture the launch sequence on a non-default stream into a hipGraph. */
+int gwl_multi_capture(gwl_multi* m, void* stream);
+int gwl_multi_destroy(gwl_multi* m);
+
 /* ------------------------------------------------------------------ ROQ */
 /* Reduced-order-quadrature likelihood, bilby/gw/likelihood/roq.py
  * (ROQGravitationalWaveTransient: calculate_snrs roq.py:467-549, time grid
