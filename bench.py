@@ -58,6 +58,227 @@ def cpu_baseline(setup, priors_fixed, seconds):
                             "numpy, not lalsimulation" % (n, dt))
 
 
+def timed(fn, steps, warmup=2):
+    import torch
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        fn()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) / steps
+
+
+def bench_sine_gaussian(steps):
+    """Plain likelihood, sine-Gaussian, H1+L1, 4 s @ 2048 Hz, 10^4 draws."""
+    import numpy as np
+    import torch
+    from bilby_amd.configs import make_sine_gaussian_plain
+    from bilby_amd.likelihood import GravitationalWaveTransient
+    from tests.helpers import draws_as_dicts, oracle_from_setup
+    setup = make_sine_gaussian_plain(n_draws=10000, seed=1234)
+    lk = GravitationalWaveTransient(setup["interferometers"], setup["waveform_generator"])
+    rows = torch.from_numpy(lk.rows_from_parameters(setup["draws"])).cuda()
+    out = {}
+    res = {}
+    for prec in ("fp32", "fp64"):
+        dt = timed(lambda: lk.log_likelihood_ratio_batch(rows, precision=prec), steps)
+        out[prec + "_evals_per_s"] = rows.shape[0] / dt
+        res[prec] = lk.log_likelihood_ratio_batch(rows, precision=prec).cpu().numpy()
+    olk, _ = oracle_from_setup(setup)
+    ref = np.array([olk.log_likelihood_ratio(p) for p in draws_as_dicts(setup["draws"], range(64))])
+    for prec in ("fp32", "fp64"):
+        d = np.abs(res[prec][:64] - ref)
+        out[prec + "_max_abs_dlnL_vs_oracle_64rows"] = float(d.max())
+        out[prec + "_rows_within_1e-3_vs_fp64_gpu"] = float(np.mean(np.abs(res[prec] - res["fp64"]) < 1e-3))
+    out["max_abs_lnL"] = float(np.max(np.abs(res["fp64"])))
+    return out
+
+
+def bench_uneven(steps):
+    """64 events, durations log-uniform over {4..128} s, 2048/4096 Hz, 2-3
+    detectors, aligned-spin TaylorF2, T+D+P marginalised, 256-4096 rows each,
+    one persistent launch replayed from a hipGraph; tile per bin-count bucket
+    chosen by timing."""
+    import numpy as np
+    import torch
+    from bilby_amd.configs import make_uneven_events
+    from bilby_amd.likelihood import GravitationalWaveTransient
+    from bilby_amd.multi import MultiEventLikelihood
+    from tests.helpers import fixed_marginalised
+    events = make_uneven_events(n_events=64, seed=97531)
+    lks, draws = [], []
+    for ev in events:
+        pri = ev["priors"].copy()
+        lks.append(GravitationalWaveTransient(ev["interferometers"], ev["waveform_generator"],
+                                              time_marginalization=True,
+                                              distance_marginalization=True,
+                                              phase_marginalization=True, priors=pri))
+        draws.append(fixed_marginalised(ev["draws"], pri))
+    multi = MultiEventLikelihood(lks, [ev["n_rows"] for ev in events])
+    for e, d in enumerate(draws):
+        multi.set_parameters(e, d)
+    total_rows = sum(ev["n_rows"] for ev in events)
+    buckets = sorted({len(lk.interferometers.frequency_array) - 1 for lk in lks})
+    steps_of = {b: 32 for b in buckets}
+    sweep = {}
+    for b in buckets:
+        best = None
+        for cand in (8, 16, 32, 64):
+            steps_of[b] = cand
+            multi.set_tile([steps_of[len(lk.interferometers.frequency_array) - 1] for lk in lks])
+            dt = timed(lambda: multi.run(), 2, warmup=1)
+            sweep["%d:%d" % (b, cand)] = dt * 1e3
+            if best is None or dt < best[0]:
+                best = (dt, cand)
+        steps_of[b] = best[1]
+    multi.set_tile([steps_of[len(lk.interferometers.frequency_array) - 1] for lk in lks])
+    multi.capture()
+    dt = timed(lambda: multi.run(), steps)
+    outs = [o.cpu().numpy() for o in multi.run()]
+    # agreement with the per-event float64 path
+    dmax = 0.0
+    for lk, d, o in zip(lks[:16], draws[:16], outs[:16]):
+        ref = lk.log_likelihood_ratio_batch(d, precision="fp64")
+        dmax = max(dmax, float(np.max(np.abs(o - ref))))
+    n_bins = [len(lk.interferometers.frequency_array) for lk in lks]
+    return dict(events=len(events), rows=int(total_rows), ms_per_launch=dt * 1e3,
+                evals_per_s=total_rows / dt, bins_min=int(min(n_bins)), bins_max=int(max(n_bins)),
+                tile_steps_by_bins={str(k): int(v) for k, v in steps_of.items()},
+                tile_sweep_ms=sweep, fp32_vs_fp64_gpu_max_16_events=dmax, graph=True)
+
+
+def bench_roq(steps):
+    """ROQ, H1+L1, 400 linear / 120 quadratic nodes, 1000 ROQ times,
+    time + phase marginalised, 65536 TaylorF2 draws: the contraction
+    (65536 x 400) . (400 x 1000) per detector on MFMA."""
+    import ctypes
+    import numpy as np
+    import torch
+    from bilby_amd._lib import LIB, check
+    from bilby_amd.configs import make_roq
+    from bilby_amd.likelihood import get_context
+    from bilby_amd.roq import ROQGravitationalWaveTransient
+    from oracle.roq import OracleROQ
+    from tests.helpers import draws_as_dicts, fixed_marginalised
+    s = make_roq(n_draws=65536)
+    w = s["weights"]
+    opri = dict(s["priors"])
+    priors = s["priors"].copy()
+    lk = ROQGravitationalWaveTransient(s["interferometers"], s["waveform_generator"], priors,
+                                       weights=w, time_marginalization=True,
+                                       phase_marginalization=True, delta_tc=s["delta_tc"])
+    draws = fixed_marginalised(s["draws"], priors)
+    rows = torch.from_numpy(lk.rows_from_parameters(draws)).cuda()
+    out = {}
+    res = {}
+    for mode in ("fp32", "bf16x3", "fp64"):
+        st = steps if mode != "fp64" else 1
+        dt = timed(lambda: lk.log_likelihood_ratio_batch(rows, precision=mode), st, warmup=1)
+        out[mode + "_evals_per_s"] = rows.shape[0] / dt
+        res[mode] = lk.log_likelihood_ratio_batch(rows, precision=mode).cpu().numpy()
+    o = OracleROQ(["H1", "L1"], s["interferometers"].start_time, w, w["frequency_nodes_linear"],
+                  w["frequency_nodes_quadratic"], opri, time_marginalization=True,
+                  phase_marginalization=True, delta_tc=s["delta_tc"],
+                  waveform_arguments=s["waveform_generator"].waveform_arguments)
+    ref = np.array([o.log_likelihood_ratio(p) for p in draws_as_dicts(draws, range(64))])
+    for mode in res:
+        out[mode + "_max_abs_dlnL_vs_oracle_64rows"] = float(np.max(np.abs(res[mode][:64] - ref)))
+        out[mode + "_vs_fp64_gpu_max_all_rows"] = float(np.max(np.abs(res[mode] - res["fp64"])))
+    # the GEMM alone at the workload's shape: M = 65536, N = 4096 (2 detectors x 1000
+    # complex times, padded), K = 800 (400 complex nodes)
+    ctx = get_context(0)
+    M, N, K = 65536, 4096, 800
+    useful = 8.0 * 65536 * 400 * 1000 * 2
+    a = torch.randn(M, K, device="cuda")
+    b = torch.randn(N, K, device="cuda")
+    c = torch.empty(M, N, device="cuda")
+    ah, bh = a.to(torch.bfloat16), b.to(torch.bfloat16)
+    al, bl = (a - ah.float()).to(torch.bfloat16), (b - bh.float()).to(torch.bfloat16)
+    def g32():
+        check(LIB.gwl_gemm(ctx, 1, ctypes.c_void_p(a.data_ptr()), None, ctypes.c_void_p(b.data_ptr()),
+                           None, ctypes.c_void_p(c.data_ptr()), M, N, K, None))
+    def g16():
+        check(LIB.gwl_gemm(ctx, 2, ctypes.c_void_p(ah.data_ptr()), ctypes.c_void_p(al.data_ptr()),
+                           ctypes.c_void_p(bh.data_ptr()), ctypes.c_void_p(bl.data_ptr()),
+                           ctypes.c_void_p(c.data_ptr()), M, N, K, None))
+    t32 = timed(g32, steps)
+    t16 = timed(g16, steps)
+    out["gemm_fp32_ms"] = t32 * 1e3
+    out["gemm_fp32_tflops_useful"] = useful / t32 / 1e12
+    out["gemm_fp32_mfma_frac"] = (2.0 * M * N * K) / t32 / 157.3e12
+    out["gemm_bf16x3_ms"] = t16 * 1e3
+    out["gemm_bf16x3_tflops_useful"] = useful / t16 / 1e12
+    out["gemm_bf16x3_mfma_frac"] = (3 * 2.0 * M * N * K) / t16 / 2.5e15
+    return out
+
+
+def bench_serving(main_lk, main_rows, duration):
+    """32 samplers, seeded Poisson arrivals at 2000 requests/s, 1-1024 rows per
+    request from a 60/30/10 mix of the TaylorF2 T+D+P event, uneven events and
+    the ROQ event; everything resident; HBM budget 200 GB."""
+    import numpy as np
+    from bilby_amd.configs import make_roq, make_uneven_events
+    from bilby_amd.likelihood import GravitationalWaveTransient
+    from bilby_amd.roq import ROQGravitationalWaveTransient
+    from bilby_amd.serving import LikelihoodServer, latency_summary, run_load
+    from tests.helpers import fixed_marginalised
+    server = LikelihoodServer(hbm_budget=200e9, max_batch_rows=4096)
+    try:
+        server.register("c2", main_lk)
+        events = make_uneven_events(n_events=64, seed=97531, rows_range=(256, 4096))
+        names, lks, pools = [], [], []
+        for k, ev in enumerate(events):
+            pri = ev["priors"].copy()
+            lk = GravitationalWaveTransient(ev["interferometers"], ev["waveform_generator"],
+                                            time_marginalization=True,
+                                            distance_marginalization=True,
+                                            phase_marginalization=True, priors=pri)
+            names.append("c3_%d" % k)
+            lks.append(lk)
+            pools.append(lk.rows_from_parameters(fixed_marginalised(ev["draws"], pri)))
+        server.register_multi(names, lks)
+        s = make_roq(n_draws=8192)
+        priors = s["priors"].copy()
+        roq = ROQGravitationalWaveTransient(s["interferometers"], s["waveform_generator"], priors,
+                                            weights=s["weights"], time_marginalization=True,
+                                            phase_marginalization=True, delta_tc=s["delta_tc"])
+        server.register("c4", roq)
+        roq_rows = roq.rows_from_parameters(fixed_marginalised(s["draws"], priors))
+        mix = [(0.6, [("c2", main_rows)]), (0.3, list(zip(names, pools))), (0.1, [("c4", roq_rows)])]
+        # warm the paths once
+        for name, pool in (("c2", main_rows), (names[0], pools[0]), ("c4", roq_rows)):
+            server.submit(name, pool[:8]).result()
+        t_start = time.perf_counter()
+        issued, wall = run_load(server, mix, n_samplers=32, rate=2000.0, duration=duration,
+                                rows_range=(1, 1024), seed=2024)
+        summ = latency_summary(server, since=t_start)
+        summ["evals_per_s"] = summ["rows"] / wall
+        summ["requests_per_s"] = summ["requests"] / wall
+        summ["rounds"] = server.rounds
+        summ["resident_gb"] = server.resident_bytes / 1e9
+        # audit ~1% of requests (capped) against the float64 path on the device
+        rng = np.random.default_rng(7)
+        pick = rng.choice(len(issued), size=max(1, min(40, len(issued) // 100)), replace=False)
+        lk_of = dict(zip(names, lks))
+        lk_of["c2"] = main_lk
+        lk_of["c4"] = roq
+        pool_of = dict(zip(names, pools))
+        pool_of["c2"] = main_rows
+        pool_of["c4"] = roq_rows
+        dmax = 0.0
+        for i in pick:
+            name, idx, fut = issued[i]
+            ref = lk_of[name].log_likelihood_ratio_batch(pool_of[name][idx], precision="fp64")
+            dmax = max(dmax, float(np.max(np.abs(fut.result() - ref))))
+        summ["audit_requests"] = int(len(pick))
+        summ["audit_max_abs_dlnL_vs_fp64"] = dmax
+        return summ
+    finally:
+        server.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -68,6 +289,8 @@ def main():
     ap.add_argument("--per-sample-calls", type=int, default=300)
     ap.add_argument("--audit", type=int, default=64, help="rows checked against the FP64 oracle")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-extra", action="store_true", help="headline workload only")
+    ap.add_argument("--serve-seconds", type=float, default=5.0)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only; --gpus %d is not supported in this build "
@@ -187,6 +410,21 @@ def main():
                             "batched_fp32_over_per_sample_fp64": evals_per_s / per_sample_rate},
     }
 
+    extra = {}
+    if not args.no_extra:
+        for name, fn in (("sine_gaussian_plain_4s_2048Hz_10k", lambda: bench_sine_gaussian(10)),
+                         ("uneven_64_events_persistent_graph", lambda: bench_uneven(5)),
+                         ("roq_400_120_nodes_1000_times_65536", lambda: bench_roq(3)),
+                         ("serving_32_samplers_2000_rps", lambda: bench_serving(
+                             lk, rows.cpu().numpy(), args.serve_seconds))):
+            log("extra:", name)
+            try:
+                extra[name] = fn()
+            except Exception as exc:  # report, keep the headline line
+                extra[name] = {"error": repr(exc)}
+            log(name, json.dumps(extra[name])[:400])
+    result_extra = extra
+
     if not args.no_cpu_baseline:
         log("cpu baseline (oracle, 1 core, %.0f s)" % args.cpu_seconds)
         olk, cb = cpu_baseline(setup, priors, args.cpu_seconds)
@@ -204,6 +442,11 @@ def main():
             % (result["accuracy"]["fp32_vs_oracle_max"], result["accuracy"]["fp64_vs_oracle_max"]))
     else:
         result["cpu_baseline"] = None
+    result["configs"] = result_extra
+    srv = result_extra.get("serving_32_samplers_2000_rps", {})
+    if "p50_ms" in srv:
+        result["latency_under_load"] = {"p50_ms": srv["p50_ms"], "p99_ms": srv["p99_ms"],
+                                        "evals_per_s": srv["evals_per_s"]}
     print(json.dumps(result), flush=True)
 
 

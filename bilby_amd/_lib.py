@@ -83,6 +83,7 @@ def _load():
                           ctypes.c_int),
         "gwl_ln_i0": ([vp, vp, vp, ctypes.c_int64, vp], ctypes.c_int),
         "gwl_noise_log_likelihood": ([vp, _dp], ctypes.c_int),
+        "gwl_event_device_bytes": ([vp, ctypes.POINTER(ctypes.c_int64)], ctypes.c_int),
         "gwl_event_timing": ([vp, ctypes.c_int32], ctypes.c_int),
         "gwl_event_set_tile": ([vp, ctypes.c_int32, ctypes.c_int32], ctypes.c_int),
         "gwl_roq_create": ([vp, ctypes.POINTER(RoqDesc), ctypes.POINTER(vp),
@@ -139,5 +140,5 @@ EXPORTED_SYMBOLS = [
     "gwl_event_timing", "gwl_event_timing_read", "gwl_event_set_tile",
     "gwl_roq_create", "gwl_roq_destroy", "gwl_roq_set_distance_spline", "gwl_roq_eval",
     "gwl_gemm", "gwl_multi_create", "gwl_multi_set_rows", "gwl_multi_set_tile",
-    "gwl_multi_launch", "gwl_multi_capture", "gwl_multi_destroy",
+    "gwl_multi_launch", "gwl_multi_capture", "gwl_multi_destroy", "gwl_event_device_bytes",
 ]

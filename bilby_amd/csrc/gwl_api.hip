@@ -527,6 +527,12 @@ int gwl_event_reserve(gwl_event* e, int64_t rows) {
   return GWL_OK;
 }
 
+int gwl_event_device_bytes(gwl_event* e, int64_t* bytes) {
+  if (!e || !bytes) { set_error("gwl_event_device_bytes: invalid argument"); return GWL_E_INVALID; }
+  *bytes = (int64_t)(e->data_bytes + e->spline_bytes + e->ws_bytes);
+  return GWL_OK;
+}
+
 int gwl_noise_log_likelihood(gwl_event* e, double* out) {
   if (!e || !out) { set_error("gwl_noise_log_likelihood: invalid argument"); return GWL_E_INVALID; }
   GWL_HIP_CHECK(hipSetDevice(e->ctx->device));

@@ -76,7 +76,8 @@ enum { ROQ_FP64 = 0, ROQ_FP32 = 1, ROQ_BF16X3 = 2 };
 // ------------------------------------------------------------------ prep
 // One workgroup (128 threads) per row.  A layout [Mpad][Kp]: column l < L
 // holds Re conj(h0(f_l)) = Re h0, column L + l holds Im conj(h0) = -Im h0,
-// where the detector response is h_d = C_d h0 (h0 = A(f) e^{-i Psi(f)}).
+// where the detector response is h_d = C_d h0, h0 = f^-7/6 e^{-i Psi(f)} and
+// the amplitude prefactor lives in C_d.
 template <int MODE>
 __global__ __launch_bounds__(128) void k_roq_prep(RoqDev rd, const double* params, int ld,
                                                    long long n, void* aout, void* aout_lo,
@@ -140,7 +141,7 @@ __global__ __launch_bounds__(128) void k_roq_prep(RoqDev rd, const double* param
         double b[kNB];
         tf2_basis(f, b);
         const double cyc = tf2_phase_cycles(c, b);
-        const double amp = amp0 * b[1] * sqrt(b[3]);
+        const double amp = b[1] * sqrt(b[3]);  // f^-7/6; amp0 sits in C_d
         double sn, cs;
         sincospi(2.0 * (cyc - floor(cyc)), &sn, &cs);
         xr = amp * cs;   // Re conj(h0) = Re h0
@@ -182,7 +183,7 @@ __global__ __launch_bounds__(128) void k_roq_prep(RoqDev rd, const double* param
       if (f >= rd.wf_fmin && f <= fisco) {
         double b[kNB];
         tf2_basis(f, b);
-        const double amp = amp0 * b[1] * sqrt(b[3]);
+        const double amp = b[1] * sqrt(b[3]);
         for (int d = 0; d < rd.nd; ++d) acc[d] += amp * amp * rd.wq[(size_t)d * rd.Qn + q];
       }
     }
@@ -404,6 +405,16 @@ __device__ inline void five_sample(const T* z, int base_c, double x, double& re,
 
 constexpr int kRoqEpiRows = 4;
 
+// Time-marginalisation grid point j (roq.py:320-331), rounded after each
+// operation as numpy does.  The grid is in absolute GPS seconds (~1e9, one
+// ulp = 2.4e-7 s) and the segment start is subtracted afterwards, so a fused
+// multiply-add would move a point by up to one ulp, about 1e-3 of a ROQ time
+// sample, and change the interpolated d_inner_h.
+__device__ inline double roq_grid_time(double tcmin, double dtc, int j) {
+#pragma clang fp contract(off)
+  return (tcmin + dtc / 2.0) + (double)j * dtc;
+}
+
 template <typename T>
 __global__ __launch_bounds__(64 * kRoqEpiRows) void k_roq_epi(RoqDev rd, const T* Z,
                                                               const RoqRow* rows, long long n,
@@ -489,7 +500,7 @@ __global__ __launch_bounds__(64 * kRoqEpiRows) void k_roq_epi(RoqDev rd, const T
   }
   double m = -INFINITY, s = 0.0;
   for (int j = lane; j < rd.ntimes; j += 64) {
-    const double tj = (rd.tcmin + rd.dtc / 2.0) + (double)j * rd.dtc;
+    const double tj = roq_grid_time(rd.tcmin, rd.dtc, j);
     const double t = tj + rr.jitter;
     if (!(t >= rd.tmin && t <= rd.tmax)) continue;
     bool ok;

@@ -119,6 +119,10 @@ int gwl_event_reserve(gwl_event* ev, int64_t max_rows);
  * -sum_det |4/T sum_mask |d|^2 / S| / 2 (base.py:402-413). */
 int gwl_noise_log_likelihood(gwl_event* ev, double* out);
 
+/* Device bytes the event currently holds (data, spline, workspace): the
+ * figure a server admits against its HBM budget. */
+int gwl_event_device_bytes(gwl_event* ev, int64_t* bytes);
+
 /* Batched log-likelihood ratio: out[i] = lnL ratio of params[i*ld ..].
  * Replaces base.py:419-446 log_likelihood_ratio (one row per call there).
  * params, out: device pointers.  aux (device, may be NULL) receives per row
