@@ -291,6 +291,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="headline workload only")
     ap.add_argument("--serve-seconds", type=float, default=5.0)
+    ap.add_argument("--profile", action="store_true",
+                    help="counter runs: --steps launches of the headline batch at tile (8, 2), "
+                         "then 3 copies of 512 MiB (a plain copy kernel of known size)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only; --gpus %d is not supported in this build "
@@ -317,6 +320,19 @@ def main():
     rows = torch.from_numpy(lk.rows_from_parameters(draws)).to(dev)
     check(LIB.gwl_event_reserve(lk._event, args.batch))
     from bilby_amd.autotune import autotune, set_tile
+    if args.profile:
+        set_tile(lk, 8, 2)
+        for _ in range(args.warmup + args.steps):
+            lk.log_likelihood_ratio_batch(rows)
+        torch.cuda.synchronize()
+        src = torch.empty(512 << 20, dtype=torch.uint8, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log("profile: %d headline launches (%d rows, %d bytes/row algorithmic), 3 copies of %d bytes"
+            % (args.warmup + args.steps, args.batch, bytes_per_eval(lk), src.numel()))
+        return
     tile, sweep = autotune(lk, rows, use_cache=False)
     log("autotune (rows/workgroup, workgroups/CU) -> ms:", sweep, "chosen", tile)
 

@@ -46,6 +46,7 @@ class RoqDesc(ctypes.Structure):
         ("flags", ctypes.c_uint32),
         ("time_prior_minimum", ctypes.c_double), ("time_prior_maximum", ctypes.c_double),
         ("delta_tc", ctypes.c_double), ("reference_distance", ctypes.c_double),
+        ("time_samples", _dp),
     ]
 
 

@@ -1,7 +1,9 @@
 """Tile autotuner for the likelihood kernels.
 
 For an event and a batch shape it times the candidate tiles (rows of the
-parameter batch per workgroup, target resident workgroups per CU) on the
+parameter batch per workgroup, target resident workgroups per CU; rows 0
+selects the wave-per-row TaylorF2 time transform where the event allows it,
+and the second number is then the target wavefronts per CU) on the
 device with HIP events, keeps the fastest, and caches the choice per
 (device, source model, marginalisation flags, detectors, bin-count bucket,
 batch bucket) in a JSON file so later processes reuse it.
@@ -15,7 +17,8 @@ import torch
 
 from ._lib import LIB, check
 
-DEFAULT_CANDIDATES_TIME = [(8, 2), (8, 4), (8, 8), (4, 4), (4, 8), (2, 8), (1, 8), (8, 16)]
+DEFAULT_CANDIDATES_TIME = [(0, 8), (0, 16), (0, 32), (0, 64), (8, 2), (8, 4), (8, 8), (4, 4),
+                           (4, 8), (2, 8), (1, 8), (8, 16)]
 DEFAULT_CANDIDATES_PLAIN = [(4, 0), (8, 0), (16, 0)]
 CACHE_PATH = os.environ.get("BILBY_AMD_AUTOTUNE_CACHE",
                             os.path.join(os.path.expanduser("~"), ".cache", "bilby_amd",

@@ -398,12 +398,37 @@ int gwl_event_create(gwl_ctx* ctx, const gwl_event_desc* d, gwl_event** out, int
       ev.wN[k] = sg2 / s;
     }
   }
+  // TaylorF2 with a 512-point time transform: tables of the wave-per-row
+  // kernel, including prefix sums of f^-7/3 sigma^2 / S over bins 0..N
+  const bool wave_tables = ev.approx == GWL_TAYLORF2 && (ev.flags & GWL_TIME_MARG) && ev.P == 512;
+  std::vector<double> hps;
+  if (wave_tables) {
+    hps.assign((size_t)nd * (N + 2), 0.0);
+    for (int k = 0; k < nd; ++k) {
+      double* ps = hps.data() + (size_t)k * (N + 2);
+      for (int n = 0; n <= N; ++n) {
+        double term = 0.0;
+        if (n > 0 && n >= ev.mlo[k] && n <= ev.mhi[k]) {
+          const double f = (double)n * ev.df;
+          const double w = (n == N) ? ev.wN[k]
+                                    : w64[(size_t)k * N + (size_t)(n % ev.Q) * ev.P + n / ev.Q];
+          const double a = 1.0 / std::cbrt(f);  // f^-1/3
+          const double a7 = a * a * a * a * a * a * a;
+          term = a7 * w;
+        }
+        ps[n + 1] = ps[n] + term;
+      }
+    }
+  }
   gwl_event* e = new (std::nothrow) gwl_event;
   if (!e) { set_error("out of host memory"); return GWL_E_NOMEM; }
   e->ctx = ctx;
   const size_t b32 = n_el * (sizeof(float2) + sizeof(float));
   const size_t b64 = n_el * (sizeof(double2) + sizeof(double));
-  e->data_bytes = b32 + b64 + 1024;
+  const size_t bw = wave_tables ? (size_t)N * 3 * sizeof(double) + n_el * sizeof(float2) +
+                                      hps.size() * sizeof(double) + 1024
+                                : 0;
+  e->data_bytes = b32 + b64 + bw + 1024;
   if (hipMalloc(&e->data, e->data_bytes) != hipSuccess) {
     delete e;
     set_error("hipMalloc failed for event data");
@@ -417,15 +442,34 @@ int gwl_event_create(gwl_ctx* ctx, const gwl_event_desc* d, gwl_event** out, int
   ev.dw32 = reinterpret_cast<float2*>(p);
   p += n_el * sizeof(float2);
   ev.w32 = reinterpret_cast<float*>(p);
+  p += n_el * sizeof(float);
+  if (wave_tables) {
+    p = reinterpret_cast<char*>((reinterpret_cast<uintptr_t>(p) + 255) & ~(uintptr_t)255);
+    ev.tf2b = reinterpret_cast<double*>(p);
+    p += (size_t)N * 3 * sizeof(double);
+    ev.dwa32 = reinterpret_cast<float2*>(p);
+    p += n_el * sizeof(float2);
+    ev.hps = reinterpret_cast<double*>(p);
+  }
   hipError_t err = hipSuccess;
   if (err == hipSuccess) err = hipMemcpy((void*)ev.dw64, dw64.data(), n_el * sizeof(double2), hipMemcpyHostToDevice);
   if (err == hipSuccess) err = hipMemcpy((void*)ev.w64, w64.data(), n_el * sizeof(double), hipMemcpyHostToDevice);
   if (err == hipSuccess) err = hipMemcpy((void*)ev.dw32, dw32.data(), n_el * sizeof(float2), hipMemcpyHostToDevice);
   if (err == hipSuccess) err = hipMemcpy((void*)ev.w32, w32.data(), n_el * sizeof(float), hipMemcpyHostToDevice);
+  if (err == hipSuccess && wave_tables)
+    err = hipMemcpy((void*)ev.hps, hps.data(), hps.size() * sizeof(double), hipMemcpyHostToDevice);
   if (err != hipSuccess) {
     (void)hipFree(e->data);
     delete e;
     return hip_fail(err, "gwl_event_create upload");
+  }
+  if (wave_tables) {
+    int rc = init_time_wave_tables(ev, const_cast<double*>(ev.tf2b), const_cast<float2*>(ev.dwa32));
+    if (rc != GWL_OK) {
+      (void)hipFree(e->data);
+      delete e;
+      return rc;
+    }
   }
   e->dev = ev;
   *out = e;

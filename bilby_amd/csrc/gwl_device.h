@@ -5,6 +5,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <cfloat>
 #include "../../include/gwl.h"
 
 namespace gwl {
@@ -70,6 +71,11 @@ struct EvDev {
   const float* w32;    // [nd][N]  sigma^2 / S
   const double2* dw64;
   const double* w64;
+  // TaylorF2 tables for the wave-per-row time transform (null otherwise):
+  const double* tf2b;   // [N][3] (f^1/3, f^-1/3, ln f), decimated order
+  const float2* dwa32;  // [nd][N] dw32 f^-7/6, decimated order
+  const double* hps;    // [nd][N + 2] prefix sums of f^-7/3 sigma^2 / S over bins
+                        //   0..N in natural order, hps[d][n] = sum of bins < n
   // Nyquist bin (n = N), kept apart from the power-of-two FFT range
   double dwN[GWL_MAX_DET][2];
   double wN[GWL_MAX_DET];
@@ -573,6 +579,88 @@ __device__ inline void fft_lds(C* buf, const C* tw, int B) {
     stockham_pass<4, NT, 2 * MAXBF8, P>(buf, tw, B, p);
   } else if (rem == 2) {
     stockham_pass<2, NT, 4 * MAXBF8, P>(buf, tw, B, p);
+  }
+}
+
+__device__ inline double row_valid_sentinel() { return -DBL_MAX; }  // base.py:424-425
+
+// Knots of the distance spline staged in LDS: tx[nx + 4] then ty[ny + 4].
+__device__ inline void stage_knots(const EvDev& ev, double* lds) {
+  const int nx4 = ev.sp.nx + 4, ny4 = ev.sp.ny + 4;
+  for (int i = threadIdx.x; i < nx4; i += blockDim.x) lds[i] = ev.sp.tx[i];
+  for (int i = threadIdx.x; i < ny4; i += blockDim.x) lds[nx4 + i] = ev.sp.ty[i];
+}
+
+// y-part of the bicubic interpolant (fixed per row): interval and basis.
+struct SplineY {
+  int ly;
+  double by[4];
+  bool inside;
+};
+
+__device__ inline SplineY spline_y(const EvDev& ev, const double* ty, double y) {
+  SplineY r;
+  r.inside = (y >= ev.sp.ymin && y <= ev.sp.ymax);
+  r.ly = 3;
+  r.by[0] = r.by[1] = r.by[2] = r.by[3] = 0.0;
+  if (r.inside) {
+    r.ly = knot_interval(ty, ev.sp.ny, y);
+    bspline3(ty, r.ly, y, r.by);
+  }
+  return r;
+}
+
+// BoundedRectBivariateSpline value at (x, y) with the y-part precomputed;
+// -inf outside the bounding box (calculus.py:252-260).
+__device__ inline double spline_x(const EvDev& ev, const double* tx, const SplineY& yy, double x) {
+  if (!yy.inside || !(x >= ev.sp.xmin && x <= ev.sp.xmax)) return -INFINITY;
+  const int lx = knot_interval(tx, ev.sp.nx, x);
+  double bx[4];
+  bspline3(tx, lx, x, bx);
+  const double* c = ev.sp.c + (size_t)(lx - 3) * ev.sp.ny + (yy.ly - 3);
+  double s = 0.0;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const double* ci = c + (size_t)i * ev.sp.ny;
+    s += bx[i] * (yy.by[0] * ci[0] + yy.by[1] * ci[1] + yy.by[2] * ci[2] + yy.by[3] * ci[3]);
+  }
+  return s;
+}
+
+// Marginalised lnL at one time (base.py:775-792): distance spline (y-part
+// precomputed per row), else ln I0 (phase) or the real part.
+__device__ inline double epi_value(const EvDev& ev, const double* knots, const SplineY& yy,
+                                   double hh, double hs, double xr, double xi) {
+  if (ev.flags & GWL_DISTANCE_MARG) {
+    const double d = (ev.flags & GWL_PHASE_MARG) ? hypot(xr, xi) * hs : xr * hs;
+    return spline_x(ev, knots, yy, d);
+  }
+  if (ev.flags & GWL_PHASE_MARG) return ln_i0(hypot(xr, xi)) - hh / 2.0;
+  return xr - hh / 2.0;
+}
+
+// Running logsumexp state (max m, sum s of exp(v - m)).
+__device__ __forceinline__ void lse_push(double& m, double& s, double v) {
+  if (v > m) {
+    s = (m > -INFINITY ? s * exp(m - v) : 0.0) + 1.0;
+    m = v;
+  } else if (v > -INFINITY) {
+    s += exp(v - m);
+  }
+}
+
+// Merge the logsumexp states of the 64 lanes of a wavefront.
+__device__ __forceinline__ void lse_wave(double& m, double& s) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    const double mo = __shfl_xor(m, off, 64), so = __shfl_xor(s, off, 64);
+    const double mn = fmax(m, mo);
+    if (mn > -INFINITY) {
+      s = (m > -INFINITY ? s * exp(m - mn) : 0.0) + (mo > -INFINITY ? so * exp(mo - mn) : 0.0);
+    } else {
+      s = 0.0;
+    }
+    m = mn;
   }
 }
 

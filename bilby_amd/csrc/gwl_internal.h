@@ -99,6 +99,17 @@ int timing_begin(gwl_event* ev, hipStream_t stream);
 int timing_end(gwl_event* ev, hipStream_t stream);
 int launch_response(gwl_event* ev, const double* params_dev, int det, int precision, double* out_dev,
                     hipStream_t stream);
+// float64 epilogue of the time-marginalised path over G window partials
+int launch_epi_time(gwl_event* ev, const double* params, int64_t n, int ld, int G,
+                    const double2* part, const double* hhpart, double* out, hipStream_t stream);
+// wave-per-row TaylorF2 time transform (gwl_time_wave.hip)
+bool time_wave_eligible(const gwl_event* ev);
+int time_wave_groups(const gwl_event* ev, long long n);
+size_t time_wave_ws_bytes(const gwl_event* ev, int64_t rows);
+size_t time_wave_lds_bytes(const EvDev& ev, bool fused);
+int init_time_wave_tables(const EvDev& ev, double* tf2b, float2* dwa32);
+int launch_time_wave(gwl_event* ev, const double* params, int64_t n, int ld, double* out,
+                     hipStream_t stream);
 }  // namespace gwl
 
 #define GWL_HIP_CHECK(expr)                                   \

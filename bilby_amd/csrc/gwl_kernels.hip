@@ -17,6 +17,7 @@
 // basis and the data are loaded once and reused by S parameter rows.
 #include "gwl_internal.h"
 #include <algorithm>
+#include <type_traits>
 #include <cfloat>
 
 namespace gwl {
@@ -449,51 +450,6 @@ struct EpiArgs {
   double* aux;
 };
 
-__device__ inline double row_valid_sentinel() { return -DBL_MAX; }  // base.py:424-425
-
-// Knots of the distance spline staged in LDS: tx[nx + 4] then ty[ny + 4].
-__device__ inline void stage_knots(const EvDev& ev, double* lds) {
-  const int nx4 = ev.sp.nx + 4, ny4 = ev.sp.ny + 4;
-  for (int i = threadIdx.x; i < nx4; i += blockDim.x) lds[i] = ev.sp.tx[i];
-  for (int i = threadIdx.x; i < ny4; i += blockDim.x) lds[nx4 + i] = ev.sp.ty[i];
-}
-
-// y-part of the bicubic interpolant (fixed per row): interval and basis.
-struct SplineY {
-  int ly;
-  double by[4];
-  bool inside;
-};
-
-__device__ inline SplineY spline_y(const EvDev& ev, const double* ty, double y) {
-  SplineY r;
-  r.inside = (y >= ev.sp.ymin && y <= ev.sp.ymax);
-  r.ly = 3;
-  r.by[0] = r.by[1] = r.by[2] = r.by[3] = 0.0;
-  if (r.inside) {
-    r.ly = knot_interval(ty, ev.sp.ny, y);
-    bspline3(ty, r.ly, y, r.by);
-  }
-  return r;
-}
-
-// BoundedRectBivariateSpline value at (x, y) with the y-part precomputed;
-// -inf outside the bounding box (calculus.py:252-260).
-__device__ inline double spline_x(const EvDev& ev, const double* tx, const SplineY& yy, double x) {
-  if (!yy.inside || !(x >= ev.sp.xmin && x <= ev.sp.xmax)) return -INFINITY;
-  const int lx = knot_interval(tx, ev.sp.nx, x);
-  double bx[4];
-  bspline3(tx, lx, x, bx);
-  const double* c = ev.sp.c + (size_t)(lx - 3) * ev.sp.ny + (yy.ly - 3);
-  double s = 0.0;
-#pragma unroll
-  for (int i = 0; i < 4; ++i) {
-    const double* ci = c + (size_t)i * ev.sp.ny;
-    s += bx[i] * (yy.by[0] * ci[0] + yy.by[1] * ci[1] + yy.by[2] * ci[2] + yy.by[3] * ci[3]);
-  }
-  return s;
-}
-
 // lnL from the summed inner products (base.py:448-477, 775-792); one thread
 // per row, knots in LDS.
 __global__ __launch_bounds__(128) void k_epi_plain(EpiArgs a) {
@@ -786,10 +742,32 @@ static int time_tile_groups(const EvDev& ev, long long n, int S, int n_cu, int w
   return G;
 }
 
+int launch_epi_time(gwl_event* e, const double* params, int64_t n, int ld, int G,
+                    const double2* part, const double* hhpart, double* out, hipStream_t st) {
+  const EvDev& ev = e->dev;
+  EpiArgs ea{};
+  ea.ev = ev;
+  ea.params = params;
+  ea.ld = ld;
+  ea.n = n;
+  ea.G = G;
+  ea.part = part;
+  ea.hhpart = hhpart;
+  ea.out = out;
+  const size_t klds = (ev.flags & GWL_DISTANCE_MARG) ? (size_t)(ev.sp.nx + ev.sp.ny + 8) * sizeof(double) : 0;
+  hipLaunchKernelGGL(k_epi_time, dim3((unsigned)((n + kEpiRows - 1) / kEpiRows)),
+                     dim3(64 * kEpiRows), klds, st, ea);
+  GWL_HIP_CHECK(hipGetLastError());
+  return GWL_OK;
+}
+
 template <typename R, int WAVE, int ND>
 static int eval_typed(gwl_event* e, const double* params, int64_t n, int ld, double* out,
                       double* aux, hipStream_t st) {
   const EvDev& ev = e->dev;
+  if constexpr (std::is_same<R, float>::value && WAVE == GWL_TAYLORF2) {
+    if (time_wave_eligible(e)) return launch_time_wave(e, params, n, ld, out, st);
+  }
   if (ev.flags & GWL_TIME_MARG) {
     // tile: S rows x P-point transforms, G groups of sub-sequences
     constexpr int NT = 512;
@@ -836,21 +814,7 @@ static int eval_typed(gwl_event* e, const double* params, int64_t n, int ld, dou
 #undef GWL_LAUNCH_TIME
     GWL_HIP_CHECK(hipGetLastError());
     if (int rc = timing_end(e, st)) return rc;
-    EpiArgs ea{};
-    ea.ev = ev;
-    ea.params = params;
-    ea.ld = ld;
-    ea.n = n;
-    ea.G = G;
-    ea.part = ta.part;
-    ea.hhpart = ta.hhpart;
-    ea.out = out;
-    ea.aux = aux;
-    const size_t klds = (ev.flags & GWL_DISTANCE_MARG) ? (size_t)(ev.sp.nx + ev.sp.ny + 8) * sizeof(double) : 0;
-    hipLaunchKernelGGL(k_epi_time, dim3((unsigned)((n + kEpiRows - 1) / kEpiRows)),
-                       dim3(64 * kEpiRows), klds, st, ea);
-    GWL_HIP_CHECK(hipGetLastError());
-    return GWL_OK;
+    return launch_epi_time(e, params, n, ld, G, ta.part, ta.hhpart, out, st);
   }
   constexpr int NT = 256;
   const int S = e->tile_rows >= 16 ? 16 : e->tile_rows >= 8 || e->tile_rows <= 0 ? 8 : 4;
@@ -915,7 +879,9 @@ size_t workspace_bytes(const gwl_event* e, int64_t rows) {
     const long long bound = std::max<long long>(
         (long long)rows * time_tile_groups(ev, rows, S, e->ctx->n_cu, e->tile_wg_per_cu),
         2LL * 8 * wpc * e->ctx->n_cu);
-    return (size_t)bound * (ev.L * sizeof(double2) + sizeof(double)) + 4096;
+    size_t b = (size_t)bound * (ev.L * sizeof(double2) + sizeof(double)) + 4096;
+    if (ev.approx == GWL_TAYLORF2 && ev.tf2b) b = std::max(b, time_wave_ws_bytes(e, rows));
+    return b;
   }
   const int nchunk = (ev.N + 1023) / 1024;
   return (size_t)rows * nchunk * 3 * sizeof(double) + 4096;

@@ -42,6 +42,7 @@ struct RoqDev {
   const __bf16* bth;  // [Np][Kp] high bfloat16 part
   const __bf16* btl;  // [Np][Kp] low bfloat16 part
   const double* bt64; // [Np][Kp] float64
+  const double* ts;   // [NT] ROQ time samples (s from start)
   SplineDev sp;
 };
 
@@ -384,9 +385,10 @@ __global__ __launch_bounds__(256) void k_gemm_f64(const double* __restrict__ A,
 
 // ------------------------------------------------------------------ epilogue
 // d_inner_h at time t from the five nearest ROQ time samples
-// (roq.py:576-602 / 636-651): a v0 + b v1 + c r1 + d r2.
+// (roq.py:576-602 / 636-651): a v0 + b v1 + c r1 + d r2, with the weight
+// of the sample after t, b = 1 - a, given by the caller.
 template <typename T>
-__device__ inline void five_sample(const T* z, int base_c, double x, double& re, double& im) {
+__device__ inline void five_sample(const T* z, int base_c, double a, double& re, double& im) {
   // z: complex row (interleaved), base_c = complex index of sample closest - 2
   double vr[5], vi[5];
 #pragma unroll
@@ -394,7 +396,7 @@ __device__ inline void five_sample(const T* z, int base_c, double x, double& re,
     vr[k] = (double)z[2 * (base_c + k)];
     vi[k] = (double)z[2 * (base_c + k) + 1];
   }
-  const double b = x, a = 1.0 - b;
+  const double b = 1.0 - a;
   const double c = (a * a * a - a) / 6.0, d = (b * b * b - b) / 6.0;
   const double r1r = (-vr[0] + 8.0 * vr[1] - 14.0 * vr[2] + 8.0 * vr[3] - vr[4]) / 4.0;
   const double r1i = (-vi[0] + 8.0 * vi[1] - 14.0 * vi[2] + 8.0 * vi[3] - vi[4]) / 4.0;
@@ -468,8 +470,10 @@ __global__ __launch_bounds__(64 * kRoqEpiRows) void k_roq_epi(RoqDev rd, const T
     return xr - hh / 2.0;
   };
   // interpolated, conj(C_d)-weighted detector sum at ifo time
-  // (toff + dt_d) + extra for each detector
-  auto dh_at = [&](double toff, double extra, bool& ok, double& sr, double& si) {
+  // (toff + dt_d) + extra for each detector.  One time (roq.py:512-526):
+  // a = (t_{c+1} - t) / (t_{c-1} - t_{c-2}) on the samples themselves; the
+  // time-marginalisation array (roq.py:636-651): b = (t - t_0) / dt - c.
+  auto dh_at = [&](double toff, double extra, bool single, bool& ok, double& sr, double& si) {
     sr = si = 0.0;
     ok = true;
     for (int d = 0; d < rd.nd; ++d) {
@@ -481,8 +485,15 @@ __global__ __launch_bounds__(64 * kRoqEpiRows) void k_roq_epi(RoqDev rd, const T
         ok = false;
         return;
       }
+      double a;
+      if (single) {
+        const int c = (int)ci;
+        a = (rd.ts[c + 1] - ifo_t) / fmax(rd.ts[c - 1] - rd.ts[c - 2], 1e-12);
+      } else {
+        a = 1.0 - (x - cl);
+      }
       double vr, vi;
-      five_sample(zr, d * rd.NT + (int)ci - 2, x - cl, vr, vi);
+      five_sample(zr, d * rd.NT + (int)ci - 2, a, vr, vi);
       // Y_d = conj(C_d) Z_d
       sr += rr.cre[d] * vr - rr.cim[d] * vi;
       si += rr.cre[d] * vi + rr.cim[d] * vr;
@@ -492,7 +503,7 @@ __global__ __launch_bounds__(64 * kRoqEpiRows) void k_roq_epi(RoqDev rd, const T
     if (lane == 0) {
       bool ok;
       double sr, si;
-      dh_at(rr.dtg, 0.0, ok, sr, si);
+      dh_at(rr.dtg, 0.0, true, ok, sr, si);
       // out-of-range time: log(in_bounds) = -inf (roq.py:529-532)
       out[r] = ok ? lnl_of(sr, si) : -INFINITY;
     }
@@ -506,7 +517,7 @@ __global__ __launch_bounds__(64 * kRoqEpiRows) void k_roq_epi(RoqDev rd, const T
     bool ok;
     double sr, si;
     // roq.py:537-540: ifo_times = (times - start) + dt + jitter
-    dh_at(tj - rd.start, rr.jitter, ok, sr, si);
+    dh_at(tj - rd.start, rr.jitter, false, ok, sr, si);
     double v = ok ? lnl_of(sr, si) : -INFINITY;
     if (v > m) {
       s = (m > -INFINITY ? s * exp(m - v) : 0.0) + 1.0;
@@ -542,6 +553,11 @@ int gwl_roq_create(gwl_ctx* ctx, const gwl_roq_desc* d, gwl_roq** out, int64_t* 
   if (d->n_det < 1 || d->n_det > GWL_MAX_DET || d->n_linear < 1 || d->n_quadratic < 1 ||
       d->n_time_samples < 5) {
     set_error("gwl_roq_create: invalid sizes");
+    return GWL_E_INVALID;
+  }
+  if (!d->linear_nodes || !d->quadratic_nodes || !d->linear_weights || !d->quadratic_weights ||
+      !d->detector_tensor || !d->vertex || !d->time_samples) {
+    set_error("gwl_roq_create: NULL array in the descriptor");
     return GWL_E_INVALID;
   }
   GWL_HIP_CHECK(hipSetDevice(ctx->device));
@@ -617,7 +633,8 @@ int gwl_roq_create(gwl_ctx* ctx, const gwl_roq_desc* d, gwl_roq** out, int64_t* 
   gwl_roq* e = new (std::nothrow) gwl_roq;
   if (!e) { set_error("out of host memory"); return GWL_E_NOMEM; }
   e->ctx = ctx;
-  const size_t bytes = nB * (8 + 4 + 2 + 2) + sizeof(double) * ((size_t)rd.L + rd.Qn + wq.size()) + 4096;
+  const size_t bytes =
+      nB * (8 + 4 + 2 + 2) + sizeof(double) * ((size_t)rd.L + rd.Qn + wq.size() + rd.NT) + 8 * 256;
   if (hipMalloc(&e->data, bytes) != hipSuccess) {
     delete e;
     set_error("hipMalloc failed for ROQ data");
@@ -637,6 +654,7 @@ int gwl_roq_create(gwl_ctx* ctx, const gwl_roq_desc* d, gwl_roq** out, int64_t* 
   rd.fl = reinterpret_cast<double*>(take(rd.L * 8));
   rd.fq = reinterpret_cast<double*>(take(rd.Qn * 8));
   rd.wq = reinterpret_cast<double*>(take(wq.size() * 8));
+  rd.ts = reinterpret_cast<double*>(take((size_t)rd.NT * 8));
   hipError_t err = hipMemcpy((void*)rd.bt64, b64.data(), nB * 8, hipMemcpyHostToDevice);
   if (err == hipSuccess) err = hipMemcpy((void*)rd.bt32, b32.data(), nB * 4, hipMemcpyHostToDevice);
   if (err == hipSuccess) err = hipMemcpy((void*)rd.bth, bh.data(), nB * 2, hipMemcpyHostToDevice);
@@ -644,6 +662,7 @@ int gwl_roq_create(gwl_ctx* ctx, const gwl_roq_desc* d, gwl_roq** out, int64_t* 
   if (err == hipSuccess) err = hipMemcpy((void*)rd.fl, d->linear_nodes, rd.L * 8, hipMemcpyHostToDevice);
   if (err == hipSuccess) err = hipMemcpy((void*)rd.fq, d->quadratic_nodes, rd.Qn * 8, hipMemcpyHostToDevice);
   if (err == hipSuccess) err = hipMemcpy((void*)rd.wq, wq.data(), wq.size() * 8, hipMemcpyHostToDevice);
+  if (err == hipSuccess) err = hipMemcpy((void*)rd.ts, d->time_samples, (size_t)rd.NT * 8, hipMemcpyHostToDevice);
   if (err != hipSuccess) {
     (void)hipFree(e->data);
     delete e;

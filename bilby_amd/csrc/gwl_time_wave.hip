@@ -1,0 +1,360 @@
+// Time-marginalised TaylorF2 likelihood with one wavefront per parameter row
+// (gfx950).
+//
+// The transform is the one k_time computes: d_inner_h_array = 4/T FFT_N of
+// h conj(d) / S at the time-prior window (bilby/gw/likelihood/base.py:325-330),
+// as a decimated two-stage DFT (element i of sub-sequence n2 is bin
+// n = Q i + n2; 512-point transforms of the Q sub-sequences, then
+// X[j] = sum_n2 W_N^(n2 j) Y_n2[j mod 512]).  It is organised for sources
+// whose support [kb, ke] is a small part of the band (TaylorF2 ends at the
+// ISCO frequency):
+//
+//  k_rowconst : per row, the source constants (phase coefficients, detector
+//               amplitudes and delays, support) and <h|h> from per-event prefix
+//               sums of f^-7/3 sigma^2 / S (gw/utils.py:118-138), one thread
+//               per row.
+//  k_time_w   : one wavefront per (row, group of sub-sequences).  For each
+//               sub-sequence the lanes holding in-support elements evaluate
+//               the integrand; the 512-point transform runs as three radix-8
+//               passes with the data in registers (lane l holds elements
+//               l + 64 s) and two LDS transposes private to the wavefront, so
+//               the sub-sequence loop has no workgroup barrier; each lane then
+//               accumulates the window bins j = l + 64 s (mod 512) in float64.
+//               With one group per row the marginalisation epilogue
+//               (base.py:775-820) runs in the same wavefront.
+#include "gwl_internal.h"
+
+#include <algorithm>
+
+namespace gwl {
+
+struct RowW {
+  double c[9];                 // phase coefficients (cycles), tf2_basis order
+  double dt[GWL_MAX_DET];      // detector time shifts [s]
+  float cre[GWL_MAX_DET];      // complex detector amplitude / sigma
+  float cim[GWL_MAX_DET];
+  double hh;                   // sum over the support of |h|^2 sigma^2 / S
+  double hscale, jitter;
+  int kb, ke, valid, pad;
+};
+
+constexpr int kWP = 512;      // transform length
+constexpr int kWaves = 4;     // wavefronts (rows) per workgroup
+constexpr int kXch = 576;     // per-wave exchange buffer, padded (float2)
+
+__device__ __forceinline__ int xpad(int e) { return e + (e >> 3); }
+
+// Orders the LDS accesses of one wavefront (the wave executes them in order;
+// this keeps the compiler from moving them across).
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+struct RowArgs {
+  EvDev ev;
+  const double* params;
+  int ld;
+  long long n;
+  RowW* rows;
+};
+
+__global__ __launch_bounds__(256) void k_rowconst(RowArgs a) {
+  const long long r = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (r >= a.n) return;
+  const EvDev& ev = a.ev;
+  SampleConst sc;
+  compute_sample_const(ev, a.params + r * a.ld, sc);
+  RowW w;
+#pragma unroll
+  for (int k = 0; k < 9; ++k) w.c[k] = sc.c[k];
+  double hh = 0.0;
+#pragma unroll
+  for (int d = 0; d < GWL_MAX_DET; ++d) {
+    const bool on = d < ev.nd;
+    w.dt[d] = on ? sc.dt[d] : 0.0;
+    w.cre[d] = on ? (float)sc.cre[d] : 0.f;
+    w.cim[d] = on ? (float)sc.cim[d] : 0.f;
+    if (on && sc.kb <= sc.ke) {
+      const double* ps = ev.hps + (size_t)d * (ev.N + 2);
+      hh += sc.cabs2[d] * (ps[sc.ke + 1] - ps[sc.kb]);
+    }
+  }
+  w.hh = hh;
+  w.hscale = sc.hscale;
+  w.jitter = sc.jitter;
+  w.kb = sc.kb;
+  w.ke = sc.ke;
+  w.valid = sc.valid;
+  w.pad = 0;
+  a.rows[r] = w;
+}
+
+// h conj(d) / S at decimated element i (bin nbin) for one row, summed over
+// detectors (interferometer.py:303-368, base.py:327-329): float64 phase,
+// reduced to [0, 1) cycles, float32 rotation and products.
+template <int ND>
+__device__ __forceinline__ float2 integrand_tf2(const EvDev& ev, const RowW& rw, int i, int nbin) {
+  const double f = (double)nbin * ev.df;
+  const double* tb = ev.tf2b + 3 * (size_t)i;
+  const double u = tb[0], v = tb[1], lf = tb[2];
+  const double v2 = v * v, v3 = v2 * v;
+  double s = rw.c[4];
+  s = fma(rw.c[0], v3 * v2, s);
+  s = fma(rw.c[1], v3, s);
+  s = fma(rw.c[2], v2, s);
+  s = fma(rw.c[3], v, s);
+  s = fma(rw.c[5], u, s);
+  s = fma(rw.c[6], u * u, s);
+  s = fma(rw.c[7], lf, s);
+  s = fma(rw.c[8], u * lf, s);
+  float ar = 0.f, ai = 0.f;
+#pragma unroll
+  for (int d = 0; d < ND; ++d) {
+    const double cyc = fma(f, rw.dt[d], s);
+    const float2 e = expm2pi((float)(cyc - floor(cyc)));
+    const float2 dw = ev.dwa32[(size_t)d * ev.N + i];
+    const float hr = rw.cre[d] * e.x - rw.cim[d] * e.y;
+    const float hi = rw.cre[d] * e.y + rw.cim[d] * e.x;
+    ar += hr * dw.x + hi * dw.y;
+    ai += hi * dw.x - hr * dw.y;
+  }
+  return make_float2(ar, ai);
+}
+
+struct TimeWArgs {
+  EvDev ev;
+  const RowW* rows;
+  long long n;
+  int G;             // groups of sub-sequences per row
+  int fused;         // G == 1: epilogue here
+  double2* part;     // [n][G][L] window partials (G > 1)
+  double* hhpart;    // [n][G]
+  double* out;
+};
+
+template <int ND>
+__global__ __launch_bounds__(64 * kWaves, 4) void k_time_w(TimeWArgs a) {
+  extern __shared__ double lds_d[];
+  const EvDev& ev = a.ev;
+  float2* tw = reinterpret_cast<float2*>(lds_d);                  // [512]
+  float2* xch = tw + kWP;                                          // [kWaves][kXch]
+  double* knots = reinterpret_cast<double*>(xch + kWaves * kXch);  // tx, ty
+  const bool dist = (ev.flags & GWL_DISTANCE_MARG) != 0;
+  init_twiddles<kWP, 64 * kWaves>(tw);
+  if (a.fused && dist) stage_knots(ev, knots);
+  __syncthreads();
+  const int lane = threadIdx.x & 63;
+  const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const long long item = (long long)blockIdx.x * kWaves + wid;
+  const long long r = item / a.G;
+  const int g = (int)(item - r * a.G);
+  if (r >= a.n) return;
+  const RowW& rw = a.rows[r];
+  const int Q = ev.Q;
+  const int n2b = (int)(((long long)g * Q) / a.G);
+  const int n2e = (int)(((long long)(g + 1) * Q) / a.G);
+  // per-lane twiddles of pass 3 (span 64: W_512^(s l)); those of pass 2
+  // (span 8: W_64^(s k), k = l mod 8, eight distinct per s) are read from LDS
+  float2 t3[7];
+#pragma unroll
+  for (int s = 1; s < 8; ++s) t3[s - 1] = tw[(s * lane) & (kWP - 1)];
+  const int k2 = 8 * (lane & 7);
+  double accr[8], acci[8];
+#pragma unroll
+  for (int s = 0; s < 8; ++s) accr[s] = acci[s] = 0.0;
+  const unsigned nmask = (unsigned)ev.N - 1u;
+  const float invN = 1.0f / (float)ev.N;
+  const int kb = rw.kb, ke = rw.ke;
+  float2* xb = xch + wid * kXch;
+  const int base2 = 64 * (lane >> 3) + (lane & 7);
+
+  for (int n2 = n2b; n2 < n2e; ++n2) {
+    // in-support elements of this sub-sequence: kb <= Q i + n2 <= ke
+    const int ilo = (kb - n2 + Q - 1) / Q;
+    const int ihi = ke >= n2 ? (ke - n2) / Q : -1;
+    float2 x[8];
+#pragma unroll
+    for (int s = 0; s < 8; ++s) {
+      x[s] = make_float2(0.f, 0.f);
+      if (ilo <= ihi && ilo <= 64 * s + 63 && ihi >= 64 * s) {
+        const int i = lane + 64 * s;
+        if (i >= ilo && i <= ihi) x[s] = integrand_tf2<ND>(ev, rw, n2 * kWP + i, Q * i + n2);
+      }
+    }
+    // pass 1 (span 1): inputs l + 64 s, outputs to 8 l + s
+    dft8(x);
+#pragma unroll
+    for (int s = 0; s < 8; ++s) xb[xpad(8 * lane + s)] = x[s];
+    wave_sync();
+    // pass 2 (span 8): inputs l + 64 s, outputs to 64 (l / 8) + l mod 8 + 8 s
+#pragma unroll
+    for (int s = 0; s < 8; ++s) x[s] = xb[xpad(lane + 64 * s)];
+#pragma unroll
+    for (int s = 1; s < 8; ++s) x[s] = cmul(x[s], tw[(s * k2) & (kWP - 1)]);
+    dft8(x);
+    wave_sync();
+#pragma unroll
+    for (int s = 0; s < 8; ++s) xb[xpad(base2 + 8 * s)] = x[s];
+    wave_sync();
+    // pass 3 (span 64): inputs l + 64 s, outputs Y[l + 64 s] stay in x[s]
+#pragma unroll
+    for (int s = 0; s < 8; ++s) x[s] = xb[xpad(lane + 64 * s)];
+#pragma unroll
+    for (int s = 1; s < 8; ++s) x[s] = cmul(x[s], t3[s - 1]);
+    dft8(x);
+    wave_sync();
+    // window bins j = jlo + jj with j = l + 64 s (mod 512)
+#pragma unroll
+    for (int s = 0; s < 8; ++s) {
+      const int jj = (lane + 64 * s - ev.jlo) & (kWP - 1);
+      if (jj < ev.L) {
+        const unsigned j = (unsigned)(ev.jlo + jj);
+        const unsigned idx = ((unsigned)n2 * j) & nmask;
+        const float2 t = expm2pi((float)idx * invN);
+        const float2 z = cmul(t, x[s]);
+        accr[s] += (double)z.x;
+        acci[s] += (double)z.y;
+      }
+    }
+  }
+
+  if (!a.fused) {
+    double2* dst = a.part + ((size_t)r * a.G + g) * ev.L;
+#pragma unroll
+    for (int s = 0; s < 8; ++s) {
+      const int jj = (lane + 64 * s - ev.jlo) & (kWP - 1);
+      if (jj < ev.L) dst[jj] = make_double2(accr[s], acci[s]);
+    }
+    if (lane == 0) a.hhpart[(size_t)r * a.G + g] = (g == 0) ? rw.hh : 0.0;
+    return;
+  }
+  // epilogue: logsumexp over the prior window (base.py:794-820)
+  if (!rw.valid) {
+    if (lane == 0) a.out[r] = row_valid_sentinel();
+    return;
+  }
+  const double norm = 4.0 / ev.T;
+  const double hh = rw.hh * norm, hs = rw.hscale, jit = rw.jitter;
+  SplineY yy;
+  if (dist) yy = spline_y(ev, knots + ev.sp.nx + 4, hh * hs * hs);
+  double m = -INFINITY, sum = 0.0;
+#pragma unroll
+  for (int s = 0; s < 8; ++s) {
+    const int jj = (lane + 64 * s - ev.jlo) & (kWP - 1);
+    if (jj < ev.L) {
+      const int j = ev.jlo + jj;
+      const double t = (ev.start + (double)(j + 1) * ev.tstep) + jit;
+      if (t >= ev.tmin && t <= ev.tmax)
+        lse_push(m, sum, epi_value(ev, knots, yy, hh, hs, norm * accr[s], norm * acci[s]));
+    }
+  }
+  lse_wave(m, sum);
+  if (lane == 0) {
+    const double b = (1.0 / (ev.tmax - ev.tmin)) * ev.tdelta;
+    a.out[r] = (m > -INFINITY) ? log(b * sum) + m : -INFINITY;
+  }
+}
+
+// f^1/3, f^-1/3, ln f and dw32 f^-7/6 at every decimated element.
+__global__ __launch_bounds__(256) void k_tf2_tables(EvDev ev, double* tb, float2* dwa) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= ev.N) return;
+  const int n = ev.Q * (i & (ev.P - 1)) + (i >> ev.logP);
+  double u = 0.0, v = 0.0, lf = 0.0, amp = 0.0;
+  if (n > 0) {
+    double b[kNB];
+    tf2_basis((double)n * ev.df, b);
+    u = b[5];
+    v = b[3];
+    lf = b[7];
+    amp = b[1] * sqrt(b[3]);
+  }
+  tb[3 * (size_t)i + 0] = u;
+  tb[3 * (size_t)i + 1] = v;
+  tb[3 * (size_t)i + 2] = lf;
+  for (int d = 0; d < ev.nd; ++d) {
+    const double2 w = ev.dw64[(size_t)d * ev.N + i];
+    dwa[(size_t)d * ev.N + i] = make_float2((float)(w.x * amp), (float)(w.y * amp));
+  }
+}
+
+bool time_wave_eligible(const gwl_event* e) {
+  const EvDev& ev = e->dev;
+  return (ev.flags & GWL_TIME_MARG) && ev.approx == GWL_TAYLORF2 && ev.P == kWP && ev.tf2b &&
+         e->tile_rows == 0;
+}
+
+int time_wave_groups(const gwl_event* e, long long n) {
+  const int Q = e->dev.Q;
+  const long long target = (long long)e->ctx->n_cu * (e->tile_wg_per_cu > 0 ? e->tile_wg_per_cu : 16);
+  int G = 1;
+  while (G < Q && G < 64 && n * G < target) G *= 2;
+  return G;
+}
+
+size_t time_wave_ws_bytes(const gwl_event* e, int64_t rows) {
+  const int wpc = e->tile_wg_per_cu > 0 ? e->tile_wg_per_cu : 16;
+  const long long items = std::max<long long>(rows, 2LL * wpc * e->ctx->n_cu);
+  return (size_t)items * (e->dev.L * sizeof(double2) + sizeof(double)) +
+         (size_t)rows * sizeof(RowW) + 1024;
+}
+
+int init_time_wave_tables(const EvDev& ev, double* tb, float2* dwa) {
+  hipLaunchKernelGGL(k_tf2_tables, dim3((unsigned)((ev.N + 255) / 256)), dim3(256), 0, 0, ev, tb, dwa);
+  GWL_HIP_CHECK(hipGetLastError());
+  GWL_HIP_CHECK(hipDeviceSynchronize());
+  return GWL_OK;
+}
+
+size_t time_wave_lds_bytes(const EvDev& ev, bool fused) {
+  size_t b = (size_t)(kWP + kWaves * kXch) * sizeof(float2);
+  if (fused && (ev.flags & GWL_DISTANCE_MARG)) b += (size_t)(ev.sp.nx + ev.sp.ny + 8) * sizeof(double);
+  return b;
+}
+
+int launch_time_wave(gwl_event* e, const double* params, int64_t n, int ld, double* out,
+                     hipStream_t st) {
+  const EvDev& ev = e->dev;
+  const int G = time_wave_groups(e, n);
+  const size_t part_bytes = (size_t)n * G * (ev.L * sizeof(double2) + sizeof(double));
+  char* ws = reinterpret_cast<char*>(e->ws);
+  RowW* rows = reinterpret_cast<RowW*>(ws + (part_bytes + 255) / 256 * 256);
+  if ((char*)(rows + n) > ws + e->ws_bytes) {
+    set_error("workspace too small for the time transform");
+    return GWL_E_INVALID;
+  }
+  RowArgs ra{ev, params, ld, n, rows};
+  hipLaunchKernelGGL(k_rowconst, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, ra);
+  GWL_HIP_CHECK(hipGetLastError());
+  TimeWArgs ta{};
+  ta.ev = ev;
+  ta.rows = rows;
+  ta.n = n;
+  ta.G = G;
+  ta.fused = G == 1;
+  ta.part = reinterpret_cast<double2*>(ws);
+  ta.hhpart = reinterpret_cast<double*>(ws + (size_t)n * G * ev.L * sizeof(double2));
+  ta.out = out;
+  const size_t lds = time_wave_lds_bytes(ev, ta.fused);
+  if (lds > e->ctx->lds_per_block) {
+    set_error("time transform exceeds the LDS of a workgroup");
+    return GWL_E_UNSUPPORTED;
+  }
+  const long long items = n * (long long)G;
+  const dim3 grid((unsigned)((items + kWaves - 1) / kWaves));
+  if (int rc = timing_begin(e, st)) return rc;
+  switch (ev.nd) {
+    case 1: hipLaunchKernelGGL(k_time_w<1>, grid, dim3(64 * kWaves), lds, st, ta); break;
+    case 2: hipLaunchKernelGGL(k_time_w<2>, grid, dim3(64 * kWaves), lds, st, ta); break;
+    case 3: hipLaunchKernelGGL(k_time_w<3>, grid, dim3(64 * kWaves), lds, st, ta); break;
+    default: hipLaunchKernelGGL(k_time_w<4>, grid, dim3(64 * kWaves), lds, st, ta); break;
+  }
+  GWL_HIP_CHECK(hipGetLastError());
+  if (int rc = timing_end(e, st)) return rc;
+  if (!ta.fused) return launch_epi_time(e, params, n, ld, G, ta.part, ta.hhpart, out, st);
+  return GWL_OK;
+}
+
+}  // namespace gwl

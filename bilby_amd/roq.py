@@ -113,6 +113,8 @@ class ROQGravitationalWaveTransient:
         d.start_time = float(ifos.start_time)
         d.time_samples_start = float(ts[0])
         d.time_samples_step = float(ts[1] - ts[0])
+        self._ts = np.ascontiguousarray(ts)
+        d.time_samples = dptr(self._ts)
         d.linear_nodes = dptr(fl)
         d.quadratic_nodes = dptr(fq)
         d.linear_weights = dptr(lin_f)

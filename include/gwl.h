@@ -136,8 +136,11 @@ int gwl_eval(gwl_event* ev, const double* params, int64_t n_rows, int32_t ld,
 /* Tile selection for this event's kernels (0 = built-in heuristic):
  * rows_per_workgroup = parameter rows sharing one workgroup's bin loop
  * (1, 2, 4, 8 or 16), workgroups_per_cu = target resident workgroups per CU
- * used to split the time-marginalised transform into groups.  Used by the
- * autotuner; no counterpart in the reference. */
+ * used to split the time-marginalised transform into groups.  With
+ * rows_per_workgroup = 0 a TaylorF2 event whose time window fits a 512-point
+ * transform uses the wave-per-row transform, and workgroups_per_cu is the
+ * target wavefronts per CU (default 16).  Used by the autotuner; no
+ * counterpart in the reference. */
 int gwl_event_set_tile(gwl_event* ev, int32_t rows_per_workgroup, int32_t workgroups_per_cu);
 
 /* Kernel timing for measurement: while enabled, every gwl_eval on this event
@@ -247,6 +250,8 @@ typedef struct {
   double time_prior_minimum, time_prior_maximum;
   double delta_tc;               /* requested time-grid spacing (roq.py:320-331) */
   double reference_distance;
+  const double* time_samples;    /* host [n_time_samples]: weights['time_samples'] itself, read
+                                    by the single-time interpolation (roq.py:576-602) */
 } gwl_roq_desc;
 
 /* ROQGravitationalWaveTransient.__init__ (roq.py:99-300): weights resident
