@@ -58,6 +58,78 @@ def cpu_baseline(setup, priors_fixed, seconds):
                             "numpy, not lalsimulation" % (n, dt))
 
 
+_POOL_LK = None
+
+
+def _pool_init(blob):
+    global _POOL_LK
+    import pickle
+    _POOL_LK = pickle.loads(blob)
+
+
+def _pool_run(job):
+    dicts, seconds = job
+    _POOL_LK.log_likelihood_ratio(dicts[0])
+    n = 0
+    t0 = time.perf_counter()
+    while time.perf_counter() - t0 < seconds:
+        _POOL_LK.log_likelihood_ratio(dicts[n % len(dicts)])
+        n += 1
+    return n, time.perf_counter() - t0
+
+
+def measured_traffic(key):
+    """HBM bytes per launch of the kernel named ``key``, as recorded from the
+    FETCH_SIZE / WRITE_SIZE counter runs in a summary under profiles/ (a line
+    'bench-traffic <key>: <bytes>'); None if no summary records it."""
+    import glob
+    for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "*.md"))):
+        with open(path) as fh:
+            for line in fh:
+                line = line.strip().strip("`")
+                if line.startswith("bench-traffic %s:" % key):
+                    return int(float(line.split(":", 1)[1]))
+    return None
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline_pool(olk, dicts, seconds, workers):
+    """The same oracle over a process pool, as bilby's npool does
+    (bilby/core/sampler/base_sampler.py:772-800): every worker evaluates one
+    dict per call on one thread for ``seconds``; the rate is the sum."""
+    import multiprocessing
+    import pickle
+    blob = pickle.dumps(olk)
+    old = os.environ.get("OMP_NUM_THREADS")
+    os.environ["OMP_NUM_THREADS"] = "1"
+    try:
+        ctx = multiprocessing.get_context("spawn")
+        with ctx.Pool(workers, initializer=_pool_init, initargs=(blob,)) as pool:
+            res = pool.map(_pool_run, [(dicts[k::workers], seconds) for k in range(workers)],
+                           chunksize=1)
+    finally:
+        if old is None:
+            os.environ.pop("OMP_NUM_THREADS", None)
+        else:
+            os.environ["OMP_NUM_THREADS"] = old
+    n = sum(r[0] for r in res)
+    rate = sum(r[0] / r[1] for r in res)
+    return dict(value=rate, unit="lnL evals/s", cores=workers, kind="port",
+                cpu=cpu_model(), host_cpus=os.cpu_count(),
+                sample="%d single-dict oracle calls over %d worker processes, %.0f s each, same "
+                       "workload as cpu_baseline" % (n, workers, seconds))
+
+
 def timed(fn, steps, warmup=2):
     import torch
     for _ in range(warmup):
@@ -289,11 +361,16 @@ def main():
     ap.add_argument("--per-sample-calls", type=int, default=300)
     ap.add_argument("--audit", type=int, default=64, help="rows checked against the FP64 oracle")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-pool-seconds", type=float, default=8.0,
+                    help="seconds of the process-pool CPU baseline (0 = skip)")
     ap.add_argument("--no-extra", action="store_true", help="headline workload only")
     ap.add_argument("--serve-seconds", type=float, default=5.0)
     ap.add_argument("--profile", action="store_true",
-                    help="counter runs: --steps launches of the headline batch at tile (8, 2), "
+                    help="counter runs: warmup + steps launches of the headline batch at --tile, "
                          "then 3 copies of 512 MiB (a plain copy kernel of known size)")
+    ap.add_argument("--tile", default="0,0",
+                    help="rows per workgroup, workgroups (or wavefronts) per CU for --profile; "
+                         "0,0 = the default kernel choice")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only; --gpus %d is not supported in this build "
@@ -321,7 +398,7 @@ def main():
     check(LIB.gwl_event_reserve(lk._event, args.batch))
     from bilby_amd.autotune import autotune, set_tile
     if args.profile:
-        set_tile(lk, 8, 2)
+        set_tile(lk, *[int(v) for v in args.tile.split(",")])
         for _ in range(args.warmup + args.steps):
             lk.log_likelihood_ratio_batch(rows)
         torch.cuda.synchronize()
@@ -373,7 +450,12 @@ def main():
     k_ms = tot.value / max(1, cnt.value)
     bpe = bytes_per_eval(lk)
     achieved = args.batch * bpe / (k_ms * 1e-3) / 1e9
-    log("k_time: %.3f ms/launch, %.1f GB/s algorithmic" % (k_ms, achieved))
+    wave = tile[0] == 0
+    kname = "k_time_w" if wave else "k_time"
+    traffic_key = ("k_time_w" if wave else "k_time_tiled") + "_taylorf2_tdp_batch%d" % args.batch
+    traffic = measured_traffic(traffic_key)
+    log("%s: %.3f ms/launch, %.1f GB/s algorithmic, measured traffic %s B/launch"
+        % (kname, k_ms, achieved, traffic))
 
     # per-sample float64 GPU path (one row per launch), device-resident row
     set_tile(lk, 0, 0)
@@ -415,9 +497,13 @@ def main():
                                "time+distance+phase marginalised, batch of prior draws",
                    "batch": args.batch, "n_freq": 16385, "detectors": 3},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": 8000.0, "unit": "GB/s",
-                     "frac": achieved / 8000.0, "traffic": None,
-                     "kernel": "k_time (fused waveform + projection + decimated LDS FFT)",
-                     "kernel_ms": k_ms, "bytes_per_eval": bpe},
+                     "frac": achieved / 8000.0, "traffic": traffic,
+                     "kernel": (kname + (" (wave per row: waveform + projection + register "
+                                         "radix-8 transform + window + fused epilogue)" if wave
+                                         else " (tiled: waveform + projection + LDS FFT)")),
+                     "kernel_ms": k_ms, "bytes_per_eval": bpe,
+                     "traffic_source": "rocprofv3 FETCH_SIZE x2.0 + WRITE_SIZE x1.0 per launch, "
+                                       "profiles/ (bench-traffic %s)" % traffic_key},
         "autotune": {"chosen": list(tile), "ms_by_tile": sweep},
         "accuracy": {"fp32_vs_fp64_gpu_max": float(d_all.max()),
                      "fp32_vs_fp64_gpu_p99": float(np.percentile(d_all, 99))},
@@ -444,7 +530,16 @@ def main():
     if not args.no_cpu_baseline:
         log("cpu baseline (oracle, 1 core, %.0f s)" % args.cpu_seconds)
         olk, cb = cpu_baseline(setup, priors, args.cpu_seconds)
+        cb["cpu"] = cpu_model()
         result["cpu_baseline"] = cb
+        if args.cpu_pool_seconds > 0:
+            workers = max(1, min(16, os.cpu_count() or 1))
+            log("cpu baseline (oracle, %d processes, %.0f s)" % (workers, args.cpu_pool_seconds))
+            from tests.helpers import fixed_marginalised
+            pdicts = draws_as_dicts(fixed_marginalised(setup["draws"], priors), range(512))
+            result["cpu_baseline_pool"] = cpu_baseline_pool(olk, pdicts, args.cpu_pool_seconds,
+                                                            workers)
+            log("cpu pool: %.1f evals/s" % result["cpu_baseline_pool"]["value"])
         # audit sample vs the FP64 oracle
         n_a = min(args.audit, args.batch)
         ref = np.array([olk.log_likelihood_ratio(p) for p in draws_as_dicts(draws, range(n_a))])

@@ -183,8 +183,25 @@ __global__ __launch_bounds__(64 * kWaves, 4) void k_time_w(TimeWArgs a) {
         if (i >= ilo && i <= ihi) x[s] = integrand_tf2<ND>(ev, rw, n2 * kWP + i, Q * i + n2);
       }
     }
-    // pass 1 (span 1): inputs l + 64 s, outputs to 8 l + s
-    dft8(x);
+    // pass 1 (span 1): inputs l + 64 s, outputs to 8 l + s.  When the
+    // support ends below element 128 (uniform) only x[0], x[1] can be
+    // nonzero and the butterfly is x0 + W_8^s x1 (the same arithmetic dft8
+    // performs on those inputs).
+    if (ilo <= ihi && ihi >= 128) {
+      dft8(x);
+    } else {
+      const float2 x0 = x[0], a = x[1];
+      const float c = 0.70710678118654752440f;
+      const float p = (a.x + a.y) * c, q = (a.y - a.x) * c;
+      x[0] = make_float2(x0.x + a.x, x0.y + a.y);
+      x[4] = make_float2(x0.x - a.x, x0.y - a.y);
+      x[1] = make_float2(x0.x + p, x0.y + q);
+      x[5] = make_float2(x0.x - p, x0.y - q);
+      x[2] = make_float2(x0.x + a.y, x0.y - a.x);
+      x[6] = make_float2(x0.x - a.y, x0.y + a.x);
+      x[3] = make_float2(x0.x + q, x0.y - p);
+      x[7] = make_float2(x0.x - q, x0.y + p);
+    }
 #pragma unroll
     for (int s = 0; s < 8; ++s) xb[xpad(8 * lane + s)] = x[s];
     wave_sync();
