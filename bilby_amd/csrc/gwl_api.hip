@@ -693,6 +693,10 @@ int gwl_multi_create(gwl_ctx* ctx, int32_t n_events, gwl_event* const* events,
   m->ctx = ctx;
   m->n_events = n_events;
   m->wave = events[0]->dev.approx;
+  m->wave_path = true;
+  for (int e = 0; e < n_events; ++e)
+    if (!events[e] || events[e]->dev.approx != GWL_TAYLORF2 || !events[e]->dev.tf2b)
+      m->wave_path = false;
   size_t items = 0, blocks = 0;
   for (int e = 0; e < n_events; ++e) {
     const gwl_event* ev = events[e];
@@ -714,7 +718,7 @@ int gwl_multi_create(gwl_ctx* ctx, int32_t n_events, gwl_event* const* events,
     int G = (Q + 31) / 32;  // ~32 sub-sequence steps per item
     G = std::max(1, std::min(G, std::min(Q, 16)));
     m->groups.push_back(G);
-    items += (size_t)((max_rows[e] + 7) / 8) * std::min(Q, 16);
+    items += (size_t)((max_rows[e] + 3) / 4) * std::min(Q, 16);
     blocks += (size_t)((max_rows[e] + 3) / 4);
   }
   m->max_items = std::max<size_t>(items, 1);
@@ -736,9 +740,18 @@ int gwl_multi_create(gwl_ctx* ctx, int32_t n_events, gwl_event* const* events,
     }
     m->parts.push_back(part);
     m->hhs.push_back(hh);
+    if (m->wave_path) {
+      void* rw = nullptr;
+      if (hipMalloc(&rw, roww_bytes(rows)) != hipSuccess) {
+        gwl_multi_destroy(m);
+        set_error("hipMalloc failed for multi-event row constants");
+        return GWL_E_NOMEM;
+      }
+      m->roww.push_back(rw);
+    }
   }
   const size_t ne = (size_t)n_events;
-  const size_t bytes = ne * sizeof(EvDev) + 5 * ne * sizeof(void*) + ne * sizeof(long long) +
+  const size_t bytes = ne * sizeof(EvDev) + 6 * ne * sizeof(void*) + ne * sizeof(long long) +
                        ne * sizeof(int) + m->max_items * sizeof(MultiItem) +
                        m->max_blocks * (sizeof(int) + sizeof(long long)) + 64 * 8 + 16 * 256;
   if (hipMalloc(&m->dev, bytes) != hipSuccess) {
@@ -758,6 +771,7 @@ int gwl_multi_create(gwl_ctx* ctx, int32_t n_events, gwl_event* const* events,
   m->d_outs = reinterpret_cast<double**>(take(ne * sizeof(void*)));
   m->d_parts = reinterpret_cast<void**>(take(ne * sizeof(void*)));
   m->d_hhs = reinterpret_cast<double**>(take(ne * sizeof(void*)));
+  m->d_roww = reinterpret_cast<void**>(take(ne * sizeof(void*)));
   m->d_rows = reinterpret_cast<long long*>(take(ne * sizeof(long long)));
   m->d_groups = reinterpret_cast<int*>(take(ne * sizeof(int)));
   m->d_items = reinterpret_cast<MultiItem*>(take(m->max_items * sizeof(MultiItem)));
@@ -774,6 +788,8 @@ int gwl_multi_create(gwl_ctx* ctx, int32_t n_events, gwl_event* const* events,
   if (err == hipSuccess) err = hipMemcpy(m->d_parts, m->parts.data(), ne * sizeof(void*), hipMemcpyHostToDevice);
   if (err == hipSuccess) err = hipMemcpy(m->d_hhs, m->hhs.data(), ne * sizeof(void*), hipMemcpyHostToDevice);
   if (err == hipSuccess) err = hipMemcpy(m->d_groups, m->groups.data(), ne * sizeof(int), hipMemcpyHostToDevice);
+  if (err == hipSuccess && m->wave_path)
+    err = hipMemcpy(m->d_roww, m->roww.data(), ne * sizeof(void*), hipMemcpyHostToDevice);
   if (err != hipSuccess) {
     gwl_multi_destroy(m);
     return hip_fail(err, "gwl_multi_create upload");
@@ -784,7 +800,7 @@ int gwl_multi_create(gwl_ctx* ctx, int32_t n_events, gwl_event* const* events,
     gwl_multi_destroy(m);
     return rc;
   }
-  if (multi_lds_bytes() > ctx->lds_per_block) {
+  if ((m->wave_path ? multi_wave_lds_bytes() : multi_lds_bytes()) > ctx->lds_per_block) {
     gwl_multi_destroy(m);
     set_error("multi-event tile exceeds the LDS of a workgroup");
     return GWL_E_UNSUPPORTED;
@@ -867,6 +883,8 @@ int gwl_multi_destroy(gwl_multi* m) {
   for (void* p : m->parts)
     if (p) (void)hipFree(p);
   for (double* p : m->hhs)
+    if (p) (void)hipFree(p);
+  for (void* p : m->roww)
     if (p) (void)hipFree(p);
   if (m->dev) (void)hipFree(m->dev);
   delete m;

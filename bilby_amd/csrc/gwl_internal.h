@@ -84,10 +84,16 @@ struct gwl_multi {
   bool captured = false;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
+  // TaylorF2 events with 512-point transforms: wave-per-row kernels, row
+  // constants [max_rows] per event
+  bool wave_path = false;
+  std::vector<void*> roww;
+  void** d_roww = nullptr;
 };
 
 namespace gwl {
 int multi_launch_raw(gwl_multi* m, hipStream_t stream);
+int multi_launch_epi(gwl_multi* m, hipStream_t stream);
 int multi_set_rows(gwl_multi* m, const int64_t* rows, hipStream_t stream);
 size_t multi_lds_bytes();
 void set_error(const std::string& msg);
@@ -110,6 +116,9 @@ size_t time_wave_lds_bytes(const EvDev& ev, bool fused);
 int init_time_wave_tables(const EvDev& ev, double* tf2b, float2* dwa32);
 int launch_time_wave(gwl_event* ev, const double* params, int64_t n, int ld, double* out,
                      hipStream_t stream);
+size_t roww_bytes(int64_t rows);
+size_t multi_wave_lds_bytes();
+int multi_launch_wave(gwl_multi* m, hipStream_t stream);
 }  // namespace gwl
 
 #define GWL_HIP_CHECK(expr)                                   \

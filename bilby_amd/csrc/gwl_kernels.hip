@@ -919,7 +919,8 @@ int multi_set_rows(gwl_multi* m, const int64_t* rows, hipStream_t st) {
     const EvDev& ev = m->events[e]->dev;
     const int Q = ev.N / kMultiP;
     const int G = m->groups[e];
-    for (long long r0 = 0; r0 < rows[e]; r0 += kMultiS) {
+    const int tile = m->wave_path ? 4 : kMultiS;  // rows per item
+    for (long long r0 = 0; r0 < rows[e]; r0 += tile) {
       for (int g = 0; g < G; ++g) {
         MultiItem it;
         it.e = e;
@@ -963,6 +964,10 @@ int multi_set_rows(gwl_multi* m, const int64_t* rows, hipStream_t st) {
 // zero the queue counter (a kernel, not a memset node), persistent pass,
 // epilogue over every block the work list can hold.
 int multi_launch_raw(gwl_multi* m, hipStream_t st) {
+  if (m->wave_path) {
+    if (int rc = multi_launch_wave(m, st)) return rc;
+    return multi_launch_epi(m, st);
+  }
   hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(1), 0, st, m->d_counter);
   GWL_HIP_CHECK(hipGetLastError());
   MultiArgs a;
@@ -982,6 +987,11 @@ int multi_launch_raw(gwl_multi* m, hipStream_t st) {
   else
     hipLaunchKernelGGL(k_time_multi<GWL_SINE_GAUSSIAN>, grid, dim3(kMultiNT), multi_lds_bytes(), st, a);
   GWL_HIP_CHECK(hipGetLastError());
+  return multi_launch_epi(m, st);
+}
+
+// float64 epilogue of every row of every event (4 rows per block)
+int multi_launch_epi(gwl_multi* m, hipStream_t st) {
   MultiEpiArgs ea;
   ea.evs = m->d_evs;
   ea.params = m->d_params;

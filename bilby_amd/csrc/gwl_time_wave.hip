@@ -41,6 +41,21 @@ struct RowW {
 constexpr int kWP = 512;      // transform length
 constexpr int kWaves = 4;     // wavefronts (rows) per workgroup
 constexpr int kXch = 576;     // per-wave exchange buffer, padded (float2)
+constexpr int kTw3 = 7 * 64;  // pass-3 twiddles W_512^(s l), [s - 1][l]
+
+// LDS tables of the transform: tw[m] = W_512^m and tw3[s - 1][l] =
+// W_512^(s l mod 512) (so that lane l reads consecutive entries), both
+// from float64 sincospi.
+__device__ inline void init_wave_tables(float2* tw, float2* tw3) {
+  init_twiddles<kWP, 64 * kWaves>(tw);
+  for (int k = threadIdx.x; k < kTw3; k += 64 * kWaves) {
+    const int s = k / 64 + 1, l = k & 63;
+    const int e = (s * l) & (kWP - 1);
+    double sn, cs;
+    sincospi(-2.0 * (double)e / (double)kWP, &sn, &cs);
+    tw3[k] = make_float2((float)cs, (float)sn);
+  }
+}
 
 __device__ __forceinline__ int xpad(int e) { return e + (e >> 3); }
 
@@ -60,12 +75,9 @@ struct RowArgs {
   RowW* rows;
 };
 
-__global__ __launch_bounds__(256) void k_rowconst(RowArgs a) {
-  const long long r = (long long)blockIdx.x * 256 + threadIdx.x;
-  if (r >= a.n) return;
-  const EvDev& ev = a.ev;
+__device__ inline void rowconst_one(const EvDev& ev, const double* row, RowW* out) {
   SampleConst sc;
-  compute_sample_const(ev, a.params + r * a.ld, sc);
+  compute_sample_const(ev, row, sc);
   RowW w;
 #pragma unroll
   for (int k = 0; k < 9; ++k) w.c[k] = sc.c[k];
@@ -88,14 +100,55 @@ __global__ __launch_bounds__(256) void k_rowconst(RowArgs a) {
   w.ke = sc.ke;
   w.valid = sc.valid;
   w.pad = 0;
-  a.rows[r] = w;
+  *out = w;
+}
+
+__global__ __launch_bounds__(256) void k_rowconst(RowArgs a) {
+  const long long r = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (r >= a.n) return;
+  rowconst_one(a.ev, a.params + r * a.ld, a.rows + r);
+}
+
+// The event fields the row loop reads.
+struct WaveEv {
+  const double* tf2b;
+  const float2* dwa32;
+  double df;
+  int N, Q, jlo, L, nd, pad;
+};
+
+__device__ __forceinline__ WaveEv wave_ev(const EvDev& ev) {
+  WaveEv w;
+  w.tf2b = ev.tf2b;
+  w.dwa32 = ev.dwa32;
+  w.df = ev.df;
+  w.N = ev.N;
+  w.Q = ev.Q;
+  w.jlo = ev.jlo;
+  w.L = ev.L;
+  w.nd = ev.nd;
+  w.pad = 0;
+  return w;
+}
+
+// Copy of a wave-uniform value held in scalar registers.
+template <typename T>
+__device__ __forceinline__ T uniform_copy(const T& v) {
+  static_assert(sizeof(T) % 4 == 0, "word-sized type");
+  T out;
+  const int* src = reinterpret_cast<const int*>(&v);
+  int* dst = reinterpret_cast<int*>(&out);
+#pragma unroll
+  for (int k = 0; k < (int)(sizeof(T) / 4); ++k) dst[k] = __builtin_amdgcn_readfirstlane(src[k]);
+  return out;
 }
 
 // h conj(d) / S at decimated element i (bin nbin) for one row, summed over
 // detectors (interferometer.py:303-368, base.py:327-329): float64 phase,
 // reduced to [0, 1) cycles, float32 rotation and products.
 template <int ND>
-__device__ __forceinline__ float2 integrand_tf2(const EvDev& ev, const RowW& rw, int i, int nbin) {
+__device__ __forceinline__ float2 integrand_tf2(const WaveEv& ev, const RowW& rw, int nd, int i,
+                                                int nbin) {
   const double f = (double)nbin * ev.df;
   const double* tb = ev.tf2b + 3 * (size_t)i;
   const double u = tb[0], v = tb[1], lf = tb[2];
@@ -112,6 +165,7 @@ __device__ __forceinline__ float2 integrand_tf2(const EvDev& ev, const RowW& rw,
   float ar = 0.f, ai = 0.f;
 #pragma unroll
   for (int d = 0; d < ND; ++d) {
+    if (d >= nd) break;
     const double cyc = fma(f, rw.dt[d], s);
     const float2 e = expm2pi((float)(cyc - floor(cyc)));
     const float2 dw = ev.dwa32[(size_t)d * ev.N + i];
@@ -134,40 +188,23 @@ struct TimeWArgs {
   double* out;
 };
 
+// The transform of one row over sub-sequences [n2b, n2e): lane l returns
+// the float64 window accumulators of bins j = l + 64 s (mod 512) in
+// accr[s], acci[s].  tw: the 512-entry twiddle table in LDS; xb: this
+// wavefront's exchange buffer.  nd <= ND detectors.
 template <int ND>
-__global__ __launch_bounds__(64 * kWaves, 4) void k_time_w(TimeWArgs a) {
-  extern __shared__ double lds_d[];
-  const EvDev& ev = a.ev;
-  float2* tw = reinterpret_cast<float2*>(lds_d);                  // [512]
-  float2* xch = tw + kWP;                                          // [kWaves][kXch]
-  double* knots = reinterpret_cast<double*>(xch + kWaves * kXch);  // tx, ty
-  const bool dist = (ev.flags & GWL_DISTANCE_MARG) != 0;
-  init_twiddles<kWP, 64 * kWaves>(tw);
-  if (a.fused && dist) stage_knots(ev, knots);
-  __syncthreads();
-  const int lane = threadIdx.x & 63;
-  const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const long long item = (long long)blockIdx.x * kWaves + wid;
-  const long long r = item / a.G;
-  const int g = (int)(item - r * a.G);
-  if (r >= a.n) return;
-  const RowW& rw = a.rows[r];
+__device__ __forceinline__ void wave_row(const WaveEv& ev, const RowW& rw, int nd, int n2b, int n2e,
+                                         const float2* tw, const float2* tw3, float2* xb, int lane,
+                                         double* accr, double* acci) {
   const int Q = ev.Q;
-  const int n2b = (int)(((long long)g * Q) / a.G);
-  const int n2e = (int)(((long long)(g + 1) * Q) / a.G);
-  // per-lane twiddles of pass 3 (span 64: W_512^(s l)); those of pass 2
-  // (span 8: W_64^(s k), k = l mod 8, eight distinct per s) are read from LDS
-  float2 t3[7];
-#pragma unroll
-  for (int s = 1; s < 8; ++s) t3[s - 1] = tw[(s * lane) & (kWP - 1)];
+  // twiddles from LDS: pass 2 (span 8) W_64^(s k), k = l mod 8 (eight
+  // distinct addresses per s); pass 3 (span 64) W_512^(s l) from tw3
   const int k2 = 8 * (lane & 7);
-  double accr[8], acci[8];
 #pragma unroll
   for (int s = 0; s < 8; ++s) accr[s] = acci[s] = 0.0;
   const unsigned nmask = (unsigned)ev.N - 1u;
   const float invN = 1.0f / (float)ev.N;
   const int kb = rw.kb, ke = rw.ke;
-  float2* xb = xch + wid * kXch;
   const int base2 = 64 * (lane >> 3) + (lane & 7);
 
   for (int n2 = n2b; n2 < n2e; ++n2) {
@@ -180,7 +217,7 @@ __global__ __launch_bounds__(64 * kWaves, 4) void k_time_w(TimeWArgs a) {
       x[s] = make_float2(0.f, 0.f);
       if (ilo <= ihi && ilo <= 64 * s + 63 && ihi >= 64 * s) {
         const int i = lane + 64 * s;
-        if (i >= ilo && i <= ihi) x[s] = integrand_tf2<ND>(ev, rw, n2 * kWP + i, Q * i + n2);
+        if (i >= ilo && i <= ihi) x[s] = integrand_tf2<ND>(ev, rw, nd, n2 * kWP + i, Q * i + n2);
       }
     }
     // pass 1 (span 1): inputs l + 64 s, outputs to 8 l + s.  When the
@@ -219,7 +256,7 @@ __global__ __launch_bounds__(64 * kWaves, 4) void k_time_w(TimeWArgs a) {
 #pragma unroll
     for (int s = 0; s < 8; ++s) x[s] = xb[xpad(lane + 64 * s)];
 #pragma unroll
-    for (int s = 1; s < 8; ++s) x[s] = cmul(x[s], t3[s - 1]);
+    for (int s = 1; s < 8; ++s) x[s] = cmul(x[s], tw3[(s - 1) * 64 + lane]);
     dft8(x);
     wave_sync();
     // window bins j = jlo + jj with j = l + 64 s (mod 512)
@@ -236,15 +273,50 @@ __global__ __launch_bounds__(64 * kWaves, 4) void k_time_w(TimeWArgs a) {
       }
     }
   }
+}
+
+// Window partials of row r, group g (layout [row][G][L], <h|h> in group 0).
+__device__ __forceinline__ void wave_store_partials(const WaveEv& ev, const RowW& rw, long long r,
+                                                    int G, int g, int lane, const double* accr,
+                                                    const double* acci, double2* part,
+                                                    double* hhpart) {
+  double2* dst = part + ((size_t)r * G + g) * ev.L;
+#pragma unroll
+  for (int s = 0; s < 8; ++s) {
+    const int jj = (lane + 64 * s - ev.jlo) & (kWP - 1);
+    if (jj < ev.L) dst[jj] = make_double2(accr[s], acci[s]);
+  }
+  if (lane == 0) hhpart[(size_t)r * G + g] = (g == 0) ? rw.hh : 0.0;
+}
+
+template <int ND>
+__global__ __launch_bounds__(64 * kWaves, 4) void k_time_w(TimeWArgs a) {
+  extern __shared__ double lds_d[];
+  const EvDev& ev = a.ev;
+  float2* tw = reinterpret_cast<float2*>(lds_d);                  // [512]
+  float2* tw3 = tw + kWP;                                          // [7][64]
+  float2* xch = tw3 + kTw3;                                        // [kWaves][kXch]
+  double* knots = reinterpret_cast<double*>(xch + kWaves * kXch);  // tx, ty
+  const bool dist = (ev.flags & GWL_DISTANCE_MARG) != 0;
+  init_wave_tables(tw, tw3);
+  if (a.fused && dist) stage_knots(ev, knots);
+  __syncthreads();
+  const int lane = threadIdx.x & 63;
+  const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const long long item = (long long)blockIdx.x * kWaves + wid;
+  const long long r = item / a.G;
+  const int g = (int)(item - r * a.G);
+  if (r >= a.n) return;
+  const RowW& rw = a.rows[r];
+  const int Q = ev.Q;
+  const int n2b = (int)(((long long)g * Q) / a.G);
+  const int n2e = (int)(((long long)(g + 1) * Q) / a.G);
+  const WaveEv wv = wave_ev(ev);
+  double accr[8], acci[8];
+  wave_row<ND>(wv, rw, ND, n2b, n2e, tw, tw3, xch + wid * kXch, lane, accr, acci);
 
   if (!a.fused) {
-    double2* dst = a.part + ((size_t)r * a.G + g) * ev.L;
-#pragma unroll
-    for (int s = 0; s < 8; ++s) {
-      const int jj = (lane + 64 * s - ev.jlo) & (kWP - 1);
-      if (jj < ev.L) dst[jj] = make_double2(accr[s], acci[s]);
-    }
-    if (lane == 0) a.hhpart[(size_t)r * a.G + g] = (g == 0) ? rw.hh : 0.0;
+    wave_store_partials(wv, rw, r, a.G, g, lane, accr, acci, a.part, a.hhpart);
     return;
   }
   // epilogue: logsumexp over the prior window (base.py:794-820)
@@ -271,6 +343,67 @@ __global__ __launch_bounds__(64 * kWaves, 4) void k_time_w(TimeWArgs a) {
   if (lane == 0) {
     const double b = (1.0 / (ev.tmax - ev.tmin)) * ev.tdelta;
     a.out[r] = (m > -INFINITY) ? log(b * sum) + m : -INFINITY;
+  }
+}
+
+// ------------------------------------------------------------------ multi-event
+// Many TaylorF2 events in one launch: row constants of every event, then a
+// persistent pass in which each workgroup takes (event, group, 4-row block)
+// items from the longest-first work list through an atomic counter, one
+// wavefront per row; window partials go to the per-event buffers that the
+// multi-event epilogue reads.
+struct MultiWArgs {
+  const EvDev* evs;
+  const double* const* params;
+  const long long* rows;
+  const int* groups;
+  RowW* const* roww;
+  double2* const* part;
+  double* const* hhpart;
+  const MultiItem* items;
+  const int* n_items;
+  int* counter;
+  int ld;
+};
+
+__global__ __launch_bounds__(256) void k_rowconst_multi(MultiWArgs a) {
+  const int e = blockIdx.y;
+  const long long r = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (r >= a.rows[e]) return;
+  rowconst_one(a.evs[e], a.params[e] + r * a.ld, a.roww[e] + r);
+}
+
+__global__ void k_zero_counter(int* p) { *p = 0; }
+
+__global__ __launch_bounds__(64 * kWaves, 4) void k_time_w_multi(MultiWArgs a) {
+  extern __shared__ double lds_d[];
+  float2* tw = reinterpret_cast<float2*>(lds_d);  // [512]
+  float2* tw3 = tw + kWP;                          // [7][64]
+  float2* xch = tw3 + kTw3;                        // [kWaves][kXch]
+  __shared__ int s_item;
+  init_wave_tables(tw, tw3);
+  const int lane = threadIdx.x & 63;
+  const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int n_items = *a.n_items;
+  for (;;) {
+    __syncthreads();
+    if (threadIdx.x == 0) s_item = atomicAdd(a.counter, 1);
+    __syncthreads();
+    const int it = __builtin_amdgcn_readfirstlane(s_item);
+    if (it >= n_items) break;
+    const MultiItem m = a.items[it];
+    const long long r = m.row0 + wid;
+    if (r < a.rows[m.e]) {
+      const WaveEv ev = uniform_copy(wave_ev(a.evs[m.e]));
+      const int G = __builtin_amdgcn_readfirstlane(a.groups[m.e]);
+      const int Q = ev.Q;
+      const int n2b = (int)(((long long)m.g * Q) / G);
+      const int n2e = (int)(((long long)(m.g + 1) * Q) / G);
+      const RowW rw = uniform_copy(a.roww[m.e][r]);
+      double accr[8], acci[8];
+      wave_row<3>(ev, rw, ev.nd, n2b, n2e, tw, tw3, xch + wid * kXch, lane, accr, acci);
+      wave_store_partials(ev, rw, r, G, m.g, lane, accr, acci, a.part[m.e], a.hhpart[m.e]);
+    }
   }
 }
 
@@ -326,7 +459,7 @@ int init_time_wave_tables(const EvDev& ev, double* tb, float2* dwa) {
 }
 
 size_t time_wave_lds_bytes(const EvDev& ev, bool fused) {
-  size_t b = (size_t)(kWP + kWaves * kXch) * sizeof(float2);
+  size_t b = (size_t)(kWP + kTw3 + kWaves * kXch) * sizeof(float2);
   if (fused && (ev.flags & GWL_DISTANCE_MARG)) b += (size_t)(ev.sp.nx + ev.sp.ny + 8) * sizeof(double);
   return b;
 }
@@ -371,6 +504,36 @@ int launch_time_wave(gwl_event* e, const double* params, int64_t n, int ld, doub
   GWL_HIP_CHECK(hipGetLastError());
   if (int rc = timing_end(e, st)) return rc;
   if (!ta.fused) return launch_epi_time(e, params, n, ld, G, ta.part, ta.hhpart, out, st);
+  return GWL_OK;
+}
+
+size_t roww_bytes(int64_t rows) { return (size_t)rows * sizeof(RowW); }
+
+size_t multi_wave_lds_bytes() { return (size_t)(kWP + kTw3 + kWaves * kXch) * sizeof(float2); }
+
+int multi_launch_wave(gwl_multi* m, hipStream_t st) {
+  MultiWArgs a;
+  a.evs = m->d_evs;
+  a.params = m->d_params;
+  a.rows = m->d_rows;
+  a.groups = m->d_groups;
+  a.roww = reinterpret_cast<RowW* const*>(m->d_roww);
+  a.part = reinterpret_cast<double2* const*>(m->d_parts);
+  a.hhpart = m->d_hhs;
+  a.items = m->d_items;
+  a.n_items = m->d_n_items;
+  a.counter = m->d_counter;
+  a.ld = GWL_NPARAM;
+  long long maxr = 1;
+  for (int64_t r : m->max_rows) maxr = std::max<long long>(maxr, r);
+  hipLaunchKernelGGL(k_rowconst_multi, dim3((unsigned)((maxr + 255) / 256), (unsigned)m->n_events),
+                     dim3(256), 0, st, a);
+  GWL_HIP_CHECK(hipGetLastError());
+  hipLaunchKernelGGL(k_zero_counter, dim3(1), dim3(1), 0, st, m->d_counter);
+  GWL_HIP_CHECK(hipGetLastError());
+  hipLaunchKernelGGL(k_time_w_multi, dim3((unsigned)(m->ctx->n_cu * 4)), dim3(64 * kWaves),
+                     multi_wave_lds_bytes(), st, a);
+  GWL_HIP_CHECK(hipGetLastError());
   return GWL_OK;
 }
 
