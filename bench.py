@@ -78,18 +78,24 @@ def _pool_run(job):
     return n, time.perf_counter() - t0
 
 
-def measured_traffic(key):
-    """HBM bytes per launch of the kernel named ``key``, as recorded from the
-    FETCH_SIZE / WRITE_SIZE counter runs in a summary under profiles/ (a line
-    'bench-traffic <key>: <bytes>'); None if no summary records it."""
+def profile_figure(tag, key):
+    """A figure recorded from a counter run in a summary under profiles/ (a
+    line '<tag> <key>: <number>'); None if no summary records it."""
     import glob
     for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "*.md"))):
         with open(path) as fh:
             for line in fh:
                 line = line.strip().strip("`")
-                if line.startswith("bench-traffic %s:" % key):
-                    return int(float(line.split(":", 1)[1]))
+                if line.startswith("%s %s:" % (tag, key)):
+                    return float(line.split(":", 1)[1])
     return None
+
+
+def measured_traffic(key):
+    """HBM bytes per launch of the kernel ``key`` from the FETCH_SIZE /
+    WRITE_SIZE counter runs ('bench-traffic' lines)."""
+    v = profile_figure("bench-traffic", key)
+    return None if v is None else int(v)
 
 
 def cpu_model():
@@ -326,9 +332,14 @@ def bench_serving(main_lk, main_rows, duration):
         issued, wall = run_load(server, mix, n_samplers=32, rate=2000.0, duration=duration,
                                 rows_range=(1, 1024), seed=2024)
         summ = latency_summary(server, since=t_start)
+        for prefix in ("c2", "c3", "c4"):
+            sub = latency_summary(server, since=t_start, prefix=prefix)
+            if sub:
+                summ["%s_p50_ms" % prefix] = sub["p50_ms"]
+                summ["%s_p99_ms" % prefix] = sub["p99_ms"]
+        summ["batches"], _ = server.stats()
         summ["evals_per_s"] = summ["rows"] / wall
         summ["requests_per_s"] = summ["requests"] / wall
-        summ["rounds"] = server.rounds
         summ["resident_gb"] = server.resident_bytes / 1e9
         # audit ~1% of requests (capped) against the float64 path on the device
         rng = np.random.default_rng(7)
@@ -365,6 +376,9 @@ def main():
                     help="seconds of the process-pool CPU baseline (0 = skip)")
     ap.add_argument("--no-extra", action="store_true", help="headline workload only")
     ap.add_argument("--serve-seconds", type=float, default=5.0)
+    ap.add_argument("--only", default="",
+                    help="comma-separated substrings: run only the extra workloads whose name "
+                         "contains one of them")
     ap.add_argument("--profile", action="store_true",
                     help="counter runs: warmup + steps launches of the headline batch at --tile, "
                          "then 3 copies of 512 MiB (a plain copy kernel of known size)")
@@ -503,7 +517,12 @@ def main():
                                          else " (tiled: waveform + projection + LDS FFT)")),
                      "kernel_ms": k_ms, "bytes_per_eval": bpe,
                      "traffic_source": "rocprofv3 FETCH_SIZE x2.0 + WRITE_SIZE x1.0 per launch, "
-                                       "profiles/ (bench-traffic %s)" % traffic_key},
+                                       "profiles/ (bench-traffic %s)" % traffic_key,
+                     "binding": "valu" if achieved > 8000.0 else "hbm",
+                     "valu_frac": profile_figure("bench-valu", traffic_key),
+                     "valu_source": "rocprofv3 SQ_INSTS_VALU / (256 wave-instr per cycle x "
+                                    "GRBM_GUI_ACTIVE per XCD), profiles/ (bench-valu %s)"
+                                    % traffic_key},
         "autotune": {"chosen": list(tile), "ms_by_tile": sweep},
         "accuracy": {"fp32_vs_fp64_gpu_max": float(d_all.max()),
                      "fp32_vs_fp64_gpu_p99": float(np.percentile(d_all, 99))},
@@ -519,6 +538,8 @@ def main():
                          ("roq_400_120_nodes_1000_times_65536", lambda: bench_roq(3)),
                          ("serving_32_samplers_2000_rps", lambda: bench_serving(
                              lk, rows.cpu().numpy(), args.serve_seconds))):
+            if args.only and not any(k in name for k in args.only.split(",")):
+                continue
             log("extra:", name)
             try:
                 extra[name] = fn()

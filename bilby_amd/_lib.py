@@ -105,6 +105,21 @@ def _load():
         "gwl_gemm": ([vp, ctypes.c_int32, vp, vp, vp, vp, vp, ctypes.c_int32, ctypes.c_int32,
                       ctypes.c_int32, vp], ctypes.c_int),
         "gwl_event_timing_read": ([vp, _dp, ctypes.POINTER(ctypes.c_int64)], ctypes.c_int),
+        "gwl_roq_reserve": ([vp, ctypes.c_int64], ctypes.c_int),
+        "gwl_server_create": ([vp, ctypes.c_int64, ctypes.POINTER(vp)], ctypes.c_int),
+        "gwl_server_destroy": ([vp], ctypes.c_int),
+        "gwl_server_add_event": ([vp, vp, ctypes.c_int32, ctypes.POINTER(ctypes.c_int32)],
+                                 ctypes.c_int),
+        "gwl_server_add_roq": ([vp, vp, ctypes.c_int32, ctypes.POINTER(ctypes.c_int32)],
+                               ctypes.c_int),
+        "gwl_server_add_group": ([vp, ctypes.c_int32, ctypes.POINTER(vp),
+                                  ctypes.POINTER(ctypes.c_int32)], ctypes.c_int),
+        "gwl_server_submit": ([vp, ctypes.c_int32, _dp, ctypes.c_int64, _dp,
+                               ctypes.POINTER(ctypes.c_int64)], ctypes.c_int),
+        "gwl_server_wait": ([vp, ctypes.c_int64, _dp], ctypes.c_int),
+        "gwl_server_stats": ([vp, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64)],
+                             ctypes.c_int),
+        "gwl_server_device_bytes": ([vp, ctypes.POINTER(ctypes.c_int64)], ctypes.c_int),
     }
     for name, (argt, rest) in sig.items():
         fn = getattr(lib, name)
@@ -142,4 +157,7 @@ EXPORTED_SYMBOLS = [
     "gwl_roq_create", "gwl_roq_destroy", "gwl_roq_set_distance_spline", "gwl_roq_eval",
     "gwl_gemm", "gwl_multi_create", "gwl_multi_set_rows", "gwl_multi_set_tile",
     "gwl_multi_launch", "gwl_multi_capture", "gwl_multi_destroy", "gwl_event_device_bytes",
+    "gwl_roq_reserve", "gwl_server_create", "gwl_server_destroy", "gwl_server_add_event",
+    "gwl_server_add_roq", "gwl_server_add_group", "gwl_server_submit", "gwl_server_wait",
+    "gwl_server_stats", "gwl_server_device_bytes",
 ]

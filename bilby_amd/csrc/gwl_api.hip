@@ -855,7 +855,9 @@ int gwl_multi_capture(gwl_multi* m, void* stream) {
     m->captured = false;
   }
   GWL_HIP_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+  m->capturing = true;  // grids sized for the largest work list
   int rc = multi_launch_raw(m, st);
+  m->capturing = false;
   hipGraph_t g = nullptr;
   hipError_t err = hipStreamEndCapture(st, &g);
   if (rc != GWL_OK) {

@@ -87,6 +87,8 @@ struct gwl_multi {
   // TaylorF2 events with 512-point transforms: wave-per-row kernels, row
   // constants [max_rows] per event
   bool wave_path = false;
+  bool capturing = false;   // a capture records grids for the largest work list
+  int n_blocks = 0;         // epilogue blocks of the current rows
   std::vector<void*> roww;
   void** d_roww = nullptr;
 };

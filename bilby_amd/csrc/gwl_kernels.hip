@@ -957,6 +957,7 @@ int multi_set_rows(gwl_multi* m, const int64_t* rows, hipStream_t st) {
   }
   GWL_HIP_CHECK(hipMemcpy(m->d_n_blocks, &n_blocks, sizeof(int), hipMemcpyHostToDevice));
   m->n_items = n_items;
+  m->n_blocks = n_blocks;
   (void)st;
   return GWL_OK;
 }
@@ -1008,7 +1009,8 @@ int multi_launch_epi(gwl_multi* m, hipStream_t st) {
   for (auto* ev : m->events)
     if (ev->dev.flags & GWL_DISTANCE_MARG)
       klds = std::max(klds, (size_t)(ev->dev.sp.nx + ev->dev.sp.ny + 8) * sizeof(double));
-  hipLaunchKernelGGL(k_epi_time_multi, dim3((unsigned)m->max_blocks), dim3(64 * kEpiRows), klds, st, ea);
+  const size_t grid = m->capturing ? m->max_blocks : (size_t)std::max(1, m->n_blocks);
+  hipLaunchKernelGGL(k_epi_time_multi, dim3((unsigned)grid), dim3(64 * kEpiRows), klds, st, ea);
   GWL_HIP_CHECK(hipGetLastError());
   return GWL_OK;
 }

@@ -721,6 +721,27 @@ static size_t roq_ws_bytes(const RoqDev& rd, int64_t n, int mode) {
   return a + c + (size_t)n * sizeof(RoqRow) + 3 * 256;
 }
 
+int gwl_roq_reserve(gwl_roq* e, int64_t rows) {
+  if (!e || rows < 1) {
+    set_error("gwl_roq_reserve: invalid argument");
+    return GWL_E_INVALID;
+  }
+  GWL_HIP_CHECK(hipSetDevice(e->ctx->device));
+  size_t need = 0;
+  for (int mode = ROQ_FP64; mode <= ROQ_BF16X3; ++mode)
+    need = std::max(need, roq_ws_bytes(e->dev, rows, mode));
+  if (need <= e->ws_bytes) return GWL_OK;
+  if (e->ws) (void)hipFree(e->ws);
+  e->ws = nullptr;
+  e->ws_bytes = 0;
+  if (hipMalloc(&e->ws, need) != hipSuccess) {
+    set_error("hipMalloc failed for ROQ workspace");
+    return GWL_E_NOMEM;
+  }
+  e->ws_bytes = need;
+  return GWL_OK;
+}
+
 int gwl_roq_eval(gwl_roq* e, const double* params, int64_t n, int32_t ld, int32_t mode,
                  double* out, void* stream) {
   if (!e || !params || !out || n < 0 || ld < GWL_NPARAM || mode < 0 || mode > 2) {

@@ -347,11 +347,13 @@ __global__ __launch_bounds__(64 * kWaves, 4) void k_time_w(TimeWArgs a) {
 }
 
 // ------------------------------------------------------------------ multi-event
-// Many TaylorF2 events in one launch: row constants of every event, then a
-// persistent pass in which each workgroup takes (event, group, 4-row block)
-// items from the longest-first work list through an atomic counter, one
-// wavefront per row; window partials go to the per-event buffers that the
-// multi-event epilogue reads.
+// Many TaylorF2 events in one launch: row constants of every event, then one
+// workgroup per (event, group, 4-row block) item of the longest-first work
+// list (the grid covers the largest list; workgroups past the current list
+// exit at once), one wavefront per row; window partials go to the per-event
+// buffers that the multi-event epilogue reads.  Workgroups are dispatched in
+// list order, so the longest items start first, and kernels on other streams
+// can interleave with this one.
 struct MultiWArgs {
   const EvDev* evs;
   const double* const* params;
@@ -362,7 +364,6 @@ struct MultiWArgs {
   double* const* hhpart;
   const MultiItem* items;
   const int* n_items;
-  int* counter;
   int ld;
 };
 
@@ -373,38 +374,29 @@ __global__ __launch_bounds__(256) void k_rowconst_multi(MultiWArgs a) {
   rowconst_one(a.evs[e], a.params[e] + r * a.ld, a.roww[e] + r);
 }
 
-__global__ void k_zero_counter(int* p) { *p = 0; }
-
 __global__ __launch_bounds__(64 * kWaves, 4) void k_time_w_multi(MultiWArgs a) {
+  const int it = blockIdx.x;
+  if (it >= *a.n_items) return;
   extern __shared__ double lds_d[];
   float2* tw = reinterpret_cast<float2*>(lds_d);  // [512]
   float2* tw3 = tw + kWP;                          // [7][64]
   float2* xch = tw3 + kTw3;                        // [kWaves][kXch]
-  __shared__ int s_item;
   init_wave_tables(tw, tw3);
+  __syncthreads();
   const int lane = threadIdx.x & 63;
   const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int n_items = *a.n_items;
-  for (;;) {
-    __syncthreads();
-    if (threadIdx.x == 0) s_item = atomicAdd(a.counter, 1);
-    __syncthreads();
-    const int it = __builtin_amdgcn_readfirstlane(s_item);
-    if (it >= n_items) break;
-    const MultiItem m = a.items[it];
-    const long long r = m.row0 + wid;
-    if (r < a.rows[m.e]) {
-      const WaveEv ev = uniform_copy(wave_ev(a.evs[m.e]));
-      const int G = __builtin_amdgcn_readfirstlane(a.groups[m.e]);
-      const int Q = ev.Q;
-      const int n2b = (int)(((long long)m.g * Q) / G);
-      const int n2e = (int)(((long long)(m.g + 1) * Q) / G);
-      const RowW rw = uniform_copy(a.roww[m.e][r]);
-      double accr[8], acci[8];
-      wave_row<3>(ev, rw, ev.nd, n2b, n2e, tw, tw3, xch + wid * kXch, lane, accr, acci);
-      wave_store_partials(ev, rw, r, G, m.g, lane, accr, acci, a.part[m.e], a.hhpart[m.e]);
-    }
-  }
+  const MultiItem m = a.items[it];
+  const long long r = m.row0 + wid;
+  if (r >= a.rows[m.e]) return;
+  const WaveEv ev = uniform_copy(wave_ev(a.evs[m.e]));
+  const int G = __builtin_amdgcn_readfirstlane(a.groups[m.e]);
+  const int Q = ev.Q;
+  const int n2b = (int)(((long long)m.g * Q) / G);
+  const int n2e = (int)(((long long)(m.g + 1) * Q) / G);
+  const RowW rw = uniform_copy(a.roww[m.e][r]);
+  double accr[8], acci[8];
+  wave_row<3>(ev, rw, ev.nd, n2b, n2e, tw, tw3, xch + wid * kXch, lane, accr, acci);
+  wave_store_partials(ev, rw, r, G, m.g, lane, accr, acci, a.part[m.e], a.hhpart[m.e]);
 }
 
 // f^1/3, f^-1/3, ln f and dw32 f^-7/6 at every decimated element.
@@ -522,16 +514,15 @@ int multi_launch_wave(gwl_multi* m, hipStream_t st) {
   a.hhpart = m->d_hhs;
   a.items = m->d_items;
   a.n_items = m->d_n_items;
-  a.counter = m->d_counter;
   a.ld = GWL_NPARAM;
   long long maxr = 1;
   for (int64_t r : m->max_rows) maxr = std::max<long long>(maxr, r);
   hipLaunchKernelGGL(k_rowconst_multi, dim3((unsigned)((maxr + 255) / 256), (unsigned)m->n_events),
                      dim3(256), 0, st, a);
   GWL_HIP_CHECK(hipGetLastError());
-  hipLaunchKernelGGL(k_zero_counter, dim3(1), dim3(1), 0, st, m->d_counter);
-  GWL_HIP_CHECK(hipGetLastError());
-  hipLaunchKernelGGL(k_time_w_multi, dim3((unsigned)(m->ctx->n_cu * 4)), dim3(64 * kWaves),
+  // eager launches cover the current list; a captured graph, the largest one
+  const size_t grid = m->capturing ? m->max_items : (size_t)std::max(1, m->n_items);
+  hipLaunchKernelGGL(k_time_w_multi, dim3((unsigned)grid), dim3(64 * kWaves),
                      multi_wave_lds_bytes(), st, a);
   GWL_HIP_CHECK(hipGetLastError());
   return GWL_OK;

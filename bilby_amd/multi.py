@@ -24,6 +24,9 @@ class MultiEventLikelihood:
         self.params = [torch.zeros((max(1, m), _lib.NPARAM), dtype=torch.float64, device=dev)
                        for m in self.max_rows]
         self.outs = [torch.zeros(max(1, m), dtype=torch.float64, device=dev) for m in self.max_rows]
+        # pinned host staging of each event's rows (asynchronous uploads)
+        self._staging = [torch.zeros((max(1, m), _lib.NPARAM), dtype=torch.float64).pin_memory()
+                         for m in self.max_rows]
         n = len(self.likelihoods)
         evs = (ctypes.c_void_p * n)(*[lk._event.value for lk in self.likelihoods])
         pp = (ctypes.c_void_p * n)(*[t.data_ptr() for t in self.params])
@@ -56,17 +59,24 @@ class MultiEventLikelihood:
         """Rows for event ``e``: a dict of arrays, an array [n, 16] or a GPU
         tensor [n, 16]."""
         lk = self.likelihoods[e]
-        if isinstance(parameters, torch.Tensor):
+        if isinstance(parameters, torch.Tensor) and parameters.is_cuda:
             rows = parameters.to(torch.float64)
         elif isinstance(parameters, dict):
-            rows = torch.from_numpy(lk.rows_from_parameters(parameters))
+            rows = lk.rows_from_parameters(parameters)
         else:
-            rows = torch.from_numpy(np.ascontiguousarray(parameters, dtype=np.float64))
+            rows = np.ascontiguousarray(parameters, dtype=np.float64)
         n = rows.shape[0]
         if n > self.max_rows[e]:
             raise ValueError("event %d: %d rows > max_rows %d" % (e, n, self.max_rows[e]))
+        # the previous launch may still read the staging buffer and rows
+        self.stream.synchronize()
         with torch.cuda.stream(self.stream):
-            self.params[e][:n].copy_(rows, non_blocking=False)
+            if isinstance(rows, torch.Tensor):
+                self.params[e][:n].copy_(rows, non_blocking=True)
+            else:
+                stage = self._staging[e]
+                stage[:n].numpy()[...] = rows
+                self.params[e][:n].copy_(stage[:n], non_blocking=True)
         self.rows[e] = n
 
     def capture(self):

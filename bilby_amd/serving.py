@@ -1,12 +1,15 @@
 """Continuous-batching likelihood server on one GPU.
 
-Many samplers submit requests (an event name and N parameter rows); one
-scheduler thread forms a batch per event from everything pending, launches
-the events' batches on separate HIP streams so they overlap on the device,
-and completes each request's future as soon as its batch finishes.  Requests
-that arrive while batches are in flight join the next round.  Event data,
-PSDs, splines and ROQ weights stay resident; registration and in-flight
-work are admitted against an HBM budget.
+Many samplers submit requests (an event name and N parameter rows); the
+scheduler in libgwl (gwl_server_*) keeps at most one batch in flight per
+event (per group of events evaluated together): whenever an event has
+pending requests and no batch running, everything pending for it (up to
+``max_batch_rows`` rows) is packed into pinned staging, evaluated on the
+event's stream and copied back, while other events' batches keep running.
+The scheduler is native, so it never waits for the interpreter lock;
+callers block in ``Future.result()`` with the lock released.  Event data,
+PSDs, splines and ROQ weights stay resident; registration is admitted
+against an HBM budget.
 
 bilby's own parallelism is a process pool of single-dict callers
 (bilby/core/sampler/base_sampler.py:772-800); here the callers share one
@@ -15,11 +18,8 @@ device-resident likelihood per event through this queue.
 import ctypes
 import threading
 import time
-from collections import deque
-from concurrent.futures import Future
 
 import numpy as np
-import torch
 
 from . import _lib
 from ._lib import LIB, check
@@ -29,184 +29,176 @@ class HBMBudgetExceeded(MemoryError):
     pass
 
 
-class _Request:
-    __slots__ = ("event", "rows", "future", "t_submit", "t_done")
-
-    def __init__(self, event, rows):
-        self.event = event
-        self.rows = rows
-        self.future = Future()
-        self.t_submit = time.perf_counter()
-        self.t_done = None
-
-
 def _device_bytes(lk):
-    if hasattr(lk, "_event") and lk._event is not None:
+    if getattr(lk, "_event", None) is not None:
         out = ctypes.c_int64()
         check(LIB.gwl_event_device_bytes(lk._event, ctypes.byref(out)), "gwl_event_device_bytes")
         return int(out.value)
     return int(getattr(lk, "hbm_bytes", 0))
 
 
+class ServedResult:
+    """Future of one request: ``result()`` blocks (without the interpreter
+    lock) until the scheduler has written the request's values."""
+
+    def __init__(self, server, ticket, event, rows, out):
+        self._server = server
+        self._ticket = ticket
+        self.event = event
+        self._rows = rows          # kept alive until the request completes
+        self._out = out
+        self.t_submit = time.perf_counter()
+        self.latency = None
+        self._rc = None
+        self._msg = ""
+        self._lock = threading.Lock()
+
+    def result(self, timeout=None):
+        with self._lock:
+            if self._rc is None:
+                lat = ctypes.c_double()
+                rc = LIB.gwl_server_wait(self._server._s, self._ticket, ctypes.byref(lat))
+                if rc != 0:
+                    self._msg = LIB.gwl_last_error().decode(errors="replace")
+                self._rc = rc
+                self.latency = lat.value
+                self._rows = None
+                self._server._outstanding.pop(self._ticket, None)
+                self._server.completed.append((self.event, len(self._out), self.t_submit,
+                                               self.t_submit + self.latency))
+        if self._rc != 0:
+            raise _lib.GWLError("request on %r failed (%d): %s" % (self.event, self._rc, self._msg))
+        return self._out
+
+    def done(self):
+        return self._rc is not None
+
+
 class LikelihoodServer:
-    """Request loop over device-resident likelihoods.
+    """Request queue over device-resident likelihoods.
 
-    ``hbm_budget``: bytes of device memory the resident events plus the
-    in-flight batches may use.  ``max_batch_rows``: rows of one event per
-    launch; larger request sets wait for the next round."""
+    ``hbm_budget``: bytes of device memory the resident events and the
+    server's buffers may use.  ``max_batch_rows``: rows of one event per
+    launch (and the largest request)."""
 
-    def __init__(self, hbm_budget=200e9, max_batch_rows=8192, n_streams=4, device=0):
+    def __init__(self, hbm_budget=200e9, max_batch_rows=8192, device=0):
+        from .likelihood import get_context
         self.hbm_budget = float(hbm_budget)
         self.max_batch_rows = int(max_batch_rows)
         self.device = device
-        dev = torch.device("cuda:%d" % device)
-        self._dev = dev
-        self.streams = [torch.cuda.Stream(device=dev) for _ in range(n_streams)]
+        h = ctypes.c_void_p()
+        check(LIB.gwl_server_create(get_context(device), self.max_batch_rows, ctypes.byref(h)),
+              "gwl_server_create")
+        self._s = h
         self.events = {}
-        self.resident_bytes = 0
-        self._pending = deque()
-        self._cv = threading.Condition()
-        self._stop = False
+        self._keep = []
+        self._outstanding = {}   # ticket -> future: its buffers live until consumed
+        self.event_bytes = 0
         self.completed = []
-        self.rounds = 0
-        self.rows_done = 0
-        self._thread = threading.Thread(target=self._loop, name="gwl-server", daemon=True)
-        self._thread.start()
+
+    @property
+    def resident_bytes(self):
+        b = ctypes.c_int64()
+        check(LIB.gwl_server_device_bytes(self._s, ctypes.byref(b)), "gwl_server_device_bytes")
+        return self.event_bytes + int(b.value)
+
+    def _admit(self, name, nbytes):
+        if self.resident_bytes + nbytes > self.hbm_budget:
+            raise HBMBudgetExceeded("event %r needs %.3g GB; %.3g of %.3g GB in use"
+                                    % (name, nbytes / 1e9, self.resident_bytes / 1e9,
+                                       self.hbm_budget / 1e9))
 
     # ------------------------------------------------------------------ events
-    def register(self, name, likelihood, row_bytes=None):
-        """Make an event resident.  Its device bytes (data, spline or ROQ
-        weights, and a workspace sized for ``max_batch_rows``) are admitted
-        against the budget; over budget raises HBMBudgetExceeded."""
-        if hasattr(likelihood, "_event"):
+    def register(self, name, likelihood):
+        """Make an event resident (its workspace sized for max_batch_rows) and
+        admit its device bytes plus the server's staging for it against the
+        budget; over budget raises HBMBudgetExceeded."""
+        staging = self.max_batch_rows * (_lib.NPARAM + 1) * 8
+        if getattr(likelihood, "_event", None) is not None:
             check(LIB.gwl_event_reserve(likelihood._event, self.max_batch_rows), "gwl_event_reserve")
-        nbytes = _device_bytes(likelihood)
-        extra = 0
-        if row_bytes is not None:
-            extra = int(row_bytes) * self.max_batch_rows
-        if self.resident_bytes + nbytes + extra > self.hbm_budget:
-            raise HBMBudgetExceeded("event %r needs %.3g GB; %.3g of %.3g GB in use"
-                                    % (name, (nbytes + extra) / 1e9, self.resident_bytes / 1e9,
-                                       self.hbm_budget / 1e9))
-        self.events[name] = dict(likelihood=likelihood, bytes=nbytes + extra)
-        self.resident_bytes += nbytes + extra
+            nbytes = _device_bytes(likelihood)
+            self._admit(name, nbytes + staging)
+            i = ctypes.c_int32()
+            check(LIB.gwl_server_add_event(self._s, likelihood._event, int(likelihood.precision),
+                                           ctypes.byref(i)), "gwl_server_add_event")
+        else:
+            check(LIB.gwl_roq_reserve(likelihood._roq, self.max_batch_rows), "gwl_roq_reserve")
+            nbytes = _device_bytes(likelihood)
+            self._admit(name, nbytes + staging)
+            i = ctypes.c_int32()
+            check(LIB.gwl_server_add_roq(self._s, likelihood._roq, int(likelihood.precision),
+                                         ctypes.byref(i)), "gwl_server_add_roq")
+        self.events[name] = i.value
+        self.event_bytes += nbytes
+        self._keep.append(likelihood)
 
     def register_multi(self, names, likelihoods):
-        """Register time-marginalised events that are evaluated together:
-        every round, all their pending rows go through one persistent
-        multi-event launch (replayed from a captured hipGraph)."""
-        from .multi import MultiEventLikelihood
+        """Register time-marginalised TaylorF2 events that are evaluated
+        together: every launch of the group takes all their pending rows
+        through one multi-event launch."""
+        n = len(likelihoods)
+        nbytes = 0
         for name, lk in zip(names, likelihoods):
-            self.register(name, lk)
-        multi = MultiEventLikelihood(likelihoods, [self.max_batch_rows] * len(likelihoods),
-                                     device=self.device)
-        multi.capture()
-        gid = len(getattr(self, "_multis", []))
-        if not hasattr(self, "_multis"):
-            self._multis = []
-        self._multis.append(multi)
+            check(LIB.gwl_event_reserve(lk._event, 1), "gwl_event_reserve")
+            nbytes += _device_bytes(lk)
+        self._admit(names[0], nbytes)
+        evs = (ctypes.c_void_p * n)(*[lk._event.value for lk in likelihoods])
+        first = ctypes.c_int32()
+        check(LIB.gwl_server_add_group(self._s, n, evs, ctypes.byref(first)), "gwl_server_add_group")
         for k, name in enumerate(names):
-            self.events[name]["multi"] = (gid, k)
-        extra = sum(int(t.numel() * 8) for t in multi.params + multi.outs)
-        self.resident_bytes += extra
-        return multi
-
-    def unregister(self, name):
-        ev = self.events.pop(name)
-        self.resident_bytes -= ev["bytes"]
+            self.events[name] = first.value + k
+        self.event_bytes += nbytes
+        self._keep.extend(likelihoods)
+        if self.resident_bytes > self.hbm_budget:
+            raise HBMBudgetExceeded("group of %d events exceeds the budget (%.3g GB in use)"
+                                    % (n, self.resident_bytes / 1e9))
 
     # ------------------------------------------------------------------ requests
     def submit(self, event, rows):
-        """rows: float64 array [n, 16] (gwl column order).  Returns a Future
+        """rows: float64 array [n, 16] (gwl column order).  Returns a future
         resolving to an array of n log-likelihood ratios."""
         if event not in self.events:
             raise KeyError(event)
         rows = np.ascontiguousarray(rows, dtype=np.float64)
         if rows.ndim != 2 or rows.shape[1] != _lib.NPARAM:
             raise ValueError("rows must be [n, %d]" % _lib.NPARAM)
-        req = _Request(event, rows)
-        with self._cv:
-            self._pending.append(req)
-            self._cv.notify()
-        return req.future
+        out = np.empty(rows.shape[0])
+        t = ctypes.c_int64()
+        check(LIB.gwl_server_submit(self._s, self.events[event], _lib.dptr(rows), rows.shape[0],
+                                    _lib.dptr(out), ctypes.byref(t)), "gwl_server_submit")
+        fut = ServedResult(self, t.value, event, rows, out)
+        self._outstanding[t.value] = fut
+        return fut
+
+    def stats(self):
+        b, r = ctypes.c_int64(), ctypes.c_int64()
+        check(LIB.gwl_server_stats(self._s, ctypes.byref(b), ctypes.byref(r)), "gwl_server_stats")
+        return int(b.value), int(r.value)
+
+    @property
+    def rounds(self):
+        return self.stats()[0]
+
+    @property
+    def rows_done(self):
+        return self.stats()[1]
 
     def close(self):
-        with self._cv:
-            self._stop = True
-            self._cv.notify()
-        self._thread.join()
+        """Finish every outstanding request, then free the server."""
+        if self._s is not None:
+            for fut in list(self._outstanding.values()):
+                try:
+                    fut.result()
+                except Exception:
+                    pass
+            LIB.gwl_server_destroy(self._s)
+            self._s = None
 
-    # ------------------------------------------------------------------ loop
-    def _take_round(self):
-        """All pending requests, grouped by event, each group capped at
-        max_batch_rows (a request is never split); the rest stays queued."""
-        groups = {}
-        keep = deque()
-        while self._pending:
-            req = self._pending.popleft()
-            g = groups.setdefault(req.event, [])
-            n = sum(r.rows.shape[0] for r in g)
-            if g and n + req.rows.shape[0] > self.max_batch_rows:
-                keep.append(req)
-            else:
-                g.append(req)
-        self._pending = keep
-        return groups
-
-    def _loop(self):
-        while True:
-            with self._cv:
-                while not self._pending and not self._stop:
-                    self._cv.wait()
-                if self._stop and not self._pending:
-                    return
-                groups = self._take_round()
-            launched = []
-            multi_groups = {}
-            for name in list(groups):
-                m = self.events[name].get("multi")
-                if m is not None:
-                    multi_groups.setdefault(m[0], []).append((m[1], name, groups.pop(name)))
-            for gid, members in multi_groups.items():
-                multi = self._multis[gid]
-                multi.rows = [0] * len(multi.rows)
-                for k, name, reqs in members:
-                    rows = np.concatenate([r.rows for r in reqs]) if len(reqs) > 1 else reqs[0].rows
-                    multi.set_parameters(k, rows)
-                outs = multi.run(synchronize=False)
-                for k, name, reqs in members:
-                    with torch.cuda.stream(multi.stream):
-                        out_host = torch.empty(outs[k].shape, dtype=torch.float64).pin_memory()
-                        out_host.copy_(outs[k], non_blocking=True)
-                        done = torch.cuda.Event()
-                        done.record(multi.stream)
-                    launched.append((reqs, out_host, done, outs[k]))
-            for k, (name, reqs) in enumerate(groups.items()):
-                stream = self.streams[k % len(self.streams)]
-                lk = self.events[name]["likelihood"]
-                rows = np.concatenate([r.rows for r in reqs]) if len(reqs) > 1 else reqs[0].rows
-                with torch.cuda.stream(stream):
-                    host = torch.from_numpy(rows).pin_memory()
-                    dev_rows = host.to(self._dev, non_blocking=True)
-                    out = lk.log_likelihood_ratio_batch(dev_rows, stream=stream.cuda_stream)
-                    out_host = torch.empty(out.shape, dtype=torch.float64).pin_memory()
-                    out_host.copy_(out, non_blocking=True)
-                    done = torch.cuda.Event()
-                    done.record(stream)
-                launched.append((reqs, out_host, done, host, dev_rows, out))
-            for reqs, out_host, done, *_keep in launched:
-                done.synchronize()
-                t = time.perf_counter()
-                vals = out_host.numpy()
-                o = 0
-                for r in reqs:
-                    n = r.rows.shape[0]
-                    r.t_done = t
-                    r.future.set_result(vals[o:o + n].copy())
-                    o += n
-                    self.completed.append((r.event, n, r.t_submit, r.t_done))
-                self.rows_done += o
-            self.rounds += 1
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def run_load(server, event_mix, n_samplers=32, rate=2000.0, duration=5.0, rows_range=(1, 1024),
@@ -251,9 +243,10 @@ def run_load(server, event_mix, n_samplers=32, rate=2000.0, duration=5.0, rows_r
     return issued, time.perf_counter() - t0
 
 
-def latency_summary(server, since=0.0):
-    lat = np.array([(t1 - t0) for (_, _, t0, t1) in server.completed if t0 >= since])
-    rows = np.array([n for (_, n, t0, _) in server.completed if t0 >= since])
+def latency_summary(server, since=0.0, prefix=""):
+    sel = [c for c in server.completed if c[2] >= since and c[0].startswith(prefix)]
+    lat = np.array([(t1 - t0) for (_, _, t0, t1) in sel])
+    rows = np.array([n for (_, n, _, _) in sel])
     if len(lat) == 0:
         return {}
     return dict(requests=int(len(lat)), rows=int(rows.sum()),

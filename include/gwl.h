@@ -261,6 +261,8 @@ int gwl_roq_destroy(gwl_roq* roq);
 /* Distance interpolant, as gwl_event_set_distance_spline. */
 int gwl_roq_set_distance_spline(gwl_roq* roq, int32_t nx, int32_t ny, const double* tx,
                                 const double* ty, const double* coeffs, const double* bbox);
+/* Workspace for up to max_rows rows in every mode (as gwl_event_reserve). */
+int gwl_roq_reserve(gwl_roq* roq, int64_t max_rows);
 /* Batched ROQ log-likelihood ratio (roq.py:463-465 -> base.py:419-446):
  * params, out device pointers; mode enum gwl_roq_mode. */
 int gwl_roq_eval(gwl_roq* roq, const double* params, int64_t n_rows, int32_t ld, int32_t mode,
@@ -272,6 +274,32 @@ int gwl_roq_eval(gwl_roq* roq, const double* params, int64_t n_rows, int32_t ld,
  * (A = A + A_lo, Bt = Bt + Bt_lo in bfloat16, C float).  Device pointers. */
 int gwl_gemm(gwl_ctx* ctx, int32_t mode, const void* A, const void* A_lo, const void* Bt,
              const void* Bt_lo, void* C, int32_t M, int32_t N, int32_t K, void* stream);
+
+/* ------------------------------------------------------------------ serving
+ * Continuous-batching request server (no counterpart in the reference, whose
+ * parallelism is a pool of single-dict callers, base_sampler.py:772-800).
+ * One scheduler thread keeps at most one batch in flight per unit (an event,
+ * a ROQ likelihood, or a group of events evaluated by one multi-event
+ * launch); requests carry host rows and a host output buffer that must stay
+ * valid until gwl_server_wait returns.  Latency is measured from submission
+ * to the copy of the results into the output buffer. */
+typedef struct gwl_server gwl_server;
+int gwl_server_create(gwl_ctx* ctx, int64_t max_rows, gwl_server** out);
+int gwl_server_destroy(gwl_server* server);
+/* Units; each returns the id (ids first_id .. first_id + n - 1 for a group)
+ * that gwl_server_submit takes.  Add every unit before submitting. */
+int gwl_server_add_event(gwl_server* server, gwl_event* ev, int32_t precision, int32_t* id);
+int gwl_server_add_roq(gwl_server* server, gwl_roq* roq, int32_t mode, int32_t* id);
+int gwl_server_add_group(gwl_server* server, int32_t n_events, gwl_event* const* events,
+                         int32_t* first_id);
+/* rows: host [n][16] float64, out: host [n] float64, 1 <= n <= max_rows. */
+int gwl_server_submit(gwl_server* server, int32_t id, const double* rows, int64_t n_rows,
+                      double* out, int64_t* ticket);
+/* Blocks until the request is done; returns its status. */
+int gwl_server_wait(gwl_server* server, int64_t ticket, double* latency_s);
+int gwl_server_stats(gwl_server* server, int64_t* batches, int64_t* rows);
+/* Device bytes of the server's own buffers (staging, group partials). */
+int gwl_server_device_bytes(gwl_server* server, int64_t* bytes);
 
 #ifdef __cplusplus
 }

@@ -74,3 +74,52 @@ def test_hbm_budget_admission(events):
             server.register("tf", events["tf"][0])
     finally:
         server.close()
+
+
+def test_group_and_roq_units_match_direct_calls():
+    """Events served as one multi-event group, and a ROQ likelihood, give the
+    values of their direct batched calls (1e-8; the group splits each row's
+    transform into a different number of float64 partial groups), for
+    requests interleaved across the units."""
+    from bilby_amd.configs import make_roq, make_uneven_events
+    from bilby_amd.likelihood import GravitationalWaveTransient
+    from bilby_amd.roq import ROQGravitationalWaveTransient
+    from bilby_amd.serving import LikelihoodServer
+    from tests.helpers import fixed_marginalised
+    say("serving group + roq")
+    events = make_uneven_events(n_events=3, seed=8, rows_range=(20, 40), durations=(4, 8))
+    names, lks, pools = [], [], []
+    for k, ev in enumerate(events):
+        pri = ev["priors"].copy()
+        lk = GravitationalWaveTransient(ev["interferometers"], ev["waveform_generator"],
+                                        time_marginalization=True, distance_marginalization=True,
+                                        phase_marginalization=True, priors=pri)
+        names.append("g%d" % k)
+        lks.append(lk)
+        pools.append(lk.rows_from_parameters(fixed_marginalised(ev["draws"], pri)))
+    s = make_roq(n_draws=64, seed=9)
+    priors = s["priors"].copy()
+    roq = ROQGravitationalWaveTransient(s["interferometers"], s["waveform_generator"], priors,
+                                        weights=s["weights"], time_marginalization=True,
+                                        phase_marginalization=True, delta_tc=s["delta_tc"])
+    roq_rows = roq.rows_from_parameters(fixed_marginalised(s["draws"], priors))
+    server = LikelihoodServer(hbm_budget=50e9, max_batch_rows=256)
+    try:
+        server.register_multi(names, lks)
+        server.register("roq", roq)
+        futs = []
+        for rep in range(4):
+            for name, pool in zip(names, pools):
+                futs.append((name, pool[rep::4], server.submit(name, pool[rep::4])))
+            futs.append(("roq", roq_rows[rep::4], server.submit("roq", roq_rows[rep::4])))
+        direct = dict(zip(names, lks))
+        direct["roq"] = roq
+        for name, rows, fut in futs:
+            ref = direct[name].log_likelihood_ratio_batch(rows)
+            got = fut.result()
+            assert np.max(np.abs(got - ref)) < 1e-8, name
+        b, r = server.stats()
+        say("batches", b, "rows", r)
+        assert r == sum(len(x[1]) for x in futs)
+    finally:
+        server.close()
