@@ -43,6 +43,8 @@ struct RoqDev {
   const __bf16* btl;  // [Np][Kp] low bfloat16 part
   const double* bt64; // [Np][Kp] float64
   const double* ts;   // [NT] ROQ time samples (s from start)
+  const double* nbl;  // [L][kNB + 1] TaylorF2 basis (tf2_basis order) and f^-7/6 at the linear nodes
+  const double* nbq;  // [Qn][kNB + 1] the same at the quadratic nodes
   SplineDev sp;
 };
 
@@ -133,47 +135,57 @@ __global__ __launch_bounds__(128) void k_roq_prep(RoqDev rd, const double* param
   __syncthreads();
   const bool valid = live && s_valid;
   const double amp0 = s_amp0, fisco = s_fisco;
-  // linear nodes -> GEMM operand
-  for (int l = tid; l < rd.Kp / 2; l += 128) {
+  (void)amp0;
+  const size_t o = (size_t)r * rd.Kp;
+  // padding columns [2L, Kp) of every row, and rows past n, are zero
+  for (int k = 2 * rd.L + tid; k < rd.Kp; k += 128) {
+    if (MODE == ROQ_FP64) reinterpret_cast<double*>(aout)[o + k] = 0.0;
+    else if (MODE == ROQ_FP32) reinterpret_cast<float*>(aout)[o + k] = 0.f;
+    else {
+      reinterpret_cast<__bf16*>(aout)[o + k] = (__bf16)0.f;
+      reinterpret_cast<__bf16*>(aout_lo)[o + k] = (__bf16)0.f;
+    }
+  }
+  // linear nodes -> GEMM operand; the basis at each node is tabled
+  for (int l = tid; l < rd.L; l += 128) {
     double xr = 0.0, xi = 0.0;
-    if (valid && l < rd.L) {
+    if (valid) {
       const double f = rd.fl[l];
       if (f >= rd.wf_fmin && f <= fisco) {
-        double b[kNB];
-        tf2_basis(f, b);
+        const double* b = rd.nbl + (size_t)l * (kNB + 1);
         const double cyc = tf2_phase_cycles(c, b);
-        const double amp = b[1] * sqrt(b[3]);  // f^-7/6; amp0 sits in C_d
-        double sn, cs;
-        sincospi(2.0 * (cyc - floor(cyc)), &sn, &cs);
-        xr = amp * cs;   // Re conj(h0) = Re h0
-        xi = amp * sn;   // Im conj(h0) = -Im h0 = +amp sin
+        const double amp = b[kNB];  // f^-7/6; amp0 sits in C_d
+        if (MODE == ROQ_FP64) {
+          double sn, cs;
+          sincospi(2.0 * (cyc - floor(cyc)), &sn, &cs);
+          xr = amp * cs;   // Re conj(h0) = Re h0
+          xi = amp * sn;   // Im conj(h0) = -Im h0 = +amp sin
+        } else {
+          // float32 operand: the rotation in float32 from the reduced phase
+          const float2 e = expm2pi((float)(cyc - floor(cyc)));
+          xr = amp * (double)e.x;
+          xi = -amp * (double)e.y;
+        }
       }
     }
-    const size_t o = (size_t)r * rd.Kp;
     const int kl = l, ki = l + rd.L;
     if (MODE == ROQ_FP64) {
       double* a = reinterpret_cast<double*>(aout);
-      if (l < rd.L) {
-        a[o + kl] = xr;
-        a[o + ki] = xi;
-      }
+      a[o + kl] = xr;
+      a[o + ki] = xi;
     } else if (MODE == ROQ_FP32) {
       float* a = reinterpret_cast<float*>(aout);
-      if (l < rd.L) {
-        a[o + kl] = (float)xr;
-        a[o + ki] = (float)xi;
-      }
+      a[o + kl] = (float)xr;
+      a[o + ki] = (float)xi;
     } else {
       __bf16* ah = reinterpret_cast<__bf16*>(aout);
       __bf16* al = reinterpret_cast<__bf16*>(aout_lo);
-      if (l < rd.L) {
-        const float fr = (float)xr, fi = (float)xi;
-        const __bf16 hr = (__bf16)fr, hi = (__bf16)fi;
-        ah[o + kl] = hr;
-        ah[o + ki] = hi;
-        al[o + kl] = (__bf16)(fr - (float)hr);
-        al[o + ki] = (__bf16)(fi - (float)hi);
-      }
+      const float fr = (float)xr, fi = (float)xi;
+      const __bf16 hr = (__bf16)fr, hi = (__bf16)fi;
+      ah[o + kl] = hr;
+      ah[o + ki] = hi;
+      al[o + kl] = (__bf16)(fr - (float)hr);
+      al[o + ki] = (__bf16)(fi - (float)hi);
     }
   }
   // quadratic nodes -> <h|h>_d = |C_d|^2 sum_q |h0_q|^2 w_dq
@@ -182,9 +194,7 @@ __global__ __launch_bounds__(128) void k_roq_prep(RoqDev rd, const double* param
     for (int q = tid; q < rd.Qn; q += 128) {
       const double f = rd.fq[q];
       if (f >= rd.wf_fmin && f <= fisco) {
-        double b[kNB];
-        tf2_basis(f, b);
-        const double amp = b[1] * sqrt(b[3]);
+        const double amp = rd.nbq[(size_t)q * (kNB + 1) + kNB];
         for (int d = 0; d < rd.nd; ++d) acc[d] += amp * amp * rd.wq[(size_t)d * rd.Qn + q];
       }
     }
@@ -205,6 +215,16 @@ __global__ __launch_bounds__(128) void k_roq_prep(RoqDev rd, const double* param
       }
     rows[r].hh = hh;
   }
+}
+
+// TaylorF2 basis (tf2_basis order) and f^-7/6 at n frequency nodes.
+__global__ void k_roq_node_table(const double* f, int n, double* out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double b[kNB];
+  tf2_basis(f[i], b);
+  for (int k = 0; k < kNB; ++k) out[(size_t)i * (kNB + 1) + k] = b[k];
+  out[(size_t)i * (kNB + 1) + kNB] = b[1] * sqrt(b[3]);
 }
 
 // ------------------------------------------------------------------ GEMM kernels
@@ -390,11 +410,14 @@ __global__ __launch_bounds__(256) void k_gemm_f64(const double* __restrict__ A,
 template <typename T>
 __device__ inline void five_sample(const T* z, int base_c, double a, double& re, double& im) {
   // z: complex row (interleaved), base_c = complex index of sample closest - 2
+  using C = typename cplx_t<T>::type;
+  const C* zc = reinterpret_cast<const C*>(z) + base_c;
   double vr[5], vi[5];
 #pragma unroll
   for (int k = 0; k < 5; ++k) {
-    vr[k] = (double)z[2 * (base_c + k)];
-    vi[k] = (double)z[2 * (base_c + k) + 1];
+    const C v = zc[k];
+    vr[k] = (double)v.x;
+    vi[k] = (double)v.y;
   }
   const double b = 1.0 - a;
   const double c = (a * a * a - a) / 6.0, d = (b * b * b - b) / 6.0;
@@ -476,7 +499,9 @@ __global__ __launch_bounds__(64 * kRoqEpiRows) void k_roq_epi(RoqDev rd, const T
   auto dh_at = [&](double toff, double extra, bool single, bool& ok, double& sr, double& si) {
     sr = si = 0.0;
     ok = true;
-    for (int d = 0; d < rd.nd; ++d) {
+#pragma unroll
+    for (int d = 0; d < GWL_MAX_DET; ++d) {
+      if (d >= rd.nd) break;
       const double ifo_t = (toff + rr.dt[d]) + extra;
       const double x = (ifo_t - rd.ts0) / rd.dts;
       const double cl = floor(x);
@@ -634,7 +659,9 @@ int gwl_roq_create(gwl_ctx* ctx, const gwl_roq_desc* d, gwl_roq** out, int64_t* 
   if (!e) { set_error("out of host memory"); return GWL_E_NOMEM; }
   e->ctx = ctx;
   const size_t bytes =
-      nB * (8 + 4 + 2 + 2) + sizeof(double) * ((size_t)rd.L + rd.Qn + wq.size() + rd.NT) + 8 * 256;
+      nB * (8 + 4 + 2 + 2) +
+      sizeof(double) * ((size_t)rd.L + rd.Qn + wq.size() + rd.NT + ((size_t)rd.L + rd.Qn) * (kNB + 1)) +
+      12 * 256;
   if (hipMalloc(&e->data, bytes) != hipSuccess) {
     delete e;
     set_error("hipMalloc failed for ROQ data");
@@ -655,6 +682,8 @@ int gwl_roq_create(gwl_ctx* ctx, const gwl_roq_desc* d, gwl_roq** out, int64_t* 
   rd.fq = reinterpret_cast<double*>(take(rd.Qn * 8));
   rd.wq = reinterpret_cast<double*>(take(wq.size() * 8));
   rd.ts = reinterpret_cast<double*>(take((size_t)rd.NT * 8));
+  rd.nbl = reinterpret_cast<double*>(take((size_t)rd.L * (kNB + 1) * 8));
+  rd.nbq = reinterpret_cast<double*>(take((size_t)rd.Qn * (kNB + 1) * 8));
   hipError_t err = hipMemcpy((void*)rd.bt64, b64.data(), nB * 8, hipMemcpyHostToDevice);
   if (err == hipSuccess) err = hipMemcpy((void*)rd.bt32, b32.data(), nB * 4, hipMemcpyHostToDevice);
   if (err == hipSuccess) err = hipMemcpy((void*)rd.bth, bh.data(), nB * 2, hipMemcpyHostToDevice);
@@ -663,6 +692,14 @@ int gwl_roq_create(gwl_ctx* ctx, const gwl_roq_desc* d, gwl_roq** out, int64_t* 
   if (err == hipSuccess) err = hipMemcpy((void*)rd.fq, d->quadratic_nodes, rd.Qn * 8, hipMemcpyHostToDevice);
   if (err == hipSuccess) err = hipMemcpy((void*)rd.wq, wq.data(), wq.size() * 8, hipMemcpyHostToDevice);
   if (err == hipSuccess) err = hipMemcpy((void*)rd.ts, d->time_samples, (size_t)rd.NT * 8, hipMemcpyHostToDevice);
+  if (err == hipSuccess) {
+    hipLaunchKernelGGL(k_roq_node_table, dim3((unsigned)((rd.L + 127) / 128)), dim3(128), 0, 0, rd.fl,
+                       rd.L, const_cast<double*>(rd.nbl));
+    hipLaunchKernelGGL(k_roq_node_table, dim3((unsigned)((rd.Qn + 127) / 128)), dim3(128), 0, 0, rd.fq,
+                       rd.Qn, const_cast<double*>(rd.nbq));
+    err = hipGetLastError();
+    if (err == hipSuccess) err = hipDeviceSynchronize();
+  }
   if (err != hipSuccess) {
     (void)hipFree(e->data);
     delete e;
@@ -777,14 +814,13 @@ int gwl_roq_eval(gwl_roq* e, const double* params, int64_t n, int32_t ld, int32_
   char* Cp = p + (abytes + 255) / 256 * 256;
   const size_t cbytes = (mode == ROQ_FP64 ? (size_t)Mp * rd.Np * 8 : (size_t)Mp * rd.Np * 4);
   RoqRow* rows = reinterpret_cast<RoqRow*>(Cp + (cbytes + 255) / 256 * 256);
-  // zero the operand (padding rows and columns must be zero)
-  GWL_HIP_CHECK(hipMemsetAsync(A, 0, abytes, st));
+  // one block per operand row: rows past n and padding columns are written as zeros
   if (mode == ROQ_FP64)
-    hipLaunchKernelGGL(k_roq_prep<ROQ_FP64>, dim3((unsigned)n), dim3(128), 0, st, rd, params, ld, n, A, nullptr, rows);
+    hipLaunchKernelGGL(k_roq_prep<ROQ_FP64>, dim3((unsigned)Mp), dim3(128), 0, st, rd, params, ld, n, A, nullptr, rows);
   else if (mode == ROQ_FP32)
-    hipLaunchKernelGGL(k_roq_prep<ROQ_FP32>, dim3((unsigned)n), dim3(128), 0, st, rd, params, ld, n, A, nullptr, rows);
+    hipLaunchKernelGGL(k_roq_prep<ROQ_FP32>, dim3((unsigned)Mp), dim3(128), 0, st, rd, params, ld, n, A, nullptr, rows);
   else
-    hipLaunchKernelGGL(k_roq_prep<ROQ_BF16X3>, dim3((unsigned)n), dim3(128), 0, st, rd, params, ld, n, A, Alo, rows);
+    hipLaunchKernelGGL(k_roq_prep<ROQ_BF16X3>, dim3((unsigned)Mp), dim3(128), 0, st, rd, params, ld, n, A, Alo, rows);
   GWL_HIP_CHECK(hipGetLastError());
   if (mode == ROQ_FP64) {
     hipLaunchKernelGGL(k_gemm_f64, dim3(rd.Np / 64, Mp / 64), dim3(256), 0, st,
