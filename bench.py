@@ -385,6 +385,9 @@ def main():
     ap.add_argument("--tile", default="0,0",
                     help="rows per workgroup, workgroups (or wavefronts) per CU for --profile; "
                          "0,0 = the default kernel choice")
+    ap.add_argument("--transform", type=int, default=0,
+                    help="time transform of the headline event (include/gwl.h gwl_transform): "
+                         "0 automatic, 1 tiled, 2 wave per row, 3 chirp-z")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only; --gpus %d is not supported in this build "
@@ -410,7 +413,8 @@ def main():
     draws = fixed_marginalised(setup["draws"], priors)
     rows = torch.from_numpy(lk.rows_from_parameters(draws)).to(dev)
     check(LIB.gwl_event_reserve(lk._event, args.batch))
-    from bilby_amd.autotune import autotune, set_tile
+    from bilby_amd.autotune import autotune, set_tile, set_transform
+    set_transform(lk, args.transform)
     if args.profile:
         set_tile(lk, *[int(v) for v in args.tile.split(",")])
         for _ in range(args.warmup + args.steps):
@@ -464,9 +468,12 @@ def main():
     k_ms = tot.value / max(1, cnt.value)
     bpe = bytes_per_eval(lk)
     achieved = args.batch * bpe / (k_ms * 1e-3) / 1e9
-    wave = tile[0] == 0
-    kname = "k_time_w" if wave else "k_time"
-    traffic_key = ("k_time_w" if wave else "k_time_tiled") + "_taylorf2_tdp_batch%d" % args.batch
+    # the transform that ran: chirp-z only when asked for, else the tile choice (include/gwl.h)
+    chirp = args.transform == 3
+    wave = not chirp and (args.transform == 2 or (args.transform == 0 and tile[0] == 0))
+    kname = "k_time_c" if chirp else "k_time_w" if wave else "k_time"
+    traffic_key = ("k_time_c" if chirp else "k_time_w" if wave else "k_time_tiled") + \
+        "_taylorf2_tdp_batch%d" % args.batch
     traffic = measured_traffic(traffic_key)
     log("%s: %.3f ms/launch, %.1f GB/s algorithmic, measured traffic %s B/launch"
         % (kname, k_ms, achieved, traffic))
@@ -512,7 +519,10 @@ def main():
                    "batch": args.batch, "n_freq": 16385, "detectors": 3},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": 8000.0, "unit": "GB/s",
                      "frac": achieved / 8000.0, "traffic": traffic,
-                     "kernel": (kname + (" (wave per row: waveform + projection + register "
+                     "kernel": (kname + (" (chirp-z: waveform + projection + Bluestein "
+                                         "convolution in LDS + fused epilogue, one workgroup per "
+                                         "row)" if chirp else
+                                         " (wave per row: waveform + projection + register "
                                          "radix-8 transform + window + fused epilogue)" if wave
                                          else " (tiled: waveform + projection + LDS FFT)")),
                      "kernel_ms": k_ms, "bytes_per_eval": bpe,

@@ -106,6 +106,7 @@ def _load():
                       ctypes.c_int32, vp], ctypes.c_int),
         "gwl_event_timing_read": ([vp, _dp, ctypes.POINTER(ctypes.c_int64)], ctypes.c_int),
         "gwl_roq_reserve": ([vp, ctypes.c_int64], ctypes.c_int),
+        "gwl_event_set_transform": ([vp, ctypes.c_int32], ctypes.c_int),
         "gwl_server_create": ([vp, ctypes.c_int64, ctypes.POINTER(vp)], ctypes.c_int),
         "gwl_server_destroy": ([vp], ctypes.c_int),
         "gwl_server_add_event": ([vp, vp, ctypes.c_int32, ctypes.POINTER(ctypes.c_int32)],
@@ -159,5 +160,5 @@ EXPORTED_SYMBOLS = [
     "gwl_multi_launch", "gwl_multi_capture", "gwl_multi_destroy", "gwl_event_device_bytes",
     "gwl_roq_reserve", "gwl_server_create", "gwl_server_destroy", "gwl_server_add_event",
     "gwl_server_add_roq", "gwl_server_add_group", "gwl_server_submit", "gwl_server_wait",
-    "gwl_server_stats", "gwl_server_device_bytes",
+    "gwl_server_stats", "gwl_server_device_bytes", "gwl_event_set_transform",
 ]

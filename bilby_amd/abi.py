@@ -16,3 +16,5 @@ SINE_GAUSSIAN, TAYLORF2 = 1, 2
 TIME_MARG, DISTANCE_MARG, PHASE_MARG, JITTER_TIME = 1, 2, 4, 8
 FP64, FP32 = 0, 1
 
+# enum gwl_transform (time-marginalised TaylorF2 transform choice)
+TRANSFORM_AUTO, TRANSFORM_TILED, TRANSFORM_WAVE, TRANSFORM_CHIRP = 0, 1, 2, 3

@@ -58,6 +58,11 @@ def set_tile(likelihood, rows_per_workgroup, workgroups_per_cu):
                                  int(workgroups_per_cu)), "gwl_event_set_tile")
 
 
+def set_transform(likelihood, transform):
+    """0 automatic, 1 tiled, 2 wave per row, 3 chirp-z (bilby_amd.abi)."""
+    check(LIB.gwl_event_set_transform(likelihood._event, int(transform)), "gwl_event_set_transform")
+
+
 def time_tile(likelihood, rows_t, tile, repeats=5, precision=None):
     """Average milliseconds of one batched evaluation with ``tile``."""
     set_tile(likelihood, *tile)

@@ -76,6 +76,11 @@ struct EvDev {
   const float2* dwa32;  // [nd][N] dw32 f^-7/6, decimated order
   const double* hps;    // [nd][N + 2] prefix sums of f^-7/3 sigma^2 / S over bins
                         //   0..N in natural order, hps[d][n] = sum of bins < n
+  // the chirp-z transform: natural-order basis and data, and the spectra of
+  // the chirp kernel for M = 1024, 2048, 4096 (digit-reversed order, / M)
+  const double* tf2n;   // [N][3]
+  const float2* dwan;   // [nd][N]
+  const float2* chirpv[3];
   // Nyquist bin (n = N), kept apart from the power-of-two FFT range
   double dwN[GWL_MAX_DET][2];
   double wN[GWL_MAX_DET];
@@ -644,6 +649,17 @@ __device__ inline double spline_x(const EvDev& ev, const double* tx, const Splin
   }
   return s;
 }
+
+// Per-row constants of the TaylorF2 time transforms (k_rowconst).
+struct RowW {
+  double c[9];                 // phase coefficients (cycles), tf2_basis order
+  double dt[GWL_MAX_DET];      // detector time shifts [s]
+  float cre[GWL_MAX_DET];      // complex detector amplitude / sigma
+  float cim[GWL_MAX_DET];
+  double hh;                   // sum over the support of |h|^2 sigma^2 / S
+  double hscale, jitter;
+  int kb, ke, valid, pad;
+};
 
 // Marginalised lnL at one time (base.py:775-792): distance spline (y-part
 // precomputed per row), else ln I0 (phase) or the real part.

@@ -32,6 +32,7 @@ struct gwl_event {
   // tile overrides (0 = heuristic)
   int tile_rows = 0;
   int tile_wg_per_cu = 0;
+  int transform = GWL_TRANSFORM_AUTO;
   // kernel timing (measurement only)
   bool timing = false;
   hipEvent_t t0 = nullptr, t1 = nullptr;
@@ -119,6 +120,15 @@ int init_time_wave_tables(const EvDev& ev, double* tf2b, float2* dwa32);
 int launch_time_wave(gwl_event* ev, const double* params, int64_t n, int ld, double* out,
                      hipStream_t stream);
 size_t roww_bytes(int64_t rows);
+// chirp-z TaylorF2 time transform (gwl_time_chirp.hip)
+bool time_chirp_eligible(const gwl_event* ev, long long n);
+void chirp_spectrum(int N, int L, int M, float2* out);
+int init_chirp_tables(const EvDev& ev, double* tf2n, float2* dwan);
+size_t time_chirp_lds_bytes(const EvDev& ev);
+int launch_chirp_kernel(const EvDev& ev, const RowW* rows, long long n, double* out, int* fallback,
+                        hipStream_t stream);
+int launch_time_chirp(gwl_event* ev, const double* params, int64_t n, int ld, double* out,
+                      hipStream_t stream);
 size_t multi_wave_lds_bytes();
 int multi_launch_wave(gwl_multi* m, hipStream_t stream);
 }  // namespace gwl

@@ -28,16 +28,6 @@
 
 namespace gwl {
 
-struct RowW {
-  double c[9];                 // phase coefficients (cycles), tf2_basis order
-  double dt[GWL_MAX_DET];      // detector time shifts [s]
-  float cre[GWL_MAX_DET];      // complex detector amplitude / sigma
-  float cim[GWL_MAX_DET];
-  double hh;                   // sum over the support of |h|^2 sigma^2 / S
-  double hscale, jitter;
-  int kb, ke, valid, pad;
-};
-
 constexpr int kWP = 512;      // transform length
 constexpr int kWaves = 4;     // wavefronts (rows) per workgroup
 constexpr int kXch = 576;     // per-wave exchange buffer, padded (float2)
@@ -186,6 +176,7 @@ struct TimeWArgs {
   double2* part;     // [n][G][L] window partials (G > 1)
   double* hhpart;    // [n][G]
   double* out;
+  const int* only;   // if set: rows r with only[r] == 0 are skipped
 };
 
 // The transform of one row over sub-sequences [n2b, n2e): lane l returns
@@ -307,6 +298,7 @@ __global__ __launch_bounds__(64 * kWaves, 4) void k_time_w(TimeWArgs a) {
   const long long r = item / a.G;
   const int g = (int)(item - r * a.G);
   if (r >= a.n) return;
+  if (a.only && !a.only[r]) return;
   const RowW& rw = a.rows[r];
   const int Q = ev.Q;
   const int n2b = (int)(((long long)g * Q) / a.G);
@@ -424,8 +416,10 @@ __global__ __launch_bounds__(256) void k_tf2_tables(EvDev ev, double* tb, float2
 
 bool time_wave_eligible(const gwl_event* e) {
   const EvDev& ev = e->dev;
-  return (ev.flags & GWL_TIME_MARG) && ev.approx == GWL_TAYLORF2 && ev.P == kWP && ev.tf2b &&
-         e->tile_rows == 0;
+  if (!(ev.flags & GWL_TIME_MARG) || ev.approx != GWL_TAYLORF2 || ev.P != kWP || !ev.tf2b)
+    return false;
+  if (e->transform == GWL_TRANSFORM_WAVE) return true;
+  return e->transform == GWL_TRANSFORM_AUTO && e->tile_rows == 0;
 }
 
 int time_wave_groups(const gwl_event* e, long long n) {
@@ -500,6 +494,48 @@ int launch_time_wave(gwl_event* e, const double* params, int64_t n, int ld, doub
 }
 
 size_t roww_bytes(int64_t rows) { return (size_t)rows * sizeof(RowW); }
+
+// Chirp-z path: row constants, one workgroup per row, then the wave-per-row
+// transform (one group, fused epilogue) for the rows flagged as too wide.
+int launch_time_chirp(gwl_event* e, const double* params, int64_t n, int ld, double* out,
+                      hipStream_t st) {
+  const EvDev& ev = e->dev;
+  char* ws = reinterpret_cast<char*>(e->ws);
+  RowW* rows = reinterpret_cast<RowW*>(ws);
+  int* fallback = reinterpret_cast<int*>(ws + ((size_t)n * sizeof(RowW) + 255) / 256 * 256);
+  if ((char*)(fallback + n) > ws + e->ws_bytes) {
+    set_error("workspace too small for the time transform");
+    return GWL_E_INVALID;
+  }
+  RowArgs ra{ev, params, ld, n, rows};
+  hipLaunchKernelGGL(k_rowconst, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, ra);
+  GWL_HIP_CHECK(hipGetLastError());
+  if (time_chirp_lds_bytes(ev) > e->ctx->lds_per_block) {
+    set_error("chirp transform exceeds the LDS of a workgroup");
+    return GWL_E_UNSUPPORTED;
+  }
+  if (int rc = timing_begin(e, st)) return rc;
+  if (int rc = launch_chirp_kernel(ev, rows, n, out, fallback, st)) return rc;
+  if (int rc = timing_end(e, st)) return rc;
+  TimeWArgs ta{};
+  ta.ev = ev;
+  ta.rows = rows;
+  ta.n = n;
+  ta.G = 1;
+  ta.fused = 1;
+  ta.out = out;
+  ta.only = fallback;
+  const size_t lds = time_wave_lds_bytes(ev, true);
+  const dim3 grid((unsigned)((n + kWaves - 1) / kWaves));
+  switch (ev.nd) {
+    case 1: hipLaunchKernelGGL(k_time_w<1>, grid, dim3(64 * kWaves), lds, st, ta); break;
+    case 2: hipLaunchKernelGGL(k_time_w<2>, grid, dim3(64 * kWaves), lds, st, ta); break;
+    case 3: hipLaunchKernelGGL(k_time_w<3>, grid, dim3(64 * kWaves), lds, st, ta); break;
+    default: hipLaunchKernelGGL(k_time_w<4>, grid, dim3(64 * kWaves), lds, st, ta); break;
+  }
+  GWL_HIP_CHECK(hipGetLastError());
+  return GWL_OK;
+}
 
 size_t multi_wave_lds_bytes() { return (size_t)(kWP + kTw3 + kWaves * kXch) * sizeof(float2); }
 

@@ -143,6 +143,15 @@ int gwl_eval(gwl_event* ev, const double* params, int64_t n_rows, int32_t ld,
  * counterpart in the reference. */
 int gwl_event_set_tile(gwl_event* ev, int32_t rows_per_workgroup, int32_t workgroups_per_cu);
 
+/* Transform of the time-marginalised TaylorF2 path: automatic (the
+ * wave-per-row or tiled kernel as gwl_event_set_tile selects) or one of
+ * them forced; the chirp-z (Bluestein zoom) transform runs only when forced.
+ * No counterpart in the reference (kernel choice). */
+enum gwl_transform {
+  GWL_TRANSFORM_AUTO = 0, GWL_TRANSFORM_TILED = 1, GWL_TRANSFORM_WAVE = 2, GWL_TRANSFORM_CHIRP = 3
+};
+int gwl_event_set_transform(gwl_event* ev, int32_t transform);
+
 /* Kernel timing for measurement: while enabled, every gwl_eval on this event
  * brackets its dominant kernel (the fused inner-product or time-marginalised
  * kernel) with HIP events on the launch stream and waits for them; the

@@ -37,7 +37,7 @@ def test_wave_paths_agree_with_tiled_and_oracle(masses, flags):
     """fused (G = 1) and split (G > 1) wave paths vs the tiled kernel:
     |dlnL| < 2e-4 (float32 transforms summed in different orders); each vs
     the FP64 oracle on 24 rows: < 1e-3."""
-    from bilby_amd.autotune import set_tile
+    from bilby_amd.autotune import set_tile, set_transform
     from bilby_amd.likelihood import GravitationalWaveTransient
     from tests.helpers import draws_as_dicts, fixed_marginalised, oracle_from_setup
     say("time wave", masses, flags)
@@ -49,29 +49,38 @@ def test_wave_paths_agree_with_tiled_and_oracle(masses, flags):
                                     phase_marginalization=flags["phase"], priors=priors,
                                     precision="fp32")
     draws = fixed_marginalised(s["draws"], priors)
+    set_transform(lk, 2)
     set_tile(lk, 0, 1)      # wave path, one group per row for 160 rows: fused
     fused = lk.log_likelihood_ratio_batch(draws)
     set_tile(lk, 0, 64)     # wave path, several groups per row: split
     split = lk.log_likelihood_ratio_batch(draws)
+    set_transform(lk, 3)    # chirp-z transform, one workgroup per row
+    set_tile(lk, 0, 0)
+    chirp = lk.log_likelihood_ratio_batch(draws)
+    set_transform(lk, 1)
     set_tile(lk, 8, 2)      # tiled kernel
     tiled = lk.log_likelihood_ratio_batch(draws)
+    set_transform(lk, 0)
     set_tile(lk, 0, 0)
     n = 24
     olk, _ = oracle_from_setup(s)
     ref = np.array([olk.log_likelihood_ratio(p) for p in draws_as_dicts(draws, range(n))])
     say("fused-tiled", np.max(np.abs(fused - tiled)), "split-tiled", np.max(np.abs(split - tiled)),
-        "fused-oracle", np.max(np.abs(fused[:n] - ref)), "lnL range", ref.min(), ref.max())
+        "chirp-tiled", np.max(np.abs(chirp - tiled)), "fused-oracle", np.max(np.abs(fused[:n] - ref)),
+        "chirp-oracle", np.max(np.abs(chirp[:n] - ref)), "lnL range", ref.min(), ref.max())
     assert np.all(np.isfinite(ref))
     assert np.max(np.abs(fused - tiled)) < 2e-4
     assert np.max(np.abs(split - tiled)) < 2e-4
+    assert np.max(np.abs(chirp - tiled)) < 2e-4
     assert np.max(np.abs(fused[:n] - ref)) < 1e-3
     assert np.max(np.abs(split[:n] - ref)) < 1e-3
+    assert np.max(np.abs(chirp[:n] - ref)) < 1e-3
 
 
 def test_wave_invalid_rows_and_repeat():
     """Invalid rows get nan_to_num(-inf) on the fused path without touching
     their neighbours, and a repeated call gives the same bits."""
-    from bilby_amd.autotune import set_tile
+    from bilby_amd.autotune import set_tile, set_transform
     from bilby_amd.likelihood import GravitationalWaveTransient
     from tests.helpers import fixed_marginalised
     s = _setup((5.0, 50.0), dict(distance=True, phase=True), n_draws=64)
@@ -79,13 +88,17 @@ def test_wave_invalid_rows_and_repeat():
     lk = GravitationalWaveTransient(s["interferometers"], s["waveform_generator"],
                                     time_marginalization=True, distance_marginalization=True,
                                     phase_marginalization=True, priors=priors)
-    set_tile(lk, 0, 1)
-    d = {k: v.copy() for k, v in fixed_marginalised(s["draws"], priors).items()}
-    good = lk.log_likelihood_ratio_batch(d)
-    again = lk.log_likelihood_ratio_batch(d)
-    assert np.array_equal(good, again)
-    d["mass_1"][7] = -3.0
-    bad = lk.log_likelihood_ratio_batch(d)
-    assert bad[7] == np.nan_to_num(-np.inf)
-    keep = [i for i in range(64) if i != 7]
-    assert np.array_equal(bad[keep], good[keep])
+    d0 = {k: v.copy() for k, v in fixed_marginalised(s["draws"], priors).items()}
+    for transform in (2, 3):
+        set_transform(lk, transform)
+        set_tile(lk, 0, 1)
+        d = {k: v.copy() for k, v in d0.items()}
+        good = lk.log_likelihood_ratio_batch(d)
+        again = lk.log_likelihood_ratio_batch(d)
+        assert np.array_equal(good, again)
+        d["mass_1"][7] = -3.0
+        bad = lk.log_likelihood_ratio_batch(d)
+        assert bad[7] == np.nan_to_num(-np.inf)
+        keep = [i for i in range(64) if i != 7]
+        assert np.array_equal(bad[keep], good[keep])
+    set_transform(lk, 0)
