@@ -292,6 +292,53 @@ def bench_roq(steps):
     return out
 
 
+def bench_sampler(lk, draws, seconds, workers=16, queue=1024):
+    """bilby-style callers of the headline event: (i) a dynesty-style queue of
+    ``queue`` proposals through BatchedPool.map (one batched call per queue,
+    host conversion and copies included); (ii) ``workers`` spawned sampler
+    processes, each calling RemoteLikelihood.log_likelihood_ratio one dict at
+    a time (bilby's pool workers, base_sampler.py:772-800) against a
+    LikelihoodService that owns the GPU."""
+    import multiprocessing
+    import numpy as np
+    from bilby_amd.sampling import (BatchedPool, LikelihoodService, client_load,
+                                    log_likelihood_wrapper)
+    keys = ["mass_1", "mass_2", "ra", "dec", "theta_jn", "psi"]
+    fixed = {k: float(np.asarray(v)[0]) for k, v in draws.items() if k not in keys}
+    thetas = np.stack([np.asarray(draws[k], float) for k in keys], axis=1)
+    pool = BatchedPool(lk, None, keys, True, fixed)
+    pool.map(log_likelihood_wrapper, thetas[:queue])
+    n_calls, t0 = 0, time.perf_counter()
+    while time.perf_counter() - t0 < 2.0:
+        k = (n_calls * queue) % (len(thetas) - queue + 1)
+        pool.map(log_likelihood_wrapper, thetas[k:k + queue])
+        n_calls += 1
+    t_pool = time.perf_counter() - t0
+    out = {"batched_pool_queue": queue,
+           "batched_pool_evals_per_s": n_calls * queue / t_pool,
+           "batched_pool_ms_per_queue": 1e3 * t_pool / n_calls}
+    one = {k: np.asarray(v)[:1] for k, v in draws.items()}
+    with LikelihoodService({"c2": lk}, hbm_budget=200e9, max_batch_rows=4096) as svc:
+        client = svc.client("c2")
+        client.log_likelihood_ratio({k: float(v[0]) for k, v in one.items()})
+        ctx = multiprocessing.get_context("spawn")
+        jobs = [(client, draws, seconds, 97 * w) for w in range(workers)]
+        with ctx.Pool(workers) as wp:
+            res = wp.starmap(client_load, jobs, chunksize=1)
+    lat = np.concatenate([r[1] for r in res])
+    n = sum(r[0] for r in res)
+    idx = sorted({i for r in res for i in r[2]})
+    direct = lk.log_likelihood_ratio_batch({k: np.asarray(v)[idx] for k, v in draws.items()})
+    served = {i: v for r in res for i, v in r[2].items()}
+    out.update(workers=workers, worker_seconds=seconds, worker_calls=n,
+               worker_evals_per_s=sum(r[0] for r in res) / seconds,
+               worker_p50_ms=1e3 * float(np.percentile(lat, 50)),
+               worker_p99_ms=1e3 * float(np.percentile(lat, 99)),
+               worker_vs_direct_max=float(np.max(np.abs(
+                   np.array([served[i] for i in idx]) - direct))))
+    return out
+
+
 def bench_serving(main_lk, main_rows, duration):
     """32 samplers, seeded Poisson arrivals at 2000 requests/s, 1-1024 rows per
     request from a 60/30/10 mix of the TaylorF2 T+D+P event, uneven events and
@@ -376,6 +423,8 @@ def main():
                     help="seconds of the process-pool CPU baseline (0 = skip)")
     ap.add_argument("--no-extra", action="store_true", help="headline workload only")
     ap.add_argument("--serve-seconds", type=float, default=5.0)
+    ap.add_argument("--sampler-seconds", type=float, default=3.0,
+                    help="seconds each of the 16 single-dict client processes runs")
     ap.add_argument("--only", default="",
                     help="comma-separated substrings: run only the extra workloads whose name "
                          "contains one of them")
@@ -547,7 +596,9 @@ def main():
                          ("uneven_64_events_persistent_graph", lambda: bench_uneven(5)),
                          ("roq_400_120_nodes_1000_times_65536", lambda: bench_roq(3)),
                          ("serving_32_samplers_2000_rps", lambda: bench_serving(
-                             lk, rows.cpu().numpy(), args.serve_seconds))):
+                             lk, rows.cpu().numpy(), args.serve_seconds)),
+                         ("sampler_callers_pool_and_16_processes", lambda: bench_sampler(
+                             lk, draws, args.sampler_seconds))):
             if args.only and not any(k in name for k in args.only.split(",")):
                 continue
             log("extra:", name)

@@ -121,6 +121,7 @@ def _load():
         "gwl_server_stats": ([vp, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64)],
                              ctypes.c_int),
         "gwl_server_device_bytes": ([vp, ctypes.POINTER(ctypes.c_int64)], ctypes.c_int),
+        "gwl_server_listen": ([vp, ctypes.c_char_p], ctypes.c_int),
     }
     for name, (argt, rest) in sig.items():
         fn = getattr(lib, name)
@@ -161,4 +162,5 @@ EXPORTED_SYMBOLS = [
     "gwl_roq_reserve", "gwl_server_create", "gwl_server_destroy", "gwl_server_add_event",
     "gwl_server_add_roq", "gwl_server_add_group", "gwl_server_submit", "gwl_server_wait",
     "gwl_server_stats", "gwl_server_device_bytes", "gwl_event_set_transform",
+    "gwl_server_listen",
 ]

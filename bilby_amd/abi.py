@@ -1,5 +1,7 @@
 """Constants of the libgwl C ABI (include/gwl.h), importable without the
-library."""
+library, and the host-side conversion of parameter dicts to its rows."""
+import numpy as np
+
 NPARAM = 16
 MAX_DET = 4
 
@@ -18,3 +20,33 @@ FP64, FP32 = 0, 1
 
 # enum gwl_transform (time-marginalised TaylorF2 transform choice)
 TRANSFORM_AUTO, TRANSFORM_TILED, TRANSFORM_WAVE, TRANSFORM_CHIRP = 0, 1, 2, 3
+
+# gwl_server_listen wire magics
+WIRE_REQUEST, WIRE_REPLY = 0x51574C47, 0x52574C47
+
+
+class RowBuilder:
+    """Parameter dict (scalars or 1-D arrays, bilby names) -> float64 rows
+    [n, NPARAM] in the column order of enum gwl_param: the waveform
+    generator's parameter conversion (waveform_generator.py:260-269), then
+    the defaults of the marginalised keys, then the columns.  Host-only and
+    picklable, so a sampler's worker process can build rows itself."""
+
+    def __init__(self, parameter_conversion, defaults):
+        self.parameter_conversion = parameter_conversion
+        self.defaults = dict(defaults)
+
+    def __call__(self, parameters):
+        p = self.parameter_conversion(dict(parameters))
+        for key, value in self.defaults.items():
+            p.setdefault(key, value)
+        n = 1
+        for v in p.values():
+            if np.ndim(v) > 0:
+                n = len(v)
+                break
+        rows = np.zeros((n, NPARAM))
+        for key, col in PARAM_COLUMNS.items():
+            if key in p:
+                rows[:, col] = np.asarray(p[key], dtype=float)
+        return rows

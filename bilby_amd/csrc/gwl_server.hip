@@ -17,6 +17,12 @@
 // bilby_amd.serving.
 #include "gwl_internal.h"
 
+#include <sys/socket.h>
+#include <sys/stat.h>
+#include <sys/un.h>
+#include <unistd.h>
+
+#include <cerrno>
 #include <chrono>
 #include <condition_variable>
 #include <cstring>
@@ -81,6 +87,18 @@ struct gwl_server {
   int64_t batches = 0, rows_done = 0;
   int64_t device_bytes = 0;  // staging and group buffers held by the server
   std::string last_error;
+  // socket front-end (gwl_server_listen)
+  int listen_fd = -1;
+  std::string sock_path;
+  std::thread accept_th;
+  std::mutex conn_mu;
+  std::vector<struct SockConn*> conns;
+  bool closing = false;
+};
+
+struct SockConn {
+  int fd = -1;
+  std::thread th;
 };
 
 namespace {
@@ -258,6 +276,121 @@ int add_unit(gwl_server* s, Unit* u, int members, int32_t* first_id) {
   return GWL_OK;
 }
 
+// ---------------------------------------------------------------- socket front-end
+// Wire format (native byte order, both directions a 16-byte header):
+//   request  {u32 magic GWL_WIRE_REQUEST, i32 id, i64 n} + n x 16 float64 rows
+//   reply    {u32 magic GWL_WIRE_REPLY, i32 status, i64 n} + n float64 values
+//            (status 0) or n bytes of error text (status != 0)
+struct WireHeader {
+  uint32_t magic;
+  int32_t word;
+  int64_t n;
+};
+static_assert(sizeof(WireHeader) == 16, "wire header is 16 bytes");
+
+bool read_full(int fd, void* buf, size_t len) {
+  char* p = static_cast<char*>(buf);
+  while (len > 0) {
+    const ssize_t k = recv(fd, p, len, 0);
+    if (k > 0) { p += k; len -= (size_t)k; continue; }
+    if (k < 0 && errno == EINTR) continue;
+    return false;
+  }
+  return true;
+}
+
+bool write_full(int fd, const void* buf, size_t len) {
+  const char* p = static_cast<const char*>(buf);
+  while (len > 0) {
+    const ssize_t k = send(fd, p, len, MSG_NOSIGNAL);
+    if (k > 0) { p += k; len -= (size_t)k; continue; }
+    if (k < 0 && errno == EINTR) continue;
+    return false;
+  }
+  return true;
+}
+
+bool reply_error(int fd, int status, const std::string& msg) {
+  const WireHeader h{GWL_WIRE_REPLY, status, (int64_t)msg.size()};
+  return write_full(fd, &h, sizeof h) && write_full(fd, msg.data(), msg.size());
+}
+
+// One connected client: its requests are served in order, each through the
+// same queue as gwl_server_submit, so rows from many clients share launches.
+void conn_loop(gwl_server* s, SockConn* c) {
+  std::vector<double> rows, out;
+  const int fd = c->fd;
+  for (;;) {
+    WireHeader h;
+    if (!read_full(fd, &h, sizeof h)) break;
+    if (h.magic != GWL_WIRE_REQUEST || h.n < 1 || h.n > s->max_rows) {
+      reply_error(fd, GWL_E_INVALID, "gwl_server socket: bad request header (magic, or n outside "
+                                     "1..max_rows)");
+      break;  // the stream cannot be resynchronised
+    }
+    rows.resize((size_t)h.n * GWL_NPARAM);
+    out.resize((size_t)h.n);
+    if (!read_full(fd, rows.data(), rows.size() * 8)) break;
+    int64_t ticket = 0;
+    int rc = gwl_server_submit(s, h.word, rows.data(), h.n, out.data(), &ticket);
+    if (rc == GWL_OK) rc = gwl_server_wait(s, ticket, nullptr);
+    if (rc != GWL_OK) {
+      if (!reply_error(fd, rc, gwl_last_error())) break;
+      continue;
+    }
+    const WireHeader r{GWL_WIRE_REPLY, 0, h.n};
+    if (!write_full(fd, &r, sizeof r) || !write_full(fd, out.data(), out.size() * 8)) break;
+  }
+  std::lock_guard<std::mutex> lk(s->conn_mu);
+  close(c->fd);
+  c->fd = -1;
+}
+
+void accept_loop(gwl_server* s) {
+  for (;;) {
+    const int fd = accept4(s->listen_fd, nullptr, nullptr, SOCK_CLOEXEC);
+    if (fd < 0) {
+      if (errno == EINTR || errno == ECONNABORTED) continue;
+      return;  // the listening socket was shut down
+    }
+    std::lock_guard<std::mutex> lk(s->conn_mu);
+    if (s->closing) {
+      close(fd);
+      return;
+    }
+    SockConn* c = new SockConn;
+    c->fd = fd;
+    s->conns.push_back(c);
+    c->th = std::thread(conn_loop, s, c);
+  }
+}
+
+// Stop accepting, end every connection (a request in flight is finished by
+// the scheduler first), remove the socket file.
+void close_socket(gwl_server* s) {
+  if (s->listen_fd < 0) return;
+  {
+    std::lock_guard<std::mutex> lk(s->conn_mu);
+    s->closing = true;
+  }
+  shutdown(s->listen_fd, SHUT_RDWR);
+  if (s->accept_th.joinable()) s->accept_th.join();
+  close(s->listen_fd);
+  s->listen_fd = -1;
+  unlink(s->sock_path.c_str());
+  std::vector<SockConn*> conns;
+  {
+    std::lock_guard<std::mutex> lk(s->conn_mu);
+    conns.swap(s->conns);
+    for (SockConn* c : conns)
+      if (c->fd >= 0) shutdown(c->fd, SHUT_RDWR);
+  }
+  for (SockConn* c : conns) {
+    if (c->th.joinable()) c->th.join();
+    delete c;
+  }
+}
+
 }  // namespace
 
 using gwl::set_error;
@@ -280,6 +413,7 @@ int gwl_server_create(gwl_ctx* ctx, int64_t max_rows, gwl_server** out) {
 
 int gwl_server_destroy(gwl_server* s) {
   if (!s) return GWL_OK;
+  close_socket(s);
   {
     std::lock_guard<std::mutex> lk(s->mu);
     s->stop = true;
@@ -420,6 +554,37 @@ int gwl_server_wait(gwl_server* s, int64_t ticket, double* latency_s) {
   if (latency_s) *latency_s = t.latency;
   if (t.status != GWL_OK) set_error(s->last_error);
   return t.status;
+}
+
+int gwl_server_listen(gwl_server* s, const char* path) {
+  sockaddr_un addr;
+  std::memset(&addr, 0, sizeof addr);
+  if (!s || !path || !*path || std::strlen(path) >= sizeof(addr.sun_path)) {
+    set_error("gwl_server_listen: invalid argument (path empty or longer than a socket path)");
+    return GWL_E_INVALID;
+  }
+  if (s->listen_fd >= 0) {
+    set_error("gwl_server_listen: already listening");
+    return GWL_E_INVALID;
+  }
+  addr.sun_family = AF_UNIX;
+  std::strcpy(addr.sun_path, path);
+  const int fd = socket(AF_UNIX, SOCK_STREAM | SOCK_CLOEXEC, 0);
+  if (fd < 0) {
+    set_error(std::string("gwl_server_listen: socket: ") + std::strerror(errno));
+    return GWL_E_INVALID;
+  }
+  if (bind(fd, (const sockaddr*)&addr, sizeof addr) != 0 || chmod(path, 0600) != 0 ||
+      listen(fd, 256) != 0) {
+    set_error(std::string("gwl_server_listen: ") + path + ": " + std::strerror(errno));
+    close(fd);
+    return GWL_E_INVALID;
+  }
+  s->listen_fd = fd;
+  s->sock_path = path;
+  s->closing = false;
+  s->accept_th = std::thread(accept_loop, s);
+  return GWL_OK;
 }
 
 int gwl_server_stats(gwl_server* s, int64_t* batches, int64_t* rows) {

@@ -172,16 +172,64 @@ class UniformComovingVolume(Interped):
         super().__init__(dl, p_dl, minimum=minimum, maximum=maximum, name=name)
 
 
+class Constraint(Prior):
+    """bilby/core/prior/base.py:495-504: the indicator minimum < value < maximum
+    (a constraint on a converted parameter, never sampled)."""
+
+    def __init__(self, minimum, maximum, name=None, **kw):
+        super().__init__(minimum=minimum, maximum=maximum, name=name)
+
+    @property
+    def is_fixed(self):
+        return True
+
+    def prob(self, val):
+        val = np.asarray(val)
+        return (val > self.minimum) & (val < self.maximum)
+
+
+def _no_conversion(sample):
+    """bilby/core/prior/dict.py:75-89 (default_conversion_function)."""
+    return sample
+
+
 class PriorDict(dict):
+    """Subset of bilby/core/prior/dict.py: priors by name, an optional
+    conversion function for constraints, ``sample``, ``rescale`` and
+    ``evaluate_constraints`` (what the samplers' likelihood wrappers call)."""
+
+    def __init__(self, dictionary=None, conversion_function=None, **priors):
+        super().__init__()
+        if dictionary:
+            self.update(dictionary)
+        self.update(priors)
+        self.conversion_function = conversion_function or _no_conversion
+
     def copy(self):
-        return PriorDict(self)
+        return PriorDict(self, conversion_function=self.conversion_function)
 
     def sample(self, size, rng=None):
         rng = rng if rng is not None else np.random.default_rng()
         out = {}
         for k, p in self.items():
+            if isinstance(p, Constraint):
+                continue
             if isinstance(p, Prior):
                 out[k] = p.sample(size, rng)
             else:
                 out[k] = np.full(size, float(p))
         return out
+
+    def rescale(self, keys, theta):
+        """dict.py:647-666: unit-cube values -> prior values, key by key."""
+        return np.asarray([self[key].rescale(t) for key, t in zip(keys, list(theta))])
+
+    def evaluate_constraints(self, sample):
+        """dict.py:64-73: product of every Constraint's indicator over the
+        converted sample (scalars or arrays); True where no constraint applies."""
+        out = self.conversion_function(dict(sample))
+        prob = np.ones_like(np.asarray(next(iter(out.values())), dtype=float), dtype=bool)
+        for key, p in self.items():
+            if isinstance(p, Constraint) and key in out:
+                prob = prob & p.prob(out[key])
+        return prob

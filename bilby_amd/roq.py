@@ -14,6 +14,7 @@ import numpy as np
 import torch
 
 from . import _lib
+from . import abi as _abi
 from ._lib import LIB, check, dptr
 from .detector import InterferometerList
 from .likelihood import distance_interpolant, get_context
@@ -164,29 +165,27 @@ class ROQGravitationalWaveTransient:
     def marginalized_parameters(self):
         return self._marginalized_parameters
 
-    def _fill_defaults(self, p):
+    def _defaults(self):
+        """Values the marginalised keys take when a dict omits them."""
+        d = {}
         if self.time_marginalization:
-            p.setdefault("geocent_time", float(self.interferometers.start_time))
-            p.setdefault("time_jitter", 0.0)
+            d["geocent_time"] = float(self.interferometers.start_time)
+            d["time_jitter"] = 0.0
         if self.phase_marginalization:
-            p.setdefault("phase", 0.0)
+            d["phase"] = 0.0
         if self.distance_marginalization:
-            p.setdefault("luminosity_distance", self._ref_dist)
-        return p
+            d["luminosity_distance"] = self._ref_dist
+        return d
+
+    def row_builder(self):
+        """The host-side dict -> rows conversion of this likelihood
+        (picklable; what worker-process clients use)."""
+        return _abi.RowBuilder(self.waveform_generator.parameter_conversion, self._defaults())
 
     def rows_from_parameters(self, parameters):
-        p = self.waveform_generator.parameter_conversion(dict(parameters))
-        p = self._fill_defaults(p)
-        n = 1
-        for v in p.values():
-            if np.ndim(v) > 0:
-                n = len(v)
-                break
-        rows = np.zeros((n, _lib.NPARAM))
-        for key, col in _lib.PARAM_COLUMNS.items():
-            if key in p:
-                rows[:, col] = np.asarray(p[key], dtype=float)
-        return rows
+        """Parameter dict (scalars or 1-D arrays) -> float64 rows [n, 16] in
+        the column order of include/gwl.h (enum gwl_param)."""
+        return self.row_builder()(parameters)
 
     def log_likelihood_ratio_batch(self, parameters, precision=None, stream=None):
         mode = self.precision if precision is None else (

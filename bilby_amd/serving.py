@@ -16,6 +16,7 @@ bilby's own parallelism is a process pool of single-dict callers
 device-resident likelihood per event through this queue.
 """
 import ctypes
+import os
 import threading
 import time
 
@@ -169,6 +170,11 @@ class LikelihoodServer:
         fut = ServedResult(self, t.value, event, rows, out)
         self._outstanding[t.value] = fut
         return fut
+
+    def listen(self, path):
+        """Also serve other processes on a Unix socket at ``path``
+        (gwl_server_listen; bilby_amd.sampling.RemoteLikelihood is the client)."""
+        check(LIB.gwl_server_listen(self._s, os.fsencode(path)), "gwl_server_listen")
 
     def stats(self):
         b, r = ctypes.c_int64(), ctypes.c_int64()

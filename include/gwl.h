@@ -309,6 +309,18 @@ int gwl_server_wait(gwl_server* server, int64_t ticket, double* latency_s);
 int gwl_server_stats(gwl_server* server, int64_t* batches, int64_t* rows);
 /* Device bytes of the server's own buffers (staging, group partials). */
 int gwl_server_device_bytes(gwl_server* server, int64_t* bytes);
+/* Serve requests from other processes on a Unix stream socket created at
+ * `path` (mode 0600; the caller picks a private directory).  Each connection
+ * sends requests and reads replies in order, native byte order:
+ *   request {uint32 GWL_WIRE_REQUEST, int32 id, int64 n} + n x 16 float64 rows
+ *   reply   {uint32 GWL_WIRE_REPLY, int32 status, int64 n} + n float64 lnL
+ *           (status 0) or n bytes of error text (status != 0)
+ * with 1 <= n <= max_rows; requests from all connections share launches with
+ * gwl_server_submit.  gwl_server_destroy closes the socket.  Replaces the
+ * pickled likelihood that bilby's pool workers hold (base_sampler.py:772-800). */
+#define GWL_WIRE_REQUEST 0x51574C47u
+#define GWL_WIRE_REPLY 0x52574C47u
+int gwl_server_listen(gwl_server* server, const char* path);
 
 #ifdef __cplusplus
 }
