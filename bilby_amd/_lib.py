@@ -50,6 +50,13 @@ class RoqDesc(ctypes.Structure):
     ]
 
 
+class RelbinDesc(ctypes.Structure):
+    _fields_ = [
+        ("n_bins", ctypes.c_int32), ("bin_frequencies", _dp), ("fiducial_response", _dp),
+        ("summary", _dp),
+    ]
+
+
 class GWLError(RuntimeError):
     pass
 
@@ -122,6 +129,13 @@ def _load():
                              ctypes.c_int),
         "gwl_server_device_bytes": ([vp, ctypes.POINTER(ctypes.c_int64)], ctypes.c_int),
         "gwl_server_listen": ([vp, ctypes.c_char_p], ctypes.c_int),
+        "gwl_relbin_create": ([vp, ctypes.POINTER(RelbinDesc), ctypes.POINTER(vp)], ctypes.c_int),
+        "gwl_relbin_destroy": ([vp], ctypes.c_int),
+        "gwl_server_add_relbin": ([vp, vp, ctypes.c_int32, ctypes.POINTER(ctypes.c_int32)],
+                                  ctypes.c_int),
+        "gwl_relbin_device_bytes": ([vp, ctypes.POINTER(ctypes.c_int64)], ctypes.c_int),
+        "gwl_relbin_eval": ([vp, vp, ctypes.c_int64, ctypes.c_int32, ctypes.c_int32, vp, vp, vp],
+                            ctypes.c_int),
     }
     for name, (argt, rest) in sig.items():
         fn = getattr(lib, name)
@@ -162,5 +176,6 @@ EXPORTED_SYMBOLS = [
     "gwl_roq_reserve", "gwl_server_create", "gwl_server_destroy", "gwl_server_add_event",
     "gwl_server_add_roq", "gwl_server_add_group", "gwl_server_submit", "gwl_server_wait",
     "gwl_server_stats", "gwl_server_device_bytes", "gwl_event_set_transform",
-    "gwl_server_listen",
+    "gwl_server_listen", "gwl_relbin_create", "gwl_relbin_destroy", "gwl_relbin_device_bytes",
+    "gwl_relbin_eval", "gwl_server_add_relbin",
 ]

@@ -1,0 +1,370 @@
+// Relative-binning likelihood (bilby/gw/likelihood/relative.py,
+// RelativeBinningGravitationalWaveTransient; Zackay, Dai & Venumadhav,
+// arXiv:1806.08792).
+//
+// Setup (host, float64, in the caller): bin edges f_0 < ... < f_B on the
+// frequency grid (relative.py:177-231), the fiducial detector responses
+// h0_d(f_e) at the edges (:233-258) and the per-bin summary data
+// a0, a1, b0, b1 (:314-360).  Per parameter row the kernel evaluates the
+// TaylorF2 detector response only at the B + 1 edges, the ratio
+// r_d(f_e) = h_d(f_e) / h0_d(f_e) (:362-374), and
+//   <d|h> = sum_b a0 conj(r0) + a1 conj(r1)
+//   <h|h> = sum_b b0 |r0|^2 + 2 b1 Re(r0 conj(r1))           (:393-407)
+// with r0 = (r_{b+1} + r_b) / 2, r1 = (r_{b+1} - r_b) / (f_{b+1} - f_b),
+// then the plain / phase / distance marginalised value of base.py:448-477.
+//
+// Device layout: the factors 1/2 and 1/width are folded into the summary
+// at creation, so with S = r_{b+1} + r_b and D = r_{b+1} - r_b
+//   <d|h> += A0 conj(S) + A1 conj(D),  A0 = a0 / 2,  A1 = a1 / w
+//   <h|h> += B0 |S|^2 + B1 Re(S conj(D)),  B0 = b0 / 4,  B1 = b1 / w
+// and g_d(e) = f_e^-7/6 / h0_d(f_e) is stored per edge, so that
+// r_d(e) = C_d g_d(e) exp(-2 pi i (Psi(f_e) / 2 pi + f_e dt_d)), C_d being the
+// complex detector amplitude of the row.
+//
+// One wavefront per row, lanes over edges: lane l evaluates edge 64 k + l
+// and takes edge 64 k + l + 1 from lane l + 1 (lane 63 from lane 0 of the
+// next chunk), so every edge's waveform is computed once.  Summary data and
+// edge tables are a few tens of KB per event and stay in L2.
+#include "gwl_internal.h"
+
+#include <cmath>
+#include <vector>
+
+using gwl::set_error;
+
+struct gwl_relbin {
+  gwl_event* ev = nullptr;
+  int nb = 0;
+  void* data = nullptr;
+  size_t data_bytes = 0;
+  double* fe = nullptr;       // [nb + 1] edge frequencies
+  double* basis = nullptr;    // [nb + 1][kNB] TaylorF2 phase basis at the edges
+  double2* g64 = nullptr;     // [nd][nb + 1] f^-7/6 / h0
+  float2* g32 = nullptr;
+  double* s64 = nullptr;      // [nd][nb][6] A0 re, A0 im, A1 re, A1 im, B0, B1
+  float* s32 = nullptr;
+};
+
+namespace gwl {
+namespace {
+
+struct RelbinArgs {
+  EvDev ev;
+  int nb;
+  const double* fe;
+  const double* basis;
+  const void* g;     // double2 or float2 [nd][nb + 1]
+  const void* s;     // double or float [nd][nb][6]
+  const double* params;
+  int ld;
+  long long n;
+  double* out;
+  double* aux;
+};
+
+template <typename R> struct RelbinT;
+template <> struct RelbinT<double> {
+  using C = double2;
+  __device__ static C load(const void* p, size_t i) { return static_cast<const double2*>(p)[i]; }
+  __device__ static double sval(const void* p, size_t i) { return static_cast<const double*>(p)[i]; }
+};
+template <> struct RelbinT<float> {
+  using C = float2;
+  __device__ static C load(const void* p, size_t i) { return static_cast<const float2*>(p)[i]; }
+  __device__ static float sval(const void* p, size_t i) { return static_cast<const float*>(p)[i]; }
+};
+
+__device__ __forceinline__ double2 shfl_c(double2 v, int src) {
+  return make_double2(__shfl(v.x, src, 64), __shfl(v.y, src, 64));
+}
+__device__ __forceinline__ float2 shfl_c(float2 v, int src) {
+  return make_float2(__shfl(v.x, src, 64), __shfl(v.y, src, 64));
+}
+
+__device__ __forceinline__ double reduce_wave(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+// r_d at edge e for every detector (zero outside the waveform's band).
+template <typename R, int ND>
+__device__ __forceinline__ void edge_ratio(const RelbinArgs& a, const double* c, const double* dt,
+                                           const R* cre, const R* cim, double flo, double fhi,
+                                           int e, typename RelbinT<R>::C* r) {
+  using C = typename RelbinT<R>::C;
+  const bool live = e <= a.nb;
+  const int ee = live ? e : a.nb;
+  const double f = a.fe[ee];
+  double b[kNB];
+#pragma unroll
+  for (int k = 0; k < kNB; ++k) b[k] = a.basis[(size_t)ee * kNB + k];
+  const double psi = tf2_phase_cycles(c, b);
+  const bool band = live && f >= flo && f <= fhi;
+#pragma unroll
+  for (int d = 0; d < ND; ++d) {
+    double ph = fma(f, dt[d], psi);
+    ph -= rint(ph);
+    const C w = expm2pi((R)ph);
+    const C g = RelbinT<R>::load(a.g, (size_t)d * (a.nb + 1) + ee);
+    const C amp = cmk<C>(cre[d], cim[d]);
+    const C v = cmul(cmul(amp, g), w);
+    r[d] = band ? v : cmk<C>((R)0, (R)0);
+  }
+}
+
+template <typename R, int ND>
+__global__ __launch_bounds__(256) void k_relbin(RelbinArgs a) {
+  using C = typename RelbinT<R>::C;
+  extern __shared__ double knots[];
+  const EvDev& ev = a.ev;
+  const bool dist = (ev.flags & GWL_DISTANCE_MARG) != 0;
+  if (dist) stage_knots(ev, knots);
+  __syncthreads();
+  const int lane = threadIdx.x & 63;
+  const long long row = (long long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (row >= a.n) return;
+  const double* p = a.params + row * a.ld;
+
+  // row prologue (every lane computes the same values)
+  const double ra = p[GWL_P_RA], dec = p[GWL_P_DEC], psi = p[GWL_P_PSI];
+  const double tc = p[GWL_P_GEOCENT_TIME], iota = p[GWL_P_THETA_JN];
+  double c[10], amp0, fisco;
+  const int valid = tf2_row_constants(p, ev.fref, 1.0, c, amp0, fisco);
+  const double ci = cos(iota);
+  const double ap = amp0 * (1.0 + ci * ci) / 2.0, ax = amp0 * ci;
+  double dt[ND];
+  R cre[ND], cim[ND];
+#pragma unroll
+  for (int d = 0; d < ND; ++d) {
+    double fp, fc, delay;
+    antenna_and_delay(ev.tensor[d], ev.vertex[d], ra, dec, tc, psi, fp, fc, delay);
+    dt[d] = (tc - ev.start) + delay;  // interferometer.py:355-358
+    cre[d] = (R)(fp * ap);
+    cim[d] = (R)(-fc * ax);
+  }
+  const double flo = ev.wf_fmin, fhi = fmin(fisco, ev.wf_fmax);
+
+  R dhr = 0, dhi = 0, hh = 0;
+  C cur[ND], nxt[ND], right[ND];
+  edge_ratio<R, ND>(a, c, dt, cre, cim, flo, fhi, lane, cur);
+  for (int base = 0; base < a.nb; base += 64) {
+    edge_ratio<R, ND>(a, c, dt, cre, cim, flo, fhi, base + 64 + lane, nxt);
+#pragma unroll
+    for (int d = 0; d < ND; ++d) {
+      const C up = shfl_c(cur[d], (lane + 1) & 63);
+      const C wrap = shfl_c(nxt[d], 0);
+      right[d] = lane == 63 ? wrap : up;
+    }
+    const int bin = base + lane;
+    if (bin < a.nb) {
+#pragma unroll
+      for (int d = 0; d < ND; ++d) {
+        const size_t o = ((size_t)d * a.nb + bin) * 6;
+        const R A0r = RelbinT<R>::sval(a.s, o + 0), A0i = RelbinT<R>::sval(a.s, o + 1);
+        const R A1r = RelbinT<R>::sval(a.s, o + 2), A1i = RelbinT<R>::sval(a.s, o + 3);
+        const R B0 = RelbinT<R>::sval(a.s, o + 4), B1 = RelbinT<R>::sval(a.s, o + 5);
+        const R Sr = right[d].x + cur[d].x, Si = right[d].y + cur[d].y;
+        const R Dr = right[d].x - cur[d].x, Di = right[d].y - cur[d].y;
+        // A conj(S) = (Ar Sr + Ai Si) + i (Ai Sr - Ar Si)
+        dhr += A0r * Sr + A0i * Si + A1r * Dr + A1i * Di;
+        dhi += A0i * Sr - A0r * Si + A1i * Dr - A1r * Di;
+        hh += B0 * (Sr * Sr + Si * Si) + B1 * (Sr * Dr + Si * Di);
+      }
+    }
+#pragma unroll
+    for (int d = 0; d < ND; ++d) cur[d] = nxt[d];
+  }
+  const double dh_re = reduce_wave((double)dhr);
+  const double dh_im = reduce_wave((double)dhi);
+  const double h2 = reduce_wave((double)hh);
+  if (lane != 0) return;
+  if (a.aux) {
+    a.aux[3 * row + 0] = dh_re;
+    a.aux[3 * row + 1] = dh_im;
+    a.aux[3 * row + 2] = h2;
+  }
+  if (!valid) {
+    a.out[row] = row_valid_sentinel();
+    return;
+  }
+  double ll;
+  if (dist) {
+    const double hs = p[GWL_P_LUMINOSITY_DISTANCE] / ev.dref;
+    const double dd = (ev.flags & GWL_PHASE_MARG) ? hypot(dh_re, dh_im) * hs : dh_re * hs;
+    const SplineY yy = spline_y(ev, knots + ev.sp.nx + 4, h2 * hs * hs);
+    ll = spline_x(ev, knots, yy, dd);
+  } else if (ev.flags & GWL_PHASE_MARG) {
+    ll = ln_i0(hypot(dh_re, dh_im)) - h2 / 2.0;
+  } else {
+    ll = dh_re - h2 / 2.0;
+  }
+  a.out[row] = ll;
+}
+
+__global__ void k_relbin_basis(const double* fe, int n, double* basis) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) tf2_basis(fe[i], basis + (size_t)i * kNB);
+}
+
+template <typename R>
+int launch_relbin_nd(const RelbinArgs& a, int nd, size_t lds, hipStream_t st) {
+  const dim3 grid((unsigned)((a.n + 3) / 4)), block(256);
+  switch (nd) {
+    case 1: hipLaunchKernelGGL((k_relbin<R, 1>), grid, block, lds, st, a); break;
+    case 2: hipLaunchKernelGGL((k_relbin<R, 2>), grid, block, lds, st, a); break;
+    case 3: hipLaunchKernelGGL((k_relbin<R, 3>), grid, block, lds, st, a); break;
+    default: hipLaunchKernelGGL((k_relbin<R, 4>), grid, block, lds, st, a); break;
+  }
+  GWL_HIP_CHECK(hipGetLastError());
+  return GWL_OK;
+}
+
+}  // namespace
+}  // namespace gwl
+
+extern "C" {
+
+int gwl_relbin_create(gwl_event* ev, const gwl_relbin_desc* desc, gwl_relbin** out) {
+  if (!ev || !desc || !out || desc->n_bins < 1 || !desc->bin_frequencies ||
+      !desc->fiducial_response || !desc->summary) {
+    set_error("gwl_relbin_create: invalid argument");
+    return GWL_E_INVALID;
+  }
+  const gwl::EvDev& d = ev->dev;
+  if (d.approx != GWL_TAYLORF2 || (d.flags & GWL_TIME_MARG)) {
+    set_error("gwl_relbin_create: the event must be TaylorF2 without time marginalisation");
+    return GWL_E_UNSUPPORTED;
+  }
+  const int nb = desc->n_bins, ne = nb + 1, nd = d.nd;
+  for (int e = 0; e < ne; ++e) {
+    const double f = desc->bin_frequencies[e];
+    if (!(f > 0.0) || !std::isfinite(f) || (e > 0 && !(f > desc->bin_frequencies[e - 1]))) {
+      set_error("gwl_relbin_create: bin frequencies must be positive and increasing");
+      return GWL_E_INVALID;
+    }
+  }
+  // host tables: g = f^-7/6 / h0, folded summary
+  std::vector<double2> g64((size_t)nd * ne);
+  std::vector<float2> g32((size_t)nd * ne);
+  for (int k = 0; k < nd; ++k)
+    for (int e = 0; e < ne; ++e) {
+      const double hr = desc->fiducial_response[((size_t)k * ne + e) * 2 + 0];
+      const double hi = desc->fiducial_response[((size_t)k * ne + e) * 2 + 1];
+      const double m2 = hr * hr + hi * hi;
+      if (!(m2 > 0.0) || !std::isfinite(m2)) {
+        set_error("gwl_relbin_create: the fiducial response is zero or not finite at a bin edge");
+        return GWL_E_INVALID;
+      }
+      const double s = std::pow(desc->bin_frequencies[e], -7.0 / 6.0) / m2;
+      g64[(size_t)k * ne + e] = make_double2(hr * s, -hi * s);
+      g32[(size_t)k * ne + e] = make_float2((float)(hr * s), (float)(-hi * s));
+    }
+  std::vector<double> s64((size_t)nd * nb * 6);
+  std::vector<float> s32((size_t)nd * nb * 6);
+  for (int k = 0; k < nd; ++k)
+    for (int b = 0; b < nb; ++b) {
+      const double* src = desc->summary + ((size_t)k * nb + b) * 8;
+      const double w = desc->bin_frequencies[b + 1] - desc->bin_frequencies[b];
+      double* o = &s64[((size_t)k * nb + b) * 6];
+      o[0] = src[0] / 2.0;
+      o[1] = src[1] / 2.0;
+      o[2] = src[2] / w;
+      o[3] = src[3] / w;
+      o[4] = src[4] / 4.0;   // b0 and b1 are real (|h0|^2 weights); their imaginary parts are dropped
+      o[5] = src[6] / w;
+      for (int j = 0; j < 6; ++j) s32[((size_t)k * nb + b) * 6 + j] = (float)o[j];
+    }
+  gwl_relbin* rb = new (std::nothrow) gwl_relbin;
+  if (!rb) { set_error("out of host memory"); return GWL_E_NOMEM; }
+  rb->ev = ev;
+  rb->nb = nb;
+  auto al = [](size_t x) { return (x + 255) / 256 * 256; };
+  const size_t b_fe = al((size_t)ne * 8), b_basis = al((size_t)ne * gwl::kNB * 8);
+  const size_t b_g64 = al(g64.size() * sizeof(double2)), b_g32 = al(g32.size() * sizeof(float2));
+  const size_t b_s64 = al(s64.size() * 8), b_s32 = al(s32.size() * 4);
+  rb->data_bytes = b_fe + b_basis + b_g64 + b_g32 + b_s64 + b_s32;
+  if (hipSetDevice(ev->ctx->device) != hipSuccess || hipMalloc(&rb->data, rb->data_bytes) != hipSuccess) {
+    delete rb;
+    set_error("hipMalloc failed for relative-binning tables");
+    return GWL_E_NOMEM;
+  }
+  char* q = static_cast<char*>(rb->data);
+  rb->fe = reinterpret_cast<double*>(q); q += b_fe;
+  rb->basis = reinterpret_cast<double*>(q); q += b_basis;
+  rb->g64 = reinterpret_cast<double2*>(q); q += b_g64;
+  rb->g32 = reinterpret_cast<float2*>(q); q += b_g32;
+  rb->s64 = reinterpret_cast<double*>(q); q += b_s64;
+  rb->s32 = reinterpret_cast<float*>(q);
+  hipError_t err = hipMemcpy(rb->fe, desc->bin_frequencies, (size_t)ne * 8, hipMemcpyHostToDevice);
+  if (err == hipSuccess) err = hipMemcpy(rb->g64, g64.data(), g64.size() * sizeof(double2), hipMemcpyHostToDevice);
+  if (err == hipSuccess) err = hipMemcpy(rb->g32, g32.data(), g32.size() * sizeof(float2), hipMemcpyHostToDevice);
+  if (err == hipSuccess) err = hipMemcpy(rb->s64, s64.data(), s64.size() * 8, hipMemcpyHostToDevice);
+  if (err == hipSuccess) err = hipMemcpy(rb->s32, s32.data(), s32.size() * 4, hipMemcpyHostToDevice);
+  if (err == hipSuccess) {
+    hipLaunchKernelGGL(gwl::k_relbin_basis, dim3((unsigned)((ne + 127) / 128)), dim3(128), 0, 0,
+                       rb->fe, ne, rb->basis);
+    err = hipGetLastError();
+  }
+  if (err == hipSuccess) err = hipDeviceSynchronize();
+  if (err != hipSuccess) {
+    (void)hipFree(rb->data);
+    delete rb;
+    return gwl::hip_fail(err, "gwl_relbin_create upload");
+  }
+  *out = rb;
+  return GWL_OK;
+}
+
+int gwl_relbin_destroy(gwl_relbin* rb) {
+  if (!rb) return GWL_OK;
+  (void)hipSetDevice(rb->ev->ctx->device);
+  if (rb->data) (void)hipFree(rb->data);
+  delete rb;
+  return GWL_OK;
+}
+
+int gwl_relbin_device_bytes(gwl_relbin* rb, int64_t* bytes) {
+  if (!rb || !bytes) { set_error("gwl_relbin_device_bytes: invalid argument"); return GWL_E_INVALID; }
+  *bytes = (int64_t)rb->data_bytes;
+  return GWL_OK;
+}
+
+int gwl_relbin_eval(gwl_relbin* rb, const double* params, int64_t n, int32_t ld, int32_t precision,
+                    double* out, double* aux, void* stream) {
+  if (!rb || !params || !out || n < 0 || ld < GWL_NPARAM ||
+      (precision != GWL_FP32 && precision != GWL_FP64)) {
+    set_error("gwl_relbin_eval: invalid argument");
+    return GWL_E_INVALID;
+  }
+  if (n == 0) return GWL_OK;
+  gwl_event* ev = rb->ev;
+  if ((ev->dev.flags & GWL_DISTANCE_MARG) && !ev->has_spline) {
+    set_error("gwl_relbin_eval: distance marginalisation requested but no spline attached");
+    return GWL_E_INVALID;
+  }
+  GWL_HIP_CHECK(hipSetDevice(ev->ctx->device));
+  (void)hipGetLastError();
+  gwl::RelbinArgs a;
+  a.ev = ev->dev;
+  a.nb = rb->nb;
+  a.fe = rb->fe;
+  a.basis = rb->basis;
+  a.g = precision == GWL_FP64 ? (const void*)rb->g64 : (const void*)rb->g32;
+  a.s = precision == GWL_FP64 ? (const void*)rb->s64 : (const void*)rb->s32;
+  a.params = params;
+  a.ld = ld;
+  a.n = n;
+  a.out = out;
+  a.aux = aux;
+  const size_t lds = (ev->dev.flags & GWL_DISTANCE_MARG)
+                         ? (size_t)(ev->dev.sp.nx + ev->dev.sp.ny + 8) * 8 : 0;
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  if (int rc = gwl::timing_begin(ev, st)) return rc;
+  const int rc = precision == GWL_FP64 ? gwl::launch_relbin_nd<double>(a, ev->dev.nd, lds, st)
+                                       : gwl::launch_relbin_nd<float>(a, ev->dev.nd, lds, st);
+  if (rc != GWL_OK) return rc;
+  return gwl::timing_end(ev, st);
+}
+
+}  // extern "C"

@@ -36,7 +36,7 @@ namespace {
 
 using Clock = std::chrono::steady_clock;
 
-enum { UNIT_EVENT = 0, UNIT_ROQ = 1, UNIT_GROUP = 2 };
+enum { UNIT_EVENT = 0, UNIT_ROQ = 1, UNIT_GROUP = 2, UNIT_RELBIN = 3 };
 
 struct Req {
   int id, unit, member;
@@ -61,6 +61,7 @@ struct Unit {
   gwl_roq* roq = nullptr;
   int mode = 1;
   gwl_multi* multi = nullptr;
+  gwl_relbin* rb = nullptr;
   int n_members = 1;
   hipStream_t stream = nullptr;
   hipEvent_t done = nullptr;
@@ -157,6 +158,9 @@ void launch(gwl_server* s, Unit* u, std::vector<Req>& reqs) {
                   u->stream);
   } else if (u->kind == UNIT_ROQ) {
     rc = gwl_roq_eval(u->roq, u->d_rows[0], rows[0], GWL_NPARAM, u->mode, u->d_out[0], u->stream);
+  } else if (u->kind == UNIT_RELBIN) {
+    rc = gwl_relbin_eval(u->rb, u->d_rows[0], rows[0], GWL_NPARAM, u->precision, u->d_out[0],
+                         nullptr, u->stream);
   } else {
     rc = gwl_multi_set_rows(u->multi, rows.data(), u->stream);
     if (rc == GWL_OK) rc = gwl_multi_launch(u->multi, u->stream);
@@ -458,6 +462,23 @@ int gwl_server_add_roq(gwl_server* s, gwl_roq* roq, int32_t mode, int32_t* id) {
   u->kind = UNIT_ROQ;
   u->roq = roq;
   u->mode = mode;
+  if (int rc = new_unit(s, u, 1)) {
+    free_unit(u);
+    return rc;
+  }
+  return add_unit(s, u, 1, id);
+}
+
+int gwl_server_add_relbin(gwl_server* s, gwl_relbin* rb, int32_t precision, int32_t* id) {
+  if (!s || !rb || !id || (precision != GWL_FP32 && precision != GWL_FP64)) {
+    set_error("gwl_server_add_relbin: invalid argument");
+    return GWL_E_INVALID;
+  }
+  GWL_HIP_CHECK(hipSetDevice(s->ctx->device));
+  Unit* u = new Unit;
+  u->kind = UNIT_RELBIN;
+  u->rb = rb;
+  u->precision = precision;
   if (int rc = new_unit(s, u, 1)) {
     free_unit(u);
     return rc;

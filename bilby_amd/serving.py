@@ -31,6 +31,12 @@ class HBMBudgetExceeded(MemoryError):
 
 
 def _device_bytes(lk):
+    if getattr(lk, "_relbin", None) is not None:
+        out = ctypes.c_int64()
+        check(LIB.gwl_relbin_device_bytes(lk._relbin, ctypes.byref(out)), "gwl_relbin_device_bytes")
+        ev = ctypes.c_int64()
+        check(LIB.gwl_event_device_bytes(lk._event, ctypes.byref(ev)), "gwl_event_device_bytes")
+        return int(out.value) + int(ev.value)
     if getattr(lk, "_event", None) is not None:
         out = ctypes.c_int64()
         check(LIB.gwl_event_device_bytes(lk._event, ctypes.byref(out)), "gwl_event_device_bytes")
@@ -115,7 +121,13 @@ class LikelihoodServer:
         admit its device bytes plus the server's staging for it against the
         budget; over budget raises HBMBudgetExceeded."""
         staging = self.max_batch_rows * (_lib.NPARAM + 1) * 8
-        if getattr(likelihood, "_event", None) is not None:
+        if getattr(likelihood, "_relbin", None) is not None:
+            nbytes = _device_bytes(likelihood)
+            self._admit(name, nbytes + staging)
+            i = ctypes.c_int32()
+            check(LIB.gwl_server_add_relbin(self._s, likelihood._relbin, int(likelihood.precision),
+                                            ctypes.byref(i)), "gwl_server_add_relbin")
+        elif getattr(likelihood, "_event", None) is not None:
             check(LIB.gwl_event_reserve(likelihood._event, self.max_batch_rows), "gwl_event_reserve")
             nbytes = _device_bytes(likelihood)
             self._admit(name, nbytes + staging)

@@ -284,6 +284,37 @@ int gwl_roq_eval(gwl_roq* roq, const double* params, int64_t n_rows, int32_t ld,
 int gwl_gemm(gwl_ctx* ctx, int32_t mode, const void* A, const void* A_lo, const void* Bt,
              const void* Bt_lo, void* C, int32_t M, int32_t N, int32_t K, void* stream);
 
+/* ------------------------------------------------------------------ relative binning
+ * RelativeBinningGravitationalWaveTransient (bilby/gw/likelihood/relative.py)
+ * on a TaylorF2 event without time marginalisation; the event supplies the
+ * detectors, the source model, the phase / distance flags and the distance
+ * spline, and must outlive the relative-binning object.
+ *   bin_frequencies:   [n_bins + 1] edge frequencies f_e, increasing
+ *                      (relative.py:177-231, setup_bins)
+ *   fiducial_response: [n_det][n_bins + 1][2] complex128 detector response
+ *                      of the fiducial parameters at the edges
+ *                      (relative.py:233-258; must be non-zero)
+ *   summary:           [n_det][n_bins][8] float64: a0, a1, b0, b1 as complex
+ *                      (re, im) pairs, relative.py:314-360 (b0 and b1 are
+ *                      real; their imaginary parts are ignored) */
+typedef struct gwl_relbin_desc {
+  int32_t n_bins;
+  const double* bin_frequencies;
+  const double* fiducial_response;
+  const double* summary;
+} gwl_relbin_desc;
+typedef struct gwl_relbin gwl_relbin;
+int gwl_relbin_create(gwl_event* ev, const gwl_relbin_desc* desc, gwl_relbin** out);
+int gwl_relbin_destroy(gwl_relbin* rb);
+int gwl_relbin_device_bytes(gwl_relbin* rb, int64_t* bytes);
+/* ln L ratio of n rows: calculate_snrs (relative.py:393-428) + the
+ * marginalisation of base.py:448-477.  params: device [n][ld] float64;
+ * out (and aux [n][3] = Re <d|h>, Im <d|h>, <h|h>, may be NULL): device.
+ * precision GWL_FP64 or GWL_FP32 (float32 ratios and bin sums, float64
+ * phase, float64 totals).  Timed by gwl_event_timing of the event. */
+int gwl_relbin_eval(gwl_relbin* rb, const double* params, int64_t n, int32_t ld,
+                    int32_t precision, double* out, double* aux, void* stream);
+
 /* ------------------------------------------------------------------ serving
  * Continuous-batching request server (no counterpart in the reference, whose
  * parallelism is a pool of single-dict callers, base_sampler.py:772-800).
@@ -301,6 +332,9 @@ int gwl_server_add_event(gwl_server* server, gwl_event* ev, int32_t precision, i
 int gwl_server_add_roq(gwl_server* server, gwl_roq* roq, int32_t mode, int32_t* id);
 int gwl_server_add_group(gwl_server* server, int32_t n_events, gwl_event* const* events,
                          int32_t* first_id);
+/* A relative-binning likelihood (gwl_relbin_create) as a unit; precision
+ * GWL_FP32 or GWL_FP64. */
+int gwl_server_add_relbin(gwl_server* server, gwl_relbin* rb, int32_t precision, int32_t* id);
 /* rows: host [n][16] float64, out: host [n] float64, 1 <= n <= max_rows. */
 int gwl_server_submit(gwl_server* server, int32_t id, const double* rows, int64_t n_rows,
                       double* out, int64_t* ticket);

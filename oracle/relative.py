@@ -1,0 +1,142 @@
+"""FP64 restatement of bilby's RelativeBinningGravitationalWaveTransient
+(bilby/gw/likelihood/relative.py), without time marginalisation.
+
+Test infrastructure only (see oracle/__init__.py).  Written bin by bin, as
+the reference loops, so that it stays independent of the vectorised setup in
+bilby_amd.relative.
+"""
+import copy
+import types
+
+import numpy as np
+
+from . import core
+from .likelihood import OracleGWTransient, detector_response, source_polarizations
+
+
+class OracleRelativeBinning(OracleGWTransient):
+    """relative.py:102-174 (constructor), :177-231 (setup_bins),
+    :233-258 (set_fiducial_waveforms), :314-360 (compute_summary_data),
+    :362-374 (ratios), :393-428 (calculate_snrs)."""
+
+    def __init__(self, event, source, fiducial_parameters, waveform_arguments=None,
+                 distance_marginalization=False, phase_marginalization=False, priors=None,
+                 chi=1, epsilon=0.5, distance_table=None):
+        super().__init__(event, source, waveform_arguments, time_marginalization=False,
+                         distance_marginalization=distance_marginalization,
+                         phase_marginalization=phase_marginalization, priors=priors,
+                         distance_table=distance_table)
+        fid = dict(fiducial_parameters)
+        if distance_marginalization:
+            fid["luminosity_distance"] = self._ref_dist
+        if phase_marginalization:
+            fid["phase"] = 0.0
+        self.fiducial_parameters = fid
+        self.chi = chi
+        self.epsilon = epsilon
+        self.gamma = np.array([-5 / 3, -2 / 3, 1, 5 / 3, 7 / 3])
+        self.maximum_frequency = event.frequency_array[-1]
+        self.set_fiducial_waveforms(fid)
+        self.setup_bins()
+        self.compute_summary_data()
+
+    def set_fiducial_waveforms(self, parameters):
+        """relative.py:233-258."""
+        f = self.event.frequency_array
+        pols = source_polarizations(self.source, f, parameters, self.waveform_arguments)
+        last = np.where(pols["plus"] != 0j)[0][-1]
+        self.maximum_frequency = f[last]
+        self.fiducial = {}
+        for det in self.event.detectors:
+            wf = detector_response(det, pols, parameters, self.event.start_time)
+            wf = wf * (det.frequency_array <= self.maximum_frequency)
+            self.fiducial[det.name] = wf
+
+    def setup_bins(self):
+        """relative.py:177-231, loop for loop."""
+        f = self.event.frequency_array
+        gamma = self.gamma[:, np.newaxis]
+        maximum_frequency = f[0]
+        minimum_frequency = f[-1]
+        for det in self.event.detectors:
+            mf = det.frequency_array[det.mask]
+            maximum_frequency = max(maximum_frequency, mf[-1])
+            minimum_frequency = min(minimum_frequency, mf[0])
+        maximum_frequency = min(maximum_frequency, self.maximum_frequency)
+        useful = f[(f >= minimum_frequency) & (f <= maximum_frequency)]
+        d_alpha = self.chi * 2 * np.pi / np.abs(
+            (minimum_frequency ** gamma) * np.heaviside(-gamma, 1)
+            - (maximum_frequency ** gamma) * np.heaviside(gamma, 1))
+        d_phi = np.sum(np.sign(gamma) * d_alpha * useful ** gamma, axis=0)
+        d_phi_from_start = d_phi - d_phi[0]
+        number_of_bins = int(d_phi_from_start[-1] // self.epsilon)
+        bin_inds, bin_freqs = [], []
+        last_index = -1
+        for i in range(number_of_bins + 1):
+            idx = np.where(d_phi_from_start >= ((i / number_of_bins) * d_phi_from_start[-1]))[0][0]
+            if idx == last_index:
+                continue
+            last_index = idx
+            bin_freqs.append(useful[idx])
+            bin_inds.append(np.where(f >= useful[idx])[0][0])
+        self.bin_inds = np.array(bin_inds, dtype=int)
+        self.bin_freqs = np.array(bin_freqs)
+        self.number_of_bins = len(self.bin_inds) - 1
+        self.bin_widths = self.bin_freqs[1:] - self.bin_freqs[:-1]
+        self.fiducial_points = {name: wf[self.bin_inds] for name, wf in self.fiducial.items()}
+
+    def compute_summary_data(self):
+        """relative.py:314-360."""
+        T = self.event.duration
+        self.summary_data = {}
+        for det in self.event.detectors:
+            m = det.mask
+            mf = det.frequency_array[m]
+            inds = [int(np.where(mf == edge)[0][0]) for edge in self.bin_freqs]
+            if inds[-1] < len(mf) - 1:
+                inds[-1] += 1
+            strain, h0, psd = det.strain[m], self.fiducial[det.name][m], det.psd[m]
+            a0, b0, a1, b1 = np.zeros((4, self.number_of_bins), dtype=complex)
+            for i in range(self.number_of_bins):
+                s, e = inds[i], inds[i + 1]
+                centre = (mf[s] + mf[e]) / 2
+                sl = slice(s, e)
+                delta = mf[sl] - centre
+                a0[i] = core.noise_weighted_inner_product(h0[sl], strain[sl], psd[sl], T)
+                b0[i] = core.noise_weighted_inner_product(h0[sl], h0[sl], psd[sl], T)
+                a1[i] = core.noise_weighted_inner_product(h0[sl], strain[sl] * delta, psd[sl], T)
+                b1[i] = core.noise_weighted_inner_product(h0[sl], h0[sl] * delta, psd[sl], T)
+            self.summary_data[det.name] = (a0, a1, b0, b1)
+
+    def edge_response(self, det, params):
+        """The detector response at the bin edges (interferometer.py:303-368
+        with ``frequencies`` given: no mask)."""
+        pols = source_polarizations(self.source, self.bin_freqs, params, self.waveform_arguments)
+        at_edges = types.SimpleNamespace(tensor=det.tensor, vertex=det.vertex,
+                                         frequency_array=self.bin_freqs,
+                                         mask=np.ones(len(self.bin_freqs), dtype=bool))
+        return detector_response(at_edges, pols, params, self.event.start_time)
+
+    def snrs_from_ratio(self, det, ratio):
+        """relative.py:370-374, 400-404: <d|h>, <h|h> from the edge ratios."""
+        r0 = (ratio[1:] + ratio[:-1]) / 2
+        r1 = (ratio[1:] - ratio[:-1]) / self.bin_widths
+        a0, a1, b0, b1 = self.summary_data[det.name]
+        d_inner_h = (a0 * r0.conj() + a1 * r1.conj()).sum()
+        h_inner_h = (b0 * abs(r0) ** 2 + 2 * b1 * (r0 * r1.conj()).real).sum()
+        return d_inner_h, h_inner_h.real
+
+    def log_likelihood_ratio(self, parameters):
+        """base.py:419-446 with relative.py:393-428 per detector."""
+        params = copy.deepcopy(parameters)
+        params.pop("fiducial", None)
+        m1, m2 = params["mass_1"], params["mass_2"]
+        if not (m1 > 0 and m2 > 0 and params["luminosity_distance"] > 0):
+            return np.nan_to_num(-np.inf)
+        d_inner_h, h_inner_h = 0j, 0.0
+        for det in self.event.detectors:
+            ratio = self.edge_response(det, params) / self.fiducial_points[det.name]
+            dh, hh = self.snrs_from_ratio(det, ratio)
+            d_inner_h += dh
+            h_inner_h += hh
+        return float(np.real(self.compute_log_likelihood(d_inner_h, h_inner_h, None, params)))
