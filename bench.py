@@ -227,6 +227,81 @@ def bench_uneven(steps):
                 tile_sweep_ms=sweep, fp32_vs_fp64_gpu_max_16_events=dmax, graph=True)
 
 
+def bench_relbin(steps):
+    """Relative binning on the headline event (TaylorF2 H1+L1+V1, 8 s @
+    4096 Hz), distance + phase marginalised, fiducial at the injection,
+    epsilon 0.5 (bilby's default) and 0.05, 65536 draws around the
+    fiducial point; the full (unbinned) likelihood with the same
+    marginalisations timed beside it."""
+    import ctypes
+    import numpy as np
+    import torch
+    from bilby_amd._lib import LIB, check
+    from bilby_amd.configs import make_taylorf2_marginalised
+    from bilby_amd.likelihood import GravitationalWaveTransient
+    from bilby_amd.relative import RelativeBinningGravitationalWaveTransient
+    from oracle.likelihood import OracleDetector, OracleEvent
+    from oracle.relative import OracleRelativeBinning
+    from tests.helpers import draws_as_dicts
+    s = make_taylorf2_marginalised(n_draws=8, seed=4321, time=False)
+    inj = dict(s["injection"])
+    n = 65536
+    rng = np.random.default_rng(77)
+    draws = {k: np.full(n, float(v)) for k, v in inj.items()}
+    for k, sd in (("mass_1", 0.003), ("mass_2", 0.003)):
+        draws[k] = draws[k] * (1 + rng.normal(0, sd, n))
+    for k, sd in (("ra", 0.1), ("dec", 0.1), ("psi", 0.3), ("theta_jn", 0.1),
+                  ("geocent_time", 5e-4)):
+        draws[k] = draws[k] + rng.normal(0, sd, n)
+    out = {"rows": n}
+    ifos = s["interferometers"]
+    for eps in (0.5, 0.05):
+        lk = RelativeBinningGravitationalWaveTransient(
+            ifos, s["waveform_generator"], fiducial_parameters=inj, distance_marginalization=True,
+            phase_marginalization=True, priors=s["priors"].copy(), epsilon=eps, precision="fp32")
+        d = dict(draws)
+        d["luminosity_distance"] = np.full(n, lk._ref_dist)
+        d["phase"] = np.zeros(n)
+        rows = torch.from_numpy(lk.rows_from_parameters(d)).cuda()
+        key = "eps%g" % eps
+        out[key + "_bins"] = lk.number_of_bins
+        for prec in ("fp32", "fp64"):
+            dt = timed(lambda: lk.log_likelihood_ratio_batch(rows, precision=prec), steps)
+            out["%s_%s_evals_per_s" % (key, prec)] = n / dt
+        check(LIB.gwl_event_timing(lk._event, 1))
+        for _ in range(steps):
+            lk.log_likelihood_ratio_batch(rows)
+        torch.cuda.synchronize()
+        tot, cnt = ctypes.c_double(), ctypes.c_int64()
+        check(LIB.gwl_event_timing_read(lk._event, ctypes.byref(tot), ctypes.byref(cnt)))
+        check(LIB.gwl_event_timing(lk._event, 0))
+        out[key + "_k_relbin_fp32_ms"] = tot.value / max(1, cnt.value)
+        r32 = lk.log_likelihood_ratio_batch(rows, precision="fp32").cpu().numpy()
+        r64 = lk.log_likelihood_ratio_batch(rows, precision="fp64").cpu().numpy()
+        out[key + "_fp32_vs_fp64_max"] = float(np.max(np.abs(r32 - r64)))
+        dets = [OracleDetector(i.name, i.frequency_array, i.frequency_domain_strain,
+                               i.power_spectral_density_array, i.frequency_mask) for i in ifos]
+        ev = OracleEvent(dets, ifos.duration, ifos.sampling_frequency, ifos.start_time)
+        olk = OracleRelativeBinning(ev, "taylorf2", inj,
+                                    waveform_arguments=dict(s["waveform_generator"].waveform_arguments),
+                                    distance_marginalization=True, phase_marginalization=True,
+                                    priors=dict(s["priors"]), epsilon=eps,
+                                    distance_table=lk._dist_margd_loglikelihood_array)
+        ref = np.array([olk.log_likelihood_ratio(p) for p in draws_as_dicts(d, range(16))])
+        out[key + "_fp64_vs_oracle_16rows"] = float(np.max(np.abs(r64[:16] - ref)))
+        out[key + "_fp32_vs_oracle_16rows"] = float(np.max(np.abs(r32[:16] - ref)))
+    full = GravitationalWaveTransient(ifos, s["waveform_generator"], distance_marginalization=True,
+                                      phase_marginalization=True, priors=s["priors"].copy(),
+                                      precision="fp32")
+    rows = torch.from_numpy(full.rows_from_parameters(d)).cuda()
+    check(LIB.gwl_event_reserve(full._event, n))
+    dt = timed(lambda: full.log_likelihood_ratio_batch(rows), steps)
+    out["full_likelihood_fp32_evals_per_s"] = n / dt
+    fr = full.log_likelihood_ratio_batch(rows).cpu().numpy()
+    out["eps0.05_vs_full_fp32_median_abs"] = float(np.median(np.abs(r32 - fr)))
+    return out
+
+
 def bench_roq(steps):
     """ROQ, H1+L1, 400 linear / 120 quadratic nodes, 1000 ROQ times,
     time + phase marginalised, 65536 TaylorF2 draws: the contraction
@@ -595,6 +670,7 @@ def main():
         for name, fn in (("sine_gaussian_plain_4s_2048Hz_10k", lambda: bench_sine_gaussian(10)),
                          ("uneven_64_events_persistent_graph", lambda: bench_uneven(5)),
                          ("roq_400_120_nodes_1000_times_65536", lambda: bench_roq(3)),
+                         ("relative_binning_dp_event_65536", lambda: bench_relbin(5)),
                          ("serving_32_samplers_2000_rps", lambda: bench_serving(
                              lk, rows.cpu().numpy(), args.serve_seconds)),
                          ("sampler_callers_pool_and_16_processes", lambda: bench_sampler(

@@ -673,6 +673,18 @@ __device__ inline double epi_value(const EvDev& ev, const double* knots, const S
   return xr - hh / 2.0;
 }
 
+// Copy of a wave-uniform value held in scalar registers.
+template <typename T>
+__device__ __forceinline__ T uniform_copy(const T& v) {
+  static_assert(sizeof(T) % 4 == 0, "word-sized type");
+  T out;
+  const int* src = reinterpret_cast<const int*>(&v);
+  int* dst = reinterpret_cast<int*>(&out);
+#pragma unroll
+  for (int k = 0; k < (int)(sizeof(T) / 4); ++k) dst[k] = __builtin_amdgcn_readfirstlane(src[k]);
+  return out;
+}
+
 // Running logsumexp state (max m, sum s of exp(v - m)).
 __device__ __forceinline__ void lse_push(double& m, double& s, double v) {
   if (v > m) {

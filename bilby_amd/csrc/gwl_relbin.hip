@@ -21,10 +21,13 @@
 // r_d(e) = C_d g_d(e) exp(-2 pi i (Psi(f_e) / 2 pi + f_e dt_d)), C_d being the
 // complex detector amplitude of the row.
 //
-// One wavefront per row, lanes over edges: lane l evaluates edge 64 k + l
-// and takes edge 64 k + l + 1 from lane l + 1 (lane 63 from lane 0 of the
-// next chunk), so every edge's waveform is computed once.  Summary data and
-// edge tables are a few tens of KB per event and stay in L2.
+// Three launches: k_relbin_rows (one thread per row: antenna patterns,
+// delays, TaylorF2 constants), k_relbin (one wavefront per row, lanes over
+// edges: lane l evaluates edge 64 k + l and takes edge 64 k + l + 1 from
+// lane l + 1, lane 63 from lane 0 of the next chunk, so every edge's
+// waveform is computed once) and k_relbin_epi (one thread per row:
+// marginalisation).  Summary data and edge tables are a few tens of KB per
+// event and stay in L2.
 #include "gwl_internal.h"
 
 #include <cmath>
@@ -43,10 +46,22 @@ struct gwl_relbin {
   float2* g32 = nullptr;
   double* s64 = nullptr;      // [nd][nb][6] A0 re, A0 im, A1 re, A1 im, B0, B1
   float* s32 = nullptr;
+  void* ws = nullptr;         // row constants and per-row sums
+  size_t ws_bytes = 0;
 };
 
 namespace gwl {
 namespace {
+
+// Per-row constants (k_relbin_rows): phase coefficients in cycles, detector
+// time shifts, complex detector amplitudes, the upper band edge.
+struct RelRow {
+  double c[kNB];
+  double dt[GWL_MAX_DET];
+  double cre[GWL_MAX_DET], cim[GWL_MAX_DET];
+  double fhi;
+  int valid, pad;
+};
 
 struct RelbinArgs {
   EvDev ev;
@@ -58,9 +73,15 @@ struct RelbinArgs {
   const double* params;
   int ld;
   long long n;
+  RelRow* rows;      // [n]
+  double* sums;      // [n][3]: Re <d|h>, Im <d|h>, <h|h>
   double* out;
   double* aux;
 };
+
+size_t relbin_ws_bytes(long long n) {
+  return ((size_t)n * sizeof(RelRow) + 255) / 256 * 256 + (size_t)n * 3 * sizeof(double);
+}
 
 template <typename R> struct RelbinT;
 template <> struct RelbinT<double> {
@@ -87,11 +108,43 @@ __device__ __forceinline__ double reduce_wave(double v) {
   return v;
 }
 
+// One thread per row: everything that depends on the parameter row only
+// (interferometer.py:303-368 antenna and delays, the TaylorF2 constants).
+__global__ __launch_bounds__(128) void k_relbin_rows(RelbinArgs a) {
+  const long long r = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= a.n) return;
+  const EvDev& ev = a.ev;
+  const double* p = a.params + r * a.ld;
+  const double ra = p[GWL_P_RA], dec = p[GWL_P_DEC], psi = p[GWL_P_PSI];
+  const double tc = p[GWL_P_GEOCENT_TIME], iota = p[GWL_P_THETA_JN];
+  RelRow* o = a.rows + r;
+  double c[10], amp0, fisco;
+  o->valid = tf2_row_constants(p, ev.fref, 1.0, c, amp0, fisco);
+  o->pad = 0;
+#pragma unroll
+  for (int k = 0; k < kNB; ++k) o->c[k] = c[k];
+  const double ci = cos(iota);
+  const double ap = amp0 * (1.0 + ci * ci) / 2.0, ax = amp0 * ci;
+  for (int d = 0; d < GWL_MAX_DET; ++d) {
+    double dt = 0.0, cre = 0.0, cim = 0.0;
+    if (d < ev.nd) {
+      double fp, fc, delay;
+      antenna_and_delay(ev.tensor[d], ev.vertex[d], ra, dec, tc, psi, fp, fc, delay);
+      dt = (tc - ev.start) + delay;  // interferometer.py:355-358
+      cre = fp * ap;
+      cim = -fc * ax;
+    }
+    o->dt[d] = dt;
+    o->cre[d] = cre;
+    o->cim[d] = cim;
+  }
+  o->fhi = fmin(fisco, ev.wf_fmax);
+}
+
 // r_d at edge e for every detector (zero outside the waveform's band).
 template <typename R, int ND>
-__device__ __forceinline__ void edge_ratio(const RelbinArgs& a, const double* c, const double* dt,
-                                           const R* cre, const R* cim, double flo, double fhi,
-                                           int e, typename RelbinT<R>::C* r) {
+__device__ __forceinline__ void edge_ratio(const RelbinArgs& a, const RelRow& rr, double flo, int e,
+                                           typename RelbinT<R>::C* r) {
   using C = typename RelbinT<R>::C;
   const bool live = e <= a.nb;
   const int ee = live ? e : a.nb;
@@ -99,57 +152,42 @@ __device__ __forceinline__ void edge_ratio(const RelbinArgs& a, const double* c,
   double b[kNB];
 #pragma unroll
   for (int k = 0; k < kNB; ++k) b[k] = a.basis[(size_t)ee * kNB + k];
-  const double psi = tf2_phase_cycles(c, b);
-  const bool band = live && f >= flo && f <= fhi;
+  const double psi = tf2_phase_cycles(rr.c, b);
+  const bool band = live && f >= flo && f <= rr.fhi;
 #pragma unroll
   for (int d = 0; d < ND; ++d) {
-    double ph = fma(f, dt[d], psi);
+    double ph = fma(f, rr.dt[d], psi);
     ph -= rint(ph);
     const C w = expm2pi((R)ph);
     const C g = RelbinT<R>::load(a.g, (size_t)d * (a.nb + 1) + ee);
-    const C amp = cmk<C>(cre[d], cim[d]);
+    const C amp = cmk<C>((R)rr.cre[d], (R)rr.cim[d]);
     const C v = cmul(cmul(amp, g), w);
     r[d] = band ? v : cmk<C>((R)0, (R)0);
   }
 }
 
+// One wavefront per row: lane l evaluates edge 64 k + l, takes edge
+// 64 k + l + 1 from lane l + 1 (lane 63: lane 0 of the next chunk) and
+// accumulates bin 64 k + l; the wave sums go to a.sums.
 template <typename R, int ND>
 __global__ __launch_bounds__(256) void k_relbin(RelbinArgs a) {
   using C = typename RelbinT<R>::C;
-  extern __shared__ double knots[];
-  const EvDev& ev = a.ev;
-  const bool dist = (ev.flags & GWL_DISTANCE_MARG) != 0;
-  if (dist) stage_knots(ev, knots);
-  __syncthreads();
   const int lane = threadIdx.x & 63;
-  const long long row = (long long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  const long long row = __builtin_amdgcn_readfirstlane(
+      (int)((long long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)));
   if (row >= a.n) return;
-  const double* p = a.params + row * a.ld;
-
-  // row prologue (every lane computes the same values)
-  const double ra = p[GWL_P_RA], dec = p[GWL_P_DEC], psi = p[GWL_P_PSI];
-  const double tc = p[GWL_P_GEOCENT_TIME], iota = p[GWL_P_THETA_JN];
-  double c[10], amp0, fisco;
-  const int valid = tf2_row_constants(p, ev.fref, 1.0, c, amp0, fisco);
-  const double ci = cos(iota);
-  const double ap = amp0 * (1.0 + ci * ci) / 2.0, ax = amp0 * ci;
-  double dt[ND];
-  R cre[ND], cim[ND];
-#pragma unroll
-  for (int d = 0; d < ND; ++d) {
-    double fp, fc, delay;
-    antenna_and_delay(ev.tensor[d], ev.vertex[d], ra, dec, tc, psi, fp, fc, delay);
-    dt[d] = (tc - ev.start) + delay;  // interferometer.py:355-358
-    cre[d] = (R)(fp * ap);
-    cim[d] = (R)(-fc * ax);
-  }
-  const double flo = ev.wf_fmin, fhi = fmin(fisco, ev.wf_fmax);
-
+  const RelRow rr = uniform_copy(a.rows[row]);
+  const double flo = a.ev.wf_fmin;
   R dhr = 0, dhi = 0, hh = 0;
   C cur[ND], nxt[ND], right[ND];
-  edge_ratio<R, ND>(a, c, dt, cre, cim, flo, fhi, lane, cur);
+  edge_ratio<R, ND>(a, rr, flo, lane, cur);
   for (int base = 0; base < a.nb; base += 64) {
-    edge_ratio<R, ND>(a, c, dt, cre, cim, flo, fhi, base + 64 + lane, nxt);
+    if (base + 64 <= a.nb) {
+      edge_ratio<R, ND>(a, rr, flo, base + 64 + lane, nxt);
+    } else {
+#pragma unroll
+      for (int d = 0; d < ND; ++d) nxt[d] = cmk<C>((R)0, (R)0);
+    }
 #pragma unroll
     for (int d = 0; d < ND; ++d) {
       const C up = shfl_c(cur[d], (lane + 1) & 63);
@@ -178,19 +216,35 @@ __global__ __launch_bounds__(256) void k_relbin(RelbinArgs a) {
   const double dh_re = reduce_wave((double)dhr);
   const double dh_im = reduce_wave((double)dhi);
   const double h2 = reduce_wave((double)hh);
-  if (lane != 0) return;
-  if (a.aux) {
-    a.aux[3 * row + 0] = dh_re;
-    a.aux[3 * row + 1] = dh_im;
-    a.aux[3 * row + 2] = h2;
+  if (lane == 0) {
+    a.sums[3 * row + 0] = dh_re;
+    a.sums[3 * row + 1] = dh_im;
+    a.sums[3 * row + 2] = h2;
   }
-  if (!valid) {
-    a.out[row] = row_valid_sentinel();
+}
+
+// One thread per row: base.py:448-477 on the summed inner products.
+__global__ __launch_bounds__(128) void k_relbin_epi(RelbinArgs a) {
+  extern __shared__ double knots[];
+  const EvDev& ev = a.ev;
+  const bool dist = (ev.flags & GWL_DISTANCE_MARG) != 0;
+  if (dist) stage_knots(ev, knots);
+  __syncthreads();
+  const long long r = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= a.n) return;
+  const double dh_re = a.sums[3 * r + 0], dh_im = a.sums[3 * r + 1], h2 = a.sums[3 * r + 2];
+  if (a.aux) {
+    a.aux[3 * r + 0] = dh_re;
+    a.aux[3 * r + 1] = dh_im;
+    a.aux[3 * r + 2] = h2;
+  }
+  if (!a.rows[r].valid) {
+    a.out[r] = row_valid_sentinel();
     return;
   }
   double ll;
   if (dist) {
-    const double hs = p[GWL_P_LUMINOSITY_DISTANCE] / ev.dref;
+    const double hs = a.params[r * a.ld + GWL_P_LUMINOSITY_DISTANCE] / ev.dref;
     const double dd = (ev.flags & GWL_PHASE_MARG) ? hypot(dh_re, dh_im) * hs : dh_re * hs;
     const SplineY yy = spline_y(ev, knots + ev.sp.nx + 4, h2 * hs * hs);
     ll = spline_x(ev, knots, yy, dd);
@@ -199,7 +253,7 @@ __global__ __launch_bounds__(256) void k_relbin(RelbinArgs a) {
   } else {
     ll = dh_re - h2 / 2.0;
   }
-  a.out[row] = ll;
+  a.out[r] = ll;
 }
 
 __global__ void k_relbin_basis(const double* fe, int n, double* basis) {
@@ -208,15 +262,30 @@ __global__ void k_relbin_basis(const double* fe, int n, double* basis) {
 }
 
 template <typename R>
-int launch_relbin_nd(const RelbinArgs& a, int nd, size_t lds, hipStream_t st) {
+int launch_relbin_nd(const RelbinArgs& a, int nd, hipStream_t st) {
   const dim3 grid((unsigned)((a.n + 3) / 4)), block(256);
   switch (nd) {
-    case 1: hipLaunchKernelGGL((k_relbin<R, 1>), grid, block, lds, st, a); break;
-    case 2: hipLaunchKernelGGL((k_relbin<R, 2>), grid, block, lds, st, a); break;
-    case 3: hipLaunchKernelGGL((k_relbin<R, 3>), grid, block, lds, st, a); break;
-    default: hipLaunchKernelGGL((k_relbin<R, 4>), grid, block, lds, st, a); break;
+    case 1: hipLaunchKernelGGL((k_relbin<R, 1>), grid, block, 0, st, a); break;
+    case 2: hipLaunchKernelGGL((k_relbin<R, 2>), grid, block, 0, st, a); break;
+    case 3: hipLaunchKernelGGL((k_relbin<R, 3>), grid, block, 0, st, a); break;
+    default: hipLaunchKernelGGL((k_relbin<R, 4>), grid, block, 0, st, a); break;
   }
   GWL_HIP_CHECK(hipGetLastError());
+  return GWL_OK;
+}
+
+int relbin_reserve(gwl_relbin* rb, long long rows) {
+  const size_t need = relbin_ws_bytes(rows);
+  if (need <= rb->ws_bytes) return GWL_OK;
+  GWL_HIP_CHECK(hipSetDevice(rb->ev->ctx->device));
+  if (rb->ws) (void)hipFree(rb->ws);
+  rb->ws = nullptr;
+  rb->ws_bytes = 0;
+  if (hipMalloc(&rb->ws, need) != hipSuccess) {
+    set_error("hipMalloc failed for the relative-binning workspace");
+    return GWL_E_NOMEM;
+  }
+  rb->ws_bytes = need;
   return GWL_OK;
 }
 
@@ -320,13 +389,19 @@ int gwl_relbin_destroy(gwl_relbin* rb) {
   if (!rb) return GWL_OK;
   (void)hipSetDevice(rb->ev->ctx->device);
   if (rb->data) (void)hipFree(rb->data);
+  if (rb->ws) (void)hipFree(rb->ws);
   delete rb;
   return GWL_OK;
 }
 
+int gwl_relbin_reserve(gwl_relbin* rb, int64_t rows) {
+  if (!rb || rows < 1) { set_error("gwl_relbin_reserve: invalid argument"); return GWL_E_INVALID; }
+  return gwl::relbin_reserve(rb, rows);
+}
+
 int gwl_relbin_device_bytes(gwl_relbin* rb, int64_t* bytes) {
   if (!rb || !bytes) { set_error("gwl_relbin_device_bytes: invalid argument"); return GWL_E_INVALID; }
-  *bytes = (int64_t)rb->data_bytes;
+  *bytes = (int64_t)(rb->data_bytes + rb->ws_bytes);
   return GWL_OK;
 }
 
@@ -343,6 +418,7 @@ int gwl_relbin_eval(gwl_relbin* rb, const double* params, int64_t n, int32_t ld,
     set_error("gwl_relbin_eval: distance marginalisation requested but no spline attached");
     return GWL_E_INVALID;
   }
+  if (int rc = gwl::relbin_reserve(rb, n)) return rc;
   GWL_HIP_CHECK(hipSetDevice(ev->ctx->device));
   (void)hipGetLastError();
   gwl::RelbinArgs a;
@@ -355,15 +431,23 @@ int gwl_relbin_eval(gwl_relbin* rb, const double* params, int64_t n, int32_t ld,
   a.params = params;
   a.ld = ld;
   a.n = n;
+  char* w = static_cast<char*>(rb->ws);
+  a.rows = reinterpret_cast<gwl::RelRow*>(w);
+  a.sums = reinterpret_cast<double*>(w + ((size_t)n * sizeof(gwl::RelRow) + 255) / 256 * 256);
   a.out = out;
   a.aux = aux;
   const size_t lds = (ev->dev.flags & GWL_DISTANCE_MARG)
                          ? (size_t)(ev->dev.sp.nx + ev->dev.sp.ny + 8) * 8 : 0;
   hipStream_t st = reinterpret_cast<hipStream_t>(stream);
   if (int rc = gwl::timing_begin(ev, st)) return rc;
-  const int rc = precision == GWL_FP64 ? gwl::launch_relbin_nd<double>(a, ev->dev.nd, lds, st)
-                                       : gwl::launch_relbin_nd<float>(a, ev->dev.nd, lds, st);
+  const unsigned g128 = (unsigned)((n + 127) / 128);
+  hipLaunchKernelGGL(gwl::k_relbin_rows, dim3(g128), dim3(128), 0, st, a);
+  GWL_HIP_CHECK(hipGetLastError());
+  const int rc = precision == GWL_FP64 ? gwl::launch_relbin_nd<double>(a, ev->dev.nd, st)
+                                       : gwl::launch_relbin_nd<float>(a, ev->dev.nd, st);
   if (rc != GWL_OK) return rc;
+  hipLaunchKernelGGL(gwl::k_relbin_epi, dim3(g128), dim3(128), lds, st, a);
+  GWL_HIP_CHECK(hipGetLastError());
   return gwl::timing_end(ev, st);
 }
 

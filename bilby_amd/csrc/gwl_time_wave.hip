@@ -121,18 +121,6 @@ __device__ __forceinline__ WaveEv wave_ev(const EvDev& ev) {
   return w;
 }
 
-// Copy of a wave-uniform value held in scalar registers.
-template <typename T>
-__device__ __forceinline__ T uniform_copy(const T& v) {
-  static_assert(sizeof(T) % 4 == 0, "word-sized type");
-  T out;
-  const int* src = reinterpret_cast<const int*>(&v);
-  int* dst = reinterpret_cast<int*>(&out);
-#pragma unroll
-  for (int k = 0; k < (int)(sizeof(T) / 4); ++k) dst[k] = __builtin_amdgcn_readfirstlane(src[k]);
-  return out;
-}
-
 // h conj(d) / S at decimated element i (bin nbin) for one row, summed over
 // detectors (interferometer.py:303-368, base.py:327-329): float64 phase,
 // reduced to [0, 1) cycles, float32 rotation and products.

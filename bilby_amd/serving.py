@@ -122,6 +122,8 @@ class LikelihoodServer:
         budget; over budget raises HBMBudgetExceeded."""
         staging = self.max_batch_rows * (_lib.NPARAM + 1) * 8
         if getattr(likelihood, "_relbin", None) is not None:
+            check(LIB.gwl_relbin_reserve(likelihood._relbin, self.max_batch_rows),
+                  "gwl_relbin_reserve")
             nbytes = _device_bytes(likelihood)
             self._admit(name, nbytes + staging)
             i = ctypes.c_int32()

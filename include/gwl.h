@@ -307,6 +307,8 @@ typedef struct gwl_relbin gwl_relbin;
 int gwl_relbin_create(gwl_event* ev, const gwl_relbin_desc* desc, gwl_relbin** out);
 int gwl_relbin_destroy(gwl_relbin* rb);
 int gwl_relbin_device_bytes(gwl_relbin* rb, int64_t* bytes);
+/* Pre-size the per-row workspace (grown on demand by gwl_relbin_eval). */
+int gwl_relbin_reserve(gwl_relbin* rb, int64_t rows);
 /* ln L ratio of n rows: calculate_snrs (relative.py:393-428) + the
  * marginalisation of base.py:448-477.  params: device [n][ld] float64;
  * out (and aux [n][3] = Re <d|h>, Im <d|h>, <h|h>, may be NULL): device.
