@@ -29,4 +29,7 @@ Pinning status (what checks each function, see tests/test_oracle_*.py):
 * phase marginalisation: pinned by numerical integration over phase.
 * distance marginalisation: pinned by direct 10^4-point integration.
 * ROQ: pinned by brute-force evaluation of the weights' defining sums.
+* relative binning: pinned at the fiducial point (equal to the full
+  likelihood), by the piecewise-linear reconstruction summed on the grid,
+  and by the reference's draw property (tests/test_oracle_relative.py).
 """
