@@ -175,6 +175,8 @@ class Interferometer:
         self.start_time = 0.0
         self.frequency_array = None
         self._strain = None
+        # calibration model (bilby_amd.calibration); None = no recalibration
+        self.calibration_model = None
 
     @property
     def maximum_frequency(self):

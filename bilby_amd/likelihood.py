@@ -142,8 +142,9 @@ class GravitationalWaveTransient:
                  distance_marginalization_lookup_table=None, calibration_lookup_table=None,
                  number_of_response_curves=1000, starting_index=0, jitter_time=True,
                  reference_frame="sky", time_reference="geocenter", precision="fp32", device=0):
-        if calibration_marginalization:
-            raise NotImplementedError("calibration marginalisation is not supported")
+        if calibration_marginalization and time_marginalization:
+            raise NotImplementedError("calibration marginalisation together with time "
+                                      "marginalisation is not supported")
         if reference_frame != "sky":
             raise NotImplementedError("only reference_frame='sky' is supported")
         if "geocent" not in time_reference:
@@ -153,14 +154,19 @@ class GravitationalWaveTransient:
         self.time_marginalization = time_marginalization
         self.distance_marginalization = distance_marginalization
         self.phase_marginalization = phase_marginalization
-        self.calibration_marginalization = False
+        self.calibration_marginalization = calibration_marginalization
+        self._calib = None
         if priors is not None:
             self.priors = priors.copy()
         elif time_marginalization or distance_marginalization or phase_marginalization:
             raise ValueError("You can't use a marginalized likelihood without specifying a priors")
         else:
             self.priors = None
-        self.precision = _precision_code(precision)
+        # calibration marginalisation also takes "bf16x3" (split-bfloat16 MFMA contraction)
+        self.calibration_mode = 2 if precision == "bf16x3" else None
+        self.precision = _precision_code("fp32" if precision == "bf16x3" else precision)
+        if self.calibration_mode is None:
+            self.calibration_mode = self.precision
         self.device = device
         self._marginalized_parameters = []
         self._noise_log_likelihood_value = None
@@ -199,6 +205,11 @@ class GravitationalWaveTransient:
         self._build_event()
         if self.distance_marginalization:
             self._setup_distance_marginalization(distance_marginalization_lookup_table)
+        if self.calibration_marginalization:
+            self.number_of_response_curves = number_of_response_curves
+            self.starting_index = starting_index
+            self._setup_calibration_marginalization(calibration_lookup_table, priors)
+            self._marginalized_parameters.append("recalib_index")
         self._one = None
 
     # ------------------------------------------------------------------ setup
@@ -289,6 +300,52 @@ class GravitationalWaveTransient:
                                                 dptr(ty), dptr(c), dptr(bbox)),
               "gwl_event_set_distance_spline")
 
+    def _setup_calibration_marginalization(self, calibration_lookup_table, priors=None):
+        """base.py:1037-1051: response curves per detector on its masked
+        frequencies; the sampled calibration parameters become fixed at 0 in
+        the caller's priors; the curves go to the device (gwl_calib_create)."""
+        from .calibration import build_calibration_lookup
+        from .prior import DeltaFunction
+        self.calibration_draws, self.calibration_parameter_draws = build_calibration_lookup(
+            self.interferometers, lookup_files=calibration_lookup_table, priors=priors,
+            number_of_response_curves=self.number_of_response_curves,
+            starting_index=self.starting_index)
+        if priors is not None:
+            for name, params in self.calibration_parameter_draws.items():
+                for key in set(params).intersection(priors.keys()):
+                    priors[key] = DeltaFunction(0.0)
+        self.calibration_abs_draws = {k: np.abs(v) ** 2 for k, v in self.calibration_draws.items()}
+        curves = np.concatenate([np.asarray(self.calibration_draws[ifo.name], dtype=complex)
+                                 for ifo in self.interferometers], axis=1)
+        blob = np.ascontiguousarray(curves.view(np.float64))
+        cal = ctypes.c_void_p()
+        check(LIB.gwl_calib_create(self._event, int(curves.shape[0]), dptr(blob), ctypes.byref(cal)),
+              "gwl_calib_create")
+        self._calib = cal
+
+    def get_calibration_log_likelihoods(self, parameters):
+        """base.py:822-858: ln L of every response curve for one parameter
+        dict (float64 contraction)."""
+        rows = self.rows_from_parameters(dict(parameters))
+        rows_t = torch.from_numpy(np.ascontiguousarray(rows[:1])).to("cuda:%d" % self.device)
+        out = torch.empty(1, dtype=torch.float64, device=rows_t.device)
+        per = torch.empty((1, self.number_of_response_curves), dtype=torch.float64,
+                          device=rows_t.device)
+        st = torch.cuda.current_stream(rows_t.device).cuda_stream
+        check(LIB.gwl_calib_eval(self._calib, ctypes.c_void_p(rows_t.data_ptr()), 1, _lib.NPARAM,
+                                 _lib.FP64, ctypes.c_void_p(out.data_ptr()),
+                                 ctypes.c_void_p(per.data_ptr()), ctypes.c_void_p(st)),
+              "gwl_calib_eval")
+        return per.cpu().numpy()[0]
+
+    def generate_calibration_sample_from_marginalized_likelihood(self, parameters, rng=None):
+        """base.py:544-578: draw a response-curve index from its posterior."""
+        log_like = self.get_calibration_log_likelihoods(parameters)
+        post = np.exp(log_like - np.max(log_like))
+        post /= np.sum(post)
+        rng = rng if rng is not None else np.random.default_rng()
+        return int(rng.choice(self.number_of_response_curves, p=post))
+
     def create_lookup_table(self):
         """base.py:994-1018 on the GPU (gwl_distance_table)."""
         return create_lookup_table(self.device, self._distance_array, self.distance_prior_array,
@@ -296,6 +353,9 @@ class GravitationalWaveTransient:
 
     def __del__(self):
         try:
+            if getattr(self, "_calib", None) is not None:
+                LIB.gwl_calib_destroy(self._calib)
+                self._calib = None
             if self._event is not None:
                 LIB.gwl_event_destroy(self._event)
                 self._event = None
@@ -355,11 +415,31 @@ class GravitationalWaveTransient:
         out = torch.empty(n, dtype=torch.float64, device=dev)
         aux_t = torch.empty((n, 3), dtype=torch.float64, device=dev) if aux else None
         st = torch.cuda.current_stream(dev).cuda_stream if stream is None else stream
+        if self._calib is not None:
+            if aux:
+                raise NotImplementedError("inner products are not returned under calibration "
+                                          "marginalisation")
+            check(LIB.gwl_calib_eval(self._calib, ctypes.c_void_p(rows_t.data_ptr()), n,
+                                     _lib.NPARAM, int(precision), ctypes.c_void_p(out.data_ptr()), None,
+                                     ctypes.c_void_p(st)), "gwl_calib_eval")
+            return out, None
         check(LIB.gwl_eval(self._event, ctypes.c_void_p(rows_t.data_ptr()), n, _lib.NPARAM,
                            precision, ctypes.c_void_p(out.data_ptr()),
                            ctypes.c_void_p(aux_t.data_ptr()) if aux else None,
                            ctypes.c_void_p(st)), "gwl_eval")
         return out, aux_t
+
+    def _mode(self, precision):
+        """Precision code of one call: the constructor's choice by default;
+        "bf16x3" selects the split-bfloat16 contraction under calibration
+        marginalisation."""
+        if precision is None:
+            return self.calibration_mode if self._calib is not None else self.precision
+        if precision == "bf16x3":
+            if self._calib is None:
+                raise ValueError("bf16x3 applies to calibration marginalisation only")
+            return 2
+        return _precision_code(precision)
 
     def log_likelihood_ratio_batch(self, parameters, precision=None, return_inner_products=False,
                                    stream=None):
@@ -370,7 +450,7 @@ class GravitationalWaveTransient:
         [N, 16] already on the GPU (then a GPU tensor is returned and nothing
         is synchronised).
         """
-        prec = self.precision if precision is None else _precision_code(precision)
+        prec = self._mode(precision)
         on_device = isinstance(parameters, torch.Tensor) and parameters.is_cuda
         if on_device:
             rows_t = parameters.to(torch.float64).contiguous()
@@ -399,7 +479,7 @@ class GravitationalWaveTransient:
         host, dev = self._one
         host.numpy()[:] = rows[:1]
         dev.copy_(host, non_blocking=True)
-        out, _ = self._device_eval(dev, self.precision)
+        out, _ = self._device_eval(dev, self._mode(None))
         return float(out.item())
 
     def log_likelihood(self, parameters):

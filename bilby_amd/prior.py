@@ -40,6 +40,24 @@ class Uniform(Prior):
         return ((val >= self.minimum) & (val <= self.maximum)) / (self.maximum - self.minimum)
 
 
+class Gaussian(Prior):
+    """bilby/core/prior/analytical.py Gaussian: rescale mu + erfinv(2u - 1)
+    sqrt(2) sigma."""
+
+    def __init__(self, mu, sigma, name=None, boundary=None, **kw):
+        super().__init__(name=name, boundary=boundary)
+        self.mu = mu
+        self.sigma = sigma
+
+    def rescale(self, val):
+        from scipy.special import erfinv
+        return self.mu + erfinv(2 * np.asarray(val, float) - 1) * 2 ** 0.5 * self.sigma
+
+    def prob(self, val):
+        val = np.asarray(val, float)
+        return np.exp(-(self.mu - val) ** 2 / (2 * self.sigma ** 2)) / (2 * np.pi) ** 0.5 / self.sigma
+
+
 class DeltaFunction(Prior):
     def __init__(self, peak, name=None, **kw):
         super().__init__(minimum=peak, maximum=peak, name=name)

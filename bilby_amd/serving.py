@@ -121,6 +121,8 @@ class LikelihoodServer:
         admit its device bytes plus the server's staging for it against the
         budget; over budget raises HBMBudgetExceeded."""
         staging = self.max_batch_rows * (_lib.NPARAM + 1) * 8
+        if getattr(likelihood, "_calib", None) is not None:
+            raise NotImplementedError("calibration-marginalised likelihoods are not served")
         if getattr(likelihood, "_relbin", None) is not None:
             check(LIB.gwl_relbin_reserve(likelihood._relbin, self.max_batch_rows),
                   "gwl_relbin_reserve")
