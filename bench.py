@@ -302,6 +302,68 @@ def bench_relbin(steps):
     return out
 
 
+def bench_calibration(steps, batch=2048, n_curves=1000):
+    """Calibration marginalisation on the headline event (TaylorF2 H1+L1+V1,
+    8 s @ 4096 Hz, 3 x 16225 masked bins), distance + phase marginalised,
+    1000 cubic-spline response curves per detector (5 nodes, 10 %% / 0.1 rad),
+    a batch of prior draws: prep, two MFMA contractions, epilogue."""
+    import ctypes
+    import tempfile
+    import numpy as np
+    import torch
+    from bilby_amd import calibration as cal
+    from bilby_amd._lib import LIB, check
+    from bilby_amd.configs import make_taylorf2_marginalised
+    from bilby_amd.likelihood import GravitationalWaveTransient
+    from oracle.calibration import OracleCalibrated
+    from oracle.likelihood import OracleDetector, OracleEvent
+    from tests.helpers import draws_as_dicts, fixed_marginalised
+    s = make_taylorf2_marginalised(n_draws=batch, seed=4321, time=False)
+    priors = s["priors"]
+    ifos = s["interferometers"]
+    for ifo in ifos:
+        ifo.calibration_model = cal.CubicSpline("recalib_%s_" % ifo.name, 20.0, 2048.0, 5)
+        priors.update(cal.constant_uncertainty_spline_priors(0.1, 0.1, 20.0, 2048.0, 5, ifo.name))
+    tmp = tempfile.mkdtemp(prefix="gwl-cal-")
+    files = {ifo.name: os.path.join(tmp, ifo.name + ".npz") for ifo in ifos}
+    opri = dict(priors)
+    lk_priors = priors.copy()
+    lk = GravitationalWaveTransient(ifos, s["waveform_generator"], calibration_marginalization=True,
+                                    distance_marginalization=True, phase_marginalization=True,
+                                    priors=lk_priors, number_of_response_curves=n_curves,
+                                    calibration_lookup_table=files, precision="fp32")
+    draws = fixed_marginalised(s["draws"], lk_priors)
+    rows = torch.from_numpy(lk.rows_from_parameters(draws)).cuda()
+    K = sum(int(i.frequency_mask.sum()) for i in ifos)
+    useful = 10.0 * K * n_curves   # 8 K C (complex <d|h>) + 2 K C (<h|h>) per row
+    out = {"rows": batch, "curves": n_curves, "masked_bins": K,
+           "useful_gflop_per_eval": useful / 1e9}
+    check(LIB.gwl_calib_reserve(lk._calib, batch))
+    res = {}
+    for mode in ("fp32", "bf16x3", "fp64"):
+        st = steps if mode != "fp64" else 1
+        dt = timed(lambda: lk.log_likelihood_ratio_batch(rows, precision=mode), st, warmup=1)
+        out[mode + "_evals_per_s"] = batch / dt
+        out[mode + "_ms_per_batch"] = dt * 1e3
+        out[mode + "_tflops_useful"] = batch * useful / dt / 1e12
+        res[mode] = lk.log_likelihood_ratio_batch(rows, precision=mode).cpu().numpy()
+    for mode in ("fp32", "bf16x3"):
+        out[mode + "_vs_fp64_max"] = float(np.max(np.abs(res[mode] - res["fp64"])))
+    dets = [OracleDetector(i.name, i.frequency_array, i.frequency_domain_strain,
+                           i.power_spectral_density_array, i.frequency_mask) for i in ifos]
+    ev = OracleEvent(dets, ifos.duration, ifos.sampling_frequency, ifos.start_time)
+    olk = OracleCalibrated(ev, "taylorf2", lk.calibration_draws,
+                           waveform_arguments=dict(s["waveform_generator"].waveform_arguments),
+                           distance_marginalization=True, phase_marginalization=True, priors=opri,
+                           distance_table=lk._dist_margd_loglikelihood_array)
+    t0 = time.perf_counter()
+    ref = np.array([olk.log_likelihood_ratio(p) for p in draws_as_dicts(draws, range(8))])
+    out["oracle_evals_per_s_1core"] = 8 / (time.perf_counter() - t0)
+    for mode in res:
+        out[mode + "_vs_oracle_8rows"] = float(np.max(np.abs(res[mode][:8] - ref)))
+    return out
+
+
 def bench_roq(steps):
     """ROQ, H1+L1, 400 linear / 120 quadratic nodes, 1000 ROQ times,
     time + phase marginalised, 65536 TaylorF2 draws: the contraction
@@ -671,6 +733,7 @@ def main():
                          ("uneven_64_events_persistent_graph", lambda: bench_uneven(5)),
                          ("roq_400_120_nodes_1000_times_65536", lambda: bench_roq(3)),
                          ("relative_binning_dp_event_65536", lambda: bench_relbin(5)),
+                         ("calibration_1000_curves_dp_event", lambda: bench_calibration(3)),
                          ("serving_32_samplers_2000_rps", lambda: bench_serving(
                              lk, rows.cpu().numpy(), args.serve_seconds)),
                          ("sampler_callers_pool_and_16_processes", lambda: bench_sampler(
