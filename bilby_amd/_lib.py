@@ -132,6 +132,12 @@ def _load():
         "gwl_relbin_create": ([vp, ctypes.POINTER(RelbinDesc), ctypes.POINTER(vp)], ctypes.c_int),
         "gwl_relbin_destroy": ([vp], ctypes.c_int),
         "gwl_relbin_reserve": ([vp, ctypes.c_int64], ctypes.c_int),
+        "gwl_mb_create": ([vp, ctypes.c_int32, _dp, _dp, _dp, ctypes.POINTER(vp)], ctypes.c_int),
+        "gwl_mb_destroy": ([vp], ctypes.c_int),
+        "gwl_mb_reserve": ([vp, ctypes.c_int64], ctypes.c_int),
+        "gwl_mb_device_bytes": ([vp, ctypes.POINTER(ctypes.c_int64)], ctypes.c_int),
+        "gwl_mb_eval": ([vp, vp, ctypes.c_int64, ctypes.c_int32, ctypes.c_int32, vp, vp, vp],
+                        ctypes.c_int),
         "gwl_calib_create": ([vp, ctypes.c_int32, _dp, ctypes.POINTER(vp)], ctypes.c_int),
         "gwl_calib_destroy": ([vp], ctypes.c_int),
         "gwl_calib_reserve": ([vp, ctypes.c_int64], ctypes.c_int),
@@ -186,4 +192,5 @@ EXPORTED_SYMBOLS = [
     "gwl_server_listen", "gwl_relbin_create", "gwl_relbin_destroy", "gwl_relbin_device_bytes",
     "gwl_relbin_eval", "gwl_server_add_relbin", "gwl_relbin_reserve", "gwl_calib_create",
     "gwl_calib_destroy", "gwl_calib_reserve", "gwl_calib_device_bytes", "gwl_calib_eval",
+    "gwl_mb_create", "gwl_mb_destroy", "gwl_mb_reserve", "gwl_mb_device_bytes", "gwl_mb_eval",
 ]

@@ -1,3 +1,6 @@
+// Binned likelihoods: relative binning (below) and multibanding (gwl_mb,
+// k_mb), sharing the per-row constants and the marginalisation epilogue.
+//
 // Relative-binning likelihood (bilby/gw/likelihood/relative.py,
 // RelativeBinningGravitationalWaveTransient; Zackay, Dai & Venumadhav,
 // arXiv:1806.08792).
@@ -31,6 +34,7 @@
 #include "gwl_internal.h"
 
 #include <cmath>
+#include <cstring>
 #include <vector>
 
 using gwl::set_error;
@@ -47,6 +51,25 @@ struct gwl_relbin {
   double* s64 = nullptr;      // [nd][nb][6] A0 re, A0 im, A1 re, A1 im, B0, B1
   float* s32 = nullptr;
   void* ws = nullptr;         // row constants and per-row sums
+  size_t ws_bytes = 0;
+};
+
+// Multibanded likelihood (bilby/gw/likelihood/multiband.py:728-807, linear
+// interpolation): per banded point the frequency, TaylorF2 phase basis,
+// f^-7/6, and per detector the linear (complex) and quadratic coefficients.
+struct gwl_mb {
+  gwl_event* ev = nullptr;
+  int n = 0;
+  void* data = nullptr;
+  size_t data_bytes = 0;
+  double* fe = nullptr;       // [n]
+  double* basis = nullptr;    // [n][kNB]
+  double* amp = nullptr;      // [n] f^-7/6
+  double2* lin64 = nullptr;   // [nd][n]
+  float2* lin32 = nullptr;
+  double* quad64 = nullptr;   // [nd][n]
+  float* quad32 = nullptr;
+  void* ws = nullptr;
   size_t ws_bytes = 0;
 };
 
@@ -223,6 +246,96 @@ __global__ __launch_bounds__(256) void k_relbin(RelbinArgs a) {
   }
 }
 
+struct MbArgs {
+  int n;
+  double flo;
+  double inv_sigma;   // tables hold L sigma and Q sigma^2 (float32 range)
+  const double* fe;
+  const double* basis;
+  const double* amp;
+  const void* lin;    // double2 or float2 [nd][n]
+  const void* quad;   // double or float [nd][n]
+  long long rows_n;
+  const RelRow* rows;
+  double* sums;
+};
+
+// One wavefront per row, lanes over the banded points: the detector
+// response at each point (interferometer.py:303-368 with the frequencies
+// given), <d|h> = conj(sum_p h_p L_p) and <h|h> = sum_p |h_p|^2 Q_p
+// (multiband.py:755-765), float64 per-lane sums.
+template <typename R, int ND>
+__global__ __launch_bounds__(256) void k_mb(MbArgs a) {
+  using C = typename RelbinT<R>::C;
+  const int lane = threadIdx.x & 63;
+  const long long row = __builtin_amdgcn_readfirstlane(
+      (int)((long long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)));
+  if (row >= a.rows_n) return;
+  const RelRow rr = uniform_copy(a.rows[row]);
+  double cabs2[ND];
+#pragma unroll
+  for (int d = 0; d < ND; ++d) {
+    const double cr = rr.cre[d] * a.inv_sigma, ci = rr.cim[d] * a.inv_sigma;
+    cabs2[d] = cr * cr + ci * ci;
+  }
+  double dr = 0.0, di = 0.0, hh = 0.0;
+  for (int p = lane; p < a.n; p += 64) {
+    const double f = a.fe[p];
+    if (!(f >= a.flo && f <= rr.fhi)) continue;
+    double b[kNB];
+#pragma unroll
+    for (int k = 0; k < kNB; ++k) b[k] = a.basis[(size_t)p * kNB + k];
+    const double psi = tf2_phase_cycles(rr.c, b);
+    const double A = a.amp[p];
+    const R Ar = (R)A;
+#pragma unroll
+    for (int d = 0; d < ND; ++d) {
+      double ph = fma(f, rr.dt[d], psi);
+      ph -= rint(ph);
+      const C w = expm2pi((R)ph);
+      const C h = cmul(cmk<C>((R)(rr.cre[d] * a.inv_sigma) * Ar, (R)(rr.cim[d] * a.inv_sigma) * Ar), w);
+      const C L = RelbinT<R>::load(a.lin, (size_t)d * a.n + p);
+      dr += (double)(h.x * L.x - h.y * L.y);
+      di += (double)(h.x * L.y + h.y * L.x);
+      hh += cabs2[d] * A * A * (double)RelbinT<R>::sval(a.quad, (size_t)d * a.n + p);
+    }
+  }
+  dr = reduce_wave(dr);
+  di = reduce_wave(di);
+  hh = reduce_wave(hh);
+  if (lane == 0) {
+    a.sums[3 * row + 0] = dr;
+    a.sums[3 * row + 1] = -di;  // d_inner_h = conj(sum h L)
+    a.sums[3 * row + 2] = hh;
+  }
+}
+
+__global__ void k_mb_tables(const double* fe, int n, double* basis, double* amp) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double f = fe[i];
+  if (f > 0.0) {
+    tf2_basis(f, basis + (size_t)i * kNB);
+    amp[i] = basis[(size_t)i * kNB + 1] * sqrt(basis[(size_t)i * kNB + 3]);  // f^-1 f^-1/6
+  } else {
+    for (int k = 0; k < kNB; ++k) basis[(size_t)i * kNB + k] = 0.0;
+    amp[i] = 0.0;
+  }
+}
+
+template <typename R>
+int launch_mb_nd(const MbArgs& a, int nd, hipStream_t st) {
+  const dim3 grid((unsigned)((a.rows_n + 3) / 4)), block(256);
+  switch (nd) {
+    case 1: hipLaunchKernelGGL((k_mb<R, 1>), grid, block, 0, st, a); break;
+    case 2: hipLaunchKernelGGL((k_mb<R, 2>), grid, block, 0, st, a); break;
+    case 3: hipLaunchKernelGGL((k_mb<R, 3>), grid, block, 0, st, a); break;
+    default: hipLaunchKernelGGL((k_mb<R, 4>), grid, block, 0, st, a); break;
+  }
+  GWL_HIP_CHECK(hipGetLastError());
+  return GWL_OK;
+}
+
 // One thread per row: base.py:448-477 on the summed inner products.
 __global__ __launch_bounds__(128) void k_relbin_epi(RelbinArgs a) {
   extern __shared__ double knots[];
@@ -274,19 +387,23 @@ int launch_relbin_nd(const RelbinArgs& a, int nd, hipStream_t st) {
   return GWL_OK;
 }
 
-int relbin_reserve(gwl_relbin* rb, long long rows) {
+int grow_ws(gwl_event* ev, void** ws, size_t* ws_bytes, long long rows) {
   const size_t need = relbin_ws_bytes(rows);
-  if (need <= rb->ws_bytes) return GWL_OK;
-  GWL_HIP_CHECK(hipSetDevice(rb->ev->ctx->device));
-  if (rb->ws) (void)hipFree(rb->ws);
-  rb->ws = nullptr;
-  rb->ws_bytes = 0;
-  if (hipMalloc(&rb->ws, need) != hipSuccess) {
-    set_error("hipMalloc failed for the relative-binning workspace");
+  if (need <= *ws_bytes) return GWL_OK;
+  GWL_HIP_CHECK(hipSetDevice(ev->ctx->device));
+  if (*ws) (void)hipFree(*ws);
+  *ws = nullptr;
+  *ws_bytes = 0;
+  if (hipMalloc(ws, need) != hipSuccess) {
+    set_error("hipMalloc failed for the row workspace");
     return GWL_E_NOMEM;
   }
-  rb->ws_bytes = need;
+  *ws_bytes = need;
   return GWL_OK;
+}
+
+int relbin_reserve(gwl_relbin* rb, long long rows) {
+  return grow_ws(rb->ev, &rb->ws, &rb->ws_bytes, rows);
 }
 
 }  // namespace
@@ -445,6 +562,154 @@ int gwl_relbin_eval(gwl_relbin* rb, const double* params, int64_t n, int32_t ld,
   GWL_HIP_CHECK(hipGetLastError());
   const int rc = precision == GWL_FP64 ? gwl::launch_relbin_nd<double>(a, ev->dev.nd, st)
                                        : gwl::launch_relbin_nd<float>(a, ev->dev.nd, st);
+  if (rc != GWL_OK) return rc;
+  hipLaunchKernelGGL(gwl::k_relbin_epi, dim3(g128), dim3(128), lds, st, a);
+  GWL_HIP_CHECK(hipGetLastError());
+  return gwl::timing_end(ev, st);
+}
+
+}  // extern "C"
+
+extern "C" {
+
+int gwl_mb_create(gwl_event* ev, int32_t n_points, const double* frequencies, const double* linear,
+                  const double* quadratic, gwl_mb** out) {
+  if (!ev || n_points < 1 || !frequencies || !linear || !quadratic || !out) {
+    set_error("gwl_mb_create: invalid argument");
+    return GWL_E_INVALID;
+  }
+  const gwl::EvDev& d = ev->dev;
+  if (d.approx != GWL_TAYLORF2 || (d.flags & GWL_TIME_MARG)) {
+    set_error("gwl_mb_create: the event must be TaylorF2 without time marginalisation");
+    return GWL_E_UNSUPPORTED;
+  }
+  const int n = n_points, nd = d.nd;
+  // scaled by the event's sigma (|h| / sigma ~ 1) so that float32 holds them
+  const double sg = d.sigma;
+  std::vector<double2> l64((size_t)nd * n);
+  std::vector<float2> l32((size_t)nd * n);
+  std::vector<double> q64((size_t)nd * n);
+  std::vector<float> q32((size_t)nd * n);
+  for (size_t i = 0; i < (size_t)nd * n; ++i) {
+    l64[i] = make_double2(linear[2 * i] * sg, linear[2 * i + 1] * sg);
+    l32[i] = make_float2((float)l64[i].x, (float)l64[i].y);
+    q64[i] = quadratic[i] * sg * sg;
+    q32[i] = (float)q64[i];
+  }
+  gwl_mb* mb = new (std::nothrow) gwl_mb;
+  if (!mb) { set_error("out of host memory"); return GWL_E_NOMEM; }
+  mb->ev = ev;
+  mb->n = n;
+  auto al = [](size_t x) { return (x + 255) / 256 * 256; };
+  const size_t b_fe = al((size_t)n * 8), b_basis = al((size_t)n * gwl::kNB * 8), b_amp = al((size_t)n * 8);
+  const size_t b_l64 = al(l64.size() * 16), b_l32 = al(l32.size() * 8);
+  const size_t b_q64 = al((size_t)nd * n * 8), b_q32 = al(q32.size() * 4);
+  mb->data_bytes = b_fe + b_basis + b_amp + b_l64 + b_l32 + b_q64 + b_q32;
+  if (hipSetDevice(ev->ctx->device) != hipSuccess || hipMalloc(&mb->data, mb->data_bytes) != hipSuccess) {
+    delete mb;
+    set_error("hipMalloc failed for multiband tables");
+    return GWL_E_NOMEM;
+  }
+  char* q = static_cast<char*>(mb->data);
+  mb->fe = reinterpret_cast<double*>(q); q += b_fe;
+  mb->basis = reinterpret_cast<double*>(q); q += b_basis;
+  mb->amp = reinterpret_cast<double*>(q); q += b_amp;
+  mb->lin64 = reinterpret_cast<double2*>(q); q += b_l64;
+  mb->lin32 = reinterpret_cast<float2*>(q); q += b_l32;
+  mb->quad64 = reinterpret_cast<double*>(q); q += b_q64;
+  mb->quad32 = reinterpret_cast<float*>(q);
+  hipError_t err = hipMemcpy(mb->fe, frequencies, (size_t)n * 8, hipMemcpyHostToDevice);
+  if (err == hipSuccess) err = hipMemcpy(mb->lin64, l64.data(), l64.size() * 16, hipMemcpyHostToDevice);
+  if (err == hipSuccess) err = hipMemcpy(mb->lin32, l32.data(), l32.size() * 8, hipMemcpyHostToDevice);
+  if (err == hipSuccess) err = hipMemcpy(mb->quad64, q64.data(), q64.size() * 8, hipMemcpyHostToDevice);
+  if (err == hipSuccess) err = hipMemcpy(mb->quad32, q32.data(), q32.size() * 4, hipMemcpyHostToDevice);
+  if (err == hipSuccess) {
+    hipLaunchKernelGGL(gwl::k_mb_tables, dim3((unsigned)((n + 127) / 128)), dim3(128), 0, 0,
+                       mb->fe, n, mb->basis, mb->amp);
+    err = hipGetLastError();
+  }
+  if (err == hipSuccess) err = hipDeviceSynchronize();
+  if (err != hipSuccess) {
+    (void)hipFree(mb->data);
+    delete mb;
+    return gwl::hip_fail(err, "gwl_mb_create upload");
+  }
+  *out = mb;
+  return GWL_OK;
+}
+
+int gwl_mb_destroy(gwl_mb* mb) {
+  if (!mb) return GWL_OK;
+  (void)hipSetDevice(mb->ev->ctx->device);
+  if (mb->data) (void)hipFree(mb->data);
+  if (mb->ws) (void)hipFree(mb->ws);
+  delete mb;
+  return GWL_OK;
+}
+
+int gwl_mb_reserve(gwl_mb* mb, int64_t rows) {
+  if (!mb || rows < 1) { set_error("gwl_mb_reserve: invalid argument"); return GWL_E_INVALID; }
+  return gwl::grow_ws(mb->ev, &mb->ws, &mb->ws_bytes, rows);
+}
+
+int gwl_mb_device_bytes(gwl_mb* mb, int64_t* bytes) {
+  if (!mb || !bytes) { set_error("gwl_mb_device_bytes: invalid argument"); return GWL_E_INVALID; }
+  *bytes = (int64_t)(mb->data_bytes + mb->ws_bytes);
+  return GWL_OK;
+}
+
+int gwl_mb_eval(gwl_mb* mb, const double* params, int64_t n, int32_t ld, int32_t precision,
+                double* out, double* aux, void* stream) {
+  if (!mb || !params || !out || n < 0 || ld < GWL_NPARAM ||
+      (precision != GWL_FP32 && precision != GWL_FP64)) {
+    set_error("gwl_mb_eval: invalid argument");
+    return GWL_E_INVALID;
+  }
+  if (n == 0) return GWL_OK;
+  gwl_event* ev = mb->ev;
+  if ((ev->dev.flags & GWL_DISTANCE_MARG) && !ev->has_spline) {
+    set_error("gwl_mb_eval: distance marginalisation requested but no spline attached");
+    return GWL_E_INVALID;
+  }
+  if (n > (int64_t)1 << 30) {
+    set_error("gwl_mb_eval: too many rows in one call");
+    return GWL_E_INVALID;
+  }
+  if (int rc = gwl::grow_ws(ev, &mb->ws, &mb->ws_bytes, n)) return rc;
+  GWL_HIP_CHECK(hipSetDevice(ev->ctx->device));
+  (void)hipGetLastError();
+  gwl::RelbinArgs a;
+  std::memset(&a, 0, sizeof a);
+  a.ev = ev->dev;
+  a.params = params;
+  a.ld = ld;
+  a.n = n;
+  char* w = static_cast<char*>(mb->ws);
+  a.rows = reinterpret_cast<gwl::RelRow*>(w);
+  a.sums = reinterpret_cast<double*>(w + ((size_t)n * sizeof(gwl::RelRow) + 255) / 256 * 256);
+  a.out = out;
+  a.aux = aux;
+  gwl::MbArgs m;
+  m.n = mb->n;
+  m.flo = ev->dev.wf_fmin;
+  m.inv_sigma = 1.0 / ev->dev.sigma;
+  m.fe = mb->fe;
+  m.basis = mb->basis;
+  m.amp = mb->amp;
+  m.lin = precision == GWL_FP64 ? (const void*)mb->lin64 : (const void*)mb->lin32;
+  m.quad = precision == GWL_FP64 ? (const void*)mb->quad64 : (const void*)mb->quad32;
+  m.rows_n = n;
+  m.rows = a.rows;
+  m.sums = a.sums;
+  const size_t lds = (ev->dev.flags & GWL_DISTANCE_MARG)
+                         ? (size_t)(ev->dev.sp.nx + ev->dev.sp.ny + 8) * 8 : 0;
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  if (int rc = gwl::timing_begin(ev, st)) return rc;
+  const unsigned g128 = (unsigned)((n + 127) / 128);
+  hipLaunchKernelGGL(gwl::k_relbin_rows, dim3(g128), dim3(128), 0, st, a);
+  GWL_HIP_CHECK(hipGetLastError());
+  const int rc = precision == GWL_FP64 ? gwl::launch_mb_nd<double>(m, ev->dev.nd, st)
+                                       : gwl::launch_mb_nd<float>(m, ev->dev.nd, st);
   if (rc != GWL_OK) return rc;
   hipLaunchKernelGGL(gwl::k_relbin_epi, dim3(g128), dim3(128), lds, st, a);
   GWL_HIP_CHECK(hipGetLastError());

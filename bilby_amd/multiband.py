@@ -1,0 +1,398 @@
+"""Multibanded likelihood on the GPU: bilby's MBGravitationalWaveTransient
+(bilby/gw/likelihood/multiband.py; Morisaki, arXiv:2104.07813) with the
+linear-interpolation algorithm for <h|h>.
+
+``MultibandSetup`` restates the host-side construction (float64 numpy, no
+GPU): frequency bands of durations T, T/2, T/4, ... chosen from the 0PN
+time-to-merger of a reference chirp mass, the banded frequency points, the
+linear coefficients (data filtered, down-sampled and windowed per band) and
+the quadratic coefficients.  Each evaluation runs ``k_mb``
+(csrc/gwl_relbin.hip): TaylorF2 at every banded point, the detector
+response, <d|h> = conj(sum h L) and <h|h> = sum |h|^2 Q, then the
+marginalisation.  The IFFT-FFT algorithm for <h|h> and time marginalisation
+are not provided.
+"""
+import ctypes
+import math
+
+import numpy as np
+import torch
+
+from . import _lib
+from .constants import gravitational_constant, radius_of_earth, solar_mass, speed_of_light
+from ._lib import LIB, check, dptr
+from .likelihood import GravitationalWaveTransient
+
+
+def round_up_to_power_of_two(x):
+    """bilby/core/utils/calculus.py:302-313."""
+    return 2 ** math.ceil(np.log2(x))
+
+
+class MultibandSetup:
+    """Bands and coefficients (multiband.py:311-647, linear interpolation).
+
+    ``interferometers``: objects with ``name``, ``frequency_array``,
+    ``frequency_mask``, ``frequency_domain_strain``,
+    ``power_spectral_density_array``, ``minimum_frequency`` and
+    ``maximum_frequency``; ``duration`` is the data duration.
+    """
+
+    def __init__(self, interferometers, duration, reference_chirp_mass, highest_mode=2,
+                 accuracy_factor=5, time_offset=2.12, delta_f_end=53.0,
+                 maximum_banding_frequency=None, minimum_banding_duration=0.0):
+        self.interferometers = list(interferometers)
+        self.duration = float(duration)
+        self.reference_chirp_mass = float(reference_chirp_mass)
+        self.highest_mode = highest_mode
+        self.accuracy_factor = accuracy_factor
+        self.time_offset = float(time_offset)
+        self.delta_f_end = float(delta_f_end)
+        self.minimum_banding_duration = minimum_banding_duration
+        # multiband.py:264-287: where f - 1/sqrt(-dtau/df) stops increasing
+        fmax = ((15 / 968) ** (3 / 5) * (self.highest_mode / (2 * np.pi)) ** (8 / 5)
+                / self.reference_chirp_mass_in_second)
+        if maximum_banding_frequency is not None and maximum_banding_frequency < fmax:
+            fmax = maximum_banding_frequency
+        self.maximum_banding_frequency = fmax
+        self._setup_frequency_bands()
+        self._setup_integers()
+        self._setup_waveform_frequency_points()
+        self._setup_linear_coefficients()
+        self._setup_quadratic_coefficients()
+
+    @property
+    def reference_chirp_mass_in_second(self):
+        return gravitational_constant * self.reference_chirp_mass * solar_mass / speed_of_light ** 3
+
+    @property
+    def minimum_frequency(self):
+        return min(i.minimum_frequency for i in self.interferometers)
+
+    @property
+    def maximum_frequency(self):
+        return max(i.maximum_frequency for i in self.interferometers)
+
+    @property
+    def number_of_bands(self):
+        return len(self.durations)
+
+    def _tau(self, f):
+        """0PN time to merger of the reference chirp mass at frequency f of
+        the highest mode (multiband.py:322-340)."""
+        mc = self.reference_chirp_mass_in_second
+        return 5 / 256 * mc * (np.pi * mc * (2 * f / self.highest_mode)) ** (-8 / 3)
+
+    def _dtaudf(self, f):
+        mc = self.reference_chirp_mass_in_second
+        return -5 / 96 * mc * (np.pi * mc * (2 * f / self.highest_mode)) ** (-8 / 3) / f
+
+    def _find_starting_frequency(self, duration, fnow):
+        """Bisection for the lowest frequency at which a band of ``duration``
+        holds the signal (Eqs. 10 and 51 of arXiv:2104.07813;
+        multiband.py:362-400)."""
+        def above(f):
+            width = np.sqrt(-self._dtaudf(f))
+            return ((duration - self.time_offset - self._tau(f) - self.accuracy_factor * width > 0)
+                    and (f - 1.0 / width - fnow > 0))
+        lo, hi = fnow, self.maximum_banding_frequency
+        if not above(hi):
+            return None, None
+        f = hi
+        while hi - lo > 1e-2 / duration:
+            f = (lo + hi) / 2.0
+            if above(f):
+                hi = f
+            else:
+                lo = f
+        return f, 1.0 / np.sqrt(-self._dtaudf(f))
+
+    def _setup_frequency_bands(self):
+        """multiband.py:402-426."""
+        durations = [self.duration]
+        fb_dfb = [[self.minimum_frequency, 0.0]]
+        dnext = self.duration / 2
+        while dnext > max(self.time_offset, self.minimum_banding_duration):
+            fnext, dfnext = self._find_starting_frequency(dnext, fb_dfb[-1][0])
+            if fnext is not None and fnext < min(self.maximum_frequency,
+                                                 self.maximum_banding_frequency):
+                durations.append(dnext)
+                fb_dfb.append([fnext, dfnext])
+                dnext /= 2
+            else:
+                break
+        fb_dfb.append([self.maximum_frequency + self.delta_f_end, self.delta_f_end])
+        self.durations = np.array(durations)
+        self.fb_dfb = np.array(fb_dfb)
+
+    def _setup_integers(self):
+        """multiband.py:428-447: N_b (down-sampled length), M_b = N_b / 2^b
+        (shortened length), [K_s, K_e] frequency index range of each band."""
+        nbs, mbs, ks_ke = [], [], []
+        for b in range(self.number_of_bands):
+            dnow = self.durations[b]
+            fnow, dfnow = self.fb_dfb[b]
+            fnext = self.fb_dfb[b + 1][0]
+            nb = max(round_up_to_power_of_two(2.0 * (fnext * self.duration + 1.0)), 2 ** b)
+            nbs.append(nb)
+            mbs.append(nb // 2 ** b)
+            ks_ke.append([math.ceil((fnow - dfnow) * dnow), math.floor(fnext * dnow)])
+        self.Nbs = np.array(nbs, dtype=int)
+        self.Mbs = np.array(mbs, dtype=int)
+        self.Ks_Ke = np.array(ks_ke, dtype=int)
+
+    def _setup_waveform_frequency_points(self):
+        """multiband.py:449-478."""
+        pts, se = [], []
+        start = 0
+        for b in range(self.number_of_bands):
+            ks, ke = self.Ks_Ke[b]
+            pts.append(np.arange(ks, ke + 1) / self.durations[b])
+            se.append([start, start + ke - ks])
+            start += ke - ks + 1
+        self.banded_frequency_points = np.concatenate(pts)
+        self.start_end_idxs = np.array(se, dtype=int)
+        self.waveform_frequencies, self.unique_to_original_frequencies = np.unique(
+            self.banded_frequency_points, return_inverse=True)
+
+    def _window(self, delta_f, start_idx, length, b):
+        """multiband.py:480-527: 1 inside band b, raised-cosine tapers of
+        widths dfnow / dfnext at its ends, on frequencies (start_idx + i)
+        delta_f."""
+        fnow, dfnow = self.fb_dfb[b]
+        fnext, dfnext = self.fb_dfb[b + 1]
+        w = np.zeros(length)
+        inc0 = int(np.clip(math.floor((fnow - dfnow) / delta_f) - start_idx + 1, 0, length))
+        one0 = int(np.clip(math.ceil(fnow / delta_f) - start_idx, 0, length))
+        dec0 = int(np.clip(math.floor((fnext - dfnext) / delta_f) - start_idx + 1, 0, length))
+        dec1 = int(np.clip(math.ceil(fnext / delta_f) - start_idx, 0, length))
+        w[one0:dec0] = 1.0
+        if inc0 < one0:
+            f = (np.arange(inc0, one0) + start_idx) * delta_f
+            w[inc0:one0] = (1.0 + np.cos(np.pi * (f - fnow) / dfnow)) / 2.0
+        if dec0 < dec1:
+            f = (np.arange(dec0, dec1) + start_idx) * delta_f
+            w[dec0:dec1] = (1.0 - np.cos(np.pi * (f - fnext) / dfnext)) / 2.0
+        return w
+
+    def _setup_linear_coefficients(self):
+        """multiband.py:529-549: per band, the whitened data d/S truncated at
+        the band's Nyquist frequency, transformed to the time domain, cut to
+        its last M_b samples and back, times 4/T_b and the window."""
+        self.linear_coeffs = {}
+        n_full = self.Nbs[-1]
+        for ifo in self.interferometers:
+            fd = np.zeros(n_full // 2 + 1, dtype=complex)
+            mask = np.asarray(ifo.frequency_mask)
+            m = mask[:len(fd)]
+            fd[:len(ifo.frequency_domain_strain)][m] += np.asarray(
+                ifo.frequency_domain_strain[mask] / ifo.power_spectral_density_array[mask])
+            parts = []
+            for b in range(self.number_of_bands):
+                ks, ke = self.Ks_Ke[b]
+                window = self._window(1.0 / self.durations[b], ks, ke - ks + 1, b)
+                band = np.copy(fd[:int(self.Nbs[b] / 2 + 1)])
+                band[-1] = 0.0
+                td = np.fft.irfft(band)[-self.Mbs[b]:]
+                shortened = np.fft.rfft(td)[ks:ke + 1]
+                parts.append((4.0 / self.durations[b]) * window * np.conj(shortened))
+            self.linear_coeffs[ifo.name] = np.concatenate(parts)
+
+    def _setup_quadratic_coefficients(self):
+        """multiband.py:551-611: |h|^2 linearly interpolated between banded
+        points; each original-grid term (window / S) is split between the two
+        banded points around it in proportion to its distance from each.
+        The grid terms between consecutive banded points are summed per
+        interval (np.bincount over the interval index)."""
+        T = self.duration
+        self.quadratic_coeffs = {ifo.name: [] for ifo in self.interferometers}
+        for b in range(self.number_of_bands):
+            s, e = self.start_end_idxs[b]
+            fb = self.banded_frequency_points[s:e + 1]
+            prefactor = 4 * self.durations[b] / T
+            fnow, dfnow = self.fb_dfb[b]
+            fnext = self.fb_dfb[b + 1][0]
+            lo = math.ceil((fnow - dfnow) * T)
+            window = self._window(1 / T, lo, math.floor(fnext * T) - lo + 1, b)
+            n = len(fb)
+            for ifo in self.interferometers:
+                psd = ifo.power_spectral_density_array
+                hi = min(lo + len(window) - 1, len(psd) - 1)
+                mask = np.asarray(ifo.frequency_mask[lo:hi + 1])
+                wps = np.zeros(hi + 1 - lo)
+                wps[mask] = 1.0 / psd[lo:hi + 1][mask]
+                wps *= window[:len(wps)]
+                f = np.arange(lo, hi + 1) / T
+                start = np.maximum(lo, np.ceil(T * fb[:-1]).astype(int))
+                start[0] = lo
+                stop = np.minimum(hi, np.ceil(T * fb[1:]).astype(int) - 1)
+                stop[-1] = hi
+                a = np.clip(start - lo, 0, len(wps))
+                z = np.maximum(np.clip(stop - lo + 1, 0, len(wps)), a)
+                # the intervals tile the grid terms: each term belongs to one k
+                owner = np.repeat(np.arange(n - 1), z - a)
+                if len(owner) != len(wps) or not np.array_equal(a[1:], z[:-1]):
+                    raise RuntimeError("multiband: interpolation intervals do not tile band %d" % b)
+                coeffs = np.zeros(n)
+                coeffs[:-1] += prefactor * np.bincount(owner, weights=(fb[owner + 1] - f) * wps,
+                                                       minlength=n - 1)
+                coeffs[1:] += prefactor * np.bincount(owner, weights=(f - fb[owner]) * wps,
+                                                      minlength=n - 1)
+                self.quadratic_coeffs[ifo.name].append(coeffs)
+        for name in self.quadratic_coeffs:
+            self.quadratic_coeffs[name] = np.concatenate(self.quadratic_coeffs[name])
+
+    @property
+    def weights(self):
+        """multiband.py:648-671 (linear interpolation)."""
+        keys = ["reference_chirp_mass", "highest_mode", "accuracy_factor", "time_offset",
+                "delta_f_end", "maximum_banding_frequency", "minimum_banding_duration",
+                "durations", "fb_dfb", "Nbs", "Mbs", "Ks_Ke", "banded_frequency_points",
+                "start_end_idxs", "unique_to_original_frequencies"]
+        out = {k: getattr(self, k) for k in keys}
+        out["linear_interpolation"] = True
+        out["waveform_frequencies"] = self.waveform_frequencies
+        out["linear_coeffs"] = dict(self.linear_coeffs)
+        out["quadratic_coeffs"] = dict(self.quadratic_coeffs)
+        return out
+
+    def save_weights(self, filename):
+        """multiband.py:673-687, as an .npz archive (bilby writes HDF5)."""
+        w = self.weights
+        flat = {}
+        for k, v in w.items():
+            if isinstance(v, dict):
+                for name, arr in v.items():
+                    flat["%s/%s" % (k, name)] = np.asarray(arr)
+            else:
+                flat[k] = np.asarray(v)
+        np.savez(filename, **flat)
+
+    @classmethod
+    def from_weights(cls, weights, interferometers):
+        """multiband.py:689-712; ``weights`` a dict or an .npz file name."""
+        if isinstance(weights, str):
+            with np.load(weights, allow_pickle=False) as f:
+                flat = {k: f[k] for k in f.files}
+            weights = {}
+            for k, v in flat.items():
+                if "/" in k:
+                    top, name = k.split("/", 1)
+                    weights.setdefault(top, {})[name] = v
+                else:
+                    weights[k] = v[()] if v.ndim == 0 else v
+        self = cls.__new__(cls)
+        self.interferometers = list(interferometers)
+        self.duration = float(weights["durations"][0])
+        for k, v in weights.items():
+            if k == "linear_interpolation":
+                continue
+            setattr(self, k, dict(v) if isinstance(v, dict) else v)
+        return self
+
+
+class MBGravitationalWaveTransient(GravitationalWaveTransient):
+    """multiband.py:19-127 constructor semantics (reference chirp mass from
+    the chirp-mass prior minimum when not given; time offset and high-end
+    taper from the geocent_time prior, multiband.py:194-259), plus
+    ``precision`` and ``device``."""
+
+    def __init__(self, interferometers, waveform_generator, reference_chirp_mass=None,
+                 highest_mode=2, linear_interpolation=True, accuracy_factor=5, time_offset=None,
+                 delta_f_end=None, maximum_banding_frequency=None, minimum_banding_duration=0.0,
+                 weights=None, distance_marginalization=False, phase_marginalization=False,
+                 priors=None, time_marginalization=False, jitter_time=True,
+                 distance_marginalization_lookup_table=None, reference_frame="sky",
+                 time_reference="geocenter", precision="fp64", device=0):
+        if time_marginalization:
+            raise NotImplementedError("multibanding with time marginalisation is not supported")
+        if not linear_interpolation:
+            raise NotImplementedError("only the linear-interpolation algorithm for <h|h> is "
+                                      "provided")
+        self._mb = None
+        super().__init__(interferometers, waveform_generator, time_marginalization=False,
+                         distance_marginalization=distance_marginalization,
+                         phase_marginalization=phase_marginalization, priors=priors,
+                         distance_marginalization_lookup_table=distance_marginalization_lookup_table,
+                         jitter_time=jitter_time, reference_frame=reference_frame,
+                         time_reference=time_reference, precision=precision, device=device)
+        if self.waveform_generator.approximant != _lib.TAYLORF2:
+            raise NotImplementedError("multibanding is provided for TaylorF2 only")
+        ifos = self.interferometers
+        if weights is not None:
+            self.setup = MultibandSetup.from_weights(weights, ifos)
+        else:
+            safety = radius_of_earth / speed_of_light
+            p = self.priors
+            if time_offset is None:
+                time_offset = (ifos.start_time + ifos.duration - p["geocent_time"].minimum + safety
+                               if p is not None and "geocent_time" in p
+                               and hasattr(p["geocent_time"], "minimum") else 2.12)
+            if delta_f_end is None:
+                delta_f_end = (100 / (ifos.start_time + ifos.duration - p["geocent_time"].maximum
+                                      - safety)
+                               if p is not None and "geocent_time" in p
+                               and hasattr(p["geocent_time"], "maximum") else 53.0)
+            if reference_chirp_mass is None:
+                reference_chirp_mass = self._minimum_chirp_mass()
+            self.setup = MultibandSetup(ifos, ifos.duration, reference_chirp_mass, highest_mode,
+                                        accuracy_factor, time_offset, delta_f_end,
+                                        maximum_banding_frequency, minimum_banding_duration)
+        self._upload()
+
+    def _minimum_chirp_mass(self):
+        """gw/prior.py:721-737."""
+        p = self.priors or {}
+        if "chirp_mass" in p:
+            return float(p["chirp_mass"].minimum)
+        if "mass_1" in p and ("mass_2" in p or "mass_ratio" in p):
+            m1 = p["mass_1"].minimum
+            m2 = p["mass_2"].minimum if "mass_2" in p else m1 * p["mass_ratio"].minimum
+            return float((m1 * m2) ** 0.6 / (m1 + m2) ** 0.2)
+        raise ValueError("reference_chirp_mass is not given and the priors do not fix a minimum "
+                         "chirp mass")
+
+    @property
+    def weights(self):
+        return self.setup.weights
+
+    def save_weights(self, filename):
+        self.setup.save_weights(filename)
+
+    def _upload(self):
+        s = self.setup
+        nd, n = len(self.interferometers), len(s.banded_frequency_points)
+        lin = np.zeros((nd, n, 2))
+        quad = np.zeros((nd, n))
+        for k, ifo in enumerate(self.interferometers):
+            lin[k, :, 0] = np.real(s.linear_coeffs[ifo.name])
+            lin[k, :, 1] = np.imag(s.linear_coeffs[ifo.name])
+            quad[k] = s.quadratic_coeffs[ifo.name]
+        freqs = np.ascontiguousarray(s.banded_frequency_points, dtype=np.float64)
+        lin = np.ascontiguousarray(lin)
+        quad = np.ascontiguousarray(quad)
+        mb = ctypes.c_void_p()
+        check(LIB.gwl_mb_create(self._event, n, dptr(freqs), dptr(lin), dptr(quad),
+                                ctypes.byref(mb)), "gwl_mb_create")
+        self._mb = mb
+
+    def _device_eval(self, rows_t, precision, aux=False, stream=None):
+        n = rows_t.shape[0]
+        dev = rows_t.device
+        out = torch.empty(n, dtype=torch.float64, device=dev)
+        aux_t = torch.empty((n, 3), dtype=torch.float64, device=dev) if aux else None
+        st = torch.cuda.current_stream(dev).cuda_stream if stream is None else stream
+        check(LIB.gwl_mb_eval(self._mb, ctypes.c_void_p(rows_t.data_ptr()), n, _lib.NPARAM,
+                              precision, ctypes.c_void_p(out.data_ptr()),
+                              ctypes.c_void_p(aux_t.data_ptr()) if aux else None,
+                              ctypes.c_void_p(st)), "gwl_mb_eval")
+        return out, aux_t
+
+    def __del__(self):
+        try:
+            if getattr(self, "_mb", None) is not None:
+                LIB.gwl_mb_destroy(self._mb)
+                self._mb = None
+        except Exception:
+            pass
+        super().__del__()

@@ -317,6 +317,28 @@ int gwl_relbin_reserve(gwl_relbin* rb, int64_t rows);
 int gwl_relbin_eval(gwl_relbin* rb, const double* params, int64_t n, int32_t ld,
                     int32_t precision, double* out, double* aux, void* stream);
 
+/* ------------------------------------------------------------------ multibanding
+ * MBGravitationalWaveTransient (bilby/gw/likelihood/multiband.py, linear
+ * interpolation for <h|h>) on a TaylorF2 event without time
+ * marginalisation; the event supplies detectors, source model, flags and
+ * the distance spline and must outlive the object.
+ *   frequencies: [n_points] banded frequency points (multiband.py:449-478)
+ *   linear:      [n_det][n_points][2] complex linear coefficients
+ *                (multiband.py:529-549)
+ *   quadratic:   [n_det][n_points] quadratic coefficients
+ *                (multiband.py:551-611) */
+typedef struct gwl_mb gwl_mb;
+int gwl_mb_create(gwl_event* ev, int32_t n_points, const double* frequencies,
+                  const double* linear, const double* quadratic, gwl_mb** out);
+int gwl_mb_destroy(gwl_mb* mb);
+int gwl_mb_reserve(gwl_mb* mb, int64_t rows);
+int gwl_mb_device_bytes(gwl_mb* mb, int64_t* bytes);
+/* ln L ratio of n rows: multiband.py:728-807 + base.py:448-477; params,
+ * out, aux (may be NULL, [n][3]) device pointers; precision GWL_FP64 or
+ * GWL_FP32 (float32 response and coefficients, float64 phase and sums). */
+int gwl_mb_eval(gwl_mb* mb, const double* params, int64_t n_rows, int32_t ld,
+                int32_t precision, double* out, double* aux, void* stream);
+
 /* ------------------------------------------------------------------ calibration
  * Calibration marginalisation (base.py:332-346, 860-877, lookup
  * base.py:1037-1051) on an event without time marginalisation; the event

@@ -1,0 +1,83 @@
+"""FP64 restatement of bilby's MBGravitationalWaveTransient evaluation
+(bilby/gw/likelihood/multiband.py:728-807, linear interpolation) on given
+multiband weights, and a literal loop form of the quadratic coefficients
+(multiband.py:551-611) to check the product's prefix-sum form against.
+
+Test infrastructure only (see oracle/__init__.py).
+"""
+import copy
+import math
+import types
+
+import numpy as np
+
+from .likelihood import OracleGWTransient, detector_response, source_polarizations
+
+
+class OracleMultiband(OracleGWTransient):
+    """``weights``: an object or dict with banded_frequency_points,
+    unique_to_original_frequencies, waveform_frequencies, linear_coeffs and
+    quadratic_coeffs (detector name -> array)."""
+
+    def __init__(self, event, source, weights, **kwargs):
+        super().__init__(event, source, time_marginalization=False, **kwargs)
+        w = weights if isinstance(weights, dict) else weights.weights
+        self.w = w
+
+    def calculate_snrs_mb(self, pols_unique, det, params):
+        """multiband.py:750-765."""
+        w = self.w
+        idx = w["unique_to_original_frequencies"]
+        modes = {k: v[idx] for k, v in pols_unique.items()}
+        at_points = types.SimpleNamespace(tensor=det.tensor, vertex=det.vertex,
+                                          frequency_array=w["banded_frequency_points"],
+                                          mask=np.ones(len(idx), dtype=bool))
+        strain = detector_response(at_points, modes, params, self.event.start_time)
+        d_inner_h = np.conj(np.dot(strain, w["linear_coeffs"][det.name]))
+        h_inner_h = np.vdot(np.abs(strain) ** 2, w["quadratic_coeffs"][det.name]).real
+        return d_inner_h, h_inner_h
+
+    def log_likelihood_ratio(self, parameters):
+        params = copy.deepcopy(parameters)
+        wa = dict(self.waveform_arguments)
+        wa.setdefault("maximum_frequency", self.event.frequency_array[-1])
+        pols = source_polarizations(self.source, self.w["waveform_frequencies"], params, wa)
+        d_inner_h, h_inner_h = 0j, 0.0
+        for det in self.event.detectors:
+            dh, hh = self.calculate_snrs_mb(pols, det, params)
+            d_inner_h += dh
+            h_inner_h += hh
+        return float(np.real(self.compute_log_likelihood(d_inner_h, h_inner_h, None, params)))
+
+
+def quadratic_coefficients_loop(setup, ifos):
+    """multiband.py:551-611 loop for loop: ``setup`` holds durations, fb_dfb,
+    start_end_idxs, banded_frequency_points and a _window(delta_f, start,
+    length, b) method; ``ifos`` as bilby's interferometers."""
+    T = float(setup.durations[0])
+    out = {ifo.name: [] for ifo in ifos}
+    for b in range(len(setup.durations)):
+        s, e = setup.start_end_idxs[b]
+        fb = setup.banded_frequency_points[s:e + 1]
+        prefactor = 4 * setup.durations[b] / T
+        fnow, dfnow = setup.fb_dfb[b]
+        fnext, _ = setup.fb_dfb[b + 1]
+        lo = math.ceil((fnow - dfnow) * T)
+        window = setup._window(1 / T, lo, math.floor(fnext * T) - lo + 1, b)
+        for ifo in ifos:
+            psd = ifo.power_spectral_density_array
+            hi = min(lo + len(window) - 1, len(psd) - 1)
+            mask = np.asarray(ifo.frequency_mask[lo:hi + 1])
+            wps = np.zeros(hi + 1 - lo)
+            wps[mask] = 1. / psd[lo:hi + 1][mask]
+            wps *= window[:len(wps)]
+            coeffs = np.zeros(len(fb))
+            for k in range(len(coeffs) - 1):
+                a = lo if k == 0 else max(lo, math.ceil(T * fb[k]))
+                z = hi if k == len(coeffs) - 2 else min(hi, math.ceil(T * fb[k + 1]) - 1)
+                f = np.arange(a, z + 1) / T
+                seg = wps[a - lo:z - lo + 1]
+                coeffs[k] += prefactor * np.sum((fb[k + 1] - f) * seg)
+                coeffs[k + 1] += prefactor * np.sum((f - fb[k]) * seg)
+            out[ifo.name].append(coeffs)
+    return {k: np.concatenate(v) for k, v in out.items()}

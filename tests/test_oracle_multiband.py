@@ -1,0 +1,106 @@
+"""The multibanded likelihood's host setup (bilby_amd.multiband.MultibandSetup)
+and its oracle evaluation (oracle/multiband.py) pinned on the CPU:
+* multibanded ln L vs the full oracle ln L on the reference test's
+  configuration (16 s, 2048 Hz, H1+L1+V1, geocent prior +-0.1 s,
+  likelihood_test.py:1313-1428): |dlnL| < 1.5e-2 (its tolerance for the
+  linear-interpolation algorithm), for a light binary whose TaylorF2
+  termination lies above the band (IMRPhenomD in the reference ends
+  smoothly);
+* accuracy_factor 50 at least halves the error of accuracy_factor 5
+  (likelihood_test.py:1430-1475);
+* the prefix-sum quadratic coefficients equal the reference's loop form;
+* weights saved and loaded give the same setup."""
+import types
+
+import numpy as np
+import pytest
+
+from oracle import core
+from oracle.likelihood import OracleDetector, OracleEvent, OracleGWTransient, detector_response
+from oracle.multiband import OracleMultiband, quadratic_coefficients_loop
+
+WF = dict(reference_frequency=20.0, minimum_frequency=20.0)
+T, FS = 16.0, 2048.0
+
+
+def _event(seed=70817, masses=(1.5, 1.3), distance=40.0):
+    rng = np.random.default_rng(seed)
+    f = core.frequency_series(FS, T)
+    tc = 1187008882.0
+    start = tc - T + 2.0
+    inj = dict(mass_1=masses[0], mass_2=masses[1], chi_1=0.0, chi_2=0.0,
+               luminosity_distance=distance, theta_jn=0.4, phase=1.3, ra=1.3, dec=-1.2, psi=0.659,
+               geocent_time=tc)
+    pols = core.taylorf2(f, inj["mass_1"], inj["mass_2"], inj["luminosity_distance"],
+                         inj["theta_jn"], inj["phase"], **WF)
+    dets, ifos = [], []
+    for name in ("H1", "L1", "V1"):
+        psd = core.analytic_aligo_psd(f)
+        noise, _ = core.noise_realisation(lambda x: core.analytic_aligo_psd(x), 0, np.inf,
+                                          len(f), T, rng)
+        mask = core.frequency_mask(f, 20, FS / 2, FS)
+        det = OracleDetector(name, f, noise, psd, mask)
+        det.strain = det.strain + detector_response(det, pols, inj, start) * mask
+        dets.append(det)
+        ifos.append(types.SimpleNamespace(
+            name=name, frequency_array=f, frequency_mask=mask, frequency_domain_strain=det.strain,
+            power_spectral_density_array=psd, minimum_frequency=20.0, maximum_frequency=FS / 2))
+    return OracleEvent(dets, T, FS, start), ifos, inj
+
+
+def _setup(ifos, inj, **kw):
+    from bilby_amd.constants import radius_of_earth, speed_of_light
+    from bilby_amd.multiband import MultibandSetup
+    start = inj["geocent_time"] - T + 2.0
+    safety = radius_of_earth / speed_of_light
+    time_offset = start + T - (inj["geocent_time"] - 0.1) + safety     # multiband.py:194-225
+    delta_f_end = 100 / (start + T - (inj["geocent_time"] + 0.1) - safety)
+    mc = (inj["mass_1"] * inj["mass_2"]) ** 0.6 / (inj["mass_1"] + inj["mass_2"]) ** 0.2
+    return MultibandSetup(ifos, T, mc, time_offset=time_offset, delta_f_end=delta_f_end, **kw)
+
+
+def test_multiband_matches_full_likelihood():
+    ev, ifos, inj = _event()
+    setup = _setup(ifos, inj)
+    assert setup.number_of_bands >= 3
+    assert len(setup.waveform_frequencies) < 0.5 * len(ev.frequency_array)
+    mb = OracleMultiband(ev, "taylorf2", setup, waveform_arguments=WF)
+    full = OracleGWTransient(ev, "taylorf2", waveform_arguments=WF)
+    ll, llmb = full.log_likelihood_ratio(inj), mb.log_likelihood_ratio(inj)
+    assert ll > 100
+    assert abs(ll - llmb) < 1.5e-2, (ll, llmb)
+    rng = np.random.default_rng(3)
+    for _ in range(5):
+        p = dict(inj)
+        p["ra"] += rng.normal(0, 0.05)
+        p["luminosity_distance"] *= rng.uniform(0.8, 1.2)
+        p["geocent_time"] += rng.normal(0, 1e-3)
+        assert abs(full.log_likelihood_ratio(p) - mb.log_likelihood_ratio(p)) < 1.5e-2
+
+
+def test_larger_accuracy_factor_is_more_accurate():
+    ev, ifos, inj = _event(seed=5)
+    full = OracleGWTransient(ev, "taylorf2", waveform_arguments=WF).log_likelihood_ratio(inj)
+    errs = []
+    for acc in (5, 50):
+        setup = _setup(ifos, inj, accuracy_factor=acc)
+        mb = OracleMultiband(ev, "taylorf2", setup, waveform_arguments=WF)
+        errs.append(abs(full - mb.log_likelihood_ratio(inj)))
+    assert errs[1] < errs[0] / 2, errs
+
+
+def test_quadratic_coefficients_equal_loop_form_and_weights_round_trip(tmp_path):
+    from bilby_amd.multiband import MultibandSetup
+    ev, ifos, inj = _event(seed=6)
+    setup = _setup(ifos, inj)
+    loop = quadratic_coefficients_loop(setup, ifos)
+    for name, q in loop.items():
+        assert np.max(np.abs(setup.quadratic_coeffs[name] - q)) < 1e-12 * np.max(np.abs(q))
+    fn = str(tmp_path / "mb.npz")
+    setup.save_weights(fn)
+    back = MultibandSetup.from_weights(fn, ifos)
+    for key in ("banded_frequency_points", "unique_to_original_frequencies", "durations", "Ks_Ke"):
+        assert np.array_equal(getattr(back, key), getattr(setup, key))
+    for name in setup.linear_coeffs:
+        assert np.array_equal(back.linear_coeffs[name], setup.linear_coeffs[name])
+        assert np.array_equal(back.quadratic_coeffs[name], setup.quadratic_coeffs[name])
