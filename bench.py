@@ -364,6 +364,70 @@ def bench_calibration(steps, batch=2048, n_curves=1000):
     return out
 
 
+def bench_multiband(steps, batch=4096, duration=128.0):
+    """Multibanding on a long segment: a light binary (1.5 + 1.3 Msun,
+    TaylorF2, H1+L1+V1), 128 s @ 2048 Hz (131,073 bins per detector before
+    the mask), distance + phase marginalised, a batch of draws around the
+    injection; the full (unbanded) likelihood timed beside it."""
+    import numpy as np
+    import torch
+    from bilby_amd import InterferometerList, WaveformGenerator, taylorf2
+    from bilby_amd._lib import LIB, check
+    from bilby_amd.likelihood import GravitationalWaveTransient, inject_signal
+    from bilby_amd.multiband import MBGravitationalWaveTransient
+    from bilby_amd.prior import PriorDict, Uniform, UniformComovingVolume
+    fs = 2048.0
+    tc = 1187008882.0
+    start = tc - duration + 2.0
+    ifos = InterferometerList(["H1", "L1", "V1"])
+    ifos.set_strain_data_from_power_spectral_densities(fs, duration, start,
+                                                       random_state=np.random.default_rng(11))
+    wfg = WaveformGenerator(duration=duration, sampling_frequency=fs,
+                            frequency_domain_source_model=taylorf2,
+                            waveform_arguments=dict(reference_frequency=20.0,
+                                                    minimum_frequency=20.0))
+    inj = dict(mass_1=1.5, mass_2=1.3, chi_1=0.0, chi_2=0.0, luminosity_distance=100.0,
+               theta_jn=0.4, phase=1.3, ra=1.3, dec=-1.2, psi=0.659, geocent_time=tc)
+    inject_signal(ifos, wfg, inj)
+    priors = PriorDict(geocent_time=Uniform(tc - 0.1, tc + 0.1),
+                       luminosity_distance=UniformComovingVolume(10.0, 300.0),
+                       phase=Uniform(0.0, 2 * np.pi, boundary="periodic"))
+    mc = (inj["mass_1"] * inj["mass_2"]) ** 0.6 / (inj["mass_1"] + inj["mass_2"]) ** 0.2
+    t0 = time.perf_counter()
+    lk = MBGravitationalWaveTransient(ifos, wfg, reference_chirp_mass=mc, priors=priors.copy(),
+                                      distance_marginalization=True, phase_marginalization=True,
+                                      precision="fp32")
+    setup_s = time.perf_counter() - t0
+    full = GravitationalWaveTransient(ifos, wfg, priors=priors.copy(),
+                                      distance_marginalization=True, phase_marginalization=True,
+                                      precision="fp32")
+    rng = np.random.default_rng(12)
+    d = {k: np.full(batch, float(v)) for k, v in inj.items()}
+    d["mass_1"] = d["mass_1"] * (1 + rng.normal(0, 2e-6, batch))
+    d["mass_2"] = d["mass_2"] * (1 + rng.normal(0, 2e-6, batch))
+    for k, sd in (("ra", 0.05), ("dec", 0.05), ("psi", 0.2), ("geocent_time", 2e-4)):
+        d[k] = d[k] + rng.normal(0, sd, batch)
+    d["luminosity_distance"] = np.full(batch, lk._ref_dist)
+    d["phase"] = np.zeros(batch)
+    rows = torch.from_numpy(lk.rows_from_parameters(d)).cuda()
+    check(LIB.gwl_event_reserve(full._event, batch))
+    out = {"rows": batch, "duration_s": duration, "bins_per_detector": len(ifos.frequency_array),
+           "banded_points": int(len(lk.setup.banded_frequency_points)),
+           "bands": int(lk.setup.number_of_bands), "setup_s": setup_s}
+    for prec in ("fp32", "fp64"):
+        dt = timed(lambda: lk.log_likelihood_ratio_batch(rows, precision=prec), steps)
+        out["mb_%s_evals_per_s" % prec] = batch / dt
+    dt = timed(lambda: full.log_likelihood_ratio_batch(rows), steps)
+    out["full_fp32_evals_per_s"] = batch / dt
+    mb64 = lk.log_likelihood_ratio_batch(rows, precision="fp64").cpu().numpy()
+    mb32 = lk.log_likelihood_ratio_batch(rows, precision="fp32").cpu().numpy()
+    f64 = full.log_likelihood_ratio_batch(rows, precision="fp64").cpu().numpy()
+    out["mb_fp32_vs_mb_fp64_max"] = float(np.max(np.abs(mb32 - mb64)))
+    out["mb_vs_full_fp64_max"] = float(np.max(np.abs(mb64 - f64)))
+    out["lnL_max"] = float(np.max(f64))
+    return out
+
+
 def bench_roq(steps):
     """ROQ, H1+L1, 400 linear / 120 quadratic nodes, 1000 ROQ times,
     time + phase marginalised, 65536 TaylorF2 draws: the contraction
@@ -734,6 +798,7 @@ def main():
                          ("roq_400_120_nodes_1000_times_65536", lambda: bench_roq(3)),
                          ("relative_binning_dp_event_65536", lambda: bench_relbin(5)),
                          ("calibration_1000_curves_dp_event", lambda: bench_calibration(3)),
+                         ("multiband_128s_light_binary", lambda: bench_multiband(5)),
                          ("serving_32_samplers_2000_rps", lambda: bench_serving(
                              lk, rows.cpu().numpy(), args.serve_seconds)),
                          ("sampler_callers_pool_and_16_processes", lambda: bench_sampler(
