@@ -132,6 +132,9 @@ def _load():
         "gwl_relbin_create": ([vp, ctypes.POINTER(RelbinDesc), ctypes.POINTER(vp)], ctypes.c_int),
         "gwl_relbin_destroy": ([vp], ctypes.c_int),
         "gwl_relbin_reserve": ([vp, ctypes.c_int64], ctypes.c_int),
+        "gwl_roq_weights_linear": ([vp, ctypes.c_int32, ctypes.c_int64, _dp,
+                                    ctypes.POINTER(ctypes.c_int64), ctypes.c_int64, ctypes.c_int64,
+                                    ctypes.c_int32, ctypes.c_double, _dp], ctypes.c_int),
         "gwl_mb_create": ([vp, ctypes.c_int32, _dp, _dp, _dp, ctypes.POINTER(vp)], ctypes.c_int),
         "gwl_mb_destroy": ([vp], ctypes.c_int),
         "gwl_mb_reserve": ([vp, ctypes.c_int64], ctypes.c_int),
@@ -193,4 +196,5 @@ EXPORTED_SYMBOLS = [
     "gwl_relbin_eval", "gwl_server_add_relbin", "gwl_relbin_reserve", "gwl_calib_create",
     "gwl_calib_destroy", "gwl_calib_reserve", "gwl_calib_device_bytes", "gwl_calib_eval",
     "gwl_mb_create", "gwl_mb_destroy", "gwl_mb_reserve", "gwl_mb_device_bytes", "gwl_mb_eval",
+    "gwl_roq_weights_linear",
 ]

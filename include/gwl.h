@@ -277,6 +277,19 @@ int gwl_roq_reserve(gwl_roq* roq, int64_t max_rows);
 int gwl_roq_eval(gwl_roq* roq, const double* params, int64_t n_rows, int32_t ld, int32_t mode,
                  double* out, void* stream);
 
+/* ROQ linear weights from a reduced basis (roq.py:849-918,
+ * _set_weights_linear) for one detector and ROQ time samples m = k0 .. k0 +
+ * n_times - 1 (time m T / N):
+ *   out[k][i] = scale * sum_j x[i][j] exp(2 pi i bins[j] (k0 + k) / N)
+ * x: host [n_basis][n_freq][2] complex128 = (d / S) conj(B_i) on the masked
+ * bins `bins` (host [n_freq], 0 <= bins < N); scale = 4 / T; out: host
+ * [n_times][n_basis][2] complex128 (bilby's weights['<ifo>_linear'] layout).
+ * Float64 GEMM on the device over chunks of frequencies; allocates its own
+ * workspace (a set-up call, not for the evaluation path). */
+int gwl_roq_weights_linear(gwl_ctx* ctx, int32_t n_basis, int64_t n_freq, const double* x,
+                           const int64_t* bins, int64_t N, int64_t k0, int32_t n_times,
+                           double scale, double* out);
+
 /* The ROQ contraction as a plain GEMM, C[M][N] = A[M][K] Bt[N][K]^T
  * (M, N multiples of 128, K of 32), for checking the MFMA kernels:
  * mode GWL_ROQ_FP64 (double), GWL_ROQ_FP32 (float), GWL_ROQ_BF16X3
