@@ -428,6 +428,55 @@ def bench_multiband(steps, batch=4096, duration=128.0):
     return out
 
 
+def bench_roq_weights(n_basis=400, duration=128.0, fs=4096.0):
+    """Building ROQ linear weights (roq.py:849-918) for a 400-element basis
+    on a 128 s, 4096 Hz segment, two detectors: the GPU GEMM form against
+    bilby's form (one numpy inverse FFT of the segment per basis element and
+    detector), the latter timed on a few basis elements and scaled."""
+    import types
+    import numpy as np
+    from bilby_amd import roq_weights as rw
+    from oracle import core
+    from oracle import roq_weights as orw
+    rng = np.random.default_rng(31)
+    f = core.frequency_series(fs, duration)
+    dets = []
+    for name in ("H1", "L1"):
+        psd = core.analytic_aligo_psd(f)
+        strain, _ = core.noise_realisation(lambda x: core.analytic_aligo_psd(x), 0, np.inf,
+                                           len(f), duration, rng)
+        mask = core.frequency_mask(f, 20, fs / 2, fs)
+        dets.append(types.SimpleNamespace(name=name, frequency_array=f, frequency_mask=mask,
+                                          frequency_domain_strain=strain * mask,
+                                          power_spectral_density_array=psd,
+                                          minimum_frequency=20.0))
+
+    class Ifos(list):
+        pass
+    ifos = Ifos(dets)
+    ifos.duration, ifos.start_time, ifos.frequency_array = duration, 1000.0, f
+    n_m = int(dets[0].frequency_mask.sum())
+    lin = (rng.normal(size=(n_basis, n_m)) + 1j * rng.normal(size=(n_basis, n_m))) / np.sqrt(n_m)
+    quad = rng.normal(size=(8, n_m))
+    tmid = 1000.0 + duration - 2.0
+    t0 = time.perf_counter()
+    w = rw.build_roq_weights(ifos, lin, quad, np.arange(n_basis, dtype=float), np.arange(8.0),
+                             tmid - 0.1, tmid + 0.1)
+    gpu_s = time.perf_counter() - t0
+    times = w["time_samples"]
+    n_seg = int(round(duration / (times[1] - times[0])))
+    k = 3
+    t0 = time.perf_counter()
+    ref = orw.linear_weights(f, dets[0].frequency_mask, dets[0].frequency_domain_strain,
+                             dets[0].power_spectral_density_array, 20.0, duration, lin[:k], times)
+    cpu_s = (time.perf_counter() - t0) * n_basis / k * len(dets)
+    err = np.max(np.abs(w["H1_linear"][:, :k] - ref)) / np.max(np.abs(ref))
+    return {"basis": n_basis, "masked_bins": n_m, "segment_samples": n_seg,
+            "roq_times": len(times), "gpu_build_s": gpu_s,
+            "numpy_ifft_build_s_estimated": cpu_s, "cpu_sample": "%d of %d basis elements, "
+            "1 of 2 detectors, scaled" % (k, n_basis), "relative_error_vs_ifft": float(err)}
+
+
 def bench_roq(steps):
     """ROQ, H1+L1, 400 linear / 120 quadratic nodes, 1000 ROQ times,
     time + phase marginalised, 65536 TaylorF2 draws: the contraction
@@ -799,6 +848,7 @@ def main():
                          ("relative_binning_dp_event_65536", lambda: bench_relbin(5)),
                          ("calibration_1000_curves_dp_event", lambda: bench_calibration(3)),
                          ("multiband_128s_light_binary", lambda: bench_multiband(5)),
+                         ("roq_weights_400_basis_128s", lambda: bench_roq_weights()),
                          ("serving_32_samplers_2000_rps", lambda: bench_serving(
                              lk, rows.cpu().numpy(), args.serve_seconds)),
                          ("sampler_callers_pool_and_16_processes", lambda: bench_sampler(
