@@ -81,6 +81,48 @@ __device__ inline void tile_prologue(const EvDev& ev, const double* params, int 
   __syncthreads();
 }
 
+// The same with the row constants read from a buffer filled by
+// k_sample_const (one thread per row), so that a row's prologue runs once
+// per evaluation rather than once per workgroup that touches the row.
+template <typename R, int S>
+__device__ inline void tile_prologue_load(const EvDev& ev, const SampleConst* rows, long long n,
+                                          long long row0, SampleConst* sc, SampleFast<R>* sf,
+                                          int* band) {
+  int t = threadIdx.x;
+  if (t < S) {
+    long long r = row0 + t;
+    if (r < n) {
+      sc[t] = rows[r];
+    } else {
+      sc[t].valid = 0;
+      sc[t].kb = 1;
+      sc[t].ke = 0;
+      sc[t].hscale = 1.0;
+      sc[t].jitter = 0.0;
+    }
+#pragma unroll
+    for (int d = 0; d < GWL_MAX_DET; ++d) {
+      sf[t].cre[d] = (R)(d < ev.nd ? sc[t].cre[d] : 0.0);
+      sf[t].cim[d] = (R)(d < ev.nd ? sc[t].cim[d] : 0.0);
+      sf[t].cabs2[d] = (R)(d < ev.nd ? sc[t].cabs2[d] : 0.0);
+    }
+    sf[t].k2 = (R)sc[t].c[0];
+  }
+  __syncthreads();
+  if (t == 0) {
+    int lo = 1 << 30, hi = -1;
+    for (int s = 0; s < S; ++s) {
+      if (sc[s].kb <= sc[s].ke) {
+        lo = min(lo, sc[s].kb);
+        hi = max(hi, sc[s].ke);
+      }
+    }
+    band[0] = lo;
+    band[1] = hi;
+  }
+  __syncthreads();
+}
+
 // Source model x projection x data for one frequency bin n and S rows:
 //   I[s]  = sum_d h_{s,d}(n) conj(dw_d(n))     (h conj(d) / S, base.py:327-329)
 //   hh[s] += sum_d |h_{s,d}(n)|^2 w_d(n)        (|h|^2 / S, gw/utils.py:137)

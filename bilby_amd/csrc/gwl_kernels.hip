@@ -38,7 +38,18 @@ struct RedArgs {
   int chunk;    // elements per chunk
   int nchunk;
   double* part; // [n][nchunk][3]
+  const SampleConst* rows;  // [n] row constants (k_sample_const)
 };
+
+// One thread per row: the row constants every k_reduce tile of the row reads.
+__global__ __launch_bounds__(128) void k_sample_const(EvDev ev, const double* params, int ld,
+                                                      long long n, SampleConst* out) {
+  const long long r = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  SampleConst sc;
+  compute_sample_const(ev, params + r * ld, sc);
+  out[r] = sc;
+}
 
 template <typename R, int WAVE, int ND, int S, int NT>
 __global__ __launch_bounds__(NT) void k_reduce(RedArgs a) {
@@ -51,7 +62,7 @@ __global__ __launch_bounds__(NT) void k_reduce(RedArgs a) {
   const int tid = threadIdx.x;
   const long long row0 = (long long)blockIdx.y * S;
   const int chunk = blockIdx.x;
-  tile_prologue<R, S>(ev, a.params, a.ld, a.n, row0, sc, sf, band);
+  tile_prologue_load<R, S>(ev, a.rows, a.n, row0, sc, sf, band);
 
   R acc_r[S], acc_i[S], acc_h[S];
 #pragma unroll
@@ -309,6 +320,7 @@ struct EpiArgs {
   const double* hhpart;
   double* out;
   double* aux;
+  const SampleConst* rows;  // [n] row constants, or null (then computed here)
 };
 
 // lnL from the summed inner products (base.py:448-477, 775-792); one thread
@@ -322,7 +334,10 @@ __global__ __launch_bounds__(128) void k_epi_plain(EpiArgs a) {
   const long long r = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= a.n) return;
   SampleConst sc;
-  compute_sample_const(ev, a.params + r * a.ld, sc);
+  if (a.rows)
+    sc = a.rows[r];
+  else
+    compute_sample_const(ev, a.params + r * a.ld, sc);
   double dr = 0.0, di = 0.0, hh = 0.0;
   const double* p = a.part3 + (size_t)r * a.nchunk * 3;
   for (int c = 0; c < a.nchunk; ++c) {
@@ -622,6 +637,13 @@ int launch_epi_time(gwl_event* e, const double* params, int64_t n, int ld, int G
   return GWL_OK;
 }
 
+// Bytes of the k_reduce partials [n][nchunk][3] double (256-aligned); the
+// row constants [n] SampleConst follow them in the event workspace.
+static size_t reduce_part_bytes(const EvDev& ev, long long rows) {
+  const long long nchunk = (ev.N + 1023) / 1024;
+  return ((size_t)rows * nchunk * 3 * sizeof(double) + 255) / 256 * 256;
+}
+
 template <typename R, int WAVE, int ND>
 static int eval_typed(gwl_event* e, const double* params, int64_t n, int ld, double* out,
                       double* aux, hipStream_t st) {
@@ -690,8 +712,12 @@ static int eval_typed(gwl_event* e, const double* params, int64_t n, int ld, dou
   ra.chunk = NT * 4;
   ra.nchunk = (ev.N + ra.chunk - 1) / ra.chunk;
   ra.part = reinterpret_cast<double*>(e->ws);
+  ra.rows = reinterpret_cast<const SampleConst*>(
+      reinterpret_cast<char*>(e->ws) + reduce_part_bytes(ev, n));
   const long long nblk = (n + S - 1) / S;
   if (int rc = timing_begin(e, st)) return rc;
+  hipLaunchKernelGGL(k_sample_const, dim3((unsigned)((n + 127) / 128)), dim3(128), 0, st, ev,
+                     params, ld, (long long)n, const_cast<SampleConst*>(ra.rows));
   if (S == 16)
     hipLaunchKernelGGL((k_reduce<R, WAVE, ND, 16, NT>), dim3(ra.nchunk, (unsigned)nblk), dim3(NT), 0,
                        st, ra);
@@ -712,6 +738,7 @@ static int eval_typed(gwl_event* e, const double* params, int64_t n, int ld, dou
   ea.part3 = ra.part;
   ea.out = out;
   ea.aux = aux;
+  ea.rows = ra.rows;
   const size_t klds = (ev.flags & GWL_DISTANCE_MARG) ? (size_t)(ev.sp.nx + ev.sp.ny + 8) * sizeof(double) : 0;
   hipLaunchKernelGGL(k_epi_plain, dim3((unsigned)((n + 127) / 128)), dim3(128), klds, st, ea);
   GWL_HIP_CHECK(hipGetLastError());
@@ -745,8 +772,7 @@ size_t workspace_bytes(const gwl_event* e, int64_t rows) {
     if (ev.approx == GWL_TAYLORF2 && ev.tf2b) b = std::max(b, time_wave_ws_bytes(e, rows));
     return b;
   }
-  const int nchunk = (ev.N + 1023) / 1024;
-  return (size_t)rows * nchunk * 3 * sizeof(double) + 4096;
+  return reduce_part_bytes(ev, rows) + (size_t)rows * sizeof(SampleConst) + 4096;
 }
 
 int launch_eval(gwl_event* e, const double* params, int64_t n, int ld, int precision, double* out,
