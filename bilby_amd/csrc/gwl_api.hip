@@ -624,6 +624,7 @@ int gwl_noise_log_likelihood(gwl_event* e, double* out) {
 
 int gwl_eval(gwl_event* e, const double* params, int64_t n, int32_t ld, int32_t precision,
              double* out, double* aux, void* stream) {
+  if (e && n == 0) return GWL_OK;  // an empty batch needs no buffers
   if (!e || !params || !out || n < 0 || ld < GWL_NPARAM) {
     set_error("gwl_eval: invalid argument");
     return GWL_E_INVALID;

@@ -372,6 +372,7 @@ int gwl_calib_device_bytes(gwl_calib* c, int64_t* bytes) {
 
 int gwl_calib_eval(gwl_calib* c, const double* params, int64_t n, int32_t ld, int32_t mode,
                    double* out, double* per_curve, void* stream) {
+  if (c && n == 0) return GWL_OK;  // an empty batch needs no buffers
   if (!c || !params || !out || n < 0 || ld < GWL_NPARAM || mode < 0 || mode > 2) {
     set_error("gwl_calib_eval: invalid argument");
     return GWL_E_INVALID;

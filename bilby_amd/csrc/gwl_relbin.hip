@@ -524,6 +524,7 @@ int gwl_relbin_device_bytes(gwl_relbin* rb, int64_t* bytes) {
 
 int gwl_relbin_eval(gwl_relbin* rb, const double* params, int64_t n, int32_t ld, int32_t precision,
                     double* out, double* aux, void* stream) {
+  if (rb && n == 0) return GWL_OK;  // an empty batch needs no buffers
   if (!rb || !params || !out || n < 0 || ld < GWL_NPARAM ||
       (precision != GWL_FP32 && precision != GWL_FP64)) {
     set_error("gwl_relbin_eval: invalid argument");
@@ -660,6 +661,7 @@ int gwl_mb_device_bytes(gwl_mb* mb, int64_t* bytes) {
 
 int gwl_mb_eval(gwl_mb* mb, const double* params, int64_t n, int32_t ld, int32_t precision,
                 double* out, double* aux, void* stream) {
+  if (mb && n == 0) return GWL_OK;  // an empty batch needs no buffers
   if (!mb || !params || !out || n < 0 || ld < GWL_NPARAM ||
       (precision != GWL_FP32 && precision != GWL_FP64)) {
     set_error("gwl_mb_eval: invalid argument");

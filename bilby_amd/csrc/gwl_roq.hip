@@ -781,6 +781,7 @@ int gwl_roq_reserve(gwl_roq* e, int64_t rows) {
 
 int gwl_roq_eval(gwl_roq* e, const double* params, int64_t n, int32_t ld, int32_t mode,
                  double* out, void* stream) {
+  if (e && n == 0) return GWL_OK;  // an empty batch needs no buffers
   if (!e || !params || !out || n < 0 || ld < GWL_NPARAM || mode < 0 || mode > 2) {
     set_error("gwl_roq_eval: invalid argument");
     return GWL_E_INVALID;

@@ -35,6 +35,19 @@ def get_context(device=0):
     return _CTX[device]
 
 
+def _batch_rows(parameters):
+    """Rows in a batch given as a dict of arrays, an [N, 16] array or tensor."""
+    if isinstance(parameters, dict):
+        for v in parameters.values():
+            if np.ndim(v) > 0:
+                return len(v)
+        return 1
+    shape = getattr(parameters, "shape", None)
+    if shape is None:
+        shape = np.shape(parameters)
+    return int(shape[0]) if len(shape) > 0 else 1
+
+
 def _precision_code(precision):
     if precision in ("fp32", "float32", _lib.FP32):
         return _lib.FP32
@@ -452,6 +465,13 @@ class GravitationalWaveTransient:
         """
         prec = self._mode(precision)
         on_device = isinstance(parameters, torch.Tensor) and parameters.is_cuda
+        if _batch_rows(parameters) == 0:
+            empty = (torch.empty(0, dtype=torch.float64, device=parameters.device) if on_device
+                     else np.zeros(0))
+            if return_inner_products:
+                return empty, (torch.empty((0, 3), dtype=torch.float64, device=parameters.device)
+                               if on_device else np.zeros((0, 3)))
+            return empty
         if on_device:
             rows_t = parameters.to(torch.float64).contiguous()
         else:

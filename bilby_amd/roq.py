@@ -17,7 +17,7 @@ from . import _lib
 from . import abi as _abi
 from ._lib import LIB, check, dptr
 from .detector import InterferometerList
-from .likelihood import distance_interpolant, get_context
+from .likelihood import _batch_rows, distance_interpolant, get_context
 from .prior import Uniform
 
 MODES = {"fp64": 0, "fp32": 1, "bf16x3": 2}
@@ -191,6 +191,9 @@ class ROQGravitationalWaveTransient:
         mode = self.precision if precision is None else (
             MODES[precision] if isinstance(precision, str) else int(precision))
         on_device = isinstance(parameters, torch.Tensor) and parameters.is_cuda
+        if _batch_rows(parameters) == 0:
+            return (torch.empty(0, dtype=torch.float64, device=parameters.device) if on_device
+                    else np.zeros(0))
         if on_device:
             rows_t = parameters.to(torch.float64).contiguous()
         else:

@@ -81,6 +81,20 @@ class ServedResult:
         return self._rc is not None
 
 
+class _EmptyResult:
+    """Future of a request with no rows: already done."""
+
+    def __init__(self, event):
+        self.event = event
+        self.latency = 0.0
+
+    def result(self, timeout=None):
+        return np.zeros(0)
+
+    def done(self):
+        return True
+
+
 class LikelihoodServer:
     """Request queue over device-resident likelihoods.
 
@@ -179,6 +193,8 @@ class LikelihoodServer:
         rows = np.ascontiguousarray(rows, dtype=np.float64)
         if rows.ndim != 2 or rows.shape[1] != _lib.NPARAM:
             raise ValueError("rows must be [n, %d]" % _lib.NPARAM)
+        if rows.shape[0] == 0:
+            return _EmptyResult(event)
         out = np.empty(rows.shape[0])
         t = ctypes.c_int64()
         check(LIB.gwl_server_submit(self._s, self.events[event], _lib.dptr(rows), rows.shape[0],
