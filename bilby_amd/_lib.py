@@ -57,6 +57,13 @@ class RelbinDesc(ctypes.Structure):
     ]
 
 
+class MbTimeDesc(ctypes.Structure):
+    """gwl_mb_time_desc (include/gwl.h)."""
+    _fields_ = [("grid_index", ctypes.POINTER(ctypes.c_int64)), ("n_fft", ctypes.c_int64),
+                ("j_first", ctypes.c_int64), ("n_window", ctypes.c_int32),
+                ("t_step", ctypes.c_double), ("antenna_time", ctypes.c_double)]
+
+
 class GWLError(RuntimeError):
     pass
 
@@ -137,6 +144,8 @@ def _load():
                                     ctypes.c_int32, ctypes.c_double, _dp], ctypes.c_int),
         "gwl_mb_create": ([vp, ctypes.c_int32, _dp, _dp, _dp, ctypes.POINTER(vp)], ctypes.c_int),
         "gwl_mb_destroy": ([vp], ctypes.c_int),
+        "gwl_mb_set_time": ([vp, ctypes.POINTER(MbTimeDesc)], ctypes.c_int),
+        "gwl_relbin_set_time": ([vp, ctypes.POINTER(ctypes.c_int64), _dp], ctypes.c_int),
         "gwl_mb_reserve": ([vp, ctypes.c_int64], ctypes.c_int),
         "gwl_mb_device_bytes": ([vp, ctypes.POINTER(ctypes.c_int64)], ctypes.c_int),
         "gwl_mb_eval": ([vp, vp, ctypes.c_int64, ctypes.c_int32, ctypes.c_int32, vp, vp, vp],
@@ -196,5 +205,6 @@ EXPORTED_SYMBOLS = [
     "gwl_relbin_eval", "gwl_server_add_relbin", "gwl_relbin_reserve", "gwl_calib_create",
     "gwl_calib_destroy", "gwl_calib_reserve", "gwl_calib_device_bytes", "gwl_calib_eval",
     "gwl_mb_create", "gwl_mb_destroy", "gwl_mb_reserve", "gwl_mb_device_bytes", "gwl_mb_eval",
+    "gwl_mb_set_time", "gwl_relbin_set_time",
     "gwl_roq_weights_linear",
 ]

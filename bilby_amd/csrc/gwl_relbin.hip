@@ -32,8 +32,11 @@
 // marginalisation).  Summary data and edge tables are a few tens of KB per
 // event and stay in L2.
 #include "gwl_internal.h"
+#include "gwl_window.h"
 
 #include <cmath>
+#include <algorithm>
+#include <cstddef>
 #include <cstring>
 #include <vector>
 
@@ -52,6 +55,10 @@ struct gwl_relbin {
   float* s32 = nullptr;
   void* ws = nullptr;         // row constants and per-row sums
   size_t ws_bytes = 0;
+  gwl::WinBasis win;          // time marginalisation (gwl_relbin_set_time)
+  bool has_win = false;
+  void* tws = nullptr;        // GEMM operand and output of one row tile
+  size_t tws_bytes = 0;
 };
 
 // Multibanded likelihood (bilby/gw/likelihood/multiband.py:728-807, linear
@@ -71,6 +78,10 @@ struct gwl_mb {
   float* quad32 = nullptr;
   void* ws = nullptr;
   size_t ws_bytes = 0;
+  gwl::WinBasis win;          // time marginalisation (gwl_mb_set_time)
+  bool has_win = false;
+  void* tws = nullptr;
+  size_t tws_bytes = 0;
 };
 
 namespace gwl {
@@ -100,6 +111,9 @@ struct RelbinArgs {
   double* sums;      // [n][3]: Re <d|h>, Im <d|h>, <h|h>
   double* out;
   double* aux;
+  double antenna_time;  // antenna patterns at this time (NaN: the row's time)
+  void* win_a;          // time marginalisation: edge ratios as GEMM operand rows, or null
+  int win_k, win_kp;
 };
 
 size_t relbin_ws_bytes(long long n) {
@@ -133,13 +147,20 @@ __device__ __forceinline__ double reduce_wave(double v) {
 
 // One thread per row: everything that depends on the parameter row only
 // (interferometer.py:303-368 antenna and delays, the TaylorF2 constants).
+// With time marginalisation and jitter the geocent time is shifted by the
+// row's time_jitter first (base.py:427-428); the antenna patterns are taken
+// at a.antenna_time when it is set (interferometer.py:336-339,
+// reference_time).
 __global__ __launch_bounds__(128) void k_relbin_rows(RelbinArgs a) {
   const long long r = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= a.n) return;
   const EvDev& ev = a.ev;
   const double* p = a.params + r * a.ld;
   const double ra = p[GWL_P_RA], dec = p[GWL_P_DEC], psi = p[GWL_P_PSI];
-  const double tc = p[GWL_P_GEOCENT_TIME], iota = p[GWL_P_THETA_JN];
+  const bool jitter = (ev.flags & GWL_TIME_MARG) && (ev.flags & GWL_JITTER_TIME);
+  const double tc = jitter ? p[GWL_P_GEOCENT_TIME] + p[GWL_P_TIME_JITTER] : p[GWL_P_GEOCENT_TIME];
+  const double iota = p[GWL_P_THETA_JN];
+  const bool fixed_antenna = !isnan(a.antenna_time);
   RelRow* o = a.rows + r;
   double c[10], amp0, fisco;
   o->valid = tf2_row_constants(p, ev.fref, 1.0, c, amp0, fisco);
@@ -153,6 +174,11 @@ __global__ __launch_bounds__(128) void k_relbin_rows(RelbinArgs a) {
     if (d < ev.nd) {
       double fp, fc, delay;
       antenna_and_delay(ev.tensor[d], ev.vertex[d], ra, dec, tc, psi, fp, fc, delay);
+      if (fixed_antenna) {
+        double delay_ref;
+        antenna_and_delay(ev.tensor[d], ev.vertex[d], ra, dec, a.antenna_time, psi, fp, fc,
+                          delay_ref);
+      }
       dt = (tc - ev.start) + delay;  // interferometer.py:355-358
       cre = fp * ap;
       cim = -fc * ax;
@@ -203,8 +229,21 @@ __global__ __launch_bounds__(256) void k_relbin(RelbinArgs a) {
   const double flo = a.ev.wf_fmin;
   R dhr = 0, dhi = 0, hh = 0;
   C cur[ND], nxt[ND], right[ND];
+  R* wa = a.win_a ? static_cast<R*>(a.win_a) + (size_t)row * a.win_kp : nullptr;
+  // operand column d (nb + 1) + e holds r_d(e) (real part; imaginary part at
+  // column win_k + d (nb + 1) + e)
+  auto put_edges = [&](int e, const C* v) {
+    if (!wa || e > a.nb) return;
+#pragma unroll
+    for (int d = 0; d < ND; ++d) {
+      const int k = d * (a.nb + 1) + e;
+      wa[k] = v[d].x;
+      wa[a.win_k + k] = v[d].y;
+    }
+  };
   edge_ratio<R, ND>(a, rr, flo, lane, cur);
   for (int base = 0; base < a.nb; base += 64) {
+    put_edges(base + lane, cur);
     if (base + 64 <= a.nb) {
       edge_ratio<R, ND>(a, rr, flo, base + 64 + lane, nxt);
     } else {
@@ -236,6 +275,7 @@ __global__ __launch_bounds__(256) void k_relbin(RelbinArgs a) {
 #pragma unroll
     for (int d = 0; d < ND; ++d) cur[d] = nxt[d];
   }
+  put_edges((a.nb + 63) / 64 * 64 + lane, cur);
   const double dh_re = reduce_wave((double)dhr);
   const double dh_im = reduce_wave((double)dhi);
   const double h2 = reduce_wave((double)hh);
@@ -258,6 +298,8 @@ struct MbArgs {
   long long rows_n;
   const RelRow* rows;
   double* sums;
+  void* win_a;        // time marginalisation: sum_d h_d L_d per point as GEMM operand rows
+  int win_kp;
 };
 
 // One wavefront per row, lanes over the banded points: the detector
@@ -279,15 +321,23 @@ __global__ __launch_bounds__(256) void k_mb(MbArgs a) {
     cabs2[d] = cr * cr + ci * ci;
   }
   double dr = 0.0, di = 0.0, hh = 0.0;
+  R* wa = a.win_a ? static_cast<R*>(a.win_a) + (size_t)row * a.win_kp : nullptr;
   for (int p = lane; p < a.n; p += 64) {
     const double f = a.fe[p];
-    if (!(f >= a.flo && f <= rr.fhi)) continue;
+    if (!(f >= a.flo && f <= rr.fhi)) {
+      if (wa) {
+        wa[p] = (R)0;
+        wa[a.n + p] = (R)0;
+      }
+      continue;
+    }
     double b[kNB];
 #pragma unroll
     for (int k = 0; k < kNB; ++k) b[k] = a.basis[(size_t)p * kNB + k];
     const double psi = tf2_phase_cycles(rr.c, b);
     const double A = a.amp[p];
     const R Ar = (R)A;
+    double yr = 0.0, yi = 0.0;
 #pragma unroll
     for (int d = 0; d < ND; ++d) {
       double ph = fma(f, rr.dt[d], psi);
@@ -295,9 +345,15 @@ __global__ __launch_bounds__(256) void k_mb(MbArgs a) {
       const C w = expm2pi((R)ph);
       const C h = cmul(cmk<C>((R)(rr.cre[d] * a.inv_sigma) * Ar, (R)(rr.cim[d] * a.inv_sigma) * Ar), w);
       const C L = RelbinT<R>::load(a.lin, (size_t)d * a.n + p);
-      dr += (double)(h.x * L.x - h.y * L.y);
-      di += (double)(h.x * L.y + h.y * L.x);
+      yr += (double)(h.x * L.x - h.y * L.y);
+      yi += (double)(h.x * L.y + h.y * L.x);
       hh += cabs2[d] * A * A * (double)RelbinT<R>::sval(a.quad, (size_t)d * a.n + p);
+    }
+    dr += yr;
+    di += yi;
+    if (wa) {  // multiband.py:792-798: the point's term of the full d h array
+      wa[p] = (R)yr;
+      wa[a.n + p] = (R)yi;
     }
   }
   dr = reduce_wave(dr);
@@ -406,6 +462,67 @@ int relbin_reserve(gwl_relbin* rb, long long rows) {
   return grow_ws(rb->ev, &rb->ws, &rb->ws_bytes, rows);
 }
 
+int grow_buf(gwl_event* ev, void** buf, size_t* bytes, size_t need) {
+  if (need <= *bytes) return GWL_OK;
+  GWL_HIP_CHECK(hipSetDevice(ev->ctx->device));
+  if (*buf) (void)hipFree(*buf);
+  *buf = nullptr;
+  *bytes = 0;
+  if (hipMalloc(buf, need) != hipSuccess) {
+    set_error("hipMalloc failed for the time-window workspace");
+    return GWL_E_NOMEM;
+  }
+  *bytes = need;
+  return GWL_OK;
+}
+
+static_assert(sizeof(RelRow) % sizeof(int) == 0, "RelRow holds whole ints");
+constexpr int kValidStride = (int)(sizeof(RelRow) / sizeof(int));
+
+const int* valid_of(const RelRow* rows) {
+  return reinterpret_cast<const int*>(reinterpret_cast<const char*>(rows) +
+                                      offsetof(RelRow, valid));
+}
+
+// Row tiles of at most kWinTileRows: row constants, the binned kernel (which
+// also writes the operand rows), the window GEMM and epilogue.  ``kernel``
+// launches the binned kernel for the tile described by ``a``.
+template <typename Launch>
+int eval_time_tiles(gwl_event* ev, const WinBasis& win, void** ws, size_t* ws_bytes, void** tws,
+                    size_t* tws_bytes, RelbinArgs a, const double* params, long long n, int ld,
+                    bool fp64, double* out, double* aux, hipStream_t st, Launch kernel) {
+  const long long tile = std::min<long long>(n, kWinTileRows);
+  if (int rc = grow_ws(ev, ws, ws_bytes, tile)) return rc;
+  if (int rc = grow_buf(ev, tws, tws_bytes, win_ws_bytes(win, tile))) return rc;
+  char* w = static_cast<char*>(*ws);
+  a.rows = reinterpret_cast<RelRow*>(w);
+  a.sums = reinterpret_cast<double*>(w + ((size_t)tile * sizeof(RelRow) + 255) / 256 * 256);
+  void* opa = win_a(win, *tws);
+  double* c = win_c(win, *tws, tile);
+  const size_t lds = (ev->dev.flags & GWL_DISTANCE_MARG)
+                         ? (size_t)(ev->dev.sp.nx + ev->dev.sp.ny + 8) * 8 : 0;
+  for (long long r0 = 0; r0 < n; r0 += tile) {
+    const long long nt = std::min(tile, n - r0);
+    a.params = params + r0 * ld;
+    a.n = nt;
+    a.out = out + r0;
+    a.aux = aux ? aux + 3 * r0 : nullptr;
+    if (int rc = win_clear(win, opa, fp64, nt, st)) return rc;
+    const unsigned g128 = (unsigned)((nt + 127) / 128);
+    hipLaunchKernelGGL(k_relbin_rows, dim3(g128), dim3(128), 0, st, a);
+    GWL_HIP_CHECK(hipGetLastError());
+    if (int rc = kernel(a, opa)) return rc;
+    if (a.aux) {  // the inner products at the row's own time
+      hipLaunchKernelGGL(k_relbin_epi, dim3(g128), dim3(128), lds, st, a);
+      GWL_HIP_CHECK(hipGetLastError());
+    }
+    if (int rc = win_finish(ev, win, opa, fp64, c, nt, a.params, ld, a.sums, valid_of(a.rows),
+                            kValidStride, a.out, st))
+      return rc;
+  }
+  return GWL_OK;
+}
+
 }  // namespace
 }  // namespace gwl
 
@@ -418,8 +535,8 @@ int gwl_relbin_create(gwl_event* ev, const gwl_relbin_desc* desc, gwl_relbin** o
     return GWL_E_INVALID;
   }
   const gwl::EvDev& d = ev->dev;
-  if (d.approx != GWL_TAYLORF2 || (d.flags & GWL_TIME_MARG)) {
-    set_error("gwl_relbin_create: the event must be TaylorF2 without time marginalisation");
+  if (d.approx != GWL_TAYLORF2) {
+    set_error("gwl_relbin_create: the event must be TaylorF2");
     return GWL_E_UNSUPPORTED;
   }
   const int nb = desc->n_bins, ne = nb + 1, nd = d.nd;
@@ -507,6 +624,8 @@ int gwl_relbin_destroy(gwl_relbin* rb) {
   (void)hipSetDevice(rb->ev->ctx->device);
   if (rb->data) (void)hipFree(rb->data);
   if (rb->ws) (void)hipFree(rb->ws);
+  if (rb->tws) (void)hipFree(rb->tws);
+  gwl::win_basis_free(&rb->win);
   delete rb;
   return GWL_OK;
 }
@@ -518,7 +637,7 @@ int gwl_relbin_reserve(gwl_relbin* rb, int64_t rows) {
 
 int gwl_relbin_device_bytes(gwl_relbin* rb, int64_t* bytes) {
   if (!rb || !bytes) { set_error("gwl_relbin_device_bytes: invalid argument"); return GWL_E_INVALID; }
-  *bytes = (int64_t)(rb->data_bytes + rb->ws_bytes);
+  *bytes = (int64_t)(rb->data_bytes + rb->ws_bytes + rb->win.bytes + rb->tws_bytes);
   return GWL_OK;
 }
 
@@ -536,10 +655,18 @@ int gwl_relbin_eval(gwl_relbin* rb, const double* params, int64_t n, int32_t ld,
     set_error("gwl_relbin_eval: distance marginalisation requested but no spline attached");
     return GWL_E_INVALID;
   }
-  if (int rc = gwl::relbin_reserve(rb, n)) return rc;
+  const bool timem = (ev->dev.flags & GWL_TIME_MARG) != 0;
+  if (timem && !rb->has_win) {
+    set_error("gwl_relbin_eval: the event marginalises over time; call gwl_relbin_set_time first");
+    return GWL_E_INVALID;
+  }
+  if (!timem)
+    if (int rc = gwl::relbin_reserve(rb, n)) return rc;
   GWL_HIP_CHECK(hipSetDevice(ev->ctx->device));
   (void)hipGetLastError();
   gwl::RelbinArgs a;
+  std::memset(&a, 0, sizeof a);
+  a.antenna_time = NAN;
   a.ev = ev->dev;
   a.nb = rb->nb;
   a.fe = rb->fe;
@@ -558,6 +685,21 @@ int gwl_relbin_eval(gwl_relbin* rb, const double* params, int64_t n, int32_t ld,
                          ? (size_t)(ev->dev.sp.nx + ev->dev.sp.ny + 8) * 8 : 0;
   hipStream_t st = reinterpret_cast<hipStream_t>(stream);
   if (int rc = gwl::timing_begin(ev, st)) return rc;
+  if (timem) {
+    const bool fp64 = precision == GWL_FP64;
+    a.win_k = rb->win.K;
+    a.win_kp = rb->win.Kp;
+    const int nd = ev->dev.nd;
+    const int rc = gwl::eval_time_tiles(
+        ev, rb->win, &rb->ws, &rb->ws_bytes, &rb->tws, &rb->tws_bytes, a, params, n, ld, fp64, out,
+        aux, st, [&](gwl::RelbinArgs t, void* opa) {
+          t.win_a = opa;
+          return fp64 ? gwl::launch_relbin_nd<double>(t, nd, st)
+                      : gwl::launch_relbin_nd<float>(t, nd, st);
+        });
+    if (rc != GWL_OK) return rc;
+    return gwl::timing_end(ev, st);
+  }
   const unsigned g128 = (unsigned)((n + 127) / 128);
   hipLaunchKernelGGL(gwl::k_relbin_rows, dim3(g128), dim3(128), 0, st, a);
   GWL_HIP_CHECK(hipGetLastError());
@@ -567,6 +709,59 @@ int gwl_relbin_eval(gwl_relbin* rb, const double* params, int64_t n, int32_t ld,
   hipLaunchKernelGGL(gwl::k_relbin_epi, dim3(g128), dim3(128), lds, st, a);
   GWL_HIP_CHECK(hipGetLastError());
   return gwl::timing_end(ev, st);
+}
+
+int gwl_relbin_set_time(gwl_relbin* rb, const int64_t* bin_index, const double* integrand) {
+  if (!rb || !bin_index || !integrand) {
+    set_error("gwl_relbin_set_time: invalid argument");
+    return GWL_E_INVALID;
+  }
+  gwl_event* ev = rb->ev;
+  const gwl::EvDev& d = ev->dev;
+  if (!(d.flags & GWL_TIME_MARG)) {
+    set_error("gwl_relbin_set_time: the event does not marginalise over time");
+    return GWL_E_UNSUPPORTED;
+  }
+  const int nb = rb->nb;
+  for (int e = 0; e <= nb; ++e)
+    if (bin_index[e] < 0 || bin_index[e] > d.N || (e > 0 && bin_index[e] <= bin_index[e - 1])) {
+      set_error("gwl_relbin_set_time: bin indices must increase within the grid");
+      return GWL_E_INVALID;
+    }
+  const long long n_x = bin_index[nb] - bin_index[0] + 1;
+  GWL_HIP_CHECK(hipSetDevice(ev->ctx->device));
+  long long* de = nullptr;
+  double2* dx = nullptr;
+  if (hipMalloc((void**)&de, (size_t)(nb + 1) * 8) != hipSuccess ||
+      hipMalloc((void**)&dx, (size_t)d.nd * n_x * sizeof(double2)) != hipSuccess) {
+    if (de) (void)hipFree(de);
+    set_error("hipMalloc failed for the time-window set-up");
+    return GWL_E_NOMEM;
+  }
+  hipError_t err = hipMemcpy(de, bin_index, (size_t)(nb + 1) * 8, hipMemcpyHostToDevice);
+  if (err == hipSuccess)
+    err = hipMemcpy(dx, integrand, (size_t)d.nd * n_x * sizeof(double2), hipMemcpyHostToDevice);
+  int rc = GWL_OK;
+  gwl::WinBasis wb;
+  if (err != hipSuccess) {
+    rc = gwl::hip_fail(err, "gwl_relbin_set_time upload");
+  } else {
+    // the event's window and time grid (base.py:1027-1035)
+    wb.L = d.L;
+    wb.j_first = d.jlo;
+    wb.start = d.start;
+    wb.offset = 1;
+    wb.t_step = d.tstep;
+    wb.delta_tc = d.tdelta;
+    rc = gwl::win_basis_relbin(ev, d.nd, nb, de, rb->fe, dx, n_x, &wb);
+  }
+  (void)hipFree(de);
+  (void)hipFree(dx);
+  if (rc != GWL_OK) return rc;
+  gwl::win_basis_free(&rb->win);
+  rb->win = wb;
+  rb->has_win = true;
+  return GWL_OK;
 }
 
 }  // extern "C"
@@ -580,8 +775,8 @@ int gwl_mb_create(gwl_event* ev, int32_t n_points, const double* frequencies, co
     return GWL_E_INVALID;
   }
   const gwl::EvDev& d = ev->dev;
-  if (d.approx != GWL_TAYLORF2 || (d.flags & GWL_TIME_MARG)) {
-    set_error("gwl_mb_create: the event must be TaylorF2 without time marginalisation");
+  if (d.approx != GWL_TAYLORF2) {
+    set_error("gwl_mb_create: the event must be TaylorF2");
     return GWL_E_UNSUPPORTED;
   }
   const int n = n_points, nd = d.nd;
@@ -644,6 +839,8 @@ int gwl_mb_destroy(gwl_mb* mb) {
   (void)hipSetDevice(mb->ev->ctx->device);
   if (mb->data) (void)hipFree(mb->data);
   if (mb->ws) (void)hipFree(mb->ws);
+  if (mb->tws) (void)hipFree(mb->tws);
+  gwl::win_basis_free(&mb->win);
   delete mb;
   return GWL_OK;
 }
@@ -655,7 +852,7 @@ int gwl_mb_reserve(gwl_mb* mb, int64_t rows) {
 
 int gwl_mb_device_bytes(gwl_mb* mb, int64_t* bytes) {
   if (!mb || !bytes) { set_error("gwl_mb_device_bytes: invalid argument"); return GWL_E_INVALID; }
-  *bytes = (int64_t)(mb->data_bytes + mb->ws_bytes);
+  *bytes = (int64_t)(mb->data_bytes + mb->ws_bytes + mb->win.bytes + mb->tws_bytes);
   return GWL_OK;
 }
 
@@ -677,7 +874,13 @@ int gwl_mb_eval(gwl_mb* mb, const double* params, int64_t n, int32_t ld, int32_t
     set_error("gwl_mb_eval: too many rows in one call");
     return GWL_E_INVALID;
   }
-  if (int rc = gwl::grow_ws(ev, &mb->ws, &mb->ws_bytes, n)) return rc;
+  const bool timem = (ev->dev.flags & GWL_TIME_MARG) != 0;
+  if (timem && !mb->has_win) {
+    set_error("gwl_mb_eval: the event marginalises over time; call gwl_mb_set_time first");
+    return GWL_E_INVALID;
+  }
+  if (!timem)
+    if (int rc = gwl::grow_ws(ev, &mb->ws, &mb->ws_bytes, n)) return rc;
   GWL_HIP_CHECK(hipSetDevice(ev->ctx->device));
   (void)hipGetLastError();
   gwl::RelbinArgs a;
@@ -691,7 +894,9 @@ int gwl_mb_eval(gwl_mb* mb, const double* params, int64_t n, int32_t ld, int32_t
   a.sums = reinterpret_cast<double*>(w + ((size_t)n * sizeof(gwl::RelRow) + 255) / 256 * 256);
   a.out = out;
   a.aux = aux;
+  a.antenna_time = NAN;
   gwl::MbArgs m;
+  std::memset(&m, 0, sizeof m);
   m.n = mb->n;
   m.flo = ev->dev.wf_fmin;
   m.inv_sigma = 1.0 / ev->dev.sigma;
@@ -707,6 +912,24 @@ int gwl_mb_eval(gwl_mb* mb, const double* params, int64_t n, int32_t ld, int32_t
                          ? (size_t)(ev->dev.sp.nx + ev->dev.sp.ny + 8) * 8 : 0;
   hipStream_t st = reinterpret_cast<hipStream_t>(stream);
   if (int rc = gwl::timing_begin(ev, st)) return rc;
+  if (timem) {
+    const bool fp64 = precision == GWL_FP64;
+    a.antenna_time = mb->win.antenna_time;
+    const int nd = ev->dev.nd;
+    const int rc = gwl::eval_time_tiles(
+        ev, mb->win, &mb->ws, &mb->ws_bytes, &mb->tws, &mb->tws_bytes, a, params, n, ld, fp64, out,
+        aux, st, [&](gwl::RelbinArgs t, void* opa) {
+          gwl::MbArgs mt = m;
+          mt.rows_n = t.n;
+          mt.rows = t.rows;
+          mt.sums = t.sums;
+          mt.win_a = opa;
+          mt.win_kp = mb->win.Kp;
+          return fp64 ? gwl::launch_mb_nd<double>(mt, nd, st) : gwl::launch_mb_nd<float>(mt, nd, st);
+        });
+    if (rc != GWL_OK) return rc;
+    return gwl::timing_end(ev, st);
+  }
   const unsigned g128 = (unsigned)((n + 127) / 128);
   hipLaunchKernelGGL(gwl::k_relbin_rows, dim3(g128), dim3(128), 0, st, a);
   GWL_HIP_CHECK(hipGetLastError());
@@ -716,6 +939,52 @@ int gwl_mb_eval(gwl_mb* mb, const double* params, int64_t n, int32_t ld, int32_t
   hipLaunchKernelGGL(gwl::k_relbin_epi, dim3(g128), dim3(128), lds, st, a);
   GWL_HIP_CHECK(hipGetLastError());
   return gwl::timing_end(ev, st);
+}
+
+int gwl_mb_set_time(gwl_mb* mb, const gwl_mb_time_desc* td) {
+  if (!mb || !td || !td->grid_index || td->n_fft < 2 || td->j_first < 0 || td->n_window < 1 ||
+      td->j_first + td->n_window > td->n_fft || !(td->t_step > 0.0)) {
+    set_error("gwl_mb_set_time: invalid argument");
+    return GWL_E_INVALID;
+  }
+  gwl_event* ev = mb->ev;
+  if (!(ev->dev.flags & GWL_TIME_MARG)) {
+    set_error("gwl_mb_set_time: the event does not marginalise over time");
+    return GWL_E_UNSUPPORTED;
+  }
+  for (int p = 0; p < mb->n; ++p)
+    if (td->grid_index[p] >= td->n_fft) {
+      set_error("gwl_mb_set_time: a grid index lies beyond the transform length");
+      return GWL_E_INVALID;
+    }
+  GWL_HIP_CHECK(hipSetDevice(ev->ctx->device));
+  long long* di = nullptr;
+  if (hipMalloc((void**)&di, (size_t)mb->n * 8) != hipSuccess) {
+    set_error("hipMalloc failed for the time-window set-up");
+    return GWL_E_NOMEM;
+  }
+  hipError_t err = hipMemcpy(di, td->grid_index, (size_t)mb->n * 8, hipMemcpyHostToDevice);
+  int rc = GWL_OK;
+  gwl::WinBasis wb;
+  if (err != hipSuccess) {
+    rc = gwl::hip_fail(err, "gwl_mb_set_time upload");
+  } else {
+    // multiband.py:714-726: times start + j T_0 / N, prior weight spacing T_0 / N
+    wb.L = td->n_window;
+    wb.j_first = td->j_first;
+    wb.start = ev->dev.start;
+    wb.offset = 0;
+    wb.t_step = td->t_step;
+    wb.delta_tc = td->t_step;
+    wb.antenna_time = td->antenna_time;
+    rc = gwl::win_basis_points(ev, mb->n, di, td->n_fft, &wb);
+  }
+  (void)hipFree(di);
+  if (rc != GWL_OK) return rc;
+  gwl::win_basis_free(&mb->win);
+  mb->win = wb;
+  mb->has_win = true;
+  return GWL_OK;
 }
 
 }  // extern "C"

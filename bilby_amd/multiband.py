@@ -9,8 +9,10 @@ linear coefficients (data filtered, down-sampled and windowed per band) and
 the quadratic coefficients.  Each evaluation runs ``k_mb``
 (csrc/gwl_relbin.hip): TaylorF2 at every banded point, the detector
 response, <d|h> = conj(sum h L) and <h|h> = sum |h|^2 Q, then the
-marginalisation.  The IFFT-FFT algorithm for <h|h> and time marginalisation
-are not provided.
+marginalisation.  With time marginalisation (multiband.py:714-726, 792-800)
+the transform of the scattered h L terms at the time-prior window is a
+fixed linear map of the per-point terms, evaluated as one GEMM per batch
+(gwl_mb_set_time).  The IFFT-FFT algorithm for <h|h> is not provided.
 """
 import ctypes
 import math
@@ -304,13 +306,12 @@ class MBGravitationalWaveTransient(GravitationalWaveTransient):
                  priors=None, time_marginalization=False, jitter_time=True,
                  distance_marginalization_lookup_table=None, reference_frame="sky",
                  time_reference="geocenter", precision="fp64", device=0):
-        if time_marginalization:
-            raise NotImplementedError("multibanding with time marginalisation is not supported")
         if not linear_interpolation:
             raise NotImplementedError("only the linear-interpolation algorithm for <h|h> is "
                                       "provided")
         self._mb = None
-        super().__init__(interferometers, waveform_generator, time_marginalization=False,
+        super().__init__(interferometers, waveform_generator,
+                         time_marginalization=time_marginalization,
                          distance_marginalization=distance_marginalization,
                          phase_marginalization=phase_marginalization, priors=priors,
                          distance_marginalization_lookup_table=distance_marginalization_lookup_table,
@@ -338,7 +339,53 @@ class MBGravitationalWaveTransient(GravitationalWaveTransient):
             self.setup = MultibandSetup(ifos, ifos.duration, reference_chirp_mass, highest_mode,
                                         accuracy_factor, time_offset, delta_f_end,
                                         maximum_banding_frequency, minimum_banding_duration)
+        if self.time_marginalization:
+            self._setup_time_marginalization_multiband()
         self._upload()
+
+    def _setup_time_marginalization_multiband(self):
+        """multiband.py:714-726: the time grid start + j T_0 / N, N = Nbs[-1] / 2,
+        with prior weight spacing T_0 / N; the grid index int(f T_0) of each
+        banded point; the antenna patterns at the middle of the geocent_time
+        prior.  bilby sets that time as ``interferometers.reference_time``
+        (which also changes later evaluations of other likelihoods sharing the
+        interferometers); here it applies to this likelihood only.
+
+        bilby fills the transformed array band by band with
+        ``a[idx] += v`` (multiband.py:792-798): within one band a repeated
+        index keeps only the last point's term, across bands the terms add.
+        ``_time_grid_index`` is that map with -1 for a point whose term is
+        dropped."""
+        s = self.setup
+        N = int(s.Nbs[-1]) // 2
+        jitter_bound = self._delta_tc / 2
+        self._delta_tc = float(s.durations[0]) / N
+        self._times = self.interferometers.start_time + np.arange(N) * self._delta_tc
+        prior = self.priors["geocent_time"]
+        self.time_prior_array = prior.prob(self._times) * self._delta_tc
+        self._full_to_multiband = [int(f * s.durations[0]) for f in s.banded_frequency_points]
+        grid = np.asarray(self._full_to_multiband, dtype=np.int64)
+        if len(grid) and grid.max() >= N:
+            raise ValueError("a banded frequency point lies beyond the time-marginalisation grid")
+        index = grid.copy()
+        for b in range(s.number_of_bands):
+            lo, hi = s.start_end_idxs[b]
+            band = grid[lo:hi + 1]
+            # the last occurrence of each index within the band survives
+            _, last = np.unique(band[::-1], return_index=True)
+            keep = np.zeros(len(band), dtype=bool)
+            keep[len(band) - 1 - last] = True
+            index[lo:hi + 1] = np.where(keep, band, -1)
+        self._time_grid_index = np.ascontiguousarray(index)
+        self._beam_pattern_reference_time = (prior.minimum + prior.maximum) / 2
+        margin = max(jitter_bound, self._delta_tc)
+        start = self.interferometers.start_time
+        j_lo = int(np.floor((prior.minimum - start - margin) / self._delta_tc)) - 3
+        j_hi = int(np.ceil((prior.maximum - start + margin) / self._delta_tc)) + 3
+        j_lo, j_hi = max(0, j_lo), min(N - 1, j_hi)
+        if j_hi < j_lo:
+            raise ValueError("the geocent_time prior lies outside the segment")
+        self._time_window = (N, j_lo, j_hi - j_lo + 1)
 
     def _minimum_chirp_mass(self):
         """gw/prior.py:721-737."""
@@ -374,6 +421,19 @@ class MBGravitationalWaveTransient(GravitationalWaveTransient):
         mb = ctypes.c_void_p()
         check(LIB.gwl_mb_create(self._event, n, dptr(freqs), dptr(lin), dptr(quad),
                                 ctypes.byref(mb)), "gwl_mb_create")
+        if self.time_marginalization:
+            N, j_first, n_window = self._time_window
+            td = _lib.MbTimeDesc()
+            td.grid_index = self._time_grid_index.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))
+            td.n_fft = N
+            td.j_first = j_first
+            td.n_window = n_window
+            td.t_step = self._delta_tc
+            td.antenna_time = self._beam_pattern_reference_time
+            rc = LIB.gwl_mb_set_time(mb, ctypes.byref(td))
+            if rc != 0:
+                LIB.gwl_mb_destroy(mb)
+                check(rc, "gwl_mb_set_time")
         self._mb = mb
 
     def _device_eval(self, rows_t, precision, aux=False, stream=None):

@@ -299,9 +299,10 @@ int gwl_gemm(gwl_ctx* ctx, int32_t mode, const void* A, const void* A_lo, const 
 
 /* ------------------------------------------------------------------ relative binning
  * RelativeBinningGravitationalWaveTransient (bilby/gw/likelihood/relative.py)
- * on a TaylorF2 event without time marginalisation; the event supplies the
- * detectors, the source model, the phase / distance flags and the distance
- * spline, and must outlive the relative-binning object.
+ * on a TaylorF2 event; the event supplies the detectors, the source model,
+ * the time / phase / distance flags and the distance spline, and must
+ * outlive the relative-binning object.  An event that marginalises over
+ * time also needs gwl_relbin_set_time before the first evaluation.
  *   bin_frequencies:   [n_bins + 1] edge frequencies f_e, increasing
  *                      (relative.py:177-231, setup_bins)
  *   fiducial_response: [n_det][n_bins + 1][2] complex128 detector response
@@ -326,15 +327,31 @@ int gwl_relbin_reserve(gwl_relbin* rb, int64_t rows);
  * marginalisation of base.py:448-477.  params: device [n][ld] float64;
  * out (and aux [n][3] = Re <d|h>, Im <d|h>, <h|h>, may be NULL): device.
  * precision GWL_FP64 or GWL_FP32 (float32 ratios and bin sums, float64
- * phase, float64 totals).  Timed by gwl_event_timing of the event. */
+ * phase, float64 totals).  With time marginalisation the lnL is that of
+ * base.py:794-820 over the event's time window, from the transform of the
+ * rebuilt full waveform (relative.py:376-391, 410-420), and aux holds the
+ * inner products at the row's own time.  Timed by gwl_event_timing of the
+ * event. */
 int gwl_relbin_eval(gwl_relbin* rb, const double* params, int64_t n, int32_t ld,
                     int32_t precision, double* out, double* aux, void* stream);
+/* Time marginalisation for relative binning (relative.py:376-391, 410-420):
+ *   bin_index: [n_bins + 1] grid indices of the edges (relative.py:216-224,
+ *              bin_inds)
+ *   integrand: [n_det][bin_index[n_bins] - bin_index[0] + 1][2] complex128,
+ *              4/T h0 conj(d) / S on the grid from bin_index[0] (h0 the
+ *              fiducial response), zero at the Nyquist bin
+ * The transform of the rebuilt waveform is linear in the edge ratios, so
+ * its values at the event's time window are built once here (one basis
+ * vector per detector and edge, float64 on the device) and each evaluation
+ * is one GEMM of the rows' edge ratios against them. */
+int gwl_relbin_set_time(gwl_relbin* rb, const int64_t* bin_index, const double* integrand);
 
 /* ------------------------------------------------------------------ multibanding
  * MBGravitationalWaveTransient (bilby/gw/likelihood/multiband.py, linear
- * interpolation for <h|h>) on a TaylorF2 event without time
- * marginalisation; the event supplies detectors, source model, flags and
- * the distance spline and must outlive the object.
+ * interpolation for <h|h>) on a TaylorF2 event; the event supplies
+ * detectors, source model, flags and the distance spline and must outlive
+ * the object.  An event that marginalises over time also needs
+ * gwl_mb_set_time before the first evaluation.
  *   frequencies: [n_points] banded frequency points (multiband.py:449-478)
  *   linear:      [n_det][n_points][2] complex linear coefficients
  *                (multiband.py:529-549)
@@ -348,9 +365,30 @@ int gwl_mb_reserve(gwl_mb* mb, int64_t rows);
 int gwl_mb_device_bytes(gwl_mb* mb, int64_t* bytes);
 /* ln L ratio of n rows: multiband.py:728-807 + base.py:448-477; params,
  * out, aux (may be NULL, [n][3]) device pointers; precision GWL_FP64 or
- * GWL_FP32 (float32 response and coefficients, float64 phase and sums). */
+ * GWL_FP32 (float32 response and coefficients, float64 phase and sums).
+ * With time marginalisation: base.py:794-820 over the window given to
+ * gwl_mb_set_time (aux: the inner products at the row's own time). */
 int gwl_mb_eval(gwl_mb* mb, const double* params, int64_t n_rows, int32_t ld,
                 int32_t precision, double* out, double* aux, void* stream);
+/* Time marginalisation for multibanding (multiband.py:714-726, 792-800):
+ * d_inner_h_array = FFT_N of the array that holds, at grid index
+ * grid_index[p], the sum of the points' h L terms.
+ *   grid_index:   [n_points] int(f_p T_0) (_full_to_multiband), or -1 for a
+ *                 point whose term does not enter (a later point of the same
+ *                 band writes the same index)
+ *   n_fft:        N = Nbs[-1] / 2
+ *   j_first, n_window: the transform bins evaluated (time start + j t_step)
+ *   t_step:       T_0 / N (also the time prior weight spacing)
+ *   antenna_time: GPS time of the antenna patterns (reference_time) */
+typedef struct gwl_mb_time_desc {
+  const int64_t* grid_index;
+  int64_t n_fft;
+  int64_t j_first;
+  int32_t n_window;
+  double t_step;
+  double antenna_time;
+} gwl_mb_time_desc;
+int gwl_mb_set_time(gwl_mb* mb, const gwl_mb_time_desc* td);
 
 /* ------------------------------------------------------------------ calibration
  * Calibration marginalisation (base.py:332-346, 860-877, lookup

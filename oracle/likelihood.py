@@ -58,10 +58,13 @@ class OracleEvent:
         self.frequency_array = detectors[0].frequency_array
 
 
-def detector_response(det, polarizations, params, start_time):
-    """bilby/gw/detector/interferometer.py:303-368 (calibration = ones)."""
+def detector_response(det, polarizations, params, start_time, antenna_time=None):
+    """bilby/gw/detector/interferometer.py:303-368 (calibration = ones); the
+    antenna patterns at ``antenna_time`` when given (reference_time,
+    :336-339), else at geocent_time."""
+    t_antenna = params["geocent_time"] if antenna_time is None else antenna_time
     fp, fc = core.antenna_response(det.tensor, params["ra"], params["dec"],
-                                   params["geocent_time"], params["psi"])
+                                   t_antenna, params["psi"])
     signal = polarizations["plus"] * det.mask * fp + polarizations["cross"] * det.mask * fc
     time_shift = core.time_delay_from_geocenter(det.vertex, params["ra"], params["dec"],
                                                 params["geocent_time"])
@@ -104,6 +107,9 @@ class OracleGWTransient:
         self.phase_marginalization = phase_marginalization
         self.priors = dict(priors) if priors is not None else None
         self.jitter_time = jitter_time if time_marginalization else False
+        # interferometers.reference_time (interferometer.py:336-339): when
+        # set, the antenna patterns are taken at this time
+        self.antenna_time = None
         self.duration = event.duration
         if time_marginalization:
             self._setup_time_marginalization()
@@ -188,7 +194,8 @@ class OracleGWTransient:
     # ---- per-detector SNRs: base.py:260-354
     def calculate_snrs(self, polarizations, det, params):
         ev = self.event
-        signal = detector_response(det, polarizations, params, ev.start_time)
+        signal = detector_response(det, polarizations, params, ev.start_time,
+                                   antenna_time=self.antenna_time)
         m = det.mask
         d_inner_h = core.noise_weighted_inner_product(signal[m], det.strain[m], det.psd[m], ev.duration)
         h_inner_h = core.noise_weighted_inner_product(signal[m], signal[m], det.psd[m], ev.duration).real

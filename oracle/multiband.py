@@ -1,6 +1,7 @@
 """FP64 restatement of bilby's MBGravitationalWaveTransient evaluation
-(bilby/gw/likelihood/multiband.py:728-807, linear interpolation) on given
-multiband weights, and a literal loop form of the quadratic coefficients
+(bilby/gw/likelihood/multiband.py:728-807, linear interpolation; time
+marginalisation :714-726, 792-800) on given multiband weights, and a
+literal loop form of the quadratic coefficients
 (multiband.py:551-611) to check the product's prefix-sum form against.
 
 Test infrastructure only (see oracle/__init__.py).
@@ -17,37 +18,66 @@ from .likelihood import OracleGWTransient, detector_response, source_polarizatio
 class OracleMultiband(OracleGWTransient):
     """``weights``: an object or dict with banded_frequency_points,
     unique_to_original_frequencies, waveform_frequencies, linear_coeffs and
-    quadratic_coeffs (detector name -> array)."""
+    quadratic_coeffs (detector name -> array); with time marginalisation
+    also durations, Nbs and start_end_idxs."""
 
-    def __init__(self, event, source, weights, **kwargs):
-        super().__init__(event, source, time_marginalization=False, **kwargs)
+    def __init__(self, event, source, weights, time_marginalization=False, **kwargs):
+        super().__init__(event, source, time_marginalization=time_marginalization, **kwargs)
         w = weights if isinstance(weights, dict) else weights.weights
         self.w = w
+        self.antenna_time = None
+        if time_marginalization:
+            self._setup_time_marginalization_multiband()
+
+    def _setup_time_marginalization_multiband(self):
+        """multiband.py:714-726."""
+        w = self.w
+        N = int(w["Nbs"][-1]) // 2
+        self._delta_tc = w["durations"][0] / N
+        self._times = self.event.start_time + np.arange(N) * self._delta_tc
+        self._full_d_h = np.zeros(N, dtype=complex)
+        self._full_to_multiband = [int(f * w["durations"][0]) for f in w["banded_frequency_points"]]
+        prior = self.priors["geocent_time"]
+        self.antenna_time = (prior.minimum + prior.maximum) / 2
 
     def calculate_snrs_mb(self, pols_unique, det, params):
-        """multiband.py:750-765."""
+        """multiband.py:750-765, 792-800."""
         w = self.w
         idx = w["unique_to_original_frequencies"]
         modes = {k: v[idx] for k, v in pols_unique.items()}
         at_points = types.SimpleNamespace(tensor=det.tensor, vertex=det.vertex,
                                           frequency_array=w["banded_frequency_points"],
                                           mask=np.ones(len(idx), dtype=bool))
-        strain = detector_response(at_points, modes, params, self.event.start_time)
-        d_inner_h = np.conj(np.dot(strain, w["linear_coeffs"][det.name]))
+        strain = detector_response(at_points, modes, params, self.event.start_time,
+                                   antenna_time=self.antenna_time)
+        lin = w["linear_coeffs"][det.name]
+        d_inner_h = np.conj(np.dot(strain, lin))
         h_inner_h = np.vdot(np.abs(strain) ** 2, w["quadratic_coeffs"][det.name]).real
-        return d_inner_h, h_inner_h
+        arr = None
+        if self.time_marginalization:
+            fm = self._full_to_multiband
+            self._full_d_h[fm] *= 0
+            for b in range(len(w["durations"])):
+                s, e = w["start_end_idxs"][b]
+                self._full_d_h[fm[s:e + 1]] += strain[s:e + 1] * lin[s:e + 1]
+            arr = np.fft.fft(self._full_d_h)
+        return d_inner_h, h_inner_h, arr
 
     def log_likelihood_ratio(self, parameters):
         params = copy.deepcopy(parameters)
         wa = dict(self.waveform_arguments)
         wa.setdefault("maximum_frequency", self.event.frequency_array[-1])
         pols = source_polarizations(self.source, self.w["waveform_frequencies"], params, wa)
-        d_inner_h, h_inner_h = 0j, 0.0
+        if self.time_marginalization and self.jitter_time:
+            params["geocent_time"] += params["time_jitter"]
+        d_inner_h, h_inner_h, arr = 0j, 0.0, None
         for det in self.event.detectors:
-            dh, hh = self.calculate_snrs_mb(pols, det, params)
+            dh, hh, a = self.calculate_snrs_mb(pols, det, params)
             d_inner_h += dh
             h_inner_h += hh
-        return float(np.real(self.compute_log_likelihood(d_inner_h, h_inner_h, None, params)))
+            if a is not None:
+                arr = a if arr is None else arr + a
+        return float(np.real(self.compute_log_likelihood(d_inner_h, h_inner_h, arr, params)))
 
 
 def quadratic_coefficients_loop(setup, ifos):

@@ -1,5 +1,6 @@
 """FP64 restatement of bilby's RelativeBinningGravitationalWaveTransient
-(bilby/gw/likelihood/relative.py), without time marginalisation.
+(bilby/gw/likelihood/relative.py), with time marginalisation through the
+rebuilt full waveform and its FFT (:376-391, 410-420).
 
 Test infrastructure only (see oracle/__init__.py).  Written bin by bin, as
 the reference loops, so that it stays independent of the vectorised setup in
@@ -17,16 +18,20 @@ from .likelihood import OracleGWTransient, detector_response, source_polarizatio
 class OracleRelativeBinning(OracleGWTransient):
     """relative.py:102-174 (constructor), :177-231 (setup_bins),
     :233-258 (set_fiducial_waveforms), :314-360 (compute_summary_data),
-    :362-374 (ratios), :393-428 (calculate_snrs)."""
+    :362-374 (ratios), :376-391 (full waveform), :393-428 (calculate_snrs)."""
 
     def __init__(self, event, source, fiducial_parameters, waveform_arguments=None,
                  distance_marginalization=False, phase_marginalization=False, priors=None,
-                 chi=1, epsilon=0.5, distance_table=None):
-        super().__init__(event, source, waveform_arguments, time_marginalization=False,
+                 chi=1, epsilon=0.5, distance_table=None, time_marginalization=False,
+                 jitter_time=True):
+        super().__init__(event, source, waveform_arguments,
+                         time_marginalization=time_marginalization,
                          distance_marginalization=distance_marginalization,
                          phase_marginalization=phase_marginalization, priors=priors,
-                         distance_table=distance_table)
+                         jitter_time=jitter_time, distance_table=distance_table)
         fid = dict(fiducial_parameters)
+        if time_marginalization:
+            fid["geocent_time"] = event.start_time
         if distance_marginalization:
             fid["luminosity_distance"] = self._ref_dist
         if phase_marginalization:
@@ -126,6 +131,22 @@ class OracleRelativeBinning(OracleGWTransient):
         h_inner_h = (b0 * abs(r0) ** 2 + 2 * b1 * (r0 * r1.conj()).real).sum()
         return d_inner_h, h_inner_h.real
 
+    def full_waveform(self, det, ratio):
+        """relative.py:376-391: the fiducial response times the ratio rebuilt
+        bin by bin as r0 + r1 (f - f_centre); bin b covers grid bins
+        [bin_inds[b], bin_inds[b + 1]), the last bin also bin_inds[-1]."""
+        r0 = (ratio[1:] + ratio[:-1]) / 2
+        r1 = (ratio[1:] - ratio[:-1]) / self.bin_widths
+        f = det.frequency_array
+        full_ratio = np.zeros(len(f), dtype=complex)
+        nb = self.number_of_bins
+        for b in range(nb):
+            lo = self.bin_inds[b]
+            hi = self.bin_inds[b + 1] + (1 if b == nb - 1 else 0)
+            centre = (self.bin_freqs[b] + self.bin_freqs[b + 1]) / 2
+            full_ratio[lo:hi] = r0[b] + r1[b] * (f[lo:hi] - centre)
+        return self.fiducial[det.name] * full_ratio
+
     def log_likelihood_ratio(self, parameters):
         """base.py:419-446 with relative.py:393-428 per detector."""
         params = copy.deepcopy(parameters)
@@ -133,10 +154,19 @@ class OracleRelativeBinning(OracleGWTransient):
         m1, m2 = params["mass_1"], params["mass_2"]
         if not (m1 > 0 and m2 > 0 and params["luminosity_distance"] > 0):
             return np.nan_to_num(-np.inf)
-        d_inner_h, h_inner_h = 0j, 0.0
+        if self.time_marginalization and self.jitter_time:
+            params["geocent_time"] += params["time_jitter"]
+        d_inner_h, h_inner_h, arr = 0j, 0.0, None
         for det in self.event.detectors:
             ratio = self.edge_response(det, params) / self.fiducial_points[det.name]
             dh, hh = self.snrs_from_ratio(det, ratio)
             d_inner_h += dh
             h_inner_h += hh
-        return float(np.real(self.compute_log_likelihood(d_inner_h, h_inner_h, None, params)))
+            if self.time_marginalization:
+                full = self.full_waveform(det, ratio)
+                with np.errstate(invalid="ignore", divide="ignore"):
+                    integrand = full[0:-1] * det.strain.conj()[0:-1] / det.psd[0:-1]
+                integrand = np.where(full[0:-1] != 0, integrand, 0)
+                a = 4 / self.event.duration * np.fft.fft(integrand)
+                arr = a if arr is None else arr + a
+        return float(np.real(self.compute_log_likelihood(d_inner_h, h_inner_h, arr, params)))

@@ -104,3 +104,94 @@ def test_quadratic_coefficients_equal_loop_form_and_weights_round_trip(tmp_path)
     for name in setup.linear_coeffs:
         assert np.array_equal(back.linear_coeffs[name], setup.linear_coeffs[name])
         assert np.array_equal(back.quadratic_coeffs[name], setup.quadratic_coeffs[name])
+
+
+class _U:
+    def __init__(self, a, b):
+        self.minimum, self.maximum = a, b
+
+    def prob(self, x):
+        x = np.asarray(x)
+        return ((x >= self.minimum) & (x <= self.maximum)) / (self.maximum - self.minimum)
+
+    def rescale(self, u):
+        return self.minimum + u * (self.maximum - self.minimum)
+
+
+def test_time_marginalised_array_matches_direct_transform():
+    """The multibanded d_inner_h_array (multiband.py:792-800: the h L terms
+    scattered to int(f T_0) and transformed with N = Nbs[-1] / 2) against a
+    direct DFT of the full-grid integrand 4/T h conj(d) / S at the same times
+    t_j = j T_0 / N (antenna patterns at the prior's middle, as
+    multiband.py:724-726 sets): within 1e-3 of the peak |X| at the window
+    bins around the peak, and the peak at the same bin."""
+    from oracle.likelihood import source_polarizations
+    ev, ifos, inj = _event()
+    setup = _setup(ifos, inj)
+    tc = inj["geocent_time"]
+    pri = {"geocent_time": _U(tc - 0.1, tc + 0.1)}
+    mb = OracleMultiband(ev, "taylorf2", setup, waveform_arguments=WF, time_marginalization=True,
+                         priors=dict(pri))
+    p = dict(inj, geocent_time=ev.start_time, time_jitter=0.0)
+    wa = dict(WF, maximum_frequency=ev.frequency_array[-1])
+    pols = source_polarizations("taylorf2", mb.w["waveform_frequencies"], p, wa)
+    pols_full = source_polarizations("taylorf2", ev.frequency_array, p, WF)
+    N = len(mb._times)
+    assert N == setup.Nbs[-1] // 2
+    j_peak = int(round((tc - ev.start_time) / mb._delta_tc))
+    js = np.arange(j_peak - 40, j_peak + 41)
+    x_mb = 0
+    x_direct = np.zeros(len(js), dtype=complex)
+    for det in ev.detectors:
+        _, _, arr = mb.calculate_snrs_mb(pols, det, p)
+        x_mb = x_mb + arr[js]
+        sig = detector_response(det, pols_full, p, ev.start_time, antenna_time=mb.antenna_time)
+        m = det.mask
+        integrand = 4 / T * sig[m] * np.conj(det.strain[m]) / det.psd[m]
+        f = det.frequency_array[m]
+        x_direct += np.exp(-2j * np.pi * np.outer(js * mb._delta_tc, f)) @ integrand
+    scale = np.max(np.abs(x_direct))
+    assert scale > 100
+    assert np.argmax(np.abs(x_mb)) == np.argmax(np.abs(x_direct))
+    assert np.max(np.abs(x_mb - x_direct)) < 1e-3 * scale, np.max(np.abs(x_mb - x_direct)) / scale
+
+
+def test_time_grid_index_follows_fancy_index_accumulation():
+    """The product's map from banded points to transform bins
+    (MBGravitationalWaveTransient._setup_time_marginalization_multiband,
+    used by the GEMM) reproduces the oracle's literal band-by-band
+    ``a[idx] += v`` (multiband.py:792-798) -- on the real setup and on one
+    with indices repeated inside a band (numpy keeps the last point's term
+    there) and across bands (terms add)."""
+    from bilby_amd.multiband import MBGravitationalWaveTransient
+    ev, ifos, inj = _event()
+    real = _setup(ifos, inj)
+    tc = inj["geocent_time"]
+    rng = np.random.default_rng(2)
+    fake_points = np.array([20.0, 20.0, 20.5, 21.0, 21.0, 21.0, 22.0, 20.5, 22.0, 23.0]) / 1.0
+    fake = types.SimpleNamespace(Nbs=np.array([64, 128]), durations=np.array([2.0, 1.0]),
+                                 banded_frequency_points=fake_points,
+                                 start_end_idxs=np.array([[0, 6], [7, 9]]), number_of_bands=2)
+    for setup in (real, fake):
+        obj = types.SimpleNamespace(setup=setup, priors={"geocent_time": _U(tc - 0.1, tc + 0.1)},
+                                    interferometers=types.SimpleNamespace(start_time=tc - 1.0),
+                                    _delta_tc=2 / FS)
+        if setup is fake:
+            obj.priors = {"geocent_time": _U(tc - 0.9, tc - 0.5)}
+        MBGravitationalWaveTransient._setup_time_marginalization_multiband(obj)
+        n = len(setup.banded_frequency_points)
+        y = rng.normal(size=n) + 1j * rng.normal(size=n)
+        N = int(setup.Nbs[-1]) // 2
+        literal = np.zeros(N, dtype=complex)
+        fm = [int(f * setup.durations[0]) for f in setup.banded_frequency_points]
+        for b in range(len(setup.durations)):
+            s, e = setup.start_end_idxs[b]
+            literal[fm[s:e + 1]] += y[s:e + 1]
+        idx = obj._time_grid_index
+        gemm_form = np.zeros(N, dtype=complex)
+        np.add.at(gemm_form, idx[idx >= 0], y[idx >= 0])
+        assert np.allclose(gemm_form, literal, rtol=0, atol=1e-14)
+        if setup is fake:
+            assert (idx < 0).sum() == 3   # 20.0, 21.0, 21.0 dropped in band 0
+        N_, j_first, n_window = obj._time_window
+        assert N_ == N and 0 <= j_first and j_first + n_window <= N
