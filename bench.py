@@ -428,6 +428,134 @@ def bench_multiband(steps, batch=4096, duration=128.0):
     return out
 
 
+def bench_binned_time(steps):
+    """Time + distance + phase marginalised binned likelihoods: relative
+    binning on the headline event (TaylorF2 H1+L1+V1, 8 s @ 4096 Hz,
+    epsilon 0.5 and 0.05, 65536 rows with time jitter; one GEMM of the
+    rows' edge ratios against the per-edge window basis per tile of 8192)
+    and multibanding on the 128 s @ 2048 Hz light binary (4096 rows; GEMM
+    against the banded-point exponentials); the full T+D+P likelihood on
+    the same events timed beside them; a few rows against the FP64 oracle."""
+    import numpy as np
+    import torch
+    from bilby_amd import InterferometerList, WaveformGenerator, taylorf2
+    from bilby_amd.configs import make_taylorf2_marginalised
+    from bilby_amd.likelihood import GravitationalWaveTransient, inject_signal
+    from bilby_amd.multiband import MBGravitationalWaveTransient
+    from bilby_amd.prior import PriorDict, Uniform, UniformComovingVolume
+    from bilby_amd.relative import RelativeBinningGravitationalWaveTransient
+    from oracle.likelihood import OracleDetector, OracleEvent
+    from oracle.multiband import OracleMultiband
+    from oracle.relative import OracleRelativeBinning
+    from tests.helpers import draws_as_dicts
+
+    def oracle_event(ifos):
+        dets = [OracleDetector(i.name, i.frequency_array, i.frequency_domain_strain,
+                               i.power_spectral_density_array, i.frequency_mask) for i in ifos]
+        return OracleEvent(dets, ifos.duration, ifos.sampling_frequency, ifos.start_time)
+
+    def draws_near(inj, n, rng, ifos, lk, mass_sd):
+        d = {k: np.full(n, float(v)) for k, v in inj.items()}
+        d["mass_1"] = d["mass_1"] * (1 + rng.normal(0, mass_sd, n))
+        d["mass_2"] = d["mass_2"] * (1 + rng.normal(0, mass_sd, n))
+        for k, sd in (("ra", 0.05), ("dec", 0.05), ("psi", 0.2), ("theta_jn", 0.05)):
+            d[k] = d[k] + rng.normal(0, sd, n)
+        d["geocent_time"] = np.full(n, float(ifos.start_time))
+        d["time_jitter"] = rng.uniform(-1, 1, n) / ifos.sampling_frequency
+        d["luminosity_distance"] = np.full(n, lk._ref_dist)
+        d["phase"] = np.zeros(n)
+        return d
+
+    out = {}
+    s = make_taylorf2_marginalised(n_draws=8, seed=4321)
+    ifos, wfg = s["interferometers"], s["waveform_generator"]
+    inj = dict(s["injection"])
+    n = 65536
+    full = GravitationalWaveTransient(ifos, wfg, time_marginalization=True,
+                                      distance_marginalization=True, phase_marginalization=True,
+                                      priors=s["priors"].copy(), precision="fp32")
+    for eps in (0.5, 0.05):
+        lk = RelativeBinningGravitationalWaveTransient(
+            ifos, wfg, fiducial_parameters=inj, time_marginalization=True,
+            distance_marginalization=True, phase_marginalization=True,
+            priors=s["priors"].copy(), epsilon=eps, precision="fp32")
+        d = draws_near(inj, n, np.random.default_rng(78), ifos, lk, 0.003)
+        rows = torch.from_numpy(lk.rows_from_parameters(d)).cuda()
+        key = "relbin_eps%g" % eps
+        out[key + "_bins"] = lk.number_of_bins
+        for prec in ("fp32", "fp64"):
+            dt = timed(lambda: lk.log_likelihood_ratio_batch(rows, precision=prec), steps)
+            out["%s_%s_evals_per_s" % (key, prec)] = n / dt
+        r32 = lk.log_likelihood_ratio_batch(rows, precision="fp32").cpu().numpy()
+        r64 = lk.log_likelihood_ratio_batch(rows, precision="fp64").cpu().numpy()
+        out[key + "_fp32_vs_fp64_max"] = float(np.max(np.abs(r32 - r64)))
+        olk = OracleRelativeBinning(oracle_event(ifos), "taylorf2", inj,
+                                    waveform_arguments=dict(wfg.waveform_arguments),
+                                    time_marginalization=True, distance_marginalization=True,
+                                    phase_marginalization=True, priors=dict(s["priors"]),
+                                    epsilon=eps, distance_table=lk._dist_margd_loglikelihood_array)
+        ref = np.array([olk.log_likelihood_ratio(p) for p in draws_as_dicts(d, range(8))])
+        out[key + "_fp64_vs_oracle_8rows"] = float(np.max(np.abs(r64[:8] - ref)))
+        out[key + "_fp32_vs_oracle_8rows"] = float(np.max(np.abs(r32[:8] - ref)))
+        del lk
+    rows = torch.from_numpy(full.rows_from_parameters(d)).cuda()
+    dt = timed(lambda: full.log_likelihood_ratio_batch(rows), steps)
+    out["relbin_event_full_tdp_fp32_evals_per_s"] = n / dt
+    fr = full.log_likelihood_ratio_batch(rows).cpu().numpy()
+    out["relbin_eps0.05_vs_full_tdp_median_abs"] = float(np.median(np.abs(r32 - fr)))
+
+    # multibanding, 128 s @ 2048 Hz
+    duration, fs, batch = 128.0, 2048.0, 4096
+    tc = 1187008882.0
+    start = tc - duration + 2.0
+    ifos = InterferometerList(["H1", "L1", "V1"])
+    ifos.set_strain_data_from_power_spectral_densities(fs, duration, start,
+                                                       random_state=np.random.default_rng(11))
+    wfg = WaveformGenerator(duration=duration, sampling_frequency=fs,
+                            frequency_domain_source_model=taylorf2,
+                            waveform_arguments=dict(reference_frequency=20.0,
+                                                    minimum_frequency=20.0))
+    inj = dict(mass_1=1.5, mass_2=1.3, chi_1=0.0, chi_2=0.0, luminosity_distance=100.0,
+               theta_jn=0.4, phase=1.3, ra=1.3, dec=-1.2, psi=0.659, geocent_time=tc)
+    inject_signal(ifos, wfg, inj)
+    priors = PriorDict(geocent_time=Uniform(tc - 0.1, tc + 0.1),
+                       luminosity_distance=UniformComovingVolume(10.0, 300.0),
+                       phase=Uniform(0.0, 2 * np.pi, boundary="periodic"))
+    mc = (inj["mass_1"] * inj["mass_2"]) ** 0.6 / (inj["mass_1"] + inj["mass_2"]) ** 0.2
+    t0 = time.perf_counter()
+    lk = MBGravitationalWaveTransient(ifos, wfg, reference_chirp_mass=mc, priors=priors.copy(),
+                                      time_marginalization=True, distance_marginalization=True,
+                                      phase_marginalization=True, precision="fp32")
+    out["mb_128s_setup_s"] = time.perf_counter() - t0
+    d = draws_near(inj, batch, np.random.default_rng(13), ifos, lk, 2e-6)
+    rows = torch.from_numpy(lk.rows_from_parameters(d)).cuda()
+    N, j_first, n_window = lk._time_window
+    out.update({"mb_128s_rows": batch, "mb_128s_banded_points": int(len(lk.setup.banded_frequency_points)),
+                "mb_128s_transform_length": N, "mb_128s_window_bins": n_window})
+    for prec in ("fp32", "fp64"):
+        dt = timed(lambda: lk.log_likelihood_ratio_batch(rows, precision=prec), steps)
+        out["mb_128s_%s_evals_per_s" % prec] = batch / dt
+    m32 = lk.log_likelihood_ratio_batch(rows, precision="fp32").cpu().numpy()
+    m64 = lk.log_likelihood_ratio_batch(rows, precision="fp64").cpu().numpy()
+    out["mb_128s_fp32_vs_fp64_max"] = float(np.max(np.abs(m32 - m64)))
+    olk = OracleMultiband(oracle_event(ifos), "taylorf2", lk.setup,
+                          waveform_arguments=dict(wfg.waveform_arguments),
+                          time_marginalization=True, distance_marginalization=True,
+                          phase_marginalization=True, priors=dict(priors),
+                          distance_table=lk._dist_margd_loglikelihood_array)
+    ref = np.array([olk.log_likelihood_ratio(p) for p in draws_as_dicts(d, range(4))])
+    out["mb_128s_fp64_vs_oracle_4rows"] = float(np.max(np.abs(m64[:4] - ref)))
+    out["mb_128s_fp32_vs_oracle_4rows"] = float(np.max(np.abs(m32[:4] - ref)))
+    del lk
+    full = GravitationalWaveTransient(ifos, wfg, time_marginalization=True,
+                                      distance_marginalization=True, phase_marginalization=True,
+                                      priors=priors.copy(), precision="fp32")
+    rows = torch.from_numpy(full.rows_from_parameters(d)).cuda()
+    dt = timed(lambda: full.log_likelihood_ratio_batch(rows), steps)
+    out["mb_128s_full_tdp_fp32_evals_per_s"] = batch / dt
+    return out
+
+
 def bench_roq_weights(n_basis=400, duration=128.0, fs=4096.0):
     """Building ROQ linear weights (roq.py:849-918) for a 400-element basis
     on a 128 s, 4096 Hz segment, two detectors: the GPU GEMM form against
@@ -848,6 +976,7 @@ def main():
                          ("relative_binning_dp_event_65536", lambda: bench_relbin(5)),
                          ("calibration_1000_curves_dp_event", lambda: bench_calibration(3)),
                          ("multiband_128s_light_binary", lambda: bench_multiband(5)),
+                         ("binned_time_marginalised_tdp", lambda: bench_binned_time(5)),
                          ("roq_weights_400_basis_128s", lambda: bench_roq_weights()),
                          ("serving_32_samplers_2000_rps", lambda: bench_serving(
                              lk, rows.cpu().numpy(), args.serve_seconds)),

@@ -498,7 +498,7 @@ int eval_time_tiles(gwl_event* ev, const WinBasis& win, void** ws, size_t* ws_by
   a.rows = reinterpret_cast<RelRow*>(w);
   a.sums = reinterpret_cast<double*>(w + ((size_t)tile * sizeof(RelRow) + 255) / 256 * 256);
   void* opa = win_a(win, *tws);
-  double* c = win_c(win, *tws, tile);
+  void* c = win_c(win, *tws, tile);
   const size_t lds = (ev->dev.flags & GWL_DISTANCE_MARG)
                          ? (size_t)(ev->dev.sp.nx + ev->dev.sp.ny + 8) * 8 : 0;
   for (long long r0 = 0; r0 < n; r0 += tile) {

@@ -10,11 +10,11 @@
 //   Bt[Np][Kp]: row jj = [Re M | -Im M], row L + jj = [Im M | Re M]
 // (columns k < K and K + k, zero padding), with the rows' operand
 //   A[Mp][Kp] = [Re a | Im a | 0].
-// float32 operands use an MFMA kernel that moves its float32 accumulators
-// into float64 every 256 k, so that sums over thousands of terms keep
-// float32 rounding of their partial sums only; float64 operands use the
-// float64 GEMM.  The epilogue is the time-marginalised lnL of
-// base.py:794-820 over the window.
+// float32 operands use an MFMA kernel split over chunks of kWinKc operand
+// columns whose float32 partial products the epilogue adds in float64, so
+// that sums over thousands of terms keep float32 rounding of short partial
+// sums only; float64 operands use the float64 GEMM.  The epilogue is the
+// time-marginalised lnL of base.py:794-820 over the window.
 #pragma once
 #include "gwl_internal.h"
 
@@ -39,8 +39,10 @@ struct WinBasis {
   float* bt32 = nullptr;
 };
 
-// Rows per GEMM tile of one evaluation call (bounds the workspace).
+// Rows per GEMM tile of one evaluation call (bounds the workspace), and
+// operand columns per split-K chunk of the float32 GEMM.
 constexpr long long kWinTileRows = 8192;
+constexpr int kWinKc = 1024;
 
 void win_basis_free(WinBasis* wb);
 
@@ -58,11 +60,12 @@ int win_basis_relbin(gwl_event* ev, int nd, int nb, const long long* edges, cons
 // for n_p < 0; ``idx`` (device) [n].
 int win_basis_points(gwl_event* ev, int n, const long long* idx, long long N, WinBasis* wb);
 
-// Device workspace of one tile: A [Mp][Kp] (double-sized) then C [Mp][Np]
-// double.
+// Device workspace of one tile: A [Mp][Kp] (double-sized) then C: [Mp][Np]
+// double (float64 GEMM) or [chunks][Mp][Np] float partials.
 size_t win_ws_bytes(const WinBasis& wb, long long rows);
 void* win_a(const WinBasis& wb, void* ws);
-double* win_c(const WinBasis& wb, void* ws, long long rows);
+void* win_c(const WinBasis& wb, void* ws, long long rows);
+int win_chunks(const WinBasis& wb);
 long long win_mp(long long rows);
 
 // Zero the operand padding: columns [2K, Kp) of rows < n and every column
@@ -72,7 +75,7 @@ int win_clear(const WinBasis& wb, void* a, bool fp64, long long n, hipStream_t s
 // GEMM of the operand tile against the basis, then the window epilogue:
 // out[r] for r < n from C, the row's <h|h> (hh[3 r + 2]), validity
 // (valid[r * valid_stride]), luminosity distance and time jitter (params).
-int win_finish(gwl_event* ev, const WinBasis& wb, const void* a, bool fp64, double* c,
+int win_finish(gwl_event* ev, const WinBasis& wb, const void* a, bool fp64, void* c,
                long long n, const double* params, int ld, const double* hh, const int* valid,
                int valid_stride, double* out, hipStream_t st);
 

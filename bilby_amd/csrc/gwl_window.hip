@@ -124,23 +124,29 @@ __global__ __launch_bounds__(256) void k_win_clear(void* a, int fp64, long long 
 }
 
 // ------------------------------------------------------------------ GEMM
-// C[M][N] (double) = A[M][K] . Bt[N][K]^T, float32 operands; M % 128 == 0,
-// N % 128 == 0, K % 32 == 0.  A workgroup of 4 waves computes a 128 x 128
-// tile, each wave 2 x 2 MFMA 32 x 32 blocks (output layout: lane l,
-// accumulator v -> row 8 (v / 4) + 4 (l / 32) + v % 4, column l % 32); the
-// float32 accumulators are added into float64 ones every kFlush k-tiles.
+// Split-K partials: C_z[M][N] = A[:, k in chunk z] . Bt[:, k in chunk z]^T
+// (float32), chunk z = blockIdx.z of kWinKc columns; M % 128 == 0,
+// N % 128 == 0, K % 32 == 0, row strides K.  Inside a chunk the MFMA
+// accumulators restart every kFlush k-tiles (256 columns, 128 MFMA steps)
+// after adding into a float32 chunk total; the epilogue adds the chunk
+// totals in float64.  The chunks also multiply the workgroups of a short
+// row tile.
+// A workgroup of 4 waves computes a 128 x 128 tile, each wave 2 x 2 MFMA
+// 32 x 32 blocks (output layout: lane l, accumulator v -> row
+// 8 (v / 4) + 4 (l / 32) + v % 4, column l % 32).
 constexpr int kBM = 128, kBN = 128, kBK = 32, kFlush = 8;
 
 __global__ __launch_bounds__(256) void k_win_gemm_f32(const float* __restrict__ A,
                                                       const float* __restrict__ Bt,
-                                                      double* __restrict__ C, int M, int N, int K) {
+                                                      float* __restrict__ C, int M, int N, int K) {
   __shared__ float As[kBM][kBK + 1];
   __shared__ float Bs[kBN][kBK + 1];
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int m0 = blockIdx.y * kBM, n0 = blockIdx.x * kBN;
   const int wr = (w >> 1) * 64, wc = (w & 1) * 64;
+  const int kb = blockIdx.z * kWinKc, ke = min(K, kb + kWinKc);
   f32x16 acc[2][2];
-  double acc64[2][2][16];
+  float tot[2][2][16];
 #pragma unroll
   for (int i = 0; i < 2; ++i)
 #pragma unroll
@@ -148,10 +154,10 @@ __global__ __launch_bounds__(256) void k_win_gemm_f32(const float* __restrict__ 
 #pragma unroll
       for (int v = 0; v < 16; ++v) {
         acc[i][j][v] = 0.f;
-        acc64[i][j][v] = 0.0;
+        tot[i][j][v] = 0.f;
       }
   int tiles = 0;
-  for (int k0 = 0; k0 < K; k0 += kBK) {
+  for (int k0 = kb; k0 < ke; k0 += kBK) {
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
       const int idx = tid + 256 * q;
@@ -176,7 +182,7 @@ __global__ __launch_bounds__(256) void k_win_gemm_f32(const float* __restrict__ 
           acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x2f32(a[i], b[j], acc[i][j], 0, 0, 0);
     }
     __syncthreads();
-    if (++tiles == kFlush || k0 + kBK >= K) {
+    if (++tiles == kFlush || k0 + kBK >= ke) {
       tiles = 0;
 #pragma unroll
       for (int i = 0; i < 2; ++i)
@@ -184,11 +190,12 @@ __global__ __launch_bounds__(256) void k_win_gemm_f32(const float* __restrict__ 
         for (int j = 0; j < 2; ++j)
 #pragma unroll
           for (int v = 0; v < 16; ++v) {
-            acc64[i][j][v] += (double)acc[i][j][v];
+            tot[i][j][v] += acc[i][j][v];
             acc[i][j][v] = 0.f;
           }
     }
   }
+  float* Cz = C + (size_t)blockIdx.z * M * N;
 #pragma unroll
   for (int i = 0; i < 2; ++i)
 #pragma unroll
@@ -197,14 +204,17 @@ __global__ __launch_bounds__(256) void k_win_gemm_f32(const float* __restrict__ 
       for (int v = 0; v < 16; ++v) {
         const int row = m0 + wr + i * 32 + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
         const int col = n0 + wc + j * 32 + (lane & 31);
-        C[(size_t)row * N + col] = acc64[i][j][v];
+        Cz[(size_t)row * N + col] = tot[i][j][v];
       }
 }
 
 // ------------------------------------------------------------------ epilogue
 struct WinEpiArgs {
   EvDev ev;
-  const double* C;
+  const double* C64;  // [Mp][Np] (float64 GEMM), or null
+  const float* C32;   // [nchunk][Mp][Np] split-K partials (float32 GEMM)
+  int nchunk;
+  long long Mp;
   int Np, L;
   long long n;
   const double* params;
@@ -244,11 +254,22 @@ __global__ __launch_bounds__(64 * kWinEpiRows) void k_win_epi(WinEpiArgs a) {
   SplineY yy;
   if (dist) yy = spline_y(ev, knots + ev.sp.nx + 4, hh * hs * hs);
   double m = -INFINITY, s = 0.0;
-  const double* crow = a.C + (size_t)r * a.Np;
   for (int jj = lane; jj < a.L; jj += 64) {
     const double t = (a.start + (double)(a.j_first + jj + a.offset) * a.t_step) + jit;
     if (!(t >= ev.tmin && t <= ev.tmax)) continue;
-    lse_push(m, s, epi_value(ev, knots, yy, hh, hs, crow[jj], crow[a.L + jj]));
+    double xr = 0.0, xi = 0.0;
+    if (a.C64) {
+      const double* crow = a.C64 + (size_t)r * a.Np;
+      xr = crow[jj];
+      xi = crow[a.L + jj];
+    } else {
+      for (int c = 0; c < a.nchunk; ++c) {
+        const float* crow = a.C32 + ((size_t)c * a.Mp + r) * a.Np;
+        xr += (double)crow[jj];
+        xi += (double)crow[a.L + jj];
+      }
+    }
+    lse_push(m, s, epi_value(ev, knots, yy, hh, hs, xr, xi));
   }
   lse_wave(m, s);
   if (lane == 0) {
@@ -339,16 +360,19 @@ int win_basis_points(gwl_event* ev, int n, const long long* idx, long long N, Wi
 
 long long win_mp(long long rows) { return up(std::max(1LL, rows), 128); }
 
+int win_chunks(const WinBasis& wb) { return (wb.Kp + kWinKc - 1) / kWinKc; }
+
 size_t win_ws_bytes(const WinBasis& wb, long long rows) {
   const long long Mp = win_mp(std::min(rows, kWinTileRows));
-  return al256((size_t)Mp * wb.Kp * 8) + al256((size_t)Mp * wb.Np * 8);
+  const size_t c = std::max<size_t>((size_t)Mp * wb.Np * 8, (size_t)win_chunks(wb) * Mp * wb.Np * 4);
+  return al256((size_t)Mp * wb.Kp * 8) + al256(c);
 }
 
 void* win_a(const WinBasis&, void* ws) { return ws; }
 
-double* win_c(const WinBasis& wb, void* ws, long long rows) {
+void* win_c(const WinBasis& wb, void* ws, long long rows) {
   const long long Mp = win_mp(std::min(rows, kWinTileRows));
-  return reinterpret_cast<double*>(static_cast<char*>(ws) + al256((size_t)Mp * wb.Kp * 8));
+  return static_cast<char*>(ws) + al256((size_t)Mp * wb.Kp * 8);
 }
 
 int win_clear(const WinBasis& wb, void* a, bool fp64, long long n, hipStream_t st) {
@@ -362,23 +386,28 @@ int win_clear(const WinBasis& wb, void* a, bool fp64, long long n, hipStream_t s
   return GWL_OK;
 }
 
-int win_finish(gwl_event* ev, const WinBasis& wb, const void* a, bool fp64, double* c,
+int win_finish(gwl_event* ev, const WinBasis& wb, const void* a, bool fp64, void* c,
                long long n, const double* params, int ld, const double* hh, const int* valid,
                int valid_stride, double* out, hipStream_t st) {
   const long long Mp = win_mp(n);
+  const int nchunk = win_chunks(wb);
   if (fp64) {
     if (int rc = gwl_gemm(ev->ctx, GWL_ROQ_FP64, a, nullptr, wb.bt64, nullptr, c, (int32_t)Mp,
                           wb.Np, wb.Kp, st))
       return rc;
   } else {
-    hipLaunchKernelGGL(k_win_gemm_f32, dim3((unsigned)(wb.Np / kBN), (unsigned)(Mp / kBM)),
-                       dim3(256), 0, st, static_cast<const float*>(a), wb.bt32, c, (int)Mp, wb.Np,
-                       wb.Kp);
+    hipLaunchKernelGGL(k_win_gemm_f32,
+                       dim3((unsigned)(wb.Np / kBN), (unsigned)(Mp / kBM), (unsigned)nchunk),
+                       dim3(256), 0, st, static_cast<const float*>(a), wb.bt32,
+                       static_cast<float*>(c), (int)Mp, wb.Np, wb.Kp);
     GWL_HIP_CHECK(hipGetLastError());
   }
   WinEpiArgs e;
   e.ev = ev->dev;
-  e.C = c;
+  e.C64 = fp64 ? static_cast<const double*>(c) : nullptr;
+  e.C32 = fp64 ? nullptr : static_cast<const float*>(c);
+  e.nchunk = nchunk;
+  e.Mp = Mp;
   e.Np = wb.Np;
   e.L = wb.L;
   e.n = n;
