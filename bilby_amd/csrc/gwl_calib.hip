@@ -1,6 +1,9 @@
 // Calibration-marginalised likelihood (bilby/gw/likelihood/base.py:332-346
 // calculate_snrs with calibration_marginalization, :860-877
-// calibration_marginalized_likelihood), without time marginalisation.
+// calibration_marginalized_likelihood).  With time marginalisation
+// (base.py:305-324) every curve needs its own transform instead; that path
+// (k_cal_time_w in gwl_time_wave.hip) uses the curves in the transform's
+// decimated order and per-curve prefix sums for <h|h>, built here.
 //
 // For C response curves cal_c(f) on each detector's masked bins (the
 // calibration lookup, base.py:1037-1051), per row
@@ -36,6 +39,12 @@ size_t up(size_t x, size_t a) { return (x + a - 1) / a * a; }
 
 }  // namespace
 
+struct gwl_calib;
+
+namespace gwl {
+int calib_create_time(gwl_event* ev, int nc, const double* curves, int K, gwl_calib** out);
+}
+
 struct gwl_calib {
   gwl_event* ev = nullptr;
   int nc = 0;           // curves
@@ -55,6 +64,11 @@ struct gwl_calib {
   void* ws = nullptr;
   size_t ws_bytes = 0;
   int64_t ws_rows = 0;
+  // time marginalisation: curves [nc][nd][N] float2 in decimated order and
+  // prefix sums [nc][nd][N + 2] of f^-7/3 sigma^2 / S |cal|^2 (natural order)
+  bool timem = false;
+  float2* cal_t = nullptr;
+  double* calps = nullptr;
 };
 
 namespace gwl {
@@ -239,6 +253,7 @@ __global__ __launch_bounds__(256) void k_cal_epi(CalArgs a) {
 }
 
 size_t cal_ws_bytes(const gwl_calib* c, long long rows) {
+  if (c->timem) return cal_time_ws_bytes(rows, c->nc);
   const size_t Mp = up((size_t)rows, kGM);
   size_t b = 0;
   b += up(Mp * c->K1p * 8, 256) + up(Mp * c->K1p * 4, 256);  // A1 (largest element) + bf16 lo
@@ -265,6 +280,80 @@ int cal_reserve(gwl_calib* c, long long rows) {
 }
 
 }  // namespace
+
+// Time-marginalised calibration tables (see gwl_calib): the curve values of
+// each detector's masked bins n < N at decimated element (n % Q) P + n / Q
+// (1 elsewhere; the integrand is zero there), and the prefix sums over bins
+// 0..N of w_d(n) |cal(n)|^2 with w_d(n) = f^-7/3 sigma^2 / S_d the event's
+// own prefix-sum increments (|cal|^2 = 1 outside the mask, where w_d = 0).
+int calib_create_time(gwl_event* ev, int nc, const double* curves, int K, gwl_calib** out) {
+  const EvDev& d = ev->dev;
+  if (!cal_time_eligible(ev)) {
+    set_error("gwl_calib_create: calibration with time marginalisation needs a TaylorF2 event on "
+              "the wave-per-row transform with a time window of at most 512 bins");
+    return GWL_E_UNSUPPORTED;
+  }
+  const int nd = d.nd, N = d.N;
+  GWL_HIP_CHECK(hipSetDevice(ev->ctx->device));
+  std::vector<double> hps((size_t)nd * (N + 2));
+  GWL_HIP_CHECK(hipMemcpy(hps.data(), d.hps, hps.size() * 8, hipMemcpyDeviceToHost));
+  std::vector<float2> tab;
+  std::vector<double> ps;
+  try {
+    tab.assign((size_t)nc * nd * N, make_float2(1.f, 0.f));
+    ps.assign((size_t)nc * nd * (N + 2), 0.0);
+  } catch (...) {
+    set_error("out of host memory for the calibration tables");
+    return GWL_E_NOMEM;
+  }
+  for (int cv = 0; cv < nc; ++cv) {
+    int k0 = 0;
+    for (int k = 0; k < nd; ++k) {
+      const int lo = d.mlo[k], hi = d.mhi[k];
+      float2* t = tab.data() + ((size_t)cv * nd + k) * N;
+      double* p = ps.data() + ((size_t)cv * nd + k) * (N + 2);
+      const double* h = hps.data() + (size_t)k * (N + 2);
+      double acc = 0.0;
+      p[0] = 0.0;
+      for (int n = 0; n <= N; ++n) {
+        double c2 = 1.0;
+        if (n >= lo && n <= hi) {
+          const double* v = curves + ((size_t)cv * K + k0 + (n - lo)) * 2;
+          c2 = v[0] * v[0] + v[1] * v[1];
+          if (n < N) t[(size_t)(n % d.Q) * d.P + n / d.Q] = make_float2((float)v[0], (float)v[1]);
+        }
+        acc += (h[n + 1] - h[n]) * c2;
+        p[n + 1] = acc;
+      }
+      k0 += hi - lo + 1;
+    }
+  }
+  gwl_calib* c = new (std::nothrow) gwl_calib;
+  if (!c) { set_error("out of host memory"); return GWL_E_NOMEM; }
+  c->ev = ev;
+  c->nc = nc;
+  c->K = K;
+  c->timem = true;
+  const size_t bt = up(tab.size() * sizeof(float2), 256), bp = up(ps.size() * 8, 256);
+  c->data_bytes = bt + bp;
+  if (hipMalloc(&c->data, c->data_bytes) != hipSuccess) {
+    delete c;
+    set_error("hipMalloc failed for the calibration curves");
+    return GWL_E_NOMEM;
+  }
+  c->cal_t = static_cast<float2*>(c->data);
+  c->calps = reinterpret_cast<double*>(static_cast<char*>(c->data) + bt);
+  hipError_t err = hipMemcpy(c->cal_t, tab.data(), tab.size() * sizeof(float2), hipMemcpyHostToDevice);
+  if (err == hipSuccess) err = hipMemcpy(c->calps, ps.data(), ps.size() * 8, hipMemcpyHostToDevice);
+  if (err != hipSuccess) {
+    (void)hipFree(c->data);
+    delete c;
+    return hip_fail(err, "gwl_calib_create upload");
+  }
+  *out = c;
+  return GWL_OK;
+}
+
 }  // namespace gwl
 
 extern "C" {
@@ -275,12 +364,9 @@ int gwl_calib_create(gwl_event* ev, int32_t n_curves, const double* curves, gwl_
     return GWL_E_INVALID;
   }
   const gwl::EvDev& d = ev->dev;
-  if (d.flags & GWL_TIME_MARG) {
-    set_error("gwl_calib_create: calibration with time marginalisation is not supported");
-    return GWL_E_UNSUPPORTED;
-  }
   int K = 0;
   for (int k = 0; k < d.nd; ++k) K += d.mhi[k] - d.mlo[k] + 1;
+  if (d.flags & GWL_TIME_MARG) return gwl::calib_create_time(ev, n_curves, curves, K, out);
   gwl_calib* c = new (std::nothrow) gwl_calib;
   if (!c) { set_error("out of host memory"); return GWL_E_NOMEM; }
   c->ev = ev;
@@ -387,9 +473,22 @@ int gwl_calib_eval(gwl_calib* c, const double* params, int64_t n, int32_t ld, in
     set_error("gwl_calib_eval: too many rows in one call");
     return GWL_E_INVALID;
   }
+  if (c->timem && mode != gwl::CAL_FP32) {
+    set_error("gwl_calib_eval: with time marginalisation the curves' transforms run in float32 "
+              "(mode GWL_ROQ_FP32)");
+    return GWL_E_UNSUPPORTED;
+  }
   if (int rc = gwl::cal_reserve(c, n)) return rc;
   GWL_HIP_CHECK(hipSetDevice(ev->ctx->device));
   (void)hipGetLastError();
+  if (c->timem) {
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    if (int rc = gwl::timing_begin(ev, st)) return rc;
+    if (int rc = gwl::launch_cal_time(ev, params, n, ld, c->nc, c->cal_t, c->calps, c->ws,
+                                      per_curve, out, st))
+      return rc;
+    return gwl::timing_end(ev, st);
+  }
   const int Mp = (int)up((size_t)n, kGM);
   gwl::CalArgs a;
   a.ev = ev->dev;

@@ -658,6 +658,7 @@ struct RowW {
   float cim[GWL_MAX_DET];
   double hh;                   // sum over the support of |h|^2 sigma^2 / S
   double hscale, jitter;
+  double cabs2[GWL_MAX_DET];   // |complex detector amplitude / sigma|^2, float64
   int kb, ke, valid, pad;
 };
 

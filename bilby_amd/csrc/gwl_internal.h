@@ -120,6 +120,12 @@ int init_time_wave_tables(const EvDev& ev, double* tf2b, float2* dwa32);
 int launch_time_wave(gwl_event* ev, const double* params, int64_t n, int ld, double* out,
                      hipStream_t stream);
 size_t roww_bytes(int64_t rows);
+// Calibration with time marginalisation (gwl_time_wave.hip)
+bool cal_time_eligible(const gwl_event* ev);
+size_t cal_time_ws_bytes(int64_t rows, int nc);
+int launch_cal_time(gwl_event* ev, const double* params, int64_t n, int ld, int nc,
+                    const float2* cal, const double* calps, void* ws, double* per_curve,
+                    double* out, hipStream_t stream);
 // chirp-z TaylorF2 time transform (gwl_time_chirp.hip)
 bool time_chirp_eligible(const gwl_event* ev, long long n);
 void chirp_spectrum(int N, int L, int M, float2* out);

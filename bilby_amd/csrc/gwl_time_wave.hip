@@ -78,6 +78,7 @@ __device__ inline void rowconst_one(const EvDev& ev, const double* row, RowW* ou
     w.dt[d] = on ? sc.dt[d] : 0.0;
     w.cre[d] = on ? (float)sc.cre[d] : 0.f;
     w.cim[d] = on ? (float)sc.cim[d] : 0.f;
+    w.cabs2[d] = on ? sc.cabs2[d] : 0.0;
     if (on && sc.kb <= sc.ke) {
       const double* ps = ev.hps + (size_t)d * (ev.N + 2);
       hh += sc.cabs2[d] * (ps[sc.ke + 1] - ps[sc.kb]);
@@ -326,6 +327,225 @@ __global__ __launch_bounds__(64 * kWaves, 4) void k_time_w(TimeWArgs a) {
   }
 }
 
+// ------------------------------------------------------------------ calibration
+// Calibration and time marginalisation together (base.py:305-324, 862-877):
+// for every response curve c the transform at the window of
+//   sum_d cal_{c,d}(n) h_d(n) conj(d_d(n)) / S_d(n)
+// and <h|h>_c = sum_d |C_d|^2 sum_n f^-7/3 sigma^2 / S_d |cal_{c,d}|^2 over
+// the support (per-curve prefix sums), the time-marginalised lnL per curve,
+// then the logsumexp over curves minus ln C.  One wavefront per (row, block
+// of kCalCurves curves): the per-detector integrand of a sub-sequence is
+// computed once and combined with each curve of the block before that
+// curve's transform; the grid runs the curve blocks slowest, so a block's
+// curve values stay in L2 while many rows use them.
+constexpr int kCalCurves = 2;
+
+// Per-detector h conj(d) / S at decimated element i (bin nbin), zero for
+// detectors past nd (the float64 phase and float32 products of
+// integrand_tf2, kept apart per detector).
+template <int ND>
+__device__ __forceinline__ void integrand_tf2_det(const WaveEv& ev, const RowW& rw, int i,
+                                                  int nbin, float2* out) {
+  const double f = (double)nbin * ev.df;
+  const double* tb = ev.tf2b + 3 * (size_t)i;
+  const double u = tb[0], v = tb[1], lf = tb[2];
+  const double v2 = v * v, v3 = v2 * v;
+  double s = rw.c[4];
+  s = fma(rw.c[0], v3 * v2, s);
+  s = fma(rw.c[1], v3, s);
+  s = fma(rw.c[2], v2, s);
+  s = fma(rw.c[3], v, s);
+  s = fma(rw.c[5], u, s);
+  s = fma(rw.c[6], u * u, s);
+  s = fma(rw.c[7], lf, s);
+  s = fma(rw.c[8], u * lf, s);
+#pragma unroll
+  for (int d = 0; d < ND; ++d) {
+    const double cyc = fma(f, rw.dt[d], s);
+    const float2 e = expm2pi((float)(cyc - floor(cyc)));
+    const float2 dw = ev.dwa32[(size_t)d * ev.N + i];
+    const float hr = rw.cre[d] * e.x - rw.cim[d] * e.y;
+    const float hi = rw.cre[d] * e.y + rw.cim[d] * e.x;
+    out[d] = make_float2(hr * dw.x + hi * dw.y, hi * dw.x - hr * dw.y);
+  }
+}
+
+// The three radix-8 passes of one 512-point sub-sequence transform (inputs
+// x[s] = element l + 64 s of lane l) and the window accumulation of
+// sub-sequence n2 into accr / acci (bins j = l + 64 s mod 512), as in
+// wave_row.
+__device__ __forceinline__ void wave_transform_accumulate(const WaveEv& ev, float2* x, int n2,
+                                                          const float2* tw, const float2* tw3,
+                                                          float2* xb, int lane, double* accr,
+                                                          double* acci) {
+  const int k2 = 8 * (lane & 7);
+  const int base2 = 64 * (lane >> 3) + (lane & 7);
+  dft8(x);
+#pragma unroll
+  for (int s = 0; s < 8; ++s) xb[xpad(8 * lane + s)] = x[s];
+  wave_sync();
+#pragma unroll
+  for (int s = 0; s < 8; ++s) x[s] = xb[xpad(lane + 64 * s)];
+#pragma unroll
+  for (int s = 1; s < 8; ++s) x[s] = cmul(x[s], tw[(s * k2) & (kWP - 1)]);
+  dft8(x);
+  wave_sync();
+#pragma unroll
+  for (int s = 0; s < 8; ++s) xb[xpad(base2 + 8 * s)] = x[s];
+  wave_sync();
+#pragma unroll
+  for (int s = 0; s < 8; ++s) x[s] = xb[xpad(lane + 64 * s)];
+#pragma unroll
+  for (int s = 1; s < 8; ++s) x[s] = cmul(x[s], tw3[(s - 1) * 64 + lane]);
+  dft8(x);
+  wave_sync();
+  const unsigned nmask = (unsigned)ev.N - 1u;
+  const float invN = 1.0f / (float)ev.N;
+#pragma unroll
+  for (int s = 0; s < 8; ++s) {
+    const int jj = (lane + 64 * s - ev.jlo) & (kWP - 1);
+    if (jj < ev.L) {
+      const unsigned j = (unsigned)(ev.jlo + jj);
+      const unsigned idx = ((unsigned)n2 * j) & nmask;
+      const float2 t = expm2pi((float)idx * invN);
+      const float2 z = cmul(t, x[s]);
+      accr[s] += (double)z.x;
+      acci[s] += (double)z.y;
+    }
+  }
+}
+
+struct CalTimeArgs {
+  EvDev ev;
+  const RowW* rows;
+  long long n;
+  long long rowblocks;  // ceil(n / kWaves)
+  int nc;
+  const float2* cal;    // [nc][nd][N] curve values, decimated order
+  const double* calps;  // [nc][nd][N + 2] prefix sums of f^-7/3 sigma^2 / S |cal|^2
+  double* lnl;          // [n][nc] time-marginalised lnL per curve
+};
+
+template <int ND>
+__global__ __launch_bounds__(64 * kWaves, 2) void k_cal_time_w(CalTimeArgs a) {
+  extern __shared__ double lds_d[];
+  const EvDev& ev = a.ev;
+  float2* tw = reinterpret_cast<float2*>(lds_d);
+  float2* tw3 = tw + kWP;
+  float2* xch = tw3 + kTw3;
+  double* knots = reinterpret_cast<double*>(xch + kWaves * kXch);
+  const bool dist = (ev.flags & GWL_DISTANCE_MARG) != 0;
+  init_wave_tables(tw, tw3);
+  if (dist) stage_knots(ev, knots);
+  __syncthreads();
+  const int lane = threadIdx.x & 63;
+  const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const long long cb = blockIdx.x / a.rowblocks;
+  const long long r = (blockIdx.x - cb * a.rowblocks) * kWaves + wid;
+  if (r >= a.n) return;
+  const int c0 = (int)cb * kCalCurves;
+  const int ncb = min(kCalCurves, a.nc - c0);
+  const RowW& rw = a.rows[r];
+  if (!rw.valid) {
+    if (lane < ncb) a.lnl[(size_t)r * a.nc + c0 + lane] = row_valid_sentinel();
+    return;
+  }
+  const WaveEv wv = wave_ev(ev);
+  const int Q = ev.Q;
+  const int kb = rw.kb, ke = rw.ke;
+  float2* xb = xch + wid * kXch;
+  double accr[kCalCurves][8], acci[kCalCurves][8];
+#pragma unroll
+  for (int cc = 0; cc < kCalCurves; ++cc)
+#pragma unroll
+    for (int s = 0; s < 8; ++s) accr[cc][s] = acci[cc][s] = 0.0;
+  for (int n2 = 0; n2 < Q; ++n2) {
+    const int ilo = (kb - n2 + Q - 1) / Q;
+    const int ihi = ke >= n2 ? (ke - n2) / Q : -1;
+    if (ilo > ihi) continue;  // nothing of the support in this sub-sequence
+    float2 xd[8][ND];
+#pragma unroll
+    for (int s = 0; s < 8; ++s) {
+      const int i = lane + 64 * s;
+      if (i >= ilo && i <= ihi) {
+        integrand_tf2_det<ND>(wv, rw, n2 * kWP + i, Q * i + n2, xd[s]);
+      } else {
+#pragma unroll
+        for (int d = 0; d < ND; ++d) xd[s][d] = make_float2(0.f, 0.f);
+      }
+    }
+#pragma unroll
+    for (int cc = 0; cc < kCalCurves; ++cc) {
+      if (cc >= ncb) break;
+      const float2* cal = a.cal + (size_t)(c0 + cc) * ND * ev.N + (size_t)n2 * kWP;
+      float2 x[8];
+#pragma unroll
+      for (int s = 0; s < 8; ++s) {
+        float xr = 0.f, xi = 0.f;
+#pragma unroll
+        for (int d = 0; d < ND; ++d) {
+          const float2 k = cal[(size_t)d * ev.N + lane + 64 * s];
+          xr += k.x * xd[s][d].x - k.y * xd[s][d].y;
+          xi += k.x * xd[s][d].y + k.y * xd[s][d].x;
+        }
+        x[s] = make_float2(xr, xi);
+      }
+      wave_transform_accumulate(wv, x, n2, tw, tw3, xb, lane, accr[cc], acci[cc]);
+    }
+  }
+  // epilogue per curve: time-marginalised lnL (base.py:794-820 per curve)
+  const double norm = 4.0 / ev.T;
+  const double hs = rw.hscale, jit = rw.jitter;
+  const double b = (1.0 / (ev.tmax - ev.tmin)) * ev.tdelta;
+#pragma unroll
+  for (int cc = 0; cc < kCalCurves; ++cc) {
+    if (cc >= ncb) break;
+    const int c = c0 + cc;
+    double hh = 0.0;
+    if (kb <= ke) {
+#pragma unroll
+      for (int d = 0; d < ND; ++d) {
+        const double* ps = a.calps + ((size_t)c * ND + d) * (ev.N + 2);
+        hh += rw.cabs2[d] * (ps[ke + 1] - ps[kb]);
+      }
+    }
+    hh *= norm;
+    SplineY yy;
+    if (dist) yy = spline_y(ev, knots + ev.sp.nx + 4, hh * hs * hs);
+    double m = -INFINITY, sum = 0.0;
+#pragma unroll
+    for (int s = 0; s < 8; ++s) {
+      const int jj = (lane + 64 * s - ev.jlo) & (kWP - 1);
+      if (jj < ev.L) {
+        const int j = ev.jlo + jj;
+        const double t = (ev.start + (double)(j + 1) * ev.tstep) + jit;
+        if (t >= ev.tmin && t <= ev.tmax)
+          lse_push(m, sum, epi_value(ev, knots, yy, hh, hs, norm * accr[cc][s],
+                                     norm * acci[cc][s]));
+      }
+    }
+    lse_wave(m, sum);
+    if (lane == 0) a.lnl[(size_t)r * a.nc + c] = (m > -INFINITY) ? log(b * sum) + m : -INFINITY;
+  }
+}
+
+// One wavefront per row, lanes over curves: logsumexp_c(lnL_c) - ln C
+// (base.py:877).
+__global__ __launch_bounds__(256) void k_cal_time_epi(const RowW* rows, long long n, int nc,
+                                                      const double* lnl, double* out) {
+  const long long r = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (r >= n) return;
+  const int lane = threadIdx.x & 63;
+  if (!rows[r].valid) {
+    if (lane == 0) out[r] = row_valid_sentinel();
+    return;
+  }
+  double m = -INFINITY, sum = 0.0;
+  for (int c = lane; c < nc; c += 64) lse_push(m, sum, lnl[(size_t)r * nc + c]);
+  lse_wave(m, sum);
+  if (lane == 0) out[r] = (m > -INFINITY) ? log(sum) + m - log((double)nc) : -INFINITY;
+}
+
 // ------------------------------------------------------------------ multi-event
 // Many TaylorF2 events in one launch: row constants of every event, then one
 // workgroup per (event, group, 4-row block) item of the longest-first work
@@ -482,6 +702,60 @@ int launch_time_wave(gwl_event* e, const double* params, int64_t n, int ld, doub
 }
 
 size_t roww_bytes(int64_t rows) { return (size_t)rows * sizeof(RowW); }
+
+bool cal_time_eligible(const gwl_event* e) {
+  const EvDev& ev = e->dev;
+  return (ev.flags & GWL_TIME_MARG) && ev.approx == GWL_TAYLORF2 && ev.P == kWP && ev.tf2b &&
+         ev.L <= kWP;
+}
+
+size_t cal_time_ws_bytes(int64_t rows, int nc) {
+  return ((size_t)rows * sizeof(RowW) + 255) / 256 * 256 + (size_t)rows * nc * sizeof(double);
+}
+
+int launch_cal_time(gwl_event* e, const double* params, int64_t n, int ld, int nc,
+                    const float2* cal, const double* calps, void* ws, double* per_curve,
+                    double* out, hipStream_t st) {
+  const EvDev& ev = e->dev;
+  char* w = static_cast<char*>(ws);
+  RowW* rows = reinterpret_cast<RowW*>(w);
+  double* lnl = per_curve ? per_curve
+                          : reinterpret_cast<double*>(w + ((size_t)n * sizeof(RowW) + 255) / 256 * 256);
+  RowArgs ra{ev, params, ld, n, rows};
+  hipLaunchKernelGGL(k_rowconst, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, ra);
+  GWL_HIP_CHECK(hipGetLastError());
+  CalTimeArgs ca{};
+  ca.ev = ev;
+  ca.rows = rows;
+  ca.n = n;
+  ca.rowblocks = (n + kWaves - 1) / kWaves;
+  ca.nc = nc;
+  ca.cal = cal;
+  ca.calps = calps;
+  ca.lnl = lnl;
+  const size_t lds = time_wave_lds_bytes(ev, true);
+  if (lds > e->ctx->lds_per_block) {
+    set_error("calibration time transform exceeds the LDS of a workgroup");
+    return GWL_E_UNSUPPORTED;
+  }
+  const long long blocks = ca.rowblocks * ((nc + kCalCurves - 1) / kCalCurves);
+  if (blocks > 0x7fffffffLL) {
+    set_error("gwl_calib_eval: too many rows x curves in one call");
+    return GWL_E_INVALID;
+  }
+  const dim3 grid((unsigned)blocks);
+  switch (ev.nd) {
+    case 1: hipLaunchKernelGGL(k_cal_time_w<1>, grid, dim3(64 * kWaves), lds, st, ca); break;
+    case 2: hipLaunchKernelGGL(k_cal_time_w<2>, grid, dim3(64 * kWaves), lds, st, ca); break;
+    case 3: hipLaunchKernelGGL(k_cal_time_w<3>, grid, dim3(64 * kWaves), lds, st, ca); break;
+    default: hipLaunchKernelGGL(k_cal_time_w<4>, grid, dim3(64 * kWaves), lds, st, ca); break;
+  }
+  GWL_HIP_CHECK(hipGetLastError());
+  hipLaunchKernelGGL(k_cal_time_epi, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, st, rows,
+                     (long long)n, nc, lnl, out);
+  GWL_HIP_CHECK(hipGetLastError());
+  return GWL_OK;
+}
 
 // Chirp-z path: row constants, one workgroup per row, then the wave-per-row
 // transform (one group, fused epilogue) for the rows flagged as too wide.

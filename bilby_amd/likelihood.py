@@ -155,9 +155,9 @@ class GravitationalWaveTransient:
                  distance_marginalization_lookup_table=None, calibration_lookup_table=None,
                  number_of_response_curves=1000, starting_index=0, jitter_time=True,
                  reference_frame="sky", time_reference="geocenter", precision="fp32", device=0):
-        if calibration_marginalization and time_marginalization:
-            raise NotImplementedError("calibration marginalisation together with time "
-                                      "marginalisation is not supported")
+        if calibration_marginalization and time_marginalization and precision != "fp32":
+            raise NotImplementedError("calibration together with time marginalisation is "
+                                      "evaluated in float32 (precision='fp32')")
         if reference_frame != "sky":
             raise NotImplementedError("only reference_frame='sky' is supported")
         if "geocent" not in time_reference:
@@ -338,15 +338,17 @@ class GravitationalWaveTransient:
 
     def get_calibration_log_likelihoods(self, parameters):
         """base.py:822-858: ln L of every response curve for one parameter
-        dict (float64 contraction)."""
+        dict (float64 contraction; with time marginalisation each curve's
+        time-marginalised value, float32 transforms)."""
         rows = self.rows_from_parameters(dict(parameters))
         rows_t = torch.from_numpy(np.ascontiguousarray(rows[:1])).to("cuda:%d" % self.device)
         out = torch.empty(1, dtype=torch.float64, device=rows_t.device)
         per = torch.empty((1, self.number_of_response_curves), dtype=torch.float64,
                           device=rows_t.device)
         st = torch.cuda.current_stream(rows_t.device).cuda_stream
+        mode = _lib.FP32 if self.time_marginalization else _lib.FP64
         check(LIB.gwl_calib_eval(self._calib, ctypes.c_void_p(rows_t.data_ptr()), 1, _lib.NPARAM,
-                                 _lib.FP64, ctypes.c_void_p(out.data_ptr()),
+                                 mode, ctypes.c_void_p(out.data_ptr()),
                                  ctypes.c_void_p(per.data_ptr()), ctypes.c_void_p(st)),
               "gwl_calib_eval")
         return per.cpu().numpy()[0]
@@ -448,6 +450,9 @@ class GravitationalWaveTransient:
         marginalisation."""
         if precision is None:
             return self.calibration_mode if self._calib is not None else self.precision
+        if self._calib is not None and self.time_marginalization and precision != "fp32":
+            raise NotImplementedError("calibration together with time marginalisation is "
+                                      "evaluated in float32")
         if precision == "bf16x3":
             if self._calib is None:
                 raise ValueError("bf16x3 applies to calibration marginalisation only")

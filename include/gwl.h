@@ -392,14 +392,18 @@ int gwl_mb_set_time(gwl_mb* mb, const gwl_mb_time_desc* td);
 
 /* ------------------------------------------------------------------ calibration
  * Calibration marginalisation (base.py:332-346, 860-877, lookup
- * base.py:1037-1051) on an event without time marginalisation; the event
- * supplies data, source model, phase / distance flags and the distance spline
- * and must outlive the calibration object.
+ * base.py:1037-1051); the event supplies data, source model, time / phase /
+ * distance flags and the distance spline and must outlive the calibration
+ * object.
  *   curves: [n_curves][K][2] complex128 response curves, K = the masked bins of
  *           every detector in detector order (bilby's calibration_draws[ifo]
  *           on frequency_array[frequency_mask], concatenated).
- * Device memory: the curves in three precisions, (2 n_curves x 2K +
- * n_curves x K) x 16 bytes; the workspace per row ~ 36 K bytes. */
+ * Device memory without time marginalisation: the curves in three
+ * precisions, (2 n_curves x 2K + n_curves x K) x 16 bytes; the workspace per
+ * row ~ 36 K bytes.  With time marginalisation (base.py:305-324; a TaylorF2
+ * event on the wave-per-row transform, time window <= 512 bins): the curves
+ * in the transform's order and per-curve prefix sums for <h|h>,
+ * n_curves x n_det x N x 16 bytes; per row (1 + n_curves) x 8 bytes. */
 typedef struct gwl_calib gwl_calib;
 int gwl_calib_create(gwl_event* ev, int32_t n_curves, const double* curves, gwl_calib** out);
 int gwl_calib_destroy(gwl_calib* cal);
@@ -408,8 +412,12 @@ int gwl_calib_device_bytes(gwl_calib* cal, int64_t* bytes);
 /* ln L ratio marginalised over the curves for n rows: the per-bin integrands
  * are contracted with the curves on MFMA in `mode` (enum gwl_roq_mode:
  * GWL_ROQ_FP64, GWL_ROQ_FP32, GWL_ROQ_BF16X3; the uncalibrated totals are
- * float64 in every mode).  per_curve (may be NULL): device [n][n_curves],
- * each curve's ln L (get_calibration_log_likelihoods, base.py:822-858). */
+ * float64 in every mode).  With time marginalisation every curve's window
+ * transform runs (float32 transform, float64 window sums; mode must be
+ * GWL_ROQ_FP32) and each curve's value is time-marginalised
+ * (base.py:794-820) before the logsumexp over curves (:877).  per_curve (may
+ * be NULL): device [n][n_curves], each curve's ln L
+ * (get_calibration_log_likelihoods, base.py:822-858). */
 int gwl_calib_eval(gwl_calib* cal, const double* params, int64_t n_rows, int32_t ld, int32_t mode,
                    double* out, double* per_curve, void* stream);
 

@@ -1,7 +1,7 @@
 """FP64 restatement of bilby's calibration marginalisation
-(bilby/gw/likelihood/base.py:286-346, 822-877) and of the cubic-spline
-response model (bilby/gw/detector/calibration.py:257-383), without time
-marginalisation.
+(bilby/gw/likelihood/base.py:286-346, 822-877), also with time
+marginalisation (one FFT per response curve, :305-324), and of the
+cubic-spline response model (bilby/gw/detector/calibration.py:257-383).
 
 Test infrastructure only (see oracle/__init__.py).
 """
@@ -50,12 +50,12 @@ def cubic_spline_factor(frequency_array, node_amplitudes, node_phases, minimum_f
 
 
 class OracleCalibrated(OracleGWTransient):
-    """GravitationalWaveTransient with calibration_marginalization=True and no
-    time marginalisation.  ``curves``: detector name -> complex
-    [n_curves, n_masked_bins] on that detector's masked frequencies."""
+    """GravitationalWaveTransient with calibration_marginalization=True.
+    ``curves``: detector name -> complex [n_curves, n_masked_bins] on that
+    detector's masked frequencies."""
 
-    def __init__(self, event, source, curves, **kwargs):
-        super().__init__(event, source, time_marginalization=False, **kwargs)
+    def __init__(self, event, source, curves, time_marginalization=False, **kwargs):
+        super().__init__(event, source, time_marginalization=time_marginalization, **kwargs)
         self.curves = {k: np.asarray(v, complex) for k, v in curves.items()}
         self.number_of_response_curves = len(next(iter(self.curves.values())))
 
@@ -78,9 +78,40 @@ class OracleCalibrated(OracleGWTransient):
             hh += (np.abs(cal) ** 2).dot(y)
         return dh, hh
 
+    def calibration_time_arrays(self, params):
+        """base.py:305-324 summed over detectors: the integrand
+        conj(d) h / S tiled over the curves, multiplied by each curve on the
+        masked bins, FFT over frequency of [0:-1] -> d_inner_h_array
+        [n_curves, N]; optimal_snr_squared_array as without time."""
+        pols = source_polarizations(self.source, self.event.frequency_array, params,
+                                    self.waveform_arguments)
+        norm = 4 / self.duration
+        arr, hh = 0, np.zeros(self.number_of_response_curves)
+        for det in self.event.detectors:
+            signal = detector_response(det, pols, params, self.event.start_time,
+                                       antenna_time=self.antenna_time)
+            m = det.mask
+            with np.errstate(invalid="ignore", divide="ignore"):
+                base = np.conj(det.strain) * signal / det.psd
+            base = np.where(signal != 0, base, 0)
+            integrand = np.tile(base, (self.number_of_response_curves, 1)).T
+            integrand[m] *= self.curves[det.name].T
+            arr = arr + norm * np.fft.fft(integrand[0:-1], axis=0).T
+            y = norm * np.abs(signal[m]) ** 2 / det.psd[m]
+            hh += (np.abs(self.curves[det.name]) ** 2).dot(y)
+        return arr, hh
+
     def per_curve_log_likelihoods(self, parameters):
-        """base.py:822-858 (get_calibration_log_likelihoods)."""
+        """base.py:822-858 (get_calibration_log_likelihoods); with time
+        marginalisation each curve's value is time-marginalised
+        (base.py:794-820 per curve)."""
         params = copy.deepcopy(parameters)
+        if self.time_marginalization:
+            if self.jitter_time:
+                params["geocent_time"] += params["time_jitter"]
+            arr, hh = self.calibration_time_arrays(params)
+            return np.array([self.time_marginalized_likelihood(arr[c], hh[c], params)
+                             for c in range(self.number_of_response_curves)])
         dh, hh = self.calibration_snrs(params)
         if self.distance_marginalization:
             return self.distance_marginalized_likelihood(dh, hh, params)
@@ -98,14 +129,19 @@ class OracleCalibrated(OracleGWTransient):
         return float(logsumexp(ll) - np.log(self.number_of_response_curves))
 
     def log_likelihood_ratio_with_curve(self, parameters, index):
-        """The un-marginalised likelihood with response curve ``index``
-        applied to the signal (base.py:286-290, 'recalib_index')."""
+        """The likelihood not marginalised over calibration, with response
+        curve ``index`` applied to the signal (base.py:286-290,
+        'recalib_index'); time-marginalised through the FFT of the calibrated
+        signal (base.py:325-330) when time marginalisation is on."""
         params = copy.deepcopy(parameters)
         pols = source_polarizations(self.source, self.event.frequency_array, params,
                                     self.waveform_arguments)
-        d_inner_h, h_inner_h = 0j, 0.0
+        if self.time_marginalization and self.jitter_time:
+            params["geocent_time"] += params["time_jitter"]
+        d_inner_h, h_inner_h, arr = 0j, 0.0, None
         for det in self.event.detectors:
-            signal = detector_response(det, pols, params, self.event.start_time)
+            signal = detector_response(det, pols, params, self.event.start_time,
+                                       antenna_time=self.antenna_time)
             m = det.mask
             signal = signal.copy()
             signal[m] *= self.curves[det.name][index]
@@ -113,4 +149,10 @@ class OracleCalibrated(OracleGWTransient):
                                                            self.duration)
             h_inner_h += core.noise_weighted_inner_product(signal[m], signal[m], det.psd[m],
                                                            self.duration).real
-        return float(np.real(self.compute_log_likelihood(d_inner_h, h_inner_h, None, params)))
+            if self.time_marginalization:
+                with np.errstate(invalid="ignore", divide="ignore"):
+                    integrand = signal[0:-1] * det.strain.conj()[0:-1] / det.psd[0:-1]
+                integrand = np.where(signal[0:-1] != 0, integrand, 0)
+                a = 4 / self.duration * np.fft.fft(integrand)
+                arr = a if arr is None else arr + a
+        return float(np.real(self.compute_log_likelihood(d_inner_h, h_inner_h, arr, params)))

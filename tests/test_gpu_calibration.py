@@ -79,3 +79,67 @@ def test_calibration_marginalised_matches_oracle(flags, tmp_path):
     assert out[5] == np.nan_to_num(-np.inf)
     keep = [i for i in range(n) if i != 5]
     assert np.max(np.abs(out[keep] - got["fp64"][keep])) < 1e-12
+
+
+@pytest.mark.parametrize("case", [
+    dict(distance=False, phase=False, T=4.0, fs=1024.0, dets=("H1", "L1"), curves=100),
+    dict(distance=False, phase=True, T=4.0, fs=1024.0, dets=("H1", "L1"), curves=100),
+    dict(distance=True, phase=True, T=4.0, fs=1024.0, dets=("H1", "L1"), curves=100),
+    dict(distance=True, phase=True, T=8.0, fs=4096.0, dets=("H1", "L1", "V1"), curves=21),
+])
+def test_calibration_and_time_marginalised_matches_oracle(case, tmp_path):
+    """Calibration together with time marginalisation (k_cal_time_w: one
+    transform per curve, base.py:305-324, 862-877) against the oracle's
+    tiled per-curve FFTs: float32 transforms < 1e-3, per-curve values,
+    jittered rows, invalid rows; float64 is refused."""
+    from bilby_amd import calibration as cal
+    from bilby_amd.configs import make_taylorf2_marginalised
+    from bilby_amd.likelihood import GravitationalWaveTransient
+    from oracle.calibration import OracleCalibrated
+    from oracle.likelihood import OracleDetector, OracleEvent
+    from tests.helpers import draws_as_dicts, fixed_marginalised
+    say("calibration + time", case)
+    s = make_taylorf2_marginalised(n_draws=16, seed=9, duration=case["T"],
+                                   sampling_frequency=case["fs"], detectors=case["dets"])
+    priors = s["priors"]
+    fmax = min(512.0, case["fs"] / 2)
+    for ifo in s["interferometers"]:
+        ifo.calibration_model = cal.CubicSpline("recalib_%s_" % ifo.name, 20.0, fmax, 5)
+        priors.update(cal.constant_uncertainty_spline_priors(0.1, 0.1, 20.0, fmax, 5, ifo.name))
+    files = {ifo.name: str(tmp_path / ("%s_t.npz" % ifo.name)) for ifo in s["interferometers"]}
+    priors = s["priors"].copy()
+    lk = GravitationalWaveTransient(
+        s["interferometers"], s["waveform_generator"], calibration_marginalization=True,
+        time_marginalization=True, distance_marginalization=case["distance"],
+        phase_marginalization=case["phase"], priors=priors,
+        number_of_response_curves=case["curves"], calibration_lookup_table=files,
+        precision="fp32")
+    ifos = s["interferometers"]
+    dets = [OracleDetector(i.name, i.frequency_array, i.frequency_domain_strain,
+                           i.power_spectral_density_array, i.frequency_mask) for i in ifos]
+    ev = OracleEvent(dets, ifos.duration, ifos.sampling_frequency, ifos.start_time)
+    olk = OracleCalibrated(ev, "taylorf2", lk.calibration_draws,
+                           waveform_arguments=dict(s["waveform_generator"].waveform_arguments),
+                           time_marginalization=True, distance_marginalization=case["distance"],
+                           phase_marginalization=case["phase"], priors=dict(s["priors"]),
+                           distance_table=getattr(lk, "_dist_margd_loglikelihood_array", None))
+    n = 8 if case["T"] > 4 else 16
+    draws = {k: v[:n] for k, v in fixed_marginalised(s["draws"], priors).items()}
+    draws["time_jitter"] = np.random.default_rng(3).uniform(-1, 1, n) / case["fs"]
+    ref = np.array([olk.log_likelihood_ratio(p) for p in draws_as_dicts(draws, range(n))])
+    got = lk.log_likelihood_ratio_batch(draws)
+    say("fp32", float(np.max(np.abs(got - ref))), "lnL range", ref.min(), ref.max())
+    assert np.all(np.isfinite(ref))
+    assert np.max(np.abs(got - ref)) < 1e-3
+    p0 = draws_as_dicts(draws, [2])[0]
+    per = lk.get_calibration_log_likelihoods(p0)
+    assert per.shape == (case["curves"],)
+    assert np.max(np.abs(per - olk.per_curve_log_likelihoods(p0))) < 1e-3
+    bad = {k: v.copy() for k, v in draws.items()}
+    bad["mass_1"][5] = -1.0
+    out = lk.log_likelihood_ratio_batch(bad)
+    assert out[5] == np.nan_to_num(-np.inf)
+    keep = [i for i in range(n) if i != 5]
+    assert np.array_equal(out[keep], got[keep])
+    with pytest.raises(NotImplementedError):
+        lk.log_likelihood_ratio_batch(draws, precision="fp64")

@@ -77,6 +77,59 @@ def test_identity_curves_give_the_uncalibrated_likelihood():
                                                          rel=1e-12, abs=1e-9)
 
 
+class _U:
+    def __init__(self, a, b):
+        self.minimum, self.maximum = a, b
+
+    def prob(self, x):
+        x = np.asarray(x)
+        return ((x >= self.minimum) & (x <= self.maximum)) / (self.maximum - self.minimum)
+
+    def rescale(self, u):
+        return self.minimum + u * (self.maximum - self.minimum)
+
+
+@pytest.mark.parametrize("phase", [False, True])
+def test_time_marginalised_equals_logsumexp_of_per_curve_likelihoods(phase):
+    """The reference's check with time marginalisation
+    (marginalization_test.py:524-554, "time", jitter off): the calibration-
+    and time-marginalised ln L (one FFT per curve of the tiled, curve-
+    multiplied integrand, base.py:305-324) equals the logsumexp over curves
+    (weights 1/C) of the time-marginalised ln L with each curve applied to
+    the signal (base.py:286-290, 325-330), to 7 places."""
+    ev, inj = _event()
+    curves = _curves(ev, 100, seed=5)
+    tc = inj["geocent_time"]
+    pri = {"geocent_time": _U(tc - 0.1, tc + 0.1)}
+    if phase:
+        pri["phase"] = 0.0
+    lk = OracleCalibrated(ev, "taylorf2", curves, waveform_arguments=WF,
+                          time_marginalization=True, jitter_time=False,
+                          phase_marginalization=phase, priors=pri)
+    p = dict(inj, geocent_time=ev.start_time)
+    p["mass_1"] *= 1.003
+    if phase:
+        p["phase"] = 0.0
+    marg = lk.log_likelihood_ratio(p)
+    per = [lk.log_likelihood_ratio_with_curve(p, i) for i in range(100)]
+    assert max(per) > 20
+    assert marg == pytest.approx(logsumexp(per, b=1 / 100), abs=1e-7)
+    assert np.allclose(lk.per_curve_log_likelihoods(p), per, rtol=0, atol=1e-8)
+
+
+def test_time_marginalised_identity_curves_give_the_time_marginalised_likelihood():
+    ev, inj = _event(seed=4)
+    ones = {det.name: np.ones((5, int(det.mask.sum())), complex) for det in ev.detectors}
+    tc = inj["geocent_time"]
+    lk = OracleCalibrated(ev, "taylorf2", ones, waveform_arguments=WF, time_marginalization=True,
+                          priors={"geocent_time": _U(tc - 0.1, tc + 0.1)})
+    full = OracleGWTransient(ev, "taylorf2", waveform_arguments=WF, time_marginalization=True,
+                             priors={"geocent_time": _U(tc - 0.1, tc + 0.1)})
+    p = dict(inj, geocent_time=ev.start_time, time_jitter=1e-4)
+    assert lk.log_likelihood_ratio(p) == pytest.approx(full.log_likelihood_ratio(p),
+                                                       rel=1e-12, abs=1e-9)
+
+
 def test_cubic_spline_is_not_a_knot_in_log_frequency():
     rng = np.random.default_rng(5)
     f = np.linspace(20.0, 512.0, 2000)
