@@ -361,6 +361,46 @@ def bench_calibration(steps, batch=2048, n_curves=1000):
     out["oracle_evals_per_s_1core"] = 8 / (time.perf_counter() - t0)
     for mode in res:
         out[mode + "_vs_oracle_8rows"] = float(np.max(np.abs(res[mode][:8] - ref)))
+    del lk
+    # calibration together with time (+ distance + phase) marginalisation: one
+    # window transform per (row, curve), the same event and curves
+    tbatch = 256
+    s2 = make_taylorf2_marginalised(n_draws=tbatch, seed=4321)
+    priors2 = s2["priors"]
+    ifos2 = s2["interferometers"]
+    for ifo in ifos2:
+        ifo.calibration_model = cal.CubicSpline("recalib_%s_" % ifo.name, 20.0, 2048.0, 5)
+        priors2.update(cal.constant_uncertainty_spline_priors(0.1, 0.1, 20.0, 2048.0, 5, ifo.name))
+    opri2 = dict(priors2)
+    lk2_priors = priors2.copy()
+    lk2 = GravitationalWaveTransient(ifos2, s2["waveform_generator"],
+                                     calibration_marginalization=True, time_marginalization=True,
+                                     distance_marginalization=True, phase_marginalization=True,
+                                     priors=lk2_priors, number_of_response_curves=n_curves,
+                                     calibration_lookup_table=files, precision="fp32")
+    draws2 = fixed_marginalised(s2["draws"], lk2_priors)
+    draws2["time_jitter"] = np.random.default_rng(5).uniform(-1, 1, tbatch) / 4096.0
+    rows2 = torch.from_numpy(lk2.rows_from_parameters(draws2)).cuda()
+    check(LIB.gwl_calib_reserve(lk2._calib, tbatch))
+    dt = timed(lambda: lk2.log_likelihood_ratio_batch(rows2), steps, warmup=1)
+    out["time_tdp_rows"] = tbatch
+    out["time_tdp_fp32_evals_per_s"] = tbatch / dt
+    out["time_tdp_ms_per_batch"] = dt * 1e3
+    out["time_tdp_curve_transforms_per_s"] = tbatch * n_curves / dt
+    r2 = lk2.log_likelihood_ratio_batch(rows2).cpu().numpy()
+    ev2 = OracleEvent([OracleDetector(i.name, i.frequency_array, i.frequency_domain_strain,
+                                      i.power_spectral_density_array, i.frequency_mask)
+                       for i in ifos2], ifos2.duration, ifos2.sampling_frequency,
+                      ifos2.start_time)
+    olk2 = OracleCalibrated(ev2, "taylorf2", lk2.calibration_draws,
+                            waveform_arguments=dict(s2["waveform_generator"].waveform_arguments),
+                            time_marginalization=True, distance_marginalization=True,
+                            phase_marginalization=True, priors=opri2,
+                            distance_table=lk2._dist_margd_loglikelihood_array)
+    t0 = time.perf_counter()
+    ref2 = np.array([olk2.log_likelihood_ratio(p) for p in draws_as_dicts(draws2, range(2))])
+    out["time_tdp_oracle_evals_per_s_1core"] = 2 / (time.perf_counter() - t0)
+    out["time_tdp_fp32_vs_oracle_2rows"] = float(np.max(np.abs(r2[:2] - ref2)))
     return out
 
 
