@@ -29,17 +29,23 @@ class RowBuilder:
     """Parameter dict (scalars or 1-D arrays, bilby names) -> float64 rows
     [n, NPARAM] in the column order of enum gwl_param: the waveform
     generator's parameter conversion (waveform_generator.py:260-269), then
-    the defaults of the marginalised keys, then the columns.  Host-only and
-    picklable, so a sampler's worker process can build rows itself."""
+    the defaults of the marginalised keys, then the likelihood's sky frame
+    (``frame``, a bilby_amd.frames.SkyFrame: detector-frame zenith / azimuth
+    and <ifo>_time to ra / dec / geocent_time, base.py:1091-1137), then the
+    columns.  Host-only and picklable, so a sampler's worker process can
+    build rows itself."""
 
-    def __init__(self, parameter_conversion, defaults):
+    def __init__(self, parameter_conversion, defaults, frame=None):
         self.parameter_conversion = parameter_conversion
         self.defaults = dict(defaults)
+        self.frame = None if frame is None or frame.is_identity else frame
 
     def __call__(self, parameters):
         p = self.parameter_conversion(dict(parameters))
         for key, value in self.defaults.items():
             p.setdefault(key, value)
+        if self.frame is not None:
+            p = self.frame(p)
         n = 1
         for v in p.values():
             if np.ndim(v) > 0:

@@ -158,10 +158,14 @@ class GravitationalWaveTransient:
         if calibration_marginalization and time_marginalization and precision != "fp32":
             raise NotImplementedError("calibration together with time marginalisation is "
                                       "evaluated in float32 (precision='fp32')")
-        if reference_frame != "sky":
-            raise NotImplementedError("only reference_frame='sky' is supported")
-        if "geocent" not in time_reference:
-            raise NotImplementedError("only time_reference='geocenter' is supported")
+        from .frames import sky_frame
+        # base.py:170-181: a detector time reference cannot be marginalised over
+        self._frame = sky_frame(reference_frame, time_reference)
+        self.reference_frame = reference_frame
+        self.time_reference = "geocent" if "geocent" in time_reference else time_reference
+        if "geocent" not in time_reference and time_marginalization:
+            time_marginalization = False
+            jitter_time = False
         self.waveform_generator = waveform_generator
         self.interferometers = InterferometerList(interferometers)
         self.time_marginalization = time_marginalization
@@ -390,6 +394,10 @@ class GravitationalWaveTransient:
                     distance_marginalization=self.distance_marginalization,
                     calibration_marginalization=False,
                     waveform_arguments=self.waveform_generator.waveform_arguments,
+                    time_reference=self.time_reference,
+                    reference_frame=(self.reference_frame if isinstance(self.reference_frame, str)
+                                     else "".join(getattr(i, "name", str(i))
+                                                  for i in self.reference_frame)),
                     precision="fp32" if self.precision == _lib.FP32 else "fp64",
                     backend="libgwl (HIP, gfx950)")
 
@@ -417,7 +425,14 @@ class GravitationalWaveTransient:
     def row_builder(self):
         """The host-side dict -> rows conversion of this likelihood
         (picklable; what worker-process clients use)."""
-        return _abi.RowBuilder(self.waveform_generator.parameter_conversion, self._defaults())
+        return _abi.RowBuilder(self.waveform_generator.parameter_conversion, self._defaults(),
+                               frame=self._frame)
+
+    def get_sky_frame_parameters(self, parameters):
+        """base.py:1091-1137: ra, dec and geocent_time of a parameter dict in
+        this likelihood's reference frame and time reference."""
+        p = self._frame(dict(parameters))
+        return dict(ra=p["ra"], dec=p["dec"], geocent_time=p["geocent_time"])
 
     def rows_from_parameters(self, parameters):
         """Parameter dict (scalars or 1-D arrays) -> float64 rows [n, 16] in

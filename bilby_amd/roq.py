@@ -35,8 +35,14 @@ class ROQGravitationalWaveTransient:
             raise NotImplementedError(
                 "building ROQ weights from bases (linear_matrix / quadratic_matrix) is not "
                 "supported; pass precomputed weights")
-        if reference_frame != "sky" or "geocent" not in time_reference:
-            raise NotImplementedError("only the sky frame and geocenter time are supported")
+        from .frames import sky_frame
+        # base.py:170-181: a detector time reference cannot be marginalised over
+        self._frame = sky_frame(reference_frame, time_reference)
+        self.reference_frame = reference_frame
+        self.time_reference = "geocent" if "geocent" in time_reference else time_reference
+        if "geocent" not in time_reference and time_marginalization:
+            time_marginalization = False
+            jitter_time = False
         if roq_scale_factor != 1:
             raise NotImplementedError("roq_scale_factor != 1 is not supported")
         self.interferometers = InterferometerList(interferometers)
@@ -180,7 +186,8 @@ class ROQGravitationalWaveTransient:
     def row_builder(self):
         """The host-side dict -> rows conversion of this likelihood
         (picklable; what worker-process clients use)."""
-        return _abi.RowBuilder(self.waveform_generator.parameter_conversion, self._defaults())
+        return _abi.RowBuilder(self.waveform_generator.parameter_conversion, self._defaults(),
+                               frame=self._frame)
 
     def rows_from_parameters(self, parameters):
         """Parameter dict (scalars or 1-D arrays) -> float64 rows [n, 16] in

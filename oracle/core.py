@@ -183,6 +183,34 @@ def time_delay_from_geocenter(vertex, ra, dec, time):
     return omega @ (np.zeros(3) - vertex) / SPEED_OF_LIGHT
 
 
+def rotate_z_to_delta(vector, delta_x):
+    """bilby/gw/geometry.py:216-282 applied to ``vector``: the rotations
+    R1(alpha) about z, R2(beta) about y, R3(gamma) about z, one after the
+    other (alpha = atan2(-d_y d_z, d_x), beta = acos(d_z), gamma =
+    atan2(d_y, d_x) of the unit separation d)."""
+    d = np.asarray(delta_x, float)
+    d = d / np.linalg.norm(d)
+    alpha = np.arctan2(-d[1] * d[2], d[0])
+    beta = np.arccos(d[2])
+    gamma = np.arctan2(d[1], d[0])
+    x, y, z = (np.asarray(c, float) for c in vector)
+    x, y = np.cos(alpha) * x - np.sin(alpha) * y, np.sin(alpha) * x + np.cos(alpha) * y
+    x, z = np.cos(beta) * x + np.sin(beta) * z, -np.sin(beta) * x + np.cos(beta) * z
+    x, y = np.cos(gamma) * x - np.sin(gamma) * y, np.sin(gamma) * x + np.cos(gamma) * y
+    return x, y, z
+
+
+def zenith_azimuth_to_ra_dec(zenith, azimuth, time, delta_x):
+    """bilby/gw/utils.py:232-256 with geometry.py:347-380 and
+    core/utils/conversion.py:27-30."""
+    wx, wy, wz = rotate_z_to_delta((np.sin(zenith) * np.cos(azimuth),
+                                    np.sin(zenith) * np.sin(azimuth), np.cos(zenith)), delta_x)
+    theta = np.arccos(wz)
+    phi = np.arctan2(wy, wx) % (2 * np.pi)
+    ra = (phi + greenwich_mean_sidereal_time(time)) % (2 * np.pi)
+    return ra, np.pi / 2 - theta
+
+
 # ---------------------------------------------------------------- sources
 def sinegaussian(frequency_array, hrss, Q, frequency, **kwargs):
     """bilby/gw/source.py:1153-1209."""
