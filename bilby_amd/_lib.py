@@ -145,6 +145,11 @@ def _load():
         "gwl_mb_create": ([vp, ctypes.c_int32, _dp, _dp, _dp, ctypes.POINTER(vp)], ctypes.c_int),
         "gwl_mb_destroy": ([vp], ctypes.c_int),
         "gwl_mb_set_time": ([vp, ctypes.POINTER(MbTimeDesc)], ctypes.c_int),
+        "gwl_eval_time_series": ([vp, vp, ctypes.c_int64, ctypes.c_int32, ctypes.c_int32, vp, vp,
+                                  vp], ctypes.c_int),
+        "gwl_event_time_window": ([vp, ctypes.POINTER(ctypes.c_int64),
+                                   ctypes.POINTER(ctypes.c_int32),
+                                   ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
         "gwl_relbin_set_time": ([vp, ctypes.POINTER(ctypes.c_int64), _dp], ctypes.c_int),
         "gwl_mb_reserve": ([vp, ctypes.c_int64], ctypes.c_int),
         "gwl_mb_device_bytes": ([vp, ctypes.POINTER(ctypes.c_int64)], ctypes.c_int),
@@ -205,6 +210,6 @@ EXPORTED_SYMBOLS = [
     "gwl_relbin_eval", "gwl_server_add_relbin", "gwl_relbin_reserve", "gwl_calib_create",
     "gwl_calib_destroy", "gwl_calib_reserve", "gwl_calib_device_bytes", "gwl_calib_eval",
     "gwl_mb_create", "gwl_mb_destroy", "gwl_mb_reserve", "gwl_mb_device_bytes", "gwl_mb_eval",
-    "gwl_mb_set_time", "gwl_relbin_set_time",
+    "gwl_mb_set_time", "gwl_relbin_set_time", "gwl_eval_time_series", "gwl_event_time_window",
     "gwl_roq_weights_linear",
 ]

@@ -643,6 +643,35 @@ int gwl_eval(gwl_event* e, const double* params, int64_t n, int32_t ld, int32_t 
   return launch_eval(e, params, n, ld, precision, out, aux, reinterpret_cast<hipStream_t>(stream));
 }
 
+int gwl_eval_time_series(gwl_event* e, const double* params, int64_t n, int32_t ld,
+                         int32_t precision, double* d_inner_h, double* h_inner_h, void* stream) {
+  if (e && n == 0) return GWL_OK;
+  if (!e || !params || !d_inner_h || !h_inner_h || n < 0 || ld < GWL_NPARAM ||
+      (precision != GWL_FP32 && precision != GWL_FP64)) {
+    set_error("gwl_eval_time_series: invalid argument");
+    return GWL_E_INVALID;
+  }
+  if (!(e->dev.flags & GWL_TIME_MARG)) {
+    set_error("gwl_eval_time_series: the event does not marginalise over time");
+    return GWL_E_UNSUPPORTED;
+  }
+  if (int rc = gwl_event_reserve(e, n)) return rc;
+  GWL_HIP_CHECK(hipSetDevice(e->ctx->device));
+  return launch_time_series(e, params, n, ld, precision, d_inner_h, h_inner_h,
+                            reinterpret_cast<hipStream_t>(stream));
+}
+
+int gwl_event_time_window(gwl_event* e, int64_t* j_first, int32_t* n_window, double* t_step) {
+  if (!e || !j_first || !n_window || !t_step) {
+    set_error("gwl_event_time_window: invalid argument");
+    return GWL_E_INVALID;
+  }
+  *j_first = e->dev.jlo;
+  *n_window = e->dev.L;
+  *t_step = e->dev.tstep;
+  return GWL_OK;
+}
+
 int gwl_detector_response(gwl_event* e, const double* params, int32_t det, int32_t precision,
                           double* out) {
   if (!e || !params || !out || det < 0 || det >= e->dev.nd) {

@@ -120,6 +120,8 @@ int init_time_wave_tables(const EvDev& ev, double* tf2b, float2* dwa32);
 int launch_time_wave(gwl_event* ev, const double* params, int64_t n, int ld, double* out,
                      hipStream_t stream);
 size_t roww_bytes(int64_t rows);
+int launch_time_series(gwl_event* ev, const double* params, int64_t n, int ld, int precision,
+                       double* dh, double* hh, hipStream_t stream);
 // Calibration with time marginalisation (gwl_time_wave.hip)
 bool cal_time_eligible(const gwl_event* ev);
 size_t cal_time_ws_bytes(int64_t rows, int nc);

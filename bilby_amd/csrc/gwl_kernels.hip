@@ -644,15 +644,13 @@ static size_t reduce_part_bytes(const EvDev& ev, long long rows) {
   return ((size_t)rows * nchunk * 3 * sizeof(double) + 255) / 256 * 256;
 }
 
+// The tiled time-marginalised transform: window partials [n][G][L] and
+// <h|h> partials [n][G] in the event workspace (G returned).
 template <typename R, int WAVE, int ND>
-static int eval_typed(gwl_event* e, const double* params, int64_t n, int ld, double* out,
-                      double* aux, hipStream_t st) {
+static int launch_time_tiled(gwl_event* e, const double* params, int64_t n, int ld,
+                             hipStream_t st, int* G_out, double2** part_out, double** hh_out) {
   const EvDev& ev = e->dev;
-  if constexpr (std::is_same<R, float>::value && WAVE == GWL_TAYLORF2) {
-    if (time_chirp_eligible(e, n)) return launch_time_chirp(e, params, n, ld, out, st);
-    if (time_wave_eligible(e)) return launch_time_wave(e, params, n, ld, out, st);
-  }
-  if (ev.flags & GWL_TIME_MARG) {
+  {
     // tile: S rows x P-point transforms, G groups of sub-sequences
     constexpr int NT = 512;
     const int P = ev.P;
@@ -698,7 +696,27 @@ static int eval_typed(gwl_event* e, const double* params, int64_t n, int ld, dou
 #undef GWL_LAUNCH_TIME
     GWL_HIP_CHECK(hipGetLastError());
     if (int rc = timing_end(e, st)) return rc;
-    return launch_epi_time(e, params, n, ld, G, ta.part, ta.hhpart, out, st);
+    *G_out = G;
+    *part_out = ta.part;
+    *hh_out = ta.hhpart;
+    return GWL_OK;
+  }
+}
+
+template <typename R, int WAVE, int ND>
+static int eval_typed(gwl_event* e, const double* params, int64_t n, int ld, double* out,
+                      double* aux, hipStream_t st) {
+  const EvDev& ev = e->dev;
+  if constexpr (std::is_same<R, float>::value && WAVE == GWL_TAYLORF2) {
+    if (time_chirp_eligible(e, n)) return launch_time_chirp(e, params, n, ld, out, st);
+    if (time_wave_eligible(e)) return launch_time_wave(e, params, n, ld, out, st);
+  }
+  if (ev.flags & GWL_TIME_MARG) {
+    int G = 0;
+    double2* part = nullptr;
+    double* hh = nullptr;
+    if (int rc = launch_time_tiled<R, WAVE, ND>(e, params, n, ld, st, &G, &part, &hh)) return rc;
+    return launch_epi_time(e, params, n, ld, G, part, hh, out, st);
   }
   constexpr int NT = 256;
   const int S = e->tile_rows >= 16 ? 16 : e->tile_rows >= 8 || e->tile_rows <= 0 ? 8 : 4;
@@ -785,6 +803,70 @@ int launch_eval(gwl_event* e, const double* params, int64_t n, int ld, int preci
   }
   if (wave == GWL_TAYLORF2) return eval_nd<double, GWL_TAYLORF2>(e, params, n, ld, out, aux, st);
   return eval_nd<double, GWL_SINE_GAUSSIAN>(e, params, n, ld, out, aux, st);
+}
+
+// d_inner_h_array at the window and <h|h> of every row (base.py:325-330
+// before the time marginalisation): the group partials summed and scaled by
+// 4/T; one thread per (row, window bin).
+__global__ __launch_bounds__(256) void k_time_series_out(long long n, int G, int L, double norm,
+                                                         const double2* part, const double* hhpart,
+                                                         double* dh, double* hh) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n * L) return;
+  const long long r = t / L;
+  const int jj = (int)(t - r * L);
+  double xr = 0.0, xi = 0.0;
+  for (int g = 0; g < G; ++g) {
+    const double2 v = part[((size_t)r * G + g) * L + jj];
+    xr += v.x;
+    xi += v.y;
+  }
+  dh[2 * t + 0] = norm * xr;
+  dh[2 * t + 1] = norm * xi;
+  if (jj == 0) {
+    double s = 0.0;
+    for (int g = 0; g < G; ++g) s += hhpart[(size_t)r * G + g];
+    hh[r] = norm * s;
+  }
+}
+
+template <typename R, int WAVE, int ND>
+static int series_typed(gwl_event* e, const double* params, int64_t n, int ld, double* dh,
+                        double* hh, hipStream_t st) {
+  int G = 0;
+  double2* part = nullptr;
+  double* hhp = nullptr;
+  if (int rc = launch_time_tiled<R, WAVE, ND>(e, params, n, ld, st, &G, &part, &hhp)) return rc;
+  const EvDev& ev = e->dev;
+  const long long total = (long long)n * ev.L;
+  hipLaunchKernelGGL(k_time_series_out, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st,
+                     (long long)n, G, ev.L, 4.0 / ev.T, part, hhp, dh, hh);
+  GWL_HIP_CHECK(hipGetLastError());
+  return GWL_OK;
+}
+
+template <typename R, int WAVE>
+static int series_nd(gwl_event* e, const double* params, int64_t n, int ld, double* dh, double* hh,
+                     hipStream_t st) {
+  switch (pick_nd(e->dev.nd)) {
+    case 1: return series_typed<R, WAVE, 1>(e, params, n, ld, dh, hh, st);
+    case 2: return series_typed<R, WAVE, 2>(e, params, n, ld, dh, hh, st);
+    case 3: return series_typed<R, WAVE, 3>(e, params, n, ld, dh, hh, st);
+    case 4: return series_typed<R, WAVE, 4>(e, params, n, ld, dh, hh, st);
+  }
+  set_error("unsupported detector count");
+  return GWL_E_INVALID;
+}
+
+int launch_time_series(gwl_event* e, const double* params, int64_t n, int ld, int precision,
+                       double* dh, double* hh, hipStream_t st) {
+  const int wave = e->dev.approx;
+  if (precision == GWL_FP32) {
+    if (wave == GWL_TAYLORF2) return series_nd<float, GWL_TAYLORF2>(e, params, n, ld, dh, hh, st);
+    return series_nd<float, GWL_SINE_GAUSSIAN>(e, params, n, ld, dh, hh, st);
+  }
+  if (wave == GWL_TAYLORF2) return series_nd<double, GWL_TAYLORF2>(e, params, n, ld, dh, hh, st);
+  return series_nd<double, GWL_SINE_GAUSSIAN>(e, params, n, ld, dh, hh, st);
 }
 
 size_t multi_lds_bytes() {

@@ -70,6 +70,32 @@ def optimal_snr_squared_ref_axis(n=400):
     return np.logspace(-5, 10, n)
 
 
+def ln_i0_host(value):
+    """gw/utils.py:1006-1022 on the host: ln I0(|x|) = ln(i0e(|x|)) + |x|."""
+    from scipy.special import i0e
+    value = np.abs(np.asarray(value, dtype=float))
+    return np.log(i0e(value)) + value
+
+
+def interped_sample(xx, yy, rng):
+    """One draw of bilby's Interped(xx, yy).sample()
+    (core/prior/interpolated.py:12-175): the density re-gridded linearly to
+    linspace(min, max, len(xx)), normalised by the trapezoid rule, its
+    cumulative trapezoid (last value set to 1) inverted by linear
+    interpolation at a uniform variate."""
+    from scipy.integrate import cumulative_trapezoid, trapezoid
+    xx = np.asarray(xx, dtype=float)
+    yy = np.asarray(yy, dtype=float)
+    order = np.argsort(xx, kind="stable")
+    xx, yy = xx[order], yy[order]
+    grid = np.linspace(xx[0], xx[-1], len(xx))
+    density = np.interp(grid, xx, yy)
+    density = density / trapezoid(density, grid)
+    cdf = cumulative_trapezoid(density, grid, initial=0)
+    cdf[-1] = 1
+    return float(np.interp(rng.uniform(0, 1), cdf, grid))
+
+
 def _test_cached_lookup_table(loaded, distance_array, prior_array, ref_dist, phase):
     """base.py:979-992."""
     pairs = dict(distance_array=distance_array, prior_array=prior_array,
@@ -255,16 +281,26 @@ class GravitationalWaveTransient:
         return f
 
     def _build_event(self):
+        self._event, self.hbm_bytes = self._make_event()
+
+    def _make_event(self, n_freq=None, flags=None, time_prior=None):
+        """The device event of this likelihood's data (gwl_event_create).
+        ``n_freq``: pad the data with zeros and the PSD with +inf (zero
+        weight) to that many frequency bins at the same spacing (sampling
+        frequency 2 (n_freq - 1) / T); ``flags`` / ``time_prior`` override
+        the likelihood's."""
         ifos = self.interferometers
         ctx = get_context(self.device)
         nd = len(ifos)
         freq = ifos.frequency_array
-        nf = len(freq)
-        strain = np.ascontiguousarray(
-            np.stack([ifo.frequency_domain_strain for ifo in ifos]).astype(np.complex128))
-        strain_f = strain.view(np.float64).reshape(nd, nf, 2).copy()
-        psd = np.ascontiguousarray(np.stack([ifo.power_spectral_density_array for ifo in ifos]),
-                                   dtype=np.float64)
+        nf0 = len(freq)
+        nf = nf0 if n_freq is None else int(n_freq)
+        strain = np.zeros((nd, nf), dtype=np.complex128)
+        strain[:, :nf0] = np.stack([ifo.frequency_domain_strain for ifo in ifos])
+        strain_f = np.ascontiguousarray(strain.view(np.float64).reshape(nd, nf, 2))
+        psd = np.full((nd, nf), np.inf)
+        psd[:, :nf0] = np.stack([ifo.power_spectral_density_array for ifo in ifos])
+        psd = np.ascontiguousarray(psd, dtype=np.float64)
         lo = np.array([ifo.mask_range[0] for ifo in ifos], np.int32)
         hi = np.array([ifo.mask_range[1] for ifo in ifos], np.int32)
         tens = np.ascontiguousarray(np.stack([ifo.geometry.detector_tensor.ravel() for ifo in ifos]),
@@ -275,7 +311,8 @@ class GravitationalWaveTransient:
         desc.n_det = nd
         desc.n_freq = nf
         desc.duration = float(ifos.duration)
-        desc.sampling_frequency = float(ifos.sampling_frequency)
+        desc.sampling_frequency = (float(ifos.sampling_frequency) if n_freq is None
+                                   else 2.0 * (nf - 1) / float(ifos.duration))
         desc.start_time = float(ifos.start_time)
         desc.mask_lo = iptr(lo)
         desc.mask_hi = iptr(hi)
@@ -288,20 +325,22 @@ class GravitationalWaveTransient:
             if desc.approximant == _lib.TAYLORF2 else 0.0
         desc.wf_maximum_frequency = float(wfa.get("maximum_frequency", freq[-1]))
         desc.reference_frequency = float(wfa.get("reference_frequency", 50.0))
-        desc.flags = self._flags()
-        if self.time_marginalization:
-            tp = self.priors["geocent_time"]
-            if not isinstance(tp, Uniform):
-                raise NotImplementedError("time marginalisation supports a Uniform geocent_time prior")
-            desc.time_prior_minimum = float(tp.minimum)
-            desc.time_prior_maximum = float(tp.maximum)
+        desc.flags = self._flags() if flags is None else int(flags)
+        if desc.flags & _lib.TIME_MARG:
+            if time_prior is None:
+                tp = self.priors["geocent_time"]
+                if not isinstance(tp, Uniform):
+                    raise NotImplementedError("time marginalisation supports a Uniform "
+                                              "geocent_time prior")
+                time_prior = (tp.minimum, tp.maximum)
+            desc.time_prior_minimum = float(time_prior[0])
+            desc.time_prior_maximum = float(time_prior[1])
         desc.reference_distance = float(self._ref_dist)
         ev = ctypes.c_void_p()
         nbytes = ctypes.c_int64()
         check(LIB.gwl_event_create(ctx, ctypes.byref(desc), ctypes.byref(ev), ctypes.byref(nbytes)),
               "gwl_event_create")
-        self._event = ev
-        self.hbm_bytes = int(nbytes.value)
+        return ev, int(nbytes.value)
 
     def _setup_distance_marginalization(self, lookup_table=None):
         """base.py:916-992: lookup table (cached to an .npz file when a file
@@ -378,8 +417,167 @@ class GravitationalWaveTransient:
             if self._event is not None:
                 LIB.gwl_event_destroy(self._event)
                 self._event = None
+            if getattr(self, "_up_event", None) is not None:
+                LIB.gwl_event_destroy(self._up_event)
+                self._up_event = None
         except Exception:
             pass
+
+    # ------------------------------------------------------------------ reconstruction
+    def generate_posterior_sample_from_marginalized_likelihood(self, parameters, rng=None):
+        """base.py:502-542: new values of the marginalised parameters drawn
+        from their conditional posteriors given the rest of ``parameters``
+        (Thrane & Talbot, arXiv:1809.02293, Eq. C29-C32), in bilby's order:
+        calibration curve, time, distance, phase.  ``rng``: a numpy Generator
+        (bilby uses its global one)."""
+        if len(self._marginalized_parameters) == 0:
+            return parameters
+        if self.calibration_marginalization and len(self._marginalized_parameters) > 1:
+            raise NotImplementedError("reconstruction with calibration and further "
+                                      "marginalisations is not provided")
+        rng = np.random.default_rng() if rng is None else rng
+        p = dict(parameters)
+        if self.calibration_marginalization:
+            p["recalib_index"] = self.generate_calibration_sample_from_marginalized_likelihood(
+                p, rng=rng)
+        if self.time_marginalization:
+            p["geocent_time"] = self.generate_time_sample_from_marginalized_likelihood(p, rng=rng)
+        if self.distance_marginalization:
+            p["luminosity_distance"] = self.generate_distance_sample_from_marginalized_likelihood(
+                p, rng=rng)
+        if self.phase_marginalization:
+            p["phase"] = self.generate_phase_sample_from_marginalized_likelihood(p, rng=rng)
+        return p.copy()
+
+    def _plain_inner_products(self, parameters):
+        """<d|h> (complex) and <h|h> of one parameter dict in the sky frame,
+        from a companion float64 likelihood without marginalisation on the
+        same data (base.py:686-694, _calculate_inner_products)."""
+        if getattr(self, "_plain", None) is None:
+            self._plain = GravitationalWaveTransient(self.interferometers, self.waveform_generator,
+                                                     precision="fp64", device=self.device)
+        rows = self._plain.rows_from_parameters(dict(parameters))[:1]
+        _, aux = self._plain.log_likelihood_ratio_batch(rows, return_inner_products=True)
+        return complex(aux[0, 0], aux[0, 1]), float(aux[0, 2])
+
+    def _distance_marginalized_values(self, d_inner_h, h_inner_h, parameters):
+        """base.py:775-784 on the host for an array of <d|h>: the lookup
+        table's bicubic interpolant (-inf outside its axes)."""
+        from scipy.interpolate import RectBivariateSpline
+        if getattr(self, "_host_interp", None) is None:
+            x = d_inner_h_ref_axis(self.phase_marginalization)
+            y = optimal_snr_squared_ref_axis()
+            self._host_interp = (RectBivariateSpline(x, y, self._dist_margd_loglikelihood_array.T,
+                                                     kx=3, ky=3, s=0), x, y)
+        spline, x, y = self._host_interp
+        scale = parameters["luminosity_distance"] / self._ref_dist
+        d = np.asarray(d_inner_h) * scale
+        d = np.abs(d) if self.phase_marginalization else d.real
+        h = np.broadcast_to(np.asarray(h_inner_h, dtype=float) * scale ** 2, d.shape)
+        out = spline(d, h, grid=False)
+        bad = (d < x.min()) | (d > x.max()) | (h < y.min()) | (h > y.max())
+        return np.where(bad, -np.inf, out)
+
+    def _upsampled_time_event(self):
+        """The reconstruction grid of base.py:580-640: the data and PSD
+        zero-padded (PSD +inf, zero weight) to 16384 T + 1 frequency bins,
+        so that the event's transform (length 16384 T) steps 1/16384 s; a
+        time window widened by the time-jitter range and a millisecond, the
+        prior itself being applied on the host."""
+        if getattr(self, "_up_event", None) is None:
+            T = float(self.interferometers.duration)
+            n_freq = int(round(16384 * T)) + 1
+            tp = self.priors["geocent_time"]
+            pad = 2.0 / self.interferometers.sampling_frequency + 1e-3
+            flags = _lib.TIME_MARG | (_lib.JITTER_TIME if self.jitter_time else 0)
+            self._up_event, _ = self._make_event(n_freq=n_freq, flags=flags,
+                                                 time_prior=(tp.minimum - pad, tp.maximum + pad))
+        return self._up_event
+
+    def _time_posterior(self, parameters):
+        """base.py:580-640: times on the 1/16384 s grid from the row's
+        (jittered) geocent time, modulo the segment, inside the prior; the
+        time-conditioned lnL (distance / phase marginalised as configured)
+        times the prior."""
+        p = self._frame(dict(parameters))
+        for key, value in self._defaults().items():
+            p.setdefault(key, value)
+        tg = float(p["geocent_time"]) + (float(p.get("time_jitter", 0.0)) if self.jitter_time
+                                         else 0.0)
+        ev = self._upsampled_time_event()
+        j_first, n_window, t_step = ctypes.c_int64(), ctypes.c_int32(), ctypes.c_double()
+        check(LIB.gwl_event_time_window(ev, ctypes.byref(j_first), ctypes.byref(n_window),
+                                        ctypes.byref(t_step)), "gwl_event_time_window")
+        rows = torch.from_numpy(self._row_builder_sky()(p)[:1]).to("cuda:%d" % self.device)
+        L = int(n_window.value)
+        dh = torch.empty((1, L, 2), dtype=torch.float64, device=rows.device)
+        hh = torch.empty(1, dtype=torch.float64, device=rows.device)
+        st = torch.cuda.current_stream(rows.device).cuda_stream
+        check(LIB.gwl_eval_time_series(ev, ctypes.c_void_p(rows.data_ptr()), 1, _lib.NPARAM,
+                                       _lib.FP64, ctypes.c_void_p(dh.data_ptr()),
+                                       ctypes.c_void_p(hh.data_ptr()), ctypes.c_void_p(st)),
+              "gwl_eval_time_series")
+        T = float(self.interferometers.duration)
+        dh = dh.cpu().numpy()[0]
+        # base.py:604-606 transforms h conj(d) / S without the 4/T of the
+        # inner product (equal for T = 4 s only); kept for identical results
+        d_inner_h = (dh[:, 0] + 1j * dh[:, 1]) * (T / 4.0)
+        h_inner_h = float(hh.cpu().numpy()[0])
+        start = float(self.interferometers.start_time)
+        j = int(j_first.value) + np.arange(L)
+        times = ((tg - start) + j / 16384.0) % T + start
+        prior = self.priors["geocent_time"]
+        inside = (times >= prior.minimum) & (times < prior.maximum)
+        times, d_inner_h = times[inside], d_inner_h[inside]
+        if self.distance_marginalization:
+            log_like = self._distance_marginalized_values(d_inner_h, h_inner_h, p)
+        elif self.phase_marginalization:
+            log_like = ln_i0_host(np.abs(d_inner_h)) - h_inner_h / 2
+        else:
+            log_like = d_inner_h.real - h_inner_h / 2
+        post = np.exp(log_like - np.max(log_like)) * prior.prob(times)
+        return times, post
+
+    def _row_builder_sky(self):
+        return _abi.RowBuilder(self.waveform_generator.parameter_conversion, {})
+
+    def generate_time_sample_from_marginalized_likelihood(self, parameters, rng=None):
+        """base.py:580-640: keep the times above 1/1000 of the posterior's
+        peak (at least three around it) and draw from the interpolated
+        posterior."""
+        times, post = self._time_posterior(parameters)
+        keep = post > np.max(post) / 1000
+        if np.sum(keep) < 3:
+            keep[1:-1] = keep[1:-1] | keep[2:] | keep[:-2]
+        return interped_sample(times[keep], post[keep],
+                               np.random.default_rng() if rng is None else rng)
+
+    def _distance_posterior(self, parameters):
+        """base.py:642-684 (Eq. C31): lnL on the distance grid from <d|h>,
+        <h|h> at the parameters' distance, times the prior."""
+        p = self._frame(dict(parameters))
+        d_inner_h, h_inner_h = self._plain_inner_products(p)
+        ratio = p["luminosity_distance"] / self._distance_array
+        d = d_inner_h * ratio
+        h = h_inner_h * ratio ** 2
+        log_like = ln_i0_host(np.abs(d)) - h / 2 if self.phase_marginalization else d.real - h / 2
+        return self._distance_array, np.exp(log_like - np.max(log_like)) * self.distance_prior_array
+
+    def generate_distance_sample_from_marginalized_likelihood(self, parameters, rng=None):
+        x, post = self._distance_posterior(parameters)
+        return interped_sample(x, post, np.random.default_rng() if rng is None else rng)
+
+    def _phase_posterior(self, parameters):
+        """base.py:712-742 (mu(phi) ~ exp(-2 i phi)): lnL on 101 phases."""
+        p = self._frame(dict(parameters))
+        d_inner_h, h_inner_h = self._plain_inner_products(p)
+        phases = np.linspace(0, 2 * np.pi, 101)
+        log_post = (d_inner_h * np.exp(-2j * phases)).real - h_inner_h / 2
+        return phases, np.exp(log_post - np.max(log_post))
+
+    def generate_phase_sample_from_marginalized_likelihood(self, parameters, rng=None):
+        x, post = self._phase_posterior(parameters)
+        return interped_sample(x, post, np.random.default_rng() if rng is None else rng)
 
     # ------------------------------------------------------------------ API
     @property

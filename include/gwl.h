@@ -133,6 +133,18 @@ int gwl_event_device_bytes(gwl_event* ev, int64_t* bytes);
 int gwl_eval(gwl_event* ev, const double* params, int64_t n_rows, int32_t ld,
              int32_t precision, double* out, double* aux, void* stream);
 
+/* The time-marginalisation arrays of n rows without the marginalisation:
+ * d_inner_h_array = 4/T FFT_N(h conj(d) / S) at the event's time window
+ * (base.py:325-330; bins j_first .. j_first + n_window - 1 of
+ * gwl_event_time_window, transform bin j at shift j T / N from the row's
+ * geocent time) and <h|h>.  For posterior reconstruction of the time
+ * (base.py:580-640).  d_inner_h: device [n][n_window][2]; h_inner_h: device
+ * [n].  The event must marginalise over time. */
+int gwl_eval_time_series(gwl_event* ev, const double* params, int64_t n_rows, int32_t ld,
+                         int32_t precision, double* d_inner_h, double* h_inner_h, void* stream);
+/* The event's time window: first transform bin, number of bins, spacing T / N. */
+int gwl_event_time_window(gwl_event* ev, int64_t* j_first, int32_t* n_window, double* t_step);
+
 /* Tile selection for this event's kernels (0 = built-in heuristic):
  * rows_per_workgroup = parameter rows sharing one workgroup's bin loop
  * (1, 2, 4, 8 or 16), workgroups_per_cu = target resident workgroups per CU

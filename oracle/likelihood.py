@@ -269,6 +269,89 @@ class OracleGWTransient:
             ll = arr.real - h_inner_h / 2
         return logsumexp(ll, b=time_prior_array)
 
+    # ---- posterior reconstruction: base.py:580-742
+    def time_arrays(self, params):
+        """base.py:580-626 line by line: times on a 1/16384 s grid from the
+        (jittered) geocent time modulo the segment, inside [min, max) of the
+        prior; d_inner_h the FFT of the zero-padded h conj(d) / S (no 4/T, as
+        there) at those times; <h|h>."""
+        params = copy.deepcopy(params)
+        ev = self.event
+        if self.jitter_time:
+            params["geocent_time"] += params["time_jitter"]
+        pols = source_polarizations(self.source, ev.frequency_array, params,
+                                    self.waveform_arguments)
+        T = ev.duration
+        n_time_steps = int(T * 16384)
+        times = (params["geocent_time"] - ev.start_time) + np.arange(n_time_steps) / 16384.0
+        times = times % T + ev.start_time
+        prior = self.priors["geocent_time"]
+        in_prior = (times >= prior.minimum) & (times < prior.maximum)
+        times = times[in_prior]
+        d_inner_h = np.zeros(len(times), dtype=complex)
+        h_inner_h = 0.0
+        for det in ev.detectors:
+            n = len(det.strain)
+            signal = detector_response(det, pols, params, ev.start_time,
+                                       antenna_time=self.antenna_time)
+            signal_long = np.zeros(n_time_steps, dtype=complex)
+            data = np.zeros(n_time_steps, dtype=complex)
+            psd = np.ones(n_time_steps)
+            signal_long[:n] = signal
+            data[:n] = np.conj(det.strain)
+            psd[:n][det.mask] = det.psd[det.mask]
+            d_inner_h += np.fft.fft(signal_long * data / psd)[in_prior]
+            m = det.mask
+            h_inner_h += core.noise_weighted_inner_product(signal[m], signal[m], det.psd[m],
+                                                           T).real
+        return times, d_inner_h, h_inner_h, params
+
+    def time_posterior(self, params):
+        """base.py:627-640 before the 1/1000 cut: the time-conditioned lnL
+        (distance / phase marginalised as configured) times the prior."""
+        times, d_inner_h, h_inner_h, params = self.time_arrays(params)
+        prior = self.priors["geocent_time"]
+        if self.distance_marginalization:
+            log_like = self.distance_marginalized_likelihood(d_inner_h, h_inner_h, params)
+        elif self.phase_marginalization:
+            log_like = core.ln_i0(np.abs(d_inner_h)) - h_inner_h / 2
+        else:
+            log_like = d_inner_h.real - h_inner_h / 2
+        return times, np.exp(log_like - np.max(log_like)) * prior.prob(times)
+
+    def plain_inner_products(self, params):
+        """base.py:686-694: <d|h>, <h|h> summed over detectors."""
+        pols = source_polarizations(self.source, self.event.frequency_array, params,
+                                    self.waveform_arguments)
+        dh, hh = 0j, 0.0
+        for det in self.event.detectors:
+            signal = detector_response(det, pols, params, self.event.start_time,
+                                       antenna_time=self.antenna_time)
+            m = det.mask
+            dh += core.noise_weighted_inner_product(signal[m], det.strain[m], det.psd[m],
+                                                    self.duration)
+            hh += core.noise_weighted_inner_product(signal[m], signal[m], det.psd[m],
+                                                    self.duration).real
+        return dh, hh
+
+    def distance_posterior(self, params):
+        """base.py:642-684."""
+        dh, hh = self.plain_inner_products(params)
+        d = dh * params["luminosity_distance"] / self._distance_array
+        h = hh * params["luminosity_distance"] ** 2 / self._distance_array ** 2
+        if self.phase_marginalization:
+            log_like = core.ln_i0(np.abs(d)) - h / 2
+        else:
+            log_like = d.real - h / 2
+        return self._distance_array, np.exp(log_like - np.max(log_like)) * self.distance_prior_array
+
+    def phase_posterior(self, params):
+        """base.py:712-742."""
+        dh, hh = self.plain_inner_products(params)
+        phases = np.linspace(0, 2 * np.pi, 101)
+        log_post = (dh * np.exp(-2j * phases)).real - hh / 2
+        return phases, np.exp(log_post - np.max(log_post))
+
     def noise_log_likelihood(self):
         """base.py:402-413."""
         log_l = 0.0

@@ -1,0 +1,110 @@
+"""Posterior reconstruction of the marginalised parameters (base.py:502-742)
+pinned on the CPU:
+* the oracle's time array (the FFT of the zero-padded h conj(d) / S on the
+  1/16384 s grid) equals (T/4) conj(<d|h_tau>) of the template shifted by
+  tau = t - t_c, from the direct masked inner product, at every grid time
+  near the peak (base.py:604-606 leaves out the 4/T of the inner product);
+* the distance posterior's lnL equals the un-marginalised lnL re-evaluated
+  with the waveform at each distance (1e-9), the phase posterior's equals it
+  with the waveform at each phase, up to the reference's constant;
+* interped_sample (bilby's Interped(xx, yy).sample()) inverts the uniform
+  and triangular densities in closed form."""
+import numpy as np
+import pytest
+
+from bilby_amd.likelihood import interped_sample
+from oracle import core
+from oracle.likelihood import OracleDetector, OracleEvent, OracleGWTransient, detector_response
+
+WF = dict(reference_frequency=50.0, minimum_frequency=20.0)
+
+
+class _U:
+    def __init__(self, a, b):
+        self.minimum, self.maximum = a, b
+
+    def prob(self, x):
+        x = np.asarray(x)
+        return ((x >= self.minimum) & (x <= self.maximum)) / (self.maximum - self.minimum)
+
+    def rescale(self, u):
+        return self.minimum + u * (self.maximum - self.minimum)
+
+
+def _event(T=4.0, fs=1024.0, distance=800.0, seed=4):
+    rng = np.random.default_rng(seed)
+    f = core.frequency_series(fs, T)
+    start = 1126259642.413 - T / 2
+    inj = dict(mass_1=30.0, mass_2=25.0, chi_1=0.0, chi_2=0.0, luminosity_distance=distance,
+               theta_jn=0.5, phase=1.1, ra=1.2, dec=-0.4, psi=0.8, geocent_time=start + T / 2)
+    pols = core.taylorf2(f, inj["mass_1"], inj["mass_2"], inj["luminosity_distance"],
+                         inj["theta_jn"], inj["phase"], **WF)
+    dets = []
+    for name in ("H1", "L1"):
+        psd = core.analytic_aligo_psd(f)
+        noise, _ = core.noise_realisation(lambda x: core.analytic_aligo_psd(x), 0, np.inf,
+                                          len(f), T, rng)
+        mask = core.frequency_mask(f, 20, fs / 2, fs)
+        det = OracleDetector(name, f, noise, psd, mask)
+        det.strain = det.strain + detector_response(det, pols, inj, start) * mask
+        dets.append(det)
+    return OracleEvent(dets, T, fs, start), inj
+
+
+@pytest.mark.parametrize("T", [4.0, 8.0])
+def test_time_array_is_the_shifted_inner_product(T):
+    ev, inj = _event(T=T)
+    tc = inj["geocent_time"]
+    lk = OracleGWTransient(ev, "taylorf2", waveform_arguments=WF, time_marginalization=True,
+                           jitter_time=False, priors={"geocent_time": _U(tc - 0.1, tc + 0.1)})
+    p = dict(inj, geocent_time=tc - 0.003)
+    times, d_inner_h, h_inner_h, _ = lk.time_arrays(p)
+    pols = core.taylorf2(ev.frequency_array, inj["mass_1"], inj["mass_2"],
+                         inj["luminosity_distance"], inj["theta_jn"], inj["phase"], **WF)
+    peak = int(np.argmax(np.abs(d_inner_h)))
+    assert abs(times[peak] - tc) < 2e-3
+    for k in (peak - 7, peak, peak + 3):
+        tau = times[k] - p["geocent_time"]
+        dh = 0j
+        for det in ev.detectors:
+            sig = detector_response(det, pols, p, ev.start_time)
+            sig = sig * np.exp(-2j * np.pi * det.frequency_array * tau)
+            m = det.mask
+            dh += core.noise_weighted_inner_product(sig[m], det.strain[m], det.psd[m], T)
+        assert abs(d_inner_h[k] - T / 4 * np.conj(dh)) < 1e-9 * abs(dh), k
+
+
+def test_distance_and_phase_posteriors_re_evaluate_the_likelihood():
+    ev, inj = _event(distance=600.0)
+    from bilby_amd.prior import UniformComovingVolume
+    pri = {"luminosity_distance": UniformComovingVolume(100.0, 2000.0), "phase": _U(0, 2 * np.pi)}
+    lk = OracleGWTransient(ev, "taylorf2", waveform_arguments=WF, distance_marginalization=True,
+                           phase_marginalization=False, priors=dict(pri))
+    plain = OracleGWTransient(ev, "taylorf2", waveform_arguments=WF)
+    p = dict(inj, luminosity_distance=lk._ref_dist)
+    dist, post = lk.distance_posterior(p)
+    top = int(np.argmax(post))
+    idx = [i for i in (top - 400, top - 100, top, top + 150, top + 600) if post[i] > 1e-250]
+    assert len(idx) >= 4
+    ll = np.log(post[idx] / lk.distance_prior_array[idx])
+    direct = np.array([plain.log_likelihood_ratio(dict(inj, luminosity_distance=dist[i]))
+                       for i in idx])
+    assert np.allclose(ll - ll[0], direct - direct[0], atol=1e-9 * np.max(np.abs(direct)))
+    phases, post = lk.phase_posterior(p)
+    lp = np.log(post[:5])
+    ref = [plain.log_likelihood_ratio(dict(p, phase=inj["phase"] + ph)) for ph in phases[:5]]
+    assert np.allclose(lp[:5] - lp[0], np.array(ref) - ref[0], atol=1e-8)
+
+
+def test_interped_sample_inverts_closed_forms():
+    rng = np.random.default_rng(9)
+    xs = [interped_sample([2.0, 3.0, 5.0], [1.0, 1.0, 1.0], np.random.default_rng(s))
+          for s in range(5)]
+    us = [np.random.default_rng(s).uniform(0, 1) for s in range(5)]
+    assert np.allclose(xs, 2.0 + 3.0 * np.array(us), atol=1e-12)
+    grid = np.linspace(0.0, 1.0, 2001)
+    for s in range(5):
+        x = interped_sample(grid, grid, np.random.default_rng(s))   # density 2x: CDF x^2
+        u = np.random.default_rng(s).uniform(0, 1)
+        assert x == pytest.approx(np.sqrt(u), abs=1e-6)
+    assert 0.0 <= interped_sample(grid, 1 - grid, rng) <= 1.0
