@@ -183,6 +183,11 @@ class ROQGravitationalWaveTransient:
             d["luminosity_distance"] = self._ref_dist
         return d
 
+    def generate_posterior_sample_from_marginalized_likelihood(self, parameters, rng=None):
+        """base.py:502-542 / roq.py:1238-1281: not provided for the ROQ
+        likelihood (its time samples and inner products are not exposed)."""
+        raise NotImplementedError("posterior reconstruction is not provided for the ROQ likelihood")
+
     def row_builder(self):
         """The host-side dict -> rows conversion of this likelihood
         (picklable; what worker-process clients use)."""

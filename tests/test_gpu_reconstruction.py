@@ -86,3 +86,39 @@ def test_reconstruction_matches_oracle(case):
     pr = s["priors"]
     if case["time"]:
         assert pr["geocent_time"].minimum <= out["geocent_time"] <= pr["geocent_time"].maximum
+
+
+def test_relative_binning_reconstructs_distance_and_phase_from_its_inner_products():
+    """The relative-binning likelihood reconstructs distance and phase from
+    its own <d|h>, <h|h> (relative.py calculate_snrs), compared with the
+    relative-binning oracle's; its time reconstruction is not provided."""
+    from bilby_amd.configs import make_taylorf2_marginalised
+    from bilby_amd.relative import RelativeBinningGravitationalWaveTransient
+    from oracle.likelihood import OracleDetector, OracleEvent
+    from oracle.relative import OracleRelativeBinning
+    say("relative binning reconstruction")
+    s = make_taylorf2_marginalised(n_draws=8, seed=5, duration=4.0, sampling_frequency=2048.0,
+                                   detectors=("H1", "L1", "V1"), time=False)
+    ifos = s["interferometers"]
+    inj = dict(s["injection"])
+    lk = RelativeBinningGravitationalWaveTransient(
+        ifos, s["waveform_generator"], fiducial_parameters=inj, distance_marginalization=True,
+        phase_marginalization=True, priors=s["priors"].copy(), epsilon=0.1, precision="fp64")
+    dets = [OracleDetector(i.name, i.frequency_array, i.frequency_domain_strain,
+                           i.power_spectral_density_array, i.frequency_mask) for i in ifos]
+    ev = OracleEvent(dets, ifos.duration, ifos.sampling_frequency, ifos.start_time)
+    olk = OracleRelativeBinning(ev, "taylorf2", inj,
+                                waveform_arguments=dict(s["waveform_generator"].waveform_arguments),
+                                epsilon=0.1)
+    q = dict(inj, luminosity_distance=lk._ref_dist, phase=0.0)
+    dh, hh = 0j, 0.0
+    for det in ev.detectors:
+        ratio = olk.edge_response(det, q) / olk.fiducial_points[det.name]
+        a, b = olk.snrs_from_ratio(det, ratio)
+        dh, hh = dh + a, hh + b
+    g_dh, g_hh = lk._plain_inner_products(q)
+    assert abs(g_dh - dh) < 1e-9 * abs(dh) and abs(g_hh - hh) < 1e-9 * hh
+    x, post = lk._distance_posterior(q)
+    assert np.isfinite(post).all() and post.max() > 0
+    out = lk.generate_posterior_sample_from_marginalized_likelihood(q, rng=np.random.default_rng(1))
+    assert out["luminosity_distance"] != q["luminosity_distance"] and out["phase"] != 0.0
