@@ -51,25 +51,31 @@ __global__ __launch_bounds__(128) void k_sample_const(EvDev ev, const double* pa
   out[r] = sc;
 }
 
+// A workgroup evaluates S rows over the chunks blockIdx.x, blockIdx.x +
+// gridDim.x, ... of chunk bins each; every chunk's sums go to its own
+// partials, so the values do not depend on the grid.  In natural order
+// (no time marginalisation) a chunk outside the union of the rows' supports
+// [kb, ke] is skipped and its partials are not written: the epilogue sums
+// only the chunks that meet a row's own support, which that row's tile
+// always computes.
+// The sums of one chunk for the S rows of a tile (see k_reduce).
 template <typename R, int WAVE, int ND, int S, int NT>
-__global__ __launch_bounds__(NT) void k_reduce(RedArgs a) {
+__device__ __forceinline__ void reduce_chunk(const RedArgs& a, int chunk, long long row0,
+                                             const SampleConst* sc, const SampleFast<R>* sf,
+                                             const int* band, double (*red)[S][3]) {
   using C = cplx<R>;
   const EvDev& ev = a.ev;
-  __shared__ SampleConst sc[S];
-  __shared__ SampleFast<R> sf[S];
-  __shared__ int band[2];
-  __shared__ double red[NT / 64][S][3];
   const int tid = threadIdx.x;
-  const long long row0 = (long long)blockIdx.y * S;
-  const int chunk = blockIdx.x;
-  tile_prologue_load<R, S>(ev, a.rows, a.n, row0, sc, sf, band);
-
+  const int lane = tid & 63, wid = tid >> 6;
+  const int i0 = chunk * a.chunk;
+  const int i1 = min(i0 + a.chunk, ev.N);
+  if (ev.Q == 1) {
+    const int b1 = chunk == a.nchunk - 1 ? ev.N : i1 - 1;  // the last also holds Nyquist
+    if (band[1] < i0 || band[0] > b1) return;
+  }
   R acc_r[S], acc_i[S], acc_h[S];
 #pragma unroll
   for (int s = 0; s < S; ++s) acc_r[s] = acc_i[s] = acc_h[s] = 0;
-
-  const int i0 = chunk * a.chunk;
-  const int i1 = min(i0 + a.chunk, ev.N);
   for (int i = i0 + tid; i < i1; i += NT) {
     const int n = ev.Q * (i & (ev.P - 1)) + (i >> ev.logP);
     if (n < band[0] || n > band[1]) continue;
@@ -119,7 +125,6 @@ __global__ __launch_bounds__(NT) void k_reduce(RedArgs a) {
       }
     }
   }
-  const int lane = tid & 63, wid = tid >> 6;
 #pragma unroll
   for (int s = 0; s < S; ++s) {
     double x = wave_sum(dr[s]), y = wave_sum(di[s]), z = wave_sum(dh[s]);
@@ -137,6 +142,31 @@ __global__ __launch_bounds__(NT) void k_reduce(RedArgs a) {
       double v = 0.0;
       for (int w = 0; w < NT / 64; ++w) v += red[w][s][c];
       a.part[(r * a.nchunk + chunk) * 3 + c] = v;
+    }
+  }
+}
+
+// A workgroup evaluates S rows over one chunk (LOOP false: chunk blockIdx.x)
+// or over the chunks blockIdx.x, blockIdx.x + gridDim.x, ... (LOOP true);
+// every chunk's sums go to its own partials, so the values do not depend
+// on the grid.  In natural order (no time marginalisation) a chunk outside
+// the union of the rows' supports [kb, ke] is skipped and its partials are
+// not written: the epilogue sums only the chunks that meet a row's own
+// support, which that row's tile always computes.
+template <typename R, int WAVE, int ND, int S, int NT, bool LOOP>
+__global__ __launch_bounds__(NT) void k_reduce(RedArgs a) {
+  __shared__ SampleConst sc[S];
+  __shared__ SampleFast<R> sf[S];
+  __shared__ int band[2];
+  __shared__ double red[NT / 64][S][3];
+  const long long row0 = (long long)blockIdx.y * S;
+  tile_prologue_load<R, S>(a.ev, a.rows, a.n, row0, sc, sf, band);
+  if constexpr (!LOOP) {
+    reduce_chunk<R, WAVE, ND, S, NT>(a, blockIdx.x, row0, sc, sf, band, red);
+  } else {
+    for (int chunk = blockIdx.x; chunk < a.nchunk; chunk += gridDim.x) {
+      reduce_chunk<R, WAVE, ND, S, NT>(a, chunk, row0, sc, sf, band, red);
+      __syncthreads();  // red is reused by the next chunk
     }
   }
 }
@@ -314,6 +344,7 @@ struct EpiArgs {
   int ld;
   long long n;
   int nchunk;          // k_reduce chunks
+  int chunk;           // bins per chunk
   const double* part3; // [n][nchunk][3]
   int G;
   const double2* part; // [n][G][L]
@@ -340,7 +371,13 @@ __global__ __launch_bounds__(128) void k_epi_plain(EpiArgs a) {
     compute_sample_const(ev, a.params + r * a.ld, sc);
   double dr = 0.0, di = 0.0, hh = 0.0;
   const double* p = a.part3 + (size_t)r * a.nchunk * 3;
-  for (int c = 0; c < a.nchunk; ++c) {
+  // the chunks that meet the row's support (natural order), else all
+  int c_lo = 0, c_hi = a.nchunk - 1;
+  if (ev.Q == 1) {
+    c_lo = sc.kb <= sc.ke ? sc.kb / a.chunk : 1;
+    c_hi = sc.kb <= sc.ke ? min(a.nchunk - 1, sc.ke / a.chunk) : 0;
+  }
+  for (int c = c_lo; c <= c_hi; ++c) {
     dr += p[3 * c + 0];
     di += p[3 * c + 1];
     hh += p[3 * c + 2];
@@ -733,18 +770,35 @@ static int eval_typed(gwl_event* e, const double* params, int64_t n, int ld, dou
   ra.rows = reinterpret_cast<const SampleConst*>(
       reinterpret_cast<char*>(e->ws) + reduce_part_bytes(ev, n));
   const long long nblk = (n + S - 1) / S;
+  // chunk groups per tile.  A source whose rows span the whole band (the
+  // sine-Gaussian) keeps every chunk in parallel; TaylorF2 rows end at
+  // their ISCO frequency, so most chunks lie outside every row's support
+  // and fewer, longer workgroups (one per tile once the tiles alone fill
+  // the CUs several times) avoid launching workgroups that skip them all
+  const long long target = 8LL * e->ctx->n_cu;
+  const int G = WAVE != GWL_TAYLORF2
+                    ? ra.nchunk
+                    : (int)std::max<long long>(1, std::min<long long>(
+                                                      ra.nchunk, (target + nblk - 1) / nblk));
   if (int rc = timing_begin(e, st)) return rc;
   hipLaunchKernelGGL(k_sample_const, dim3((unsigned)((n + 127) / 128)), dim3(128), 0, st, ev,
                      params, ld, (long long)n, const_cast<SampleConst*>(ra.rows));
-  if (S == 16)
-    hipLaunchKernelGGL((k_reduce<R, WAVE, ND, 16, NT>), dim3(ra.nchunk, (unsigned)nblk), dim3(NT), 0,
-                       st, ra);
-  else if (S == 4)
-    hipLaunchKernelGGL((k_reduce<R, WAVE, ND, 4, NT>), dim3(ra.nchunk, (unsigned)nblk), dim3(NT), 0,
-                       st, ra);
-  else
-    hipLaunchKernelGGL((k_reduce<R, WAVE, ND, 8, NT>), dim3(ra.nchunk, (unsigned)nblk), dim3(NT), 0,
-                       st, ra);
+  const dim3 grid(G, (unsigned)nblk);
+  if (G == ra.nchunk) {
+    if (S == 16)
+      hipLaunchKernelGGL((k_reduce<R, WAVE, ND, 16, NT, false>), grid, dim3(NT), 0, st, ra);
+    else if (S == 4)
+      hipLaunchKernelGGL((k_reduce<R, WAVE, ND, 4, NT, false>), grid, dim3(NT), 0, st, ra);
+    else
+      hipLaunchKernelGGL((k_reduce<R, WAVE, ND, 8, NT, false>), grid, dim3(NT), 0, st, ra);
+  } else {
+    if (S == 16)
+      hipLaunchKernelGGL((k_reduce<R, WAVE, ND, 16, NT, true>), grid, dim3(NT), 0, st, ra);
+    else if (S == 4)
+      hipLaunchKernelGGL((k_reduce<R, WAVE, ND, 4, NT, true>), grid, dim3(NT), 0, st, ra);
+    else
+      hipLaunchKernelGGL((k_reduce<R, WAVE, ND, 8, NT, true>), grid, dim3(NT), 0, st, ra);
+  }
   GWL_HIP_CHECK(hipGetLastError());
   if (int rc = timing_end(e, st)) return rc;
   EpiArgs ea{};
@@ -753,6 +807,7 @@ static int eval_typed(gwl_event* e, const double* params, int64_t n, int ld, dou
   ea.ld = ld;
   ea.n = n;
   ea.nchunk = ra.nchunk;
+  ea.chunk = ra.chunk;
   ea.part3 = ra.part;
   ea.out = out;
   ea.aux = aux;
