@@ -151,6 +151,16 @@ def _load():
                                    ctypes.POINTER(ctypes.c_int32),
                                    ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
         "gwl_relbin_set_time": ([vp, ctypes.POINTER(ctypes.c_int64), _dp], ctypes.c_int),
+        "gwl_relbin_time_series": ([vp, vp, ctypes.c_int64, ctypes.c_int32, ctypes.c_int32, vp,
+                                    vp, vp], ctypes.c_int),
+        "gwl_relbin_set_upsampled": ([vp, ctypes.POINTER(ctypes.c_int64), _dp, _dp,
+                                      ctypes.c_int64], ctypes.c_int),
+        "gwl_relbin_upsampled_series": ([vp, vp, ctypes.c_int64, ctypes.c_int32, ctypes.c_int64,
+                                         ctypes.c_int32, vp, vp, vp], ctypes.c_int),
+        "gwl_mb_time_series": ([vp, vp, ctypes.c_int64, ctypes.c_int32, ctypes.c_int32, vp, vp,
+                                vp], ctypes.c_int),
+        "gwl_roq_inner_products": ([vp, vp, ctypes.c_int64, ctypes.c_int32, ctypes.c_int32,
+                                    ctypes.c_int32, vp, vp, vp], ctypes.c_int),
         "gwl_mb_reserve": ([vp, ctypes.c_int64], ctypes.c_int),
         "gwl_mb_device_bytes": ([vp, ctypes.POINTER(ctypes.c_int64)], ctypes.c_int),
         "gwl_mb_eval": ([vp, vp, ctypes.c_int64, ctypes.c_int32, ctypes.c_int32, vp, vp, vp],
@@ -211,5 +221,6 @@ EXPORTED_SYMBOLS = [
     "gwl_calib_destroy", "gwl_calib_reserve", "gwl_calib_device_bytes", "gwl_calib_eval",
     "gwl_mb_create", "gwl_mb_destroy", "gwl_mb_reserve", "gwl_mb_device_bytes", "gwl_mb_eval",
     "gwl_mb_set_time", "gwl_relbin_set_time", "gwl_eval_time_series", "gwl_event_time_window",
-    "gwl_roq_weights_linear",
+    "gwl_roq_weights_linear", "gwl_relbin_time_series", "gwl_mb_time_series",
+    "gwl_roq_inner_products", "gwl_relbin_set_upsampled", "gwl_relbin_upsampled_series",
 ]

@@ -59,6 +59,18 @@ struct gwl_relbin {
   bool has_win = false;
   void* tws = nullptr;        // GEMM operand and output of one row tile
   size_t tws_bytes = 0;
+  // posterior reconstruction of the time on the 1/16384 s grid
+  // (gwl_relbin_set_upsampled): the integrand, the <h|h> weights and the
+  // bin of each grid bin on [edges[0], edges[nb]], and the basis of the
+  // last window asked for
+  void* up_data = nullptr;
+  double2* up_x = nullptr;    // [nd][n_x] h0 conj(d) / S (zero off the mask)
+  double* up_y = nullptr;     // [nd][n_x] 4/T |h0|^2 / S on the mask
+  int* up_bin = nullptr;      // [n_x]
+  long long* up_edges = nullptr;
+  long long up_nx = 0, up_nfft = 0, up_edges_first = 0;
+  gwl::WinBasis up;
+  bool has_up = false;
 };
 
 // Multibanded likelihood (bilby/gw/likelihood/multiband.py:728-807, linear
@@ -490,7 +502,8 @@ const int* valid_of(const RelRow* rows) {
 template <typename Launch>
 int eval_time_tiles(gwl_event* ev, const WinBasis& win, void** ws, size_t* ws_bytes, void** tws,
                     size_t* tws_bytes, RelbinArgs a, const double* params, long long n, int ld,
-                    bool fp64, double* out, double* aux, hipStream_t st, Launch kernel) {
+                    bool fp64, double* out, double* aux, hipStream_t st, Launch kernel,
+                    double* sdh = nullptr, double* shh = nullptr) {
   const long long tile = std::min<long long>(n, kWinTileRows);
   if (int rc = grow_ws(ev, ws, ws_bytes, tile)) return rc;
   if (int rc = grow_buf(ev, tws, tws_bytes, win_ws_bytes(win, tile))) return rc;
@@ -505,7 +518,7 @@ int eval_time_tiles(gwl_event* ev, const WinBasis& win, void** ws, size_t* ws_by
     const long long nt = std::min(tile, n - r0);
     a.params = params + r0 * ld;
     a.n = nt;
-    a.out = out + r0;
+    a.out = out ? out + r0 : nullptr;
     a.aux = aux ? aux + 3 * r0 : nullptr;
     if (int rc = win_clear(win, opa, fp64, nt, st)) return rc;
     const unsigned g128 = (unsigned)((nt + 127) / 128);
@@ -517,10 +530,44 @@ int eval_time_tiles(gwl_event* ev, const WinBasis& win, void** ws, size_t* ws_by
       GWL_HIP_CHECK(hipGetLastError());
     }
     if (int rc = win_finish(ev, win, opa, fp64, c, nt, a.params, ld, a.sums, valid_of(a.rows),
-                            kValidStride, a.out, st))
+                            kValidStride, a.out, st, sdh ? sdh + 2 * r0 * win.L : nullptr,
+                            sdh ? shh + r0 : nullptr))
       return rc;
   }
   return GWL_OK;
+}
+
+// <h|h> of the rebuilt full waveform (relative.py:380-393) on the mask,
+// base.py:629 (optimal_snr_squared of the rebuilt signal), one workgroup
+// per row: sum over detectors and grid bins of w |r(f)|^2 with r(f) the
+// edge ratios of the row's operand (float64, column d (nb + 1) + e, the
+// imaginary part K columns on) interpolated linearly across the bin.
+__global__ __launch_bounds__(256) void k_relbin_full_hh(const double* A, int Kp, int K, int nb,
+                                                         int nd, long long n_x, long long e0,
+                                                         double df, const double* fe,
+                                                         const int* bin_of, const double* y,
+                                                         double* hh) {
+  __shared__ double red[4];
+  const double* ar = A + (size_t)blockIdx.x * Kp;
+  double s = 0.0;
+  for (long long t = threadIdx.x; t < (long long)nd * n_x; t += blockDim.x) {
+    const int d = (int)(t / n_x);
+    const long long i = t - (long long)d * n_x;
+    const int b = bin_of[i];
+    const double wy = y[t];
+    if (b < 0 || wy == 0.0) continue;
+    const double f = (double)(e0 + i) * df;
+    const double w = fe[b + 1] - fe[b];
+    const double lo = (fe[b + 1] - f) / w, hi = (f - fe[b]) / w;
+    const int k = d * (nb + 1) + b;
+    const double rr = ar[k] * lo + ar[k + 1] * hi;
+    const double ri = ar[K + k] * lo + ar[K + k + 1] * hi;
+    s += wy * (rr * rr + ri * ri);
+  }
+  s = reduce_wave(s);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) hh[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
 }
 
 }  // namespace
@@ -626,6 +673,8 @@ int gwl_relbin_destroy(gwl_relbin* rb) {
   if (rb->ws) (void)hipFree(rb->ws);
   if (rb->tws) (void)hipFree(rb->tws);
   gwl::win_basis_free(&rb->win);
+  gwl::win_basis_free(&rb->up);
+  if (rb->up_data) (void)hipFree(rb->up_data);
   delete rb;
   return GWL_OK;
 }
@@ -641,10 +690,11 @@ int gwl_relbin_device_bytes(gwl_relbin* rb, int64_t* bytes) {
   return GWL_OK;
 }
 
-int gwl_relbin_eval(gwl_relbin* rb, const double* params, int64_t n, int32_t ld, int32_t precision,
-                    double* out, double* aux, void* stream) {
+static int relbin_eval_impl(gwl_relbin* rb, const double* params, int64_t n, int32_t ld,
+                          int32_t precision, double* out, double* aux, double* sdh, double* shh,
+                          void* stream, gwl::WinBasis* basis = nullptr) {
   if (rb && n == 0) return GWL_OK;  // an empty batch needs no buffers
-  if (!rb || !params || !out || n < 0 || ld < GWL_NPARAM ||
+  if (!rb || !params || (!out && !sdh) || (sdh && !shh) || n < 0 || ld < GWL_NPARAM ||
       (precision != GWL_FP32 && precision != GWL_FP64)) {
     set_error("gwl_relbin_eval: invalid argument");
     return GWL_E_INVALID;
@@ -656,6 +706,10 @@ int gwl_relbin_eval(gwl_relbin* rb, const double* params, int64_t n, int32_t ld,
     return GWL_E_INVALID;
   }
   const bool timem = (ev->dev.flags & GWL_TIME_MARG) != 0;
+  if (sdh && !timem) {
+    set_error("gwl_relbin_time_series: the event does not marginalise over time");
+    return GWL_E_UNSUPPORTED;
+  }
   if (timem && !rb->has_win) {
     set_error("gwl_relbin_eval: the event marginalises over time; call gwl_relbin_set_time first");
     return GWL_E_INVALID;
@@ -687,16 +741,18 @@ int gwl_relbin_eval(gwl_relbin* rb, const double* params, int64_t n, int32_t ld,
   if (int rc = gwl::timing_begin(ev, st)) return rc;
   if (timem) {
     const bool fp64 = precision == GWL_FP64;
-    a.win_k = rb->win.K;
-    a.win_kp = rb->win.Kp;
+    gwl::WinBasis& win = basis ? *basis : rb->win;
+    a.win_k = win.K;
+    a.win_kp = win.Kp;
     const int nd = ev->dev.nd;
     const int rc = gwl::eval_time_tiles(
-        ev, rb->win, &rb->ws, &rb->ws_bytes, &rb->tws, &rb->tws_bytes, a, params, n, ld, fp64, out,
+        ev, win, &rb->ws, &rb->ws_bytes, &rb->tws, &rb->tws_bytes, a, params, n, ld, fp64, out,
         aux, st, [&](gwl::RelbinArgs t, void* opa) {
           t.win_a = opa;
           return fp64 ? gwl::launch_relbin_nd<double>(t, nd, st)
                       : gwl::launch_relbin_nd<float>(t, nd, st);
-        });
+        },
+        sdh, shh);
     if (rc != GWL_OK) return rc;
     return gwl::timing_end(ev, st);
   }
@@ -709,6 +765,131 @@ int gwl_relbin_eval(gwl_relbin* rb, const double* params, int64_t n, int32_t ld,
   hipLaunchKernelGGL(gwl::k_relbin_epi, dim3(g128), dim3(128), lds, st, a);
   GWL_HIP_CHECK(hipGetLastError());
   return gwl::timing_end(ev, st);
+}
+
+int gwl_relbin_eval(gwl_relbin* rb, const double* params, int64_t n, int32_t ld, int32_t precision,
+                    double* out, double* aux, void* stream) {
+  return relbin_eval_impl(rb, params, n, ld, precision, out, aux, nullptr, nullptr, stream);
+}
+
+int gwl_relbin_time_series(gwl_relbin* rb, const double* params, int64_t n, int32_t ld,
+                           int32_t precision, double* d_inner_h, double* h_inner_h, void* stream) {
+  if (rb && n > 0 && (!d_inner_h || !h_inner_h)) {
+    set_error("gwl_relbin_time_series: invalid argument");
+    return GWL_E_INVALID;
+  }
+  return relbin_eval_impl(rb, params, n, ld, precision, nullptr, nullptr, d_inner_h, h_inner_h,
+                          stream);
+}
+
+int gwl_relbin_set_upsampled(gwl_relbin* rb, const int64_t* bin_index, const double* integrand,
+                             const double* weight, int64_t n_fft) {
+  if (!rb || !bin_index || !integrand || !weight) {
+    set_error("gwl_relbin_set_upsampled: invalid argument");
+    return GWL_E_INVALID;
+  }
+  gwl_event* ev = rb->ev;
+  const gwl::EvDev& d = ev->dev;
+  const int nb = rb->nb;
+  for (int e = 0; e <= nb; ++e)
+    if (bin_index[e] < 0 || bin_index[e] > d.N || (e > 0 && bin_index[e] <= bin_index[e - 1])) {
+      set_error("gwl_relbin_set_upsampled: bin indices must increase within the grid");
+      return GWL_E_INVALID;
+    }
+  if (n_fft <= d.N) {
+    set_error("gwl_relbin_set_upsampled: n_fft must exceed the event's frequency bins");
+    return GWL_E_INVALID;
+  }
+  const long long e0 = bin_index[0], n_x = bin_index[nb] - e0 + 1;
+  std::vector<int> bin_of((size_t)n_x, -1);
+  for (int b = 0; b < nb; ++b) {
+    const long long hi = b == nb - 1 ? bin_index[b + 1] : bin_index[b + 1] - 1;
+    for (long long n = bin_index[b]; n <= hi; ++n) bin_of[(size_t)(n - e0)] = b;
+  }
+  const size_t bx = (size_t)d.nd * n_x * sizeof(double2), by = (size_t)d.nd * n_x * 8;
+  const size_t bb = (size_t)n_x * 4, be = (size_t)(nb + 1) * 8;
+  auto al = [](size_t x) { return (x + 255) / 256 * 256; };
+  GWL_HIP_CHECK(hipSetDevice(ev->ctx->device));
+  void* mem = nullptr;
+  if (hipMalloc(&mem, al(bx) + al(by) + al(bb) + be) != hipSuccess) {
+    set_error("hipMalloc failed for the upsampled time set-up");
+    return GWL_E_NOMEM;
+  }
+  char* m = static_cast<char*>(mem);
+  hipError_t err = hipMemcpy(m, integrand, bx, hipMemcpyHostToDevice);
+  if (err == hipSuccess) err = hipMemcpy(m + al(bx), weight, by, hipMemcpyHostToDevice);
+  if (err == hipSuccess)
+    err = hipMemcpy(m + al(bx) + al(by), bin_of.data(), bb, hipMemcpyHostToDevice);
+  if (err == hipSuccess)
+    err = hipMemcpy(m + al(bx) + al(by) + al(bb), bin_index, be, hipMemcpyHostToDevice);
+  if (err != hipSuccess) {
+    (void)hipFree(mem);
+    return gwl::hip_fail(err, "gwl_relbin_set_upsampled upload");
+  }
+  gwl::win_basis_free(&rb->up);
+  if (rb->up_data) (void)hipFree(rb->up_data);
+  rb->up_data = mem;
+  rb->up_x = reinterpret_cast<double2*>(m);
+  rb->up_y = reinterpret_cast<double*>(m + al(bx));
+  rb->up_bin = reinterpret_cast<int*>(m + al(bx) + al(by));
+  rb->up_edges = reinterpret_cast<long long*>(m + al(bx) + al(by) + al(bb));
+  rb->up_nx = n_x;
+  rb->up_nfft = n_fft;
+  rb->up_edges_first = e0;
+  rb->has_up = true;
+  return GWL_OK;
+}
+
+int gwl_relbin_upsampled_series(gwl_relbin* rb, const double* params, int64_t n, int32_t ld,
+                                int64_t j_first, int32_t n_window, double* d_inner_h,
+                                double* h_inner_h, void* stream) {
+  if (rb && n == 0) return GWL_OK;
+  if (!rb || !params || !d_inner_h || !h_inner_h || n < 0 || ld < GWL_NPARAM || j_first < 0 ||
+      n_window < 1) {
+    set_error("gwl_relbin_upsampled_series: invalid argument");
+    return GWL_E_INVALID;
+  }
+  if (!rb->has_up || !rb->has_win) {
+    set_error("gwl_relbin_upsampled_series: call gwl_relbin_set_time and "
+              "gwl_relbin_set_upsampled first");
+    return GWL_E_INVALID;
+  }
+  if (n > gwl::kWinTileRows) {
+    set_error("gwl_relbin_upsampled_series: at most 8192 rows per call");
+    return GWL_E_INVALID;
+  }
+  if (j_first + (long long)n_window > 2 * rb->up_nfft) {
+    set_error("gwl_relbin_upsampled_series: the window runs past one wrap of the transform");
+    return GWL_E_INVALID;
+  }
+  gwl_event* ev = rb->ev;
+  if (!rb->up.data || rb->up.j_first != j_first || rb->up.L != n_window) {
+    gwl::win_basis_free(&rb->up);
+    gwl::WinBasis wb;
+    wb.L = n_window;
+    wb.n_fft = rb->up_nfft;
+    wb.j_first = j_first;
+    wb.start = ev->dev.start;
+    wb.offset = 0;
+    wb.t_step = 1.0 / ((double)rb->up_nfft * ev->dev.df);
+    wb.delta_tc = wb.t_step;
+    GWL_HIP_CHECK(hipSetDevice(ev->ctx->device));
+    if (int rc = gwl::win_basis_relbin(ev, ev->dev.nd, rb->nb, rb->up_edges, rb->fe, rb->up_x,
+                                       rb->up_nx, &wb))
+      return rc;
+    rb->up = wb;
+  }
+  if (int rc = relbin_eval_impl(rb, params, n, ld, GWL_FP64, nullptr, nullptr, d_inner_h,
+                                h_inner_h, stream, &rb->up))
+    return rc;
+  // the exact <h|h> of the rebuilt waveform replaces the binned one
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  hipLaunchKernelGGL(gwl::k_relbin_full_hh, dim3((unsigned)n), dim3(256), 0, st,
+                     static_cast<const double*>(gwl::win_a(rb->up, rb->tws)), rb->up.Kp,
+                     rb->up.K, rb->nb, ev->dev.nd, rb->up_nx, rb->up_edges_first, ev->dev.df,
+                     rb->fe, rb->up_bin, rb->up_y, h_inner_h);
+  GWL_HIP_CHECK(hipGetLastError());
+  return GWL_OK;
 }
 
 int gwl_relbin_set_time(gwl_relbin* rb, const int64_t* bin_index, const double* integrand) {
@@ -856,10 +1037,11 @@ int gwl_mb_device_bytes(gwl_mb* mb, int64_t* bytes) {
   return GWL_OK;
 }
 
-int gwl_mb_eval(gwl_mb* mb, const double* params, int64_t n, int32_t ld, int32_t precision,
-                double* out, double* aux, void* stream) {
+static int mb_eval_impl(gwl_mb* mb, const double* params, int64_t n, int32_t ld,
+                          int32_t precision, double* out, double* aux, double* sdh, double* shh,
+                          void* stream) {
   if (mb && n == 0) return GWL_OK;  // an empty batch needs no buffers
-  if (!mb || !params || !out || n < 0 || ld < GWL_NPARAM ||
+  if (!mb || !params || (!out && !sdh) || (sdh && !shh) || n < 0 || ld < GWL_NPARAM ||
       (precision != GWL_FP32 && precision != GWL_FP64)) {
     set_error("gwl_mb_eval: invalid argument");
     return GWL_E_INVALID;
@@ -875,6 +1057,10 @@ int gwl_mb_eval(gwl_mb* mb, const double* params, int64_t n, int32_t ld, int32_t
     return GWL_E_INVALID;
   }
   const bool timem = (ev->dev.flags & GWL_TIME_MARG) != 0;
+  if (sdh && !timem) {
+    set_error("gwl_mb_time_series: the event does not marginalise over time");
+    return GWL_E_UNSUPPORTED;
+  }
   if (timem && !mb->has_win) {
     set_error("gwl_mb_eval: the event marginalises over time; call gwl_mb_set_time first");
     return GWL_E_INVALID;
@@ -926,7 +1112,8 @@ int gwl_mb_eval(gwl_mb* mb, const double* params, int64_t n, int32_t ld, int32_t
           mt.win_a = opa;
           mt.win_kp = mb->win.Kp;
           return fp64 ? gwl::launch_mb_nd<double>(mt, nd, st) : gwl::launch_mb_nd<float>(mt, nd, st);
-        });
+        },
+        sdh, shh);
     if (rc != GWL_OK) return rc;
     return gwl::timing_end(ev, st);
   }
@@ -939,6 +1126,20 @@ int gwl_mb_eval(gwl_mb* mb, const double* params, int64_t n, int32_t ld, int32_t
   hipLaunchKernelGGL(gwl::k_relbin_epi, dim3(g128), dim3(128), lds, st, a);
   GWL_HIP_CHECK(hipGetLastError());
   return gwl::timing_end(ev, st);
+}
+
+int gwl_mb_eval(gwl_mb* mb, const double* params, int64_t n, int32_t ld, int32_t precision,
+                double* out, double* aux, void* stream) {
+  return mb_eval_impl(mb, params, n, ld, precision, out, aux, nullptr, nullptr, stream);
+}
+
+int gwl_mb_time_series(gwl_mb* mb, const double* params, int64_t n, int32_t ld, int32_t precision,
+                       double* d_inner_h, double* h_inner_h, void* stream) {
+  if (mb && n > 0 && (!d_inner_h || !h_inner_h)) {
+    set_error("gwl_mb_time_series: invalid argument");
+    return GWL_E_INVALID;
+  }
+  return mb_eval_impl(mb, params, n, ld, precision, nullptr, nullptr, d_inner_h, h_inner_h, stream);
 }
 
 int gwl_mb_set_time(gwl_mb* mb, const gwl_mb_time_desc* td) {

@@ -440,6 +440,40 @@ __device__ inline double roq_grid_time(double tcmin, double dtc, int j) {
   return (tcmin + dtc / 2.0) + (double)j * dtc;
 }
 
+// The interpolated, conj(C_d)-weighted detector sum of <d|h> at ifo time
+// (toff + dt_d) + extra for each detector; false where the five samples
+// leave the weights' time grid.  One time (roq.py:512-526):
+// a = (t_{c+1} - t) / (t_{c-1} - t_{c-2}) on the samples themselves; the
+// time-marginalisation array (roq.py:636-651): b = (t - t_0) / dt - c.
+template <typename T>
+__device__ __forceinline__ bool roq_dh_at(const RoqDev& rd, const T* zr, const RoqRow& rr,
+                                          double toff, double extra, bool single, double& sr,
+                                          double& si) {
+  sr = si = 0.0;
+#pragma unroll
+  for (int d = 0; d < GWL_MAX_DET; ++d) {
+    if (d >= rd.nd) break;
+    const double ifo_t = (toff + rr.dt[d]) + extra;
+    const double x = (ifo_t - rd.ts0) / rd.dts;
+    const double cl = floor(x);
+    const long long ci = (long long)cl;
+    if (ci - 2 < 0 || ci + 2 >= rd.NT) return false;
+    double a;
+    if (single) {
+      const int c = (int)ci;
+      a = (rd.ts[c + 1] - ifo_t) / fmax(rd.ts[c - 1] - rd.ts[c - 2], 1e-12);
+    } else {
+      a = 1.0 - (x - cl);
+    }
+    double vr, vi;
+    five_sample(zr, d * rd.NT + (int)ci - 2, a, vr, vi);
+    // Y_d = conj(C_d) Z_d
+    sr += rr.cre[d] * vr - rr.cim[d] * vi;
+    si += rr.cre[d] * vi + rr.cim[d] * vr;
+  }
+  return true;
+}
+
 template <typename T>
 __global__ __launch_bounds__(64 * kRoqEpiRows) void k_roq_epi(RoqDev rd, const T* Z,
                                                               const RoqRow* rows, long long n,
@@ -492,37 +526,8 @@ __global__ __launch_bounds__(64 * kRoqEpiRows) void k_roq_epi(RoqDev rd, const T
     if (rd.flags & GWL_PHASE_MARG) return ln_i0(hypot(xr, xi)) - hh / 2.0;
     return xr - hh / 2.0;
   };
-  // interpolated, conj(C_d)-weighted detector sum at ifo time
-  // (toff + dt_d) + extra for each detector.  One time (roq.py:512-526):
-  // a = (t_{c+1} - t) / (t_{c-1} - t_{c-2}) on the samples themselves; the
-  // time-marginalisation array (roq.py:636-651): b = (t - t_0) / dt - c.
   auto dh_at = [&](double toff, double extra, bool single, bool& ok, double& sr, double& si) {
-    sr = si = 0.0;
-    ok = true;
-#pragma unroll
-    for (int d = 0; d < GWL_MAX_DET; ++d) {
-      if (d >= rd.nd) break;
-      const double ifo_t = (toff + rr.dt[d]) + extra;
-      const double x = (ifo_t - rd.ts0) / rd.dts;
-      const double cl = floor(x);
-      const long long ci = (long long)cl;
-      if (ci - 2 < 0 || ci + 2 >= rd.NT) {
-        ok = false;
-        return;
-      }
-      double a;
-      if (single) {
-        const int c = (int)ci;
-        a = (rd.ts[c + 1] - ifo_t) / fmax(rd.ts[c - 1] - rd.ts[c - 2], 1e-12);
-      } else {
-        a = 1.0 - (x - cl);
-      }
-      double vr, vi;
-      five_sample(zr, d * rd.NT + (int)ci - 2, a, vr, vi);
-      // Y_d = conj(C_d) Z_d
-      sr += rr.cre[d] * vr - rr.cim[d] * vi;
-      si += rr.cre[d] * vi + rr.cim[d] * vr;
-    }
+    ok = roq_dh_at(rd, zr, rr, toff, extra, single, sr, si);
   };
   if (!(rd.flags & GWL_TIME_MARG)) {
     if (lane == 0) {
@@ -567,9 +572,98 @@ __global__ __launch_bounds__(64 * kRoqEpiRows) void k_roq_epi(RoqDev rd, const T
   }
 }
 
+// One thread per (row, output): <d|h> at the row's own time (n_times = 0,
+// roq.py:512-526) or on the time-marginalisation grid (roq.py:536-540,
+// 604-651), NaN where the five samples leave the weights' grid or the
+// waveform failed; and the row's <h|h>.
+template <typename T>
+__global__ __launch_bounds__(256) void k_roq_series(RoqDev rd, const T* Z, const RoqRow* rows,
+                                                    long long n, int n_times, double* dh,
+                                                    double* hh) {
+  const int nout = n_times > 0 ? n_times : 1;
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n * nout) return;
+  const long long r = t / nout;
+  const int j = (int)(t - r * nout);
+  const RoqRow rr = rows[r];
+  double sr = NAN, si = NAN;
+  if (rr.valid) {
+    const T* zr = Z + (size_t)r * rd.Np;
+    const bool ok = n_times > 0
+                        ? roq_dh_at(rd, zr, rr, roq_grid_time(rd.tcmin, rd.dtc, j) - rd.start,
+                                    rr.jitter, false, sr, si)
+                        : roq_dh_at(rd, zr, rr, rr.dtg, 0.0, true, sr, si);
+    if (!ok) sr = si = NAN;
+  }
+  dh[2 * t] = sr;
+  dh[2 * t + 1] = si;
+  if (j == 0) hh[r] = rr.valid ? rr.hh : NAN;
+}
+
 }  // namespace gwl
 
 using namespace gwl;
+
+static size_t roq_ws_bytes(const RoqDev& rd, int64_t n, int mode) {
+  const size_t Mp = (size_t)((n + kBM - 1) / kBM) * kBM;
+  const size_t a = mode == ROQ_FP64 ? Mp * rd.Kp * 8 : Mp * rd.Kp * 4;  // bf16: two halves
+  const size_t c = mode == ROQ_FP64 ? Mp * rd.Np * 8 : Mp * rd.Np * 4;
+  return a + c + (size_t)n * sizeof(RoqRow) + 3 * 256;
+}
+
+// Workspace, node waveforms (k_roq_prep) and the weight GEMM of one call:
+// on return *C holds the products Z [Mp][Np] and *rows the row constants.
+static int roq_prep_gemm(gwl_roq* e, const double* params, int64_t n, int32_t ld, int32_t mode,
+                         hipStream_t st, const void** C, RoqRow** rows_out) {
+  GWL_HIP_CHECK(hipSetDevice(e->ctx->device));
+  const size_t need = roq_ws_bytes(e->dev, n, ROQ_FP64) > roq_ws_bytes(e->dev, n, mode)
+                          ? roq_ws_bytes(e->dev, n, ROQ_FP64) : roq_ws_bytes(e->dev, n, mode);
+  if (need > e->ws_bytes) {
+    if (e->ws) (void)hipFree(e->ws);
+    e->ws = nullptr;
+    e->ws_bytes = 0;
+    if (hipMalloc(&e->ws, need) != hipSuccess) {
+      set_error("hipMalloc failed for ROQ workspace");
+      return GWL_E_NOMEM;
+    }
+    e->ws_bytes = need;
+  }
+  (void)hipGetLastError();
+  const RoqDev& rd = e->dev;
+  const int Mp = (int)(((n + kBM - 1) / kBM) * kBM);
+  char* p = reinterpret_cast<char*>(e->ws);
+  const size_t abytes = (mode == ROQ_FP64 ? (size_t)Mp * rd.Kp * 8 : (size_t)Mp * rd.Kp * 4);
+  void* A = p;
+  void* Alo = p + (size_t)Mp * rd.Kp * 2;  // bf16 low half follows the high half
+  char* Cp = p + (abytes + 255) / 256 * 256;
+  const size_t cbytes = (mode == ROQ_FP64 ? (size_t)Mp * rd.Np * 8 : (size_t)Mp * rd.Np * 4);
+  RoqRow* rows = reinterpret_cast<RoqRow*>(Cp + (cbytes + 255) / 256 * 256);
+  // one block per operand row: rows past n and padding columns are written as zeros
+  if (mode == ROQ_FP64)
+    hipLaunchKernelGGL(k_roq_prep<ROQ_FP64>, dim3((unsigned)Mp), dim3(128), 0, st, rd, params, ld, n, A, nullptr, rows);
+  else if (mode == ROQ_FP32)
+    hipLaunchKernelGGL(k_roq_prep<ROQ_FP32>, dim3((unsigned)Mp), dim3(128), 0, st, rd, params, ld, n, A, nullptr, rows);
+  else
+    hipLaunchKernelGGL(k_roq_prep<ROQ_BF16X3>, dim3((unsigned)Mp), dim3(128), 0, st, rd, params, ld, n, A, Alo, rows);
+  GWL_HIP_CHECK(hipGetLastError());
+  if (mode == ROQ_FP64) {
+    hipLaunchKernelGGL(k_gemm_f64, dim3(rd.Np / 64, Mp / 64), dim3(256), 0, st,
+                       reinterpret_cast<const double*>(A), rd.bt64, reinterpret_cast<double*>(Cp),
+                       Mp, rd.Np, rd.Kp);
+  } else if (mode == ROQ_FP32) {
+    hipLaunchKernelGGL(k_gemm_f32, dim3(rd.Np / kBN, Mp / kBM), dim3(256), 0, st,
+                       reinterpret_cast<const float*>(A), rd.bt32, reinterpret_cast<float*>(Cp),
+                       Mp, rd.Np, rd.Kp);
+  } else {
+    hipLaunchKernelGGL(k_gemm_bf16x3, dim3(rd.Np / kBN, Mp / kBM), dim3(256), 0, st,
+                       reinterpret_cast<const __bf16*>(A), reinterpret_cast<const __bf16*>(Alo),
+                       rd.bth, rd.btl, reinterpret_cast<float*>(Cp), Mp, rd.Np, rd.Kp);
+  }
+  GWL_HIP_CHECK(hipGetLastError());
+  *C = Cp;
+  *rows_out = rows;
+  return GWL_OK;
+}
 
 extern "C" {
 
@@ -751,12 +845,6 @@ int gwl_roq_set_distance_spline(gwl_roq* e, int32_t nx, int32_t ny, const double
   return GWL_OK;
 }
 
-static size_t roq_ws_bytes(const RoqDev& rd, int64_t n, int mode) {
-  const size_t Mp = (size_t)((n + kBM - 1) / kBM) * kBM;
-  const size_t a = mode == ROQ_FP64 ? Mp * rd.Kp * 8 : Mp * rd.Kp * 4;  // bf16: two halves
-  const size_t c = mode == ROQ_FP64 ? Mp * rd.Np * 8 : Mp * rd.Np * 4;
-  return a + c + (size_t)n * sizeof(RoqRow) + 3 * 256;
-}
 
 int gwl_roq_reserve(gwl_roq* e, int64_t rows) {
   if (!e || rows < 1) {
@@ -791,52 +879,12 @@ int gwl_roq_eval(gwl_roq* e, const double* params, int64_t n, int32_t ld, int32_
     set_error("gwl_roq_eval: distance marginalisation requested but no spline attached");
     return GWL_E_INVALID;
   }
-  GWL_HIP_CHECK(hipSetDevice(e->ctx->device));
-  const size_t need = roq_ws_bytes(e->dev, n, ROQ_FP64) > roq_ws_bytes(e->dev, n, mode)
-                          ? roq_ws_bytes(e->dev, n, ROQ_FP64) : roq_ws_bytes(e->dev, n, mode);
-  if (need > e->ws_bytes) {
-    if (e->ws) (void)hipFree(e->ws);
-    e->ws = nullptr;
-    e->ws_bytes = 0;
-    if (hipMalloc(&e->ws, need) != hipSuccess) {
-      set_error("hipMalloc failed for ROQ workspace");
-      return GWL_E_NOMEM;
-    }
-    e->ws_bytes = need;
-  }
-  (void)hipGetLastError();
-  const RoqDev& rd = e->dev;
   hipStream_t st = reinterpret_cast<hipStream_t>(stream);
-  const int Mp = (int)(((n + kBM - 1) / kBM) * kBM);
-  char* p = reinterpret_cast<char*>(e->ws);
-  const size_t abytes = (mode == ROQ_FP64 ? (size_t)Mp * rd.Kp * 8 : (size_t)Mp * rd.Kp * 4);
-  void* A = p;
-  void* Alo = p + (size_t)Mp * rd.Kp * 2;  // bf16 low half follows the high half
-  char* Cp = p + (abytes + 255) / 256 * 256;
-  const size_t cbytes = (mode == ROQ_FP64 ? (size_t)Mp * rd.Np * 8 : (size_t)Mp * rd.Np * 4);
-  RoqRow* rows = reinterpret_cast<RoqRow*>(Cp + (cbytes + 255) / 256 * 256);
-  // one block per operand row: rows past n and padding columns are written as zeros
-  if (mode == ROQ_FP64)
-    hipLaunchKernelGGL(k_roq_prep<ROQ_FP64>, dim3((unsigned)Mp), dim3(128), 0, st, rd, params, ld, n, A, nullptr, rows);
-  else if (mode == ROQ_FP32)
-    hipLaunchKernelGGL(k_roq_prep<ROQ_FP32>, dim3((unsigned)Mp), dim3(128), 0, st, rd, params, ld, n, A, nullptr, rows);
-  else
-    hipLaunchKernelGGL(k_roq_prep<ROQ_BF16X3>, dim3((unsigned)Mp), dim3(128), 0, st, rd, params, ld, n, A, Alo, rows);
-  GWL_HIP_CHECK(hipGetLastError());
-  if (mode == ROQ_FP64) {
-    hipLaunchKernelGGL(k_gemm_f64, dim3(rd.Np / 64, Mp / 64), dim3(256), 0, st,
-                       reinterpret_cast<const double*>(A), rd.bt64, reinterpret_cast<double*>(Cp),
-                       Mp, rd.Np, rd.Kp);
-  } else if (mode == ROQ_FP32) {
-    hipLaunchKernelGGL(k_gemm_f32, dim3(rd.Np / kBN, Mp / kBM), dim3(256), 0, st,
-                       reinterpret_cast<const float*>(A), rd.bt32, reinterpret_cast<float*>(Cp),
-                       Mp, rd.Np, rd.Kp);
-  } else {
-    hipLaunchKernelGGL(k_gemm_bf16x3, dim3(rd.Np / kBN, Mp / kBM), dim3(256), 0, st,
-                       reinterpret_cast<const __bf16*>(A), reinterpret_cast<const __bf16*>(Alo),
-                       rd.bth, rd.btl, reinterpret_cast<float*>(Cp), Mp, rd.Np, rd.Kp);
-  }
-  GWL_HIP_CHECK(hipGetLastError());
+  const void* Cv = nullptr;
+  RoqRow* rows = nullptr;
+  if (int rc = roq_prep_gemm(e, params, n, ld, mode, st, &Cv, &rows)) return rc;
+  const RoqDev& rd = e->dev;
+  const char* Cp = static_cast<const char*>(Cv);
   const size_t klds = (rd.flags & GWL_DISTANCE_MARG) ? (size_t)(rd.sp.nx + rd.sp.ny + 8) * 8 : 0;
   const dim3 eg((unsigned)((n + kRoqEpiRows - 1) / kRoqEpiRows));
   if (mode == ROQ_FP64)
@@ -845,6 +893,42 @@ int gwl_roq_eval(gwl_roq* e, const double* params, int64_t n, int32_t ld, int32_
   else
     hipLaunchKernelGGL(k_roq_epi<float>, eg, dim3(64 * kRoqEpiRows), klds, st, rd,
                        reinterpret_cast<const float*>(Cp), rows, (long long)n, out);
+  GWL_HIP_CHECK(hipGetLastError());
+  return GWL_OK;
+}
+
+int gwl_roq_inner_products(gwl_roq* e, const double* params, int64_t n, int32_t ld,
+                           int32_t mode, int32_t n_times, double* d_inner_h, double* h_inner_h,
+                           void* stream) {
+  if (e && n == 0) return GWL_OK;
+  if (!e || !params || !d_inner_h || !h_inner_h || n < 0 || ld < GWL_NPARAM || mode < 0 ||
+      mode > 2 || n_times < 0) {
+    set_error("gwl_roq_inner_products: invalid argument");
+    return GWL_E_INVALID;
+  }
+  if (n_times > 0 && (!(e->dev.flags & GWL_TIME_MARG) || n_times != e->dev.ntimes)) {
+    set_error("gwl_roq_inner_products: n_times must be 0 or the length of the likelihood's "
+              "time-marginalisation grid");
+    return GWL_E_INVALID;
+  }
+  if (n > ((int64_t)1 << 31) / (n_times > 0 ? n_times : 1)) {
+    set_error("gwl_roq_inner_products: too many outputs in one call");
+    return GWL_E_INVALID;
+  }
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  const void* C = nullptr;
+  RoqRow* rows = nullptr;
+  if (int rc = roq_prep_gemm(e, params, n, ld, mode, st, &C, &rows)) return rc;
+  const long long total = n * (long long)(n_times > 0 ? n_times : 1);
+  const dim3 g((unsigned)((total + 255) / 256));
+  if (mode == ROQ_FP64)
+    hipLaunchKernelGGL(k_roq_series<double>, g, dim3(256), 0, st, e->dev,
+                       static_cast<const double*>(C), rows, (long long)n, (int)n_times, d_inner_h,
+                       h_inner_h);
+  else
+    hipLaunchKernelGGL(k_roq_series<float>, g, dim3(256), 0, st, e->dev,
+                       static_cast<const float*>(C), rows, (long long)n, (int)n_times, d_inner_h,
+                       h_inner_h);
   GWL_HIP_CHECK(hipGetLastError());
   return GWL_OK;
 }

@@ -27,6 +27,7 @@ struct WinBasis {
   int Kp = 0;                // padded real operand width (>= 2K, multiple of 32)
   int L = 0;                 // window bins
   int Np = 0;                // padded 2L (multiple of 128)
+  long long n_fft = 0;       // transform length (0: the event's N)
   long long j_first = 0;     // first transform bin
   double start = 0.0;        // time of transform bin j: start + (j + offset) t_step
   int offset = 0;
@@ -75,8 +76,12 @@ int win_clear(const WinBasis& wb, void* a, bool fp64, long long n, hipStream_t s
 // GEMM of the operand tile against the basis, then the window epilogue:
 // out[r] for r < n from C, the row's <h|h> (hh[3 r + 2]), validity
 // (valid[r * valid_stride]), luminosity distance and time jitter (params).
+// With ``series_dh`` set, the epilogue instead writes the transform itself,
+// series_dh[(r L + jj) 2 + {0, 1}] = X_{j_first + jj}, and series_hh[r] =
+// <h|h> (NaN for rows whose waveform failed); ``out`` is not written.
 int win_finish(gwl_event* ev, const WinBasis& wb, const void* a, bool fp64, void* c,
                long long n, const double* params, int ld, const double* hh, const int* valid,
-               int valid_stride, double* out, hipStream_t st);
+               int valid_stride, double* out, hipStream_t st, double* series_dh = nullptr,
+               double* series_hh = nullptr);
 
 }  // namespace gwl

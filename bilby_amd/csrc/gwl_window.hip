@@ -278,6 +278,35 @@ __global__ __launch_bounds__(64 * kWinEpiRows) void k_win_epi(WinEpiArgs a) {
   }
 }
 
+// One thread per (row, window bin): the transform X_j itself (the float32
+// chunk partials summed in float64) and the row's <h|h>; NaN for rows whose
+// waveform failed.
+__global__ __launch_bounds__(256) void k_win_series(WinEpiArgs a, double* dh, double* hho) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= a.n * a.L) return;
+  const long long r = t / a.L;
+  const int jj = (int)(t - r * a.L);
+  const bool ok = a.valid[r * a.vstride] != 0;
+  double xr = NAN, xi = NAN;
+  if (ok) {
+    if (a.C64) {
+      const double* crow = a.C64 + (size_t)r * a.Np;
+      xr = crow[jj];
+      xi = crow[a.L + jj];
+    } else {
+      xr = xi = 0.0;
+      for (int c = 0; c < a.nchunk; ++c) {
+        const float* crow = a.C32 + ((size_t)c * a.Mp + r) * a.Np;
+        xr += (double)crow[jj];
+        xi += (double)crow[a.L + jj];
+      }
+    }
+  }
+  dh[2 * t] = xr;
+  dh[2 * t + 1] = xi;
+  if (jj == 0) hho[r] = ok ? a.hh[3 * r + 2] : NAN;
+}
+
 int pack_basis(gwl_event* ev, const double2* m, int K, int L, WinBasis* wb) {
   wb->K = K;
   wb->L = L;
@@ -317,7 +346,7 @@ int win_basis_relbin(gwl_event* ev, int nd, int nb, const long long* edges, cons
   a.nd = nd;
   a.nb = nb;
   a.L = L;
-  a.N = ev->dev.N;
+  a.N = wb->n_fft > 0 ? wb->n_fft : ev->dev.N;
   a.j_first = wb->j_first;
   a.n_x = n_x;
   a.edges = edges;
@@ -388,7 +417,8 @@ int win_clear(const WinBasis& wb, void* a, bool fp64, long long n, hipStream_t s
 
 int win_finish(gwl_event* ev, const WinBasis& wb, const void* a, bool fp64, void* c,
                long long n, const double* params, int ld, const double* hh, const int* valid,
-               int valid_stride, double* out, hipStream_t st) {
+               int valid_stride, double* out, hipStream_t st, double* series_dh,
+               double* series_hh) {
   const long long Mp = win_mp(n);
   const int nchunk = win_chunks(wb);
   if (fp64) {
@@ -422,6 +452,13 @@ int win_finish(gwl_event* ev, const WinBasis& wb, const void* a, bool fp64, void
   e.t_step = wb.t_step;
   e.delta_tc = wb.delta_tc;
   e.out = out;
+  if (series_dh) {
+    const long long total = n * (long long)wb.L;
+    hipLaunchKernelGGL(k_win_series, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, e,
+                       series_dh, series_hh);
+    GWL_HIP_CHECK(hipGetLastError());
+    return GWL_OK;
+  }
   const size_t lds = (ev->dev.flags & GWL_DISTANCE_MARG)
                          ? (size_t)(ev->dev.sp.nx + ev->dev.sp.ny + 8) * sizeof(double) : 0;
   hipLaunchKernelGGL(k_win_epi, dim3((unsigned)((n + kWinEpiRows - 1) / kWinEpiRows)),

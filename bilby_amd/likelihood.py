@@ -77,6 +77,29 @@ def ln_i0_host(value):
     return np.log(i0e(value)) + value
 
 
+def grid_time_posterior(likelihood, d_inner_h, h_inner_h, parameters, times):
+    """The time posterior on a likelihood's own time grid (roq.py:1255-1281,
+    multiband.py:828-848): the time-conditioned lnL (distance / phase
+    marginalised as configured) times the time prior, normalised.  NaN
+    inner products (times the interpolation cannot reach) count as
+    lnL = -inf."""
+    if likelihood.distance_marginalization:
+        log_like = likelihood._distance_marginalized_values(d_inner_h, h_inner_h, parameters)
+    elif likelihood.phase_marginalization:
+        log_like = ln_i0_host(np.abs(d_inner_h)) - h_inner_h / 2
+    else:
+        log_like = d_inner_h.real - h_inner_h / 2
+    log_like = np.where(np.isnan(log_like), -np.inf, log_like)
+    if not np.any(np.isfinite(log_like)):
+        raise ValueError("the time-conditioned likelihood is not finite anywhere on the grid "
+                         "(did the waveform fail at these parameters?)")
+    post = np.exp(log_like - np.max(log_like)) * likelihood.priors["geocent_time"].prob(times)
+    total = np.sum(post)
+    if not total > 0:
+        raise ValueError("the time posterior vanishes on the grid")
+    return post / total
+
+
 def interped_sample(xx, yy, rng):
     """One draw of bilby's Interped(xx, yy).sample()
     (core/prior/interpolated.py:12-175): the density re-gridded linearly to

@@ -23,7 +23,7 @@ import torch
 from . import _lib
 from .constants import gravitational_constant, radius_of_earth, solar_mass, speed_of_light
 from ._lib import LIB, check, dptr
-from .likelihood import GravitationalWaveTransient
+from .likelihood import GravitationalWaveTransient, grid_time_posterior
 
 
 def round_up_to_power_of_two(x):
@@ -446,8 +446,35 @@ class MBGravitationalWaveTransient(GravitationalWaveTransient):
                                                  return_inner_products=True)
         return complex(aux[0, 0], aux[0, 1]), float(aux[0, 2])
 
+    def _time_posterior(self, parameters):
+        """multiband.py:813-848: d_inner_h_array of the waveform at the
+        sample's geocent_time (the jitter is not added to it) over the time
+        window, the time-conditioned lnL on _times plus the jitter, times
+        the prior, normalised.  Outside the window the prior is zero."""
+        q = dict(parameters)
+        for key, value in self._defaults().items():
+            q.setdefault(key, value)
+        jitter = float(q.get("time_jitter", 0.0)) if self.jitter_time else 0.0
+        q["time_jitter"] = 0.0
+        rows = torch.from_numpy(self.rows_from_parameters(q)[:1]).to("cuda:%d" % self.device)
+        N, j_first, L = self._time_window
+        dh = torch.empty((1, L, 2), dtype=torch.float64, device=rows.device)
+        hh = torch.empty(1, dtype=torch.float64, device=rows.device)
+        st = torch.cuda.current_stream(rows.device).cuda_stream
+        check(LIB.gwl_mb_time_series(self._mb, ctypes.c_void_p(rows.data_ptr()), 1, _lib.NPARAM,
+                                     _lib.FP64, ctypes.c_void_p(dh.data_ptr()),
+                                     ctypes.c_void_p(hh.data_ptr()), ctypes.c_void_p(st)),
+              "gwl_mb_time_series")
+        dh = dh.cpu().numpy()[0]
+        times = self._times[j_first:j_first + L] + jitter
+        return times, grid_time_posterior(self, dh[:, 0] + 1j * dh[:, 1],
+                                          float(hh.cpu().numpy()[0]), self._frame(q), times)
+
     def generate_time_sample_from_marginalized_likelihood(self, parameters, rng=None):
-        raise NotImplementedError("reconstruction of the time is not provided for multibanding")
+        """multiband.py:813-848: one of the grid times, drawn with the
+        posterior's probabilities."""
+        times, post = self._time_posterior(parameters)
+        return float((np.random.default_rng() if rng is None else rng).choice(times, p=post))
 
     def _device_eval(self, rows_t, precision, aux=False, stream=None):
         n = rows_t.shape[0]

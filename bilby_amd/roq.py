@@ -17,7 +17,8 @@ from . import _lib
 from . import abi as _abi
 from ._lib import LIB, check, dptr
 from .detector import InterferometerList
-from .likelihood import _batch_rows, distance_interpolant, get_context
+from .likelihood import (GravitationalWaveTransient, _batch_rows, distance_interpolant,
+                         get_context, grid_time_posterior)
 from .prior import Uniform
 
 MODES = {"fp64": 0, "fp32": 1, "bf16x3": 2}
@@ -155,6 +156,7 @@ class ROQGravitationalWaveTransient:
                 self.device, self._distance_array, self.distance_prior_array, self._ref_dist,
                 self.phase_marginalization, lookup_table)
             self._spline = (tx, ty, c, bbox)
+            self._dist_margd_loglikelihood_array = table
             check(LIB.gwl_roq_set_distance_spline(self._roq, len(tx) - 4, len(ty) - 4, dptr(tx),
                                                   dptr(ty), dptr(c), dptr(bbox)),
                   "gwl_roq_set_distance_spline")
@@ -183,10 +185,63 @@ class ROQGravitationalWaveTransient:
             d["luminosity_distance"] = self._ref_dist
         return d
 
-    def generate_posterior_sample_from_marginalized_likelihood(self, parameters, rng=None):
-        """base.py:502-542 / roq.py:1238-1281: not provided for the ROQ
-        likelihood (its time samples and inner products are not exposed)."""
-        raise NotImplementedError("posterior reconstruction is not provided for the ROQ likelihood")
+    # ------------------------------------------------------------------ reconstruction
+    # base.py:502-742 with the ROQ's own time reconstruction (roq.py:1238-1281)
+    calibration_marginalization = False
+    generate_posterior_sample_from_marginalized_likelihood = (
+        GravitationalWaveTransient.generate_posterior_sample_from_marginalized_likelihood)
+    _distance_marginalized_values = GravitationalWaveTransient._distance_marginalized_values
+    _distance_posterior = GravitationalWaveTransient._distance_posterior
+    _phase_posterior = GravitationalWaveTransient._phase_posterior
+    generate_distance_sample_from_marginalized_likelihood = (
+        GravitationalWaveTransient.generate_distance_sample_from_marginalized_likelihood)
+    generate_phase_sample_from_marginalized_likelihood = (
+        GravitationalWaveTransient.generate_phase_sample_from_marginalized_likelihood)
+
+    def _inner_products(self, parameters, n_times):
+        """gwl_roq_inner_products for one parameter dict (float64 GEMM)."""
+        q = dict(parameters)
+        for key, value in self._defaults().items():
+            q.setdefault(key, value)
+        rows = torch.from_numpy(self.rows_from_parameters(q)[:1]).to("cuda:%d" % self.device)
+        dh = torch.empty((1, max(1, n_times), 2), dtype=torch.float64, device=rows.device)
+        hh = torch.empty(1, dtype=torch.float64, device=rows.device)
+        st = torch.cuda.current_stream(rows.device).cuda_stream
+        check(LIB.gwl_roq_inner_products(self._roq, ctypes.c_void_p(rows.data_ptr()), 1,
+                                         _lib.NPARAM, MODES["fp64"], int(n_times),
+                                         ctypes.c_void_p(dh.data_ptr()),
+                                         ctypes.c_void_p(hh.data_ptr()), ctypes.c_void_p(st)),
+              "gwl_roq_inner_products")
+        dh = dh.cpu().numpy()[0]
+        return dh[:, 0] + 1j * dh[:, 1], float(hh.cpu().numpy()[0]), q
+
+    def _plain_inner_products(self, parameters):
+        """<d|h>, <h|h> at the parameters' own time (roq.py:467-549 as
+        base.py:686-694 calls it; the jitter is not added)."""
+        q = dict(parameters)
+        q["time_jitter"] = 0.0
+        dh, hh, _ = self._inner_products(q, 0)
+        if not np.isfinite(dh[0]):
+            raise ValueError("the parameters' time lies outside the ROQ time samples, or the "
+                             "waveform failed")
+        return complex(dh[0]), hh
+
+    def _time_posterior(self, parameters):
+        """roq.py:1238-1281: d_inner_h_array on the time-marginalisation grid
+        shifted by the jitter, the time-conditioned lnL, times the prior at
+        _times plus the jitter, normalised."""
+        if not self.time_marginalization:
+            raise ValueError("the likelihood does not marginalise over time")
+        dh, hh, q = self._inner_products(parameters, len(self._times))
+        jitter = float(q.get("time_jitter", 0.0)) if self.jitter_time else 0.0
+        times = self._times + jitter
+        return times, grid_time_posterior(self, dh, hh, self._frame(q), times)
+
+    def generate_time_sample_from_marginalized_likelihood(self, parameters, rng=None):
+        """roq.py:1238-1281: one of the grid times, drawn with the
+        posterior's probabilities."""
+        times, post = self._time_posterior(parameters)
+        return float((np.random.default_rng() if rng is None else rng).choice(times, p=post))
 
     def row_builder(self):
         """The host-side dict -> rows conversion of this likelihood

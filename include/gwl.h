@@ -288,6 +288,21 @@ int gwl_roq_reserve(gwl_roq* roq, int64_t max_rows);
  * params, out device pointers; mode enum gwl_roq_mode. */
 int gwl_roq_eval(gwl_roq* roq, const double* params, int64_t n_rows, int32_t ld, int32_t mode,
                  double* out, void* stream);
+/* Inner products of the ROQ likelihood for posterior reconstruction
+ * (base.py:642-742 via roq.py:467-549; roq.py:1238-1281):
+ *   n_times = 0: d_inner_h [n_rows][2] = <d|h> at each row's own time (time
+ *                jitter included; antenna patterns at the middle of the time
+ *                prior when the likelihood marginalises over time);
+ *   n_times > 0: the likelihood's time-marginalisation grid (n_times must be
+ *                its length, tcmin + (k + 1/2) delta_tc): d_inner_h
+ *                [n_rows][n_times][2] = d_inner_h_array shifted by the row's
+ *                time jitter (roq.py:536-540, 604-651).
+ * h_inner_h [n_rows] = <h|h>.  NaN where the five samples leave the
+ * weights' time grid (the reference's log(in_bounds)) or the waveform
+ * failed.  Device pointers; mode enum gwl_roq_mode. */
+int gwl_roq_inner_products(gwl_roq* roq, const double* params, int64_t n_rows, int32_t ld,
+                           int32_t mode, int32_t n_times, double* d_inner_h, double* h_inner_h,
+                           void* stream);
 
 /* ROQ linear weights from a reduced basis (roq.py:849-918,
  * _set_weights_linear) for one detector and ROQ time samples m = k0 .. k0 +
@@ -357,6 +372,37 @@ int gwl_relbin_eval(gwl_relbin* rb, const double* params, int64_t n, int32_t ld,
  * vector per detector and edge, float64 on the device) and each evaluation
  * is one GEMM of the rows' edge ratios against them. */
 int gwl_relbin_set_time(gwl_relbin* rb, const int64_t* bin_index, const double* integrand);
+/* d_inner_h_array of relative binning over the event's time window
+ * (relative.py:410-420: 4/T FFT of the rebuilt full waveform times
+ * conj(d) / S), for time samples drawn on the likelihood's own grid:
+ * d_inner_h [n][L][2] for transform bins j_first .. j_first + L - 1 of
+ * gwl_event_time_window (time start + (j + 1) t_step + the row's jitter),
+ * h_inner_h [n] the binned <h|h>; NaN for rows whose waveform failed.
+ * Device pointers; needs gwl_relbin_set_time. */
+int gwl_relbin_time_series(gwl_relbin* rb, const double* params, int64_t n, int32_t ld,
+                           int32_t precision, double* d_inner_h, double* h_inner_h,
+                           void* stream);
+/* Posterior reconstruction of the time for relative binning (base.py:580-640
+ * with the rebuilt waveform of relative.py:380-393 as the signal):
+ *   gwl_relbin_set_upsampled: bin_index as for gwl_relbin_set_time;
+ *     integrand [n_det][bin_index[n_bins] - bin_index[0] + 1][2] complex128
+ *     h0 conj(d) / S (no 4/T, as base.py:617-622; zero off the mask, the
+ *     Nyquist bin included), weight [n_det][same] 4/T |h0|^2 / S on the
+ *     mask (zero off it), n_fft = 16384 T (the upsampled transform length);
+ *   gwl_relbin_upsampled_series: for n <= 8192 rows, d_inner_h
+ *     [n][n_window][2] = the transform of the rebuilt h conj(d) / S at bins
+ *     j_first .. j_first + n_window - 1 of n_fft (time (geocent + jitter -
+ *     start) + j / 16384 modulo the segment; j may run past n_fft, which
+ *     wraps), h_inner_h [n] = <h|h> of the rebuilt waveform on the mask;
+ *     NaN for rows whose waveform failed.  The basis of a window (one
+ *     vector per detector and edge, float64, 12 (2 n_window) (2 K) bytes)
+ *     is built on the first call for it and kept until another window is
+ *     asked for.  Float64; device pointers. */
+int gwl_relbin_set_upsampled(gwl_relbin* rb, const int64_t* bin_index, const double* integrand,
+                             const double* weight, int64_t n_fft);
+int gwl_relbin_upsampled_series(gwl_relbin* rb, const double* params, int64_t n, int32_t ld,
+                                int64_t j_first, int32_t n_window, double* d_inner_h,
+                                double* h_inner_h, void* stream);
 
 /* ------------------------------------------------------------------ multibanding
  * MBGravitationalWaveTransient (bilby/gw/likelihood/multiband.py, linear
@@ -401,6 +447,14 @@ typedef struct gwl_mb_time_desc {
   double antenna_time;
 } gwl_mb_time_desc;
 int gwl_mb_set_time(gwl_mb* mb, const gwl_mb_time_desc* td);
+/* d_inner_h_array of multibanding over the window given to gwl_mb_set_time
+ * (multiband.py:792-800), for the time reconstruction of
+ * multiband.py:813-848: d_inner_h [n_rows][n_window][2] for transform bins
+ * j_first .. j_first + n_window - 1 (time start + j t_step; the row's time
+ * jitter enters the waveform's time shift), h_inner_h [n_rows] = <h|h>;
+ * NaN for rows whose waveform failed.  Device pointers. */
+int gwl_mb_time_series(gwl_mb* mb, const double* params, int64_t n_rows, int32_t ld,
+                       int32_t precision, double* d_inner_h, double* h_inner_h, void* stream);
 
 /* ------------------------------------------------------------------ calibration
  * Calibration marginalisation (base.py:332-346, 860-877, lookup

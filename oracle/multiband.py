@@ -12,6 +12,7 @@ import types
 
 import numpy as np
 
+from . import core
 from .likelihood import OracleGWTransient, detector_response, source_polarizations
 
 
@@ -78,6 +79,48 @@ class OracleMultiband(OracleGWTransient):
             if a is not None:
                 arr = a if arr is None else arr + a
         return float(np.real(self.compute_log_likelihood(d_inner_h, h_inner_h, arr, params)))
+
+
+    def inner_products(self, parameters):
+        """multiband.py:750-807 as base.py:686-694 calls it: <d|h>, <h|h> at
+        the parameters' own time, summed over detectors."""
+        params = copy.deepcopy(parameters)
+        wa = dict(self.waveform_arguments)
+        wa.setdefault("maximum_frequency", self.event.frequency_array[-1])
+        pols = source_polarizations(self.source, self.w["waveform_frequencies"], params, wa)
+        d_inner_h, h_inner_h = 0j, 0.0
+        for det in self.event.detectors:
+            dh, hh, _ = self.calculate_snrs_mb(pols, det, params)
+            d_inner_h += dh
+            h_inner_h += hh
+        return d_inner_h, h_inner_h
+
+    def time_posterior_on_grid(self, parameters):
+        """multiband.py:813-848: d_inner_h_array of the waveform at the
+        parameters' geocent_time (no jitter added), the time-conditioned lnL
+        (distance / phase marginalised as configured) on _times plus the
+        jitter, times the prior, normalised (the reference then draws with
+        numpy's choice)."""
+        params = copy.deepcopy(parameters)
+        wa = dict(self.waveform_arguments)
+        wa.setdefault("maximum_frequency", self.event.frequency_array[-1])
+        pols = source_polarizations(self.source, self.w["waveform_frequencies"], params, wa)
+        h_inner_h, arr = 0.0, None
+        for det in self.event.detectors:
+            _, hh, a = self.calculate_snrs_mb(pols, det, params)
+            h_inner_h += hh
+            arr = a.copy() if arr is None else arr + a
+        if self.distance_marginalization:
+            log_like = self.distance_marginalized_likelihood(arr, h_inner_h, params)
+        elif self.phase_marginalization:
+            log_like = core.ln_i0(np.abs(arr)) - h_inner_h / 2
+        else:
+            log_like = arr.real - h_inner_h / 2
+        times = self._times
+        if self.jitter_time:
+            times = times + params["time_jitter"]
+        post = np.exp(log_like - np.max(log_like)) * self.priors["geocent_time"].prob(times)
+        return times, post / np.sum(post)
 
 
 def quadratic_coefficients_loop(setup, ifos):

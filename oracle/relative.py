@@ -147,6 +147,44 @@ class OracleRelativeBinning(OracleGWTransient):
             full_ratio[lo:hi] = r0[b] + r1[b] * (f[lo:hi] - centre)
         return self.fiducial[det.name] * full_ratio
 
+    def time_arrays(self, params):
+        """base.py:580-626 with relative.py:380-393 (_compute_full_waveform):
+        the time array of the posterior reconstruction from the rebuilt full
+        waveform, which relative binning inherits from the base likelihood:
+        times on the 1/16384 s grid from the (jittered) geocent time modulo
+        the segment inside [min, max) of the prior, the FFT of the
+        zero-padded rebuilt h conj(d) / S (no 4/T), and <h|h> of the
+        rebuilt waveform on the mask."""
+        params = copy.deepcopy(params)
+        params.pop("fiducial", None)
+        ev = self.event
+        if self.jitter_time:
+            params["geocent_time"] += params["time_jitter"]
+        T = ev.duration
+        n_time_steps = int(T * 16384)
+        times = (params["geocent_time"] - ev.start_time) + np.arange(n_time_steps) / 16384.0
+        times = times % T + ev.start_time
+        prior = self.priors["geocent_time"]
+        in_prior = (times >= prior.minimum) & (times < prior.maximum)
+        times = times[in_prior]
+        d_inner_h = np.zeros(len(times), dtype=complex)
+        h_inner_h = 0.0
+        for det in ev.detectors:
+            n = len(det.strain)
+            ratio = self.edge_response(det, params) / self.fiducial_points[det.name]
+            signal = self.full_waveform(det, ratio)
+            signal_long = np.zeros(n_time_steps, dtype=complex)
+            data = np.zeros(n_time_steps, dtype=complex)
+            psd = np.ones(n_time_steps)
+            signal_long[:n] = signal
+            data[:n] = np.conj(det.strain)
+            psd[:n][det.mask] = det.psd[det.mask]
+            d_inner_h += np.fft.fft(signal_long * data / psd)[in_prior]
+            m = det.mask
+            h_inner_h += core.noise_weighted_inner_product(signal[m], signal[m], det.psd[m],
+                                                           T).real
+        return times, d_inner_h, h_inner_h, params
+
     def log_likelihood_ratio(self, parameters):
         """base.py:419-446 with relative.py:393-428 per detector."""
         params = copy.deepcopy(parameters)

@@ -143,3 +143,38 @@ class OracleROQ:
         if self.phase_marginalization:
             return float(core.ln_i0(abs(d_inner_h)) - h_inner_h / 2)
         return float(np.real(d_inner_h) - h_inner_h / 2)
+
+    def inner_products(self, parameters):
+        """base.py:686-694 via roq.py:467-549: <d|h>, <h|h> at the
+        parameters' own time (no time jitter added), summed over detectors;
+        what the distance and phase reconstruction use."""
+        params = copy.deepcopy(parameters)
+        lin, quad = self._node_waveform(params)
+        d_inner_h, h_inner_h = 0j, 0.0
+        for name in self.names:
+            dh, hh, _ = self.calculate_snrs(lin, quad, name, params)
+            d_inner_h += dh
+            h_inner_h += hh
+        return d_inner_h, h_inner_h
+
+    def time_posterior_on_grid(self, parameters):
+        """roq.py:1238-1281: the time-marginalisation grid (plus the
+        jitter), the time-conditioned lnL from d_inner_h_array of the
+        waveform at the parameters' geocent_time, times the prior,
+        normalised (the reference then draws with rng.choice)."""
+        params = copy.deepcopy(parameters)
+        lin, quad = self._node_waveform(params)
+        h_inner_h, arr = 0.0, None
+        for name in self.names:
+            _, hh, a = self.calculate_snrs(lin, quad, name, params)
+            h_inner_h += hh
+            arr = a if arr is None else arr + a
+        if self.phase_marginalization:
+            log_like = core.ln_i0(np.abs(arr)) - h_inner_h / 2
+        else:
+            log_like = arr.real - h_inner_h / 2
+        times = self._times
+        if self.jitter_time:
+            times = times + params["time_jitter"]
+        post = np.exp(log_like - np.max(log_like)) * self.priors["geocent_time"].prob(times)
+        return times, post / np.sum(post)

@@ -108,3 +108,42 @@ def test_interped_sample_inverts_closed_forms():
         u = np.random.default_rng(s).uniform(0, 1)
         assert x == pytest.approx(np.sqrt(u), abs=1e-6)
     assert 0.0 <= interped_sample(grid, 1 - grid, rng) <= 1.0
+
+
+def test_roq_grid_time_posterior_reproduces_the_lnl_at_each_time():
+    """roq.py:1238-1281 against roq.py:463-549 in the ROQ oracle: without
+    jitter, ln post_j - ln prior(t_j) + (time-marginalised lnL) - ln delta_tc
+    equals lnL = Re<d|h>(t_j) - <h|h>/2 from the single-time five-sample
+    interpolation at geocent_time = t_j (antenna at the same reference
+    time), at every grid time inside the prior."""
+    from bilby_amd.configs import make_roq
+    from oracle.roq import OracleROQ
+    s = make_roq(n_draws=2, seed=78, duration=16.0, sampling_frequency=2048.0, n_linear=60,
+                 n_quadratic=30, n_time_samples=300, time_half_span=0.1, prior_half_width=0.05,
+                 delta_tc=5e-4)
+    w = s["weights"]
+    names = ["H1", "L1"]
+    start = s["interferometers"].start_time
+    pri = dict(s["priors"])
+    o = OracleROQ(names, start, w, w["frequency_nodes_linear"], w["frequency_nodes_quadratic"],
+                  dict(pri), time_marginalization=True, jitter_time=False,
+                  delta_tc=s["delta_tc"],
+                  waveform_arguments=s["waveform_generator"].waveform_arguments)
+    p = {k: float(v[0]) for k, v in s["draws"].items()}
+    p.update(geocent_time=float(start), time_jitter=0.0)
+    times, post = o.time_posterior_on_grid(p)
+    assert abs(np.sum(post) - 1.0) < 1e-12
+    lnl_m = o.log_likelihood_ratio(p)
+    prior = pri["geocent_time"]
+    lin, quad = o._node_waveform(p)
+    checked = 0
+    for j in np.flatnonzero(post > 1e-12 * post.max()):
+        q = dict(p, geocent_time=float(times[j]))
+        dh, hh = 0j, 0.0
+        for name in names:
+            a, b, _ = o.calculate_snrs(lin, quad, name, q)
+            dh, hh = dh + a, hh + b
+        ll = np.log(post[j]) - np.log(prior.prob(times[j])) + lnl_m - np.log(o._delta_tc)
+        assert abs(ll - (dh.real - hh / 2)) < 1e-8 * max(1.0, abs(ll)), (j, ll, dh.real - hh / 2)
+        checked += 1
+    assert checked >= 3

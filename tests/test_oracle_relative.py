@@ -251,3 +251,26 @@ def test_time_marginalised_draws_near_fiducial():
         p["ra"] += rng.normal(0, 0.02)
         a, b = rb.log_likelihood_ratio(p), full.log_likelihood_ratio(p)
         assert abs(a - b) < 0.1 * abs(ref - b) + 0.05, (a, b, ref)
+
+
+@pytest.mark.parametrize("phase", [False, True])
+def test_time_reconstruction_at_fiducial_equals_full(phase):
+    """The time posterior of the reconstruction (base.py:580-640, inherited
+    by relative binning with relative.py:380-393 as the waveform) at the
+    fiducial point: every edge ratio is 1, the rebuilt waveform is the
+    fiducial response on the binned range, and the posterior on the
+    1/16384 s grid equals the full likelihood's."""
+    ev, inj = _event(seed=6, masses=(2.6, 2.2), distance=100.0)
+    rb, full, _ = _time_pair(ev, inj, 0.1, flags={"phase": phase})
+    p = dict(inj, geocent_time=ev.start_time, time_jitter=0.0)
+    if phase:
+        p["phase"] = 0.0
+    t_a, post_a = rb.time_posterior(p)
+    t_b, post_b = full.time_posterior(p)
+    assert np.array_equal(t_a, t_b) and len(t_a) > 1000
+    assert np.max(np.abs(post_a - post_b)) < 1e-8 * np.max(post_b)
+    # away from the fiducial point the rebuilt waveform is an approximation
+    q = dict(p, geocent_time=ev.start_time + 2e-4, mass_1=inj["mass_1"] * (1 + 1e-4))
+    t_a, post_a = rb.time_posterior(q)
+    t_b, post_b = full.time_posterior(q)
+    assert abs(t_a[np.argmax(post_a)] - t_b[np.argmax(post_b)]) < 2 / 16384
