@@ -250,6 +250,55 @@ struct LU {
   }
 };
 
+// Per detector (one workgroup each): the sums over bins of conj(h F) d / S
+// and |h F|^2 / S, h the detector response (physical units, [nd][nf][2]), F a
+// per-bin complex factor ([nd][nf][2]), d / S and 1 / S from the event's
+// float64 data (sigma-scaled, zero off the mask; the Nyquist bin from the
+// argument block).  part[3 d + {0, 1, 2}] = Re, Im, |.|^2 sums (without 4/T).
+__global__ __launch_bounds__(256) void k_weighted_inner(EvDev ev, const double* resp,
+                                                         const double* fac, double* part) {
+  __shared__ double red[4][3];
+  const int d = blockIdx.x;
+  double sr = 0.0, si = 0.0, sh = 0.0;
+  for (int n = threadIdx.x; n < ev.nf; n += blockDim.x) {
+    double dr, di, w;
+    if (n == ev.N) {
+      dr = ev.dwN[d][0];
+      di = ev.dwN[d][1];
+      w = ev.wN[d];
+    } else {
+      const size_t i = (size_t)d * ev.N + (size_t)(n % ev.Q) * ev.P + n / ev.Q;
+      dr = ev.dw64[i].x;
+      di = ev.dw64[i].y;
+      w = ev.w64[i];
+    }
+    if (w == 0.0 && dr == 0.0 && di == 0.0) continue;
+    const size_t o = ((size_t)d * ev.nf + n) * 2;
+    const double hr = resp[o] * fac[o] - resp[o + 1] * fac[o + 1];
+    const double hi = resp[o] * fac[o + 1] + resp[o + 1] * fac[o];
+    // conj(h F) (d / S sigma) / sigma
+    sr += (hr * dr + hi * di) / ev.sigma;
+    si += (hr * di - hi * dr) / ev.sigma;
+    sh += (hr * hr + hi * hi) * (w / (ev.sigma * ev.sigma));
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    sr += __shfl_xor(sr, off, 64);
+    si += __shfl_xor(si, off, 64);
+    sh += __shfl_xor(sh, off, 64);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    red[threadIdx.x >> 6][0] = sr;
+    red[threadIdx.x >> 6][1] = si;
+    red[threadIdx.x >> 6][2] = sh;
+  }
+  __syncthreads();
+  if (threadIdx.x < 3) {
+    double v = 0.0;
+    for (int w = 0; w < 4; ++w) v += red[w][threadIdx.x];
+    part[3 * d + threadIdx.x] = v;
+  }
+}
+
 }  // namespace gwl
 
 using namespace gwl;
@@ -669,6 +718,47 @@ int gwl_event_time_window(gwl_event* e, int64_t* j_first, int32_t* n_window, dou
   *j_first = e->dev.jlo;
   *n_window = e->dev.L;
   *t_step = e->dev.tstep;
+  return GWL_OK;
+}
+
+int gwl_weighted_inner_products(gwl_event* e, const double* params, const double* factor,
+                                double* out) {
+  if (!e || !params || !factor || !out) {
+    set_error("gwl_weighted_inner_products: invalid argument");
+    return GWL_E_INVALID;
+  }
+  const int nd = e->dev.nd, nf = e->dev.nf;
+  const size_t fb = (size_t)2 * nd * nf * sizeof(double);
+  GWL_HIP_CHECK(hipSetDevice(e->ctx->device));
+  char* mem = nullptr;
+  const size_t pb = 256, pp = 256;
+  if (hipMalloc((void**)&mem, pb + pp + 2 * fb) != hipSuccess) {
+    set_error("hipMalloc failed");
+    return GWL_E_NOMEM;
+  }
+  double* dp = reinterpret_cast<double*>(mem);
+  double* part = reinterpret_cast<double*>(mem + pb);
+  double* resp = reinterpret_cast<double*>(mem + pb + pp);
+  double* fac = reinterpret_cast<double*>(mem + pb + pp + fb);
+  int rc = GWL_OK;
+  hipError_t err = hipMemcpy(dp, params, GWL_NPARAM * sizeof(double), hipMemcpyHostToDevice);
+  if (err == hipSuccess) err = hipMemcpy(fac, factor, fb, hipMemcpyHostToDevice);
+  for (int d = 0; err == hipSuccess && rc == GWL_OK && d < nd; ++d)
+    rc = launch_response(e, dp, d, GWL_FP64, resp + (size_t)2 * d * nf, nullptr);
+  double h[3 * GWL_MAX_DET];
+  if (err == hipSuccess && rc == GWL_OK) {
+    hipLaunchKernelGGL(k_weighted_inner, dim3((unsigned)nd), dim3(256), 0, 0, e->dev, resp, fac,
+                       part);
+    err = hipGetLastError();
+    if (err == hipSuccess) err = hipMemcpy(h, part, (size_t)3 * nd * 8, hipMemcpyDeviceToHost);
+  }
+  (void)hipFree(mem);
+  if (err != hipSuccess) return hip_fail(err, "gwl_weighted_inner_products");
+  if (rc != GWL_OK) return rc;
+  const double norm = 4.0 / e->dev.T;
+  out[0] = out[1] = out[2] = 0.0;
+  for (int d = 0; d < nd; ++d)
+    for (int c = 0; c < 3; ++c) out[c] += norm * h[3 * d + c];
   return GWL_OK;
 }
 

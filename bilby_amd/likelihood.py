@@ -455,9 +455,6 @@ class GravitationalWaveTransient:
         (bilby uses its global one)."""
         if len(self._marginalized_parameters) == 0:
             return parameters
-        if self.calibration_marginalization and len(self._marginalized_parameters) > 1:
-            raise NotImplementedError("reconstruction with calibration and further "
-                                      "marginalisations is not provided")
         rng = np.random.default_rng() if rng is None else rng
         p = dict(parameters)
         if self.calibration_marginalization:
@@ -479,9 +476,28 @@ class GravitationalWaveTransient:
         if getattr(self, "_plain", None) is None:
             self._plain = GravitationalWaveTransient(self.interferometers, self.waveform_generator,
                                                      precision="fp64", device=self.device)
+        if self.calibration_marginalization and "recalib_index" in parameters:
+            return self._calibrated_inner_products(parameters, int(parameters["recalib_index"]))
         rows = self._plain.rows_from_parameters(dict(parameters))[:1]
         _, aux = self._plain.log_likelihood_ratio_batch(rows, return_inner_products=True)
         return complex(aux[0, 0], aux[0, 1]), float(aux[0, 2])
+
+    def _calibrated_inner_products(self, parameters, index):
+        """base.py:286-294: <d|h>, <h|h> with response curve ``index`` applied
+        to the signal on each detector's mask (gwl_weighted_inner_products on
+        the companion plain event)."""
+        row = np.ascontiguousarray(self._plain.rows_from_parameters(dict(parameters))[0])
+        nf = len(self.interferometers.frequency_array)
+        factor = np.zeros((len(self.interferometers), nf, 2))
+        factor[..., 0] = 1.0
+        for k, ifo in enumerate(self.interferometers):
+            m = np.asarray(ifo.frequency_mask)
+            c = np.asarray(self.calibration_draws[ifo.name][index])
+            factor[k, m, 0], factor[k, m, 1] = c.real, c.imag
+        out = np.zeros(3)
+        check(LIB.gwl_weighted_inner_products(self._plain._event, dptr(row), dptr(factor),
+                                              dptr(out)), "gwl_weighted_inner_products")
+        return complex(out[0], out[1]), float(out[2])
 
     def _distance_marginalized_values(self, d_inner_h, h_inner_h, parameters):
         """base.py:775-784 on the host for an array of <d|h>: the lookup

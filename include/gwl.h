@@ -172,6 +172,17 @@ int gwl_event_set_transform(gwl_event* ev, int32_t transform);
 int gwl_event_timing(gwl_event* ev, int32_t enable);
 int gwl_event_timing_read(gwl_event* ev, double* total_ms, int64_t* launches);
 
+/* Inner products of one parameter row with a per-bin complex factor F on
+ * the response (base.py:286-294: the signal times the drawn calibration
+ * curve, signal[mask] *= calibration_draws[ifo][recalib_index]), summed over
+ * detectors: out[0] + i out[1] = <d|h F> = 4/T sum conj(h F) d / S,
+ * out[2] = <h F|h F>, over the masked bins (Nyquist included when masked).
+ * params: host [GWL_NPARAM] (sky frame); factor: host [n_det][n_freq][2];
+ * out: host [3].  Float64; allocates its own workspace (a reconstruction
+ * call, not for the evaluation path). */
+int gwl_weighted_inner_products(gwl_event* ev, const double* params, const double* factor,
+                                double* out);
+
 /* Detector response of one parameter row: bilby
  * Interferometer.get_detector_response (interferometer.py:303-368) of the
  * event's source model.  params: host [GWL_NPARAM]; out: host

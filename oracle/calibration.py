@@ -128,6 +128,28 @@ class OracleCalibrated(OracleGWTransient):
         ll = self.per_curve_log_likelihoods(parameters)
         return float(logsumexp(ll) - np.log(self.number_of_response_curves))
 
+    def plain_inner_products(self, params):
+        """base.py:286-294 via _calculate_inner_products (:686-694): <d|h>,
+        <h|h> summed over detectors, the signal multiplied on the mask by
+        response curve params['recalib_index'] when it is given (the
+        distance and phase reconstruction after a calibration draw)."""
+        if "recalib_index" not in params:
+            return super().plain_inner_products(params)
+        index = int(params["recalib_index"])
+        pols = source_polarizations(self.source, self.event.frequency_array, params,
+                                    self.waveform_arguments)
+        dh, hh = 0j, 0.0
+        for det in self.event.detectors:
+            signal = detector_response(det, pols, params, self.event.start_time,
+                                       antenna_time=self.antenna_time).copy()
+            m = det.mask
+            signal[m] *= self.curves[det.name][index]
+            dh += core.noise_weighted_inner_product(signal[m], det.strain[m], det.psd[m],
+                                                    self.duration)
+            hh += core.noise_weighted_inner_product(signal[m], signal[m], det.psd[m],
+                                                    self.duration).real
+        return dh, hh
+
     def log_likelihood_ratio_with_curve(self, parameters, index):
         """The likelihood not marginalised over calibration, with response
         curve ``index`` applied to the signal (base.py:286-290,

@@ -143,3 +143,75 @@ def test_calibration_and_time_marginalised_matches_oracle(case, tmp_path):
     assert np.array_equal(out[keep], got[keep])
     with pytest.raises(NotImplementedError):
         lk.log_likelihood_ratio_batch(draws, precision="fp64")
+
+
+@pytest.mark.parametrize("time", [False, True])
+def test_reconstruction_with_calibration_and_other_marginalisations(time, tmp_path):
+    """generate_posterior_sample_from_marginalized_likelihood (base.py:502-542)
+    with calibration, (time,) distance and phase marginalised: the curve drawn
+    from the per-curve likelihoods, the time from the uncalibrated waveform
+    (base.py:580-640 leaves the curve out), distance and phase from the
+    inner products with the drawn curve applied (base.py:286-294,
+    gwl_weighted_inner_products), against the oracle with the same draws."""
+    from bilby_amd import calibration as cal
+    from bilby_amd.configs import make_taylorf2_marginalised
+    from bilby_amd.likelihood import GravitationalWaveTransient, interped_sample
+    from oracle.calibration import OracleCalibrated
+    from oracle.likelihood import OracleDetector, OracleEvent
+    from tests.helpers import draws_as_dicts, fixed_marginalised
+    say("reconstruction with calibration, time", time)
+    s = make_taylorf2_marginalised(n_draws=4, seed=19, duration=4.0, sampling_frequency=1024.0,
+                                   detectors=("H1", "L1"))
+    priors = s["priors"]
+    for ifo in s["interferometers"]:
+        ifo.calibration_model = cal.CubicSpline("recalib_%s_" % ifo.name, 20.0, 512.0, 5)
+        priors.update(cal.constant_uncertainty_spline_priors(0.1, 0.1, 20.0, 512.0, 5, ifo.name))
+    files = {ifo.name: str(tmp_path / ("%s_r.npz" % ifo.name)) for ifo in s["interferometers"]}
+    priors = s["priors"].copy()
+    lk = GravitationalWaveTransient(
+        s["interferometers"], s["waveform_generator"], calibration_marginalization=True,
+        time_marginalization=time, distance_marginalization=True, phase_marginalization=True,
+        priors=priors, number_of_response_curves=50, calibration_lookup_table=files,
+        precision="fp32" if time else "fp64")
+    ifos = s["interferometers"]
+    dets = [OracleDetector(i.name, i.frequency_array, i.frequency_domain_strain,
+                           i.power_spectral_density_array, i.frequency_mask) for i in ifos]
+    ev = OracleEvent(dets, ifos.duration, ifos.sampling_frequency, ifos.start_time)
+    olk = OracleCalibrated(ev, "taylorf2", lk.calibration_draws,
+                           waveform_arguments=dict(s["waveform_generator"].waveform_arguments),
+                           time_marginalization=time, distance_marginalization=True,
+                           phase_marginalization=True, priors=dict(s["priors"]),
+                           distance_table=lk._dist_margd_loglikelihood_array)
+    inj = dict(s["injection"])
+    p = dict(inj)
+    fixed = fixed_marginalised({k: np.array([v]) for k, v in inj.items()}, priors)
+    p.update({k: float(v[0]) for k, v in fixed.items()})
+    if time:
+        p["time_jitter"] = 4.1e-4
+    q = dict(p, recalib_index=7)
+    dh_g, hh_g = lk._plain_inner_products(q)
+    dh_o, hh_o = olk.plain_inner_products(q)
+    say("curve 7 <d|h>", dh_g, dh_o, "<h|h>", hh_g, hh_o)
+    assert abs(dh_g - dh_o) < 1e-9 * abs(dh_o) and abs(hh_g - hh_o) < 1e-9 * hh_o
+    out = lk.generate_posterior_sample_from_marginalized_likelihood(dict(p),
+                                                                    rng=np.random.default_rng(8))
+    rng = np.random.default_rng(8)
+    ref = dict(p)
+    ll = olk.per_curve_log_likelihoods(dict(p))
+    post = np.exp(ll - np.max(ll))
+    ref["recalib_index"] = int(rng.choice(50, p=post / np.sum(post)))
+    if time:
+        t_o, post_o = olk.time_posterior(ref)
+        keep = post_o > np.max(post_o) / 1000
+        if np.sum(keep) < 3:
+            keep[1:-1] = keep[1:-1] | keep[2:] | keep[:-2]
+        ref["geocent_time"] = interped_sample(t_o[keep], post_o[keep], rng)
+    ref["luminosity_distance"] = interped_sample(*olk.distance_posterior(ref), rng)
+    ref["phase"] = interped_sample(*olk.phase_posterior(ref), rng)
+    keys = ("recalib_index", "geocent_time", "luminosity_distance", "phase")
+    say({k: (out[k], ref[k]) for k in keys})
+    assert out["recalib_index"] == ref["recalib_index"]
+    assert abs(out["geocent_time"] - ref["geocent_time"]) < 1e-7
+    assert abs(out["luminosity_distance"] - ref["luminosity_distance"]) < 1e-6 * ref[
+        "luminosity_distance"]
+    assert abs(out["phase"] - ref["phase"]) < 1e-6

@@ -169,3 +169,27 @@ def test_product_calibration_models_equal_the_oracle(tmp_path):
     back, bp = cal._read_lookup(fn, f, 5, 3)
     assert np.array_equal(back, curves[3:8])
     assert np.array_equal(bp["recalib_H1_amplitude_0"], draws["recalib_H1_amplitude_0"][3:8])
+
+
+def test_inner_products_with_a_drawn_curve():
+    """base.py:286-294 (signal[mask] *= curve) as the distance / phase
+    reconstruction uses it after a calibration draw: a constant curve c
+    scales <d|h> by conj(c) and <h|h> by |c|^2; the identity curve and no
+    recalib_index give the plain inner products; a random curve's values
+    equal the curve-applied signal's inner products computed from the
+    per-curve arrays of calibration_snrs (conjugated: those hold
+    conj(d) h cal / S)."""
+    ev, inj = _event(seed=5)
+    c = 0.9 * np.exp(0.3j)
+    n = {det.name: int(det.mask.sum()) for det in ev.detectors}
+    rand = _curves(ev, 3, 11)
+    curves = {k: np.stack([np.full(n[k], c), np.ones(n[k]), rand[k][2]]) for k in n}
+    o = OracleCalibrated(ev, "taylorf2", curves, waveform_arguments=WF)
+    dh, hh = o.plain_inner_products(dict(inj))
+    a, b = o.plain_inner_products(dict(inj, recalib_index=0))
+    assert abs(a - np.conj(c) * dh) < 1e-12 * abs(dh) and abs(b - abs(c) ** 2 * hh) < 1e-12 * hh
+    a, b = o.plain_inner_products(dict(inj, recalib_index=1))
+    assert abs(a - dh) < 1e-12 * abs(dh) and abs(b - hh) < 1e-12 * hh
+    a, b = o.plain_inner_products(dict(inj, recalib_index=2))
+    arr, hh_arr = o.calibration_snrs(dict(inj))
+    assert abs(a - np.conj(arr[2])) < 1e-10 * abs(a) and abs(b - hh_arr[2]) < 1e-10 * b
