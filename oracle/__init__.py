@@ -21,9 +21,11 @@ Pinning status (what checks each function, see tests/test_oracle_*.py):
   sidereal-time known values.
 * sine-Gaussian: pinned by its closed-form <h|h> for a flat PSD.
 * TaylorF2: partially pinned (Newtonian SPA amplitude/phase vs an FFT of a
-  numerically generated Newtonian chirp; 1PN-2PN phase coefficients vs the
-  energy/flux re-expansion).  Parity with lalsimulation is UNPINNED
-  (lalsuite is not available).
+  numerically generated Newtonian chirp; every non-spinning phase
+  coefficient, 1PN-3.5PN with the 2.5PN and 3PN logarithms, vs the 3PN
+  energy / 3.5PN flux re-expansion; the 1.5PN spin-orbit term).  The
+  higher spin terms are transcribed only.  Parity with lalsimulation is
+  UNPINNED (lalsuite is not available).
 * time marginalisation: pinned by brute-force DFT and by brute-force
   trapezoid integration of the un-marginalised likelihood.
 * phase marginalisation: pinned by numerical integration over phase.

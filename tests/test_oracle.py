@@ -211,6 +211,59 @@ def test_taylorf2_pn_coefficients_from_energy_and_flux():
             assert abs(phi[k] - expected) < 1e-9 * max(1, abs(expected)), (eta, k, phi[k], expected)
 
 
+def test_taylorf2_high_order_coefficients_from_energy_and_flux():
+    """The 2.5PN, 3PN and 3.5PN non-spinning phase terms from the 3PN binding
+    energy and the 3.5PN flux (Blanchet, Living Rev. Rel. 17, 2, Eqs. 233
+    and 314 with x = v^2):
+      e_6 = -675/64 + (34445/576 - 205 pi^2/96) eta - 155 eta^2/96
+            - 35 eta^3/5184,
+      f_5 = -(8191/672 + 583 eta/24) pi,
+      f_6 = 6643739519/69854400 + 16 pi^2/3 - 1712 gamma_E/105
+            - 856/105 ln(16 v^2) + (-134543/7776 + 41 pi^2/48) eta
+            - 94403 eta^2/3024 - 775 eta^3/324,
+      f_7 = (-16285/504 + 214745 eta/1728 + 193385 eta^2/3024) pi.
+    With A = (dE/dv / F) / (Newtonian) = sum_k (a_k + a_k^l ln v) v^k, the
+    SPA phase 6 int v^2 t dv gives psi_k = 40 a_k / ((8-k)(5-k)) except
+    where the integrals produce logarithms: the 2.5PN term
+    -(40/3) a_5 ln v (the implementation's phi_5 (1 + 3 ln v), its constant
+    part a phase convention), and at 3PN psi_6 = -20 a_6 + 10 a_6^l,
+    psi_6^l = -20 a_6^l (int v^-3 ln v = -(2 ln v + 1)/(4 v^2))."""
+    from oracle.core import EULER_GAMMA
+    pi = np.pi
+    for eta in (0.25, 0.16, 0.09):
+        e = np.zeros(8)
+        e[0] = 1.0
+        e[2] = -(3 / 4 + eta / 12)
+        e[4] = -(27 / 8 - 19 * eta / 8 + eta ** 2 / 24)
+        e[6] = (-675 / 64 + (34445 / 576 - 205 * pi ** 2 / 96) * eta - 155 * eta ** 2 / 96
+                - 35 * eta ** 3 / 5184)
+        de = np.array([(k + 2) / 2 * e[k] for k in range(8)])
+        fl = np.zeros(8)
+        fl[0] = 1.0
+        fl[2] = -(1247 / 336 + 35 * eta / 12)
+        fl[3] = 4 * pi
+        fl[4] = -44711 / 9072 + 9271 * eta / 504 + 65 * eta ** 2 / 18
+        fl[5] = -(8191 / 672 + 583 * eta / 24) * pi
+        fl[6] = (6643739519 / 69854400 + 16 * pi ** 2 / 3 - 1712 * EULER_GAMMA / 105
+                 - 856 / 105 * np.log(16) + (-134543 / 7776 + 41 * pi ** 2 / 48) * eta
+                 - 94403 * eta ** 2 / 3024 - 775 * eta ** 3 / 324)
+        fl6_log = -1712 / 105          # -856/105 ln(v^2)
+        fl[7] = (-16285 / 504 + 214745 * eta / 1728 + 193385 * eta ** 2 / 3024) * pi
+        a = np.zeros(8)
+        for k in range(8):
+            a[k] = de[k] - sum(a[j] * fl[k - j] for j in range(k))
+        a6_log = -fl6_log              # a_0 = 1; a_1 = 0 keeps it out of a_7
+        phi, phil, _ = core.taylorf2_phase_coefficients(*_masses_for_eta(eta), 0.0, 0.0)
+        checks = {
+            "phil5": (phil[5], -40 / 3 * a[5]),
+            "phi6": (phi[6], -20 * a[6] + 10 * a6_log),
+            "phil6": (phil[6], -20 * a6_log),
+            "phi7": (phi[7], -20 * a[7]),
+        }
+        for name, (got, want) in checks.items():
+            assert abs(got - want) < 1e-9 * max(1.0, abs(want)), (eta, name, got, want)
+
+
 def _masses_for_eta(eta):
     # m1 + m2 = 1, m1 m2 = eta
     d = np.sqrt(1 - 4 * eta)
