@@ -59,7 +59,7 @@ def linear_weights(interferometer, basis, duration, k0, n_times, n_samples, devi
     from .likelihood import get_context
     ifo = interferometer
     m = ifo.frequency_mask
-    basis = np.asarray(basis, dtype=complex)
+    basis = np.ascontiguousarray(basis, dtype=complex)
     d_over_s = ifo.frequency_domain_strain[m] / ifo.power_spectral_density_array[m]
     if basis.shape[1] != len(d_over_s):
         raise ValueError("Mismatch between ROQ basis and frequency array for %s" % ifo.name)

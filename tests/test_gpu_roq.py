@@ -103,3 +103,59 @@ def test_roq_lnl_matches_oracle(time, phase):
     assert errs["fp64"] < 1e-8
     assert errs["fp32"] < 1e-3
     assert errs["bf16x3"] < 5e-2
+
+
+def test_roq_from_bases_matches_oracle(tmp_path):
+    """ROQGravitationalWaveTransient from linear_matrix / quadratic_matrix
+    (roq.py:146-190, 334-366, 736-1005): the bases as arrays [n_freq, n_basis]
+    and as a .npy file [n_basis, n_freq]; the weights it builds (time
+    resolution, time samples, linear weights on the GPU) against the oracle's
+    restatement of the weight sums, and its lnL (time + phase marginalised)
+    against the ROQ oracle on those weights."""
+    from bilby_amd.configs import make_taylorf2_marginalised
+    from bilby_amd.roq import ROQGravitationalWaveTransient
+    from oracle import roq_weights as orw
+    from oracle.roq import OracleROQ
+    from tests.helpers import draws_as_dicts, fixed_marginalised
+    say("roq from bases")
+    s = make_taylorf2_marginalised(n_draws=16, seed=31, duration=4.0, sampling_frequency=1024.0,
+                                   detectors=("H1", "L1"), time=True, distance=False, phase=True)
+    ifos, wfg = s["interferometers"], s["waveform_generator"]
+    rng = np.random.default_rng(5)
+    n_m = int(ifos[0].frequency_mask.sum())
+    L, Q = 40, 12
+    lin = np.linalg.qr(rng.normal(size=(n_m, L)) + 1j * rng.normal(size=(n_m, L)))[0]
+    quad = np.abs(rng.normal(size=(n_m, Q)))
+    wfg.waveform_arguments.update(frequency_nodes_linear=np.sort(rng.uniform(20, 480, L)),
+                                  frequency_nodes_quadratic=np.sort(rng.uniform(20, 480, Q)))
+    opri = dict(s["priors"])
+    priors = s["priors"].copy()
+    np.save(tmp_path / "quad.npy", quad.T)
+    lk = ROQGravitationalWaveTransient(ifos, wfg, priors, linear_matrix=lin,
+                                       quadratic_matrix=str(tmp_path / "quad.npy"),
+                                       time_marginalization=True, phase_marginalization=True,
+                                       precision="fp64")
+    w = lk.weights
+    ts = w["time_samples"]
+    for ifo in ifos:
+        ref = orw.linear_weights(ifo.frequency_array, ifo.frequency_mask,
+                                 ifo.frequency_domain_strain, ifo.power_spectral_density_array,
+                                 ifo.minimum_frequency, ifos.duration, lin.T, ts)
+        err = np.max(np.abs(w[ifo.name + "_linear"] - ref)) / np.max(np.abs(ref))
+        q_ref = orw.quadratic_weights(ifo.frequency_mask, ifo.power_spectral_density_array,
+                                      ifos.duration, quad.T)
+        say(ifo.name, "linear weights relative error", err, "samples", len(ts))
+        assert err < 1e-10
+        assert np.allclose(w[ifo.name + "_quadratic"], q_ref, rtol=1e-13, atol=0)
+    o = OracleROQ([i.name for i in ifos], ifos.start_time, w, w["frequency_nodes_linear"],
+                  w["frequency_nodes_quadratic"], opri, time_marginalization=True,
+                  phase_marginalization=True, delta_tc=ts[1] - ts[0],
+                  waveform_arguments=wfg.waveform_arguments)
+    draws = fixed_marginalised(s["draws"], priors)
+    n = 16
+    ref = np.array([o.log_likelihood_ratio(p) for p in draws_as_dicts(draws, range(n))])
+    got = lk.log_likelihood_ratio_batch({k: v[:n] for k, v in draws.items()})
+    say("lnL max |diff|", float(np.max(np.abs(got - ref))), "range", ref.min(), ref.max())
+    assert np.all(np.isfinite(ref)) and np.max(np.abs(got - ref)) < 1e-8 * max(1.0, np.max(np.abs(ref)))
+    with pytest.raises(ValueError):
+        ROQGravitationalWaveTransient(ifos, wfg, s["priors"].copy(), linear_matrix=lin)
