@@ -24,6 +24,36 @@ from .prior import Uniform
 MODES = {"fp64": 0, "fp32": 1, "bf16x3": 2}
 
 
+def load_weights(filename, interferometers, waveform_generator, format=None):
+    """roq.py:1107-1160 for the npz format (read without pickles): the time
+    samples and each detector's weights, with the frequency nodes from the
+    waveform arguments when the file does not hold them.  HDF5 needs h5py,
+    which is not available."""
+    if format is None:
+        format = filename.split(".")[-1]
+    if format not in ("npz", "hdf5"):
+        raise IOError("Format %s not recognized." % format)
+    if format == "hdf5":
+        raise NotImplementedError("h5py is not available; save the weights as npz")
+    weights = dict(np.load(filename))
+    wfa = waveform_generator.waveform_arguments
+    for kind in ("linear", "quadratic"):
+        key = "frequency_nodes_" + kind
+        if key not in weights:
+            if key not in wfa:
+                raise ValueError("%s is neither in %s nor in the waveform arguments"
+                                 % (key, filename))
+            weights[key] = np.asarray(wfa[key], dtype=float)
+        for ifo in interferometers:
+            w = np.asarray(weights["%s_%s" % (ifo.name, kind)])
+            if kind == "linear" and w.ndim == 3:    # a one-basis list as bilby stores it
+                w = w[0]
+            if kind == "quadratic" and w.ndim == 2:
+                w = w[0]
+            weights["%s_%s" % (ifo.name, kind)] = w
+    return weights
+
+
 class ROQGravitationalWaveTransient:
     def __init__(self, interferometers, waveform_generator, priors, weights=None,
                  linear_matrix=None, quadratic_matrix=None, roq_params=None,
@@ -32,6 +62,9 @@ class ROQGravitationalWaveTransient:
                  delta_tc=None, distance_marginalization_lookup_table=None,
                  reference_frame="sky", time_reference="geocenter", parameter_conversion=None,
                  precision="fp32", device=0):
+        if isinstance(weights, str):
+            weights = load_weights(weights, InterferometerList(interferometers),
+                                   waveform_generator)
         if weights is None:
             if linear_matrix is None or quadratic_matrix is None:
                 raise ValueError("pass weights, or linear_matrix and quadratic_matrix")
@@ -93,6 +126,24 @@ class ROQGravitationalWaveTransient:
             priors["luminosity_distance"] = float(self._ref_dist)
             self._marginalized_parameters.append("luminosity_distance")
         self._build(ts, delta_tc, distance_marginalization_lookup_table)
+
+    def save_weights(self, filename, format="hdf5"):
+        """roq.py:1055-1105: the weights in one file; "npz" holds the time
+        samples and each detector's linear and quadratic weights (the
+        frequency nodes stay in the waveform arguments, as in the
+        reference).  HDF5 needs h5py, which is not available."""
+        if format not in ("hdf5", "npz"):
+            raise IOError("Format %s not recognized." % format)
+        if format not in filename:
+            filename += "." + format
+        if format == "hdf5":
+            raise NotImplementedError("h5py is not available; use format='npz'")
+        out = {"time_samples": np.asarray(self.weights["time_samples"])}
+        for ifo in self.interferometers:
+            for kind in ("linear", "quadratic"):
+                key = "%s_%s" % (ifo.name, kind)
+                out[key] = np.asarray(self.weights[key])
+        np.savez(filename, **out)
 
     @staticmethod
     def _parse_basis(basis, basis_type):

@@ -159,3 +159,13 @@ def test_roq_from_bases_matches_oracle(tmp_path):
     assert np.all(np.isfinite(ref)) and np.max(np.abs(got - ref)) < 1e-8 * max(1.0, np.max(np.abs(ref)))
     with pytest.raises(ValueError):
         ROQGravitationalWaveTransient(ifos, wfg, s["priors"].copy(), linear_matrix=lin)
+    # roq.py:1055-1160: the weights saved as npz and read back give the same likelihood
+    lk.save_weights(str(tmp_path / "w"), format="npz")
+    lk2 = ROQGravitationalWaveTransient(ifos, wfg, s["priors"].copy(),
+                                        weights=str(tmp_path / "w.npz"),
+                                        time_marginalization=True, phase_marginalization=True,
+                                        precision="fp64")
+    assert np.array_equal(lk2.log_likelihood_ratio_batch({k: v[:n] for k, v in draws.items()}),
+                          got)
+    with pytest.raises(NotImplementedError):
+        lk.save_weights(str(tmp_path / "w"))
