@@ -446,6 +446,41 @@ class GravitationalWaveTransient:
         except Exception:
             pass
 
+    # ------------------------------------------------------------------ per detector
+    def compute_per_detector_log_likelihood(self, parameters):
+        """base.py:479-500: each detector's own log-likelihood ratio, with
+        the time / distance / phase marginalisation of this likelihood, as
+        parameters['<ifo>_log_likelihood'] (the dict is updated, as there,
+        and a copy returned).  Evaluated by one float64 single-detector
+        companion likelihood per detector on the GPU."""
+        if self.calibration_marginalization:
+            raise NotImplementedError("per-detector likelihoods with calibration marginalisation "
+                                      "are not provided")
+        if getattr(self, "_per_detector", None) is None:
+            self._per_detector = [
+                GravitationalWaveTransient(
+                    InterferometerList([ifo]), self.waveform_generator,
+                    time_marginalization=self.time_marginalization,
+                    distance_marginalization=self.distance_marginalization,
+                    phase_marginalization=self.phase_marginalization,
+                    priors=None if self.priors is None else self.priors.copy(),
+                    jitter_time=self.jitter_time, reference_frame=self.reference_frame,
+                    time_reference=self.time_reference, precision="fp64", device=self.device)
+                for ifo in self.interferometers]
+        for ifo, lk in zip(self.interferometers, self._per_detector):
+            parameters["%s_log_likelihood" % ifo.name] = lk.log_likelihood_ratio(dict(parameters))
+        return parameters.copy()
+
+    @property
+    def lal_version(self):
+        """base.py:1139-1149: lal is not used (the source models are this
+        library's own); "N/A" as the reference reports without lal."""
+        return "N/A"
+
+    @property
+    def lalsimulation_version(self):
+        return "N/A"
+
     # ------------------------------------------------------------------ reconstruction
     def generate_posterior_sample_from_marginalized_likelihood(self, parameters, rng=None):
         """base.py:502-542: new values of the marginalised parameters drawn

@@ -367,3 +367,49 @@ def test_relative_binning_time_reconstruction_matches_oracle(distance):
     ev_lnl = lk.log_likelihood_ratio_batch({k: np.array([v]) for k, v in p.items()})[0]
     say("own-window series lnL", lnl, "evaluation", ev_lnl)
     assert abs(lnl - ev_lnl) < 1e-8 * max(1.0, abs(ev_lnl))
+
+
+@pytest.mark.parametrize("marginalised", [False, True])
+def test_per_detector_log_likelihoods(marginalised):
+    """compute_per_detector_log_likelihood (base.py:479-500): each
+    detector's own lnL ratio against the oracle on that detector alone;
+    without marginalisation they add up to the network lnL (the inner
+    products are sums over detectors)."""
+    from bilby_amd.configs import make_taylorf2_marginalised
+    from bilby_amd.likelihood import GravitationalWaveTransient
+    from oracle.likelihood import OracleDetector, OracleEvent, OracleGWTransient
+    from tests.helpers import fixed_marginalised
+    say("per-detector lnL, marginalised", marginalised)
+    s = make_taylorf2_marginalised(n_draws=4, seed=23, duration=4.0, sampling_frequency=1024.0,
+                                   detectors=("H1", "L1", "V1"), time=marginalised,
+                                   distance=marginalised, phase=marginalised)
+    opri = dict(s["priors"])
+    priors = s["priors"].copy()
+    flags = dict(time_marginalization=marginalised, distance_marginalization=marginalised,
+                 phase_marginalization=marginalised)
+    lk = GravitationalWaveTransient(s["interferometers"], s["waveform_generator"], priors=priors,
+                                    precision="fp64", **flags)
+    inj = dict(s["injection"])
+    p = dict(inj)
+    fixed = fixed_marginalised({k: np.array([v]) for k, v in inj.items()}, priors)
+    p.update({k: float(v[0]) for k, v in fixed.items()})
+    if marginalised:
+        p["time_jitter"] = 1.1e-4
+    out = lk.compute_per_detector_log_likelihood(p)
+    ifos = s["interferometers"]
+    for ifo in ifos:
+        det = OracleDetector(ifo.name, ifo.frequency_array, ifo.frequency_domain_strain,
+                             ifo.power_spectral_density_array, ifo.frequency_mask)
+        ev = OracleEvent([det], ifos.duration, ifos.sampling_frequency, ifos.start_time)
+        o = OracleGWTransient(ev, "taylorf2",
+                              waveform_arguments=dict(s["waveform_generator"].waveform_arguments),
+                              priors=dict(opri), **flags)
+        ref = o.log_likelihood_ratio(dict(p))
+        got = out["%s_log_likelihood" % ifo.name]
+        say(ifo.name, got, ref)
+        assert abs(got - ref) < 1e-7 * max(1.0, abs(ref))
+        assert p["%s_log_likelihood" % ifo.name] == got
+    if not marginalised:
+        total = sum(out["%s_log_likelihood" % i.name] for i in ifos)
+        assert abs(total - lk.log_likelihood_ratio(dict(inj))) < 1e-9 * abs(total)
+    assert lk.lal_version == "N/A"
