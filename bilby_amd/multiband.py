@@ -399,6 +399,16 @@ class MBGravitationalWaveTransient(GravitationalWaveTransient):
         raise ValueError("reference_chirp_mass is not given and the priors do not fix a minimum "
                          "chirp mass")
 
+    # multiband.py:128-309: the banding settings, read from the set-up
+    reference_chirp_mass = property(lambda self: self.setup.reference_chirp_mass)
+    highest_mode = property(lambda self: self.setup.highest_mode)
+    accuracy_factor = property(lambda self: self.setup.accuracy_factor)
+    time_offset = property(lambda self: self.setup.time_offset)
+    delta_f_end = property(lambda self: self.setup.delta_f_end)
+    maximum_banding_frequency = property(lambda self: self.setup.maximum_banding_frequency)
+    minimum_banding_duration = property(lambda self: self.setup.minimum_banding_duration)
+    linear_interpolation = property(lambda self: True)
+
     @property
     def weights(self):
         return self.setup.weights

@@ -145,6 +145,10 @@ class ROQGravitationalWaveTransient:
                 out[key] = np.asarray(self.weights[key])
         np.savez(filename, **out)
 
+    def load_weights(self, filename, format=None):
+        """roq.py:1107-1160 (npz; see the module-level load_weights)."""
+        return load_weights(filename, self.interferometers, self.waveform_generator, format)
+
     @staticmethod
     def _parse_basis(basis, basis_type):
         """roq.py:334-366: an array [n_freq, n_basis] (stored transposed) or a

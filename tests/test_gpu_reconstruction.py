@@ -246,6 +246,8 @@ def test_multiband_reconstruction_matches_oracle(distance):
                           time_marginalization=True, distance_marginalization=distance,
                           phase_marginalization=True, priors=dict(priors),
                           distance_table=getattr(lk, "_dist_margd_loglikelihood_array", None))
+    assert lk.accuracy_factor == 5 and lk.linear_interpolation and lk.highest_mode == 2
+    assert lk.reference_chirp_mass == lk.setup.reference_chirp_mass
     p = dict(inj, geocent_time=float(ifos.start_time), phase=0.0, time_jitter=3.1e-5)
     if distance:
         p["luminosity_distance"] = lk._ref_dist

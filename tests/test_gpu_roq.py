@@ -167,5 +167,7 @@ def test_roq_from_bases_matches_oracle(tmp_path):
                                         precision="fp64")
     assert np.array_equal(lk2.log_likelihood_ratio_batch({k: v[:n] for k, v in draws.items()}),
                           got)
+    again = lk.load_weights(str(tmp_path / "w.npz"))
+    assert np.array_equal(again["H1_linear"], w["H1_linear"])
     with pytest.raises(NotImplementedError):
         lk.save_weights(str(tmp_path / "w"))
