@@ -61,3 +61,28 @@ def test_errors_are_codes_not_exceptions():
     rc = LIB.gwl_spline2d_coefficients(5, 5, dptr(x), dptr(y), dptr(z), dptr(t), dptr(t), dptr(z))
     assert rc == -1
     assert b"increasing" in LIB.gwl_last_error()
+
+
+def test_new_entry_points_reject_null_handles_without_a_device():
+    """The reconstruction and series entry points validate their arguments
+    before any device call: a null handle or output is GWL_E_INVALID with a
+    message naming the entry point."""
+    from bilby_amd._lib import LIB, NPARAM
+    calls = {
+        "gwl_roq_inner_products": lambda: LIB.gwl_roq_inner_products(None, None, 1, NPARAM, 0, 0,
+                                                                     None, None, None),
+        "gwl_relbin_time_series": lambda: LIB.gwl_relbin_time_series(None, None, 1, NPARAM, 0,
+                                                                     None, None, None),
+        "gwl_mb_time_series": lambda: LIB.gwl_mb_time_series(None, None, 1, NPARAM, 0, None, None,
+                                                             None),
+        "gwl_relbin_set_upsampled": lambda: LIB.gwl_relbin_set_upsampled(None, None, None, None,
+                                                                         1 << 16),
+        "gwl_relbin_upsampled_series": lambda: LIB.gwl_relbin_upsampled_series(
+            None, None, 1, NPARAM, 0, 8, None, None, None),
+        "gwl_weighted_inner_products": lambda: LIB.gwl_weighted_inner_products(None, None, None,
+                                                                               None),
+    }
+    for name, call in calls.items():
+        assert call() == -1, name
+        msg = LIB.gwl_last_error()
+        assert name.encode() in msg or b"invalid" in msg, (name, msg)
