@@ -602,6 +602,16 @@ class GravitationalWaveTransient:
         times = ((tg - start) + j / 16384.0) % T + start
         prior = self.priors["geocent_time"]
         inside = (times >= prior.minimum) & (times < prior.maximum)
+        # the companion event's window covers the prior for rows at the
+        # likelihood's fixed time (the segment start, plus jitter); every grid
+        # time inside the prior must come from it
+        n_fft = int(round(16384 * T))
+        all_times = ((tg - start) + np.arange(n_fft) / 16384.0) % T + start
+        n_inside = int(np.count_nonzero((all_times >= prior.minimum) & (all_times < prior.maximum)))
+        if n_inside != int(np.count_nonzero(inside)):
+            raise NotImplementedError(
+                "time reconstruction is provided for geocent_time at the likelihood's fixed "
+                "time-marginalisation value (%r, plus the jitter); got %r" % (start, tg))
         times, d_inner_h = times[inside], d_inner_h[inside]
         if self.distance_marginalization:
             log_like = self._distance_marginalized_values(d_inner_h, h_inner_h, p)
