@@ -158,6 +158,8 @@ def _load():
         "gwl_relbin_upsampled_series": ([vp, vp, ctypes.c_int64, ctypes.c_int32, ctypes.c_int64,
                                          ctypes.c_int32, vp, vp, vp], ctypes.c_int),
         "gwl_weighted_inner_products": ([vp, _dp, _dp, _dp], ctypes.c_int),
+        "gwl_window_time_series": ([vp, _dp, ctypes.c_int64, ctypes.c_int64, ctypes.c_int32, _dp,
+                                    _dp], ctypes.c_int),
         "gwl_mb_time_series": ([vp, vp, ctypes.c_int64, ctypes.c_int32, ctypes.c_int32, vp, vp,
                                 vp], ctypes.c_int),
         "gwl_roq_inner_products": ([vp, vp, ctypes.c_int64, ctypes.c_int32, ctypes.c_int32,
@@ -224,5 +226,5 @@ EXPORTED_SYMBOLS = [
     "gwl_mb_set_time", "gwl_relbin_set_time", "gwl_eval_time_series", "gwl_event_time_window",
     "gwl_roq_weights_linear", "gwl_relbin_time_series", "gwl_mb_time_series",
     "gwl_roq_inner_products", "gwl_relbin_set_upsampled", "gwl_relbin_upsampled_series",
-    "gwl_weighted_inner_products",
+    "gwl_weighted_inner_products", "gwl_window_time_series",
 ]

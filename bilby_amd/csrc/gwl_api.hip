@@ -299,6 +299,46 @@ __global__ __launch_bounds__(256) void k_weighted_inner(EvDev ev, const double* 
   }
 }
 
+// One thread per window bin j = j_first + jj (j may pass n_fft: the
+// exponential wraps): sum over detectors and masked bins n of
+// h_n conj(d_n) / S_n exp(-2 pi i n j / n_fft) (without 4/T), the transform
+// of the zero-padded integrand of length n_fft; h the response
+// ([nd][nf][2]), d / S sigma from the event's float64 data.  Every lane of a
+// wave walks the same bins, so the loads are broadcasts.
+__global__ __launch_bounds__(256) void k_window_dft(EvDev ev, const double* resp,
+                                                     long long n_fft, long long j_first, int L,
+                                                     double* out) {
+  const int jj = blockIdx.x * blockDim.x + threadIdx.x;
+  if (jj >= L) return;
+  const unsigned long long j = (unsigned long long)(j_first + jj);
+  double sr = 0.0, si = 0.0;
+  for (int d = 0; d < ev.nd; ++d) {
+    for (int n = ev.mlo[d]; n <= ev.mhi[d] && n < ev.nf; ++n) {
+      double dr, di;
+      if (n == ev.N) {
+        dr = ev.dwN[d][0];
+        di = ev.dwN[d][1];
+      } else {
+        const size_t i = (size_t)d * ev.N + (size_t)(n % ev.Q) * ev.P + n / ev.Q;
+        dr = ev.dw64[i].x;
+        di = ev.dw64[i].y;
+      }
+      const size_t o = ((size_t)d * ev.nf + n) * 2;
+      const double hr = resp[o], hi = resp[o + 1];
+      // h conj(d / S sigma) / sigma
+      const double xr = (hr * dr + hi * di) / ev.sigma;
+      const double xi = (hi * dr - hr * di) / ev.sigma;
+      const unsigned long long r = ((unsigned long long)n * j) % (unsigned long long)n_fft;
+      double sn, cs;
+      sincospi(-2.0 * (double)r / (double)n_fft, &sn, &cs);
+      sr += xr * cs - xi * sn;
+      si += xr * sn + xi * cs;
+    }
+  }
+  out[2 * jj] = sr;
+  out[2 * jj + 1] = si;
+}
+
 }  // namespace gwl
 
 using namespace gwl;
@@ -759,6 +799,57 @@ int gwl_weighted_inner_products(gwl_event* e, const double* params, const double
   out[0] = out[1] = out[2] = 0.0;
   for (int d = 0; d < nd; ++d)
     for (int c = 0; c < 3; ++c) out[c] += norm * h[3 * d + c];
+  return GWL_OK;
+}
+
+int gwl_window_time_series(gwl_event* e, const double* params, int64_t n_fft, int64_t j_first,
+                           int32_t n_window, double* d_inner_h, double* h_inner_h) {
+  if (!e || !params || !d_inner_h || !h_inner_h || n_window < 1 || j_first < 0 ||
+      n_fft < e->dev.nf || j_first + (int64_t)n_window > 2 * n_fft) {
+    set_error("gwl_window_time_series: invalid argument");
+    return GWL_E_INVALID;
+  }
+  const int nd = e->dev.nd, nf = e->dev.nf;
+  const size_t fb = (size_t)2 * nd * nf * sizeof(double);
+  const size_t ob = (size_t)2 * n_window * sizeof(double);
+  GWL_HIP_CHECK(hipSetDevice(e->ctx->device));
+  char* mem = nullptr;
+  if (hipMalloc((void**)&mem, 256 + 256 + 2 * fb + ob) != hipSuccess) {
+    set_error("hipMalloc failed");
+    return GWL_E_NOMEM;
+  }
+  double* dp = reinterpret_cast<double*>(mem);
+  double* part = reinterpret_cast<double*>(mem + 256);
+  double* resp = reinterpret_cast<double*>(mem + 512);
+  double* ones = reinterpret_cast<double*>(mem + 512 + fb);
+  double* dout = reinterpret_cast<double*>(mem + 512 + 2 * fb);
+  std::vector<double> one((size_t)2 * nd * nf, 0.0);
+  for (size_t k = 0; k < one.size(); k += 2) one[k] = 1.0;
+  int rc = GWL_OK;
+  hipError_t err = hipMemcpy(dp, params, GWL_NPARAM * sizeof(double), hipMemcpyHostToDevice);
+  if (err == hipSuccess) err = hipMemcpy(ones, one.data(), fb, hipMemcpyHostToDevice);
+  for (int d = 0; err == hipSuccess && rc == GWL_OK && d < nd; ++d)
+    rc = launch_response(e, dp, d, GWL_FP64, resp + (size_t)2 * d * nf, nullptr);
+  double h[3 * GWL_MAX_DET];
+  if (err == hipSuccess && rc == GWL_OK) {
+    hipLaunchKernelGGL(k_window_dft, dim3((unsigned)((n_window + 255) / 256)), dim3(256), 0, 0,
+                       e->dev, resp, (long long)n_fft, (long long)j_first, (int)n_window, dout);
+    err = hipGetLastError();
+    if (err == hipSuccess)
+      hipLaunchKernelGGL(k_weighted_inner, dim3((unsigned)nd), dim3(256), 0, 0, e->dev, resp,
+                         ones, part);
+    if (err == hipSuccess) err = hipGetLastError();
+    if (err == hipSuccess) err = hipMemcpy(d_inner_h, dout, ob, hipMemcpyDeviceToHost);
+    if (err == hipSuccess) err = hipMemcpy(h, part, (size_t)3 * nd * 8, hipMemcpyDeviceToHost);
+  }
+  (void)hipFree(mem);
+  if (err != hipSuccess) return hip_fail(err, "gwl_window_time_series");
+  if (rc != GWL_OK) return rc;
+  const double norm = 4.0 / e->dev.T;
+  for (int k = 0; k < 2 * n_window; ++k) d_inner_h[k] *= norm;
+  double hh = 0.0;
+  for (int d = 0; d < nd; ++d) hh += norm * h[3 * d + 2];
+  *h_inner_h = hh;
   return GWL_OK;
 }
 

@@ -607,12 +607,12 @@ class GravitationalWaveTransient:
         # time inside the prior must come from it
         n_fft = int(round(16384 * T))
         all_times = ((tg - start) + np.arange(n_fft) / 16384.0) % T + start
-        n_inside = int(np.count_nonzero((all_times >= prior.minimum) & (all_times < prior.maximum)))
-        if n_inside != int(np.count_nonzero(inside)):
-            raise NotImplementedError(
-                "time reconstruction is provided for geocent_time at the likelihood's fixed "
-                "time-marginalisation value (%r, plus the jitter); got %r" % (start, tg))
-        times, d_inner_h = times[inside], d_inner_h[inside]
+        js = np.flatnonzero((all_times >= prior.minimum) & (all_times < prior.maximum))
+        if len(js) == int(np.count_nonzero(inside)):
+            times, d_inner_h = times[inside], d_inner_h[inside]
+        else:
+            # another geocent_time: the direct transform at the window it needs
+            times, d_inner_h, h_inner_h = self._direct_time_window(p, all_times, js, n_fft)
         if self.distance_marginalization:
             log_like = self._distance_marginalized_values(d_inner_h, h_inner_h, p)
         elif self.phase_marginalization:
@@ -621,6 +621,28 @@ class GravitationalWaveTransient:
             log_like = d_inner_h.real - h_inner_h / 2
         post = np.exp(log_like - np.max(log_like)) * prior.prob(times)
         return times, post
+
+    def _direct_time_window(self, p, all_times, js, n_fft):
+        """base.py:604-629 at the grid times ``js`` (ascending, possibly
+        wrapping past the segment end) by gwl_window_time_series on this
+        likelihood's event."""
+        if len(js) == 0:
+            raise ValueError("no reconstruction time lies inside the geocent_time prior")
+        gaps = np.flatnonzero(np.diff(js) > 1)
+        if len(gaps) == 0:
+            j0 = int(js[0])
+        elif len(gaps) == 1 and js[0] == 0 and js[-1] == n_fft - 1:
+            j0 = int(js[gaps[0] + 1])
+        else:
+            raise ValueError("the geocent_time prior is not one interval on the grid")
+        row = np.ascontiguousarray(self._row_builder_sky()(dict(p))[0])
+        out = np.zeros((len(js), 2))
+        hh = np.zeros(1)
+        check(LIB.gwl_window_time_series(self._event, dptr(row), n_fft, j0, len(js), dptr(out),
+                                         dptr(hh)), "gwl_window_time_series")
+        out = out[(js - j0) % n_fft]
+        T = float(self.interferometers.duration)
+        return all_times[js], (out[:, 0] + 1j * out[:, 1]) * (T / 4.0), float(hh[0])
 
     def _row_builder_sky(self):
         return _abi.RowBuilder(self.waveform_generator.parameter_conversion, {})

@@ -183,6 +183,18 @@ int gwl_event_timing_read(gwl_event* ev, double* total_ms, int64_t* launches);
 int gwl_weighted_inner_products(gwl_event* ev, const double* params, const double* factor,
                                 double* out);
 
+/* The transform of one row's zero-padded integrand at any window (the time
+ * reconstruction of base.py:604-629 for a geocent_time whose window the
+ * upsampled companion event does not hold):
+ *   d_inner_h[jj] = 4/T sum_d sum_n h_n conj(d_n) / S_n exp(-2 pi i n j / n_fft),
+ *   j = j_first + jj (j may pass n_fft, the exponential wraps), over each
+ *   detector's masked bins; h_inner_h = <h|h>.
+ * A direct float64 sum (one thread per window bin), for single rows.
+ * params: host [GWL_NPARAM] (sky frame); d_inner_h: host [n_window][2];
+ * h_inner_h: host [1].  n_fft >= n_freq; allocates its own workspace. */
+int gwl_window_time_series(gwl_event* ev, const double* params, int64_t n_fft, int64_t j_first,
+                           int32_t n_window, double* d_inner_h, double* h_inner_h);
+
 /* Detector response of one parameter row: bilby
  * Interferometer.get_detector_response (interferometer.py:303-368) of the
  * event's source model.  params: host [GWL_NPARAM]; out: host

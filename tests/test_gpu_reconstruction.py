@@ -49,9 +49,14 @@ def test_reconstruction_matches_oracle(case):
         say("time grid", len(t_g), "max |dpost|", float(np.max(np.abs(post_g - post_o))))
         assert len(t_g) == len(t_o) and np.max(np.abs(t_g - t_o)) < 1e-9
         assert np.max(np.abs(post_g - post_o)) < 1e-6 * np.max(post_o)
-        # a geocent_time away from the fixed value would need another window
-        with pytest.raises(NotImplementedError):
-            lk._time_posterior(dict(p, geocent_time=inj["geocent_time"]))
+        # a geocent_time away from the fixed value: the window it needs wraps past the
+        # segment end; the direct window transform (gwl_window_time_series) gives it
+        far = dict(p, geocent_time=inj["geocent_time"])
+        t_g, post_g = lk._time_posterior(far)
+        t_o, post_o = olk.time_posterior(far)
+        say("direct window", len(t_g), "max |dpost|", float(np.max(np.abs(post_g - post_o))))
+        assert len(t_g) == len(t_o) and np.max(np.abs(t_g - t_o)) < 1e-9
+        assert np.max(np.abs(post_g - post_o)) < 1e-6 * np.max(post_o)
     q = dict(p, geocent_time=inj["geocent_time"])
     if case["distance"]:
         d_g, post_g = lk._distance_posterior(q)
