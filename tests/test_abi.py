@@ -81,6 +81,8 @@ def test_new_entry_points_reject_null_handles_without_a_device():
             None, None, 1, NPARAM, 0, 8, None, None, None),
         "gwl_weighted_inner_products": lambda: LIB.gwl_weighted_inner_products(None, None, None,
                                                                                None),
+        "gwl_window_time_series": lambda: LIB.gwl_window_time_series(None, None, 1 << 16, 0, 8,
+                                                                     None, None),
     }
     for name, call in calls.items():
         assert call() == -1, name
