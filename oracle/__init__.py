@@ -23,8 +23,9 @@ Pinning status (what checks each function, see tests/test_oracle_*.py):
 * TaylorF2: partially pinned (Newtonian SPA amplitude/phase vs an FFT of a
   numerically generated Newtonian chirp; every non-spinning phase
   coefficient, 1PN-3.5PN with the 2.5PN and 3PN logarithms, vs the 3PN
-  energy / 3.5PN flux re-expansion; the 1.5PN spin-orbit term).  The
-  higher spin terms are transcribed only.  Parity with lalsimulation is
+  energy / 3.5PN flux re-expansion; the spin-orbit terms at 1.5-3.5PN vs
+  the spin-orbit energy and flux).  The 2PN spin-spin terms are
+  transcribed only.  Parity with lalsimulation is
   UNPINNED (lalsuite is not available).
 * time marginalisation: pinned by brute-force DFT and by brute-force
   trapezoid integration of the un-marginalised likelihood.

@@ -264,6 +264,73 @@ def test_taylorf2_high_order_coefficients_from_energy_and_flux():
             assert abs(got - want) < 1e-9 * max(1.0, abs(want)), (eta, name, got, want)
 
 
+def test_taylorf2_spin_orbit_coefficients_from_energy_and_flux():
+    """The aligned-spin spin-orbit phase terms at 1.5PN, 2.5PN (logarithm),
+    3PN and 3.5PN from the spin-orbit parts of the binding energy and flux
+    (Blanchet, Living Rev. Rel. 17, 2, Eqs. 414-415; M = 1,
+    S_l = m1^2 chi1 + m2^2 chi2, Sigma_l = m2 chi2 - m1 chi1, delta = m1 - m2):
+      E: x^(3/2) [14/3 S_l + 2 delta Sigma_l]
+         + x^(5/2) [(11 - 61 eta/9) S_l + delta (3 - 10 eta/3) Sigma_l]
+         + x^(7/2) [(135/4 - 367 eta/4 + 29 eta^2/12) S_l
+                    + delta (27/4 - 39 eta + 5 eta^2/4) Sigma_l],
+      F: x^(3/2) [-4 S_l - 5/4 delta Sigma_l]
+         + x^(5/2) [(-9/2 + 272 eta/9) S_l + (-13/16 + 43 eta/4) delta Sigma_l]
+         + x^3 [-16 pi S_l - 31 pi/6 delta Sigma_l]
+         + x^(7/2) [(476645/6804 + 6172 eta/189 - 2810 eta^2/27) S_l
+                    + (9535/336 + 1849 eta/126 - 1501 eta^2/36) delta Sigma_l].
+    The linear-in-spin part of each coefficient (a symmetric difference in
+    the spins, which removes the spin-squared products of the re-expansion)
+    equals the implementation's."""
+    from oracle.core import EULER_GAMMA
+    pi = np.pi
+
+    def derived(m1, m2, c1, c2):
+        M = m1 + m2
+        m1, m2 = m1 / M, m2 / M
+        eta, delta = m1 * m2, m1 - m2
+        S = m1 ** 2 * c1 + m2 ** 2 * c2
+        dS = delta * (m2 * c2 - m1 * c1)
+        e = np.zeros(8)
+        e[0] = 1.0
+        e[2] = -(3 / 4 + eta / 12)
+        e[3] = 14 / 3 * S + 2 * dS
+        e[4] = -(27 / 8 - 19 * eta / 8 + eta ** 2 / 24)
+        e[5] = (11 - 61 / 9 * eta) * S + (3 - 10 / 3 * eta) * dS
+        e[6] = (-675 / 64 + (34445 / 576 - 205 * pi ** 2 / 96) * eta - 155 * eta ** 2 / 96
+                - 35 * eta ** 3 / 5184)
+        e[7] = ((135 / 4 - 367 / 4 * eta + 29 / 12 * eta ** 2) * S
+                + (27 / 4 - 39 * eta + 5 / 4 * eta ** 2) * dS)
+        de = np.array([(k + 2) / 2 * e[k] for k in range(8)])
+        f = np.zeros(8)
+        f[0] = 1.0
+        f[2] = -(1247 / 336 + 35 * eta / 12)
+        f[3] = 4 * pi - 4 * S - 5 / 4 * dS
+        f[4] = -44711 / 9072 + 9271 * eta / 504 + 65 * eta ** 2 / 18
+        f[5] = (-(8191 / 672 + 583 * eta / 24) * pi + (-9 / 2 + 272 / 9 * eta) * S
+                + (-13 / 16 + 43 / 4 * eta) * dS)
+        f[6] = (6643739519 / 69854400 + 16 * pi ** 2 / 3 - 1712 * EULER_GAMMA / 105
+                - 856 / 105 * np.log(16) + (-134543 / 7776 + 41 * pi ** 2 / 48) * eta
+                - 94403 * eta ** 2 / 3024 - 775 * eta ** 3 / 324 - 16 * pi * S - 31 * pi / 6 * dS)
+        f[7] = ((-16285 / 504 + 214745 * eta / 1728 + 193385 * eta ** 2 / 3024) * pi
+                + (476645 / 6804 + 6172 / 189 * eta - 2810 / 27 * eta ** 2) * S
+                + (9535 / 336 + 1849 / 126 * eta - 1501 / 36 * eta ** 2) * dS)
+        a = np.zeros(8)
+        for k in range(8):
+            a[k] = de[k] - sum(a[j] * f[k - j] for j in range(k))
+        return np.array([4 * a[3], -40 / 3 * a[5], -20 * a[6] + 10 * 1712 / 105, -20 * a[7]])
+
+    def implemented(m1, m2, c1, c2):
+        phi, phil, _ = core.taylorf2_phase_coefficients(m1, m2, c1, c2)
+        return np.array([phi[3], phil[5], phi[6], phi[7]])
+
+    eps = 1e-4
+    for m1, m2, c1, c2 in [(30.0, 12.0, 0.7, -0.4), (10.0, 9.0, -0.3, 0.8), (20.0, 5.0, 0.5, 0.5)]:
+        got = (implemented(m1, m2, eps * c1, eps * c2)
+               - implemented(m1, m2, -eps * c1, -eps * c2)) / (2 * eps)
+        want = (derived(m1, m2, eps * c1, eps * c2) - derived(m1, m2, -eps * c1, -eps * c2)) / (2 * eps)
+        assert np.max(np.abs(got - want) / np.maximum(1.0, np.abs(want))) < 1e-7, (got, want)
+
+
 def _masses_for_eta(eta):
     # m1 + m2 = 1, m1 m2 = eta
     d = np.sqrt(1 - 4 * eta)
