@@ -423,3 +423,51 @@ def test_per_detector_log_likelihoods(marginalised):
         total = sum(out["%s_log_likelihood" % i.name] for i in ifos)
         assert abs(total - lk.log_likelihood_ratio(dict(inj))) < 1e-9 * abs(total)
     assert lk.lal_version == "N/A"
+
+
+def test_reconstruction_at_the_headline_shape():
+    """The reconstruction chain (time on the 1/16384 s grid: transform
+    length 131,072; distance; phase) on the 8 s @ 4096 Hz H1+L1+V1 event
+    with time, distance and phase marginalised, against the oracle with the
+    same draws; and the direct window transform at the injected time."""
+    from bilby_amd.configs import make_taylorf2_marginalised
+    from bilby_amd.likelihood import GravitationalWaveTransient, interped_sample
+    from tests.helpers import fixed_marginalised, oracle_from_setup
+    say("reconstruction, 8 s @ 4096 Hz, H1+L1+V1")
+    s = make_taylorf2_marginalised(n_draws=4, seed=27, duration=8.0, sampling_frequency=4096.0)
+    priors = s["priors"].copy()
+    lk = GravitationalWaveTransient(s["interferometers"], s["waveform_generator"],
+                                    time_marginalization=True, distance_marginalization=True,
+                                    phase_marginalization=True, priors=priors, precision="fp64")
+    olk, _ = oracle_from_setup(s)
+    inj = dict(s["injection"])
+    p = dict(inj)
+    fixed = fixed_marginalised({k: np.array([v]) for k, v in inj.items()}, priors)
+    p.update({k: float(v[0]) for k, v in fixed.items()})
+    p["time_jitter"] = -2.2e-4
+    t_g, post_g = lk._time_posterior(p)
+    t_o, post_o = olk.time_posterior(p)
+    say("grid", len(t_g), "max |dpost|", float(np.max(np.abs(post_g - post_o))))
+    assert len(t_g) == len(t_o) and np.max(np.abs(t_g - t_o)) < 1e-9
+    assert np.max(np.abs(post_g - post_o)) < 1e-6 * np.max(post_o)
+    far = dict(p, geocent_time=inj["geocent_time"] + 0.03)
+    t_g, post_g = lk._time_posterior(far)
+    t_o, post_o = olk.time_posterior(far)
+    assert np.max(np.abs(post_g - post_o)) < 1e-6 * np.max(post_o)
+    out = lk.generate_posterior_sample_from_marginalized_likelihood(dict(p),
+                                                                    rng=np.random.default_rng(9))
+    rng = np.random.default_rng(9)
+    ref = dict(p)
+    t_o, post_o = olk.time_posterior(ref)
+    keep = post_o > np.max(post_o) / 1000
+    if np.sum(keep) < 3:
+        keep[1:-1] = keep[1:-1] | keep[2:] | keep[:-2]
+    ref["geocent_time"] = interped_sample(t_o[keep], post_o[keep], rng)
+    ref["luminosity_distance"] = interped_sample(*olk.distance_posterior(ref), rng)
+    ref["phase"] = interped_sample(*olk.phase_posterior(ref), rng)
+    say({k: (out[k], ref[k]) for k in ("geocent_time", "luminosity_distance", "phase")})
+    assert abs(out["geocent_time"] - ref["geocent_time"]) < 1e-7
+    assert abs(out["luminosity_distance"] - ref["luminosity_distance"]) < 1e-6 * ref[
+        "luminosity_distance"]
+    assert abs(out["phase"] - ref["phase"]) < 1e-6
+    assert abs(out["geocent_time"] - inj["geocent_time"]) < 2e-3
