@@ -24,6 +24,7 @@
 // logsumexp, one wavefront per row with the lanes over curves.
 #include "gwl_internal.h"
 #include "gwl_bins.h"
+#include "gwl_window.h"
 
 #include <cmath>
 #include <vector>
@@ -34,6 +35,10 @@ namespace {
 
 constexpr int kGM = 128;  // GEMM tile (M, N multiples), gwl_gemm
 constexpr int kGK = 32;   // GEMM K multiple
+// float32 GEMMs split K into at most this many chunks (about 2048 workgroups
+// for the two long-K contractions, whose row x curve tiles alone are too
+// few to fill 256 CUs); the epilogue adds the chunk partials in float64
+constexpr int kCalSplit = 8;
 
 size_t up(size_t x, size_t a) { return (x + a - 1) / a * a; }
 
@@ -90,8 +95,9 @@ struct CalArgs {
   void* a2_lo;
   double* tot;       // [Mp][3]: Re D0, Im D0, H0
   int* valid;        // [Mp]
-  const void* g1;    // [Mp][N1p] (double or float)
-  const void* g2;    // [Mp][N2p]
+  const void* g1;    // [nz1][Mp][N1p] (double, nz1 = 1; or float chunk partials)
+  const void* g2;    // [nz2][Mp][N2p]
+  int nz1, nz2;      // K chunks of the float32 GEMMs, summed in float64 here
   double* out;
   double* per_curve; // [n][nc] or null
 };
@@ -208,9 +214,13 @@ __global__ __launch_bounds__(256) void k_cal_prep(CalArgs a) {
   }
 }
 
-__device__ __forceinline__ double gval(const CalArgs& a, const void* g, size_t i) {
-  return a.mode == CAL_FP64 ? static_cast<const double*>(g)[i]
-                            : (double)static_cast<const float*>(g)[i];
+__device__ __forceinline__ double gval(const CalArgs& a, const void* g, size_t i, int nz,
+                                       size_t zs) {
+  if (a.mode == CAL_FP64) return static_cast<const double*>(g)[i];
+  const float* f = static_cast<const float*>(g);
+  double s = 0.0;
+  for (int z = 0; z < nz; ++z) s += (double)f[(size_t)z * zs + i];
+  return s;
 }
 
 // One wavefront per row, lanes over curves: ln L_c and their logsumexp
@@ -232,9 +242,10 @@ __global__ __launch_bounds__(256) void k_cal_epi(CalArgs a) {
   const double hs = dist ? a.params[r * a.ld + GWL_P_LUMINOSITY_DISTANCE] / ev.dref : 1.0;
   double m = -INFINITY, s = 0.0;
   for (int c = lane; c < a.nc; c += 64) {
-    const double dr = d0r + gval(a, a.g1, (size_t)r * a.N1p + c);
-    const double di = d0i + gval(a, a.g1, (size_t)r * a.N1p + a.nc + c);
-    const double hh = h0 + gval(a, a.g2, (size_t)r * a.N2p + c);
+    const size_t z1 = (size_t)a.Mp * a.N1p, z2 = (size_t)a.Mp * a.N2p;
+    const double dr = d0r + gval(a, a.g1, (size_t)r * a.N1p + c, a.nz1, z1);
+    const double di = d0i + gval(a, a.g1, (size_t)r * a.N1p + a.nc + c, a.nz1, z1);
+    const double hh = h0 + gval(a, a.g2, (size_t)r * a.N2p + c, a.nz2, z2);
     double v;
     if (dist) {
       const double dd = (ev.flags & GWL_PHASE_MARG) ? hypot(dr, di) * hs : dr * hs;
@@ -258,7 +269,8 @@ size_t cal_ws_bytes(const gwl_calib* c, long long rows) {
   size_t b = 0;
   b += up(Mp * c->K1p * 8, 256) + up(Mp * c->K1p * 4, 256);  // A1 (largest element) + bf16 lo
   b += up(Mp * c->K2p * 8, 256) + up(Mp * c->K2p * 4, 256);
-  b += up(Mp * c->N1p * 8, 256) + up(Mp * c->N2p * 8, 256);  // G1, G2
+  // G1, G2: float64, or up to kCalSplit float32 K-chunk partials
+  b += up(Mp * c->N1p * 4 * kCalSplit, 256) + up(Mp * c->N2p * 4 * kCalSplit, 256);
   b += up(Mp * 3 * 8, 256) + up(Mp * 4, 256);                 // totals, valid
   return b;
 }
@@ -512,9 +524,9 @@ int gwl_calib_eval(gwl_calib* c, const double* params, int64_t n, int32_t ld, in
   a.a2_lo = w + (size_t)Mp * c->K2p * 2;
   w += up((size_t)Mp * c->K2p * 8, 256) + up((size_t)Mp * c->K2p * 4, 256);
   void* g1 = w;
-  w += up((size_t)Mp * c->N1p * 8, 256);
+  w += up((size_t)Mp * c->N1p * 4 * kCalSplit, 256);
   void* g2 = w;
-  w += up((size_t)Mp * c->N2p * 8, 256);
+  w += up((size_t)Mp * c->N2p * 4 * kCalSplit, 256);
   a.tot = reinterpret_cast<double*>(w);
   w += up((size_t)Mp * 3 * 8, 256);
   a.valid = reinterpret_cast<int*>(w);
@@ -536,12 +548,35 @@ int gwl_calib_eval(gwl_calib* c, const double* params, int64_t n, int32_t ld, in
                    : mode == gwl::CAL_FP32 ? (const void*)c->b2_32 : (const void*)c->b2_hi;
   const void* b1lo = mode == gwl::CAL_BF16X3 ? (const void*)c->b1_lo : nullptr;
   const void* b2lo = mode == gwl::CAL_BF16X3 ? (const void*)c->b2_lo : nullptr;
-  if (int rc = gwl_gemm(ev->ctx, mode, a.a1, mode == gwl::CAL_BF16X3 ? a.a1_lo : nullptr, b1, b1lo,
-                        g1, Mp, c->N1p, c->K1p, stream))
-    return rc;
-  if (int rc = gwl_gemm(ev->ctx, mode, a.a2, mode == gwl::CAL_BF16X3 ? a.a2_lo : nullptr, b2, b2lo,
-                        g2, Mp, c->N2p, c->K2p, stream))
-    return rc;
+  a.nz1 = a.nz2 = 1;
+  if (mode == gwl::CAL_FP32) {
+    // split K: chunks (multiples of 32 columns) sized for ~2048 workgroups
+    auto chunk = [&](int Np, int Kp, int* nz) {
+      const long long tiles = (long long)(Np / kGM) * (Mp / kGM);
+      const int want = (int)std::max<long long>(1, std::min<long long>(kCalSplit,
+                                                                         (2048 + tiles - 1) / tiles));
+      const int kc = (int)up((size_t)((Kp + want - 1) / want), kGK);
+      *nz = (Kp + kc - 1) / kc;
+      return kc;
+    };
+    const int kc1 = chunk(c->N1p, c->K1p, &a.nz1);
+    const int kc2 = chunk(c->N2p, c->K2p, &a.nz2);
+    if (int rc = gwl::win_gemm_f32(static_cast<const float*>(a.a1),
+                                   static_cast<const float*>(b1), static_cast<float*>(g1), Mp,
+                                   c->N1p, c->K1p, kc1, st))
+      return rc;
+    if (int rc = gwl::win_gemm_f32(static_cast<const float*>(a.a2),
+                                   static_cast<const float*>(b2), static_cast<float*>(g2), Mp,
+                                   c->N2p, c->K2p, kc2, st))
+      return rc;
+  } else {
+    if (int rc = gwl_gemm(ev->ctx, mode, a.a1, mode == gwl::CAL_BF16X3 ? a.a1_lo : nullptr, b1,
+                          b1lo, g1, Mp, c->N1p, c->K1p, stream))
+      return rc;
+    if (int rc = gwl_gemm(ev->ctx, mode, a.a2, mode == gwl::CAL_BF16X3 ? a.a2_lo : nullptr, b2,
+                          b2lo, g2, Mp, c->N2p, c->K2p, stream))
+      return rc;
+  }
   const size_t lds = (ev->dev.flags & GWL_DISTANCE_MARG)
                          ? (size_t)(ev->dev.sp.nx + ev->dev.sp.ny + 8) * 8 : 0;
   hipLaunchKernelGGL(gwl::k_cal_epi, dim3((unsigned)((n + 3) / 4)), dim3(256), lds, st, a);

@@ -47,6 +47,13 @@ constexpr int kWinKc = 1024;
 
 void win_basis_free(WinBasis* wb);
 
+// C[z][Mp][Np] (float32 partials) = A[Mp][Kp] . Bt[Np][Kp]^T over the
+// column chunks [z kc, (z + 1) kc), z < ceil(Kp / kc), each summed in float32
+// with a second accumulation level every 256 columns; the caller adds the
+// chunks in float64.  Mp, Np multiples of 128; Kp, kc multiples of 32.
+int win_gemm_f32(const float* A, const float* Bt, float* C, int Mp, int Np, int Kp, int kc,
+                 hipStream_t st);
+
 // Relative-binning basis: for detector d and edge e (k = d (nb + 1) + e),
 //   M[k][j] = sum_n x_d[n] phi_e(n) exp(-2 pi i n j / N)
 // over the grid bins n of the two bins adjacent to edge e, phi_e the hat

@@ -138,13 +138,14 @@ constexpr int kBM = 128, kBN = 128, kBK = 32, kFlush = 8;
 
 __global__ __launch_bounds__(256) void k_win_gemm_f32(const float* __restrict__ A,
                                                       const float* __restrict__ Bt,
-                                                      float* __restrict__ C, int M, int N, int K) {
+                                                      float* __restrict__ C, int M, int N, int K,
+                                                      int kc) {
   __shared__ float As[kBM][kBK + 1];
   __shared__ float Bs[kBN][kBK + 1];
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int m0 = blockIdx.y * kBM, n0 = blockIdx.x * kBN;
   const int wr = (w >> 1) * 64, wc = (w & 1) * 64;
-  const int kb = blockIdx.z * kWinKc, ke = min(K, kb + kWinKc);
+  const int kb = blockIdx.z * kc, ke = min(K, kb + kc);
   f32x16 acc[2][2];
   float tot[2][2][16];
 #pragma unroll
@@ -329,6 +330,19 @@ int pack_basis(gwl_event* ev, const double2* m, int K, int L, WinBasis* wb) {
 
 }  // namespace
 
+int win_gemm_f32(const float* A, const float* Bt, float* C, int Mp, int Np, int Kp, int kc,
+                 hipStream_t st) {
+  if (Mp % kBM || Np % kBN || Kp % kBK || kc % kBK || kc <= 0) {
+    set_error("win_gemm_f32: shapes must be multiples of the tile");
+    return GWL_E_INVALID;
+  }
+  const int nz = (Kp + kc - 1) / kc;
+  hipLaunchKernelGGL(k_win_gemm_f32, dim3((unsigned)(Np / kBN), (unsigned)(Mp / kBM), (unsigned)nz),
+                     dim3(256), 0, st, A, Bt, C, Mp, Np, Kp, kc);
+  GWL_HIP_CHECK(hipGetLastError());
+  return GWL_OK;
+}
+
 void win_basis_free(WinBasis* wb) {
   if (wb->data) (void)hipFree(wb->data);
   *wb = WinBasis();
@@ -429,7 +443,7 @@ int win_finish(gwl_event* ev, const WinBasis& wb, const void* a, bool fp64, void
     hipLaunchKernelGGL(k_win_gemm_f32,
                        dim3((unsigned)(wb.Np / kBN), (unsigned)(Mp / kBM), (unsigned)nchunk),
                        dim3(256), 0, st, static_cast<const float*>(a), wb.bt32,
-                       static_cast<float*>(c), (int)Mp, wb.Np, wb.Kp);
+                       static_cast<float*>(c), (int)Mp, wb.Np, wb.Kp, kWinKc);
     GWL_HIP_CHECK(hipGetLastError());
   }
   WinEpiArgs e;
