@@ -549,16 +549,16 @@ int gwl_calib_eval(gwl_calib* c, const double* params, int64_t n, int32_t ld, in
   const void* b1lo = mode == gwl::CAL_BF16X3 ? (const void*)c->b1_lo : nullptr;
   const void* b2lo = mode == gwl::CAL_BF16X3 ? (const void*)c->b2_lo : nullptr;
   a.nz1 = a.nz2 = 1;
-  if (mode == gwl::CAL_FP32) {
-    // split K: chunks (multiples of 32 columns) sized for ~2048 workgroups
-    auto chunk = [&](int Np, int Kp, int* nz) {
+  // split K: chunks (multiples of 32 columns) sized for ~2048 workgroups
+  auto chunk = [&](int Np, int Kp, int* nz) {
       const long long tiles = (long long)(Np / kGM) * (Mp / kGM);
       const int want = (int)std::max<long long>(1, std::min<long long>(kCalSplit,
                                                                          (2048 + tiles - 1) / tiles));
       const int kc = (int)up((size_t)((Kp + want - 1) / want), kGK);
       *nz = (Kp + kc - 1) / kc;
       return kc;
-    };
+  };
+  if (mode == gwl::CAL_FP32) {
     const int kc1 = chunk(c->N1p, c->K1p, &a.nz1);
     const int kc2 = chunk(c->N2p, c->K2p, &a.nz2);
     if (int rc = gwl::win_gemm_f32(static_cast<const float*>(a.a1),
@@ -569,12 +569,25 @@ int gwl_calib_eval(gwl_calib* c, const double* params, int64_t n, int32_t ld, in
                                    static_cast<const float*>(b2), static_cast<float*>(g2), Mp,
                                    c->N2p, c->K2p, kc2, st))
       return rc;
-  } else {
-    if (int rc = gwl_gemm(ev->ctx, mode, a.a1, mode == gwl::CAL_BF16X3 ? a.a1_lo : nullptr, b1,
-                          b1lo, g1, Mp, c->N1p, c->K1p, stream))
+  } else if (mode == gwl::CAL_BF16X3) {
+    const int kc1 = chunk(c->N1p, c->K1p, &a.nz1);
+    const int kc2 = chunk(c->N2p, c->K2p, &a.nz2);
+    if (int rc = gwl::gemm_bf16x3_split(
+            static_cast<const __bf16*>(a.a1), static_cast<const __bf16*>(a.a1_lo),
+            static_cast<const __bf16*>(b1), static_cast<const __bf16*>(b1lo),
+            static_cast<float*>(g1), Mp, c->N1p, c->K1p, kc1, st))
       return rc;
-    if (int rc = gwl_gemm(ev->ctx, mode, a.a2, mode == gwl::CAL_BF16X3 ? a.a2_lo : nullptr, b2,
-                          b2lo, g2, Mp, c->N2p, c->K2p, stream))
+    if (int rc = gwl::gemm_bf16x3_split(
+            static_cast<const __bf16*>(a.a2), static_cast<const __bf16*>(a.a2_lo),
+            static_cast<const __bf16*>(b2), static_cast<const __bf16*>(b2lo),
+            static_cast<float*>(g2), Mp, c->N2p, c->K2p, kc2, st))
+      return rc;
+  } else {
+    if (int rc = gwl_gemm(ev->ctx, mode, a.a1, nullptr, b1, nullptr, g1, Mp, c->N1p, c->K1p,
+                          stream))
+      return rc;
+    if (int rc = gwl_gemm(ev->ctx, mode, a.a2, nullptr, b2, nullptr, g2, Mp, c->N2p, c->K2p,
+                          stream))
       return rc;
   }
   const size_t lds = (ev->dev.flags & GWL_DISTANCE_MARG)

@@ -106,6 +106,10 @@ int launch_eval(gwl_event* ev, const double* params, int64_t n, int ld, int prec
                 double* aux, hipStream_t stream);
 int timing_begin(gwl_event* ev, hipStream_t stream);
 int timing_end(gwl_event* ev, hipStream_t stream);
+// Split-bfloat16 x3 GEMM over K chunks into float32 partials C[z][M][N]
+// (z < ceil(K / kcol)); M, N multiples of 128, K and kcol multiples of 32.
+int gemm_bf16x3_split(const __bf16* Ah, const __bf16* Al, const __bf16* Bh, const __bf16* Bl,
+                      float* C, int M, int N, int K, int kcol, hipStream_t st);
 int launch_response(gwl_event* ev, const double* params_dev, int det, int precision, double* out_dev,
                     hipStream_t stream);
 // float64 epilogue of the time-marginalised path over G window partials

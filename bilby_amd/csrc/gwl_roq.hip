@@ -296,7 +296,8 @@ __global__ __launch_bounds__(256) void k_gemm_bf16x3(const __bf16* __restrict__ 
                                                      const __bf16* __restrict__ Al,
                                                      const __bf16* __restrict__ Bh,
                                                      const __bf16* __restrict__ Bl,
-                                                     float* __restrict__ C, int M, int N, int K) {
+                                                     float* __restrict__ C, int M, int N, int K,
+                                                     int kcol) {
   __shared__ __attribute__((aligned(16))) __bf16 sAh[kBM][kBK + kPadH];
   __shared__ __attribute__((aligned(16))) __bf16 sAl[kBM][kBK + kPadH];
   __shared__ __attribute__((aligned(16))) __bf16 sBh[kBN][kBK + kPadH];
@@ -311,7 +312,10 @@ __global__ __launch_bounds__(256) void k_gemm_bf16x3(const __bf16* __restrict__ 
     for (int j = 0; j < 2; ++j)
 #pragma unroll
       for (int v = 0; v < 16; ++v) acc[i][j][v] = 0.f;
-  for (int k0 = 0; k0 < K; k0 += kBK) {
+  // K chunk blockIdx.z (columns [z kcol, (z + 1) kcol)) into partial plane z
+  const int kb = blockIdx.z * kcol, ke = min(K, kb + kcol);
+  C += (size_t)blockIdx.z * M * N;
+  for (int k0 = kb; k0 < ke; k0 += kBK) {
     // 128 rows x 32 bf16 = 512 chunks of 8 per operand: 2 per thread
 #pragma unroll
     for (int q = 0; q < 2; ++q) {
@@ -611,6 +615,23 @@ static size_t roq_ws_bytes(const RoqDev& rd, int64_t n, int mode) {
   return a + c + (size_t)n * sizeof(RoqRow) + 3 * 256;
 }
 
+// C[z][M][N] float32 partials of Ah/Al . Bh/Bl^T (split bfloat16 x3) over the
+// column chunks [z kcol, (z + 1) kcol); M, N multiples of 128, K and kcol
+// multiples of 32.
+int gwl::gemm_bf16x3_split(const __bf16* Ah, const __bf16* Al, const __bf16* Bh,
+                           const __bf16* Bl, float* C, int M, int N, int K, int kcol,
+                           hipStream_t st) {
+  if (M % kBM || N % kBN || K % kBK || kcol % kBK || kcol <= 0) {
+    set_error("gemm_bf16x3_split: shapes must be multiples of the tile");
+    return GWL_E_INVALID;
+  }
+  const int nz = (K + kcol - 1) / kcol;
+  hipLaunchKernelGGL(k_gemm_bf16x3, dim3((unsigned)(N / kBN), (unsigned)(M / kBM), (unsigned)nz),
+                     dim3(256), 0, st, Ah, Al, Bh, Bl, C, M, N, K, kcol);
+  GWL_HIP_CHECK(hipGetLastError());
+  return GWL_OK;
+}
+
 // Workspace, node waveforms (k_roq_prep) and the weight GEMM of one call:
 // on return *C holds the products Z [Mp][Np] and *rows the row constants.
 static int roq_prep_gemm(gwl_roq* e, const double* params, int64_t n, int32_t ld, int32_t mode,
@@ -657,7 +678,7 @@ static int roq_prep_gemm(gwl_roq* e, const double* params, int64_t n, int32_t ld
   } else {
     hipLaunchKernelGGL(k_gemm_bf16x3, dim3(rd.Np / kBN, Mp / kBM), dim3(256), 0, st,
                        reinterpret_cast<const __bf16*>(A), reinterpret_cast<const __bf16*>(Alo),
-                       rd.bth, rd.btl, reinterpret_cast<float*>(Cp), Mp, rd.Np, rd.Kp);
+                       rd.bth, rd.btl, reinterpret_cast<float*>(Cp), Mp, rd.Np, rd.Kp, rd.Kp);
   }
   GWL_HIP_CHECK(hipGetLastError());
   *C = Cp;
@@ -952,7 +973,7 @@ int gwl_gemm(gwl_ctx* ctx, int32_t mode, const void* A, const void* A_lo, const 
   else
     hipLaunchKernelGGL(k_gemm_bf16x3, dim3(N / kBN, M / kBM), dim3(256), 0, st,
                        (const __bf16*)A, (const __bf16*)A_lo, (const __bf16*)Bt,
-                       (const __bf16*)Bt_lo, (float*)C, M, N, K);
+                       (const __bf16*)Bt_lo, (float*)C, M, N, K, K);
   GWL_HIP_CHECK(hipGetLastError());
   return GWL_OK;
 }
