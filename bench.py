@@ -857,8 +857,9 @@ def main():
                          "0 automatic, 1 tiled, 2 wave per row, 3 chirp-z")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU only; --gpus %d is not supported in this build "
-              "(rows would shard across GPUs with no exchange, see DESIGN.md)" % args.gpus,
+        print("bench.py measures one GPU only; --gpus %d is not supported (parameter rows "
+              "would shard across GPUs, each holding the event data, with only the 8-byte "
+              "results gathered)" % args.gpus,
               file=sys.stderr)
         sys.exit(2)
 
