@@ -369,42 +369,58 @@ __global__ __launch_bounds__(256) void k_gemm_bf16x3(const __bf16* __restrict__ 
 }
 
 // float64 reference GEMM: 64 x 64 tile, 256 threads x 4 x 4 outputs.
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+
+// C[M][N] = A[M][K] . Bt[N][K]^T in float64 on the matrix cores
+// (v_mfma_f64_16x16x4f64): 64 x 64 tiles, four wavefronts of 2 x 2 16 x 16
+// blocks, K staged through LDS 16 columns at a time.  Lane l holds
+// A[l % 16][l / 16] and B[l / 16][l % 16] of a 16 x 4 step and accumulator r
+// holds D[4 r + l / 16][l % 16].  M, N multiples of 64, K of 16.
 __global__ __launch_bounds__(256) void k_gemm_f64(const double* __restrict__ A,
                                                   const double* __restrict__ Bt,
                                                   double* __restrict__ C, int M, int N, int K) {
-  __shared__ double As[16][64 + 1];
-  __shared__ double Bs[16][64 + 1];
-  const int tid = threadIdx.x;
+  __shared__ double As[64][16 + 1];
+  __shared__ double Bs[64][16 + 1];
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int m0 = blockIdx.y * 64, n0 = blockIdx.x * 64;
-  const int tr = (tid >> 4) * 4, tc = (tid & 15) * 4;
-  double acc[4][4] = {};
+  const int wr = (w >> 1) * 32, wc = (w & 1) * 32;
+  f64x4 acc[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) acc[i][j] = f64x4{0.0, 0.0, 0.0, 0.0};
   for (int k0 = 0; k0 < K; k0 += 16) {
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
       const int idx = tid + 256 * q;  // 1024 = 64 rows x 16 k
       const int row = idx >> 4, kk = idx & 15;
-      As[kk][row] = A[(size_t)(m0 + row) * K + k0 + kk];
-      Bs[kk][row] = Bt[(size_t)(n0 + row) * K + k0 + kk];
+      As[row][kk] = A[(size_t)(m0 + row) * K + k0 + kk];
+      Bs[row][kk] = Bt[(size_t)(n0 + row) * K + k0 + kk];
     }
     __syncthreads();
 #pragma unroll
-    for (int kk = 0; kk < 16; ++kk) {
-      double a[4], b[4];
+    for (int kk = 0; kk < 16; kk += 4) {
+      double a[2], b[2];
 #pragma unroll
-      for (int i = 0; i < 4; ++i) a[i] = As[kk][tr + i];
+      for (int i = 0; i < 2; ++i) a[i] = As[wr + i * 16 + (lane & 15)][kk + (lane >> 4)];
 #pragma unroll
-      for (int j = 0; j < 4; ++j) b[j] = Bs[kk][tc + j];
+      for (int j = 0; j < 2; ++j) b[j] = Bs[wc + j * 16 + (lane & 15)][kk + (lane >> 4)];
 #pragma unroll
-      for (int i = 0; i < 4; ++i)
+      for (int i = 0; i < 2; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) acc[i][j] = fma(a[i], b[j], acc[i][j]);
+        for (int j = 0; j < 2; ++j)
+          acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0, 0, 0);
     }
     __syncthreads();
   }
 #pragma unroll
-  for (int i = 0; i < 4; ++i)
+  for (int i = 0; i < 2; ++i)
 #pragma unroll
-    for (int j = 0; j < 4; ++j) C[(size_t)(m0 + tr + i) * N + n0 + tc + j] = acc[i][j];
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+        C[(size_t)(m0 + wr + i * 16 + 4 * r + (lane >> 4)) * N + n0 + wc + j * 16 + (lane & 15)] =
+            acc[i][j][r];
 }
 
 // ------------------------------------------------------------------ epilogue
