@@ -16,10 +16,13 @@ PARAM_COLUMNS = {
 }
 SINE_GAUSSIAN, TAYLORF2 = 1, 2
 TIME_MARG, DISTANCE_MARG, PHASE_MARG, JITTER_TIME = 1, 2, 4, 8
+# TaylorF2 ends at the ISCO as well as at maximum_frequency (opt-in; see
+# waveform_generator.taylorf2_flags)
+TF2_END_ISCO = 16
 FP64, FP32 = 0, 1
 
 # enum gwl_transform (time-marginalised TaylorF2 transform choice)
-TRANSFORM_AUTO, TRANSFORM_TILED, TRANSFORM_WAVE, TRANSFORM_CHIRP = 0, 1, 2, 3
+TRANSFORM_AUTO, TRANSFORM_TILED, TRANSFORM_WAVE = 0, 1, 2
 
 # gwl_server_listen wire magics
 WIRE_REQUEST, WIRE_REPLY = 0x51574C47, 0x52574C47

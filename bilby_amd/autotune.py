@@ -59,7 +59,7 @@ def set_tile(likelihood, rows_per_workgroup, workgroups_per_cu):
 
 
 def set_transform(likelihood, transform):
-    """0 automatic, 1 tiled, 2 wave per row, 3 chirp-z (bilby_amd.abi)."""
+    """0 automatic, 1 tiled, 2 wave per row (bilby_amd.abi)."""
     check(LIB.gwl_event_set_transform(likelihood._event, int(transform)), "gwl_event_set_transform")
 
 

@@ -10,7 +10,7 @@ from concurrent.futures import ThreadPoolExecutor
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libgwl.so")
-SOURCES = ["gwl_api.hip", "gwl_kernels.hip", "gwl_roq.hip", "gwl_time_wave.hip", "gwl_time_chirp.hip", "gwl_server.hip",
+SOURCES = ["gwl_api.hip", "gwl_kernels.hip", "gwl_roq.hip", "gwl_time_wave.hip", "gwl_server.hip",
            "gwl_relbin.hip", "gwl_calib.hip",
            "gwl_roq_weights.hip", "gwl_window.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

@@ -514,17 +514,8 @@ int gwl_event_create(gwl_ctx* ctx, const gwl_event_desc* d, gwl_event** out, int
   e->ctx = ctx;
   const size_t b32 = n_el * (sizeof(float2) + sizeof(float));
   const size_t b64 = n_el * (sizeof(double2) + sizeof(double));
-  // chirp-z spectra for M = 1024, 2048, 4096 (the window must be shorter than M)
-  std::vector<float2> chirp;
-  if (wave_tables) {
-    chirp.resize(1024 + 2048 + 4096);
-    chirp_spectrum(N, ev.L, 1024, chirp.data());
-    chirp_spectrum(N, ev.L, 2048, chirp.data() + 1024);
-    chirp_spectrum(N, ev.L, 4096, chirp.data() + 3072);
-  }
-  const size_t bw = wave_tables ? 2 * ((size_t)N * 3 * sizeof(double) + n_el * sizeof(float2)) +
-                                      hps.size() * sizeof(double) + chirp.size() * sizeof(float2) +
-                                      4096
+  const size_t bw = wave_tables ? ((size_t)N * 3 * sizeof(double) + n_el * sizeof(float2)) +
+                                      hps.size() * sizeof(double) + 4096
                                 : 0;
   e->data_bytes = b32 + b64 + bw + 1024;
   if (hipMalloc(&e->data, e->data_bytes) != hipSuccess) {
@@ -549,14 +540,6 @@ int gwl_event_create(gwl_ctx* ctx, const gwl_event_desc* d, gwl_event** out, int
     p += n_el * sizeof(float2);
     ev.hps = reinterpret_cast<double*>(p);
     p += hps.size() * sizeof(double);
-    p = reinterpret_cast<char*>((reinterpret_cast<uintptr_t>(p) + 255) & ~(uintptr_t)255);
-    ev.tf2n = reinterpret_cast<double*>(p);
-    p += (size_t)N * 3 * sizeof(double);
-    ev.dwan = reinterpret_cast<float2*>(p);
-    p += n_el * sizeof(float2);
-    ev.chirpv[0] = reinterpret_cast<float2*>(p);
-    ev.chirpv[1] = ev.chirpv[0] + 1024;
-    ev.chirpv[2] = ev.chirpv[0] + 3072;
   }
   hipError_t err = hipSuccess;
   if (err == hipSuccess) err = hipMemcpy((void*)ev.dw64, dw64.data(), n_el * sizeof(double2), hipMemcpyHostToDevice);
@@ -565,9 +548,6 @@ int gwl_event_create(gwl_ctx* ctx, const gwl_event_desc* d, gwl_event** out, int
   if (err == hipSuccess) err = hipMemcpy((void*)ev.w32, w32.data(), n_el * sizeof(float), hipMemcpyHostToDevice);
   if (err == hipSuccess && wave_tables)
     err = hipMemcpy((void*)ev.hps, hps.data(), hps.size() * sizeof(double), hipMemcpyHostToDevice);
-  if (err == hipSuccess && wave_tables)
-    err = hipMemcpy((void*)ev.chirpv[0], chirp.data(), chirp.size() * sizeof(float2),
-                    hipMemcpyHostToDevice);
   if (err != hipSuccess) {
     (void)hipFree(e->data);
     delete e;
@@ -575,8 +555,6 @@ int gwl_event_create(gwl_ctx* ctx, const gwl_event_desc* d, gwl_event** out, int
   }
   if (wave_tables) {
     int rc = init_time_wave_tables(ev, const_cast<double*>(ev.tf2b), const_cast<float2*>(ev.dwa32));
-    if (rc == GWL_OK)
-      rc = init_chirp_tables(ev, const_cast<double*>(ev.tf2n), const_cast<float2*>(ev.dwan));
     if (rc != GWL_OK) {
       (void)hipFree(e->data);
       delete e;
@@ -600,7 +578,7 @@ int gwl_event_set_tile(gwl_event* e, int32_t rows, int32_t wg_per_cu) {
 }
 
 int gwl_event_set_transform(gwl_event* e, int32_t transform) {
-  if (!e || transform < GWL_TRANSFORM_AUTO || transform > GWL_TRANSFORM_CHIRP) {
+  if (!e || transform < GWL_TRANSFORM_AUTO || transform > GWL_TRANSFORM_WAVE) {
     set_error("gwl_event_set_transform: invalid argument");
     return GWL_E_INVALID;
   }

@@ -76,11 +76,6 @@ struct EvDev {
   const float2* dwa32;  // [nd][N] dw32 f^-7/6, decimated order
   const double* hps;    // [nd][N + 2] prefix sums of f^-7/3 sigma^2 / S over bins
                         //   0..N in natural order, hps[d][n] = sum of bins < n
-  // the chirp-z transform: natural-order basis and data, and the spectra of
-  // the chirp kernel for M = 1024, 2048, 4096 (digit-reversed order, / M)
-  const double* tf2n;   // [N][3]
-  const float2* dwan;   // [nd][N]
-  const float2* chirpv[3];
   // Nyquist bin (n = N), kept apart from the power-of-two FFT range
   double dwN[GWL_MAX_DET][2];
   double wN[GWL_MAX_DET];
@@ -256,9 +251,12 @@ __device__ inline int last_bin_le(double f, double df, int lo, int hi) {
 // TaylorF2 per-row constants: phase-basis coefficients c[kNB] in cycles
 // (Psi(f) / 2pi = sum_k c_k b_k(f), reference-frequency phase and t_c = 0
 // folded in), the amplitude prefactor amp0 (divided by ``sigma``) of
-// A(f) = amp0 f^-7/6, and the ISCO frequency.  Returns 0 for rows whose
-// waveform is undefined (non-positive masses or distance, |chi| >= 1,
-// non-finite values).
+// A(f) = amp0 f^-7/6, and the ISCO frequency.  amp0 is negative, as
+// lalsimulation's TaylorF2 writes it (amp0 = -4 m1 m2 / r * MRSUN * MTSUN *
+// sqrt(pi / 12) times the Newtonian sqrt(-dE/dv / F) v; bilby takes h+, hx
+// from there, source.py:633-643), so h+ = amp0 f^-7/6 (1 + cos^2 i)/2
+// exp(-i Psi).  Returns 0 for rows whose waveform is undefined
+// (non-positive masses or distance, |chi| >= 1, non-finite values).
 __device__ inline int tf2_row_constants(const double* row, double fref, double sigma, double* cout,
                                         double& amp0, double& fisco) {
   double m1 = row[GWL_P_MASS_1], m2 = row[GWL_P_MASS_2];
@@ -296,7 +294,7 @@ __device__ inline int tf2_row_constants(const double* row, double fref, double s
   for (int k = 0; k < kNB; ++k) cout[k] = c[k] / kTwoPi;
   cout[9] = 0.0;
   double mc = pow(eta, 0.6) * msec;
-  amp0 = sqrt(5.0 / 24.0) * pow(kPi, -2.0 / 3.0) * kSpeedOfLight / (dl * 1e6 * kParsec) *
+  amp0 = -sqrt(5.0 / 24.0) * pow(kPi, -2.0 / 3.0) * kSpeedOfLight / (dl * 1e6 * kParsec) *
          pow(mc, 5.0 / 6.0) / sigma;
   fisco = 1.0 / (pow(6.0, 1.5) * kPi * msec);
   return valid;
@@ -337,7 +335,9 @@ __device__ inline void compute_sample_const(const EvDev& ev, const double* row, 
       sc.cim[d] = -fc[d] * ax;
       sc.cabs2[d] = sc.cre[d] * sc.cre[d] + sc.cim[d] * sc.cim[d];
     }
-    double fmax = fmin(fisco, ev.wf_fmax);
+    // lalsimulation ends TaylorF2 at the fEnd bilby passes (maximum_frequency);
+    // at the ISCO as well only when the event asks for it
+    const double fmax = (ev.flags & GWL_TF2_END_ISCO) ? fmin(fisco, ev.wf_fmax) : ev.wf_fmax;
     int kb = ev.wf_fmin > 0.0 ? first_bin_ge(ev.wf_fmin, ev.df, lo, hi) : lo;
     int ke = last_bin_le(fmax, ev.df, lo, hi);
     sc.kb = max(kb, lo);

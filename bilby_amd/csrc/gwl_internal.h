@@ -132,15 +132,6 @@ size_t cal_time_ws_bytes(int64_t rows, int nc);
 int launch_cal_time(gwl_event* ev, const double* params, int64_t n, int ld, int nc,
                     const float2* cal, const double* calps, void* ws, double* per_curve,
                     double* out, hipStream_t stream);
-// chirp-z TaylorF2 time transform (gwl_time_chirp.hip)
-bool time_chirp_eligible(const gwl_event* ev, long long n);
-void chirp_spectrum(int N, int L, int M, float2* out);
-int init_chirp_tables(const EvDev& ev, double* tf2n, float2* dwan);
-size_t time_chirp_lds_bytes(const EvDev& ev);
-int launch_chirp_kernel(const EvDev& ev, const RowW* rows, long long n, double* out, int* fallback,
-                        hipStream_t stream);
-int launch_time_chirp(gwl_event* ev, const double* params, int64_t n, int ld, double* out,
-                      hipStream_t stream);
 size_t multi_wave_lds_bytes();
 int multi_launch_wave(gwl_multi* m, hipStream_t stream);
 }  // namespace gwl

@@ -745,7 +745,6 @@ static int eval_typed(gwl_event* e, const double* params, int64_t n, int ld, dou
                       double* aux, hipStream_t st) {
   const EvDev& ev = e->dev;
   if constexpr (std::is_same<R, float>::value && WAVE == GWL_TAYLORF2) {
-    if (time_chirp_eligible(e, n)) return launch_time_chirp(e, params, n, ld, out, st);
     if (time_wave_eligible(e)) return launch_time_wave(e, params, n, ld, out, st);
   }
   if (ev.flags & GWL_TIME_MARG) {

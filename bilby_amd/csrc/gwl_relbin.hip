@@ -199,7 +199,7 @@ __global__ __launch_bounds__(128) void k_relbin_rows(RelbinArgs a) {
     o->cre[d] = cre;
     o->cim[d] = cim;
   }
-  o->fhi = fmin(fisco, ev.wf_fmax);
+  o->fhi = (ev.flags & GWL_TF2_END_ISCO) ? fmin(fisco, ev.wf_fmax) : ev.wf_fmax;
 }
 
 // r_d at edge e for every detector (zero outside the waveform's band).

@@ -136,6 +136,10 @@ __global__ __launch_bounds__(128) void k_roq_prep(RoqDev rd, const double* param
   const bool valid = live && s_valid;
   const double amp0 = s_amp0, fisco = s_fisco;
   (void)amp0;
+  // lalsimulation evaluates a frequency-sequence TaylorF2 at every node
+  // (source.py:1125-1128); the waveform band and the ISCO cut the nodes only
+  // when the likelihood asks for ISCO termination
+  const bool cut = (rd.flags & GWL_TF2_END_ISCO) != 0;
   const size_t o = (size_t)r * rd.Kp;
   // padding columns [2L, Kp) of every row, and rows past n, are zero
   for (int k = 2 * rd.L + tid; k < rd.Kp; k += 128) {
@@ -151,7 +155,7 @@ __global__ __launch_bounds__(128) void k_roq_prep(RoqDev rd, const double* param
     double xr = 0.0, xi = 0.0;
     if (valid) {
       const double f = rd.fl[l];
-      if (f >= rd.wf_fmin && f <= fisco) {
+      if (!cut || (f >= rd.wf_fmin && f <= fisco)) {
         const double* b = rd.nbl + (size_t)l * (kNB + 1);
         const double cyc = tf2_phase_cycles(c, b);
         const double amp = b[kNB];  // f^-7/6; amp0 sits in C_d
@@ -193,7 +197,7 @@ __global__ __launch_bounds__(128) void k_roq_prep(RoqDev rd, const double* param
     double acc[GWL_MAX_DET] = {0.0, 0.0, 0.0, 0.0};
     for (int q = tid; q < rd.Qn; q += 128) {
       const double f = rd.fq[q];
-      if (f >= rd.wf_fmin && f <= fisco) {
+      if (!cut || (f >= rd.wf_fmin && f <= fisco)) {
         const double amp = rd.nbq[(size_t)q * (kNB + 1) + kNB];
         for (int d = 0; d < rd.nd; ++d) acc[d] += amp * amp * rd.wq[(size_t)d * rd.Qn + q];
       }

@@ -757,48 +757,6 @@ int launch_cal_time(gwl_event* e, const double* params, int64_t n, int ld, int n
   return GWL_OK;
 }
 
-// Chirp-z path: row constants, one workgroup per row, then the wave-per-row
-// transform (one group, fused epilogue) for the rows flagged as too wide.
-int launch_time_chirp(gwl_event* e, const double* params, int64_t n, int ld, double* out,
-                      hipStream_t st) {
-  const EvDev& ev = e->dev;
-  char* ws = reinterpret_cast<char*>(e->ws);
-  RowW* rows = reinterpret_cast<RowW*>(ws);
-  int* fallback = reinterpret_cast<int*>(ws + ((size_t)n * sizeof(RowW) + 255) / 256 * 256);
-  if ((char*)(fallback + n) > ws + e->ws_bytes) {
-    set_error("workspace too small for the time transform");
-    return GWL_E_INVALID;
-  }
-  RowArgs ra{ev, params, ld, n, rows};
-  hipLaunchKernelGGL(k_rowconst, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, ra);
-  GWL_HIP_CHECK(hipGetLastError());
-  if (time_chirp_lds_bytes(ev) > e->ctx->lds_per_block) {
-    set_error("chirp transform exceeds the LDS of a workgroup");
-    return GWL_E_UNSUPPORTED;
-  }
-  if (int rc = timing_begin(e, st)) return rc;
-  if (int rc = launch_chirp_kernel(ev, rows, n, out, fallback, st)) return rc;
-  if (int rc = timing_end(e, st)) return rc;
-  TimeWArgs ta{};
-  ta.ev = ev;
-  ta.rows = rows;
-  ta.n = n;
-  ta.G = 1;
-  ta.fused = 1;
-  ta.out = out;
-  ta.only = fallback;
-  const size_t lds = time_wave_lds_bytes(ev, true);
-  const dim3 grid((unsigned)((n + kWaves - 1) / kWaves));
-  switch (ev.nd) {
-    case 1: hipLaunchKernelGGL(k_time_w<1>, grid, dim3(64 * kWaves), lds, st, ta); break;
-    case 2: hipLaunchKernelGGL(k_time_w<2>, grid, dim3(64 * kWaves), lds, st, ta); break;
-    case 3: hipLaunchKernelGGL(k_time_w<3>, grid, dim3(64 * kWaves), lds, st, ta); break;
-    default: hipLaunchKernelGGL(k_time_w<4>, grid, dim3(64 * kWaves), lds, st, ta); break;
-  }
-  GWL_HIP_CHECK(hipGetLastError());
-  return GWL_OK;
-}
-
 size_t multi_wave_lds_bytes() { return (size_t)(kWP + kTw3 + kWaves * kXch) * sizeof(float2); }
 
 int multi_launch_wave(gwl_multi* m, hipStream_t st) {

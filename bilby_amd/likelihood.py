@@ -19,6 +19,7 @@ from . import abi as _abi
 from ._lib import LIB, check, dptr, iptr
 from .detector import InterferometerList
 from .prior import Uniform
+from .waveform_generator import taylorf2_flags
 
 _CTX = {}
 _INTERPOLANTS = {}  # distance interpolants by (grid and prior digest, phase flag)
@@ -301,7 +302,7 @@ class GravitationalWaveTransient:
             f |= _lib.DISTANCE_MARG
         if self.phase_marginalization:
             f |= _lib.PHASE_MARG
-        return f
+        return f | taylorf2_flags(self.waveform_generator.waveform_arguments)
 
     def _build_event(self):
         self._event, self.hbm_bytes = self._make_event()

@@ -20,6 +20,7 @@ from .detector import InterferometerList
 from .likelihood import (GravitationalWaveTransient, _batch_rows, distance_interpolant,
                          get_context, grid_time_posterior)
 from .prior import Uniform
+from .waveform_generator import taylorf2_flags
 
 MODES = {"fp64": 0, "fp32": 1, "bf16x3": 2}
 
@@ -240,7 +241,7 @@ class ROQGravitationalWaveTransient:
             flags |= _lib.PHASE_MARG
         if self.distance_marginalization:
             flags |= _lib.DISTANCE_MARG
-        d.flags = flags
+        d.flags = flags | taylorf2_flags(wfa)
         d.reference_distance = float(self._ref_dist)
         ctx = get_context(self.device)
         h = ctypes.c_void_p()

@@ -29,6 +29,21 @@ def taylorf2(frequency_array, mass_1, mass_2, luminosity_distance, theta_jn, pha
 taylorf2._gwl_approximant = _lib.TAYLORF2
 
 
+def taylorf2_flags(waveform_arguments):
+    """Event flag for the end of TaylorF2.  By default the waveform ends at
+    ``maximum_frequency``: bilby hands that frequency to lalsimulation as fEnd
+    (source.py:338-343, 633-643) and XLALSimInspiralTaylorF2 ends at the ISCO
+    only for fEnd = 0, which bilby never passes.  ``waveform_arguments
+    ['taylorf2_termination'] = 'isco'`` (no counterpart in the reference)
+    also ends it at the Schwarzschild ISCO, 1 / (6^3/2 pi M)."""
+    end = (waveform_arguments or {}).get("taylorf2_termination", "maximum_frequency")
+    if end == "maximum_frequency":
+        return 0
+    if end == "isco":
+        return _lib.TF2_END_ISCO
+    raise ValueError("taylorf2_termination must be 'maximum_frequency' or 'isco', not %r" % (end,))
+
+
 def lal_binary_black_hole(frequency_array, mass_1, mass_2, luminosity_distance, a_1, tilt_1,
                           phi_12, a_2, tilt_2, phi_jl, theta_jn, phase, **kwargs):
     """bilby/gw/source.py:269-348 entry point; only

@@ -53,9 +53,18 @@ enum gwl_approximant { GWL_SINE_GAUSSIAN = 1, GWL_TAYLORF2 = 2 };
 
 /* Marginalisation flags: base.py:150-229 constructor arguments
  * time_marginalization, distance_marginalization, phase_marginalization,
- * jitter_time. */
+ * jitter_time.
+ *
+ * GWL_TF2_END_ISCO has no counterpart in the reference: by default TaylorF2
+ * runs from wf_minimum_frequency up to wf_maximum_frequency, which is what
+ * lalsimulation's XLALSimInspiralTaylorF2 does for the non-zero fEnd bilby
+ * always passes (maximum_frequency, source.py:338-343, 633-643; ROQ nodes:
+ * every node, source.py:1125-1128).  With the flag the waveform also ends
+ * at the Schwarzschild ISCO 1 / (6^3/2 pi M) (lalsimulation's rule for
+ * fEnd = 0), for callers that want inspiral-only templates. */
 enum gwl_flags {
-  GWL_TIME_MARG = 1, GWL_DISTANCE_MARG = 2, GWL_PHASE_MARG = 4, GWL_JITTER_TIME = 8
+  GWL_TIME_MARG = 1, GWL_DISTANCE_MARG = 2, GWL_PHASE_MARG = 4, GWL_JITTER_TIME = 8,
+  GWL_TF2_END_ISCO = 16
 };
 
 /* Arithmetic of an evaluation: all-float64, or float32 bulk arithmetic with
@@ -157,10 +166,9 @@ int gwl_event_set_tile(gwl_event* ev, int32_t rows_per_workgroup, int32_t workgr
 
 /* Transform of the time-marginalised TaylorF2 path: automatic (the
  * wave-per-row or tiled kernel as gwl_event_set_tile selects) or one of
- * them forced; the chirp-z (Bluestein zoom) transform runs only when forced.
- * No counterpart in the reference (kernel choice). */
+ * them forced.  No counterpart in the reference (kernel choice). */
 enum gwl_transform {
-  GWL_TRANSFORM_AUTO = 0, GWL_TRANSFORM_TILED = 1, GWL_TRANSFORM_WAVE = 2, GWL_TRANSFORM_CHIRP = 3
+  GWL_TRANSFORM_AUTO = 0, GWL_TRANSFORM_TILED = 1, GWL_TRANSFORM_WAVE = 2
 };
 int gwl_event_set_transform(gwl_event* ev, int32_t transform);
 

@@ -297,19 +297,31 @@ def taylorf2_psi_pn(f, m1, m2, chi1, chi2):
 
 def taylorf2(frequency_array, mass_1, mass_2, luminosity_distance, theta_jn, phase,
              chi_1=0.0, chi_2=0.0, reference_frequency=50.0, minimum_frequency=20.0,
-             maximum_frequency=None, **kwargs):
+             maximum_frequency=None, taylorf2_termination="maximum_frequency", **kwargs):
     """Frequency-domain TaylorF2 (stationary-phase, leading-order amplitude,
-    3.5PN phase, aligned spins), terminated at the Schwarzschild ISCO
-    f_isco = 1/(6^(3/2) pi M).  Restates the bilby wrapper behaviour
-    bilby/gw/source.py:552-690 (masses in solar masses, distance in Mpc,
-    aligned spins -> iota = theta_jn per conversion.py:146-153,
-    zero outside [minimum_frequency, maximum_frequency], source.py:618-619,
-    678-679; default maximum_frequency = frequency_array[-1], source.py:338-343).
+    3.5PN phase, aligned spins) as lalsimulation's XLALSimInspiralTaylorF2
+    computes it under the bilby wrapper bilby/gw/source.py:552-690 (masses
+    in solar masses, distance in Mpc, aligned spins -> iota = theta_jn per
+    conversion.py:146-153).
+
+    Band: bins from minimum_frequency up to maximum_frequency (default
+    frequency_array[-1], source.py:338-343).  bilby hands maximum_frequency to
+    lalsimulation as fEnd (source.py:633-643), which fills every bin up to
+    fEnd and ends at the Schwarzschild ISCO only for fEnd = 0; bilby then
+    zeroes the bins outside [minimum_frequency, maximum_frequency]
+    (source.py:618-619, 678-679).  taylorf2_termination='isco' (an opt-in
+    of this build, no counterpart in the reference) also cuts at
+    f_isco = 1/(6^(3/2) pi M).  maximum_frequency = np.inf with no
+    minimum_frequency cut is the frequency-sequence form bilby's ROQ model
+    uses (source.py:1125-1128: every node evaluated).
 
         h+ = A(f) (1 + cos^2 iota)/2 exp(-i Psi(f))
         hx = -i A(f) cos(iota) exp(-i Psi(f))
-        A(f) = sqrt(5/24) pi^(-2/3) (c/D) (G Mc/c^3)^(5/6) f^(-7/6)
+        A(f) = -sqrt(5/24) pi^(-2/3) (c/D) (G Mc/c^3)^(5/6) f^(-7/6)
         Psi(f) = Psi_PN(f) - Psi_PN(f_ref) - 2 phase - pi/4   (t_c = 0)
+
+    The negative sign of A is lalsimulation's (amp0 = -4 m1 m2 / r MRSUN
+    MTSUN sqrt(pi/12) times the Newtonian sqrt(-E'(v)/F(v)) v).
     """
     f = np.asarray(frequency_array, float)
     if maximum_frequency is None:
@@ -318,12 +330,15 @@ def taylorf2(frequency_array, mass_1, mass_2, luminosity_distance, theta_jn, pha
     eta = mass_1 * mass_2 / m_tot ** 2
     mchirp_sec = eta ** 0.6 * m_tot * MTSUN_SI
     dist_m = luminosity_distance * 1e6 * PARSEC
-    f_isco = 1.0 / (6.0 ** 1.5 * np.pi * m_tot * MTSUN_SI)
-    band = (f >= minimum_frequency) & (f <= maximum_frequency) & (f <= f_isco)
+    band = (f >= minimum_frequency) & (f <= maximum_frequency)
+    if taylorf2_termination == "isco":
+        band &= f <= 1.0 / (6.0 ** 1.5 * np.pi * m_tot * MTSUN_SI)
+    elif taylorf2_termination != "maximum_frequency":
+        raise ValueError("taylorf2_termination %r" % (taylorf2_termination,))
     hp = np.zeros(f.shape, complex)
     hc = np.zeros(f.shape, complex)
     fb = f[band]
-    amp = (np.sqrt(5.0 / 24.0) * np.pi ** (-2.0 / 3.0) * SPEED_OF_LIGHT / dist_m
+    amp = (-np.sqrt(5.0 / 24.0) * np.pi ** (-2.0 / 3.0) * SPEED_OF_LIGHT / dist_m
            * mchirp_sec ** (5.0 / 6.0) * fb ** (-7.0 / 6.0))
     psi = (taylorf2_psi_pn(fb, mass_1, mass_2, chi_1, chi_2)
            - taylorf2_psi_pn(np.array([reference_frequency]), mass_1, mass_2, chi_1, chi_2)[0]

@@ -44,10 +44,16 @@ class OracleROQ:
             priors["phase"] = 0.0
 
     def _node_waveform(self, p):
-        """source.py:802-898: the model at the linear and quadratic nodes."""
-        kw = dict(minimum_frequency=self.waveform_arguments.get("minimum_frequency", 20.0),
+        """source.py:802-898: the model at the linear and quadratic nodes
+        (lalsimulation's frequency-sequence TaylorF2 evaluates every node,
+        source.py:1125-1128; with taylorf2_termination='isco' the nodes
+        outside [minimum_frequency, f_isco] are zero)."""
+        isco = self.waveform_arguments.get("taylorf2_termination") == "isco"
+        kw = dict(minimum_frequency=(self.waveform_arguments.get("minimum_frequency", 20.0)
+                                     if isco else 0.0),
                   reference_frequency=self.waveform_arguments.get("reference_frequency", 20.0),
-                  maximum_frequency=np.inf)
+                  maximum_frequency=np.inf,
+                  taylorf2_termination="isco" if isco else "maximum_frequency")
         nodes = np.concatenate([self.fl, self.fq])
         h = core.taylorf2(nodes, p["mass_1"], p["mass_2"], p["luminosity_distance"], p["theta_jn"],
                           p["phase"], chi_1=p.get("chi_1", 0.0), chi_2=p.get("chi_2", 0.0), **kw)

@@ -133,8 +133,8 @@ def _bns_event():
 
 def test_fiducial_update_raises_the_full_likelihood():
     """update_fiducial_parameters (relative.py:158-171, 260-289) from a point
-    2e-4 off in both masses of a light binary (ISCO above the band, so the
-    binned surface is valid across the bounds): the full likelihood at the
+    2e-4 off in both masses of a light binary (the binned surface is valid
+    across the bounds): the full likelihood at the
     updated fiducial point is higher than at the starting one."""
     from bilby_amd.likelihood import GravitationalWaveTransient
     from bilby_amd.prior import PriorDict, Uniform

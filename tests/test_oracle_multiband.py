@@ -1,13 +1,14 @@
 """The multibanded likelihood's host setup (bilby_amd.multiband.MultibandSetup)
 and its oracle evaluation (oracle/multiband.py) pinned on the CPU:
 * multibanded ln L vs the full oracle ln L on the reference test's
-  configuration (16 s, 2048 Hz, H1+L1+V1, geocent prior +-0.1 s,
+  configuration (16 s, 2048 Hz, H1+L1+V1, geocent prior +-0.1 s, chirp mass
+  6, mass ratio 0.5, network SNR ~50, noise seed 70817;
   likelihood_test.py:1313-1428): |dlnL| < 1.5e-2 (its tolerance for the
-  linear-interpolation algorithm), for a light binary whose TaylorF2
-  termination lies above the band (IMRPhenomD in the reference ends
-  smoothly);
+  linear-interpolation algorithm); TaylorF2 runs on to the band's end as
+  lalsimulation's does for bilby's fEnd (the reference uses IMRPhenomD);
+  with the analytic PSD here, 280 Mpc gives the reference's SNR ~50;
 * accuracy_factor 50 at least halves the error of accuracy_factor 5
-  (likelihood_test.py:1430-1475);
+  (likelihood_test.py:1430-1475), same configuration;
 * the prefix-sum quadratic coefficients equal the reference's loop form;
 * weights saved and loaded give the same setup."""
 import types
@@ -23,7 +24,12 @@ WF = dict(reference_frequency=20.0, minimum_frequency=20.0)
 T, FS = 16.0, 2048.0
 
 
-def _event(seed=70817, masses=(1.5, 1.3), distance=40.0):
+# likelihood_test.py:1317-1333: chirp_mass 6, mass_ratio 0.5
+_MC, _Q = 6.0, 0.5
+REF_MASSES = (_MC * (1 + _Q) ** 0.2 / _Q ** 0.6, _Q * _MC * (1 + _Q) ** 0.2 / _Q ** 0.6)
+
+
+def _event(seed=70817, masses=REF_MASSES, distance=280.0):
     rng = np.random.default_rng(seed)
     f = core.frequency_series(FS, T)
     tc = 1187008882.0
@@ -79,7 +85,7 @@ def test_multiband_matches_full_likelihood():
 
 
 def test_larger_accuracy_factor_is_more_accurate():
-    ev, ifos, inj = _event(seed=5)
+    ev, ifos, inj = _event()
     full = OracleGWTransient(ev, "taylorf2", waveform_arguments=WF).log_likelihood_ratio(inj)
     errs = []
     for acc in (5, 50):
