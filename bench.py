@@ -91,6 +91,41 @@ def profile_figure(tag, key):
     return None
 
 
+# VALU issue peak of the MI355X: 256 CUs x 4 SIMDs, a wave64 VALU instruction
+# every 4 cycles on a 16-lane SIMD, at the 2.4 GHz peak engine clock
+VALU_PEAK_GIPS = 256 * 4 / 4 * 2.4
+
+
+def roofline_block(kname, wave, k_ms, achieved_gbps, bpe, traffic, key):
+    """The roofline of the headline kernel.  Its event data (0.6 MB) stays in
+    L2/MALL and every row re-reads it, so the algorithmic bytes over the
+    kernel time exceed what HBM alone delivers and HBM traffic is a small
+    fraction of them: the bound is VALU issue.  achieved = the kernel's VALU
+    wave-instructions per launch (rocprofv3 SQ_INSTS_VALU, a property of the
+    kernel and the batch, in profiles/) over its launch time measured here
+    with HIP events; the HBM figures are kept under "hbm"."""
+    desc = (" (wave per row: waveform + projection + register radix-8 transform + window + "
+            "fused epilogue)" if wave else " (tiled: waveform + projection + LDS FFT)")
+    hbm = {"achieved": achieved_gbps, "peak": 8000.0, "unit": "GB/s",
+           "frac": achieved_gbps / 8000.0, "bytes_per_eval": bpe,
+           "note": "algorithmic bytes / kernel time; L2/MALL-resident data re-read per row"}
+    insts = profile_figure("bench-valu-insts", key)
+    out = {"kernel": kname + desc, "kernel_ms": k_ms, "traffic": traffic,
+           "traffic_source": "rocprofv3 FETCH_SIZE x2.0 + WRITE_SIZE x1.0 per launch, "
+                             "profiles/ (bench-traffic %s)" % key, "hbm": hbm}
+    if insts is None:
+        out.update(bound="hbm", achieved=achieved_gbps, peak=8000.0, unit="GB/s",
+                   frac=achieved_gbps / 8000.0)
+        return out
+    gips = insts / (k_ms * 1e-3) / 1e9
+    out.update(bound="valu", achieved=gips, peak=VALU_PEAK_GIPS,
+               unit="G VALU wave-instructions/s", frac=gips / VALU_PEAK_GIPS,
+               valu_insts_per_launch=insts,
+               valu_source="rocprofv3 SQ_INSTS_VALU per dispatch, profiles/ "
+                           "(bench-valu-insts %s); launch time from HIP events in this run" % key)
+    return out
+
+
 def measured_traffic(key):
     """HBM bytes per launch of the kernel ``key`` from the FETCH_SIZE /
     WRITE_SIZE counter runs ('bench-traffic' lines)."""
@@ -148,8 +183,10 @@ def timed(fn, steps, warmup=2):
     return (time.perf_counter() - t0) / steps
 
 
-def bench_sine_gaussian(steps):
-    """Plain likelihood, sine-Gaussian, H1+L1, 4 s @ 2048 Hz, 10^4 draws."""
+def bench_sine_gaussian(steps, n_oracle=256):
+    """Plain likelihood, sine-Gaussian, H1+L1, 4 s @ 2048 Hz, 10^4 draws:
+    float32 with per-row float64 escalation (the default), float32 without
+    it (not a metric figure: it misses 1e-3 on loud rows), float64."""
     import numpy as np
     import torch
     from bilby_amd.configs import make_sine_gaussian_plain
@@ -160,16 +197,28 @@ def bench_sine_gaussian(steps):
     rows = torch.from_numpy(lk.rows_from_parameters(setup["draws"])).cuda()
     out = {}
     res = {}
-    for prec in ("fp32", "fp64"):
+    for name, prec, esc in (("fp32_escalated", "fp32", True), ("fp32_raw", "fp32", False),
+                            ("fp64", "fp64", True)):
+        if esc:
+            lk.set_precision_escalation()
+        else:
+            lk.set_precision_escalation(0.0)
         dt = timed(lambda: lk.log_likelihood_ratio_batch(rows, precision=prec), steps)
-        out[prec + "_evals_per_s"] = rows.shape[0] / dt
-        res[prec] = lk.log_likelihood_ratio_batch(rows, precision=prec).cpu().numpy()
+        out[name + "_evals_per_s"] = rows.shape[0] / dt
+        res[name] = lk.log_likelihood_ratio_batch(rows, precision=prec).cpu().numpy()
+        if name == "fp32_escalated":
+            out["escalated_fraction"] = lk.escalated_rows() / rows.shape[0]
+    lk.set_precision_escalation()
     olk, _ = oracle_from_setup(setup)
-    ref = np.array([olk.log_likelihood_ratio(p) for p in draws_as_dicts(setup["draws"], range(64))])
-    for prec in ("fp32", "fp64"):
-        d = np.abs(res[prec][:64] - ref)
-        out[prec + "_max_abs_dlnL_vs_oracle_64rows"] = float(d.max())
-        out[prec + "_rows_within_1e-3_vs_fp64_gpu"] = float(np.mean(np.abs(res[prec] - res["fp64"]) < 1e-3))
+    ref = np.array([olk.log_likelihood_ratio(p)
+                    for p in draws_as_dicts(setup["draws"], range(n_oracle))])
+    for name in res:
+        d = np.abs(res[name][:n_oracle] - ref)
+        out[name + "_max_abs_dlnL_vs_oracle_%drows" % n_oracle] = float(d.max())
+        out[name + "_rows_within_1e-3_vs_fp64_gpu"] = float(
+            np.mean(np.abs(res[name] - res["fp64"]) < 1e-3))
+    out["fp32_escalated_max_abs_dlnL_vs_fp64_gpu"] = float(
+        np.max(np.abs(res["fp32_escalated"] - res["fp64"])))
     out["max_abs_lnL"] = float(np.max(np.abs(res["fp64"])))
     return out
 
@@ -178,7 +227,7 @@ def bench_uneven(steps):
     """64 events, durations log-uniform over {4..128} s, 2048/4096 Hz, 2-3
     detectors, aligned-spin TaylorF2, T+D+P marginalised, 256-4096 rows each,
     one persistent launch replayed from a hipGraph; tile per bin-count bucket
-    chosen by timing."""
+    chosen by bilby_amd.autotune.autotune_multi."""
     import numpy as np
     import torch
     from bilby_amd.configs import make_uneven_events
@@ -198,33 +247,32 @@ def bench_uneven(steps):
     for e, d in enumerate(draws):
         multi.set_parameters(e, d)
     total_rows = sum(ev["n_rows"] for ev in events)
-    buckets = sorted({len(lk.interferometers.frequency_array) - 1 for lk in lks})
-    steps_of = {b: 32 for b in buckets}
-    sweep = {}
-    for b in buckets:
-        best = None
-        for cand in (8, 16, 32, 64):
-            steps_of[b] = cand
-            multi.set_tile([steps_of[len(lk.interferometers.frequency_array) - 1] for lk in lks])
-            dt = timed(lambda: multi.run(), 2, warmup=1)
-            sweep["%d:%d" % (b, cand)] = dt * 1e3
-            if best is None or dt < best[0]:
-                best = (dt, cand)
-        steps_of[b] = best[1]
-    multi.set_tile([steps_of[len(lk.interferometers.frequency_array) - 1] for lk in lks])
+    from bilby_amd.autotune import autotune_multi
+    steps_of, sweep = autotune_multi(multi, use_cache=False)
     multi.capture()
     dt = timed(lambda: multi.run(), steps)
     outs = [o.cpu().numpy() for o in multi.run()]
-    # agreement with the per-event float64 path
+    # agreement with the per-event float64 path, and with the FP64 oracle on
+    # 4 rows of every event (one distance table: every event has the same
+    # distance prior and phase flag)
+    from tests.helpers import draws_as_dicts, oracle_from_setup
     dmax = 0.0
     for lk, d, o in zip(lks[:16], draws[:16], outs[:16]):
         ref = lk.log_likelihood_ratio_batch(d, precision="fp64")
         dmax = max(dmax, float(np.max(np.abs(o - ref))))
+    omax, table = 0.0, None
+    for ev, d, o in zip(events, draws, outs):
+        olk, _ = oracle_from_setup(ev, distance_table=table)
+        table = olk._dist_margd_loglikelihood_array
+        idx = [0, 1, ev["n_rows"] // 2, ev["n_rows"] - 1]
+        ref = np.array([olk.log_likelihood_ratio(p) for p in draws_as_dicts(d, idx)])
+        omax = max(omax, float(np.max(np.abs(o[idx] - ref))))
     n_bins = [len(lk.interferometers.frequency_array) for lk in lks]
     return dict(events=len(events), rows=int(total_rows), ms_per_launch=dt * 1e3,
                 evals_per_s=total_rows / dt, bins_min=int(min(n_bins)), bins_max=int(max(n_bins)),
                 tile_steps_by_bins={str(k): int(v) for k, v in steps_of.items()},
-                tile_sweep_ms=sweep, fp32_vs_fp64_gpu_max_16_events=dmax, graph=True)
+                tile_sweep_ms=sweep, fp32_vs_fp64_gpu_max_16_events=dmax,
+                fp32_vs_fp64_oracle_max_4_rows_x_64_events=omax, graph=True)
 
 
 def bench_relbin(steps):
@@ -757,7 +805,7 @@ def bench_sampler(lk, draws, seconds, workers=16, queue=1024):
     return out
 
 
-def bench_serving(main_lk, main_rows, duration):
+def bench_serving(main_lk, main_rows, duration, main_setup, main_draws):
     """32 samplers, seeded Poisson arrivals at 2000 requests/s, 1-1024 rows per
     request from a 60/30/10 mix of the TaylorF2 T+D+P event, uneven events and
     the ROQ event; everything resident; HBM budget 200 GB."""
@@ -771,7 +819,7 @@ def bench_serving(main_lk, main_rows, duration):
     try:
         server.register("c2", main_lk)
         events = make_uneven_events(n_events=64, seed=97531, rows_range=(256, 4096))
-        names, lks, pools = [], [], []
+        names, lks, pools, ev_draws = [], [], [], []
         for k, ev in enumerate(events):
             pri = ev["priors"].copy()
             lk = GravitationalWaveTransient(ev["interferometers"], ev["waveform_generator"],
@@ -780,7 +828,8 @@ def bench_serving(main_lk, main_rows, duration):
                                             phase_marginalization=True, priors=pri)
             names.append("c3_%d" % k)
             lks.append(lk)
-            pools.append(lk.rows_from_parameters(fixed_marginalised(ev["draws"], pri)))
+            ev_draws.append(fixed_marginalised(ev["draws"], pri))
+            pools.append(lk.rows_from_parameters(ev_draws[-1]))
         server.register_multi(names, lks)
         s = make_roq(n_draws=8192)
         priors = s["priors"].copy()
@@ -806,22 +855,47 @@ def bench_serving(main_lk, main_rows, duration):
         summ["evals_per_s"] = summ["rows"] / wall
         summ["requests_per_s"] = summ["requests"] / wall
         summ["resident_gb"] = server.resident_bytes / 1e9
-        # audit ~1% of requests (capped) against the float64 path on the device
+        # audit 1% of the requests (at least 1) against the FP64 oracle: the
+        # first and last row of each audited request
+        from oracle.roq import OracleROQ
+        from tests.helpers import draws_as_dicts, oracle_from_setup
         rng = np.random.default_rng(7)
-        pick = rng.choice(len(issued), size=max(1, min(40, len(issued) // 100)), replace=False)
-        lk_of = dict(zip(names, lks))
-        lk_of["c2"] = main_lk
-        lk_of["c4"] = roq
-        pool_of = dict(zip(names, pools))
-        pool_of["c2"] = main_rows
-        pool_of["c4"] = roq_rows
+        pick = rng.choice(len(issued), size=max(1, -(-len(issued) // 100)), replace=False)
+        w = s["weights"]
+        oracles = {"c4": OracleROQ(["H1", "L1"], s["interferometers"].start_time, w,
+                                   w["frequency_nodes_linear"], w["frequency_nodes_quadratic"],
+                                   dict(s["priors"]), time_marginalization=True,
+                                   phase_marginalization=True, delta_tc=s["delta_tc"],
+                                   waveform_arguments=s["waveform_generator"].waveform_arguments)}
+        draws_of = {"c4": fixed_marginalised(s["draws"], priors), "c2": main_draws}
+        draws_of.update(zip(names, ev_draws))
+        setup_of = dict(zip(names, events))
+        setup_of["c2"] = main_setup
+        # one distance table for the uneven events (same distance prior and
+        # phase flag); the headline event builds its own
+        c3_table = None
         dmax = 0.0
+        n_rows_audited = 0
         for i in pick:
             name, idx, fut = issued[i]
-            ref = lk_of[name].log_likelihood_ratio_batch(pool_of[name][idx], precision="fp64")
-            dmax = max(dmax, float(np.max(np.abs(fut.result() - ref))))
+            if name not in oracles:
+                c3 = name.startswith("c3_")
+                olk, _ = oracle_from_setup(setup_of[name], distance_table=c3_table if c3 else None)
+                if c3:
+                    c3_table = olk._dist_margd_loglikelihood_array
+                oracles[name] = olk
+            # the request's first and last row, in request order
+            pos = [0] if len(idx) == 1 else [0, len(idx) - 1]
+            sel = [int(idx[k]) for k in pos]
+            ref = np.array([oracles[name].log_likelihood_ratio(p)
+                            for p in draws_as_dicts(draws_of[name], sel)])
+            got = fut.result()
+            dmax = max(dmax, float(np.max(np.abs(got[pos] - ref))))
+            n_rows_audited += len(sel)
         summ["audit_requests"] = int(len(pick))
-        summ["audit_max_abs_dlnL_vs_fp64"] = dmax
+        summ["audit_fraction"] = len(pick) / max(1, len(issued))
+        summ["audit_rows"] = n_rows_audited
+        summ["audit_max_abs_dlnL_vs_fp64_oracle"] = dmax
         return summ
     finally:
         server.close()
@@ -854,7 +928,7 @@ def main():
                          "0,0 = the default kernel choice")
     ap.add_argument("--transform", type=int, default=0,
                     help="time transform of the headline event (include/gwl.h gwl_transform): "
-                         "0 automatic, 1 tiled, 2 wave per row, 3 chirp-z")
+                         "0 automatic, 1 tiled, 2 wave per row")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only; --gpus %d is not supported (parameter rows "
@@ -936,12 +1010,10 @@ def main():
     k_ms = tot.value / max(1, cnt.value)
     bpe = bytes_per_eval(lk)
     achieved = args.batch * bpe / (k_ms * 1e-3) / 1e9
-    # the transform that ran: chirp-z only when asked for, else the tile choice (include/gwl.h)
-    chirp = args.transform == 3
-    wave = not chirp and (args.transform == 2 or (args.transform == 0 and tile[0] == 0))
-    kname = "k_time_c" if chirp else "k_time_w" if wave else "k_time"
-    traffic_key = ("k_time_c" if chirp else "k_time_w" if wave else "k_time_tiled") + \
-        "_taylorf2_tdp_batch%d" % args.batch
+    # the transform that ran: the tile choice (include/gwl.h)
+    wave = args.transform == 2 or (args.transform == 0 and tile[0] == 0)
+    kname = "k_time_w" if wave else "k_time"
+    traffic_key = ("k_time_w" if wave else "k_time_tiled") + "_taylorf2_tdp_batch%d" % args.batch
     traffic = measured_traffic(traffic_key)
     log("%s: %.3f ms/launch, %.1f GB/s algorithmic, measured traffic %s B/launch"
         % (kname, k_ms, achieved, traffic))
@@ -985,22 +1057,7 @@ def main():
         "config": {"workload": "TaylorF2 non-spinning, H1+L1+V1, 8 s @ 4096 Hz (16385 bins), "
                                "time+distance+phase marginalised, batch of prior draws",
                    "batch": args.batch, "n_freq": 16385, "detectors": 3},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": 8000.0, "unit": "GB/s",
-                     "frac": achieved / 8000.0, "traffic": traffic,
-                     "kernel": (kname + (" (chirp-z: waveform + projection + Bluestein "
-                                         "convolution in LDS + fused epilogue, one workgroup per "
-                                         "row)" if chirp else
-                                         " (wave per row: waveform + projection + register "
-                                         "radix-8 transform + window + fused epilogue)" if wave
-                                         else " (tiled: waveform + projection + LDS FFT)")),
-                     "kernel_ms": k_ms, "bytes_per_eval": bpe,
-                     "traffic_source": "rocprofv3 FETCH_SIZE x2.0 + WRITE_SIZE x1.0 per launch, "
-                                       "profiles/ (bench-traffic %s)" % traffic_key,
-                     "binding": "valu" if achieved > 8000.0 else "hbm",
-                     "valu_frac": profile_figure("bench-valu", traffic_key),
-                     "valu_source": "rocprofv3 SQ_INSTS_VALU / (256 wave-instr per cycle x "
-                                    "GRBM_GUI_ACTIVE per XCD), profiles/ (bench-valu %s)"
-                                    % traffic_key},
+        "roofline": roofline_block(kname, wave, k_ms, achieved, bpe, traffic, traffic_key),
         "autotune": {"chosen": list(tile), "ms_by_tile": sweep},
         "accuracy": {"fp32_vs_fp64_gpu_max": float(d_all.max()),
                      "fp32_vs_fp64_gpu_p99": float(np.percentile(d_all, 99))},
@@ -1020,7 +1077,7 @@ def main():
                          ("binned_time_marginalised_tdp", lambda: bench_binned_time(5)),
                          ("roq_weights_400_basis_128s", lambda: bench_roq_weights()),
                          ("serving_32_samplers_2000_rps", lambda: bench_serving(
-                             lk, rows.cpu().numpy(), args.serve_seconds)),
+                             lk, rows.cpu().numpy(), args.serve_seconds, setup, draws)),
                          ("sampler_callers_pool_and_16_processes", lambda: bench_sampler(
                              lk, draws, args.sampler_seconds))):
             if args.only and not any(k in name for k in args.only.split(",")):

@@ -121,6 +121,13 @@ def _load():
         "gwl_event_timing_read": ([vp, _dp, ctypes.POINTER(ctypes.c_int64)], ctypes.c_int),
         "gwl_roq_reserve": ([vp, ctypes.c_int64], ctypes.c_int),
         "gwl_event_set_transform": ([vp, ctypes.c_int32], ctypes.c_int),
+        "gwl_event_set_escalation": ([vp, ctypes.c_double, ctypes.c_double], ctypes.c_int),
+        "gwl_event_set_multi_steps": ([vp, ctypes.c_int32], ctypes.c_int),
+        "gwl_event_set_time_prior": ([vp, ctypes.c_int64, ctypes.c_double, ctypes.c_double, _dp],
+                                     ctypes.c_int),
+        "gwl_roq_set_time_prior": ([vp, ctypes.c_int64, ctypes.c_double, ctypes.c_double, _dp],
+                                   ctypes.c_int),
+        "gwl_event_escalated_rows": ([vp, ctypes.POINTER(ctypes.c_int64)], ctypes.c_int),
         "gwl_server_create": ([vp, ctypes.c_int64, ctypes.POINTER(vp)], ctypes.c_int),
         "gwl_server_destroy": ([vp], ctypes.c_int),
         "gwl_server_add_event": ([vp, vp, ctypes.c_int32, ctypes.POINTER(ctypes.c_int32)],
@@ -226,5 +233,7 @@ EXPORTED_SYMBOLS = [
     "gwl_mb_set_time", "gwl_relbin_set_time", "gwl_eval_time_series", "gwl_event_time_window",
     "gwl_roq_weights_linear", "gwl_relbin_time_series", "gwl_mb_time_series",
     "gwl_roq_inner_products", "gwl_relbin_set_upsampled", "gwl_relbin_upsampled_series",
-    "gwl_weighted_inner_products", "gwl_window_time_series",
+    "gwl_weighted_inner_products", "gwl_window_time_series", "gwl_event_set_escalation",
+    "gwl_event_escalated_rows", "gwl_event_set_time_prior",
+    "gwl_roq_set_time_prior", "gwl_event_set_multi_steps",
 ]

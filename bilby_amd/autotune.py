@@ -63,6 +63,72 @@ def set_transform(likelihood, transform):
     check(LIB.gwl_event_set_transform(likelihood._event, int(transform)), "gwl_event_set_transform")
 
 
+MULTI_CANDIDATES = (8, 16, 32, 64)
+
+
+def multi_bucket_key(likelihoods, rows):
+    """Cache key of a multi-event set: the device and every event's
+    (source, flags, detectors, log2 bins, log2 rows), sorted."""
+    dev = torch.cuda.get_device_name(likelihoods[0].device)
+    shapes = sorted("%d:%d:%d:%d:%d" % (lk.waveform_generator.approximant, lk._flags(),
+                                          len(lk.interferometers),
+                                          int(math.log2(max(2, len(lk.interferometers.frequency_array) - 1))),
+                                          int(math.log2(max(1, n))))
+                    for lk, n in zip(likelihoods, rows))
+    return dev + "|multi|" + ",".join(shapes)
+
+
+def set_multi_steps(likelihood, steps):
+    check(LIB.gwl_event_set_multi_steps(likelihood._event, int(steps)),
+          "gwl_event_set_multi_steps")
+
+
+def autotune_multi(multi, candidates=MULTI_CANDIDATES, repeats=2, use_cache=True):
+    """Tile of a multi-event launch (bilby_amd.multi.MultiEventLikelihood
+    whose rows are set): the decimated sub-sequence steps per work item,
+    chosen per bin-count bucket by coordinate descent over the buckets
+    (each bucket's candidates timed with the others held), cached per event
+    set, and stored on every event (gwl_event_set_multi_steps) so that later
+    multi-event sets and server groups of these events start from it.
+    Returns ({bins: steps}, {"bins:steps": ms})."""
+    lks = multi.likelihoods
+    key = multi_bucket_key(lks, multi.rows)
+    bins = [len(lk.interferometers.frequency_array) - 1 for lk in lks]
+    cache = _load_cache() if use_cache else {}
+    sweep = {}
+    if use_cache and key in cache:
+        steps_of = {int(b): int(s) for b, s in cache[key]["steps"].items()}
+        sweep = cache[key].get("timings", {})
+    else:
+        steps_of = {b: 32 for b in sorted(set(bins))}
+        for b in sorted(steps_of):
+            best = None
+            for cand in candidates:
+                steps_of[b] = cand
+                multi.set_tile([steps_of[x] for x in bins])
+                multi.run()
+                torch.cuda.synchronize()
+                e0 = torch.cuda.Event(enable_timing=True)
+                e1 = torch.cuda.Event(enable_timing=True)
+                e0.record(multi.stream)
+                for _ in range(repeats):
+                    multi.run(synchronize=False)
+                e1.record(multi.stream)
+                torch.cuda.synchronize()
+                ms = e0.elapsed_time(e1) / repeats
+                sweep["%d:%d" % (b, cand)] = ms
+                if best is None or ms < best[0]:
+                    best = (ms, cand)
+            steps_of[b] = best[1]
+        if use_cache:
+            cache[key] = {"steps": {str(b): s for b, s in steps_of.items()}, "timings": sweep}
+            _save_cache(cache)
+    multi.set_tile([steps_of[x] for x in bins])
+    for lk, b in zip(lks, bins):
+        set_multi_steps(lk, steps_of[b])
+    return steps_of, sweep
+
+
 def time_tile(likelihood, rows_t, tile, repeats=5, precision=None):
     """Average milliseconds of one batched evaluation with ``tile``."""
     set_tile(likelihood, *tile)

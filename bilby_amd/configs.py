@@ -57,17 +57,19 @@ def make_sine_gaussian_plain(n_draws=10000, seed=1234, duration=4.0, sampling_fr
 
 def make_taylorf2_marginalised(n_draws=4096, seed=4321, duration=8.0, sampling_frequency=4096.0,
                                detectors=("H1", "L1", "V1"), chi_range=0.0, inject=True,
-                               distance=True, phase=True, time=True, device=0):
+                               distance=True, phase=True, time=True, device=0,
+                               taylorf2_termination=None):
     rng = np.random.default_rng(seed)
     start = GPS_REFERENCE - duration / 2
     centre = start + duration / 2
     ifos = InterferometerList(list(detectors))
     ifos.set_strain_data_from_power_spectral_densities(sampling_frequency, duration, start,
                                                        random_state=rng)
+    wfa = dict(reference_frequency=50.0, minimum_frequency=20.0)
+    if taylorf2_termination is not None:
+        wfa["taylorf2_termination"] = taylorf2_termination
     wfg = WaveformGenerator(duration=duration, sampling_frequency=sampling_frequency,
-                            frequency_domain_source_model=taylorf2,
-                            waveform_arguments=dict(reference_frequency=50.0,
-                                                    minimum_frequency=20.0))
+                            frequency_domain_source_model=taylorf2, waveform_arguments=wfa)
     injection = dict(mass_1=36.0, mass_2=29.0, chi_1=0.0, chi_2=0.0, luminosity_distance=800.0,
                      theta_jn=0.4, phase=1.3, ra=1.375, dec=-1.2108, psi=2.659,
                      geocent_time=centre + 0.013)

@@ -343,6 +343,26 @@ __global__ __launch_bounds__(256) void k_window_dft(EvDev ev, const double* resp
 
 using namespace gwl;
 
+namespace gwl {
+int upload_time_prior(int64_t n, double t0, double dt, const double* density, double** dst) {
+  if (n < 2 || !density || !(dt > 0) || !std::isfinite(t0)) {
+    set_error("time prior table: need n >= 2 points, a positive spacing and finite values");
+    return GWL_E_INVALID;
+  }
+  for (int64_t k = 0; k < n; ++k) {
+    if (!(density[k] >= 0) || !std::isfinite(density[k])) {
+      set_error("time prior table: densities must be finite and non-negative");
+      return GWL_E_INVALID;
+    }
+  }
+  if (*dst) (void)hipFree(*dst);
+  *dst = nullptr;
+  GWL_HIP_CHECK(hipMalloc((void**)dst, (size_t)n * sizeof(double)));
+  GWL_HIP_CHECK(hipMemcpy(*dst, density, (size_t)n * sizeof(double), hipMemcpyHostToDevice));
+  return GWL_OK;
+}
+}  // namespace gwl
+
 extern "C" {
 
 const char* gwl_last_error(void) { return g_last_error.c_str(); }
@@ -429,15 +449,15 @@ int gwl_event_create(gwl_ctx* ctx, const gwl_event_desc* d, gwl_event** out, int
     if (jhi < jlo) { set_error("time prior outside the segment"); return GWL_E_INVALID; }
     ev.jlo = (int)jlo;
     ev.L = (int)(jhi - jlo + 1);
+    // transform length: the power of two covering the window, at most 8192
+    // (float64 tiles of one row fit a workgroup's LDS up to that length);
+    // a longer window is evaluated in slices of P bins, one transform pass
+    // each (bilby's default prior, the whole segment, on long segments)
     long long P = 512;
-    while (P < ev.L) P *= 2;
+    while (P < ev.L && P < 8192) P *= 2;
     if (P > N) P = N;
-    if ((N % P) != 0 || (P & (P - 1)) != 0 || ev.L > P) {
-      set_error("time marginalisation needs N = n_freq - 1 divisible by a power of two >= the window");
-      return GWL_E_UNSUPPORTED;
-    }
-    if (P > 16384) {
-      set_error("time-prior window longer than 16384 samples is not supported");
+    if ((N % P) != 0 || (P & (P - 1)) != 0) {
+      set_error("time marginalisation needs N = n_freq - 1 divisible by a power of two");
       return GWL_E_UNSUPPORTED;
     }
     ev.P = (int)P;
@@ -577,6 +597,40 @@ int gwl_event_set_tile(gwl_event* e, int32_t rows, int32_t wg_per_cu) {
   return GWL_OK;
 }
 
+int gwl_event_set_escalation(gwl_event* e, double coeff, double tol) {
+  if (!e || !(coeff >= 0.0) || !(tol > 0.0)) {
+    set_error("gwl_event_set_escalation: invalid argument");
+    return GWL_E_INVALID;
+  }
+  e->esc_coeff = coeff;
+  e->esc_tol = tol;
+  return GWL_OK;
+}
+
+int gwl_event_escalated_rows(gwl_event* e, int64_t* rows) {
+  if (!e || !rows) {
+    set_error("gwl_event_escalated_rows: invalid argument");
+    return GWL_E_INVALID;
+  }
+  *rows = 0;
+  if (!e->esc_count_dev) return GWL_OK;
+  GWL_HIP_CHECK(hipSetDevice(e->ctx->device));
+  GWL_HIP_CHECK(hipDeviceSynchronize());
+  int c = 0;
+  GWL_HIP_CHECK(hipMemcpy(&c, e->esc_count_dev, sizeof(int), hipMemcpyDeviceToHost));
+  *rows = c;
+  return GWL_OK;
+}
+
+int gwl_event_set_multi_steps(gwl_event* e, int32_t steps) {
+  if (!e || steps < 0 || steps > 4096) {
+    set_error("gwl_event_set_multi_steps: invalid argument");
+    return GWL_E_INVALID;
+  }
+  e->multi_steps = steps;
+  return GWL_OK;
+}
+
 int gwl_event_set_transform(gwl_event* e, int32_t transform) {
   if (!e || transform < GWL_TRANSFORM_AUTO || transform > GWL_TRANSFORM_WAVE) {
     set_error("gwl_event_set_transform: invalid argument");
@@ -613,8 +667,25 @@ int gwl_event_destroy(gwl_event* e) {
   if (e->t1) (void)hipEventDestroy(e->t1);
   if (e->data) (void)hipFree(e->data);
   if (e->spline) (void)hipFree(e->spline);
+  if (e->tprior) (void)hipFree(e->tprior);
   if (e->ws) (void)hipFree(e->ws);
   delete e;
+  return GWL_OK;
+}
+
+
+int gwl_event_set_time_prior(gwl_event* e, int64_t n, double t0, double dt,
+                             const double* density) {
+  if (!e || !(e->dev.flags & GWL_TIME_MARG)) {
+    set_error("gwl_event_set_time_prior: the event does not marginalise over time");
+    return GWL_E_INVALID;
+  }
+  GWL_HIP_CHECK(hipSetDevice(e->ctx->device));
+  if (int rc = upload_time_prior(n, t0, dt, density, &e->tprior)) return rc;
+  e->dev.tpd = e->tprior;
+  e->dev.tp_t0 = t0;
+  e->dev.tp_ih = 1.0 / dt;
+  e->dev.tp_n = (int)n;
   return GWL_OK;
 }
 
@@ -660,6 +731,7 @@ int gwl_event_reserve(gwl_event* e, int64_t rows) {
     (void)hipFree(e->ws);
     e->ws = nullptr;
     e->ws_bytes = 0;
+    e->esc_count_dev = nullptr;  // lived in the old workspace
   }
   if (hipMalloc(&e->ws, need) != hipSuccess) {
     set_error("hipMalloc failed for workspace");
@@ -936,7 +1008,9 @@ int gwl_multi_create(gwl_ctx* ctx, int32_t n_events, gwl_event* const* events,
     m->params.push_back(params_dev[e]);
     m->outs.push_back(out_dev[e]);
     const int Q = ev->dev.N / 512;
-    int G = (Q + 31) / 32;  // ~32 sub-sequence steps per item
+    // sub-sequence steps per item: the event's tuned choice, else ~32
+    const int st = ev->multi_steps > 0 ? ev->multi_steps : 32;
+    int G = (Q + st - 1) / st;
     G = std::max(1, std::min(G, std::min(Q, 16)));
     m->groups.push_back(G);
     items += (size_t)((max_rows[e] + 3) / 4) * std::min(Q, 16);

@@ -66,6 +66,11 @@ struct EvDev {
   double wf_fmin, wf_fmax, fref;
   unsigned flags;
   double tmin, tmax, dref, sigma;
+  // time prior other than Uniform: its density on the grid tp_t0 + k / tp_ih,
+  // k < tp_n (null: Uniform on [tmin, tmax])
+  const double* tpd;
+  double tp_t0, tp_ih;
+  int tp_n;
   // data in decimated order: element i <-> bin n = Q*(i % P) + i / P, i < N.
   const float2* dw32;  // [nd][N]  (d/S) * sigma
   const float* w32;    // [nd][N]  sigma^2 / S
@@ -684,6 +689,29 @@ __device__ __forceinline__ T uniform_copy(const T& v) {
 #pragma unroll
   for (int k = 0; k < (int)(sizeof(T) / 4); ++k) dst[k] = __builtin_amdgcn_readfirstlane(src[k]);
   return out;
+}
+
+// ln of the geocent_time prior density at t (base.py:808-819 weighs each
+// time by prior.prob(t) delta_tc): the event's table, linear between its
+// points; -inf outside the table or where the density is zero.
+__device__ inline double time_log_density(const double* tpd, double t0, double ih, int n,
+                                          double t) {
+  const double x = (t - t0) * ih;
+  if (!(x >= 0.0) || !(x <= (double)(n - 1))) return -INFINITY;
+  const int k = min((int)x, n - 2);
+  const double w = x - (double)k;
+  const double p = fma(w, tpd[k + 1] - tpd[k], tpd[k]);
+  return p > 0.0 ? log(p) : -INFINITY;
+}
+
+// The time weight of one transform bin: with a Uniform prior nothing here
+// (the constant delta_tc / (max - min) is applied after the logsumexp, see
+// time_lse_scale), else ln prior.prob(t).
+__device__ __forceinline__ double time_log_weight(const EvDev& ev, double t) {
+  return ev.tpd ? time_log_density(ev.tpd, ev.tp_t0, ev.tp_ih, ev.tp_n, t) : 0.0;
+}
+__device__ __forceinline__ double time_lse_scale(const EvDev& ev, double delta_tc) {
+  return ev.tpd ? delta_tc : delta_tc / (ev.tmax - ev.tmin);
 }
 
 // Running logsumexp state (max m, sum s of exp(v - m)).

@@ -25,6 +25,7 @@ struct gwl_event {
   size_t data_bytes = 0;
   double* spline = nullptr;      // device: tx, ty, coefficients
   size_t spline_bytes = 0;
+  double* tprior = nullptr;      // device: time-prior density table
   void* ws = nullptr;            // device workspace (partials)
   size_t ws_bytes = 0;
   int64_t ws_rows = 0;
@@ -33,6 +34,13 @@ struct gwl_event {
   int tile_rows = 0;
   int tile_wg_per_cu = 0;
   int transform = GWL_TRANSFORM_AUTO;
+  int multi_steps = 0;  // sub-sequence steps per multi-event work item (0: default)
+  // float32 evaluations without time marginalisation re-evaluate in float64,
+  // in the same call, every row whose float32 error bound esc_coeff * Q
+  // exceeds esc_tol (Q: the row's inner-product scale); esc_coeff = 0: off
+  double esc_coeff = 2e-7;
+  double esc_tol = 8e-4;
+  int* esc_count_dev = nullptr;  // counter of the last escalating evaluation
   // kernel timing (measurement only)
   bool timing = false;
   hipEvent_t t0 = nullptr, t1 = nullptr;
@@ -102,6 +110,11 @@ size_t multi_lds_bytes();
 void set_error(const std::string& msg);
 int hip_fail(hipError_t e, const char* what);
 size_t workspace_bytes(const gwl_event* ev, int64_t rows);
+// the workspace of an evaluation at one precision (enum gwl_precision)
+size_t workspace_bytes_for(const gwl_event* ev, int64_t rows, int precision);
+// upload a time-prior density table (n values at t0 + k dt) into *dst (freed
+// and replaced), checking the arguments
+int upload_time_prior(int64_t n, double t0, double dt, const double* density, double** dst);
 int launch_eval(gwl_event* ev, const double* params, int64_t n, int ld, int precision, double* out,
                 double* aux, hipStream_t stream);
 int timing_begin(gwl_event* ev, hipStream_t stream);
@@ -113,8 +126,12 @@ int gemm_bf16x3_split(const __bf16* Ah, const __bf16* Al, const __bf16* Bh, cons
 int launch_response(gwl_event* ev, const double* params_dev, int det, int precision, double* out_dev,
                     hipStream_t stream);
 // float64 epilogue of the time-marginalised path over G window partials
+// (slice: one window slice of the event, with the logsumexp state lse carried
+// from the first slice to the last)
 int launch_epi_time(gwl_event* ev, const double* params, int64_t n, int ld, int G,
-                    const double2* part, const double* hhpart, double* out, hipStream_t stream);
+                    const double2* part, const double* hhpart, double* out, hipStream_t stream,
+                    const gwl::EvDev* slice = nullptr, double2* lse = nullptr, int first = 1,
+                    int last = 1);
 // wave-per-row TaylorF2 time transform (gwl_time_wave.hip)
 bool time_wave_eligible(const gwl_event* ev);
 int time_wave_groups(const gwl_event* ev, long long n);

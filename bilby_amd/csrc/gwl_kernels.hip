@@ -39,13 +39,22 @@ struct RedArgs {
   int nchunk;
   double* part; // [n][nchunk][3]
   const SampleConst* rows;  // [n] row constants (k_sample_const)
+  const int* count;         // if set: only rows r < *count exist (n bounds it)
 };
 
+__device__ __forceinline__ long long live_rows(long long n, const int* count) {
+  return count ? min(n, (long long)*count) : n;
+}
+
 // One thread per row: the row constants every k_reduce tile of the row reads.
+// zero (optional) is cleared by thread 0: the escalation counter the
+// epilogue of the same evaluation fills.
 __global__ __launch_bounds__(128) void k_sample_const(EvDev ev, const double* params, int ld,
-                                                      long long n, SampleConst* out) {
+                                                      long long n, SampleConst* out,
+                                                      const int* count, int* zero) {
   const long long r = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= n) return;
+  if (zero && r == 0) *zero = 0;
+  if (r >= live_rows(n, count)) return;
   SampleConst sc;
   compute_sample_const(ev, params + r * ld, sc);
   out[r] = sc;
@@ -60,9 +69,10 @@ __global__ __launch_bounds__(128) void k_sample_const(EvDev ev, const double* pa
 // always computes.
 // The sums of one chunk for the S rows of a tile (see k_reduce).
 template <typename R, int WAVE, int ND, int S, int NT>
-__device__ __forceinline__ void reduce_chunk(const RedArgs& a, int chunk, long long row0,
-                                             const SampleConst* sc, const SampleFast<R>* sf,
-                                             const int* band, double (*red)[S][3]) {
+__device__ __forceinline__ void reduce_chunk(const RedArgs& a, long long n, int chunk,
+                                             long long row0, const SampleConst* sc,
+                                             const SampleFast<R>* sf, const int* band,
+                                             double (*red)[S][3]) {
   using C = cplx<R>;
   const EvDev& ev = a.ev;
   const int tid = threadIdx.x;
@@ -138,7 +148,7 @@ __device__ __forceinline__ void reduce_chunk(const RedArgs& a, int chunk, long l
   if (tid < S * 3) {
     const int s = tid / 3, c = tid % 3;
     const long long r = row0 + s;
-    if (r < a.n) {
+    if (r < n) {
       double v = 0.0;
       for (int w = 0; w < NT / 64; ++w) v += red[w][s][c];
       a.part[(r * a.nchunk + chunk) * 3 + c] = v;
@@ -160,12 +170,14 @@ __global__ __launch_bounds__(NT) void k_reduce(RedArgs a) {
   __shared__ int band[2];
   __shared__ double red[NT / 64][S][3];
   const long long row0 = (long long)blockIdx.y * S;
-  tile_prologue_load<R, S>(a.ev, a.rows, a.n, row0, sc, sf, band);
+  const long long n = live_rows(a.n, a.count);
+  if (row0 >= n) return;  // uniform over the workgroup
+  tile_prologue_load<R, S>(a.ev, a.rows, n, row0, sc, sf, band);
   if constexpr (!LOOP) {
-    reduce_chunk<R, WAVE, ND, S, NT>(a, blockIdx.x, row0, sc, sf, band, red);
+    reduce_chunk<R, WAVE, ND, S, NT>(a, n, blockIdx.x, row0, sc, sf, band, red);
   } else {
     for (int chunk = blockIdx.x; chunk < a.nchunk; chunk += gridDim.x) {
-      reduce_chunk<R, WAVE, ND, S, NT>(a, chunk, row0, sc, sf, band, red);
+      reduce_chunk<R, WAVE, ND, S, NT>(a, n, chunk, row0, sc, sf, band, red);
       __syncthreads();  // red is reused by the next chunk
     }
   }
@@ -351,8 +363,39 @@ struct EpiArgs {
   const double* hhpart;
   double* out;
   double* aux;
+  // a window evaluated in slices: running (max, scaled sum) logsumexp state
+  // per row [n], first / last slice flags
+  double2* lse;
+  int first, last;
   const SampleConst* rows;  // [n] row constants, or null (then computed here)
+  // live rows and output slots: with count set only rows r < *count exist
+  // and row r's results go to out[scatter[r]] (the float64 pass over the
+  // escalated rows of a float32 evaluation)
+  const int* count;
+  const int* scatter;
+  // escalation (float32 pass): rows with esc_coeff * Q > esc_tol append
+  // their index to esc_idx and their parameter row to esc_params
+  // ([n][GWL_NPARAM]) through the counter esc_count
+  int* esc_count;
+  int* esc_idx;
+  double* esc_params;
+  double esc_coeff, esc_tol;
 };
+
+// The scale Q of a row's inner products that bounds the float32 error of
+// its ln L as esc_coeff * Q: |<d|h>| + <h|h> for the plain and phase-
+// marginalised forms; under distance marginalisation the table is read at
+// dt = <d|h> d / d_ref, ht = <h|h> (d / d_ref)^2, and ln L moves by
+// s dt - s^2 ht / 2 near its peak s = dt / ht, so Q = 2 dt^2 / ht.
+__device__ inline double escalation_scale(unsigned flags, double dhr, double dhi, double hh,
+                                          double hs) {
+  if (flags & GWL_DISTANCE_MARG) {
+    const double dt = ((flags & GWL_PHASE_MARG) ? hypot(dhr, dhi) : fabs(dhr)) * hs;
+    const double ht = hh * hs * hs;
+    return ht > 0.0 ? 2.0 * dt * dt / ht + dt + ht : dt;
+  }
+  return hypot(dhr, dhi) + hh;
+}
 
 // lnL from the summed inner products (base.py:448-477, 775-792); one thread
 // per row, knots in LDS.
@@ -363,7 +406,8 @@ __global__ __launch_bounds__(128) void k_epi_plain(EpiArgs a) {
   if (dist) stage_knots(ev, knots);
   __syncthreads();
   const long long r = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= a.n) return;
+  if (r >= live_rows(a.n, a.count)) return;
+  const long long o = a.scatter ? (long long)a.scatter[r] : r;
   SampleConst sc;
   if (a.rows)
     sc = a.rows[r];
@@ -387,13 +431,20 @@ __global__ __launch_bounds__(128) void k_epi_plain(EpiArgs a) {
   const double dhr = norm * dr, dhi = -norm * di;
   hh *= norm;
   if (a.aux) {
-    a.aux[3 * r + 0] = dhr;
-    a.aux[3 * r + 1] = dhi;
-    a.aux[3 * r + 2] = hh;
+    a.aux[3 * o + 0] = dhr;
+    a.aux[3 * o + 1] = dhi;
+    a.aux[3 * o + 2] = hh;
   }
   if (!sc.valid) {
-    a.out[r] = row_valid_sentinel();
+    a.out[o] = row_valid_sentinel();
     return;
+  }
+  if (a.esc_count && a.esc_coeff * escalation_scale(ev.flags, dhr, dhi, hh, sc.hscale) > a.esc_tol) {
+    const int slot = atomicAdd(a.esc_count, 1);
+    a.esc_idx[slot] = (int)r;
+    const double* src = a.params + r * a.ld;
+    double* dst = a.esc_params + (size_t)slot * GWL_NPARAM;
+    for (int k = 0; k < GWL_NPARAM; ++k) dst[k] = src[k];
   }
   double ll;
   if (dist) {
@@ -406,7 +457,7 @@ __global__ __launch_bounds__(128) void k_epi_plain(EpiArgs a) {
   } else {
     ll = dhr - hh / 2.0;
   }
-  a.out[r] = ll;
+  a.out[o] = ll;
 }
 
 // Time-marginalised lnL: logsumexp over the prior window (base.py:794-820).
@@ -418,7 +469,8 @@ constexpr int kEpiRows = 4;
 // logsumexp.  ``knots`` holds the spline knots in LDS (distance only).
 __device__ inline void epi_time_row(const EvDev& ev, const double* params, int ld, long long r,
                                     int G, const double2* part, const double* hhpart, double* out,
-                                    const double* knots) {
+                                    const double* knots, double2* lse = nullptr, int first = 1,
+                                    int last = 1) {
   const int lane = threadIdx.x & 63;
   const bool dist = (ev.flags & GWL_DISTANCE_MARG) != 0;
   double hh = 0.0, jit = 0.0, hs = 1.0;
@@ -467,6 +519,7 @@ __device__ inline void epi_time_row(const EvDev& ev, const double* params, int l
     } else {
       v = xr - hh / 2.0;
     }
+    v += time_log_weight(ev, t);
     if (v > m) {
       s = (m > -INFINITY ? s * exp(m - v) : 0.0) + 1.0;
       m = v;
@@ -486,8 +539,21 @@ __device__ inline void epi_time_row(const EvDev& ev, const double* params, int l
     m = mn;
   }
   if (lane == 0) {
-    const double b = (1.0 / (ev.tmax - ev.tmin)) * ev.tdelta;
-    out[r] = (m > -INFINITY) ? log(b * s) + m : -INFINITY;
+    if (!first) {  // merge the slices before this one
+      const double2 st = lse[r];
+      const double mn = fmax(m, st.x);
+      if (mn > -INFINITY)
+        s = (m > -INFINITY ? s * exp(m - mn) : 0.0) + (st.x > -INFINITY ? st.y * exp(st.x - mn) : 0.0);
+      else
+        s = 0.0;
+      m = mn;
+    }
+    if (!last) {
+      lse[r] = make_double2(m, s);
+    } else {
+      const double b = time_lse_scale(ev, ev.tdelta);
+      out[r] = (m > -INFINITY) ? log(b * s) + m : -INFINITY;
+    }
   }
 }
 
@@ -497,7 +563,8 @@ __global__ __launch_bounds__(64 * kEpiRows) void k_epi_time(EpiArgs a) {
   __syncthreads();
   const long long r = (long long)blockIdx.x * kEpiRows + (threadIdx.x >> 6);
   if (r >= a.n) return;
-  epi_time_row(a.ev, a.params, a.ld, r, a.G, a.part, a.hhpart, a.out, knots);
+  epi_time_row(a.ev, a.params, a.ld, r, a.G, a.part, a.hhpart, a.out, knots, a.lse, a.first,
+               a.last);
 }
 
 // ------------------------------------------------------------------ multi-event
@@ -656,8 +723,9 @@ static int time_tile_groups(const EvDev& ev, long long n, int S, int n_cu, int w
 }
 
 int launch_epi_time(gwl_event* e, const double* params, int64_t n, int ld, int G,
-                    const double2* part, const double* hhpart, double* out, hipStream_t st) {
-  const EvDev& ev = e->dev;
+                    const double2* part, const double* hhpart, double* out, hipStream_t st,
+                    const EvDev* slice, double2* lse, int first, int last) {
+  const EvDev& ev = slice ? *slice : e->dev;
   EpiArgs ea{};
   ea.ev = ev;
   ea.params = params;
@@ -667,6 +735,9 @@ int launch_epi_time(gwl_event* e, const double* params, int64_t n, int ld, int G
   ea.part = part;
   ea.hhpart = hhpart;
   ea.out = out;
+  ea.lse = lse;
+  ea.first = first;
+  ea.last = last;
   const size_t klds = (ev.flags & GWL_DISTANCE_MARG) ? (size_t)(ev.sp.nx + ev.sp.ny + 8) * sizeof(double) : 0;
   hipLaunchKernelGGL(k_epi_time, dim3((unsigned)((n + kEpiRows - 1) / kEpiRows)),
                      dim3(64 * kEpiRows), klds, st, ea);
@@ -681,12 +752,21 @@ static size_t reduce_part_bytes(const EvDev& ev, long long rows) {
   return ((size_t)rows * nchunk * 3 * sizeof(double) + 255) / 256 * 256;
 }
 
+// Bytes in front of the tiled transform's partials in the event workspace:
+// the per-row logsumexp state of a window evaluated in slices.
+static size_t time_lse_bytes(long long rows) {
+  return ((size_t)rows * sizeof(double2) + 255) / 256 * 256;
+}
+
 // The tiled time-marginalised transform: window partials [n][G][L] and
-// <h|h> partials [n][G] in the event workspace (G returned).
+// <h|h> partials [n][G] in the event workspace after the logsumexp state
+// (G returned).  slice: the event with jlo / L of one window slice (null:
+// the whole window).
 template <typename R, int WAVE, int ND>
 static int launch_time_tiled(gwl_event* e, const double* params, int64_t n, int ld,
-                             hipStream_t st, int* G_out, double2** part_out, double** hh_out) {
-  const EvDev& ev = e->dev;
+                             hipStream_t st, int* G_out, double2** part_out, double** hh_out,
+                             const EvDev* slice = nullptr) {
+  const EvDev& ev = slice ? *slice : e->dev;
   {
     // tile: S rows x P-point transforms, G groups of sub-sequences
     constexpr int NT = 512;
@@ -700,9 +780,9 @@ static int launch_time_tiled(gwl_event* e, const double* params, int64_t n, int 
     const long long nblk = (n + S - 1) / S;
     const int G = time_tile_groups(ev, n, S, e->ctx->n_cu, e->tile_wg_per_cu);
     ta.G = G;
-    ta.part = reinterpret_cast<double2*>(e->ws);
-    ta.hhpart = reinterpret_cast<double*>(reinterpret_cast<char*>(e->ws) +
-                                          (size_t)n * G * ev.L * sizeof(double2));
+    char* base = reinterpret_cast<char*>(e->ws) + time_lse_bytes(n);
+    ta.part = reinterpret_cast<double2*>(base);
+    ta.hhpart = reinterpret_cast<double*>(base + (size_t)n * G * ev.L * sizeof(double2));
     dim3 grid(G, (unsigned)nblk);
 #define GWL_LAUNCH_TIME(SS, PP)                                                                  \
   do {                                                                                           \
@@ -740,22 +820,50 @@ static int launch_time_tiled(gwl_event* e, const double* params, int64_t n, int 
   }
 }
 
+// Workspace of the inner-product path: partials [n][nchunk][3], row
+// constants [n], then the escalation list (parameter rows [n][GWL_NPARAM],
+// row indices [n], the counter).
+struct ReduceWs {
+  double* part;
+  SampleConst* rows;
+  double* esc_params;
+  int* esc_idx;
+  int* esc_count;
+};
+
+static ReduceWs reduce_ws(gwl_event* e, long long n) {
+  ReduceWs w;
+  char* p = reinterpret_cast<char*>(e->ws);
+  w.part = reinterpret_cast<double*>(p);
+  p += reduce_part_bytes(e->dev, n);
+  w.rows = reinterpret_cast<SampleConst*>(p);
+  p += ((size_t)n * sizeof(SampleConst) + 255) / 256 * 256;
+  w.esc_params = reinterpret_cast<double*>(p);
+  p += ((size_t)n * GWL_NPARAM * sizeof(double) + 255) / 256 * 256;
+  w.esc_idx = reinterpret_cast<int*>(p);
+  p += ((size_t)n * sizeof(int) + 255) / 256 * 256;
+  w.esc_count = reinterpret_cast<int*>(p);
+  return w;
+}
+
+static size_t reduce_ws_bytes(const EvDev& ev, long long n) {
+  return reduce_part_bytes(ev, n) + ((size_t)n * sizeof(SampleConst) + 255) / 256 * 256 +
+         ((size_t)n * GWL_NPARAM * sizeof(double) + 255) / 256 * 256 +
+         ((size_t)n * sizeof(int) + 255) / 256 * 256 + 256;
+}
+
+// One pass of the inner-product path (no time marginalisation): row
+// constants, k_reduce, epilogue.  count / scatter: the float64 pass over the
+// rows a float32 pass escalated (their number on the device, n bounds it);
+// esc: the float32 pass that selects them.
 template <typename R, int WAVE, int ND>
-static int eval_typed(gwl_event* e, const double* params, int64_t n, int ld, double* out,
-                      double* aux, hipStream_t st) {
+static int launch_reduce(gwl_event* e, const double* params, int64_t n, int ld, double* out,
+                         double* aux, hipStream_t st, const int* count, const int* scatter,
+                         bool esc) {
   const EvDev& ev = e->dev;
-  if constexpr (std::is_same<R, float>::value && WAVE == GWL_TAYLORF2) {
-    if (time_wave_eligible(e)) return launch_time_wave(e, params, n, ld, out, st);
-  }
-  if (ev.flags & GWL_TIME_MARG) {
-    int G = 0;
-    double2* part = nullptr;
-    double* hh = nullptr;
-    if (int rc = launch_time_tiled<R, WAVE, ND>(e, params, n, ld, st, &G, &part, &hh)) return rc;
-    return launch_epi_time(e, params, n, ld, G, part, hh, out, st);
-  }
   constexpr int NT = 256;
   const int S = e->tile_rows >= 16 ? 16 : e->tile_rows >= 8 || e->tile_rows <= 0 ? 8 : 4;
+  const ReduceWs ws = reduce_ws(e, n);
   RedArgs ra;
   ra.ev = ev;
   ra.params = params;
@@ -765,23 +873,24 @@ static int eval_typed(gwl_event* e, const double* params, int64_t n, int ld, dou
   // reduction order does not depend on what it is batched with
   ra.chunk = NT * 4;
   ra.nchunk = (ev.N + ra.chunk - 1) / ra.chunk;
-  ra.part = reinterpret_cast<double*>(e->ws);
-  ra.rows = reinterpret_cast<const SampleConst*>(
-      reinterpret_cast<char*>(e->ws) + reduce_part_bytes(ev, n));
+  ra.part = ws.part;
+  ra.rows = ws.rows;
+  ra.count = count;
   const long long nblk = (n + S - 1) / S;
-  // chunk groups per tile.  A source whose rows span the whole band (the
-  // sine-Gaussian) keeps every chunk in parallel; TaylorF2 rows end at
-  // their ISCO frequency, so most chunks lie outside every row's support
-  // and fewer, longer workgroups (one per tile once the tiles alone fill
-  // the CUs several times) avoid launching workgroups that skip them all
+  // chunk groups per tile: every chunk in parallel when the rows' supports
+  // span the band; with the ISCO termination TaylorF2 supports are short,
+  // most chunks lie outside every row's support and fewer, longer
+  // workgroups (one per tile once the tiles alone fill the CUs several
+  // times) avoid launching workgroups that skip them all
   const long long target = 8LL * e->ctx->n_cu;
-  const int G = WAVE != GWL_TAYLORF2
+  const int G = (WAVE != GWL_TAYLORF2 || !(ev.flags & GWL_TF2_END_ISCO))
                     ? ra.nchunk
                     : (int)std::max<long long>(1, std::min<long long>(
                                                       ra.nchunk, (target + nblk - 1) / nblk));
-  if (int rc = timing_begin(e, st)) return rc;
+  if (!count)
+    if (int rc = timing_begin(e, st)) return rc;
   hipLaunchKernelGGL(k_sample_const, dim3((unsigned)((n + 127) / 128)), dim3(128), 0, st, ev,
-                     params, ld, (long long)n, const_cast<SampleConst*>(ra.rows));
+                     params, ld, (long long)n, ws.rows, count, esc ? ws.esc_count : nullptr);
   const dim3 grid(G, (unsigned)nblk);
   if (G == ra.nchunk) {
     if (S == 16)
@@ -799,7 +908,8 @@ static int eval_typed(gwl_event* e, const double* params, int64_t n, int ld, dou
       hipLaunchKernelGGL((k_reduce<R, WAVE, ND, 8, NT, true>), grid, dim3(NT), 0, st, ra);
   }
   GWL_HIP_CHECK(hipGetLastError());
-  if (int rc = timing_end(e, st)) return rc;
+  if (!count)
+    if (int rc = timing_end(e, st)) return rc;
   EpiArgs ea{};
   ea.ev = ev;
   ea.params = params;
@@ -811,10 +921,58 @@ static int eval_typed(gwl_event* e, const double* params, int64_t n, int ld, dou
   ea.out = out;
   ea.aux = aux;
   ea.rows = ra.rows;
+  ea.count = count;
+  ea.scatter = scatter;
+  if (esc) {
+    ea.esc_count = ws.esc_count;
+    ea.esc_idx = ws.esc_idx;
+    ea.esc_params = ws.esc_params;
+    ea.esc_coeff = e->esc_coeff;
+    ea.esc_tol = e->esc_tol;
+  }
   const size_t klds = (ev.flags & GWL_DISTANCE_MARG) ? (size_t)(ev.sp.nx + ev.sp.ny + 8) * sizeof(double) : 0;
   hipLaunchKernelGGL(k_epi_plain, dim3((unsigned)((n + 127) / 128)), dim3(128), klds, st, ea);
   GWL_HIP_CHECK(hipGetLastError());
   return GWL_OK;
+}
+
+template <typename R, int WAVE, int ND>
+static int eval_typed(gwl_event* e, const double* params, int64_t n, int ld, double* out,
+                      double* aux, hipStream_t st) {
+  const EvDev& ev = e->dev;
+  if constexpr (std::is_same<R, float>::value && WAVE == GWL_TAYLORF2) {
+    if (time_wave_eligible(e)) return launch_time_wave(e, params, n, ld, out, st);
+  }
+  if (ev.flags & GWL_TIME_MARG) {
+    // a window longer than one transform: slices of P bins, one pass each,
+    // the logsumexp carried between them
+    const int passes = (ev.L + ev.P - 1) / ev.P;
+    double2* lse = reinterpret_cast<double2*>(e->ws);
+    for (int p = 0; p < passes; ++p) {
+      EvDev slice = ev;
+      slice.jlo = ev.jlo + p * ev.P;
+      slice.L = std::min(ev.P, ev.L - p * ev.P);
+      int G = 0;
+      double2* part = nullptr;
+      double* hh = nullptr;
+      if (int rc = launch_time_tiled<R, WAVE, ND>(e, params, n, ld, st, &G, &part, &hh, &slice))
+        return rc;
+      if (int rc = launch_epi_time(e, params, n, ld, G, part, hh, out, st, &slice, lse, p == 0,
+                                   p == passes - 1))
+        return rc;
+    }
+    return GWL_OK;
+  }
+  const bool esc = std::is_same<R, float>::value && e->esc_coeff > 0.0;
+  if (int rc = launch_reduce<R, WAVE, ND>(e, params, n, ld, out, aux, st, nullptr, nullptr, esc))
+    return rc;
+  if (!esc) return GWL_OK;
+  // the escalated rows again in float64: same stream, no host round trip;
+  // the grid is sized for n rows and the kernels read the live count
+  const ReduceWs ws = reduce_ws(e, n);
+  e->esc_count_dev = ws.esc_count;
+  return launch_reduce<double, WAVE, ND>(e, ws.esc_params, n, GWL_NPARAM, out, aux, st,
+                                         ws.esc_count, ws.esc_idx, false);
 }
 
 template <typename R, int WAVE>
@@ -830,6 +988,15 @@ static int eval_nd(gwl_event* e, const double* params, int64_t n, int ld, double
   return GWL_E_INVALID;
 }
 
+size_t workspace_bytes_for(const gwl_event* e, int64_t rows, int precision) {
+  // a float32 evaluation on the wave-per-row transform uses that path's
+  // workspace only
+  if ((e->dev.flags & GWL_TIME_MARG) && precision == GWL_FP32 && e->dev.approx == GWL_TAYLORF2 &&
+      time_wave_eligible(e))
+    return time_wave_ws_bytes(e, rows);
+  return workspace_bytes(e, rows);
+}
+
 size_t workspace_bytes(const gwl_event* e, int64_t rows) {
   const EvDev& ev = e->dev;
   if (ev.flags & GWL_TIME_MARG) {
@@ -840,11 +1007,12 @@ size_t workspace_bytes(const gwl_event* e, int64_t rows) {
     const long long bound = std::max<long long>(
         (long long)rows * time_tile_groups(ev, rows, S, e->ctx->n_cu, e->tile_wg_per_cu),
         2LL * 8 * wpc * e->ctx->n_cu);
-    size_t b = (size_t)bound * (ev.L * sizeof(double2) + sizeof(double)) + 4096;
+    size_t b = (size_t)bound * (std::min(ev.L, ev.P) * sizeof(double2) + sizeof(double)) +
+               time_lse_bytes(rows) + 4096;
     if (ev.approx == GWL_TAYLORF2 && ev.tf2b) b = std::max(b, time_wave_ws_bytes(e, rows));
     return b;
   }
-  return reduce_part_bytes(ev, rows) + (size_t)rows * sizeof(SampleConst) + 4096;
+  return reduce_ws_bytes(ev, rows) + 4096;
 }
 
 int launch_eval(gwl_event* e, const double* params, int64_t n, int ld, int precision, double* out,
@@ -862,9 +1030,10 @@ int launch_eval(gwl_event* e, const double* params, int64_t n, int ld, int preci
 // d_inner_h_array at the window and <h|h> of every row (base.py:325-330
 // before the time marginalisation): the group partials summed and scaled by
 // 4/T; one thread per (row, window bin).
+// A window slice of L bins goes to columns [off, off + L) of rows of ldo bins.
 __global__ __launch_bounds__(256) void k_time_series_out(long long n, int G, int L, double norm,
                                                          const double2* part, const double* hhpart,
-                                                         double* dh, double* hh) {
+                                                         double* dh, double* hh, int ldo, int off) {
   const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n * L) return;
   const long long r = t / L;
@@ -875,8 +1044,9 @@ __global__ __launch_bounds__(256) void k_time_series_out(long long n, int G, int
     xr += v.x;
     xi += v.y;
   }
-  dh[2 * t + 0] = norm * xr;
-  dh[2 * t + 1] = norm * xi;
+  const size_t o = (size_t)r * ldo + off + jj;
+  dh[2 * o + 0] = norm * xr;
+  dh[2 * o + 1] = norm * xi;
   if (jj == 0) {
     double s = 0.0;
     for (int g = 0; g < G; ++g) s += hhpart[(size_t)r * G + g];
@@ -887,15 +1057,22 @@ __global__ __launch_bounds__(256) void k_time_series_out(long long n, int G, int
 template <typename R, int WAVE, int ND>
 static int series_typed(gwl_event* e, const double* params, int64_t n, int ld, double* dh,
                         double* hh, hipStream_t st) {
-  int G = 0;
-  double2* part = nullptr;
-  double* hhp = nullptr;
-  if (int rc = launch_time_tiled<R, WAVE, ND>(e, params, n, ld, st, &G, &part, &hhp)) return rc;
   const EvDev& ev = e->dev;
-  const long long total = (long long)n * ev.L;
-  hipLaunchKernelGGL(k_time_series_out, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st,
-                     (long long)n, G, ev.L, 4.0 / ev.T, part, hhp, dh, hh);
-  GWL_HIP_CHECK(hipGetLastError());
+  // a window longer than one transform in slices of P bins
+  for (int off = 0; off < ev.L; off += ev.P) {
+    EvDev slice = ev;
+    slice.jlo = ev.jlo + off;
+    slice.L = std::min(ev.P, ev.L - off);
+    int G = 0;
+    double2* part = nullptr;
+    double* hhp = nullptr;
+    if (int rc = launch_time_tiled<R, WAVE, ND>(e, params, n, ld, st, &G, &part, &hhp, &slice))
+      return rc;
+    const long long total = (long long)n * slice.L;
+    hipLaunchKernelGGL(k_time_series_out, dim3((unsigned)((total + 255) / 256)), dim3(256), 0,
+                       st, (long long)n, G, slice.L, 4.0 / ev.T, part, hhp, dh, hh, ev.L, off);
+    GWL_HIP_CHECK(hipGetLastError());
+  }
   return GWL_OK;
 }
 

@@ -928,6 +928,13 @@ int gwl_relbin_set_time(gwl_relbin* rb, const int64_t* bin_index, const double* 
     rc = gwl::hip_fail(err, "gwl_relbin_set_time upload");
   } else {
     // the event's window and time grid (base.py:1027-1035)
+    if (d.L > 16384) {
+      (void)hipFree(de);
+      (void)hipFree(dx);
+      set_error("gwl_relbin_set_time: the binned window transform covers at most 16384 "
+                "samples of time window");
+      return GWL_E_UNSUPPORTED;
+    }
     wb.L = d.L;
     wb.j_first = d.jlo;
     wb.start = d.start;

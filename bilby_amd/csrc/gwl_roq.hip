@@ -46,6 +46,10 @@ struct RoqDev {
   const double* nbl;  // [L][kNB + 1] TaylorF2 basis (tf2_basis order) and f^-7/6 at the linear nodes
   const double* nbq;  // [Qn][kNB + 1] the same at the quadratic nodes
   SplineDev sp;
+  // time prior other than Uniform: density table (null: Uniform)
+  const double* tpd;
+  double tp_t0, tp_ih;
+  int tp_n;
 };
 
 struct RoqRow {
@@ -70,6 +74,7 @@ struct gwl_roq {
   bool has_spline = false;
   void* ws = nullptr;
   size_t ws_bytes = 0;
+  double* tprior = nullptr;
 };
 
 namespace gwl {
@@ -573,6 +578,7 @@ __global__ __launch_bounds__(64 * kRoqEpiRows) void k_roq_epi(RoqDev rd, const T
     // roq.py:537-540: ifo_times = (times - start) + dt + jitter
     dh_at(tj - rd.start, rr.jitter, false, ok, sr, si);
     double v = ok ? lnl_of(sr, si) : -INFINITY;
+    if (rd.tpd) v += time_log_density(rd.tpd, rd.tp_t0, rd.tp_ih, rd.tp_n, t);
     if (v > m) {
       s = (m > -INFINITY ? s * exp(m - v) : 0.0) + 1.0;
       m = v;
@@ -591,7 +597,7 @@ __global__ __launch_bounds__(64 * kRoqEpiRows) void k_roq_epi(RoqDev rd, const T
     m = mn;
   }
   if (lane == 0) {
-    const double b = (1.0 / (rd.tmax - rd.tmin)) * rd.dtc;
+    const double b = rd.tpd ? rd.dtc : rd.dtc / (rd.tmax - rd.tmin);
     out[r] = (m > -INFINITY) ? log(b * s) + m : -INFINITY;
   }
 }
@@ -851,8 +857,23 @@ int gwl_roq_destroy(gwl_roq* e) {
   (void)hipSetDevice(e->ctx->device);
   if (e->data) (void)hipFree(e->data);
   if (e->spline) (void)hipFree(e->spline);
+  if (e->tprior) (void)hipFree(e->tprior);
   if (e->ws) (void)hipFree(e->ws);
   delete e;
+  return GWL_OK;
+}
+
+int gwl_roq_set_time_prior(gwl_roq* e, int64_t n, double t0, double dt, const double* density) {
+  if (!e || !(e->dev.flags & GWL_TIME_MARG)) {
+    set_error("gwl_roq_set_time_prior: the likelihood does not marginalise over time");
+    return GWL_E_INVALID;
+  }
+  GWL_HIP_CHECK(hipSetDevice(e->ctx->device));
+  if (int rc = upload_time_prior(n, t0, dt, density, &e->tprior)) return rc;
+  e->dev.tpd = e->tprior;
+  e->dev.tp_t0 = t0;
+  e->dev.tp_ih = 1.0 / dt;
+  e->dev.tp_n = (int)n;
   return GWL_OK;
 }
 

@@ -168,10 +168,32 @@ struct TimeWArgs {
   const int* only;   // if set: rows r with only[r] == 0 are skipped
 };
 
+// Psi(f) / 2pi (cycles) of one row at a tabled basis point: the nine-term
+// sum of tf2_phase_cycles in Horner form, float64.
+__device__ __forceinline__ double tf2_cycles_horner(const RowW& rw, const double* tb) {
+  const double u = tb[0], v = tb[1], lf = tb[2];
+  // c0 v^5 + c1 v^3 + c2 v^2 + c3 v + c4
+  double a = fma(rw.c[0], v * v, rw.c[1]);
+  a = fma(a, v, rw.c[2]);
+  a = fma(a, v, rw.c[3]);
+  double s = fma(a, v, rw.c[4]);
+  // + c5 u + c6 u^2 + (c7 + c8 u) ln f
+  s = fma(fma(rw.c[6], u, rw.c[5]), u, s);
+  return fma(fma(rw.c[8], u, rw.c[7]), lf, s);
+}
+
 // The transform of one row over sub-sequences [n2b, n2e): lane l returns
 // the float64 window accumulators of bins j = l + 64 s (mod 512) in
 // accr[s], acci[s].  tw: the 512-entry twiddle table in LDS; xb: this
 // wavefront's exchange buffer.  nd <= ND detectors.
+//
+// The integrand h conj(d) / S = exp(-2 pi i Psi(f) / 2pi) sum_d C_d
+// exp(-2 pi i f dt_d) conj(dw_d f^-7/6): the source phase (float64, reduced
+// to [0, 1) cycles) is rotated once per element; each detector's delay
+// rotation starts from an exact float64 phase at the lane's first element
+// of the sub-sequence, f0 = (Q l + n2) df, and advances by the step
+// w_d = exp(-2 pi i 64 Q df dt_d) (float64, rounded to float32) to the
+// lane's elements l + 64 s, whose frequencies are f0 + 64 Q df s.
 template <int ND>
 __device__ __forceinline__ void wave_row(const WaveEv& ev, const RowW& rw, int nd, int n2b, int n2e,
                                          const float2* tw, const float2* tw3, float2* xb, int lane,
@@ -186,19 +208,52 @@ __device__ __forceinline__ void wave_row(const WaveEv& ev, const RowW& rw, int n
   const float invN = 1.0f / (float)ev.N;
   const int kb = rw.kb, ke = rw.ke;
   const int base2 = 64 * (lane >> 3) + (lane & 7);
+  // the steps in float64 (once per row), rounded once to float32
+  float2 wd[ND];
+#pragma unroll
+  for (int d = 0; d < ND; ++d) {
+    const double st = (64.0 * (double)Q * ev.df) * rw.dt[d];
+    const double2 w = expm2pi(st - floor(st));
+    wd[d] = make_float2((float)w.x, (float)w.y);
+  }
 
   for (int n2 = n2b; n2 < n2e; ++n2) {
     // in-support elements of this sub-sequence: kb <= Q i + n2 <= ke
     const int ilo = (kb - n2 + Q - 1) / Q;
     const int ihi = ke >= n2 ? (ke - n2) / Q : -1;
     float2 x[8];
+    if (ilo <= ihi) {
+      const double f0 = (double)(Q * lane + n2) * ev.df;
+      float2 r[ND];
 #pragma unroll
-    for (int s = 0; s < 8; ++s) {
-      x[s] = make_float2(0.f, 0.f);
-      if (ilo <= ihi && ilo <= 64 * s + 63 && ihi >= 64 * s) {
-        const int i = lane + 64 * s;
-        if (i >= ilo && i <= ihi) x[s] = integrand_tf2<ND>(ev, rw, nd, n2 * kWP + i, Q * i + n2);
+      for (int d = 0; d < ND; ++d) {
+        const double c0 = f0 * rw.dt[d];
+        r[d] = cmul(make_float2(rw.cre[d], rw.cim[d]), expm2pi((float)(c0 - floor(c0))));
       }
+      const float2* dwa = ev.dwa32 + (size_t)n2 * kWP + lane;
+      const double* tb = ev.tf2b + 3 * ((size_t)n2 * kWP + lane);
+#pragma unroll
+      for (int s = 0; s < 8; ++s) {
+        x[s] = make_float2(0.f, 0.f);
+        const int i = lane + 64 * s;
+        if (i >= ilo && i <= ihi) {
+          const double cyc = tf2_cycles_horner(rw, tb + 3 * 64 * s);
+          float2 y = make_float2(0.f, 0.f);
+#pragma unroll
+          for (int d = 0; d < ND; ++d) {
+            if (d >= nd) break;  // the event holds nd detectors' tables
+            const float2 dw = dwa[(size_t)d * ev.N + 64 * s];
+            y.x += r[d].x * dw.x + r[d].y * dw.y;
+            y.y += r[d].y * dw.x - r[d].x * dw.y;
+          }
+          x[s] = cmul(expm2pi((float)(cyc - floor(cyc))), y);
+        }
+#pragma unroll
+        for (int d = 0; d < ND; ++d) r[d] = cmul(r[d], wd[d]);
+      }
+    } else {
+#pragma unroll
+      for (int s = 0; s < 8; ++s) x[s] = make_float2(0.f, 0.f);
     }
     // pass 1 (span 1): inputs l + 64 s, outputs to 8 l + s.  When the
     // support ends below element 128 (uniform) only x[0], x[1] can be
@@ -317,12 +372,13 @@ __global__ __launch_bounds__(64 * kWaves, 4) void k_time_w(TimeWArgs a) {
       const int j = ev.jlo + jj;
       const double t = (ev.start + (double)(j + 1) * ev.tstep) + jit;
       if (t >= ev.tmin && t <= ev.tmax)
-        lse_push(m, sum, epi_value(ev, knots, yy, hh, hs, norm * accr[s], norm * acci[s]));
+        lse_push(m, sum, epi_value(ev, knots, yy, hh, hs, norm * accr[s], norm * acci[s]) +
+                             time_log_weight(ev, t));
     }
   }
   lse_wave(m, sum);
   if (lane == 0) {
-    const double b = (1.0 / (ev.tmax - ev.tmin)) * ev.tdelta;
+    const double b = time_lse_scale(ev, ev.tdelta);
     a.out[r] = (m > -INFINITY) ? log(b * sum) + m : -INFINITY;
   }
 }
@@ -496,7 +552,7 @@ __global__ __launch_bounds__(64 * kWaves, 2) void k_cal_time_w(CalTimeArgs a) {
   // epilogue per curve: time-marginalised lnL (base.py:794-820 per curve)
   const double norm = 4.0 / ev.T;
   const double hs = rw.hscale, jit = rw.jitter;
-  const double b = (1.0 / (ev.tmax - ev.tmin)) * ev.tdelta;
+  const double b = time_lse_scale(ev, ev.tdelta);
 #pragma unroll
   for (int cc = 0; cc < kCalCurves; ++cc) {
     if (cc >= ncb) break;
@@ -521,7 +577,8 @@ __global__ __launch_bounds__(64 * kWaves, 2) void k_cal_time_w(CalTimeArgs a) {
         const double t = (ev.start + (double)(j + 1) * ev.tstep) + jit;
         if (t >= ev.tmin && t <= ev.tmax)
           lse_push(m, sum, epi_value(ev, knots, yy, hh, hs, norm * accr[cc][s],
-                                     norm * acci[cc][s]));
+                                     norm * acci[cc][s]) +
+                               time_log_weight(ev, t));
       }
     }
     lse_wave(m, sum);

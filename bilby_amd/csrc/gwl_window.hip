@@ -127,14 +127,14 @@ __global__ __launch_bounds__(256) void k_win_clear(void* a, int fp64, long long 
 // Split-K partials: C_z[M][N] = A[:, k in chunk z] . Bt[:, k in chunk z]^T
 // (float32), chunk z = blockIdx.z of kWinKc columns; M % 128 == 0,
 // N % 128 == 0, K % 32 == 0, row strides K.  Inside a chunk the MFMA
-// accumulators restart every kFlush k-tiles (256 columns, 128 MFMA steps)
+// accumulators restart every kFlush k-tiles (32 columns, 16 MFMA steps)
 // after adding into a float32 chunk total; the epilogue adds the chunk
 // totals in float64.  The chunks also multiply the workgroups of a short
 // row tile.
 // A workgroup of 4 waves computes a 128 x 128 tile, each wave 2 x 2 MFMA
 // 32 x 32 blocks (output layout: lane l, accumulator v -> row
 // 8 (v / 4) + 4 (l / 32) + v % 4, column l % 32).
-constexpr int kBM = 128, kBN = 128, kBK = 32, kFlush = 8;
+constexpr int kBM = 128, kBN = 128, kBK = 32, kFlush = 1;
 
 __global__ __launch_bounds__(256) void k_win_gemm_f32(const float* __restrict__ A,
                                                       const float* __restrict__ Bt,
@@ -270,11 +270,11 @@ __global__ __launch_bounds__(64 * kWinEpiRows) void k_win_epi(WinEpiArgs a) {
         xi += (double)crow[a.L + jj];
       }
     }
-    lse_push(m, s, epi_value(ev, knots, yy, hh, hs, xr, xi));
+    lse_push(m, s, epi_value(ev, knots, yy, hh, hs, xr, xi) + time_log_weight(ev, t));
   }
   lse_wave(m, s);
   if (lane == 0) {
-    const double b = a.delta_tc / (ev.tmax - ev.tmin);
+    const double b = time_lse_scale(ev, a.delta_tc);
     a.out[r] = (m > -INFINITY) ? log(b * s) + m : -INFINITY;
   }
 }

@@ -18,7 +18,7 @@ from . import _lib
 from . import abi as _abi
 from ._lib import LIB, check, dptr, iptr
 from .detector import InterferometerList
-from .prior import Uniform
+from .prior import Cosmological, Uniform
 from .waveform_generator import taylorf2_flags
 
 _CTX = {}
@@ -191,6 +191,56 @@ def distance_interpolant(device, distance_array, prior_array, ref_dist, phase, l
     return table, (tx, ty, c, bbox)
 
 
+def _is_uniform(prior):
+    return isinstance(prior, Uniform) or type(prior).__name__ == "Uniform"
+
+
+def time_prior_table(prior, start_time, duration, delta_tc, points_per_step=64):
+    """The geocent_time prior as the kernels weigh it (base.py:794-820:
+    ``prior.prob(times) * delta_tc`` at the jittered times inside
+    [prior.minimum, prior.maximum]).  A Uniform prior: (minimum, maximum,
+    None), its constant weight applied on the device.  Any other prior
+    (bilby's Prior API: ``minimum``, ``maximum``, ``prob``): the support
+    clipped to the segment and to where the density is non-zero, and the
+    density on points_per_step points per delta_tc with nodes on both ends
+    (so no interpolation crosses the support's edges), interpolated
+    linearly on the device: (lo, hi, (t0, dt, density))."""
+    if _is_uniform(prior):
+        return float(prior.minimum), float(prior.maximum), None
+    lo = max(float(prior.minimum), float(start_time))
+    hi = min(float(prior.maximum), float(start_time) + float(duration))
+    if not hi > lo:
+        raise ValueError("the geocent_time prior has no support inside the segment")
+    probe = np.linspace(lo, hi, max(3, int(np.ceil((hi - lo) / delta_tc)) * 4 + 1))
+    nz = np.nonzero(np.asarray(prior.prob(probe), dtype=float) > 0)[0]
+    if len(nz) == 0:
+        raise ValueError("the geocent_time prior density is zero on the whole segment")
+    # an end where the density vanishes (a tail, an end outside the segment)
+    # moves in to the last probe point before its non-zero part; an end where
+    # it does not (a hard edge) stays a table node
+    if nz[0] > 0:
+        lo = float(probe[nz[0] - 1])
+    if nz[-1] < len(probe) - 1:
+        hi = float(probe[nz[-1] + 1])
+    return lo, hi, density_table(prior, lo, hi, delta_tc, points_per_step)
+
+
+def density_table(prior, lo, hi, delta_tc, points_per_step=64):
+    """The prior density on nodes lo + k dt, k < n, for the device's linear
+    interpolation: dt the power of two at or below delta_tc /
+    points_per_step, so that every node is an exact float64 (GPS times
+    carry 2^-22 s steps; a node rounded to them would put each density
+    value up to 1.2e-7 s off its time); the last node at or after hi, with
+    the density there taken at hi (the times the kernels weigh stay inside
+    [lo, hi], so no interpolation reaches past a hard upper edge).  Returns
+    (lo, dt, density)."""
+    dt = 2.0 ** np.floor(np.log2(delta_tc / points_per_step))
+    n = int(np.ceil((hi - lo) / dt)) + 1
+    grid = lo + dt * np.arange(n)
+    dens = np.asarray(prior.prob(np.minimum(grid, hi)), dtype=float)
+    return float(lo), float(dt), np.ascontiguousarray(np.nan_to_num(dens), dtype=np.float64)
+
+
 class GravitationalWaveTransient:
     """bilby/gw/likelihood/base.py:150-229 constructor semantics.
 
@@ -281,7 +331,11 @@ class GravitationalWaveTransient:
 
     # ------------------------------------------------------------------ setup
     def _check_marginalized_prior_is_set(self, key):
-        """base.py:356-390 (fixed priors rejected; missing time prior -> the segment)."""
+        """base.py:356-390: fixed priors are rejected; a missing time prior is
+        Uniform over the segment; a missing distance prior comes from a
+        Cosmological prior on redshift or comoving distance (which is
+        removed); a missing phase prior is the BBH default, Uniform over
+        [0, 2 pi] with periodic boundary."""
         if key in self.priors and getattr(self.priors[key], "is_fixed", False):
             raise ValueError("Cannot use marginalized likelihood for {}: prior is fixed".format(key))
         if key not in self.priors or isinstance(self.priors[key], (int, float)):
@@ -289,6 +343,22 @@ class GravitationalWaveTransient:
                 self.priors[key] = Uniform(self.interferometers.start_time,
                                            self.interferometers.start_time
                                            + self.interferometers.duration)
+            elif key == "luminosity_distance":
+                for other in ("redshift", "comoving_distance"):
+                    if other in self.priors:
+                        if not isinstance(self.priors[other], Cosmological):
+                            raise TypeError(
+                                "To marginalize over {}, the prior must be specified as a "
+                                "subclass of bilby.gw.prior.Cosmological.".format(other))
+                        self.priors["luminosity_distance"] = \
+                            self.priors[other].get_corresponding_prior("luminosity_distance")
+                        del self.priors[other]
+                        break
+                else:
+                    raise ValueError("a prior for luminosity_distance (or a Cosmological prior "
+                                     "on redshift or comoving_distance) is required")
+            elif key == "phase":
+                self.priors[key] = Uniform(0.0, 2 * np.pi, name="phase", boundary="periodic")
             else:
                 raise ValueError("a prior for {} is required".format(key))
 
@@ -350,13 +420,12 @@ class GravitationalWaveTransient:
         desc.wf_maximum_frequency = float(wfa.get("maximum_frequency", freq[-1]))
         desc.reference_frequency = float(wfa.get("reference_frequency", 50.0))
         desc.flags = self._flags() if flags is None else int(flags)
+        table = None
         if desc.flags & _lib.TIME_MARG:
             if time_prior is None:
-                tp = self.priors["geocent_time"]
-                if not isinstance(tp, Uniform):
-                    raise NotImplementedError("time marginalisation supports a Uniform "
-                                              "geocent_time prior")
-                time_prior = (tp.minimum, tp.maximum)
+                lo, hi, table = time_prior_table(self.priors["geocent_time"], ifos.start_time,
+                                                 ifos.duration, 2.0 / ifos.sampling_frequency)
+                time_prior = (lo, hi)
             desc.time_prior_minimum = float(time_prior[0])
             desc.time_prior_maximum = float(time_prior[1])
         desc.reference_distance = float(self._ref_dist)
@@ -364,6 +433,10 @@ class GravitationalWaveTransient:
         nbytes = ctypes.c_int64()
         check(LIB.gwl_event_create(ctx, ctypes.byref(desc), ctypes.byref(ev), ctypes.byref(nbytes)),
               "gwl_event_create")
+        if table is not None:
+            t0, dt, dens = table
+            check(LIB.gwl_event_set_time_prior(ev, len(dens), t0, dt, dptr(dens)),
+                  "gwl_event_set_time_prior")
         return ev, int(nbytes.value)
 
     def _setup_distance_marginalization(self, lookup_table=None):
@@ -697,7 +770,7 @@ class GravitationalWaveTransient:
                     time_marginalization=self.time_marginalization,
                     phase_marginalization=self.phase_marginalization,
                     distance_marginalization=self.distance_marginalization,
-                    calibration_marginalization=False,
+                    calibration_marginalization=self.calibration_marginalization,
                     waveform_arguments=self.waveform_generator.waveform_arguments,
                     time_reference=self.time_reference,
                     reference_frame=(self.reference_frame if isinstance(self.reference_frame, str)
@@ -763,6 +836,23 @@ class GravitationalWaveTransient:
                            ctypes.c_void_p(aux_t.data_ptr()) if aux else None,
                            ctypes.c_void_p(st)), "gwl_eval")
         return out, aux_t
+
+    def set_precision_escalation(self, coeff=2e-7, tol=8e-4):
+        """Float32 evaluations without time marginalisation re-evaluate in
+        float64, inside the same gwl_eval call, every row whose float32 error
+        bound ``coeff * Q`` exceeds ``tol`` (Q: the row's inner-product
+        scale, include/gwl.h gwl_event_set_escalation); ``coeff = 0`` turns
+        it off.  No counterpart in the reference (float64 throughout)."""
+        check(LIB.gwl_event_set_escalation(self._event, float(coeff), float(tol)),
+              "gwl_event_set_escalation")
+
+    def escalated_rows(self):
+        """Rows the last escalating float32 evaluation re-evaluated in
+        float64 (synchronises the device)."""
+        v = ctypes.c_int64()
+        check(LIB.gwl_event_escalated_rows(self._event, ctypes.byref(v)),
+              "gwl_event_escalated_rows")
+        return int(v.value)
 
     def _mode(self, precision):
         """Precision code of one call: the constructor's choice by default;
@@ -846,7 +936,7 @@ class GravitationalWaveTransient:
                 "calibration_marginalization={}, priors={})".format(
                     [i.name for i in self.interferometers], self.waveform_generator,
                     self.time_marginalization, self.distance_marginalization,
-                    self.phase_marginalization, False, self.priors))
+                    self.phase_marginalization, self.calibration_marginalization, self.priors))
 
 
 def inject_signal(interferometers, waveform_generator, parameters, device=0):

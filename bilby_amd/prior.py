@@ -58,6 +58,34 @@ class Gaussian(Prior):
         return np.exp(-(self.mu - val) ** 2 / (2 * self.sigma ** 2)) / (2 * np.pi) ** 0.5 / self.sigma
 
 
+class TruncatedGaussian(Prior):
+    """bilby/core/prior/analytical.py TruncatedGaussian: a Gaussian density
+    restricted to [minimum, maximum] and renormalised."""
+
+    def __init__(self, mu, sigma, minimum, maximum, name=None, boundary=None, **kw):
+        super().__init__(minimum=minimum, maximum=maximum, name=name, boundary=boundary)
+        self.mu = mu
+        self.sigma = sigma
+
+    @property
+    def normalisation(self):
+        from scipy.special import erf
+        return (erf((self.maximum - self.mu) / 2 ** 0.5 / self.sigma)
+                - erf((self.minimum - self.mu) / 2 ** 0.5 / self.sigma)) / 2
+
+    def rescale(self, val):
+        from scipy.special import erf, erfinv
+        lo = erf((self.minimum - self.mu) / 2 ** 0.5 / self.sigma)
+        return self.mu + erfinv(2 * np.asarray(val, float) * self.normalisation + lo) \
+            * 2 ** 0.5 * self.sigma
+
+    def prob(self, val):
+        val = np.asarray(val, float)
+        inside = (val >= self.minimum) & (val <= self.maximum)
+        return (np.exp(-(self.mu - val) ** 2 / (2 * self.sigma ** 2)) / (2 * np.pi) ** 0.5
+                / self.sigma / self.normalisation * inside)
+
+
 class DeltaFunction(Prior):
     def __init__(self, peak, name=None, **kw):
         super().__init__(minimum=peak, maximum=peak, name=name)
@@ -173,21 +201,61 @@ class FlatLambdaCDM:
         return self.hubble_distance * self.comoving_distance(z) ** 2 / ez
 
 
-class UniformComovingVolume(Interped):
-    """bilby/gw/prior.py:310-326 (p(z) ~ dVc/dz on z in [0.99 z_min,
-    1.01 z_max], mapped to luminosity distance, prior.py:277-282)."""
+class Cosmological(Interped):
+    """bilby/gw/prior.py:97-300: a prior on redshift, comoving distance or
+    luminosity distance (``name``) defined through a redshift density p(z)
+    on [0.99 z_min, 1.01 z_max] (1000 points) and the cosmology's distance
+    relations, p(x) = p(z) / (dx/dz) (prior.py:277-290);
+    ``get_corresponding_prior`` gives the same prior on another of the three
+    (prior.py:224-249), as the likelihood's distance-marginalisation setup
+    uses it (base.py:370-380)."""
+
+    NAMES = ("redshift", "comoving_distance", "luminosity_distance")
 
     def __init__(self, minimum, maximum, name="luminosity_distance", cosmology=None, **kw):
-        if name != "luminosity_distance":
-            raise ValueError("only luminosity_distance is supported")
+        if name not in self.NAMES:
+            raise ValueError("Name {} not recognized.".format(name))
         self.cosmology = cosmology or FlatLambdaCDM()
-        zmin = self.cosmology.z_at_luminosity_distance(minimum)
-        zmax = self.cosmology.z_at_luminosity_distance(maximum)
-        zs = np.linspace(zmin * 0.99, zmax * 1.01, 1000)
-        p_dz = self.cosmology.differential_comoving_volume(zs)
-        dl = self.cosmology.luminosity_distance(zs)
-        p_dl = p_dz / np.gradient(dl, zs)
-        super().__init__(dl, p_dl, minimum=minimum, maximum=maximum, name=name)
+        self._zmin, self._zmax = self._to_redshift(minimum, name), self._to_redshift(maximum, name)
+        zs = np.linspace(self._zmin * 0.99, self._zmax * 1.01, 1000)
+        p_dz = self._redshift_density(zs)
+        if name == "redshift":
+            xx, yy = zs, p_dz
+        else:
+            xx = (self.cosmology.comoving_distance(zs) if name == "comoving_distance"
+                  else self.cosmology.luminosity_distance(zs))
+            yy = p_dz / np.gradient(xx, zs)
+        super().__init__(xx, yy, minimum=minimum, maximum=maximum, name=name)
+
+    def _to_redshift(self, value, name):
+        c = self.cosmology
+        if name == "redshift" or value == 0:
+            return float(value)
+        if name == "luminosity_distance":
+            return float(c.z_at_luminosity_distance(value))
+        return float(np.interp(value, c._dc, c._z))
+
+    def _from_redshift(self, z, name):
+        if name == "redshift":
+            return float(z)
+        if name == "comoving_distance":
+            return float(self.cosmology.comoving_distance(z))
+        return float(self.cosmology.luminosity_distance(z))
+
+    def _redshift_density(self, zs):
+        raise NotImplementedError
+
+    def get_corresponding_prior(self, name=None, unit=None):
+        return type(self)(self._from_redshift(self._zmin, name),
+                          self._from_redshift(self._zmax, name), name=name,
+                          cosmology=self.cosmology)
+
+
+class UniformComovingVolume(Cosmological):
+    """bilby/gw/prior.py:310-326: p(z) ~ dVc/dz."""
+
+    def _redshift_density(self, zs):
+        return self.cosmology.differential_comoving_volume(zs)
 
 
 class Constraint(Prior):

@@ -17,8 +17,8 @@ from . import _lib
 from . import abi as _abi
 from ._lib import LIB, check, dptr
 from .detector import InterferometerList
-from .likelihood import (GravitationalWaveTransient, _batch_rows, distance_interpolant,
-                         get_context, grid_time_posterior)
+from .likelihood import (GravitationalWaveTransient, _batch_rows, _is_uniform, density_table,
+                         distance_interpolant, get_context, grid_time_posterior)
 from .prior import Uniform
 from .waveform_generator import taylorf2_flags
 
@@ -104,8 +104,9 @@ class ROQGravitationalWaveTransient:
             if delta_tc is None:
                 delta_tc = ts[1] - ts[0]
             tp = self.priors["geocent_time"]
-            if not isinstance(tp, Uniform):
-                raise NotImplementedError("time marginalisation supports a Uniform geocent_time prior")
+            if not (np.isfinite(tp.minimum) and np.isfinite(tp.maximum)):
+                raise ValueError("ROQ time marginalisation needs a geocent_time prior with finite "
+                                 "bounds (its grid spans them, roq.py:320-331)")
             n = int(np.ceil((tp.maximum - tp.minimum) / delta_tc))
             self._delta_tc = (tp.maximum - tp.minimum) / n
             self._times = tp.minimum + self._delta_tc / 2. + np.arange(n) * self._delta_tc
@@ -249,6 +250,12 @@ class ROQGravitationalWaveTransient:
         check(LIB.gwl_roq_create(ctx, ctypes.byref(d), ctypes.byref(h), ctypes.byref(nbytes)),
               "gwl_roq_create")
         self._roq = h
+        if self.time_marginalization and not _is_uniform(self.priors["geocent_time"]):
+            # the prior's own support bounds the grid; its density weighs each time
+            tp = self.priors["geocent_time"]
+            t0, dt, dens = density_table(tp, float(tp.minimum), float(tp.maximum), self._delta_tc)
+            check(LIB.gwl_roq_set_time_prior(self._roq, len(dens), t0, dt, dptr(dens)),
+                  "gwl_roq_set_time_prior")
         self.hbm_bytes = int(nbytes.value)
         if self.distance_marginalization:
             table, (tx, ty, c, bbox) = distance_interpolant(

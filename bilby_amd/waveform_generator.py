@@ -54,39 +54,185 @@ def lal_binary_black_hole(frequency_array, mass_1, mass_2, luminosity_distance, 
 lal_binary_black_hole._gwl_approximant = "lal"
 
 
-def generate_component_masses(p):
-    """bilby/gw/conversion.py generate_component_masses for chirp_mass /
-    mass_ratio and total_mass / mass_ratio inputs."""
-    if "mass_1" in p and "mass_2" in p:
-        return p
-    q = p.get("mass_ratio")
-    if "chirp_mass" in p and q is not None:
-        mc = np.asarray(p["chirp_mass"], float)
-        q = np.asarray(q, float)
-        total = mc * (1 + q) ** 1.2 / q ** 0.6
-        p["mass_1"] = total / (1 + q)
-        p["mass_2"] = p["mass_1"] * q
-    elif "total_mass" in p and q is not None:
-        q = np.asarray(q, float)
-        p["mass_1"] = np.asarray(p["total_mass"], float) / (1 + q)
-        p["mass_2"] = p["mass_1"] * q
-    return p
+def _mass_ratio_from_component_and_chirp_mass(m, mc):
+    """The other mass over m, x, from a component mass m and the chirp mass:
+    the positive root of x^3 - a x - a = 0, a = (mc / m)^5 (from mc =
+    m x^(3/5) / (1 + x)^(1/5)); there is exactly one (Descartes).  Cardano's
+    formula for a < 27/4, where the cubic has one real root; the
+    trigonometric form of its largest root above, where it has three (x > 3,
+    reached from the lighter mass of a pair with q < 1/3, where bilby's
+    closed form, conversion.py:1072-1095, returns NaN)."""
+    a = np.asarray((np.asarray(mc, float) / np.asarray(m, float)) ** 5)
+    with np.errstate(invalid="ignore"):
+        disc = np.sqrt(np.maximum(a * a / 4 - a ** 3 / 27, 0.0))
+        cardano = np.cbrt(a / 2 + disc) + np.cbrt(a / 2 - disc)
+        trig = 2 * np.sqrt(a / 3) * np.cos(np.arccos(np.minimum(1.5 * np.sqrt(3 / a), 1.0)) / 3)
+    out = np.where(a < 6.75, cardano, trig)
+    return out[()] if out.ndim == 0 else out
+
+
+def _mass_ratio_from_symmetric(eta):
+    """q <= 1 with q / (1 + q)^2 = eta."""
+    t = 1 / (2 * np.asarray(eta, float)) - 1
+    return t - np.sqrt(t * t - 1)
+
+
+def generate_component_masses(sample, require_add=False, source=False):
+    """bilby/gw/conversion.py:1826-1975: add mass_1 and mass_2 (or the
+    ``_source`` pair) from whichever complete set of mass parameters the
+    sample holds (a component mass with total mass, mass ratio, symmetric
+    mass ratio or chirp mass; total or chirp mass with mass ratio or
+    symmetric mass ratio; chirp and total mass).  Incomplete sets add
+    nothing, or raise KeyError with ``require_add``."""
+    out = dict(sample)
+    sfx = "_source" if source else ""
+    m1k, m2k, mtk, mck = "mass_1" + sfx, "mass_2" + sfx, "total_mass" + sfx, "chirp_mass" + sfx
+
+    def incomplete():
+        if require_add:
+            raise KeyError("Insufficient mass parameters in input dictionary")
+        return dict(sample)
+
+    if m1k in out or m2k in out:
+        known, other = (m1k, m2k) if m1k in out else (m2k, m1k)
+        if other in out:
+            return out
+        if mtk in out:
+            out[other] = np.asarray(out[mtk], float) - np.asarray(out[known], float)
+            return out
+        if "mass_ratio" in out:
+            q = np.asarray(out["mass_ratio"], float)
+        elif "symmetric_mass_ratio" in out:
+            q = out["mass_ratio"] = _mass_ratio_from_symmetric(out["symmetric_mass_ratio"])
+        elif mck in out:
+            q = _mass_ratio_from_component_and_chirp_mass(out[known], out[mck])
+            if known == m2k:   # the lighter mass gives 1 / q
+                q = 1 / q
+            out["mass_ratio"] = q
+        else:
+            return incomplete()
+        m = np.asarray(out[known], float)
+        out[other] = m * q if known == m1k else m / q
+        return out
+    if mtk in out:
+        if "mass_ratio" not in out:
+            if "symmetric_mass_ratio" in out:
+                out["mass_ratio"] = _mass_ratio_from_symmetric(out["symmetric_mass_ratio"])
+            elif mck in out:
+                out["symmetric_mass_ratio"] = (np.asarray(out[mck], float)
+                                               / np.asarray(out[mtk], float)) ** (5 / 3)
+                out["mass_ratio"] = _mass_ratio_from_symmetric(out["symmetric_mass_ratio"])
+            else:
+                return incomplete()
+    elif mck in out:
+        if "mass_ratio" not in out:
+            if "symmetric_mass_ratio" in out:
+                out["mass_ratio"] = _mass_ratio_from_symmetric(out["symmetric_mass_ratio"])
+            else:
+                return incomplete()
+        q = np.asarray(out["mass_ratio"], float)
+        with np.errstate(invalid="ignore", divide="ignore"):
+            out[mtk] = np.asarray(out[mck], float) * (1 + q) ** 1.2 / q ** 0.6
+    else:
+        return incomplete()
+    q = np.asarray(out["mass_ratio"], float)
+    out[m1k] = np.asarray(out[mtk], float) / (1 + q)
+    out[m2k] = out[m1k] * q
+    return out
+
+
+_COSMOLOGY = None
+
+
+def _cosmology():
+    global _COSMOLOGY
+    if _COSMOLOGY is None:
+        from .prior import FlatLambdaCDM
+        _COSMOLOGY = FlatLambdaCDM()
+    return _COSMOLOGY
+
+
+def redshift_to_luminosity_distance(redshift):
+    """bilby/gw/conversion.py redshift_to_luminosity_distance, with this
+    package's flat Lambda-CDM cosmology (H0 = 67.74, Om0 = 0.3075) in place
+    of astropy's Planck15."""
+    return _cosmology().luminosity_distance(redshift)
+
+
+def comoving_distance_to_luminosity_distance(comoving_distance):
+    c = _cosmology()
+    z = np.interp(comoving_distance, c._dc, c._z)
+    return (1 + z) * np.asarray(comoving_distance, float)
+
+
+def luminosity_distance_to_redshift(distance):
+    return _cosmology().z_at_luminosity_distance(distance)
 
 
 def convert_to_lal_binary_black_hole_parameters(parameters):
-    """The parts of bilby/gw/conversion.py:182-283 that apply to aligned
-    spins: component masses, chi_i from a_i cos(tilt_i), cos_theta_jn."""
+    """bilby/gw/conversion.py:182-283 for the aligned-spin TaylorF2 the
+    device evaluates: luminosity distance from redshift or comoving distance;
+    ``*_source`` masses times (1 + z); component masses from any complete
+    mass set; spins from chi_i (a_i = |chi_i|, cos tilt_i = sign chi_i, or
+    chi_i / a_i when a_i is given), from a_i and tilt_i / cos_tilt_i, and
+    chi_i_in_plane; cos_theta_jn -> theta_jn; delta_phase -> phase =
+    (delta_phase - sign(cos theta_jn) psi) mod 2 pi.  Adds chi_1, chi_2 (the
+    aligned components the kernels read).  Returns the converted dict (bilby
+    also returns the list of added keys).
+
+    TaylorF2 is a non-precessing approximant: lalsimulation refuses non-zero
+    in-plane spins for it (XLAL_EINVAL, which bilby re-raises whatever
+    catch_waveform_errors says, source.py:644-662), and so does this
+    conversion (ValueError)."""
     p = dict(parameters)
-    generate_component_masses(p)
+    original = list(p)
+    if "luminosity_distance" not in original:
+        if "redshift" in p:
+            p["luminosity_distance"] = redshift_to_luminosity_distance(p["redshift"])
+        elif "comoving_distance" in p:
+            p["luminosity_distance"] = comoving_distance_to_luminosity_distance(
+                p["comoving_distance"])
+    for key in original:
+        if key.endswith("_source"):
+            if "redshift" not in p:
+                p["redshift"] = luminosity_distance_to_redshift(p["luminosity_distance"])
+            p[key[:-7]] = np.asarray(p[key], float) * (1 + np.asarray(p["redshift"], float))
+    p = generate_component_masses(p, require_add=False)
     for idx in ("1", "2"):
-        if "chi_" + idx not in p:
-            a = p.get("a_" + idx, 0.0)
-            if "cos_tilt_" + idx in p:
-                p["chi_" + idx] = np.asarray(a) * np.asarray(p["cos_tilt_" + idx])
+        chi, a, ct = "chi_" + idx, "a_" + idx, "cos_tilt_" + idx
+        if chi in original:
+            inplane = p.get(chi + "_in_plane", 0.0)
+            if np.any(np.asarray(inplane, float) != 0):
+                raise ValueError("TaylorF2 takes aligned spins: %s_in_plane must be zero" % chi)
+            if a not in original:
+                p[a] = np.abs(np.asarray(p[chi], float))
+                p[ct] = np.sign(np.asarray(p[chi], float))
             else:
-                p["chi_" + idx] = np.asarray(a) * np.cos(p.get("tilt_" + idx, 0.0))
-    if "theta_jn" not in p and "cos_theta_jn" in p:
-        p["theta_jn"] = np.arccos(p["cos_theta_jn"])
+                with np.errstate(invalid="ignore", divide="ignore"):
+                    cos_tilt = np.asarray(p[chi], float) / np.asarray(p[a], float)
+                p[ct] = np.where(np.isfinite(cos_tilt), cos_tilt, 1.0)
+    for key in ("phi_jl", "phi_12"):
+        p.setdefault(key, 0.0)
+    for angle in ("tilt_1", "tilt_2", "theta_jn"):
+        if "cos_" + angle in p:
+            with np.errstate(invalid="ignore"):
+                p[angle] = np.arccos(p["cos_" + angle])
+    if "delta_phase" in p:
+        with np.errstate(invalid="ignore"):
+            p["phase"] = np.mod(np.asarray(p["delta_phase"], float)
+                                - np.sign(np.cos(p["theta_jn"])) * np.asarray(p["psi"], float),
+                                2 * np.pi)
+    for idx in ("1", "2"):
+        chi = "chi_" + idx
+        if chi not in original:
+            amp = np.asarray(p.get("a_" + idx, 0.0), float)
+            tilt = np.asarray(p.get("tilt_" + idx, 0.0), float)
+            # bilby_to_lalsimulation_spins (conversion.py:106-166): the aligned
+            # case needs a_i = 0 or tilt_i in {0, pi}; anything else has
+            # in-plane spin components
+            if np.any(np.abs(amp * np.sin(tilt)) > 1e-12 * np.maximum(amp, 1.0)):
+                raise ValueError("TaylorF2 takes aligned spins: tilt_%s must be 0 or pi" % idx)
+            p[chi] = amp * np.cos(tilt)
     return p
 
 

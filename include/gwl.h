@@ -120,6 +120,15 @@ int gwl_event_destroy(gwl_event* ev);
 int gwl_event_set_distance_spline(gwl_event* ev, int32_t nx, int32_t ny, const double* tx,
                                   const double* ty, const double* coeffs, const double* bbox);
 
+/* Time prior other than Uniform (base.py:794-820 weighs each time by
+ * priors['geocent_time'].prob(times) * delta_tc; base.py:362-369): its
+ * density at n >= 2 points t0 + k dt, interpolated linearly at each
+ * (jittered) time; the time_prior_minimum / maximum of the descriptor bound
+ * the window as before.  Replaces the Uniform weight delta_tc / (max - min).
+ * The event must marginalise over time. */
+int gwl_event_set_time_prior(gwl_event* ev, int64_t n, double t0, double dt,
+                             const double* density);
+
 /* Pre-size the event's device workspace for batches up to max_rows (so
  * that gwl_eval allocates nothing afterwards). */
 int gwl_event_reserve(gwl_event* ev, int64_t max_rows);
@@ -164,6 +173,12 @@ int gwl_event_time_window(gwl_event* ev, int64_t* j_first, int32_t* n_window, do
  * counterpart in the reference. */
 int gwl_event_set_tile(gwl_event* ev, int32_t rows_per_workgroup, int32_t workgroups_per_cu);
 
+/* This event's tile inside multi-event launches: decimated sub-sequence
+ * steps per work item (0 = the default, ~32), read when a multi-event set
+ * (gwl_multi_create, gwl_server_add_group) is created.  Set by the
+ * autotuner; no counterpart in the reference. */
+int gwl_event_set_multi_steps(gwl_event* ev, int32_t steps);
+
 /* Transform of the time-marginalised TaylorF2 path: automatic (the
  * wave-per-row or tiled kernel as gwl_event_set_tile selects) or one of
  * them forced.  No counterpart in the reference (kernel choice). */
@@ -171,6 +186,22 @@ enum gwl_transform {
   GWL_TRANSFORM_AUTO = 0, GWL_TRANSFORM_TILED = 1, GWL_TRANSFORM_WAVE = 2
 };
 int gwl_event_set_transform(gwl_event* ev, int32_t transform);
+
+/* Per-row precision escalation of float32 evaluations without time
+ * marginalisation (no counterpart in the reference, which is float64
+ * throughout): within the same gwl_eval call and stream, every row whose
+ * float32 error bound coeff * Q exceeds tol is evaluated again in float64
+ * and its float64 result returned, so each row meets |dlnL| <= tol against
+ * float64.  Q is the row's inner-product scale, |<d|h>| + <h|h> (under
+ * distance marginalisation 2 dt^2 / ht + dt + ht of the table arguments
+ * dt = <d|h> d / d_ref, ht = <h|h> (d / d_ref)^2).  coeff = 0 turns it off;
+ * defaults coeff = 2e-7 (the largest float32 |dlnL| / Q measured over 10^4
+ * sine-Gaussian rows with |lnL| up to 5e5 is 1.4e-7), tol = 8e-4.  The workspace holds the selected rows
+ * (gwl_event_reserve sizes it), so the call still allocates nothing. */
+int gwl_event_set_escalation(gwl_event* ev, double coeff, double tol);
+/* Rows escalated by the last escalating gwl_eval on this event (waits for
+ * the device; for measurement). */
+int gwl_event_escalated_rows(gwl_event* ev, int64_t* rows);
 
 /* Kernel timing for measurement: while enabled, every gwl_eval on this event
  * brackets its dominant kernel (the fused inner-product or time-marginalised
@@ -313,6 +344,9 @@ int gwl_roq_destroy(gwl_roq* roq);
 /* Distance interpolant, as gwl_event_set_distance_spline. */
 int gwl_roq_set_distance_spline(gwl_roq* roq, int32_t nx, int32_t ny, const double* tx,
                                 const double* ty, const double* coeffs, const double* bbox);
+/* Time prior other than Uniform, as gwl_event_set_time_prior (roq.py uses
+ * the same time_marginalized_likelihood weights, base.py:794-820). */
+int gwl_roq_set_time_prior(gwl_roq* roq, int64_t n, double t0, double dt, const double* density);
 /* Workspace for up to max_rows rows in every mode (as gwl_event_reserve). */
 int gwl_roq_reserve(gwl_roq* roq, int64_t max_rows);
 /* Batched ROQ log-likelihood ratio (roq.py:463-465 -> base.py:419-446):

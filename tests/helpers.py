@@ -7,7 +7,10 @@ from oracle.likelihood import OracleDetector, OracleEvent, OracleGWTransient
 _TABLES = {}
 
 
-def oracle_from_setup(setup, **overrides):
+def oracle_from_setup(setup, distance_table=None, **overrides):
+    """The FP64 oracle of a configs.make_* setup.  distance_table: an
+    oracle-built lookup table for the same distance prior (the table depends
+    only on the prior and the phase flag), to skip rebuilding it."""
     ifos = setup["interferometers"]
     dets = [OracleDetector(ifo.name, ifo.frequency_array, ifo.frequency_domain_strain,
                            ifo.power_spectral_density_array, ifo.frequency_mask)
@@ -20,13 +23,13 @@ def oracle_from_setup(setup, **overrides):
               phase_marginalization=setup["phase_marginalization"])
     kw.update(overrides)
     priors = dict(setup["priors"])
-    table = None
-    if kw["distance_marginalization"]:
+    table = distance_table
+    if kw["distance_marginalization"] and table is None:
         key = (kw["phase_marginalization"], id(setup["priors"]["luminosity_distance"]))
         table = _TABLES.get(key)
     lk = OracleGWTransient(ev, source, waveform_arguments=dict(wfg.waveform_arguments),
                            priors=priors, distance_table=table, **kw)
-    if kw["distance_marginalization"]:
+    if kw["distance_marginalization"] and distance_table is None:
         _TABLES[key] = lk._dist_margd_loglikelihood_array
     return lk, priors
 

@@ -47,6 +47,9 @@ def test_calibration_marginalised_matches_oracle(flags, tmp_path):
         priors=priors, number_of_response_curves=100, calibration_lookup_table=files,
         precision="fp64")
     assert "recalib_index" in lk.marginalized_parameters
+    # base.py:1170, 234-236: meta_data and repr report the calibration flag
+    assert lk.meta_data["calibration_marginalization"] is True
+    assert "calibration_marginalization=True" in repr(lk)
     assert all(float(priors[k].peak) == 0.0 for k in priors if "recalib" in k and "frequency" not in k)
     ifos = s["interferometers"]
     dets = [OracleDetector(i.name, i.frequency_array, i.frequency_domain_strain,

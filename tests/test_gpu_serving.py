@@ -123,3 +123,85 @@ def test_group_and_roq_units_match_direct_calls():
         assert r == sum(len(x[1]) for x in futs)
     finally:
         server.close()
+
+
+def test_serving_mix_of_the_three_unit_kinds_alone_equals_mix_and_oracle():
+    """A short serving load over the unit kinds of the serving workload: a
+    TaylorF2 T+D+P event, a multi-event group of uneven aligned-spin events
+    and a ROQ likelihood, 8 samplers at 400 requests/s for 1.5 s with 1-256
+    rows per request.  Probe requests give the same values alone and in the
+    mix (1e-8: the group and the single-event paths may split a row's
+    transform into different numbers of float64 partial sums) and the direct
+    batched call's; their first rows match the FP64 oracle within 1e-3."""
+    from bilby_amd.configs import make_roq, make_taylorf2_marginalised, make_uneven_events
+    from bilby_amd.likelihood import GravitationalWaveTransient
+    from bilby_amd.roq import ROQGravitationalWaveTransient
+    from bilby_amd.serving import LikelihoodServer, run_load
+    from oracle.roq import OracleROQ
+    from tests.helpers import draws_as_dicts, fixed_marginalised, oracle_from_setup
+    say("serving mix: event, group, ROQ")
+    c2 = make_taylorf2_marginalised(n_draws=256, seed=41, duration=4.0, sampling_frequency=2048.0,
+                                    detectors=("H1", "L1", "V1"))
+    p2 = c2["priors"].copy()
+    lk2 = GravitationalWaveTransient(c2["interferometers"], c2["waveform_generator"],
+                                     time_marginalization=True, distance_marginalization=True,
+                                     phase_marginalization=True, priors=p2)
+    d2 = fixed_marginalised(c2["draws"], p2)
+    events = make_uneven_events(n_events=3, seed=42, rows_range=(40, 80), durations=(4, 8))
+    names, lks, draws3, setups3 = [], [], [], []
+    for k, ev in enumerate(events):
+        pri = ev["priors"].copy()
+        lks.append(GravitationalWaveTransient(ev["interferometers"], ev["waveform_generator"],
+                                              time_marginalization=True,
+                                              distance_marginalization=True,
+                                              phase_marginalization=True, priors=pri))
+        names.append("c3_%d" % k)
+        draws3.append(fixed_marginalised(ev["draws"], pri))
+        setups3.append(ev)
+    s4 = make_roq(n_draws=64, seed=43)
+    p4 = s4["priors"].copy()
+    roq = ROQGravitationalWaveTransient(s4["interferometers"], s4["waveform_generator"], p4,
+                                        weights=s4["weights"], time_marginalization=True,
+                                        phase_marginalization=True, delta_tc=s4["delta_tc"])
+    d4 = fixed_marginalised(s4["draws"], p4)
+    units = {"c2": (lk2, d2), "c4": (roq, d4)}
+    units.update({n: (lk, d) for n, lk, d in zip(names, lks, draws3)})
+    rows = {n: lk.rows_from_parameters(d) for n, (lk, d) in units.items()}
+    server = LikelihoodServer(hbm_budget=50e9, max_batch_rows=512)
+    try:
+        server.register("c2", lk2)
+        server.register_multi(names, lks)
+        server.register("c4", roq)
+        probe = {n: r[:9] for n, r in rows.items()}
+        alone = {n: server.submit(n, r).result() for n, r in probe.items()}
+        mix = [(0.6, [("c2", rows["c2"])]), (0.3, [(n, rows[n]) for n in names]),
+               (0.1, [("c4", rows["c4"])])]
+        inmix = {}
+        th = threading.Thread(target=lambda: run_load(server, mix, n_samplers=8, rate=400.0,
+                                                      duration=1.5, rows_range=(1, 256), seed=5))
+        th.start()
+        for n, r in probe.items():
+            inmix[n] = server.submit(n, r).result()
+        th.join()
+        for n in probe:
+            direct = units[n][0].log_likelihood_ratio_batch(probe[n])
+            assert np.max(np.abs(inmix[n] - alone[n])) <= 1e-8, n
+            assert np.max(np.abs(direct - alone[n])) <= 1e-8, n
+        b, r = server.stats()
+        say("batches", b, "rows", r)
+        assert b > len(probe)
+    finally:
+        server.close()
+    # oracle spot checks: the first 3 rows of each unit kind
+    o2, _ = oracle_from_setup(c2)
+    o3, _ = oracle_from_setup(setups3[0])
+    w = s4["weights"]
+    o4 = OracleROQ(["H1", "L1"], s4["interferometers"].start_time, w, w["frequency_nodes_linear"],
+                   w["frequency_nodes_quadratic"], dict(s4["priors"]), time_marginalization=True,
+                   phase_marginalization=True, delta_tc=s4["delta_tc"],
+                   waveform_arguments=s4["waveform_generator"].waveform_arguments)
+    for name, olk, d in (("c2", o2, d2), (names[0], o3, draws3[0]), ("c4", o4, d4)):
+        ref = np.array([olk.log_likelihood_ratio(p) for p in draws_as_dicts(d, range(3))])
+        err = np.max(np.abs(inmix[name][:3] - ref))
+        say(name, "vs oracle", err)
+        assert err < 1e-3, (name, err)

@@ -485,3 +485,178 @@ def test_distance_table_equals_direct_integral():
             direct = logsumexp(terms, b=pr) - np.log(pr.sum())
             val = lk.distance_marginalized_likelihood(dh, hh, {"luminosity_distance": lk._ref_dist})
             assert abs(val - direct) < 2e-3, (phase, dh, hh, val, direct)
+
+
+def test_antenna_pattern_closed_form_at_the_pole():
+    """An L-shaped detector at the north pole with arms along x and y,
+    D = (x x - y y) / 2, has the classical closed-form response (Schutz &
+    Tinto 1987; Thorne 1987), here in bilby's polarisation convention
+    (geometry.py:118-186 uses psi_bilby = pi/2 - psi of those papers):
+      F+ = -(1 + cos^2 th)/2 cos 2ph cos 2psi - cos th sin 2ph sin 2psi,
+      Fx =  (1 + cos^2 th)/2 cos 2ph sin 2psi - cos th sin 2ph cos 2psi,
+    with th = pi/2 - dec and ph = ra - GMST.  A sign flip of Fx, swapped m
+    and n vectors or a transposed angle fails this."""
+    D = np.diag([0.5, -0.5, 0.0])
+    rng = np.random.default_rng(21)
+    for _ in range(50):
+        ra, dec, psi = rng.uniform(0, 2 * np.pi), np.arcsin(rng.uniform(-1, 1)), rng.uniform(0, np.pi)
+        t = 1e9 + rng.uniform(0, 1e8)
+        fp, fc = core.antenna_response(D, ra, dec, t, psi)
+        th = np.pi / 2 - dec
+        ph = ra - core.greenwich_mean_sidereal_time(t) % (2 * np.pi)
+        c2 = (1 + np.cos(th) ** 2) / 2
+        want_p = -c2 * np.cos(2 * ph) * np.cos(2 * psi) - np.cos(th) * np.sin(2 * ph) * np.sin(2 * psi)
+        want_c = c2 * np.cos(2 * ph) * np.sin(2 * psi) - np.cos(th) * np.sin(2 * ph) * np.cos(2 * psi)
+        assert abs(fp - want_p) < 1e-12 and abs(fc - want_c) < 1e-12
+    # overhead source: F+ + i Fx = -exp(2 i (ph - psi))
+    t = 1e9
+    g = core.greenwich_mean_sidereal_time(t) % (2 * np.pi)
+    fp, fc = core.antenna_response(D, g + 0.3, np.pi / 2, t, 0.1)
+    assert abs((fp + 1j * fc) + np.exp(2j * (0.3 - 0.1))) < 1e-12
+
+
+def test_detector_tensors_equal_lal_published_values():
+    """The H1, L1 and V1 detector tensors built from the arm geometry
+    (detector/geometry.py:250-269, detectors/*.interferometer) equal the
+    response tensors LAL publishes for these sites (LALDetectors.h,
+    LAL_LHO_4K / LAL_LLO_4K / LAL_VIRGO_DETECTOR_RESPONSE, stored in single
+    precision from the LIGO-T980044 / Virgo site data)."""
+    lal = {
+        "H1": [[-0.3926141, -0.0776130, -0.2473886],
+               [-0.0776130, 0.3195244, 0.2279981],
+               [-0.2473886, 0.2279981, 0.0730903]],
+        "L1": [[0.4112809, 0.1402097, 0.2472943],
+               [0.1402097, -0.1090056, -0.1816157],
+               [0.2472943, -0.1816157, -0.3022755]],
+        "V1": [[0.2438740, -0.0990838, -0.2325762],
+               [-0.0990838, -0.4478258, 0.1878331],
+               [-0.2325762, 0.1878331, 0.2039518]],
+    }
+    for name, want in lal.items():
+        tensor, _ = core.detector_geometry(name)
+        assert np.max(np.abs(tensor - np.array(want))) < 2e-6, name
+
+
+def test_time_delay_sign_along_the_vertex():
+    """A source in the direction of a detector's vertex reaches the detector
+    |r|/c before the geocentre (time_delay_from_geocenter = -|r|/c); from the
+    opposite direction, |r|/c after it."""
+    for name in ("H1", "L1", "V1"):
+        _, vertex = core.detector_geometry(name)
+        r = np.linalg.norm(vertex)
+        for t in (1e9, 1.3e9):
+            g = core.greenwich_mean_sidereal_time(t) % (2 * np.pi)
+            for sign in (1.0, -1.0):
+                x, y, z = sign * vertex / r
+                ra = np.arctan2(y, x) + g
+                dec = np.pi / 2 - np.arccos(z)
+                dt = core.time_delay_from_geocenter(vertex, ra, dec, t)
+                assert abs(dt + sign * r / core.SPEED_OF_LIGHT) < 1e-15, (name, sign, dt)
+
+
+def _series_mul(a, b):
+    n = len(a)
+    out = np.zeros(n)
+    for i in range(n):
+        out[i:] += a[i] * b[:n - i]
+    return out
+
+
+def _series_pow(a, p):
+    """a(v)^p for a[0] = 1 (binomial series in u = a - 1)."""
+    n = len(a)
+    u = np.array(a, float)
+    u[0] = 0.0
+    out = np.zeros(n)
+    out[0] = 1.0
+    term = np.zeros(n)
+    term[0] = 1.0
+    coef = 1.0
+    for k in range(1, n):
+        coef *= (p - k + 1) / k
+        term = _series_mul(term, u)
+        out += coef * term
+    return out
+
+
+def _kerr_binding_energy_in_x(q, n=9):
+    """Binding energy per unit mass of a test particle on a prograde circular
+    equatorial orbit of a Kerr black hole (Bardeen, Press & Teukolsky 1972),
+    E = (1 - 2 v^2 + q v^3) / sqrt(1 - 3 v^2 + 2 q v^3) - 1 with v = sqrt(M/r),
+    as a power series in x = (M Omega)^(1/3), M Omega = v^3 / (1 + q v^3)."""
+    num = np.zeros(n)
+    num[0], num[2], num[3] = 1.0, -2.0, q
+    den = np.zeros(n)
+    den[0], den[2], den[3] = 1.0, -3.0, 2 * q
+    e_v = _series_mul(num, _series_pow(den, -0.5))
+    e_v[0] -= 1.0
+    # v(x) = x (1 - q x^3)^(-1/3): compose e_v(v(x)) term by term
+    g = np.zeros(n)
+    g[0], g[3] = 1.0, -q
+    vx = np.concatenate([[0.0], _series_pow(g, -1.0 / 3.0)[:n - 1]])
+    out = np.zeros(n)
+    vk = np.zeros(n)
+    vk[0] = 1.0
+    for k in range(n):
+        out += e_v[k] * vk
+        vk = _series_mul(vk, vx)
+    return out
+
+
+def test_taylorf2_spin_spin_self_term_from_the_kerr_test_mass_limit():
+    """The 2PN quadratic-in-spin phase coefficient of the heavier body in the
+    test-mass limit (m2 -> 0, chi1 = q), derived from the exact Kerr
+    circular-orbit binding energy (above) and the test-particle Kerr flux of
+    Tagoshi, Shibata, Tanaka & Sasaki 1996 (Phys. Rev. D 54, 1439) to v^4,
+      F / F_N = 1 - 1247/336 v^2 + (4 pi - 11 q / 4) v^3
+                + (-44711/9072 + 33 q^2 / 16) v^4,
+    by the SPA re-expansion used for the other coefficients (psi_4 = 10 a_4):
+    -405/8 q^2, the eta -> 0 limit of the implementation's chi_s / chi_a
+    form.  The q^2 part is the second difference in q."""
+    def psi4(q):
+        e = _kerr_binding_energy_in_x(q)
+        # E = -(x^2 / 2) (1 + e_2 x^2 + e_3 x^3 + e_4 x^4): relative coefficients
+        rel = -2.0 * e[2:7]
+        de = np.array([(k + 2) / 2 * rel[k] for k in range(5)])
+        f = np.array([1.0, 0.0, -1247 / 336, 4 * np.pi - 11 * q / 4, -44711 / 9072 + 33 * q ** 2 / 16])
+        a = np.zeros(5)
+        for k in range(5):
+            a[k] = de[k] - sum(a[j] * f[k - j] for j in range(k))
+        return 10.0 * a[4]
+
+    h = 0.5
+    derived_q2 = (psi4(h) + psi4(-h) - 2 * psi4(0.0)) / (2 * h * h)
+    assert abs(derived_q2 + 405 / 8) < 1e-9, derived_q2
+    # the implementation's coefficient of chi1^2 as m2 / m1 -> 0
+    m1, m2 = 1.0, 1e-9
+
+    def impl(c1):
+        return core.taylorf2_phase_coefficients(m1, m2, c1, 0.0)[0][4]
+    impl_q2 = (impl(h) + impl(-h) - 2 * impl(0.0)) / (2 * h * h)
+    assert abs(impl_q2 - derived_q2) < 1e-6, (impl_q2, derived_q2)
+    # the 1.5PN spin-orbit term of the same limit: the Kerr energy's
+    # x^3 coefficient is (8/3) q, the flux's -11/4 q
+    e = _kerr_binding_energy_in_x(0.3)
+    assert abs(-2.0 * e[5] - 8 / 3 * 0.3) < 1e-12
+
+
+def test_taylorf2_spin_spin_equals_the_sigma_form():
+    """The 2PN spin-spin coefficient in the chi_s / chi_a form equals the
+    sigma form of Arun, Buonanno, Faye & Ochsner 2009 (arXiv:0810.5336
+    Eq. 6.24; Mikoczi, Vasuth & Gergely 2005 for the self-spin terms), which
+    lalsimulation's TaylorF2 phasing uses: psi_4 contains -10 sigma with
+      sigma = eta (721/48 chi1 chi2 - 247/48 chi1 . chi2)
+              + sum_i (m_i/M)^2 [(720 k_i - 1)/96 chi_i^2 - (240 k_i - 7)/96 |chi_i|^2]
+    for aligned spins (chi_i . L = chi_i) and black holes (k_i = 1)."""
+    rng = np.random.default_rng(5)
+    for _ in range(20):
+        m1, m2 = np.sort(rng.uniform(1.0, 60.0, 2))[::-1]
+        c1, c2 = rng.uniform(-0.95, 0.95, 2)
+        M = m1 + m2
+        eta = m1 * m2 / M ** 2
+        sigma = eta * (721 / 48 - 247 / 48) * c1 * c2
+        for m, c in ((m1, c1), (m2, c2)):
+            sigma += (m / M) ** 2 * ((720 - 1) / 96 - (240 - 7) / 96) * c * c
+        ss = (core.taylorf2_phase_coefficients(m1, m2, c1, c2)[0][4]
+              - core.taylorf2_phase_coefficients(m1, m2, 0.0, 0.0)[0][4])
+        assert abs(ss + 10 * sigma) < 1e-10 * max(1.0, abs(10 * sigma)), (m1, m2, c1, c2)

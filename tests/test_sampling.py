@@ -212,20 +212,23 @@ def test_remote_client_wire_format_in_worker_processes(tmp_path):
 
 
 def test_row_builder_columns_and_defaults():
-    """Conversion (chirp mass / mass ratio, a_i cos tilt_i), marginalised-key
-    defaults that an explicit value overrides, and the gwl_param columns."""
+    """Conversion (chirp mass / mass ratio, a_i cos tilt_i for aligned spins;
+    TaylorF2 refuses in-plane spin), marginalised-key defaults that an
+    explicit value overrides, and the gwl_param columns."""
     from bilby_amd import abi
     from bilby_amd.waveform_generator import convert_to_lal_binary_black_hole_parameters
     rb = pickle.loads(pickle.dumps(abi.RowBuilder(convert_to_lal_binary_black_hole_parameters,
                                                   {"geocent_time": 7.0, "phase": 0.0})))
     q, mc = np.array([0.5, 0.8]), np.array([20.0, 30.0])
-    rows = rb(dict(chirp_mass=mc, mass_ratio=q, a_1=0.3, tilt_1=np.pi / 3, ra=1.0, dec=0.5,
+    rows = rb(dict(chirp_mass=mc, mass_ratio=q, a_1=0.3, tilt_1=np.pi, ra=1.0, dec=0.5,
                    phase=np.array([1.0, 2.0])))
+    with pytest.raises(ValueError):
+        rb(dict(chirp_mass=mc, mass_ratio=q, a_1=0.3, tilt_1=np.pi / 3, ra=1.0, dec=0.5))
     total = mc * (1 + q) ** 1.2 / q ** 0.6
     assert rows.shape == (2, abi.NPARAM)
     assert np.allclose(rows[:, abi.P_MASS_1], total / (1 + q), rtol=1e-14)
     assert np.allclose(rows[:, abi.P_MASS_2], total / (1 + q) * q, rtol=1e-14)
-    assert np.allclose(rows[:, abi.P_CHI_1], 0.3 * 0.5, rtol=1e-14)
+    assert np.allclose(rows[:, abi.P_CHI_1], -0.3, rtol=1e-14)
     assert np.all(rows[:, abi.P_GEOCENT_TIME] == 7.0)
     assert np.all(rows[:, abi.P_PHASE] == [1.0, 2.0])
     assert np.all(rows[:, abi.P_RA] == 1.0) and np.all(rows[:, abi.P_CHI_2] == 0.0)
