@@ -664,7 +664,18 @@ struct RowW {
   double hh;                   // sum over the support of |h|^2 sigma^2 / S
   double hscale, jitter;
   double cabs2[GWL_MAX_DET];   // |complex detector amplitude / sigma|^2, float64
-  int kb, ke, valid, pad;
+  // the wave-per-row transform's reference detector ref (the largest
+  // amplitude C_ref): its delay and the phase of C_ref enter the float64
+  // phase, |C_ref| scales the result, and the other detectors (in index
+  // order, ref skipped) carry rho_k = C_k / C_ref and rotate relative to it
+  float rre[GWL_MAX_DET - 1];
+  float rim[GWL_MAX_DET - 1];
+  double amp;                  // |C_ref|
+  double c4r;                  // c[4] - arg(C_ref) / 2 pi
+  int kb, ke, valid, ref;
+  // the support [kb, ke] covers every detector's mask, outside which the
+  // data are zero: no element needs masking
+  int nomask, pad;
 };
 
 // Marginalised lnL at one time (base.py:775-792): distance spline (y-part

@@ -32,6 +32,7 @@ constexpr int kWP = 512;      // transform length
 constexpr int kWaves = 4;     // wavefronts (rows) per workgroup
 constexpr int kXch = 576;     // per-wave exchange buffer, padded (float2)
 constexpr int kTw3 = 7 * 64;  // pass-3 twiddles W_512^(s l), [s - 1][l]
+constexpr int kTw2 = 7 * 8;   // pass-2 twiddles W_64^(s k), [s - 1][k]
 
 // LDS tables of the transform: tw[m] = W_512^m and tw3[s - 1][l] =
 // W_512^(s l mod 512) (so that lane l reads consecutive entries), both
@@ -44,6 +45,35 @@ __device__ inline void init_wave_tables(float2* tw, float2* tw3) {
     double sn, cs;
     sincospi(-2.0 * (double)e / (double)kWP, &sn, &cs);
     tw3[k] = make_float2((float)cs, (float)sn);
+  }
+}
+
+// LDS tables of the row transform (k_time_w, k_time_w_multi): tw2[s - 1][k]
+// = W_64^(s k) for pass 2, where lane l uses k = l mod 8 (eight consecutive
+// entries per s: no bank conflicts); tw3 as above; wt[l + 64 s] = W_N^j in
+// float64 for the window bin j = l + 64 s (mod 512) of lane l
+// (j = jlo + ((l + 64 s - jlo) mod 512)), the Horner step of the second
+// transform stage.
+__device__ inline void init_row_tables(float2* tw2, float2* tw3, double2* wt, int N, int jlo) {
+  for (int k = threadIdx.x; k < kTw2; k += blockDim.x) {
+    const int s = k / 8 + 1, q = k & 7;
+    double sn, cs;
+    sincospi(-2.0 * (double)((s * q) & 63) / 64.0, &sn, &cs);
+    tw2[k] = make_float2((float)cs, (float)sn);
+  }
+  for (int k = threadIdx.x; k < kTw3; k += blockDim.x) {
+    const int s = k / 64 + 1, l = k & 63;
+    const int e = (s * l) & (kWP - 1);
+    double sn, cs;
+    sincospi(-2.0 * (double)e / (double)kWP, &sn, &cs);
+    tw3[k] = make_float2((float)cs, (float)sn);
+  }
+  for (int k = threadIdx.x; k < kWP; k += blockDim.x) {
+    const long long j = (long long)jlo + ((k - jlo) & (kWP - 1));
+    const long long e = j & (long long)(N - 1);
+    double sn, cs;
+    sincospi(-2.0 * (double)e / (double)N, &sn, &cs);
+    wt[k] = make_double2(cs, sn);
   }
 }
 
@@ -90,7 +120,35 @@ __device__ inline void rowconst_one(const EvDev& ev, const double* row, RowW* ou
   w.kb = sc.kb;
   w.ke = sc.ke;
   w.valid = sc.valid;
-  w.pad = 0;
+  {
+    int lo = ev.mlo[0], hi = ev.mhi[0];
+    for (int d = 1; d < ev.nd; ++d) {
+      lo = min(lo, ev.mlo[d]);
+      hi = max(hi, ev.mhi[d]);
+    }
+    w.nomask = (sc.kb <= lo && sc.ke >= hi) ? 1 : 0;
+    w.pad = 0;
+  }
+  // reference detector: the largest |C_d|, so that |rho_k| <= 1
+  int ref = 0;
+  for (int d = 1; d < ev.nd && d < GWL_MAX_DET; ++d)
+    if (sc.cabs2[d] > sc.cabs2[ref]) ref = d;
+  const double cr = sc.cre[ref], ci = sc.cim[ref];
+  const double a2 = cr * cr + ci * ci;
+  w.ref = ref;
+  w.amp = sqrt(a2);
+  w.c4r = sc.c[4] - (a2 > 0.0 ? atan2(ci, cr) / kTwoPi : 0.0);
+#pragma unroll
+  for (int k = 0; k < GWL_MAX_DET - 1; ++k) {
+    const int d = k < ref ? k : k + 1;
+    double re = 0.0, im = 0.0;
+    if (d < ev.nd && a2 > 0.0) {  // C_d conj(C_ref) / |C_ref|^2
+      re = (sc.cre[d] * cr + sc.cim[d] * ci) / a2;
+      im = (sc.cim[d] * cr - sc.cre[d] * ci) / a2;
+    }
+    w.rre[k] = (float)re;
+    w.rim[k] = (float)im;
+  }
   *out = w;
 }
 
@@ -122,40 +180,6 @@ __device__ __forceinline__ WaveEv wave_ev(const EvDev& ev) {
   return w;
 }
 
-// h conj(d) / S at decimated element i (bin nbin) for one row, summed over
-// detectors (interferometer.py:303-368, base.py:327-329): float64 phase,
-// reduced to [0, 1) cycles, float32 rotation and products.
-template <int ND>
-__device__ __forceinline__ float2 integrand_tf2(const WaveEv& ev, const RowW& rw, int nd, int i,
-                                                int nbin) {
-  const double f = (double)nbin * ev.df;
-  const double* tb = ev.tf2b + 3 * (size_t)i;
-  const double u = tb[0], v = tb[1], lf = tb[2];
-  const double v2 = v * v, v3 = v2 * v;
-  double s = rw.c[4];
-  s = fma(rw.c[0], v3 * v2, s);
-  s = fma(rw.c[1], v3, s);
-  s = fma(rw.c[2], v2, s);
-  s = fma(rw.c[3], v, s);
-  s = fma(rw.c[5], u, s);
-  s = fma(rw.c[6], u * u, s);
-  s = fma(rw.c[7], lf, s);
-  s = fma(rw.c[8], u * lf, s);
-  float ar = 0.f, ai = 0.f;
-#pragma unroll
-  for (int d = 0; d < ND; ++d) {
-    if (d >= nd) break;
-    const double cyc = fma(f, rw.dt[d], s);
-    const float2 e = expm2pi((float)(cyc - floor(cyc)));
-    const float2 dw = ev.dwa32[(size_t)d * ev.N + i];
-    const float hr = rw.cre[d] * e.x - rw.cim[d] * e.y;
-    const float hi = rw.cre[d] * e.y + rw.cim[d] * e.x;
-    ar += hr * dw.x + hi * dw.y;
-    ai += hi * dw.x - hr * dw.y;
-  }
-  return make_float2(ar, ai);
-}
-
 struct TimeWArgs {
   EvDev ev;
   const RowW* rows;
@@ -168,98 +192,95 @@ struct TimeWArgs {
   const int* only;   // if set: rows r with only[r] == 0 are skipped
 };
 
-// Psi(f) / 2pi (cycles) of one row at a tabled basis point: the nine-term
-// sum of tf2_phase_cycles in Horner form, float64.
-__device__ __forceinline__ double tf2_cycles_horner(const RowW& rw, const double* tb) {
+// Psi(f) / 2pi + f dt_ref - arg(C_ref) / 2pi (cycles) of one row at a tabled
+// basis point (u = f^1/3, v = f^-1/3, ln f): the nine-term sum of
+// tf2_phase_cycles in Horner form with the reference detector's delay as the
+// u^3 = f term and its amplitude phase in the constant, float64.
+__device__ __forceinline__ double tf2_cycles_ref(const RowW& rw, double dtr, const double* tb) {
   const double u = tb[0], v = tb[1], lf = tb[2];
-  // c0 v^5 + c1 v^3 + c2 v^2 + c3 v + c4
+  // c0 v^5 + c1 v^3 + c2 v^2 + c3 v + c4r
   double a = fma(rw.c[0], v * v, rw.c[1]);
   a = fma(a, v, rw.c[2]);
   a = fma(a, v, rw.c[3]);
-  double s = fma(a, v, rw.c[4]);
-  // + c5 u + c6 u^2 + (c7 + c8 u) ln f
-  s = fma(fma(rw.c[6], u, rw.c[5]), u, s);
+  double s = fma(a, v, rw.c4r);
+  // + c5 u + c6 u^2 + dt_ref u^3 + (c7 + c8 u) ln f
+  s = fma(fma(fma(dtr, u, rw.c[6]), u, rw.c[5]), u, s);
   return fma(fma(rw.c[8], u, rw.c[7]), lf, s);
 }
 
+// x - floor(x) in one instruction (v_fract_f64)
+__device__ __forceinline__ double frac1(double x) { return __builtin_amdgcn_fract(x); }
+
 // The transform of one row over sub-sequences [n2b, n2e): lane l returns
-// the float64 window accumulators of bins j = l + 64 s (mod 512) in
-// accr[s], acci[s].  tw: the 512-entry twiddle table in LDS; xb: this
+// the float64 window sums of bins j = l + 64 s (mod 512) in accr[s],
+// acci[s].  tw2 / tw3 / wt: the LDS tables of init_row_tables; xb: this
 // wavefront's exchange buffer.  nd <= ND detectors.
 //
-// The integrand h conj(d) / S = exp(-2 pi i Psi(f) / 2pi) sum_d C_d
-// exp(-2 pi i f dt_d) conj(dw_d f^-7/6): the source phase (float64, reduced
-// to [0, 1) cycles) is rotated once per element; each detector's delay
-// rotation starts from an exact float64 phase at the lane's first element
-// of the sub-sequence, f0 = (Q l + n2) df, and advances by the step
-// w_d = exp(-2 pi i 64 Q df dt_d) (float64, rounded to float32) to the
-// lane's elements l + 64 s, whose frequencies are f0 + 64 Q df s.
-template <int ND>
-__device__ __forceinline__ void wave_row(const WaveEv& ev, const RowW& rw, int nd, int n2b, int n2e,
-                                         const float2* tw, const float2* tw3, float2* xb, int lane,
-                                         double* accr, double* acci) {
+// The integrand h conj(d) / S = |C_r| exp(-2 pi i phi(f)) [conj(dw_r) +
+// sum_k rho_k exp(-2 pi i f (dt_k - dt_r)) conj(dw_k)] (dw: the tabled
+// dw f^-7/6), where r is the row's reference detector and phi(f) =
+// Psi(f) / 2pi + f dt_r - arg(C_r) / 2pi is evaluated in float64 and reduced
+// to [0, 1) cycles.  The other detectors' rotations start from an exact
+// float64 phase at the lane's first element of the sub-sequence,
+// f0 = (Q l + n2) df, and advance by w_k = exp(-2 pi i 64 Q df (dt_k - dt_r))
+// (float64, rounded to float32) to the lane's elements l + 64 s.
+//
+// Second stage: X[j] = sum_n2 W_N^(n2 j) Y_n2[j mod 512] by Horner's rule
+// over descending n2 (acc = acc W_N^j + Y_n2, float64, the step from the
+// float64 table wt), then times W_N^(n2b j) and |C_r|.
+// One sub-sequence n2 of wave_row: the integrand (elements outside the
+// support zeroed when MASK), the 512-point transform and the Horner step.
+template <int ND, bool MASK>
+__device__ __forceinline__ void wave_step(const WaveEv& ev, const RowW& rw, int nd, int n2, int kb,
+                                          int ke, double dtr, const double* dd, const float2* wd,
+                                          const float2* const* dwo, const float2* dwr,
+                                          const float2* tw2l, const float2* tw3, const double2* wt,
+                                          float2* xb, int lane, int base2, double* accr,
+                                          double* acci) {
+  constexpr int NO = ND - 1;
   const int Q = ev.Q;
-  // twiddles from LDS: pass 2 (span 8) W_64^(s k), k = l mod 8 (eight
-  // distinct addresses per s); pass 3 (span 64) W_512^(s l) from tw3
-  const int k2 = 8 * (lane & 7);
-#pragma unroll
-  for (int s = 0; s < 8; ++s) accr[s] = acci[s] = 0.0;
-  const unsigned nmask = (unsigned)ev.N - 1u;
-  const float invN = 1.0f / (float)ev.N;
-  const int kb = rw.kb, ke = rw.ke;
-  const int base2 = 64 * (lane >> 3) + (lane & 7);
-  // the steps in float64 (once per row), rounded once to float32
-  float2 wd[ND];
-#pragma unroll
-  for (int d = 0; d < ND; ++d) {
-    const double st = (64.0 * (double)Q * ev.df) * rw.dt[d];
-    const double2 w = expm2pi(st - floor(st));
-    wd[d] = make_float2((float)w.x, (float)w.y);
-  }
-
-  for (int n2 = n2b; n2 < n2e; ++n2) {
-    // in-support elements of this sub-sequence: kb <= Q i + n2 <= ke
-    const int ilo = (kb - n2 + Q - 1) / Q;
-    const int ihi = ke >= n2 ? (ke - n2) / Q : -1;
+  // in-support elements of this sub-sequence: kb <= Q i + n2 <= ke
+  const int ilo = (kb - n2 + Q - 1) / Q;
+  const int ihi = ke >= n2 ? (ke - n2) / Q : -1;
+  if (ilo <= ihi) {
     float2 x[8];
-    if (ilo <= ihi) {
-      const double f0 = (double)(Q * lane + n2) * ev.df;
-      float2 r[ND];
+    const double f0 = (double)(Q * lane + n2) * ev.df;
+    float2 r[NO > 0 ? NO : 1];
 #pragma unroll
-      for (int d = 0; d < ND; ++d) {
-        const double c0 = f0 * rw.dt[d];
-        r[d] = cmul(make_float2(rw.cre[d], rw.cim[d]), expm2pi((float)(c0 - floor(c0))));
+    for (int k = 0; k < NO; ++k)
+      r[k] = cmul(make_float2(rw.rre[k], rw.rim[k]), expm2pi((float)frac1(f0 * dd[k])));
+    const size_t e0 = (size_t)n2 * kWP + lane;
+    const double* tb = ev.tf2b + 3 * e0;
+    // every element is evaluated (the tables cover the whole sub-sequence);
+    // with MASK those outside the support are zeroed, without divergence
+#pragma unroll
+    for (int s = 0; s < 8; ++s) {
+      const double cyc = tf2_cycles_ref(rw, dtr, tb + 3 * 64 * s);
+      const float2 d0 = dwr[e0 + 64 * s];
+      float2 y = make_float2(d0.x, -d0.y);
+#pragma unroll
+      for (int k = 0; k < NO; ++k) {
+        if (k >= nd - 1) break;  // the event holds nd detectors' tables
+        const float2 dw = dwo[k][e0 + 64 * s];
+        y.x += r[k].x * dw.x + r[k].y * dw.y;
+        y.y += r[k].y * dw.x - r[k].x * dw.y;
       }
-      const float2* dwa = ev.dwa32 + (size_t)n2 * kWP + lane;
-      const double* tb = ev.tf2b + 3 * ((size_t)n2 * kWP + lane);
-#pragma unroll
-      for (int s = 0; s < 8; ++s) {
-        x[s] = make_float2(0.f, 0.f);
+      const float2 v = cmul(expm2pi((float)frac1(cyc)), y);
+      if constexpr (MASK) {
         const int i = lane + 64 * s;
-        if (i >= ilo && i <= ihi) {
-          const double cyc = tf2_cycles_horner(rw, tb + 3 * 64 * s);
-          float2 y = make_float2(0.f, 0.f);
-#pragma unroll
-          for (int d = 0; d < ND; ++d) {
-            if (d >= nd) break;  // the event holds nd detectors' tables
-            const float2 dw = dwa[(size_t)d * ev.N + 64 * s];
-            y.x += r[d].x * dw.x + r[d].y * dw.y;
-            y.y += r[d].y * dw.x - r[d].x * dw.y;
-          }
-          x[s] = cmul(expm2pi((float)(cyc - floor(cyc))), y);
-        }
-#pragma unroll
-        for (int d = 0; d < ND; ++d) r[d] = cmul(r[d], wd[d]);
+        const bool in = i >= ilo && i <= ihi;
+        x[s] = make_float2(in ? v.x : 0.f, in ? v.y : 0.f);
+      } else {
+        x[s] = v;
       }
-    } else {
 #pragma unroll
-      for (int s = 0; s < 8; ++s) x[s] = make_float2(0.f, 0.f);
+      for (int k = 0; k < NO; ++k) r[k] = cmul(r[k], wd[k]);
     }
     // pass 1 (span 1): inputs l + 64 s, outputs to 8 l + s.  When the
     // support ends below element 128 (uniform) only x[0], x[1] can be
     // nonzero and the butterfly is x0 + W_8^s x1 (the same arithmetic dft8
     // performs on those inputs).
-    if (ilo <= ihi && ihi >= 128) {
+    if (!MASK || ihi >= 128) {
       dft8(x);
     } else {
       const float2 x0 = x[0], a = x[1];
@@ -281,7 +302,7 @@ __device__ __forceinline__ void wave_row(const WaveEv& ev, const RowW& rw, int n
 #pragma unroll
     for (int s = 0; s < 8; ++s) x[s] = xb[xpad(lane + 64 * s)];
 #pragma unroll
-    for (int s = 1; s < 8; ++s) x[s] = cmul(x[s], tw[(s * k2) & (kWP - 1)]);
+    for (int s = 1; s < 8; ++s) x[s] = cmul(x[s], tw2l[(s - 1) * 8]);
     dft8(x);
     wave_sync();
 #pragma unroll
@@ -294,19 +315,109 @@ __device__ __forceinline__ void wave_row(const WaveEv& ev, const RowW& rw, int n
     for (int s = 1; s < 8; ++s) x[s] = cmul(x[s], tw3[(s - 1) * 64 + lane]);
     dft8(x);
     wave_sync();
-    // window bins j = jlo + jj with j = l + 64 s (mod 512)
+    // Horner step of the window bins j = l + 64 s (mod 512)
 #pragma unroll
     for (int s = 0; s < 8; ++s) {
-      const int jj = (lane + 64 * s - ev.jlo) & (kWP - 1);
-      if (jj < ev.L) {
-        const unsigned j = (unsigned)(ev.jlo + jj);
-        const unsigned idx = ((unsigned)n2 * j) & nmask;
-        const float2 t = expm2pi((float)idx * invN);
-        const float2 z = cmul(t, x[s]);
-        accr[s] += (double)z.x;
-        acci[s] += (double)z.y;
-      }
+      const double2 w = wt[lane + 64 * s];
+      const double ar = accr[s], ai = acci[s];
+      accr[s] = fma(ar, w.x, fma(-ai, w.y, (double)x[s].x));
+      acci[s] = fma(ar, w.y, fma(ai, w.x, (double)x[s].y));
     }
+  } else {
+    // nothing of the support in this sub-sequence (uniform): Y_n2 = 0, only
+    // the Horner step
+#pragma unroll
+    for (int s = 0; s < 8; ++s) {
+      const double2 w = wt[lane + 64 * s];
+      const double ar = accr[s], ai = acci[s];
+      accr[s] = fma(ar, w.x, -ai * w.y);
+      acci[s] = fma(ar, w.y, ai * w.x);
+    }
+  }
+}
+
+// The sub-sequences [n2b, n2e) in descending order.
+template <int ND, bool MASK>
+__device__ __forceinline__ void wave_loop(const WaveEv& ev, const RowW& rw, int nd, int n2b,
+                                          int n2e, double dtr, const double* dd, const float2* wd,
+                                          const float2* const* dwo, const float2* dwr,
+                                          const float2* tw2l, const float2* tw3, const double2* wt,
+                                          float2* xb, int lane, double* accr, double* acci) {
+  const int kb = rw.kb, ke = rw.ke;
+  const int base2 = 64 * (lane >> 3) + (lane & 7);
+  for (int n2 = n2e - 1; n2 >= n2b; --n2)
+    wave_step<ND, MASK>(ev, rw, nd, n2, kb, ke, dtr, dd, wd, dwo, dwr, tw2l, tw3, wt, xb, lane,
+                        base2, accr, acci);
+}
+
+// The transform of one row over sub-sequences [n2b, n2e): lane l returns
+// the float64 window sums of bins j = l + 64 s (mod 512) in accr[s],
+// acci[s].  tw2 / tw3 / wt: the LDS tables of init_row_tables; xb: this
+// wavefront's exchange buffer.  nd <= ND detectors.  FAST: rows whose
+// support covers every detector's mask run without element masking.
+//
+// The integrand h conj(d) / S = |C_r| exp(-2 pi i phi(f)) [conj(dw_r) +
+// sum_k rho_k exp(-2 pi i f (dt_k - dt_r)) conj(dw_k)] (dw: the tabled
+// dw f^-7/6), where r is the row's reference detector and phi(f) =
+// Psi(f) / 2pi + f dt_r - arg(C_r) / 2pi is evaluated in float64 and reduced
+// to [0, 1) cycles.  The other detectors' rotations start from an exact
+// float64 phase at the lane's first element of the sub-sequence,
+// f0 = (Q l + n2) df, and advance by w_k = exp(-2 pi i 64 Q df (dt_k - dt_r))
+// (float64, rounded to float32) to the lane's elements l + 64 s.
+//
+// Second stage: X[j] = sum_n2 W_N^(n2 j) Y_n2[j mod 512] by Horner's rule
+// over descending n2 (acc = acc W_N^j + Y_n2, float64, the step from the
+// float64 table wt), then times W_N^(n2b j) and |C_r|.
+template <int ND, bool FAST = true>
+__device__ __forceinline__ void wave_row(const WaveEv& ev, const RowW& rw, int nd, int n2b, int n2e,
+                                         const float2* tw2, const float2* tw3, const double2* wt,
+                                         float2* xb, int lane, double* accr, double* acci) {
+  constexpr int NO = ND - 1;  // detectors other than the reference
+  const int Q = ev.Q;
+  const float2* tw2l = tw2 + (lane & 7);
+#pragma unroll
+  for (int s = 0; s < 8; ++s) accr[s] = acci[s] = 0.0;
+  const int ref = rw.ref;
+  // (static indices only: rw may be a register copy)
+  double dtr = rw.dt[0];
+#pragma unroll
+  for (int d = 1; d < ND; ++d)
+    if (d == ref) dtr = rw.dt[d];
+  // the others' delays relative to the reference and their steps (float64
+  // once per row, rounded once to float32)
+  double dd[NO > 0 ? NO : 1];
+  float2 wd[NO > 0 ? NO : 1];
+  const float2* dwo[NO > 0 ? NO : 1];
+#pragma unroll
+  for (int k = 0; k < NO; ++k) {
+    const int d = k < ref ? k : k + 1;
+    dd[k] = (k < ref ? rw.dt[k] : rw.dt[k + 1]) - dtr;
+    const double2 w = expm2pi(frac1((64.0 * (double)Q * ev.df) * dd[k]));
+    wd[k] = make_float2((float)w.x, (float)w.y);
+    dwo[k] = ev.dwa32 + (size_t)d * ev.N;
+  }
+  const float2* dwr = ev.dwa32 + (size_t)ref * ev.N;
+  if (FAST && rw.nomask)
+    wave_loop<ND, false>(ev, rw, nd, n2b, n2e, dtr, dd, wd, dwo, dwr, tw2l, tw3, wt, xb, lane,
+                         accr, acci);
+  else
+    wave_loop<ND, true>(ev, rw, nd, n2b, n2e, dtr, dd, wd, dwo, dwr, tw2l, tw3, wt, xb, lane,
+                        accr, acci);
+  // the group's first sub-sequence n2b: times W_N^(n2b j); and |C_r|
+  const double amp = rw.amp;
+#pragma unroll
+  for (int s = 0; s < 8; ++s) {
+    double tr = amp, ti = 0.0;
+    if (n2b > 0) {
+      const long long j = (long long)ev.jlo + ((lane + 64 * s - ev.jlo) & (kWP - 1));
+      const long long e = ((long long)n2b * j) & (long long)(ev.N - 1);
+      const double2 t = expm2pi((double)e / (double)ev.N);
+      tr = amp * t.x;
+      ti = amp * t.y;
+    }
+    const double ar = accr[s], ai = acci[s];
+    accr[s] = ar * tr - ai * ti;
+    acci[s] = ar * ti + ai * tr;
   }
 }
 
@@ -328,12 +439,13 @@ template <int ND>
 __global__ __launch_bounds__(64 * kWaves, 4) void k_time_w(TimeWArgs a) {
   extern __shared__ double lds_d[];
   const EvDev& ev = a.ev;
-  float2* tw = reinterpret_cast<float2*>(lds_d);                  // [512]
-  float2* tw3 = tw + kWP;                                          // [7][64]
+  double2* wt = reinterpret_cast<double2*>(lds_d);                 // [512]
+  float2* tw2 = reinterpret_cast<float2*>(wt + kWP);               // [7][8]
+  float2* tw3 = tw2 + kTw2;                                        // [7][64]
   float2* xch = tw3 + kTw3;                                        // [kWaves][kXch]
   double* knots = reinterpret_cast<double*>(xch + kWaves * kXch);  // tx, ty
   const bool dist = (ev.flags & GWL_DISTANCE_MARG) != 0;
-  init_wave_tables(tw, tw3);
+  init_row_tables(tw2, tw3, wt, ev.N, ev.jlo);
   if (a.fused && dist) stage_knots(ev, knots);
   __syncthreads();
   const int lane = threadIdx.x & 63;
@@ -349,7 +461,7 @@ __global__ __launch_bounds__(64 * kWaves, 4) void k_time_w(TimeWArgs a) {
   const int n2e = (int)(((long long)(g + 1) * Q) / a.G);
   const WaveEv wv = wave_ev(ev);
   double accr[8], acci[8];
-  wave_row<ND>(wv, rw, ND, n2b, n2e, tw, tw3, xch + wid * kXch, lane, accr, acci);
+  wave_row<ND>(wv, rw, ND, n2b, n2e, tw2, tw3, wt, xch + wid * kXch, lane, accr, acci);
 
   if (!a.fused) {
     wave_store_partials(wv, rw, r, a.G, g, lane, accr, acci, a.part, a.hhpart);
@@ -635,14 +747,15 @@ __global__ __launch_bounds__(64 * kWaves, 4) void k_time_w_multi(MultiWArgs a) {
   const int it = blockIdx.x;
   if (it >= *a.n_items) return;
   extern __shared__ double lds_d[];
-  float2* tw = reinterpret_cast<float2*>(lds_d);  // [512]
-  float2* tw3 = tw + kWP;                          // [7][64]
-  float2* xch = tw3 + kTw3;                        // [kWaves][kXch]
-  init_wave_tables(tw, tw3);
+  double2* wt = reinterpret_cast<double2*>(lds_d);    // [512]
+  float2* tw2 = reinterpret_cast<float2*>(wt + kWP);  // [7][8]
+  float2* tw3 = tw2 + kTw2;                            // [7][64]
+  float2* xch = tw3 + kTw3;                            // [kWaves][kXch]
+  const MultiItem m = a.items[it];
+  init_row_tables(tw2, tw3, wt, a.evs[m.e].N, a.evs[m.e].jlo);
   __syncthreads();
   const int lane = threadIdx.x & 63;
   const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const MultiItem m = a.items[it];
   const long long r = m.row0 + wid;
   if (r >= a.rows[m.e]) return;
   const WaveEv ev = uniform_copy(wave_ev(a.evs[m.e]));
@@ -650,9 +763,15 @@ __global__ __launch_bounds__(64 * kWaves, 4) void k_time_w_multi(MultiWArgs a) {
   const int Q = ev.Q;
   const int n2b = (int)(((long long)m.g * Q) / G);
   const int n2e = (int)(((long long)(m.g + 1) * Q) / G);
-  const RowW rw = uniform_copy(a.roww[m.e][r]);
+  const RowW& rw = a.roww[m.e][r];
   double accr[8], acci[8];
-  wave_row<3>(ev, rw, ev.nd, n2b, n2e, tw, tw3, xch + wid * kXch, lane, accr, acci);
+  // specialised on the detector count (wave-uniform)
+  if (ev.nd == 1)
+    wave_row<1, false>(ev, rw, 1, n2b, n2e, tw2, tw3, wt, xch + wid * kXch, lane, accr, acci);
+  else if (ev.nd == 2)
+    wave_row<2, false>(ev, rw, 2, n2b, n2e, tw2, tw3, wt, xch + wid * kXch, lane, accr, acci);
+  else
+    wave_row<3, false>(ev, rw, 3, n2b, n2e, tw2, tw3, wt, xch + wid * kXch, lane, accr, acci);
   wave_store_partials(ev, rw, r, G, m.g, lane, accr, acci, a.part[m.e], a.hhpart[m.e]);
 }
 
@@ -710,8 +829,15 @@ int init_time_wave_tables(const EvDev& ev, double* tb, float2* dwa) {
 }
 
 size_t time_wave_lds_bytes(const EvDev& ev, bool fused) {
-  size_t b = (size_t)(kWP + kTw3 + kWaves * kXch) * sizeof(float2);
+  size_t b = (size_t)kWP * sizeof(double2) + (size_t)(kTw2 + kTw3 + kWaves * kXch) * sizeof(float2);
   if (fused && (ev.flags & GWL_DISTANCE_MARG)) b += (size_t)(ev.sp.nx + ev.sp.ny + 8) * sizeof(double);
+  return b;
+}
+
+// k_cal_time_w: the 512-entry pass-2 table, tw3, the exchange buffers, knots
+static size_t cal_time_lds_bytes(const EvDev& ev) {
+  size_t b = (size_t)(kWP + kTw3 + kWaves * kXch) * sizeof(float2);
+  if (ev.flags & GWL_DISTANCE_MARG) b += (size_t)(ev.sp.nx + ev.sp.ny + 8) * sizeof(double);
   return b;
 }
 
@@ -790,7 +916,7 @@ int launch_cal_time(gwl_event* e, const double* params, int64_t n, int ld, int n
   ca.cal = cal;
   ca.calps = calps;
   ca.lnl = lnl;
-  const size_t lds = time_wave_lds_bytes(ev, true);
+  const size_t lds = cal_time_lds_bytes(ev);
   if (lds > e->ctx->lds_per_block) {
     set_error("calibration time transform exceeds the LDS of a workgroup");
     return GWL_E_UNSUPPORTED;
@@ -814,7 +940,9 @@ int launch_cal_time(gwl_event* e, const double* params, int64_t n, int ld, int n
   return GWL_OK;
 }
 
-size_t multi_wave_lds_bytes() { return (size_t)(kWP + kTw3 + kWaves * kXch) * sizeof(float2); }
+size_t multi_wave_lds_bytes() {
+  return (size_t)kWP * sizeof(double2) + (size_t)(kTw2 + kTw3 + kWaves * kXch) * sizeof(float2);
+}
 
 int multi_launch_wave(gwl_multi* m, hipStream_t st) {
   MultiWArgs a;
