@@ -87,6 +87,8 @@ def _load():
         "gwl_event_reserve": ([vp, ctypes.c_int64], ctypes.c_int),
         "gwl_eval": ([vp, vp, ctypes.c_int64, ctypes.c_int32, ctypes.c_int32, vp, vp, vp],
                      ctypes.c_int),
+        "gwl_eval_noalloc": ([vp, vp, ctypes.c_int64, ctypes.c_int32, ctypes.c_int32, vp, vp, vp],
+                             ctypes.c_int),
         "gwl_detector_response": ([vp, _dp, ctypes.c_int32, ctypes.c_int32, _dp], ctypes.c_int),
         "gwl_distance_table": ([vp, _dp, _dp, ctypes.c_int32, ctypes.c_double, ctypes.c_int32,
                                 _dp, ctypes.c_int32, _dp, ctypes.c_int32, _dp], ctypes.c_int),
@@ -234,6 +236,6 @@ EXPORTED_SYMBOLS = [
     "gwl_roq_weights_linear", "gwl_relbin_time_series", "gwl_mb_time_series",
     "gwl_roq_inner_products", "gwl_relbin_set_upsampled", "gwl_relbin_upsampled_series",
     "gwl_weighted_inner_products", "gwl_window_time_series", "gwl_event_set_escalation",
-    "gwl_event_escalated_rows", "gwl_event_set_time_prior",
+    "gwl_event_escalated_rows", "gwl_eval_noalloc", "gwl_event_set_time_prior",
     "gwl_roq_set_time_prior", "gwl_event_set_multi_steps",
 ]

@@ -12,7 +12,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libgwl.so")
 SOURCES = ["gwl_api.hip", "gwl_kernels.hip", "gwl_roq.hip", "gwl_time_wave.hip", "gwl_server.hip",
            "gwl_relbin.hip", "gwl_calib.hip",
-           "gwl_roq_weights.hip", "gwl_window.hip"]
+           "gwl_roq_weights.hip", "gwl_window.hip", "gwl_noalloc.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
     "-O3", "-std=c++17", "-fPIC", "--offload-arch=gfx950",

@@ -380,6 +380,10 @@ int gwl_ctx_create(int32_t device, gwl_ctx** out) {
   c->device = device;
   c->n_cu = prop.multiProcessorCount;
   c->lds_per_block = prop.sharedMemPerBlock;
+  if (int rc = noalloc_scratch_init(device, &c->scratch)) {
+    delete c;
+    return rc;
+  }
   *out = c;
   return GWL_OK;
 }

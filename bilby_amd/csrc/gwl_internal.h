@@ -10,6 +10,7 @@ struct gwl_ctx {
   int device;
   int n_cu;
   size_t lds_per_block;
+  void* scratch = nullptr;  // this device's address of the gwl_eval_noalloc scratch array
 };
 
 // Tile of the time-marginalised kernel: S rows per workgroup, P-point LDS
@@ -115,6 +116,8 @@ size_t workspace_bytes_for(const gwl_event* ev, int64_t rows, int precision);
 // upload a time-prior density table (n values at t0 + k dt) into *dst (freed
 // and replaced), checking the arguments
 int upload_time_prior(int64_t n, double t0, double dt, const double* density, double** dst);
+// this device's address of the gwl_eval_noalloc scratch array (gwl_noalloc.hip)
+int noalloc_scratch_init(int device, void** ptr);
 int launch_eval(gwl_event* ev, const double* params, int64_t n, int ld, int precision, double* out,
                 double* aux, hipStream_t stream);
 int timing_begin(gwl_event* ev, hipStream_t stream);

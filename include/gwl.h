@@ -151,6 +151,22 @@ int gwl_event_device_bytes(gwl_event* ev, int64_t* bytes);
 int gwl_eval(gwl_event* ev, const double* params, int64_t n_rows, int32_t ld,
              int32_t precision, double* out, double* aux, void* stream);
 
+/* gwl_eval that allocates nothing.  No counterpart in the reference: for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  The evaluation's
+ * workspace (row constants, partial sums, the float32 escalation list) is a
+ * file-scope device array of GWL_NOALLOC_SCRATCH_BYTES per device, held
+ * until the process ends, instead of the event's own workspace; a batch
+ * whose workspace would exceed it is rejected with GWL_E_INVALID (no
+ * partial evaluation).  The caller passes params, out and aux as for
+ * gwl_eval, with the same results.  Every call on one device shares that
+ * device's array: calls that overlap on one GPU (other streams or threads,
+ * gwl_eval_noalloc on any event) race, and must be serialised by the
+ * caller; the same event must not be evaluated concurrently by gwl_eval. */
+#define GWL_NOALLOC_SCRATCH_BYTES (64u << 20)
+int gwl_eval_noalloc(gwl_event* ev, const double* params, int64_t n_rows, int32_t ld,
+                     int32_t precision, double* out, double* aux, void* stream);
+
 /* The time-marginalisation arrays of n rows without the marginalisation:
  * d_inner_h_array = 4/T FFT_N(h conj(d) / S) at the event's time window
  * (base.py:325-330; bins j_first .. j_first + n_window - 1 of
