@@ -319,3 +319,45 @@ class PriorDict(dict):
             if isinstance(p, Constraint) and key in out:
                 prob = prob & p.prob(out[key])
         return prob
+
+
+class CBCPriorDict(PriorDict):
+    """The mass bounds of bilby/gw/prior.py:721-782 that the ROQ checks read
+    (perform_roq_params_check); None where the priors do not fix them."""
+
+    def copy(self):
+        return CBCPriorDict(self, conversion_function=self.conversion_function)
+
+    def _chirp_mass_bound(self, which):
+        if "chirp_mass" in self:
+            return getattr(self["chirp_mass"], which)
+        m1 = m2 = None
+        if "mass_1" in self:
+            m1 = getattr(self["mass_1"], which)
+            if "mass_2" in self:
+                m2 = getattr(self["mass_2"], which)
+            elif "mass_ratio" in self:
+                m2 = m1 * getattr(self["mass_ratio"], which)
+        if m1 is None or m2 is None:
+            return None
+        return (m1 * m2) ** 0.6 / (m1 + m2) ** 0.2
+
+    @property
+    def minimum_chirp_mass(self):
+        return self._chirp_mass_bound("minimum")
+
+    @property
+    def maximum_chirp_mass(self):
+        return self._chirp_mass_bound("maximum")
+
+    @property
+    def minimum_component_mass(self):
+        """mass_2's minimum, or the lighter mass at the chirp-mass and
+        mass-ratio minima."""
+        if "mass_2" in self:
+            return self["mass_2"].minimum
+        if "chirp_mass" in self and "mass_ratio" in self:
+            mc, q = self["chirp_mass"].minimum, self["mass_ratio"].minimum
+            total = mc * (1 + q) ** 1.2 / q ** 0.6
+            return total * q / (1 + q)
+        return None

@@ -9,9 +9,11 @@ frequency nodes and per detector
 The linear weights, bilby's dominant set-up cost (one inverse FFT of the
 full segment per basis element and detector), are computed on the GPU
 (gwl_roq_weights_linear: a float64 GEMM over frequency chunks against the
-exponentials at the window of time samples).  One basis (no prior-range
-selection among several) and bases on the full masked grid (not
-multibanded bases) are supported.
+exponentials at the window of time samples).  The bases are given on each
+detector's masked frequencies, or on the ROQ frequency grid of roq_params
+(flow, fhigh, seglen, scaled by roq_scale_factor), whose intersection with
+the masked frequencies the sums run over (roq.py:800-838); multibanded
+bases are not supported.
 """
 import ctypes
 
@@ -54,19 +56,53 @@ def time_samples(interferometers, time_prior_minimum, time_prior_maximum, delta_
     return np.arange(lo, hi + 1) * float(delta_t), lo, n
 
 
-def linear_weights(interferometer, basis, duration, k0, n_times, n_samples, device=0):
+def roq_frequency_indices(interferometers, roq_params, roq_scale_factor=1.0):
+    """roq.py:800-838: per detector, the indices into the ROQ frequency grid
+    (create_frequency_series(2 fhigh s, seglen / s) from flow s, s the scale
+    factor) and into the detector's masked frequencies of the frequencies
+    both hold.  Returns ({name: roq_idxs}, {name: ifo_idxs})."""
+    from .detector import create_frequency_series
+    s = float(roq_scale_factor)
+    fmin = float(roq_params["flow"]) * s
+    fmax = float(roq_params["fhigh"]) * s
+    seglen = float(roq_params["seglen"]) / s
+    f_roq = create_frequency_series(sampling_frequency=fmax * 2, duration=seglen)
+    f_roq = f_roq[f_roq >= fmin]
+    roq_idxs, ifo_idxs = {}, {}
+    for ifo in interferometers:
+        _, i_ifo, i_roq = np.intersect1d(np.asarray(ifo.frequency_array[ifo.frequency_mask]),
+                                         f_roq, return_indices=True)
+        roq_idxs[ifo.name] = i_roq
+        ifo_idxs[ifo.name] = i_ifo
+    return roq_idxs, ifo_idxs
+
+
+def _selection(interferometer, basis, roq_idxs, ifo_idxs):
+    """The basis columns and masked-frequency indices the sums run over:
+    every masked frequency when no ROQ grid is given (the basis must then
+    cover exactly those), else the grid intersection."""
+    n_m = int(np.sum(interferometer.frequency_mask))
+    if roq_idxs is None:
+        if basis.shape[1] != n_m:
+            raise ValueError("Mismatch between ROQ basis and frequency array for %s"
+                             % interferometer.name)
+        return np.arange(n_m), np.arange(n_m)
+    return np.asarray(roq_idxs[interferometer.name]), np.asarray(ifo_idxs[interferometer.name])
+
+
+def linear_weights(interferometer, basis, duration, k0, n_times, n_samples, device=0,
+                   roq_idxs=None, ifo_idxs=None):
     """roq.py:862-918 for one detector: [n_times, L] complex on the GPU."""
     from .likelihood import get_context
     ifo = interferometer
     m = ifo.frequency_mask
     basis = np.ascontiguousarray(basis, dtype=complex)
-    d_over_s = ifo.frequency_domain_strain[m] / ifo.power_spectral_density_array[m]
-    if basis.shape[1] != len(d_over_s):
-        raise ValueError("Mismatch between ROQ basis and frequency array for %s" % ifo.name)
+    cols, sel = _selection(ifo, basis, roq_idxs, ifo_idxs)
+    d_over_s = (ifo.frequency_domain_strain[m] / ifo.power_spectral_density_array[m])[sel]
+    basis = np.ascontiguousarray(basis[:, cols])
     x = np.ascontiguousarray((d_over_s[None, :] * np.conj(basis)).view(np.float64))
     # roq.py:885-886: the masked bins start at int(minimum_frequency * T)
-    bins = np.ascontiguousarray(np.arange(len(d_over_s), dtype=np.int64)
-                                + int(ifo.minimum_frequency * duration))
+    bins = np.ascontiguousarray(sel.astype(np.int64) + int(ifo.minimum_frequency * duration))
     out = np.zeros((n_times, basis.shape[0], 2))
     check(LIB.gwl_roq_weights_linear(get_context(device), basis.shape[0], len(d_over_s), dptr(x),
                                      bins.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
@@ -75,27 +111,35 @@ def linear_weights(interferometer, basis, duration, k0, n_times, n_samples, devi
     return out[..., 0] + 1j * out[..., 1]
 
 
-def quadratic_weights(interferometer, basis, duration):
+def quadratic_weights(interferometer, basis, duration, roq_idxs=None, ifo_idxs=None):
     """roq.py:976-1005: 4/T Re(B_q) . (1 / S) over the masked frequencies."""
     ifo = interferometer
     m = ifo.frequency_mask
-    return 4.0 / duration * np.asarray(basis).real @ (1.0 / ifo.power_spectral_density_array[m])
+    basis = np.asarray(basis)
+    cols, sel = _selection(ifo, basis, roq_idxs, ifo_idxs)
+    return 4.0 / duration * basis[:, cols].real @ (1.0 / ifo.power_spectral_density_array[m][sel])
 
 
 def build_roq_weights(interferometers, linear_basis, quadratic_basis, frequency_nodes_linear,
                       frequency_nodes_quadratic, time_prior_minimum, time_prior_maximum,
-                      delta_t=None, optimal_snrs=None, device=0):
+                      delta_t=None, optimal_snrs=None, device=0, roq_params=None,
+                      roq_scale_factor=1.0):
     """The weights dict of roq.py:736-847 for one linear and one quadratic
-    basis given on each detector's masked frequencies."""
+    basis given on each detector's masked frequencies, or with roq_params on
+    the ROQ frequency grid (see roq_frequency_indices)."""
     if delta_t is None:
         delta_t = time_resolution(interferometers, optimal_snrs)
     times, k0, n = time_samples(interferometers, time_prior_minimum, time_prior_maximum, delta_t)
     weights = {"time_samples": times,
                "frequency_nodes_linear": np.asarray(frequency_nodes_linear, float),
                "frequency_nodes_quadratic": np.asarray(frequency_nodes_quadratic, float)}
+    roq_idxs = ifo_idxs = None
+    if roq_params is not None:
+        roq_idxs, ifo_idxs = roq_frequency_indices(interferometers, roq_params, roq_scale_factor)
     T = interferometers.duration
     for ifo in interferometers:
         weights[ifo.name + "_linear"] = linear_weights(ifo, linear_basis, T, k0, len(times), n,
-                                                       device)
-        weights[ifo.name + "_quadratic"] = quadratic_weights(ifo, quadratic_basis, T)
+                                                       device, roq_idxs, ifo_idxs)
+        weights[ifo.name + "_quadratic"] = quadratic_weights(ifo, quadratic_basis, T, roq_idxs,
+                                                             ifo_idxs)
     return weights

@@ -153,6 +153,10 @@ class LikelihoodServer:
             check(LIB.gwl_server_add_event(self._s, likelihood._event, int(likelihood.precision),
                                            ctypes.byref(i)), "gwl_server_add_event")
         else:
+            if getattr(likelihood, "_multi_basis", False):
+                raise NotImplementedError(
+                    "a ROQ likelihood with several bases is served per basis pair: register "
+                    "one likelihood per pair (the rows' basis selection is the caller's)")
             check(LIB.gwl_roq_reserve(likelihood._roq, self.max_batch_rows), "gwl_roq_reserve")
             nbytes = _device_bytes(likelihood)
             self._admit(name, nbytes + staging)
