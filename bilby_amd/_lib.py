@@ -125,6 +125,7 @@ def _load():
         "gwl_event_set_transform": ([vp, ctypes.c_int32], ctypes.c_int),
         "gwl_event_set_escalation": ([vp, ctypes.c_double, ctypes.c_double], ctypes.c_int),
         "gwl_event_set_multi_steps": ([vp, ctypes.c_int32], ctypes.c_int),
+        "gwl_event_set_polarizations": ([vp, vp, ctypes.c_int64], ctypes.c_int),
         "gwl_event_set_time_prior": ([vp, ctypes.c_int64, ctypes.c_double, ctypes.c_double, _dp],
                                      ctypes.c_int),
         "gwl_roq_set_time_prior": ([vp, ctypes.c_int64, ctypes.c_double, ctypes.c_double, _dp],
@@ -237,5 +238,5 @@ EXPORTED_SYMBOLS = [
     "gwl_roq_inner_products", "gwl_relbin_set_upsampled", "gwl_relbin_upsampled_series",
     "gwl_weighted_inner_products", "gwl_window_time_series", "gwl_event_set_escalation",
     "gwl_event_escalated_rows", "gwl_eval_noalloc", "gwl_event_set_time_prior",
-    "gwl_roq_set_time_prior", "gwl_event_set_multi_steps",
+    "gwl_roq_set_time_prior", "gwl_event_set_multi_steps", "gwl_event_set_polarizations",
 ]

@@ -14,7 +14,7 @@ PARAM_COLUMNS = {
     "theta_jn": P_THETA_JN, "phase": P_PHASE, "mass_1": P_MASS_1, "mass_2": P_MASS_2,
     "chi_1": P_CHI_1, "chi_2": P_CHI_2, "hrss": P_HRSS, "Q": P_Q, "frequency": P_FREQUENCY,
 }
-SINE_GAUSSIAN, TAYLORF2 = 1, 2
+SINE_GAUSSIAN, TAYLORF2, HOST_POLARIZATIONS = 1, 2, 3
 TIME_MARG, DISTANCE_MARG, PHASE_MARG, JITTER_TIME = 1, 2, 4, 8
 # TaylorF2 ends at the ISCO as well as at maximum_frequency (opt-in; see
 # waveform_generator.taylorf2_flags)

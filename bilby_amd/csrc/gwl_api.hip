@@ -397,7 +397,8 @@ int gwl_event_create(gwl_ctx* ctx, const gwl_event_desc* d, gwl_event** out, int
   if (!ctx || !d || !out) { set_error("gwl_event_create: NULL argument"); return GWL_E_INVALID; }
   if (d->n_det < 1 || d->n_det > GWL_MAX_DET) { set_error("n_det must be 1..4"); return GWL_E_INVALID; }
   if (d->n_freq < 3) { set_error("n_freq too small"); return GWL_E_INVALID; }
-  if (d->approximant != GWL_SINE_GAUSSIAN && d->approximant != GWL_TAYLORF2) {
+  if (d->approximant != GWL_SINE_GAUSSIAN && d->approximant != GWL_TAYLORF2 &&
+      d->approximant != GWL_HOST_POLARIZATIONS) {
     set_error("unknown approximant");
     return GWL_E_INVALID;
   }
@@ -623,6 +624,25 @@ int gwl_event_escalated_rows(gwl_event* e, int64_t* rows) {
   int c = 0;
   GWL_HIP_CHECK(hipMemcpy(&c, e->esc_count_dev, sizeof(int), hipMemcpyDeviceToHost));
   *rows = c;
+  return GWL_OK;
+}
+
+int gwl_event_set_polarizations(gwl_event* e, const double* polarizations, int64_t n_rows) {
+  if (!e || n_rows < 0 || (n_rows > 0 && !polarizations)) {
+    set_error("gwl_event_set_polarizations: invalid argument");
+    return GWL_E_INVALID;
+  }
+  if (e->dev.approx != GWL_HOST_POLARIZATIONS) {
+    set_error("gwl_event_set_polarizations: the event's source model is evaluated by the "
+              "library (create it with GWL_HOST_POLARIZATIONS)");
+    return GWL_E_INVALID;
+  }
+  if (n_rows > 0x7fffffffLL) {
+    set_error("gwl_event_set_polarizations: more than 2^31 - 1 rows");
+    return GWL_E_INVALID;
+  }
+  e->dev.pol = reinterpret_cast<const double2*>(polarizations);
+  e->pol_rows = n_rows;
   return GWL_OK;
 }
 
@@ -998,7 +1018,7 @@ int gwl_multi_create(gwl_ctx* ctx, int32_t n_events, gwl_event* const* events,
   for (int e = 0; e < n_events; ++e) {
     const gwl_event* ev = events[e];
     if (!ev || !(ev->dev.flags & GWL_TIME_MARG) || ev->dev.P != 512 || ev->dev.nd > 3 ||
-        ev->dev.approx != m->wave || max_rows[e] < 0 ||
+        ev->dev.approx != m->wave || ev->dev.approx == GWL_HOST_POLARIZATIONS || max_rows[e] < 0 ||
         ((ev->dev.flags & GWL_DISTANCE_MARG) && !ev->has_spline)) {
       delete m;
       set_error("gwl_multi_create: every event must be time-marginalised with a 512-sample "

@@ -51,6 +51,7 @@ __device__ inline void tile_prologue(const EvDev& ev, const double* params, int 
     long long r = row0 + t;
     if (r < n) {
       compute_sample_const(ev, params + r * (long long)ld, sc[t]);
+      sc[t].row = (int)r;
     } else {
       sc[t].valid = 0;
       sc[t].kb = 1;
@@ -127,11 +128,13 @@ __device__ inline void tile_prologue_load(const EvDev& ev, const SampleConst* ro
 //   I[s]  = sum_d h_{s,d}(n) conj(dw_d(n))     (h conj(d) / S, base.py:327-329)
 //   hh[s] += sum_d |h_{s,d}(n)|^2 w_d(n)        (|h|^2 / S, gw/utils.py:137)
 // with h_{s,d} = H_{s,d}(f) exp(-2 pi i (Psi_s(f) / 2pi + f dt_{s,d})),
-// the phase argument in float64 (interferometer.py:357-362).
+// the phase argument in float64 (interferometer.py:357-362).  pol / nf: the
+// caller's polarizations (GWL_HOST_POLARIZATIONS) and their bins per row.
 template <typename R, int WAVE, int ND, int S>
 __device__ __forceinline__ void bin_rows(int n, double f, const cplx<R>* dw, const R* w,
                                          const SampleConst* sc, const SampleFast<R>* sf,
-                                         cplx<R>* I, R* hh, int nd = ND) {
+                                         cplx<R>* I, R* hh, int nd = ND,
+                                         const double2* pol = nullptr, int nf = 0) {
   using C = cplx<R>;
   if constexpr (WAVE == GWL_TAYLORF2) {
     double b[kNB];
@@ -159,6 +162,33 @@ __device__ __forceinline__ void bin_rows(int n, double f, const cplx<R>* dw, con
       }
       I[s] = cmk<C>(ar, ai);
       hh[s] += amp2 * hw;
+    }
+  } else if constexpr (WAVE == GWL_HOST_POLARIZATIONS) {
+    // the caller's h+ and hx at bin n of row sc[s].row, projected with
+    // F+/sigma and Fx/sigma (float64) and shifted by each detector's delay
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+      if (n < sc[s].kb || n > sc[s].ke) {
+        I[s] = cmk<C>((R)0, (R)0);
+        continue;
+      }
+      const double2 hp = pol[(size_t)sc[s].row * 2 * nf + n];
+      const double2 hx = pol[((size_t)sc[s].row * 2 + 1) * nf + n];
+      R ar = 0, ai = 0, hw = 0;
+#pragma unroll
+      for (int d = 0; d < ND; ++d) {
+        if (d >= nd) break;
+        const C p = cmk<C>((R)(sc[s].cre[d] * hp.x + sc[s].cim[d] * hx.x),
+                           (R)(sc[s].cre[d] * hp.y + sc[s].cim[d] * hx.y));
+        const double cyc = f * sc[s].dt[d];
+        const C e = expm2pi((R)(cyc - floor(cyc)));
+        const R hr = p.x * e.x - p.y * e.y, hi = p.x * e.y + p.y * e.x;
+        ar += hr * dw[d].x + hi * dw[d].y;
+        ai += hi * dw[d].x - hr * dw[d].y;
+        hw += (hr * hr + hi * hi) * w[d];
+      }
+      I[s] = cmk<C>(ar, ai);
+      hh[s] += hw;
     }
   } else {  // sine-Gaussian
 #pragma unroll

@@ -376,6 +376,11 @@ int gwl_calib_create(gwl_event* ev, int32_t n_curves, const double* curves, gwl_
     return GWL_E_INVALID;
   }
   const gwl::EvDev& d = ev->dev;
+  if (d.approx == GWL_HOST_POLARIZATIONS) {
+    set_error("gwl_calib_create: calibration marginalisation needs a source model the library "
+              "evaluates (sine-Gaussian or TaylorF2)");
+    return GWL_E_UNSUPPORTED;
+  }
   int K = 0;
   for (int k = 0; k < d.nd; ++k) K += d.mhi[k] - d.mlo[k] + 1;
   if (d.flags & GWL_TIME_MARG) return gwl::calib_create_time(ev, n_curves, curves, K, out);

@@ -89,6 +89,8 @@ struct EvDev {
   double tstep;  // spacing of the time grid, T / N = 2 / fs
   double tdelta; // bilby's _delta_tc
   SplineDev sp;
+  // GWL_HOST_POLARIZATIONS: the caller's [rows][2][nf] complex polarizations
+  const double2* pol;
 };
 
 // ------------------------------------------------------------ sidereal time, geometry
@@ -169,7 +171,7 @@ struct SampleConst {
   double jitter;
   int kb, ke;                   // waveform support, bins [kb, ke]
   int valid;
-  int pad;
+  int row;                      // the row's index into caller data (host polarizations)
 };
 
 // TaylorF2 3.5PN phase coefficients (non-spinning: Buonanno et al. 2009,
@@ -310,6 +312,7 @@ __device__ inline void compute_sample_const(const EvDev& ev, const double* row, 
   double ra = row[GWL_P_RA], dec = row[GWL_P_DEC], psi = row[GWL_P_PSI];
   double tc = row[GWL_P_GEOCENT_TIME], jit = row[GWL_P_TIME_JITTER];
   double dl = row[GWL_P_LUMINOSITY_DISTANCE];
+  sc.row = 0;
   bool timem = (ev.flags & GWL_TIME_MARG) != 0;
   bool jitter = timem && (ev.flags & GWL_JITTER_TIME);
   // base.py:427-428: geocent_time += time_jitter before the detector loop
@@ -347,6 +350,16 @@ __device__ inline void compute_sample_const(const EvDev& ev, const double* row, 
     int ke = last_bin_le(fmax, ev.df, lo, hi);
     sc.kb = max(kb, lo);
     sc.ke = min(ke, hi);
+  } else if (ev.approx == GWL_HOST_POLARIZATIONS) {
+    // the caller's polarizations: F+ and Fx (over sigma) per detector
+    for (int k = 0; k < 10; ++k) sc.c[k] = 0.0;
+    for (int d = 0; d < ev.nd; ++d) {
+      sc.cre[d] = fp[d] / ev.sigma;
+      sc.cim[d] = fc[d] / ev.sigma;
+      sc.cabs2[d] = 0.0;
+    }
+    sc.kb = lo;
+    sc.ke = hi;
   } else {  // sine-Gaussian, source.py:1153-1209
     double hrss = row[GWL_P_HRSS], Q = row[GWL_P_Q], f0 = row[GWL_P_FREQUENCY];
     if (!(Q > 0.0) || !(f0 > 0.0) || !isfinite(hrss)) sc.valid = 0;

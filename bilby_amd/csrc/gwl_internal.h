@@ -36,6 +36,7 @@ struct gwl_event {
   int tile_wg_per_cu = 0;
   int transform = GWL_TRANSFORM_AUTO;
   int multi_steps = 0;  // sub-sequence steps per multi-event work item (0: default)
+  int64_t pol_rows = 0;  // rows of the caller's polarizations (GWL_HOST_POLARIZATIONS)
   // float32 evaluations without time marginalisation re-evaluate in float64,
   // in the same call, every row whose float32 error bound esc_coeff * Q
   // exceeds esc_tol (Q: the row's inner-product scale); esc_coeff = 0: off

@@ -48,15 +48,18 @@ __device__ __forceinline__ long long live_rows(long long n, const int* count) {
 
 // One thread per row: the row constants every k_reduce tile of the row reads.
 // zero (optional) is cleared by thread 0: the escalation counter the
-// epilogue of the same evaluation fills.
+// epilogue of the same evaluation fills.  scatter (optional): the rows'
+// indices into the caller's data (the escalated rows of a float32 pass).
 __global__ __launch_bounds__(128) void k_sample_const(EvDev ev, const double* params, int ld,
                                                       long long n, SampleConst* out,
-                                                      const int* count, int* zero) {
+                                                      const int* count, int* zero,
+                                                      const int* scatter) {
   const long long r = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (zero && r == 0) *zero = 0;
   if (r >= live_rows(n, count)) return;
   SampleConst sc;
   compute_sample_const(ev, params + r * ld, sc);
+  sc.row = scatter ? scatter[r] : (int)r;
   out[r] = sc;
 }
 
@@ -97,7 +100,7 @@ __device__ __forceinline__ void reduce_chunk(const RedArgs& a, long long n, int 
       w[d] = load_w<R>(ev, d, i);
     }
     C I[S];
-    bin_rows<R, WAVE, ND, S>(n, (double)n * ev.df, dw, w, sc, sf, I, acc_h);
+    bin_rows<R, WAVE, ND, S>(n, (double)n * ev.df, dw, w, sc, sf, I, acc_h, ND, ev.pol, ev.nf);
 #pragma unroll
     for (int s = 0; s < S; ++s) {
       acc_r[s] += I[s].x;
@@ -126,7 +129,7 @@ __device__ __forceinline__ void reduce_chunk(const RedArgs& a, long long n, int 
       R h[S];
 #pragma unroll
       for (int s = 0; s < S; ++s) h[s] = 0;
-      bin_rows<R, WAVE, ND, S>(n, (double)n * ev.df, dw, w, sc, sf, I, h);
+      bin_rows<R, WAVE, ND, S>(n, (double)n * ev.df, dw, w, sc, sf, I, h, ND, ev.pol, ev.nf);
 #pragma unroll
       for (int s = 0; s < S; ++s) {
         dr[s] += I[s].x;
@@ -248,7 +251,7 @@ __device__ __forceinline__ void time_tile(const EvDev& ev, int nd, const double*
               w[d] = (R)0;
             }
           }
-          bin_rows<R, WAVE, ND, S>(nb, (double)nb * ev.df, dw, w, sc, sf, I, h, nd);
+          bin_rows<R, WAVE, ND, S>(nb, (double)nb * ev.df, dw, w, sc, sf, I, h, nd, ev.pol, ev.nf);
         } else {
 #pragma unroll
           for (int s = 0; s < S; ++s) I[s] = cmk<C>((R)0, (R)0);
@@ -296,7 +299,7 @@ __device__ __forceinline__ void time_tile(const EvDev& ev, int nd, const double*
     R h[S];
 #pragma unroll
     for (int s = 0; s < S; ++s) h[s] = 0;
-    bin_rows<R, WAVE, ND, S>(ev.N, (double)ev.N * ev.df, dw, w, sc, sf, I, h, nd);
+    bin_rows<R, WAVE, ND, S>(ev.N, (double)ev.N * ev.df, dw, w, sc, sf, I, h, nd, ev.pol, ev.nf);
 #pragma unroll
     for (int s = 0; s < S; ++s) hacc[s] += (double)h[s];
   }
@@ -681,6 +684,8 @@ __global__ void k_response(RespArgs a) {
     SampleConst one = sc[0];
     SampleFast<R> onef = sf[0];
     one.dt[0] = sc[0].dt[d];
+    one.cre[0] = sc[0].cre[d];
+    one.cim[0] = sc[0].cim[d];
     onef.cre[0] = sf[0].cre[d];
     onef.cim[0] = sf[0].cim[d];
     onef.cabs2[0] = sf[0].cabs2[d];
@@ -688,7 +693,7 @@ __global__ void k_response(RespArgs a) {
     R w[1] = {(R)0};
     C I[1];
     R h[1] = {(R)0};
-    bin_rows<R, WAVE, 1, 1>(n, (double)n * ev.df, dw, w, &one, &onef, I, h);
+    bin_rows<R, WAVE, 1, 1>(n, (double)n * ev.df, dw, w, &one, &onef, I, h, 1, ev.pol, ev.nf);
     re = (double)I[0].x * ev.sigma;
     im = (double)I[0].y * ev.sigma;
   }
@@ -890,7 +895,8 @@ static int launch_reduce(gwl_event* e, const double* params, int64_t n, int ld, 
   if (!count)
     if (int rc = timing_begin(e, st)) return rc;
   hipLaunchKernelGGL(k_sample_const, dim3((unsigned)((n + 127) / 128)), dim3(128), 0, st, ev,
-                     params, ld, (long long)n, ws.rows, count, esc ? ws.esc_count : nullptr);
+                     params, ld, (long long)n, ws.rows, count, esc ? ws.esc_count : nullptr,
+                     scatter);
   const dim3 grid(G, (unsigned)nblk);
   if (G == ra.nchunk) {
     if (S == 16)
@@ -1015,15 +1021,34 @@ size_t workspace_bytes(const gwl_event* e, int64_t rows) {
   return reduce_ws_bytes(ev, rows) + 4096;
 }
 
+// A host-polarization event needs the caller's polarizations for every row.
+static int check_polarizations(const gwl_event* e, int64_t n, const char* who) {
+  if (e->dev.approx != GWL_HOST_POLARIZATIONS) return GWL_OK;
+  if (!e->dev.pol || n > e->pol_rows) {
+    char msg[200];
+    snprintf(msg, sizeof msg, "%s: the event's source model is evaluated by the caller; "
+             "gwl_event_set_polarizations must hold at least %lld rows (it holds %lld)",
+             who, (long long)n, (long long)e->pol_rows);
+    set_error(msg);
+    return GWL_E_INVALID;
+  }
+  return GWL_OK;
+}
+
 int launch_eval(gwl_event* e, const double* params, int64_t n, int ld, int precision, double* out,
                 double* aux, hipStream_t st) {
   (void)hipGetLastError();  // clear a stale error left by an unrelated call
+  if (int rc = check_polarizations(e, n, "gwl_eval")) return rc;
   const int wave = e->dev.approx;
   if (precision == GWL_FP32) {
     if (wave == GWL_TAYLORF2) return eval_nd<float, GWL_TAYLORF2>(e, params, n, ld, out, aux, st);
+    if (wave == GWL_HOST_POLARIZATIONS)
+      return eval_nd<float, GWL_HOST_POLARIZATIONS>(e, params, n, ld, out, aux, st);
     return eval_nd<float, GWL_SINE_GAUSSIAN>(e, params, n, ld, out, aux, st);
   }
   if (wave == GWL_TAYLORF2) return eval_nd<double, GWL_TAYLORF2>(e, params, n, ld, out, aux, st);
+  if (wave == GWL_HOST_POLARIZATIONS)
+    return eval_nd<double, GWL_HOST_POLARIZATIONS>(e, params, n, ld, out, aux, st);
   return eval_nd<double, GWL_SINE_GAUSSIAN>(e, params, n, ld, out, aux, st);
 }
 
@@ -1091,12 +1116,17 @@ static int series_nd(gwl_event* e, const double* params, int64_t n, int ld, doub
 
 int launch_time_series(gwl_event* e, const double* params, int64_t n, int ld, int precision,
                        double* dh, double* hh, hipStream_t st) {
+  if (int rc = check_polarizations(e, n, "gwl_eval_time_series")) return rc;
   const int wave = e->dev.approx;
   if (precision == GWL_FP32) {
     if (wave == GWL_TAYLORF2) return series_nd<float, GWL_TAYLORF2>(e, params, n, ld, dh, hh, st);
+    if (wave == GWL_HOST_POLARIZATIONS)
+      return series_nd<float, GWL_HOST_POLARIZATIONS>(e, params, n, ld, dh, hh, st);
     return series_nd<float, GWL_SINE_GAUSSIAN>(e, params, n, ld, dh, hh, st);
   }
   if (wave == GWL_TAYLORF2) return series_nd<double, GWL_TAYLORF2>(e, params, n, ld, dh, hh, st);
+  if (wave == GWL_HOST_POLARIZATIONS)
+    return series_nd<double, GWL_HOST_POLARIZATIONS>(e, params, n, ld, dh, hh, st);
   return series_nd<double, GWL_SINE_GAUSSIAN>(e, params, n, ld, dh, hh, st);
 }
 
@@ -1241,14 +1271,21 @@ int launch_response(gwl_event* e, const double* params_dev, int det, int precisi
   ra.det = det;
   ra.out = out_dev;
   dim3 grid((e->dev.nf + 255) / 256);
+  if (int rc = check_polarizations(e, 1, "gwl_detector_response")) return rc;
+  const int wave = e->dev.approx;
   if (precision == GWL_FP32) {
-    if (e->dev.approx == GWL_TAYLORF2)
+    if (wave == GWL_TAYLORF2)
       hipLaunchKernelGGL((k_response<float, GWL_TAYLORF2>), grid, dim3(256), 0, st, ra);
+    else if (wave == GWL_HOST_POLARIZATIONS)
+      hipLaunchKernelGGL((k_response<float, GWL_HOST_POLARIZATIONS>), grid, dim3(256), 0, st, ra);
     else
       hipLaunchKernelGGL((k_response<float, GWL_SINE_GAUSSIAN>), grid, dim3(256), 0, st, ra);
   } else {
-    if (e->dev.approx == GWL_TAYLORF2)
+    if (wave == GWL_TAYLORF2)
       hipLaunchKernelGGL((k_response<double, GWL_TAYLORF2>), grid, dim3(256), 0, st, ra);
+    else if (wave == GWL_HOST_POLARIZATIONS)
+      hipLaunchKernelGGL((k_response<double, GWL_HOST_POLARIZATIONS>), grid, dim3(256), 0, st,
+                         ra);
     else
       hipLaunchKernelGGL((k_response<double, GWL_SINE_GAUSSIAN>), grid, dim3(256), 0, st, ra);
   }

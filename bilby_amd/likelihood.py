@@ -267,6 +267,10 @@ class GravitationalWaveTransient:
             time_marginalization = False
             jitter_time = False
         self.waveform_generator = waveform_generator
+        if calibration_marginalization and \
+                waveform_generator.approximant == _lib.HOST_POLARIZATIONS:
+            raise NotImplementedError("calibration marginalisation needs a source model evaluated "
+                                      "on the device (sine-Gaussian or TaylorF2)")
         self.interferometers = InterferometerList(interferometers)
         self.time_marginalization = time_marginalization
         self.distance_marginalization = distance_marginalization
@@ -587,7 +591,10 @@ class GravitationalWaveTransient:
                                                      precision="fp64", device=self.device)
         if self.calibration_marginalization and "recalib_index" in parameters:
             return self._calibrated_inner_products(parameters, int(parameters["recalib_index"]))
-        rows = self._plain.rows_from_parameters(dict(parameters))[:1]
+        if self._host_model:   # the caller's model reads the named parameters
+            rows = {k: np.atleast_1d(v) for k, v in parameters.items()}
+        else:
+            rows = self._plain.rows_from_parameters(dict(parameters))[:1]
         _, aux = self._plain.log_likelihood_ratio_batch(rows, return_inner_products=True)
         return complex(aux[0, 0], aux[0, 1]), float(aux[0, 2])
 
@@ -647,6 +654,9 @@ class GravitationalWaveTransient:
         (jittered) geocent time, modulo the segment, inside the prior; the
         time-conditioned lnL (distance / phase marginalised as configured)
         times the prior."""
+        if self._host_model:
+            raise NotImplementedError("the time reconstruction needs a source model evaluated on "
+                                      "the device (sine-Gaussian or TaylorF2)")
         p = self._frame(dict(parameters))
         for key, value in self._defaults().items():
             p.setdefault(key, value)
@@ -869,6 +879,77 @@ class GravitationalWaveTransient:
             return 2
         return _precision_code(precision)
 
+    # ------------------------------------------------------------------ host source models
+    @property
+    def _host_model(self):
+        return self.waveform_generator.approximant == _lib.HOST_POLARIZATIONS
+
+    def _model_dicts(self, parameters, rows):
+        """Per-row parameter dicts for a source model the caller evaluates:
+        a dict's entries row by row (with the marginalised keys' defaults),
+        or the named columns of parameter rows."""
+        n = rows.shape[0]
+        if isinstance(parameters, dict):
+            cols = {k: np.asarray(v) for k, v in parameters.items()}
+            out = []
+            for i in range(n):
+                d = {k: (v.item() if v.ndim == 0 else v[i].item()) for k, v in cols.items()}
+                for k, v in self._defaults().items():
+                    d.setdefault(k, v)
+                out.append(d)
+            return out
+        return [{k: float(rows[i, c]) for k, c in _lib.PARAM_COLUMNS.items()} for i in range(n)]
+
+    def _polarizations(self, dicts):
+        """waveform_generator.py:113-141 per row: the model's h+ and hx on
+        the frequency array, [n, 2, n_freq] complex, and the rows whose model
+        returned None (base.py:424-425 gives them -inf as nan_to_num)."""
+        wfg = self.waveform_generator
+        nf = len(self.interferometers.frequency_array)
+        pol = np.zeros((len(dicts), 2, nf), dtype=np.complex128)
+        invalid = np.zeros(len(dicts), dtype=bool)
+        for i, d in enumerate(dicts):
+            h = wfg.frequency_domain_strain(d)
+            if h is None:
+                invalid[i] = True
+                continue
+            pol[i, 0] = h["plus"]
+            pol[i, 1] = h["cross"]
+        return pol, invalid
+
+    def _host_batch(self, parameters, prec, return_inner_products, stream, chunk=256):
+        """log_likelihood_ratio_batch for a source model the caller
+        evaluates: per chunk of rows the model on the host, its polarizations
+        to the device (gwl_event_set_polarizations), then gwl_eval."""
+        on_device = isinstance(parameters, torch.Tensor) and parameters.is_cuda
+        if isinstance(parameters, dict):
+            rows = self.rows_from_parameters(parameters)
+        elif on_device:
+            rows = parameters.detach().to(torch.float64).cpu().numpy()
+        else:
+            rows = np.ascontiguousarray(parameters, dtype=np.float64)
+        dicts = self._model_dicts(parameters, rows)
+        dev = "cuda:%d" % self.device
+        outs, auxs, keep = [], [], []
+        for a in range(0, rows.shape[0], chunk):
+            pol, invalid = self._polarizations(dicts[a:a + chunk])
+            pol_t = torch.from_numpy(np.ascontiguousarray(pol).view(np.float64)).to(dev)
+            rows_t = torch.from_numpy(np.ascontiguousarray(rows[a:a + chunk])).to(dev)
+            keep.append(pol_t)
+            check(LIB.gwl_event_set_polarizations(self._event, ctypes.c_void_p(pol_t.data_ptr()),
+                                                  pol.shape[0]), "gwl_event_set_polarizations")
+            out, aux = self._device_eval(rows_t, prec, aux=return_inner_products, stream=stream)
+            if invalid.any():
+                out[torch.from_numpy(np.nonzero(invalid)[0]).to(dev)] = -np.finfo(float).max
+            outs.append(out)
+            auxs.append(aux)
+        out = torch.cat(outs)
+        aux = torch.cat(auxs) if return_inner_products else None
+        if on_device:
+            return (out, aux) if return_inner_products else out
+        out_h = out.cpu().numpy()
+        return (out_h, aux.cpu().numpy()) if return_inner_products else out_h
+
     def log_likelihood_ratio_batch(self, parameters, precision=None, return_inner_products=False,
                                    stream=None):
         """Batched ln L ratio over N rows.
@@ -887,6 +968,8 @@ class GravitationalWaveTransient:
                 return empty, (torch.empty((0, 3), dtype=torch.float64, device=parameters.device)
                                if on_device else np.zeros((0, 3)))
             return empty
+        if self._host_model:
+            return self._host_batch(parameters, prec, return_inner_products, stream)
         if on_device:
             rows_t = parameters.to(torch.float64).contiguous()
         else:
@@ -906,6 +989,9 @@ class GravitationalWaveTransient:
     def log_likelihood_ratio(self, parameters):
         """base.py:419-446: one parameter dict -> float (the caller's dict is
         not modified)."""
+        if self._host_model:
+            q = {k: np.atleast_1d(v) for k, v in parameters.items()}
+            return float(self._host_batch(q, self._mode(None), False, None)[0])
         rows = self.rows_from_parameters(copy.copy(parameters))
         if self._one is None:
             self._one = (torch.empty((1, _lib.NPARAM), dtype=torch.float64).pin_memory(),
@@ -925,6 +1011,14 @@ class GravitationalWaveTransient:
         rows = self.rows_from_parameters(parameters)
         row = np.ascontiguousarray(rows[0])
         out = np.zeros((len(self.interferometers.frequency_array), 2))
+        pol_t = None
+        if self._host_model:
+            pol, _ = self._polarizations(self._model_dicts(
+                {k: np.atleast_1d(v) for k, v in parameters.items()}, rows[:1]))
+            pol_t = torch.from_numpy(np.ascontiguousarray(pol).view(np.float64)).to(
+                "cuda:%d" % self.device)
+            check(LIB.gwl_event_set_polarizations(self._event, ctypes.c_void_p(pol_t.data_ptr()),
+                                                  1), "gwl_event_set_polarizations")
         check(LIB.gwl_detector_response(self._event, dptr(row), int(detector_index),
                                         _precision_code(precision), dptr(out)),
               "gwl_detector_response")

@@ -1,10 +1,16 @@
 """Source models and the waveform generator, as the likelihood sees them.
 
 bilby's WaveformGenerator (bilby/gw/waveform_generator.py:113-286) calls a
-Python source function on the host per parameter dict.  Here the source
-model is evaluated inside the fused GPU kernels; the generator only names
-it and carries its ``waveform_arguments`` and parameter conversion.
+Python source function on the host per parameter dict.  The built-in models
+(sine-Gaussian, TaylorF2) are evaluated inside the fused GPU kernels; the
+generator then only names them and carries their ``waveform_arguments`` and
+parameter conversion.  Any other ``frequency_domain_source_model`` is called
+on the host per sample, as bilby calls it, and its polarizations go to the
+GPU for the projection, inner products and marginalisations.
 """
+import functools
+import inspect
+
 import numpy as np
 
 from . import abi as _lib
@@ -169,7 +175,7 @@ def luminosity_distance_to_redshift(distance):
     return _cosmology().z_at_luminosity_distance(distance)
 
 
-def convert_to_lal_binary_black_hole_parameters(parameters):
+def convert_to_lal_binary_black_hole_parameters(parameters, aligned=True):
     """bilby/gw/conversion.py:182-283 for the aligned-spin TaylorF2 the
     device evaluates: luminosity distance from redshift or comoving distance;
     ``*_source`` masses times (1 + z); component masses from any complete
@@ -183,7 +189,8 @@ def convert_to_lal_binary_black_hole_parameters(parameters):
     TaylorF2 is a non-precessing approximant: lalsimulation refuses non-zero
     in-plane spins for it (XLAL_EINVAL, which bilby re-raises whatever
     catch_waveform_errors says, source.py:644-662), and so does this
-    conversion (ValueError)."""
+    conversion (ValueError) unless ``aligned`` is False (source models
+    evaluated by the caller, which take what they take)."""
     p = dict(parameters)
     original = list(p)
     if "luminosity_distance" not in original:
@@ -202,7 +209,7 @@ def convert_to_lal_binary_black_hole_parameters(parameters):
         chi, a, ct = "chi_" + idx, "a_" + idx, "cos_tilt_" + idx
         if chi in original:
             inplane = p.get(chi + "_in_plane", 0.0)
-            if np.any(np.asarray(inplane, float) != 0):
+            if aligned and np.any(np.asarray(inplane, float) != 0):
                 raise ValueError("TaylorF2 takes aligned spins: %s_in_plane must be zero" % chi)
             if a not in original:
                 p[a] = np.abs(np.asarray(p[chi], float))
@@ -230,7 +237,7 @@ def convert_to_lal_binary_black_hole_parameters(parameters):
             # bilby_to_lalsimulation_spins (conversion.py:106-166): the aligned
             # case needs a_i = 0 or tilt_i in {0, pi}; anything else has
             # in-plane spin components
-            if np.any(np.abs(amp * np.sin(tilt)) > 1e-12 * np.maximum(amp, 1.0)):
+            if aligned and np.any(np.abs(amp * np.sin(tilt)) > 1e-12 * np.maximum(amp, 1.0)):
                 raise ValueError("TaylorF2 takes aligned spins: tilt_%s must be 0 or pi" % idx)
             p[chi] = amp * np.cos(tilt)
     return p
@@ -251,6 +258,9 @@ class WaveformGenerator:
         self.waveform_arguments = dict(waveform_arguments or {})
         if parameter_conversion is None:
             parameter_conversion = convert_to_lal_binary_black_hole_parameters
+            if self.approximant == _lib.HOST_POLARIZATIONS:
+                parameter_conversion = functools.partial(
+                    convert_to_lal_binary_black_hole_parameters, aligned=False)
         self.parameter_conversion = parameter_conversion
 
     @property
@@ -265,9 +275,35 @@ class WaveformGenerator:
                 raise NotImplementedError(
                     "waveform_approximant %r is not available on the device" % approx)
             tag = _lib.TAYLORF2
+        if tag is None and callable(model):
+            return _lib.HOST_POLARIZATIONS
         if tag not in (_lib.SINE_GAUSSIAN, _lib.TAYLORF2):
             raise NotImplementedError("unsupported source model %r" % (model,))
         return tag
+
+    @property
+    def source_parameter_keys(self):
+        """waveform_generator.py:271-286 (core/utils/introspection.py): the
+        positional-or-keyword arguments of the source model after the
+        frequency array (``*args``, keyword-only and ``**kwargs`` dropped)."""
+        sig = inspect.signature(self.frequency_domain_source_model)
+        names = [p.name for p in sig.parameters.values() if p.kind == p.POSITIONAL_OR_KEYWORD]
+        return set(names[1:])
+
+    def frequency_domain_strain(self, parameters):
+        """waveform_generator.py:113-141, 196-212, 254-269 for a source model
+        the caller evaluates: the converted parameters the model names, plus
+        the waveform arguments, handed to the model with the frequency array;
+        its polarizations dict (or None) is returned as the model gives it."""
+        if not isinstance(parameters, dict):
+            raise TypeError('"parameters" must be a dictionary.')
+        p = self.parameter_conversion(dict(parameters))
+        if isinstance(p, tuple):  # bilby's conversions also return the added keys
+            p = p[0]
+        keys = self.source_parameter_keys
+        p = {k: v for k, v in p.items() if k in keys}
+        p.update(self.waveform_arguments)
+        return self.frequency_domain_source_model(self.frequency_array, **p)
 
     @property
     def frequency_array(self):

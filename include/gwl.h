@@ -49,7 +49,10 @@ enum gwl_param {
 
 /* Source models: bilby/gw/source.py:1153-1209 (sinegaussian) and the
  * TaylorF2 approximant reached through source.py:269-348, 552-690. */
-enum gwl_approximant { GWL_SINE_GAUSSIAN = 1, GWL_TAYLORF2 = 2 };
+/* GWL_HOST_POLARIZATIONS: the source model is evaluated by the caller (any
+ * bilby frequency_domain_source_model), which hands its polarizations over
+ * with gwl_event_set_polarizations before each evaluation. */
+enum gwl_approximant { GWL_SINE_GAUSSIAN = 1, GWL_TAYLORF2 = 2, GWL_HOST_POLARIZATIONS = 3 };
 
 /* Marginalisation flags: base.py:150-229 constructor arguments
  * time_marginalization, distance_marginalization, phase_marginalization,
@@ -150,6 +153,20 @@ int gwl_event_device_bytes(gwl_event* ev, int64_t* bytes);
  * base.py:424-425 does. */
 int gwl_eval(gwl_event* ev, const double* params, int64_t n_rows, int32_t ld,
              int32_t precision, double* out, double* aux, void* stream);
+
+/* Polarizations of the next evaluations of a GWL_HOST_POLARIZATIONS event:
+ * `polarizations` is device memory [n_rows][2][n_freq] of complex128 stored
+ * as (re, im) float64 pairs, h_plus then h_cross of row r on the event's
+ * whole frequency array (bins 0 .. n_freq - 1, as
+ * WaveformGenerator.frequency_domain_strain returns them,
+ * waveform_generator.py:113-141); row r of every later gwl_eval /
+ * gwl_eval_time_series call (n_rows at most) reads row r here.  The caller
+ * owns the buffer and keeps it until those evaluations have completed on
+ * their stream.  Replaces the source-model call of calculate_snrs
+ * (base.py:260-354) for source models the library does not evaluate; the
+ * projection (interferometer.py:303-368), inner products and
+ * marginalisations then run as for the built-in models. */
+int gwl_event_set_polarizations(gwl_event* ev, const double* polarizations, int64_t n_rows);
 
 /* gwl_eval that allocates nothing.  No counterpart in the reference: for
  * callers that cannot allocate device memory (code under HIP graph capture,
