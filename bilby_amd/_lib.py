@@ -184,6 +184,8 @@ def _load():
         "gwl_calib_device_bytes": ([vp, ctypes.POINTER(ctypes.c_int64)], ctypes.c_int),
         "gwl_calib_eval": ([vp, vp, ctypes.c_int64, ctypes.c_int32, ctypes.c_int32, vp, vp, vp],
                            ctypes.c_int),
+        "gwl_server_add_calib": ([vp, vp, ctypes.c_int32, ctypes.POINTER(ctypes.c_int32)],
+                                 ctypes.c_int),
         "gwl_server_add_relbin": ([vp, vp, ctypes.c_int32, ctypes.POINTER(ctypes.c_int32)],
                                   ctypes.c_int),
         "gwl_relbin_device_bytes": ([vp, ctypes.POINTER(ctypes.c_int64)], ctypes.c_int),
@@ -230,7 +232,7 @@ EXPORTED_SYMBOLS = [
     "gwl_server_add_roq", "gwl_server_add_group", "gwl_server_submit", "gwl_server_wait",
     "gwl_server_stats", "gwl_server_device_bytes", "gwl_event_set_transform",
     "gwl_server_listen", "gwl_relbin_create", "gwl_relbin_destroy", "gwl_relbin_device_bytes",
-    "gwl_relbin_eval", "gwl_server_add_relbin", "gwl_relbin_reserve", "gwl_calib_create",
+    "gwl_relbin_eval", "gwl_server_add_relbin", "gwl_server_add_calib", "gwl_relbin_reserve", "gwl_calib_create",
     "gwl_calib_destroy", "gwl_calib_reserve", "gwl_calib_device_bytes", "gwl_calib_eval",
     "gwl_mb_create", "gwl_mb_destroy", "gwl_mb_reserve", "gwl_mb_device_bytes", "gwl_mb_eval",
     "gwl_mb_set_time", "gwl_relbin_set_time", "gwl_eval_time_series", "gwl_event_time_window",

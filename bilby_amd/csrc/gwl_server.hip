@@ -36,7 +36,7 @@ namespace {
 
 using Clock = std::chrono::steady_clock;
 
-enum { UNIT_EVENT = 0, UNIT_ROQ = 1, UNIT_GROUP = 2, UNIT_RELBIN = 3 };
+enum { UNIT_EVENT = 0, UNIT_ROQ = 1, UNIT_GROUP = 2, UNIT_RELBIN = 3, UNIT_CALIB = 4 };
 
 struct Req {
   int id, unit, member;
@@ -62,6 +62,7 @@ struct Unit {
   int mode = 1;
   gwl_multi* multi = nullptr;
   gwl_relbin* rb = nullptr;
+  gwl_calib* cal = nullptr;
   int n_members = 1;
   hipStream_t stream = nullptr;
   hipEvent_t done = nullptr;
@@ -161,6 +162,9 @@ void launch(gwl_server* s, Unit* u, std::vector<Req>& reqs) {
   } else if (u->kind == UNIT_RELBIN) {
     rc = gwl_relbin_eval(u->rb, u->d_rows[0], rows[0], GWL_NPARAM, u->precision, u->d_out[0],
                          nullptr, u->stream);
+  } else if (u->kind == UNIT_CALIB) {
+    rc = gwl_calib_eval(u->cal, u->d_rows[0], rows[0], GWL_NPARAM, u->mode, u->d_out[0], nullptr,
+                        u->stream);
   } else {
     rc = gwl_multi_set_rows(u->multi, rows.data(), u->stream);
     if (rc == GWL_OK) rc = gwl_multi_launch(u->multi, u->stream);
@@ -479,6 +483,24 @@ int gwl_server_add_relbin(gwl_server* s, gwl_relbin* rb, int32_t precision, int3
   u->kind = UNIT_RELBIN;
   u->rb = rb;
   u->precision = precision;
+  if (int rc = new_unit(s, u, 1)) {
+    free_unit(u);
+    return rc;
+  }
+  return add_unit(s, u, 1, id);
+}
+
+int gwl_server_add_calib(gwl_server* s, gwl_calib* cal, int32_t mode, int32_t* id) {
+  if (!s || !cal || !id || mode < 0 || mode > 2) {
+    set_error("gwl_server_add_calib: invalid argument");
+    return GWL_E_INVALID;
+  }
+  GWL_HIP_CHECK(hipSetDevice(s->ctx->device));
+  if (int rc = gwl_calib_reserve(cal, s->max_rows)) return rc;
+  Unit* u = new Unit;
+  u->kind = UNIT_CALIB;
+  u->cal = cal;
+  u->mode = mode;
   if (int rc = new_unit(s, u, 1)) {
     free_unit(u);
     return rc;

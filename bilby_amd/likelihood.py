@@ -471,9 +471,17 @@ class GravitationalWaveTransient:
             for name, params in self.calibration_parameter_draws.items():
                 for key in set(params).intersection(priors.keys()):
                     priors[key] = DeltaFunction(0.0)
+        self._attach_calibration_curves(self.calibration_draws)
+
+    def _attach_calibration_curves(self, draws):
+        """The response curves (detector name -> complex [curves, masked
+        bins]) of this likelihood's detectors to the device event."""
+        self.calibration_draws = {ifo.name: np.asarray(draws[ifo.name], dtype=complex)
+                                  for ifo in self.interferometers}
         self.calibration_abs_draws = {k: np.abs(v) ** 2 for k, v in self.calibration_draws.items()}
-        curves = np.concatenate([np.asarray(self.calibration_draws[ifo.name], dtype=complex)
-                                 for ifo in self.interferometers], axis=1)
+        curves = np.concatenate([self.calibration_draws[ifo.name] for ifo in self.interferometers],
+                                axis=1)
+        self.number_of_response_curves = int(curves.shape[0])
         blob = np.ascontiguousarray(curves.view(np.float64))
         cal = ctypes.c_void_p()
         check(LIB.gwl_calib_create(self._event, int(curves.shape[0]), dptr(blob), ctypes.byref(cal)),
@@ -529,22 +537,30 @@ class GravitationalWaveTransient:
         """base.py:479-500: each detector's own log-likelihood ratio, with
         the time / distance / phase marginalisation of this likelihood, as
         parameters['<ifo>_log_likelihood'] (the dict is updated, as there,
-        and a copy returned).  Evaluated by one float64 single-detector
-        companion likelihood per detector on the GPU."""
-        if self.calibration_marginalization:
-            raise NotImplementedError("per-detector likelihoods with calibration marginalisation "
-                                      "are not provided")
+        and a copy returned).  Evaluated by one single-detector companion
+        likelihood per detector on the GPU (float64; float32 transforms with
+        calibration and time together); under calibration marginalisation
+        each companion holds this likelihood's curves of its detector, so a
+        detector's value is marginalised over its own curves
+        (calculate_snrs per interferometer, base.py:296-346, 860-877)."""
         if getattr(self, "_per_detector", None) is None:
-            self._per_detector = [
-                GravitationalWaveTransient(
+            prec = "fp32" if (self.calibration_marginalization and self.time_marginalization) \
+                else "fp64"
+            self._per_detector = []
+            for ifo in self.interferometers:
+                lk = GravitationalWaveTransient(
                     InterferometerList([ifo]), self.waveform_generator,
                     time_marginalization=self.time_marginalization,
                     distance_marginalization=self.distance_marginalization,
                     phase_marginalization=self.phase_marginalization,
                     priors=None if self.priors is None else self.priors.copy(),
                     jitter_time=self.jitter_time, reference_frame=self.reference_frame,
-                    time_reference=self.time_reference, precision="fp64", device=self.device)
-                for ifo in self.interferometers]
+                    time_reference=self.time_reference, precision=prec, device=self.device)
+                if self.calibration_marginalization:
+                    lk.calibration_marginalization = True
+                    lk._attach_calibration_curves({ifo.name: self.calibration_draws[ifo.name]})
+                    lk.calibration_mode = lk.precision
+                self._per_detector.append(lk)
         for ifo, lk in zip(self.interferometers, self._per_detector):
             parameters["%s_log_likelihood" % ifo.name] = lk.log_likelihood_ratio(dict(parameters))
         return parameters.copy()

@@ -40,6 +40,11 @@ def _device_bytes(lk):
     if getattr(lk, "_event", None) is not None:
         out = ctypes.c_int64()
         check(LIB.gwl_event_device_bytes(lk._event, ctypes.byref(out)), "gwl_event_device_bytes")
+        if getattr(lk, "_calib", None) is not None:
+            cal = ctypes.c_int64()
+            check(LIB.gwl_calib_device_bytes(lk._calib, ctypes.byref(cal)),
+                  "gwl_calib_device_bytes")
+            return int(out.value) + int(cal.value)
         return int(out.value)
     return int(getattr(lk, "hbm_bytes", 0))
 
@@ -136,8 +141,15 @@ class LikelihoodServer:
         budget; over budget raises HBMBudgetExceeded."""
         staging = self.max_batch_rows * (_lib.NPARAM + 1) * 8
         if getattr(likelihood, "_calib", None) is not None:
-            raise NotImplementedError("calibration-marginalised likelihoods are not served")
-        if getattr(likelihood, "_relbin", None) is not None:
+            check(LIB.gwl_calib_reserve(likelihood._calib, self.max_batch_rows),
+                  "gwl_calib_reserve")
+            nbytes = _device_bytes(likelihood)
+            self._admit(name, nbytes + staging)
+            i = ctypes.c_int32()
+            check(LIB.gwl_server_add_calib(self._s, likelihood._calib,
+                                           int(likelihood.calibration_mode), ctypes.byref(i)),
+                  "gwl_server_add_calib")
+        elif getattr(likelihood, "_relbin", None) is not None:
             check(LIB.gwl_relbin_reserve(likelihood._relbin, self.max_batch_rows),
                   "gwl_relbin_reserve")
             nbytes = _device_bytes(likelihood)

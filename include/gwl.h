@@ -605,6 +605,9 @@ int gwl_server_add_group(gwl_server* server, int32_t n_events, gwl_event* const*
 /* A relative-binning likelihood (gwl_relbin_create) as a unit; precision
  * GWL_FP32 or GWL_FP64. */
 int gwl_server_add_relbin(gwl_server* server, gwl_relbin* rb, int32_t precision, int32_t* id);
+/* A calibration-marginalised likelihood as a unit (gwl_calib_eval in `mode`,
+ * enum gwl_roq_mode); no counterpart in the reference. */
+int gwl_server_add_calib(gwl_server* server, gwl_calib* cal, int32_t mode, int32_t* id);
 /* rows: host [n][16] float64, out: host [n] float64, 1 <= n <= max_rows. */
 int gwl_server_submit(gwl_server* server, int32_t id, const double* rows, int64_t n_rows,
                       double* out, int64_t* ticket);

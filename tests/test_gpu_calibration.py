@@ -218,3 +218,44 @@ def test_reconstruction_with_calibration_and_other_marginalisations(time, tmp_pa
     assert abs(out["luminosity_distance"] - ref["luminosity_distance"]) < 1e-6 * ref[
         "luminosity_distance"]
     assert abs(out["phase"] - ref["phase"]) < 1e-6
+
+
+@pytest.mark.parametrize("time", [False, True])
+def test_per_detector_log_likelihoods_with_calibration(time, tmp_path):
+    """base.py:479-500 under calibration marginalisation: each detector's
+    lnL marginalised over its own response curves (calculate_snrs per
+    interferometer) equals the calibrated FP64 oracle on that detector alone
+    with the same curves (float64 1e-7; with time marginalisation float32
+    transforms, 1e-3)."""
+    from bilby_amd.likelihood import GravitationalWaveTransient
+    from oracle.calibration import OracleCalibrated
+    from oracle.likelihood import OracleDetector, OracleEvent
+    from tests.helpers import draws_as_dicts, fixed_marginalised
+    say("per-detector calibration, time", time)
+    s, files = _setup(tmp_path, n_curves=40)
+    priors = s["priors"].copy()
+    lk = GravitationalWaveTransient(
+        s["interferometers"], s["waveform_generator"], calibration_marginalization=True,
+        distance_marginalization=True, phase_marginalization=True, time_marginalization=time,
+        priors=priors, number_of_response_curves=40, calibration_lookup_table=files,
+        precision="fp32" if time else "fp64")
+    draws = fixed_marginalised(s["draws"], priors)
+    table = getattr(lk, "_dist_margd_loglikelihood_array", None)
+    tol = 1e-3 if time else 1e-7
+    for p in draws_as_dicts(draws, [0, 7]):
+        out = lk.compute_per_detector_log_likelihood(dict(p))
+        for ifo in s["interferometers"]:
+            det = OracleDetector(ifo.name, ifo.frequency_array, ifo.frequency_domain_strain,
+                                 ifo.power_spectral_density_array, ifo.frequency_mask)
+            ev = OracleEvent([det], s["interferometers"].duration,
+                             s["interferometers"].sampling_frequency,
+                             s["interferometers"].start_time)
+            olk = OracleCalibrated(ev, "taylorf2", {ifo.name: lk.calibration_draws[ifo.name]},
+                                   waveform_arguments=dict(s["waveform_generator"].waveform_arguments),
+                                   time_marginalization=time, distance_marginalization=True,
+                                   phase_marginalization=True, priors=dict(s["priors"]),
+                                   distance_table=table)
+            ref = olk.log_likelihood_ratio(dict(p))
+            got = out["%s_log_likelihood" % ifo.name]
+            say(ifo.name, got, ref)
+            assert abs(got - ref) < tol

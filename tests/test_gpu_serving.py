@@ -205,3 +205,48 @@ def test_serving_mix_of_the_three_unit_kinds_alone_equals_mix_and_oracle():
         err = np.max(np.abs(inmix[name][:3] - ref))
         say(name, "vs oracle", err)
         assert err < 1e-3, (name, err)
+
+
+def test_calibration_marginalised_likelihood_is_served(tmp_path):
+    """A calibration-marginalised likelihood as a serving unit
+    (gwl_server_add_calib): requests give the direct batched values bit for
+    bit, alone and interleaved with an uncalibrated event."""
+    from bilby_amd import calibration as cal
+    from bilby_amd.configs import make_taylorf2_marginalised
+    from bilby_amd.likelihood import GravitationalWaveTransient
+    from bilby_amd.serving import LikelihoodServer
+    from tests.helpers import fixed_marginalised
+    say("serving a calibration-marginalised likelihood")
+    s = make_taylorf2_marginalised(n_draws=64, seed=8, duration=4.0, sampling_frequency=1024.0,
+                                   detectors=("H1", "L1"))
+    priors = s["priors"]
+    for ifo in s["interferometers"]:
+        ifo.calibration_model = cal.CubicSpline("recalib_%s_" % ifo.name, 20.0, 512.0, 5)
+        priors.update(cal.constant_uncertainty_spline_priors(0.1, 0.1, 20.0, 512.0, 5, ifo.name))
+    files = {ifo.name: str(tmp_path / ("%s.npz" % ifo.name)) for ifo in s["interferometers"]}
+    p = priors.copy()
+    lk = GravitationalWaveTransient(s["interferometers"], s["waveform_generator"],
+                                    calibration_marginalization=True, distance_marginalization=True,
+                                    phase_marginalization=True, priors=p,
+                                    number_of_response_curves=50, calibration_lookup_table=files,
+                                    precision="fp32")
+    plain = GravitationalWaveTransient(s["interferometers"], s["waveform_generator"],
+                                       distance_marginalization=True, phase_marginalization=True,
+                                       priors=s["priors"].copy(), precision="fp32")
+    draws = fixed_marginalised(s["draws"], p)
+    rows = lk.rows_from_parameters(draws)
+    direct = lk.log_likelihood_ratio_batch(rows)
+    direct_plain = plain.log_likelihood_ratio_batch(rows)
+    server = LikelihoodServer(hbm_budget=50e9, max_batch_rows=128)
+    try:
+        server.register("cal", lk)
+        server.register("plain", plain)
+        futs = [(server.submit("cal", rows[a:a + 16]), a) for a in range(0, 64, 16)]
+        futs_p = [(server.submit("plain", rows[a:a + 16]), a) for a in range(0, 64, 16)]
+        for f, a in futs:
+            assert np.array_equal(f.result(), direct[a:a + 16])
+        for f, a in futs_p:
+            assert np.array_equal(f.result(), direct_plain[a:a + 16])
+        assert not np.array_equal(direct, direct_plain)
+    finally:
+        server.close()
