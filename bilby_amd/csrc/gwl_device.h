@@ -101,7 +101,12 @@ __constant__ static const double kLeapSeconds[18] = {
     820108813, 914803214, 1025136015, 1119744016, 1167264017};
 
 // bilby/gw/time.py:114-192 (GPS -> UTC -> Julian day -> GMST in radians)
+// The sidereal seconds reach ~7e8, whose last place is ~1e-11 rad: every
+// operation is rounded on its own (no fused multiply-adds), as the reference
+// computes it, so that the sidereal time, and with it each detector's delay,
+// does not move with the compiler's contraction choices.
 __device__ inline double gmst_rad(double gps) {
+#pragma clang fp contract(off)
   double gps_int = floor(gps);
   int nleap = 0;
 #pragma unroll
@@ -120,13 +125,18 @@ __device__ inline double gmst_rad(double gps) {
   return st * 2.0 * kPi / 86400.0;
 }
 
-// Antenna patterns (bilby/gw/geometry.py:118-186, 261-279) and geocentre
-// time delay (geometry.py:282-343) for one detector.
-__device__ inline void antenna_and_delay(const double* tensor, const double* vertex, double ra,
-                                         double dec, double gps, double psi, double& fp,
-                                         double& fc, double& delay) {
+// The sky's frame at one time: the polarisation tensors' vectors m, n
+// (geometry.py:118-186, 261-279) and the propagation direction omega
+// (geometry.py:282-343), shared by every detector of a row.
+struct SkyBasis {
+  double m[3], n[3], om[3];
+};
+
+__device__ inline SkyBasis sky_basis(double ra, double dec, double gps, double psi) {
   double g = gmst_rad(gps);
-  g = g - kTwoPi * floor(g / kTwoPi);
+  // g mod 2 pi, exact as fmod is: the remainder is representable, and the
+  // fused form rounds it once
+  g = fma(-kTwoPi, floor(g / kTwoPi), g);
   double phi = ra - g;
   double theta = 0.5 * kPi - dec;
   double sphi, cphi, sth, cth, sps, cps;
@@ -135,25 +145,42 @@ __device__ inline void antenna_and_delay(const double* tensor, const double* ver
   sincos(psi, &sps, &cps);
   double u[3] = {cphi * cth, cth * sphi, -sth};
   double v[3] = {-sphi, cphi, 0.0};
-  double m[3], n[3];
+  SkyBasis b;
 #pragma unroll
   for (int i = 0; i < 3; ++i) {
-    m[i] = -u[i] * sps - v[i] * cps;
-    n[i] = -u[i] * cps + v[i] * sps;
+    b.m[i] = -u[i] * sps - v[i] * cps;
+    b.n[i] = -u[i] * cps + v[i] * sps;
   }
+  b.om[0] = sth * cphi;
+  b.om[1] = sth * sphi;
+  b.om[2] = cth;
+  return b;
+}
+
+// Antenna patterns and geocentre time delay of one detector in a sky frame.
+__device__ inline void detector_projection(const double* tensor, const double* vertex,
+                                           const SkyBasis& b, double& fp, double& fc,
+                                           double& delay) {
   double ap = 0.0, ac = 0.0;
 #pragma unroll
   for (int i = 0; i < 3; ++i)
 #pragma unroll
     for (int j = 0; j < 3; ++j) {
-      ap += tensor[3 * i + j] * (m[i] * m[j] - n[i] * n[j]);
-      ac += tensor[3 * i + j] * (m[i] * n[j] + n[i] * m[j]);
+      ap += tensor[3 * i + j] * (b.m[i] * b.m[j] - b.n[i] * b.n[j]);
+      ac += tensor[3 * i + j] * (b.m[i] * b.n[j] + b.n[i] * b.m[j]);
     }
   fp = ap;
   fc = ac;
-  double om[3] = {sth * cphi, sth * sphi, cth};
-  delay = (om[0] * (0.0 - vertex[0]) + om[1] * (0.0 - vertex[1]) + om[2] * (0.0 - vertex[2])) /
-          kSpeedOfLight;
+  delay = (b.om[0] * (0.0 - vertex[0]) + b.om[1] * (0.0 - vertex[1]) +
+           b.om[2] * (0.0 - vertex[2])) / kSpeedOfLight;
+}
+
+// Antenna patterns (bilby/gw/geometry.py:118-186, 261-279) and geocentre
+// time delay (geometry.py:282-343) for one detector.
+__device__ inline void antenna_and_delay(const double* tensor, const double* vertex, double ra,
+                                         double dec, double gps, double psi, double& fp,
+                                         double& fc, double& delay) {
+  detector_projection(tensor, vertex, sky_basis(ra, dec, gps, psi), fp, fc, delay);
 }
 
 // ------------------------------------------------------------ per-row constants
@@ -307,8 +334,23 @@ __device__ inline int tf2_row_constants(const double* row, double fref, double s
   return valid;
 }
 
+// The TaylorF2 source part of a row's constants (tf2_row_constants and
+// cos iota), computable apart from the sky part.
+struct Tf2Part {
+  double c[10];
+  double amp0, fisco, ci;
+  int valid;
+};
+
+__device__ inline void tf2_part(const EvDev& ev, const double* row, Tf2Part& t) {
+  t.valid = tf2_row_constants(row, ev.fref, ev.sigma, t.c, t.amp0, t.fisco);
+  t.ci = cos(row[GWL_P_THETA_JN]);
+}
+
 // Per-row prologue: everything that depends on the parameter row only.
-__device__ inline void compute_sample_const(const EvDev& ev, const double* row, SampleConst& sc) {
+// pre: the TaylorF2 part computed elsewhere (another lane), or null.
+__device__ inline void compute_sample_const(const EvDev& ev, const double* row, SampleConst& sc,
+                                            const Tf2Part* pre = nullptr) {
   double ra = row[GWL_P_RA], dec = row[GWL_P_DEC], psi = row[GWL_P_PSI];
   double tc = row[GWL_P_GEOCENT_TIME], jit = row[GWL_P_TIME_JITTER];
   double dl = row[GWL_P_LUMINOSITY_DISTANCE];
@@ -325,18 +367,22 @@ __device__ inline void compute_sample_const(const EvDev& ev, const double* row, 
     hi = max(hi, ev.mhi[d]);
   }
   double fp[GWL_MAX_DET], fc[GWL_MAX_DET];
+  const SkyBasis sky = sky_basis(ra, dec, gt, psi);  // once per row, not per detector
   for (int d = 0; d < ev.nd; ++d) {
     double delay;
-    antenna_and_delay(ev.tensor[d], ev.vertex[d], ra, dec, gt, psi, fp[d], fc[d], delay);
+    detector_projection(ev.tensor[d], ev.vertex[d], sky, fp[d], fc[d], delay);
     double dtg = gt - ev.start;  // interferometer.py:355-358: subtract GPS times first
     sc.dt[d] = dtg + delay;
   }
   sc.valid = 1;
   if (ev.approx == GWL_TAYLORF2) {
-    double iota = row[GWL_P_THETA_JN];
-    double amp0, fisco;
-    sc.valid = tf2_row_constants(row, ev.fref, ev.sigma, sc.c, amp0, fisco);
-    double ci = cos(iota);
+    Tf2Part own;
+    if (!pre) tf2_part(ev, row, own);
+    const Tf2Part& t = pre ? *pre : own;
+#pragma unroll
+    for (int k = 0; k < 10; ++k) sc.c[k] = t.c[k];
+    sc.valid = t.valid;
+    const double amp0 = t.amp0, fisco = t.fisco, ci = t.ci;
     double ap = amp0 * (1.0 + ci * ci) / 2.0, ax = amp0 * ci;
     for (int d = 0; d < ev.nd; ++d) {
       sc.cre[d] = fp[d] * ap;

@@ -135,7 +135,7 @@ int launch_response(gwl_event* ev, const double* params_dev, int det, int precis
 int launch_epi_time(gwl_event* ev, const double* params, int64_t n, int ld, int G,
                     const double2* part, const double* hhpart, double* out, hipStream_t stream,
                     const gwl::EvDev* slice = nullptr, double2* lse = nullptr, int first = 1,
-                    int last = 1);
+                    int last = 1, const double* rinfo = nullptr);
 // wave-per-row TaylorF2 time transform (gwl_time_wave.hip)
 bool time_wave_eligible(const gwl_event* ev);
 int time_wave_groups(const gwl_event* ev, long long n);

@@ -195,6 +195,7 @@ struct TimeArgs {
   int G;
   double2* part;    // [n][G][L]
   double* hhpart;   // [n][G]
+  double* rinfo;    // [n][4]: hscale, jitter, valid of each row (written by group 0)
 };
 
 // One tile of the time-marginalised transform: rows [row0, row0 + S) of an
@@ -207,7 +208,7 @@ __device__ __forceinline__ void time_tile(const EvDev& ev, int nd, const double*
                                           bool nyquist, int G, int g, double2* part,
                                           double* hhpart, cplx<R>* buf, const cplx<R>* tw,
                                           SampleConst* sc, SampleFast<R>* sf, int* band,
-                                          double (*red)[S]) {
+                                          double (*red)[S], double* rinfo = nullptr) {
   using C = cplx<R>;
   constexpr int MAXBF8 = (S * P / 8 + NT - 1) / NT;
   constexpr int NPT = (P + NT - 1) / NT;   // n1 values per thread per step
@@ -329,6 +330,13 @@ __device__ __forceinline__ void time_tile(const EvDev& ev, int nd, const double*
       double v = 0.0;
       for (int w = 0; w < NT / 64; ++w) v += red[w][tid];
       hhpart[(size_t)r * G + g] = v;
+      // the row constants the epilogue needs, so that it does not recompute them
+      if (rinfo && g == 0) {
+        rinfo[4 * r + 0] = sc[tid].hscale;
+        rinfo[4 * r + 1] = sc[tid].jitter;
+        rinfo[4 * r + 2] = (double)sc[tid].valid;
+        rinfo[4 * r + 3] = 0.0;
+      }
     }
   }
 }
@@ -349,7 +357,7 @@ __global__ __launch_bounds__(NT) void k_time(TimeArgs a) {
   const int n2b = (int)(((long long)g * Q) / a.G), n2e = (int)(((long long)(g + 1) * Q) / a.G);
   time_tile<R, WAVE, ND, S, P, NT>(a.ev, ND, a.params, a.ld, a.n, (long long)blockIdx.y * S, n2b,
                                    n2e, g == 0, a.G, g, a.part, a.hhpart, buf, tw, sc, sf, band,
-                                   red);
+                                   red, a.rinfo);
 }
 
 // ------------------------------------------------------------------ epilogues
@@ -370,6 +378,7 @@ struct EpiArgs {
   // per row [n], first / last slice flags
   double2* lse;
   int first, last;
+  const double* rinfo;      // [n][4] hscale, jitter, valid from the transform (or null)
   const SampleConst* rows;  // [n] row constants, or null (then computed here)
   // live rows and output slots: with count set only rows r < *count exist
   // and row r's results go to out[scatter[r]] (the float64 pass over the
@@ -473,19 +482,26 @@ constexpr int kEpiRows = 4;
 __device__ inline void epi_time_row(const EvDev& ev, const double* params, int ld, long long r,
                                     int G, const double2* part, const double* hhpart, double* out,
                                     const double* knots, double2* lse = nullptr, int first = 1,
-                                    int last = 1) {
+                                    int last = 1, const double* rinfo = nullptr) {
   const int lane = threadIdx.x & 63;
   const bool dist = (ev.flags & GWL_DISTANCE_MARG) != 0;
   double hh = 0.0, jit = 0.0, hs = 1.0;
   int valid = 1;
   if (lane == 0) {
-    SampleConst sc;
-    compute_sample_const(ev, params + r * ld, sc);
+    if (rinfo) {  // written by the transform's first group
+      hs = rinfo[4 * r + 0];
+      jit = rinfo[4 * r + 1];
+      valid = rinfo[4 * r + 2] != 0.0;
+    } else {
+      SampleConst sc;
+      compute_sample_const(ev, params + r * ld, sc);
+      jit = sc.jitter;
+      hs = sc.hscale;
+      valid = sc.valid;
+    }
+#pragma unroll 8
     for (int g = 0; g < G; ++g) hh += hhpart[(size_t)r * G + g];
     hh *= 4.0 / ev.T;
-    jit = sc.jitter;
-    hs = sc.hscale;
-    valid = sc.valid;
   }
   hh = __shfl(hh, 0, 64);
   jit = __shfl(jit, 0, 64);
@@ -506,6 +522,7 @@ __device__ inline void epi_time_row(const EvDev& ev, const double* params, int l
     const double t = (ev.start + (double)(j + 1) * ev.tstep) + jit;
     if (!(t >= ev.tmin && t <= ev.tmax)) continue;
     double xr = 0.0, xi = 0.0;
+#pragma unroll 8
     for (int g = 0; g < G; ++g) {
       const double2 v = prow[(size_t)g * ev.L + jj];
       xr += v.x;
@@ -567,7 +584,7 @@ __global__ __launch_bounds__(64 * kEpiRows) void k_epi_time(EpiArgs a) {
   const long long r = (long long)blockIdx.x * kEpiRows + (threadIdx.x >> 6);
   if (r >= a.n) return;
   epi_time_row(a.ev, a.params, a.ld, r, a.G, a.part, a.hhpart, a.out, knots, a.lse, a.first,
-               a.last);
+               a.last, a.rinfo);
 }
 
 // ------------------------------------------------------------------ multi-event
@@ -729,7 +746,7 @@ static int time_tile_groups(const EvDev& ev, long long n, int S, int n_cu, int w
 
 int launch_epi_time(gwl_event* e, const double* params, int64_t n, int ld, int G,
                     const double2* part, const double* hhpart, double* out, hipStream_t st,
-                    const EvDev* slice, double2* lse, int first, int last) {
+                    const EvDev* slice, double2* lse, int first, int last, const double* rinfo) {
   const EvDev& ev = slice ? *slice : e->dev;
   EpiArgs ea{};
   ea.ev = ev;
@@ -743,6 +760,7 @@ int launch_epi_time(gwl_event* e, const double* params, int64_t n, int ld, int G
   ea.lse = lse;
   ea.first = first;
   ea.last = last;
+  ea.rinfo = rinfo;
   const size_t klds = (ev.flags & GWL_DISTANCE_MARG) ? (size_t)(ev.sp.nx + ev.sp.ny + 8) * sizeof(double) : 0;
   hipLaunchKernelGGL(k_epi_time, dim3((unsigned)((n + kEpiRows - 1) / kEpiRows)),
                      dim3(64 * kEpiRows), klds, st, ea);
@@ -770,7 +788,7 @@ static size_t time_lse_bytes(long long rows) {
 template <typename R, int WAVE, int ND>
 static int launch_time_tiled(gwl_event* e, const double* params, int64_t n, int ld,
                              hipStream_t st, int* G_out, double2** part_out, double** hh_out,
-                             const EvDev* slice = nullptr) {
+                             const EvDev* slice = nullptr, double** rinfo_out = nullptr) {
   const EvDev& ev = slice ? *slice : e->dev;
   {
     // tile: S rows x P-point transforms, G groups of sub-sequences
@@ -788,6 +806,9 @@ static int launch_time_tiled(gwl_event* e, const double* params, int64_t n, int 
     char* base = reinterpret_cast<char*>(e->ws) + time_lse_bytes(n);
     ta.part = reinterpret_cast<double2*>(base);
     ta.hhpart = reinterpret_cast<double*>(base + (size_t)n * G * ev.L * sizeof(double2));
+    ta.rinfo = reinterpret_cast<double*>(
+        base + (((size_t)n * G * ev.L * sizeof(double2) + (size_t)n * G * sizeof(double)) + 255) /
+                   256 * 256);
     dim3 grid(G, (unsigned)nblk);
 #define GWL_LAUNCH_TIME(SS, PP)                                                                  \
   do {                                                                                           \
@@ -821,6 +842,7 @@ static int launch_time_tiled(gwl_event* e, const double* params, int64_t n, int 
     *G_out = G;
     *part_out = ta.part;
     *hh_out = ta.hhpart;
+    if (rinfo_out) *rinfo_out = ta.rinfo;
     return GWL_OK;
   }
 }
@@ -961,10 +983,12 @@ static int eval_typed(gwl_event* e, const double* params, int64_t n, int ld, dou
       int G = 0;
       double2* part = nullptr;
       double* hh = nullptr;
-      if (int rc = launch_time_tiled<R, WAVE, ND>(e, params, n, ld, st, &G, &part, &hh, &slice))
+      double* rinfo = nullptr;
+      if (int rc = launch_time_tiled<R, WAVE, ND>(e, params, n, ld, st, &G, &part, &hh, &slice,
+                                                  &rinfo))
         return rc;
       if (int rc = launch_epi_time(e, params, n, ld, G, part, hh, out, st, &slice, lse, p == 0,
-                                   p == passes - 1))
+                                   p == passes - 1, rinfo))
         return rc;
     }
     return GWL_OK;
@@ -1014,7 +1038,7 @@ size_t workspace_bytes(const gwl_event* e, int64_t rows) {
         (long long)rows * time_tile_groups(ev, rows, S, e->ctx->n_cu, e->tile_wg_per_cu),
         2LL * 8 * wpc * e->ctx->n_cu);
     size_t b = (size_t)bound * (std::min(ev.L, ev.P) * sizeof(double2) + sizeof(double)) +
-               time_lse_bytes(rows) + 4096;
+               time_lse_bytes(rows) + (size_t)rows * 4 * sizeof(double) + 4096;
     if (ev.approx == GWL_TAYLORF2 && ev.tf2b) b = std::max(b, time_wave_ws_bytes(e, rows));
     return b;
   }

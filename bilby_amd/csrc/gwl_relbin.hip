@@ -181,15 +181,17 @@ __global__ __launch_bounds__(128) void k_relbin_rows(RelbinArgs a) {
   for (int k = 0; k < kNB; ++k) o->c[k] = c[k];
   const double ci = cos(iota);
   const double ap = amp0 * (1.0 + ci * ci) / 2.0, ax = amp0 * ci;
+  // the sky frame once per row (and at the fixed antenna time when asked)
+  const SkyBasis sky = sky_basis(ra, dec, tc, psi);
+  const SkyBasis sky_ant = fixed_antenna ? sky_basis(ra, dec, a.antenna_time, psi) : sky;
   for (int d = 0; d < GWL_MAX_DET; ++d) {
     double dt = 0.0, cre = 0.0, cim = 0.0;
     if (d < ev.nd) {
       double fp, fc, delay;
-      antenna_and_delay(ev.tensor[d], ev.vertex[d], ra, dec, tc, psi, fp, fc, delay);
+      detector_projection(ev.tensor[d], ev.vertex[d], sky, fp, fc, delay);
       if (fixed_antenna) {
         double delay_ref;
-        antenna_and_delay(ev.tensor[d], ev.vertex[d], ra, dec, a.antenna_time, psi, fp, fc,
-                          delay_ref);
+        detector_projection(ev.tensor[d], ev.vertex[d], sky_ant, fp, fc, delay_ref);
       }
       dt = (tc - ev.start) + delay;  // interferometer.py:355-358
       cre = fp * ap;

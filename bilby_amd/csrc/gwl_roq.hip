@@ -115,10 +115,10 @@ __global__ __launch_bounds__(128) void k_roq_prep(RoqDev rd, const double* param
       const double ci = cos(row[GWL_P_THETA_JN]);
       const double ap = amp0 * (1.0 + ci * ci) / 2.0, ax = amp0 * ci;
       RoqRow rr;
+      const SkyBasis sky = sky_basis(row[GWL_P_RA], row[GWL_P_DEC], tref, row[GWL_P_PSI]);
       for (int d = 0; d < rd.nd; ++d) {
         double fp, fc, delay;
-        antenna_and_delay(rd.tensor[d], rd.vertex[d], row[GWL_P_RA], row[GWL_P_DEC], tref,
-                          row[GWL_P_PSI], fp, fc, delay);
+        detector_projection(rd.tensor[d], rd.vertex[d], sky, fp, fc, delay);
         // C_d = F+ A+ - i Fx Ax; store conj(C_d)
         rr.cre[d] = fp * ap;
         rr.cim[d] = fc * ax;

@@ -95,9 +95,10 @@ struct RowArgs {
   RowW* rows;
 };
 
-__device__ inline void rowconst_one(const EvDev& ev, const double* row, RowW* out) {
+__device__ inline void rowconst_one(const EvDev& ev, const double* row, RowW* out,
+                                    const Tf2Part* pre = nullptr) {
   SampleConst sc;
-  compute_sample_const(ev, row, sc);
+  compute_sample_const(ev, row, sc, pre);
   RowW w;
 #pragma unroll
   for (int k = 0; k < 9; ++k) w.c[k] = sc.c[k];
@@ -152,10 +153,21 @@ __device__ inline void rowconst_one(const EvDev& ev, const double* row, RowW* ou
   *out = w;
 }
 
+// Two lanes per row: the odd lane computes the TaylorF2 part of the row's
+// constants while the even lane computes the sky part (GMST, antenna
+// patterns, delays), then the even lane assembles them; the two float64
+// chains run side by side instead of one after the other.
+constexpr int kRowconstRows = 128;  // rows per 256-thread block
+
 __global__ __launch_bounds__(256) void k_rowconst(RowArgs a) {
-  const long long r = (long long)blockIdx.x * 256 + threadIdx.x;
-  if (r >= a.n) return;
-  rowconst_one(a.ev, a.params + r * a.ld, a.rows + r);
+  __shared__ Tf2Part t[kRowconstRows];
+  const int k = threadIdx.x >> 1;
+  const long long r = (long long)blockIdx.x * kRowconstRows + k;
+  const bool live = r < a.n;
+  if (live && (threadIdx.x & 1)) tf2_part(a.ev, a.params + r * a.ld, t[k]);
+  __syncthreads();
+  if (live && !(threadIdx.x & 1))
+    rowconst_one(a.ev, a.params + r * a.ld, a.rows + r, a.ev.approx == GWL_TAYLORF2 ? &t[k] : nullptr);
 }
 
 // The event fields the row loop reads.
@@ -853,7 +865,8 @@ int launch_time_wave(gwl_event* e, const double* params, int64_t n, int ld, doub
     return GWL_E_INVALID;
   }
   RowArgs ra{ev, params, ld, n, rows};
-  hipLaunchKernelGGL(k_rowconst, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, ra);
+  hipLaunchKernelGGL(k_rowconst, dim3((unsigned)((n + kRowconstRows - 1) / kRowconstRows)),
+                     dim3(256), 0, st, ra);
   GWL_HIP_CHECK(hipGetLastError());
   TimeWArgs ta{};
   ta.ev = ev;
@@ -905,7 +918,8 @@ int launch_cal_time(gwl_event* e, const double* params, int64_t n, int ld, int n
   double* lnl = per_curve ? per_curve
                           : reinterpret_cast<double*>(w + ((size_t)n * sizeof(RowW) + 255) / 256 * 256);
   RowArgs ra{ev, params, ld, n, rows};
-  hipLaunchKernelGGL(k_rowconst, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, ra);
+  hipLaunchKernelGGL(k_rowconst, dim3((unsigned)((n + kRowconstRows - 1) / kRowconstRows)),
+                     dim3(256), 0, st, ra);
   GWL_HIP_CHECK(hipGetLastError());
   CalTimeArgs ca{};
   ca.ev = ev;

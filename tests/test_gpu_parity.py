@@ -246,9 +246,12 @@ def test_fp32_escalation_meets_1e3_on_every_config1_row():
 
 def test_sine_gaussian_plain_lnl():
     """Plain likelihood (no marginalisation), sine-Gaussian, H1+L1, 4 s at
-    2048 Hz, 256 prior draws.  fp64: |dlnL| < 1e-6 (|lnL| up to ~1e5);
-    fp32 (rows beyond the float32 error bound escalated to float64 inside
-    gwl_eval): |dlnL| < 1e-3.  Inner products also checked."""
+    2048 Hz, 256 prior draws.  fp64: |dlnL| < 1e-6 + 1e-10 |lnL| (|lnL| up
+    to ~1.4e5, where lnL is the difference of inner products of that size
+    summed in different orders by the two codes: float64 agreement measured
+    at 1e-11 relative); fp32 (rows beyond the float32 error bound escalated
+    to float64 inside gwl_eval): |dlnL| < 1e-3.  Inner products also
+    checked."""
     from bilby_amd.likelihood import GravitationalWaveTransient
     from tests.helpers import oracle_from_setup, draws_as_dicts
     say("sine-Gaussian plain lnL")
@@ -263,7 +266,7 @@ def test_sine_gaussian_plain_lnl():
     out32 = lk.log_likelihood_ratio_batch(d, precision="fp32")
     say("fp64 max", np.max(np.abs(out64 - ref)), "fp32 max", np.max(np.abs(out32 - ref)),
         "max|lnL|", np.max(np.abs(ref)))
-    assert np.all(np.abs(out64 - ref) < 1e-6)
+    assert np.all(np.abs(out64 - ref) < 1e-6 + 1e-10 * np.abs(ref))
     assert np.all(np.abs(out32 - ref) < 1e-3)
     # noise log-likelihood, base.py:402-413
     assert abs(lk.noise_log_likelihood() - olk.noise_log_likelihood()) < 1e-6
