@@ -64,6 +64,14 @@ class MbTimeDesc(ctypes.Structure):
                 ("t_step", ctypes.c_double), ("antenna_time", ctypes.c_double)]
 
 
+class MbFftBand(ctypes.Structure):
+    """gwl_mb_fft_band (include/gwl.h)."""
+    _fields_ = [("half_length", ctypes.c_int32), ("first_point", ctypes.c_int32),
+                ("n_points", ctypes.c_int32), ("first_bin", ctypes.c_int32),
+                ("sqrt_window", ctypes.POINTER(ctypes.c_double)),
+                ("odd_weights", ctypes.POINTER(ctypes.c_double))]
+
+
 class GWLError(RuntimeError):
     pass
 
@@ -155,6 +163,7 @@ def _load():
         "gwl_mb_create": ([vp, ctypes.c_int32, _dp, _dp, _dp, ctypes.POINTER(vp)], ctypes.c_int),
         "gwl_mb_destroy": ([vp], ctypes.c_int),
         "gwl_mb_set_time": ([vp, ctypes.POINTER(MbTimeDesc)], ctypes.c_int),
+        "gwl_mb_set_fft": ([vp, ctypes.c_int32, ctypes.POINTER(MbFftBand)], ctypes.c_int),
         "gwl_eval_time_series": ([vp, vp, ctypes.c_int64, ctypes.c_int32, ctypes.c_int32, vp, vp,
                                   vp], ctypes.c_int),
         "gwl_event_time_window": ([vp, ctypes.POINTER(ctypes.c_int64),
@@ -235,7 +244,7 @@ EXPORTED_SYMBOLS = [
     "gwl_relbin_eval", "gwl_server_add_relbin", "gwl_server_add_calib", "gwl_relbin_reserve", "gwl_calib_create",
     "gwl_calib_destroy", "gwl_calib_reserve", "gwl_calib_device_bytes", "gwl_calib_eval",
     "gwl_mb_create", "gwl_mb_destroy", "gwl_mb_reserve", "gwl_mb_device_bytes", "gwl_mb_eval",
-    "gwl_mb_set_time", "gwl_relbin_set_time", "gwl_eval_time_series", "gwl_event_time_window",
+    "gwl_mb_set_time", "gwl_mb_set_fft", "gwl_relbin_set_time", "gwl_eval_time_series", "gwl_event_time_window",
     "gwl_roq_weights_linear", "gwl_relbin_time_series", "gwl_mb_time_series",
     "gwl_roq_inner_products", "gwl_relbin_set_upsampled", "gwl_relbin_upsampled_series",
     "gwl_weighted_inner_products", "gwl_window_time_series", "gwl_event_set_escalation",

@@ -38,6 +38,7 @@
 #include <algorithm>
 #include <cstddef>
 #include <cstring>
+#include <string>
 #include <vector>
 
 using gwl::set_error;
@@ -73,9 +74,23 @@ struct gwl_relbin {
   bool has_up = false;
 };
 
-// Multibanded likelihood (bilby/gw/likelihood/multiband.py:728-807, linear
-// interpolation): per banded point the frequency, TaylorF2 phase basis,
-// f^-7/6, and per detector the linear (complex) and quadratic coefficients.
+namespace gwl {
+// One band of the IFFT-FFT <h|h> (gwl_mb_set_fft), on the device: the
+// band's M' = M_b / 2 point transforms split as M' = R P (P points in LDS).
+struct MbFftBand {
+  int mh, P, logP, R;     // M', LDS transform length, log2 P, M' / P
+  int p0, L, ks;          // banded points [p0, p0 + L) at bins ks .. ks + L - 1
+  const double2* tab;     // [4 M'] exp(-2 pi i k / (4 M'))
+  const double* odd;      // [nd][M'] (4 / T^_b) Re Ibc_b[2 j + 1] / M'^2
+  const double* sqw;      // [L] square root of the band's window at its points
+};
+}  // namespace gwl
+
+// Multibanded likelihood (bilby/gw/likelihood/multiband.py:728-807): per
+// banded point the frequency, TaylorF2 phase basis, f^-7/6, and per detector
+// the linear (complex) and quadratic coefficients; for the IFFT-FFT <h|h>
+// the quadratic coefficients hold the diagonal part and the bands' odd-bin
+// transforms are added by k_mb_fft_lds / k_mb_fft_split.
 struct gwl_mb {
   gwl_event* ev = nullptr;
   int n = 0;
@@ -94,6 +109,10 @@ struct gwl_mb {
   bool has_win = false;
   void* tws = nullptr;
   size_t tws_bytes = 0;
+  std::vector<gwl::MbFftBand> fft;   // IFFT-FFT bands (empty: linear interpolation)
+  void* fft_data = nullptr;          // their tables
+  void* fft_ws = nullptr;            // per-band row sums and split-transform scratch
+  size_t fft_ws_bytes = 0;
 };
 
 namespace gwl {
@@ -406,6 +425,355 @@ int launch_mb_nd(const MbArgs& a, int nd, hipStream_t st) {
   return GWL_OK;
 }
 
+// ------------------------------------------------------------ IFFT-FFT <h|h>
+// multiband.py:613-646, 766-783.  For band b >= 1 and detector d the
+// reference places sqrt(w) h at bins K_s .. K_e of an M_b / 2 + 1 half
+// spectrum, takes its inverse real transform x (M_b samples), zero-pads it in
+// front to 2 M_b samples, transforms back and sums
+//   (4 / T^_b) sum_{k <= M_b} Re Ibc_b[k] |thbc_k|^2.
+// The even bins thbc_{2j} are the half spectrum itself, so their terms are
+// per-point coefficients folded into the quadratic table on the host.  The
+// odd bins are the spectrum at half-integer frequencies,
+//   Y_j = sum_n x_n exp(-i pi (2 j + 1) n / M_b),  j < M' = M_b / 2,
+// computed with two M'-point complex transforms per (row, detector, band):
+//   1. Zc_k = (X_k + conj X_{M'-k}) / 2 + i e^{i pi k / M'} (X_k - conj X_{M'-k}) / 2
+//      (X the half spectrum), z = inverse DFT(Zc): z_p = M' (x_{2p} + i x_{2p+1});
+//   2. u_p = z_p e^{-i pi p / M'}, U = DFT(u);
+//   3. E_j = (U_j + conj U_{M'-1-j}) / 2, O_j = (U_j - conj U_{M'-1-j}) / 2i,
+//      Y_j = (E_j + e^{-i pi (2j+1) / M_b} O_j) / M'
+// and sum_j odd_j |Y_j|^2 with the M'^-2 folded into the odd-bin weights.
+// Transforms of P <= 8192 points run in LDS (Stockham passes of radix 16 and
+// one of radix 2, 4 or 8, each thread holding 16 points); longer ones
+// (M' = R P) add a radix-R step over global scratch (four-step split).
+// Every twiddle is an entry of the band's table exp(-2 pi i k / (4 M')).
+
+// LDS index with one pad slot per 16 points (the radix-16 pass reads and
+// writes at strides of 16 points)
+__device__ __forceinline__ int lpad(int i) { return i + (i >> 4); }
+
+__device__ __forceinline__ double2 conj2(double2 a) { return make_double2(a.x, -a.y); }
+
+// exp(-+2 pi i m / 16) for compile-time m (INV: the + sign)
+template <bool INV>
+__device__ __forceinline__ double2 w16(int m) {
+  constexpr double c1 = 0.92387953251128675613, s1 = 0.38268343236508977173;
+  constexpr double h = 0.70710678118654752440;
+  double c, s;
+  switch (m & 15) {
+    case 0: c = 1.0; s = 0.0; break;
+    case 1: c = c1; s = s1; break;
+    case 2: c = h; s = h; break;
+    case 3: c = s1; s = c1; break;
+    case 4: c = 0.0; s = 1.0; break;
+    case 5: c = -s1; s = c1; break;
+    case 6: c = -h; s = h; break;
+    case 7: c = -c1; s = s1; break;
+    case 8: c = -1.0; s = 0.0; break;
+    case 9: c = -c1; s = -s1; break;
+    case 10: c = -h; s = -h; break;
+    case 11: c = -s1; s = -c1; break;
+    case 12: c = 0.0; s = -1.0; break;
+    case 13: c = s1; s = -c1; break;
+    case 14: c = h; s = -h; break;
+    default: c = c1; s = -s1; break;
+  }
+  return make_double2(c, INV ? s : -s);
+}
+
+// In-register DFT of R = 2, 4, 8, 16 points: a_k = sum_n a_n exp(-+2 pi i n k / R)
+template <int R, bool INV>
+__device__ __forceinline__ void dft_reg(double2* a) {
+  if constexpr (R == 2) {
+    const double2 t = a[1];
+    a[1] = make_double2(a[0].x - t.x, a[0].y - t.y);
+    a[0] = make_double2(a[0].x + t.x, a[0].y + t.y);
+  } else if constexpr (R == 4) {
+    const double2 s0 = make_double2(a[0].x + a[2].x, a[0].y + a[2].y);
+    const double2 d0 = make_double2(a[0].x - a[2].x, a[0].y - a[2].y);
+    const double2 s1 = make_double2(a[1].x + a[3].x, a[1].y + a[3].y);
+    double2 d1 = make_double2(a[1].x - a[3].x, a[1].y - a[3].y);
+    d1 = INV ? make_double2(-d1.y, d1.x) : make_double2(d1.y, -d1.x);  // * (+-i)
+    a[0] = make_double2(s0.x + s1.x, s0.y + s1.y);
+    a[2] = make_double2(s0.x - s1.x, s0.y - s1.y);
+    a[1] = make_double2(d0.x + d1.x, d0.y + d1.y);
+    a[3] = make_double2(d0.x - d1.x, d0.y - d1.y);
+  } else {
+    // R = A B with A = R / 4, B = 4: n = n1 + A n2, k = B k1 + k2;
+    // a_k = sum_n1 W_A^{n1 k1} W_R^{n1 k2} sum_n2 a_{n1 + A n2} W_4^{n2 k2}
+    constexpr int A = R / 4;
+    double2 t[R];
+#pragma unroll
+    for (int n1 = 0; n1 < A; ++n1) {
+      double2 v[4] = {a[n1], a[n1 + A], a[n1 + 2 * A], a[n1 + 3 * A]};
+      dft_reg<4, INV>(v);
+#pragma unroll
+      for (int k2 = 0; k2 < 4; ++k2) t[n1 * 4 + k2] = cmul(v[k2], w16<INV>(n1 * k2 * (16 / R)));
+    }
+#pragma unroll
+    for (int k2 = 0; k2 < 4; ++k2) {
+      double2 v[A];
+#pragma unroll
+      for (int n1 = 0; n1 < A; ++n1) v[n1] = t[n1 * 4 + k2];
+      dft_reg<A, INV>(v);
+#pragma unroll
+      for (int k1 = 0; k1 < A; ++k1) a[4 * k1 + k2] = v[k1];
+    }
+  }
+}
+
+// One Stockham pass of radix R over P points in LDS (16 points per thread,
+// P / 16 threads): butterfly j reads x[j + q P / R], is twiddled by
+// W_{Ns R}^{(j mod Ns) q}, transformed, and written to
+// x[(j - j mod Ns) R + j mod Ns + q Ns].  All reads finish before any write.
+template <int R, bool INV>
+__device__ __forceinline__ void fft_pass(double2* x, int P, int Ns, const double2* tab, int tstep) {
+  constexpr int G = 16 / R;
+  const int T = P >> 4, PR = P / R;
+  double2 v[G][R];
+  int dst[G];
+#pragma unroll
+  for (int g = 0; g < G; ++g) {
+    const int j = threadIdx.x + g * T;
+    const int k = j & (Ns - 1);
+#pragma unroll
+    for (int q = 0; q < R; ++q) v[g][q] = x[lpad(j + q * PR)];
+    if (Ns > 1) {
+      const int step = (P / (Ns * R)) * tstep;  // W_P^{k q P / (Ns R)} = tab[k q step]
+#pragma unroll
+      for (int q = 1; q < R; ++q) {
+        const double2 w = tab[k * q * step];
+        v[g][q] = cmul(v[g][q], INV ? conj2(w) : w);
+      }
+    }
+    dft_reg<R, INV>(v[g]);
+    dst[g] = (j - k) * R + k;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int g = 0; g < G; ++g)
+#pragma unroll
+    for (int q = 0; q < R; ++q) x[lpad(dst[g] + q * Ns)] = v[g][q];
+  __syncthreads();
+}
+
+// P-point transform in LDS, natural order in and out, unnormalised;
+// tab[k tstep] = exp(-2 pi i k / P).
+template <bool INV>
+__device__ void lds_fft(double2* x, int P, int logP, const double2* tab, int tstep) {
+  int Ns = 1, bits = logP;
+  for (; bits >= 4; bits -= 4, Ns <<= 4) fft_pass<16, INV>(x, P, Ns, tab, tstep);
+  if (bits == 3) fft_pass<8, INV>(x, P, Ns, tab, tstep);
+  else if (bits == 2) fft_pass<4, INV>(x, P, Ns, tab, tstep);
+  else if (bits == 1) fft_pass<2, INV>(x, P, Ns, tab, tstep);
+}
+
+struct MbFftArgs {
+  const RelRow* rows;
+  long long n;
+  int nd;
+  double flo;
+  const double* fe;
+  const double* basis;
+  const double* amp;
+  MbFftBand band;
+  double* part;        // [n][nd] this band's odd-bin sums
+  double2* scratch;    // split transforms: [gridDim.x][2 M']
+};
+
+// sqrt(w_p) h_d(f_p) at banded point p (the float64 detector response of k_mb)
+__device__ __forceinline__ double2 mb_point_strain(const MbFftArgs& a, const RelRow& rr, int d,
+                                                   int i) {
+  const int p = a.band.p0 + i;
+  const double f = a.fe[p];
+  if (!(f >= a.flo && f <= rr.fhi)) return make_double2(0.0, 0.0);
+  double b[kNB];
+#pragma unroll
+  for (int k = 0; k < kNB; ++k) b[k] = a.basis[(size_t)p * kNB + k];
+  double ph = fma(f, rr.dt[d], tf2_phase_cycles(rr.c, b));
+  ph -= rint(ph);
+  const double s = a.amp[p] * a.band.sqw[i];
+  return cmul(make_double2(rr.cre[d] * s, rr.cim[d] * s), expm2pi(ph));
+}
+
+// Zc_k from X_k and X_{M'-k} (step 1 above); e^{i pi k / M'} = conj tab[2k]
+__device__ __forceinline__ double2 mb_pack(double2 xk, double2 xm, double2 t2k) {
+  const double2 s = make_double2(0.5 * (xk.x + xm.x), 0.5 * (xk.y - xm.y));
+  const double2 dd = make_double2(0.5 * (xk.x - xm.x), 0.5 * (xk.y + xm.y));
+  const double2 e = cmul(conj2(t2k), dd);
+  return make_double2(s.x - e.y, s.y + e.x);  // s + i e
+}
+
+// odd_j |Y_j|^2 from U_j and U_{M'-1-j} (step 3 above); tab[2j+1] = e^{-i pi (2j+1) / M_b}
+__device__ __forceinline__ double mb_odd_term(double2 u, double2 ur, double2 t, double wgt) {
+  const double2 e = make_double2(0.5 * (u.x + ur.x), 0.5 * (u.y - ur.y));
+  const double2 dd = make_double2(0.5 * (u.x - ur.x), 0.5 * (u.y + ur.y));
+  const double2 o = make_double2(dd.y, -dd.x);  // dd / i
+  const double2 y = cmul(t, o);
+  const double yr = e.x + y.x, yi = e.y + y.y;
+  return wgt * (yr * yr + yi * yi);
+}
+
+// sum over the workgroup, written by thread 0
+__device__ __forceinline__ void mb_block_sum(double s, double* red, double* dst) {
+  s = reduce_wave(s);
+  const int w = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
+  if ((threadIdx.x & 63) == 0) red[w] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int k = 0; k < nw; ++k) t += red[k];
+    *dst = t;
+  }
+}
+
+// One workgroup of M' / 16 threads per (row, detector): the whole band in
+// LDS (M' = P).
+__global__ __launch_bounds__(512) void k_mb_fft_lds(MbFftArgs a) {
+  extern __shared__ double2 xs[];
+  __shared__ double red[8];
+  const MbFftBand& bd = a.band;
+  const long long prob = blockIdx.x;
+  const long long r = prob / a.nd;
+  const int d = (int)(prob - r * a.nd);
+  const RelRow& rr = a.rows[r];
+  if (!rr.valid) {  // the epilogue discards the row
+    if (threadIdx.x == 0) a.part[prob] = 0.0;
+    return;
+  }
+  const int P = bd.P, T = P >> 4;
+  for (int i = threadIdx.x; i < P; i += T) xs[lpad(i)] = make_double2(0.0, 0.0);
+  __syncthreads();
+  for (int i = threadIdx.x; i < bd.L; i += T) xs[lpad(bd.ks + i)] = mb_point_strain(a, rr, d, i);
+  __syncthreads();
+  double2 z[16];
+#pragma unroll
+  for (int g = 0; g < 16; ++g) {
+    const int k = threadIdx.x + g * T;
+    const double2 xm = k ? xs[lpad(P - k)] : make_double2(0.0, 0.0);  // X_{M'} = 0
+    z[g] = mb_pack(xs[lpad(k)], xm, bd.tab[2 * k]);
+  }
+  __syncthreads();
+#pragma unroll
+  for (int g = 0; g < 16; ++g) xs[lpad(threadIdx.x + g * T)] = z[g];
+  __syncthreads();
+  lds_fft<true>(xs, P, bd.logP, bd.tab, 4);
+#pragma unroll
+  for (int g = 0; g < 16; ++g) {
+    const int p = threadIdx.x + g * T;
+    xs[lpad(p)] = cmul(xs[lpad(p)], bd.tab[2 * p]);
+  }
+  __syncthreads();
+  lds_fft<false>(xs, P, bd.logP, bd.tab, 4);
+  const double* odd = bd.odd + (size_t)d * bd.mh;
+  double s = 0.0;
+#pragma unroll 4
+  for (int g = 0; g < 16; ++g) {
+    const int j = threadIdx.x + g * T;
+    s += mb_odd_term(xs[lpad(j)], xs[lpad(P - 1 - j)], bd.tab[2 * j + 1], odd[j]);
+  }
+  mb_block_sum(s, red, a.part + prob);
+}
+
+// Bands with M' = R P > P: each workgroup takes (row, detector) problems in
+// turn with 2 M' points of global scratch; both transforms are R LDS
+// transforms of P points after a radix-R step:
+//   X_{k1 + R k2} = DFT_P[ W_{M'}^{n2 k1} sum_n1 x_{n2 + P n1} W_R^{n1 k1} ](k2).
+__global__ __launch_bounds__(256) void k_mb_fft_split(MbFftArgs a) {
+  extern __shared__ double2 xs[];
+  __shared__ double red[4];
+  const MbFftBand& bd = a.band;
+  const int P = bd.P, T = P >> 4, R = bd.R, M = bd.mh;
+  const int rstep = 4 * M / R, tstep = 4 * R;  // tab[m rstep] = W_R^m, tab[k tstep] = W_P^k
+  double2* A = a.scratch + (size_t)blockIdx.x * 2 * M;
+  double2* B = A + M;
+  const long long total = a.n * a.nd;
+  for (long long prob = blockIdx.x; prob < total; prob += gridDim.x) {
+    const long long r = prob / a.nd;
+    const int d = (int)(prob - r * a.nd);
+    const RelRow& rr = a.rows[r];
+    if (!rr.valid) {
+      if (threadIdx.x == 0) a.part[prob] = 0.0;
+      continue;
+    }
+    for (int i = threadIdx.x; i < M; i += T) A[i] = make_double2(0.0, 0.0);
+    __threadfence();
+    __syncthreads();
+    for (int i = threadIdx.x; i < bd.L; i += T) A[bd.ks + i] = mb_point_strain(a, rr, d, i);
+    __threadfence();
+    __syncthreads();
+    // inverse transform of Zc; outputs modulated into B
+    for (int k1 = 0; k1 < R; ++k1) {
+#pragma unroll 4
+      for (int g = 0; g < 16; ++g) {
+        const int n2 = threadIdx.x + g * T;
+        double2 acc = make_double2(0.0, 0.0);
+        for (int n1 = 0; n1 < R; ++n1) {
+          const int k = n2 + P * n1;
+          const double2 xm = k ? A[M - k] : make_double2(0.0, 0.0);
+          const double2 zc = mb_pack(A[k], xm, bd.tab[2 * k]);
+          const double2 w = conj2(bd.tab[((n1 * k1) & (R - 1)) * rstep]);
+          acc.x += zc.x * w.x - zc.y * w.y;
+          acc.y += zc.x * w.y + zc.y * w.x;
+        }
+        xs[lpad(n2)] = cmul(acc, conj2(bd.tab[4 * n2 * k1]));
+      }
+      __syncthreads();
+      lds_fft<true>(xs, P, bd.logP, bd.tab, tstep);
+#pragma unroll 4
+      for (int g = 0; g < 16; ++g) {
+        const int k2 = threadIdx.x + g * T;
+        const int p = k1 + R * k2;
+        B[p] = cmul(xs[lpad(k2)], bd.tab[2 * p]);
+      }
+      __syncthreads();
+    }
+    __threadfence();
+    __syncthreads();
+    // forward transform of B into A
+    for (int k1 = 0; k1 < R; ++k1) {
+#pragma unroll 4
+      for (int g = 0; g < 16; ++g) {
+        const int n2 = threadIdx.x + g * T;
+        double2 acc = make_double2(0.0, 0.0);
+        for (int n1 = 0; n1 < R; ++n1) {
+          const double2 u = B[n2 + P * n1];
+          const double2 w = bd.tab[((n1 * k1) & (R - 1)) * rstep];
+          acc.x += u.x * w.x - u.y * w.y;
+          acc.y += u.x * w.y + u.y * w.x;
+        }
+        xs[lpad(n2)] = cmul(acc, bd.tab[4 * n2 * k1]);
+      }
+      __syncthreads();
+      lds_fft<false>(xs, P, bd.logP, bd.tab, tstep);
+#pragma unroll 4
+      for (int g = 0; g < 16; ++g) {
+        const int k2 = threadIdx.x + g * T;
+        A[k1 + R * k2] = xs[lpad(k2)];
+      }
+      __syncthreads();
+    }
+    __threadfence();
+    __syncthreads();
+    const double* odd = bd.odd + (size_t)d * M;
+    double s = 0.0;
+    for (int j = threadIdx.x; j < M; j += T)
+      s += mb_odd_term(A[j], A[M - 1 - j], bd.tab[2 * j + 1], odd[j]);
+    mb_block_sum(s, red, a.part + prob);
+    __syncthreads();  // red and the scratch are reused by the next problem
+  }
+}
+
+// <h|h> += the bands' odd-bin sums, in band and detector order
+__global__ __launch_bounds__(128) void k_mb_fft_sum(const double* part, long long n, int nd,
+                                                    int nbands, double* sums) {
+  const long long r = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  double t = 0.0;
+  for (int b = 0; b < nbands; ++b)
+    for (int d = 0; d < nd; ++d) t += part[((size_t)b * n + r) * nd + d];
+  sums[3 * r + 2] += t;
+}
+
 // One thread per row: base.py:448-477 on the summed inner products.
 __global__ __launch_bounds__(128) void k_relbin_epi(RelbinArgs a) {
   extern __shared__ double knots[];
@@ -491,6 +859,60 @@ int grow_buf(gwl_event* ev, void** buf, size_t* bytes, size_t need) {
 }
 
 static_assert(sizeof(RelRow) % sizeof(int) == 0, "RelRow holds whole ints");
+
+// The IFFT-FFT bands' odd-bin sums of rows [0, n) (row constants rows,
+// their <h|h> in sums[3 r + 2]), added in band and detector order.
+constexpr long long kMbFftChunk = 1LL << 26;  // rows per launch of the LDS kernel
+constexpr int kMbFftSplitP = 4096;            // LDS transform of the split bands
+
+size_t mb_fft_lds_bytes(int P) { return (size_t)(P + P / 16) * sizeof(double2); }
+
+int mb_fft_launch(gwl_mb* mb, const RelRow* rows, long long n, double* sums, hipStream_t st) {
+  if (mb->fft.empty() || n < 1) return GWL_OK;
+  gwl_event* ev = mb->ev;
+  const int nd = ev->dev.nd, nb = (int)mb->fft.size();
+  int split_m = 0;
+  for (const MbFftBand& b : mb->fft)
+    if (b.R > 1) split_m = std::max(split_m, b.mh);
+  const long long slots = std::min<long long>(n * nd, 2LL * ev->ctx->n_cu);
+  const size_t part_bytes = ((size_t)nb * n * nd * 8 + 255) / 256 * 256;
+  const size_t need = part_bytes + (split_m ? (size_t)slots * 2 * split_m * sizeof(double2) : 0);
+  if (int rc = grow_buf(ev, &mb->fft_ws, &mb->fft_ws_bytes, need)) return rc;
+  double* part = static_cast<double*>(mb->fft_ws);
+  MbFftArgs f;
+  f.nd = nd;
+  f.flo = ev->dev.wf_fmin;
+  f.fe = mb->fe;
+  f.basis = mb->basis;
+  f.amp = mb->amp;
+  f.scratch = reinterpret_cast<double2*>(static_cast<char*>(mb->fft_ws) + part_bytes);
+  for (int b = 0; b < nb; ++b) {
+    f.band = mb->fft[b];
+    double* pb = part + (size_t)b * n * nd;
+    const int P = f.band.P;
+    if (f.band.R == 1) {
+      for (long long r0 = 0; r0 < n; r0 += kMbFftChunk) {
+        f.rows = rows + r0;
+        f.n = std::min(kMbFftChunk, n - r0);
+        f.part = pb + r0 * nd;
+        hipLaunchKernelGGL(k_mb_fft_lds, dim3((unsigned)(f.n * nd)), dim3(P / 16),
+                           mb_fft_lds_bytes(P), st, f);
+        GWL_HIP_CHECK(hipGetLastError());
+      }
+    } else {
+      f.rows = rows;
+      f.n = n;
+      f.part = pb;
+      hipLaunchKernelGGL(k_mb_fft_split, dim3((unsigned)slots), dim3(P / 16), mb_fft_lds_bytes(P),
+                         st, f);
+      GWL_HIP_CHECK(hipGetLastError());
+    }
+  }
+  hipLaunchKernelGGL(k_mb_fft_sum, dim3((unsigned)((n + 127) / 128)), dim3(128), 0, st, part, n,
+                     nd, nb, sums);
+  GWL_HIP_CHECK(hipGetLastError());
+  return GWL_OK;
+}
 constexpr int kValidStride = (int)(sizeof(RelRow) / sizeof(int));
 
 const int* valid_of(const RelRow* rows) {
@@ -1030,6 +1452,8 @@ int gwl_mb_destroy(gwl_mb* mb) {
   if (mb->data) (void)hipFree(mb->data);
   if (mb->ws) (void)hipFree(mb->ws);
   if (mb->tws) (void)hipFree(mb->tws);
+  if (mb->fft_data) (void)hipFree(mb->fft_data);
+  if (mb->fft_ws) (void)hipFree(mb->fft_ws);
   gwl::win_basis_free(&mb->win);
   delete mb;
   return GWL_OK;
@@ -1042,7 +1466,10 @@ int gwl_mb_reserve(gwl_mb* mb, int64_t rows) {
 
 int gwl_mb_device_bytes(gwl_mb* mb, int64_t* bytes) {
   if (!mb || !bytes) { set_error("gwl_mb_device_bytes: invalid argument"); return GWL_E_INVALID; }
-  *bytes = (int64_t)(mb->data_bytes + mb->ws_bytes + mb->win.bytes + mb->tws_bytes);
+  size_t fft = mb->fft_ws_bytes;
+  for (const gwl::MbFftBand& b : mb->fft)
+    fft += (size_t)b.mh * (4 * 16 + mb->ev->dev.nd * 8) + (size_t)b.L * 8;
+  *bytes = (int64_t)(mb->data_bytes + mb->ws_bytes + mb->win.bytes + mb->tws_bytes + fft);
   return GWL_OK;
 }
 
@@ -1120,7 +1547,9 @@ static int mb_eval_impl(gwl_mb* mb, const double* params, int64_t n, int32_t ld,
           mt.sums = t.sums;
           mt.win_a = opa;
           mt.win_kp = mb->win.Kp;
-          return fp64 ? gwl::launch_mb_nd<double>(mt, nd, st) : gwl::launch_mb_nd<float>(mt, nd, st);
+          const int rc = fp64 ? gwl::launch_mb_nd<double>(mt, nd, st)
+                              : gwl::launch_mb_nd<float>(mt, nd, st);
+          return rc != GWL_OK ? rc : gwl::mb_fft_launch(mb, t.rows, t.n, t.sums, st);
         },
         sdh, shh);
     if (rc != GWL_OK) return rc;
@@ -1132,6 +1561,7 @@ static int mb_eval_impl(gwl_mb* mb, const double* params, int64_t n, int32_t ld,
   const int rc = precision == GWL_FP64 ? gwl::launch_mb_nd<double>(m, ev->dev.nd, st)
                                        : gwl::launch_mb_nd<float>(m, ev->dev.nd, st);
   if (rc != GWL_OK) return rc;
+  if (int rf = gwl::mb_fft_launch(mb, a.rows, n, a.sums, st)) return rf;
   hipLaunchKernelGGL(gwl::k_relbin_epi, dim3(g128), dim3(128), lds, st, a);
   GWL_HIP_CHECK(hipGetLastError());
   return gwl::timing_end(ev, st);
@@ -1194,6 +1624,88 @@ int gwl_mb_set_time(gwl_mb* mb, const gwl_mb_time_desc* td) {
   gwl::win_basis_free(&mb->win);
   mb->win = wb;
   mb->has_win = true;
+  return GWL_OK;
+}
+
+int gwl_mb_set_fft(gwl_mb* mb, int32_t n_bands, const gwl_mb_fft_band* bands) {
+  if (!mb || n_bands < 0 || (n_bands > 0 && !bands)) {
+    set_error("gwl_mb_set_fft: invalid argument");
+    return GWL_E_INVALID;
+  }
+  gwl_event* ev = mb->ev;
+  const int nd = ev->dev.nd;
+  for (int b = 0; b < n_bands; ++b) {
+    const gwl_mb_fft_band& d = bands[b];
+    const int m = d.half_length;
+    if (m < 16 || m > (1 << 18) || (m & (m - 1)) || d.first_point < 0 || d.n_points < 1 ||
+        d.first_point + (int64_t)d.n_points > mb->n || d.first_bin < 1 ||
+        d.first_bin + (int64_t)d.n_points > m || !d.sqrt_window || !d.odd_weights) {
+      set_error("gwl_mb_set_fft: invalid band " + std::to_string(b));
+      return GWL_E_INVALID;
+    }
+  }
+  GWL_HIP_CHECK(hipSetDevice(ev->ctx->device));
+  // layout: per band the table, the odd-bin weights and the window roots
+  auto al = [](size_t x) { return (x + 255) / 256 * 256; };
+  size_t bytes = 0;
+  for (int b = 0; b < n_bands; ++b)
+    bytes += al((size_t)4 * bands[b].half_length * 16) + al((size_t)nd * bands[b].half_length * 8) +
+             al((size_t)bands[b].n_points * 8);
+  void* data = nullptr;
+  if (bytes && hipMalloc(&data, bytes) != hipSuccess) {
+    set_error("hipMalloc failed for the IFFT-FFT tables");
+    return GWL_E_NOMEM;
+  }
+  std::vector<gwl::MbFftBand> out;
+  hipError_t err = hipSuccess;
+  char* q = static_cast<char*>(data);
+  const size_t lds_max = ev->ctx->lds_per_block;
+  for (int b = 0; b < n_bands && err == hipSuccess; ++b) {
+    const gwl_mb_fft_band& d = bands[b];
+    const int m = d.half_length;
+    gwl::MbFftBand fb;
+    fb.mh = m;
+    fb.P = m;
+    if (m > 8192 || gwl::mb_fft_lds_bytes(m) > lds_max) {
+      fb.P = gwl::kMbFftSplitP;
+      while (gwl::mb_fft_lds_bytes(fb.P) > lds_max) fb.P /= 2;
+      fb.P = std::min(fb.P, m);
+    }
+    fb.R = m / fb.P;
+    fb.logP = 0;
+    while ((1 << fb.logP) < fb.P) ++fb.logP;
+    fb.p0 = d.first_point;
+    fb.L = d.n_points;
+    fb.ks = d.first_bin;
+    // exp(-2 pi i k / (4 M')), in extended precision
+    std::vector<double2> tab((size_t)4 * m);
+    const long double step = 2.0L * 3.141592653589793238462643383279502884L / (4.0L * m);
+    for (size_t k = 0; k < tab.size(); ++k) {
+      const long double t = step * (long double)k;
+      tab[k] = make_double2((double)std::cos(t), (double)-std::sin(t));
+    }
+    std::vector<double> odd((size_t)nd * m);
+    const double inv = 1.0 / ((double)m * (double)m);
+    for (size_t i = 0; i < odd.size(); ++i) odd[i] = d.odd_weights[i] * inv;
+    double2* t_dev = reinterpret_cast<double2*>(q); q += al((size_t)4 * m * 16);
+    double* o_dev = reinterpret_cast<double*>(q); q += al((size_t)nd * m * 8);
+    double* w_dev = reinterpret_cast<double*>(q); q += al((size_t)d.n_points * 8);
+    err = hipMemcpy(t_dev, tab.data(), tab.size() * 16, hipMemcpyHostToDevice);
+    if (err == hipSuccess) err = hipMemcpy(o_dev, odd.data(), odd.size() * 8, hipMemcpyHostToDevice);
+    if (err == hipSuccess)
+      err = hipMemcpy(w_dev, d.sqrt_window, (size_t)d.n_points * 8, hipMemcpyHostToDevice);
+    fb.tab = t_dev;
+    fb.odd = o_dev;
+    fb.sqw = w_dev;
+    out.push_back(fb);
+  }
+  if (err != hipSuccess) {
+    (void)hipFree(data);
+    return gwl::hip_fail(err, "gwl_mb_set_fft upload");
+  }
+  if (mb->fft_data) (void)hipFree(mb->fft_data);
+  mb->fft_data = data;
+  mb->fft = std::move(out);
   return GWL_OK;
 }
 

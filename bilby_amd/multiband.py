@@ -1,6 +1,6 @@
 """Multibanded likelihood on the GPU: bilby's MBGravitationalWaveTransient
-(bilby/gw/likelihood/multiband.py; Morisaki, arXiv:2104.07813) with the
-linear-interpolation algorithm for <h|h>.
+(bilby/gw/likelihood/multiband.py; Morisaki, arXiv:2104.07813) with either
+algorithm for <h|h>: linear interpolation or IFFT-FFT.
 
 ``MultibandSetup`` restates the host-side construction (float64 numpy, no
 GPU): frequency bands of durations T, T/2, T/4, ... chosen from the 0PN
@@ -12,7 +12,11 @@ response, <d|h> = conj(sum h L) and <h|h> = sum |h|^2 Q, then the
 marginalisation.  With time marginalisation (multiband.py:714-726, 792-800)
 the transform of the scattered h L terms at the time-prior window is a
 fixed linear map of the per-point terms, evaluated as one GEMM per batch
-(gwl_mb_set_time).  The IFFT-FFT algorithm for <h|h> is not provided.
+(gwl_mb_set_time).  With linear_interpolation=False (multiband.py:613-646,
+766-783) the per-point quadratic table holds the diagonal part of the
+IFFT-FFT <h|h> (the first band's direct sum and the even bins of the other
+bands' transforms) and gwl_mb_set_fft adds, per band, the odd bins from two
+half-length transforms per row and detector on the GPU.
 """
 import ctypes
 import math
@@ -32,7 +36,7 @@ def round_up_to_power_of_two(x):
 
 
 class MultibandSetup:
-    """Bands and coefficients (multiband.py:311-647, linear interpolation).
+    """Bands and coefficients (multiband.py:311-647).
 
     ``interferometers``: objects with ``name``, ``frequency_array``,
     ``frequency_mask``, ``frequency_domain_strain``,
@@ -42,7 +46,11 @@ class MultibandSetup:
 
     def __init__(self, interferometers, duration, reference_chirp_mass, highest_mode=2,
                  accuracy_factor=5, time_offset=2.12, delta_f_end=53.0,
-                 maximum_banding_frequency=None, minimum_banding_duration=0.0):
+                 maximum_banding_frequency=None, minimum_banding_duration=0.0,
+                 linear_interpolation=True):
+        if not isinstance(linear_interpolation, (bool, np.bool_)):  # multiband.py:175-180
+            raise TypeError("linear_interpolation must be a bool")
+        self.linear_interpolation = bool(linear_interpolation)
         self.interferometers = list(interferometers)
         self.duration = float(duration)
         self.reference_chirp_mass = float(reference_chirp_mass)
@@ -61,7 +69,10 @@ class MultibandSetup:
         self._setup_integers()
         self._setup_waveform_frequency_points()
         self._setup_linear_coefficients()
-        self._setup_quadratic_coefficients()
+        if self.linear_interpolation:
+            self._setup_quadratic_coefficients()
+        else:
+            self._setup_quadratic_coefficients_ifft_fft()
 
     @property
     def reference_chirp_mass_in_second(self):
@@ -244,30 +255,111 @@ class MultibandSetup:
         for name in self.quadratic_coeffs:
             self.quadratic_coeffs[name] = np.concatenate(self.quadratic_coeffs[name])
 
+    def _setup_quadratic_coefficients_ifft_fft(self):
+        """multiband.py:613-646: for each band the inverse PSD (zero off the
+        mask) truncated at the band's Nyquist bin and taken to the time
+        domain (N_b samples), kept at lags |n| <= N^_b / 2 with
+        N^_b = min(2 M_b, N_b), and transformed back (Ibc); T^_b = T N^_b / N_b;
+        the windows and their square roots on the banded points; zeroed work
+        arrays wths / hbcs as the reference keeps them."""
+        n_full = int(self.Nbs[-1])
+        nhat = [int(min(2 * m, n)) for m, n in zip(self.Mbs, self.Nbs)]
+        self.Tbhats = np.array([self.duration * nh / n for nh, n in zip(nhat, self.Nbs)])
+        self.Ibcs, self.hbcs, self.wths = {}, {}, {}
+        for ifo in self.interferometers:
+            inv = np.zeros(n_full // 2 + 1)
+            psd = np.asarray(ifo.power_spectral_density_array)
+            mask = np.asarray(ifo.frequency_mask, dtype=bool)
+            inv[:len(psd)][mask[:len(inv)]] = 1.0 / psd[mask]
+            ibc, hbc, wth = [], [], []
+            for b in range(self.number_of_bands):
+                lag = np.fft.irfft(inv[:int(self.Nbs[b]) // 2 + 1])
+                half = nhat[b] // 2
+                tail = nhat[b] - half - 1
+                kept = np.concatenate([lag[:half + 1], lag[len(lag) - tail:] if tail else []])
+                ibc.append(np.fft.rfft(kept))
+                hbc.append(np.zeros(nhat[b]))
+                wth.append(np.zeros(int(self.Mbs[b]) // 2 + 1, dtype=complex))
+            self.Ibcs[ifo.name], self.hbcs[ifo.name], self.wths[ifo.name] = ibc, hbc, wth
+        win = [self._window(1.0 / self.durations[b], int(self.Ks_Ke[b][0]),
+                            int(self.Ks_Ke[b][1] - self.Ks_Ke[b][0]) + 1, b)
+               for b in range(self.number_of_bands)]
+        self.windows = np.concatenate(win)
+        self.square_root_windows = np.sqrt(self.windows)
+
+    def ifft_fft_tables(self):
+        """The device form of the IFFT-FFT <h|h> (multiband.py:766-783):
+        per detector the diagonal coefficients on the banded points (the
+        first band's 4/T mask w / S, and (4 / T^_b) Re Ibc_b[2 j] w_j at bin j
+        of band b >= 1: the even bins of that band's 2 M_b-point transform
+        are its half spectrum sqrt(w) h), and per band b >= 1 the odd-bin
+        weights (4 / T^_b) Re Ibc_b[2 j + 1], j < M_b / 2."""
+        T = self.duration
+        diag = {}
+        for ifo in self.interferometers:
+            q = np.zeros(len(self.banded_frequency_points))
+            psd = np.asarray(ifo.power_spectral_density_array)
+            mask = np.asarray(ifo.frequency_mask, dtype=bool)
+            for b in range(self.number_of_bands):
+                s, e = (int(v) for v in self.start_end_idxs[b])
+                ks, ke = (int(v) for v in self.Ks_Ke[b])
+                w = self.windows[s:e + 1]
+                if b == 0:
+                    m = mask[ks:ke + 1]
+                    q[s:e + 1] = np.where(m, 4.0 / T * w / np.where(m, psd[ks:ke + 1], 1.0), 0.0)
+                else:
+                    q[s:e + 1] = 4.0 / self.Tbhats[b] * self.Ibcs[ifo.name][b][2 * ks:2 * ke + 1:2].real * w
+            diag[ifo.name] = q
+        bands = []
+        for b in range(1, self.number_of_bands):
+            s, e = (int(v) for v in self.start_end_idxs[b])
+            ks, ke = (int(v) for v in self.Ks_Ke[b])
+            mh = int(self.Mbs[b]) // 2
+            if ks < 1 or ke >= mh or mh < 16:
+                raise NotImplementedError("IFFT-FFT band %d reaches the zero or Nyquist bin of its "
+                                          "transform" % b)
+            odd = np.stack([4.0 / self.Tbhats[b] * self.Ibcs[ifo.name][b][1:2 * mh:2].real
+                            for ifo in self.interferometers])
+            bands.append(dict(half_length=mh, first_point=s, n_points=e - s + 1, first_bin=ks,
+                              sqrt_window=np.ascontiguousarray(self.square_root_windows[s:e + 1]),
+                              odd_weights=np.ascontiguousarray(odd)))
+        return diag, bands
+
     @property
     def weights(self):
-        """multiband.py:648-671 (linear interpolation)."""
+        """multiband.py:648-671."""
         keys = ["reference_chirp_mass", "highest_mode", "accuracy_factor", "time_offset",
                 "delta_f_end", "maximum_banding_frequency", "minimum_banding_duration",
                 "durations", "fb_dfb", "Nbs", "Mbs", "Ks_Ke", "banded_frequency_points",
                 "start_end_idxs", "unique_to_original_frequencies"]
         out = {k: getattr(self, k) for k in keys}
-        out["linear_interpolation"] = True
+        out["linear_interpolation"] = self.linear_interpolation
         out["waveform_frequencies"] = self.waveform_frequencies
         out["linear_coeffs"] = dict(self.linear_coeffs)
-        out["quadratic_coeffs"] = dict(self.quadratic_coeffs)
+        if self.linear_interpolation:
+            out["quadratic_coeffs"] = dict(self.quadratic_coeffs)
+        else:
+            for k in ("Tbhats", "windows", "square_root_windows"):
+                out[k] = getattr(self, k)
+            for k in ("wths", "hbcs", "Ibcs"):
+                out[k] = {name: {str(b): v for b, v in enumerate(arrs)}
+                          for name, arrs in getattr(self, k).items()}
         return out
 
     def save_weights(self, filename):
-        """multiband.py:673-687, as an .npz archive (bilby writes HDF5)."""
-        w = self.weights
+        """multiband.py:673-687, as an .npz archive (bilby writes HDF5);
+        nested dictionaries become keys joined with '/'."""
         flat = {}
-        for k, v in w.items():
+
+        def put(prefix, v):
             if isinstance(v, dict):
-                for name, arr in v.items():
-                    flat["%s/%s" % (k, name)] = np.asarray(arr)
+                for name, sub in v.items():
+                    put("%s/%s" % (prefix, name), sub)
             else:
-                flat[k] = np.asarray(v)
+                flat[prefix] = np.asarray(v)
+
+        for k, v in self.weights.items():
+            put(k, v)
         np.savez(filename, **flat)
 
     @classmethod
@@ -278,17 +370,20 @@ class MultibandSetup:
                 flat = {k: f[k] for k in f.files}
             weights = {}
             for k, v in flat.items():
-                if "/" in k:
-                    top, name = k.split("/", 1)
-                    weights.setdefault(top, {})[name] = v
-                else:
-                    weights[k] = v[()] if v.ndim == 0 else v
+                *path, leaf = k.split("/")
+                node = weights
+                for part in path:
+                    node = node.setdefault(part, {})
+                node[leaf] = v[()] if v.ndim == 0 else v
         self = cls.__new__(cls)
         self.interferometers = list(interferometers)
         self.duration = float(weights["durations"][0])
+        self.linear_interpolation = bool(weights.get("linear_interpolation", True))
         for k, v in weights.items():
             if k == "linear_interpolation":
                 continue
+            if k in ("wths", "hbcs", "Ibcs"):  # multiband.py:704-708: per band lists
+                v = {name: [d[str(b)] for b in range(len(d))] for name, d in v.items()}
             setattr(self, k, dict(v) if isinstance(v, dict) else v)
         return self
 
@@ -306,9 +401,6 @@ class MBGravitationalWaveTransient(GravitationalWaveTransient):
                  priors=None, time_marginalization=False, jitter_time=True,
                  distance_marginalization_lookup_table=None, reference_frame="sky",
                  time_reference="geocenter", precision="fp64", device=0):
-        if not linear_interpolation:
-            raise NotImplementedError("only the linear-interpolation algorithm for <h|h> is "
-                                      "provided")
         self._mb = None
         super().__init__(interferometers, waveform_generator,
                          time_marginalization=time_marginalization,
@@ -338,7 +430,8 @@ class MBGravitationalWaveTransient(GravitationalWaveTransient):
                 reference_chirp_mass = self._minimum_chirp_mass()
             self.setup = MultibandSetup(ifos, ifos.duration, reference_chirp_mass, highest_mode,
                                         accuracy_factor, time_offset, delta_f_end,
-                                        maximum_banding_frequency, minimum_banding_duration)
+                                        maximum_banding_frequency, minimum_banding_duration,
+                                        linear_interpolation)
         if self.time_marginalization:
             self._setup_time_marginalization_multiband()
         self._upload()
@@ -407,7 +500,7 @@ class MBGravitationalWaveTransient(GravitationalWaveTransient):
     delta_f_end = property(lambda self: self.setup.delta_f_end)
     maximum_banding_frequency = property(lambda self: self.setup.maximum_banding_frequency)
     minimum_banding_duration = property(lambda self: self.setup.minimum_banding_duration)
-    linear_interpolation = property(lambda self: True)
+    linear_interpolation = property(lambda self: self.setup.linear_interpolation)
 
     @property
     def weights(self):
@@ -421,16 +514,32 @@ class MBGravitationalWaveTransient(GravitationalWaveTransient):
         nd, n = len(self.interferometers), len(s.banded_frequency_points)
         lin = np.zeros((nd, n, 2))
         quad = np.zeros((nd, n))
+        fft_bands = []
+        if s.linear_interpolation:
+            diag = s.quadratic_coeffs
+        else:
+            diag, fft_bands = s.ifft_fft_tables()
         for k, ifo in enumerate(self.interferometers):
             lin[k, :, 0] = np.real(s.linear_coeffs[ifo.name])
             lin[k, :, 1] = np.imag(s.linear_coeffs[ifo.name])
-            quad[k] = s.quadratic_coeffs[ifo.name]
+            quad[k] = diag[ifo.name]
         freqs = np.ascontiguousarray(s.banded_frequency_points, dtype=np.float64)
         lin = np.ascontiguousarray(lin)
         quad = np.ascontiguousarray(quad)
         mb = ctypes.c_void_p()
         check(LIB.gwl_mb_create(self._event, n, dptr(freqs), dptr(lin), dptr(quad),
                                 ctypes.byref(mb)), "gwl_mb_create")
+        if fft_bands:
+            descs = (_lib.MbFftBand * len(fft_bands))()
+            for k, band in enumerate(fft_bands):
+                for key in ("half_length", "first_point", "n_points", "first_bin"):
+                    setattr(descs[k], key, band[key])
+                descs[k].sqrt_window = dptr(band["sqrt_window"])
+                descs[k].odd_weights = dptr(band["odd_weights"])
+            rc = LIB.gwl_mb_set_fft(mb, len(fft_bands), descs)
+            if rc != 0:
+                LIB.gwl_mb_destroy(mb)
+                check(rc, "gwl_mb_set_fft")
         if self.time_marginalization:
             N, j_first, n_window = self._time_window
             td = _lib.MbTimeDesc()

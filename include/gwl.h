@@ -503,8 +503,9 @@ int gwl_relbin_upsampled_series(gwl_relbin* rb, const double* params, int64_t n,
                                 double* h_inner_h, void* stream);
 
 /* ------------------------------------------------------------------ multibanding
- * MBGravitationalWaveTransient (bilby/gw/likelihood/multiband.py, linear
- * interpolation for <h|h>) on a TaylorF2 event; the event supplies
+ * MBGravitationalWaveTransient (bilby/gw/likelihood/multiband.py; <h|h> by
+ * linear interpolation, or by the IFFT-FFT algorithm after gwl_mb_set_fft)
+ * on a TaylorF2 event; the event supplies
  * detectors, source model, flags and the distance spline and must outlive
  * the object.  An event that marginalises over time also needs
  * gwl_mb_set_time before the first evaluation.
@@ -512,7 +513,7 @@ int gwl_relbin_upsampled_series(gwl_relbin* rb, const double* params, int64_t n,
  *   linear:      [n_det][n_points][2] complex linear coefficients
  *                (multiband.py:529-549)
  *   quadratic:   [n_det][n_points] quadratic coefficients
- *                (multiband.py:551-611) */
+ *                (multiband.py:551-611; see gwl_mb_set_fft for IFFT-FFT) */
 typedef struct gwl_mb gwl_mb;
 int gwl_mb_create(gwl_event* ev, int32_t n_points, const double* frequencies,
                   const double* linear, const double* quadratic, gwl_mb** out);
@@ -545,6 +546,29 @@ typedef struct gwl_mb_time_desc {
   double antenna_time;
 } gwl_mb_time_desc;
 int gwl_mb_set_time(gwl_mb* mb, const gwl_mb_time_desc* td);
+/* The IFFT-FFT algorithm for <h|h> (linear_interpolation=False;
+ * multiband.py:613-646, 766-783; replaces the linear-interpolation sum of
+ * multiband.py:760-765).  The quadratic coefficients given to gwl_mb_create
+ * then hold the diagonal part: 4/T mask w / S on the first band's points and
+ * (4 / T^_b) Re Ibc_b[2 j] w_j on the points of band b >= 1 at bin j (the
+ * even bins of that band's 2 M_b-point transform are its half spectrum).
+ * Each band b >= 1 listed here adds the odd bins:
+ *   half_length:  M' = M_b / 2 (a power of two, 16 .. 2^18)
+ *   first_point, n_points: the band's banded points (start_end_idxs[b])
+ *   first_bin:    K_s of the band (>= 1, K_s + n_points <= M')
+ *   sqrt_window:  [n_points] square_root_windows of the band
+ *   odd_weights:  [n_det][M'] (4 / T^_b) Re Ibc_b[2 j + 1], j < M'
+ * n_bands = 0 returns to the linear-interpolation form.  Host pointers,
+ * copied. */
+typedef struct gwl_mb_fft_band {
+  int32_t half_length;
+  int32_t first_point;
+  int32_t n_points;
+  int32_t first_bin;
+  const double* sqrt_window;
+  const double* odd_weights;
+} gwl_mb_fft_band;
+int gwl_mb_set_fft(gwl_mb* mb, int32_t n_bands, const gwl_mb_fft_band* bands);
 /* d_inner_h_array of multibanding over the window given to gwl_mb_set_time
  * (multiband.py:792-800), for the time reconstruction of
  * multiband.py:813-848: d_inner_h [n_rows][n_window][2] for transform bins

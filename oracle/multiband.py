@@ -1,8 +1,9 @@
 """FP64 restatement of bilby's MBGravitationalWaveTransient evaluation
-(bilby/gw/likelihood/multiband.py:728-807, linear interpolation; time
-marginalisation :714-726, 792-800) on given multiband weights, and a
-literal loop form of the quadratic coefficients
-(multiband.py:551-611) to check the product's prefix-sum form against.
+(bilby/gw/likelihood/multiband.py:728-807: <h|h> by linear interpolation or
+by the IFFT-FFT algorithm of :766-783; time marginalisation :714-726,
+792-800) on given multiband weights, and a literal loop form of the
+quadratic coefficients (multiband.py:551-611) to check the product's
+prefix-sum form against.
 
 Test infrastructure only (see oracle/__init__.py).
 """
@@ -20,7 +21,10 @@ class OracleMultiband(OracleGWTransient):
     """``weights``: an object or dict with banded_frequency_points,
     unique_to_original_frequencies, waveform_frequencies, linear_coeffs and
     quadratic_coeffs (detector name -> array); with time marginalisation
-    also durations, Nbs and start_end_idxs."""
+    also durations, Nbs and start_end_idxs; with linear_interpolation False
+    instead of quadratic_coeffs: durations, Nbs, Mbs, Ks_Ke,
+    start_end_idxs, Tbhats, windows, square_root_windows and Ibcs
+    (detector name -> {str(band): array})."""
 
     def __init__(self, event, source, weights, time_marginalization=False, **kwargs):
         super().__init__(event, source, time_marginalization=time_marginalization, **kwargs)
@@ -53,7 +57,10 @@ class OracleMultiband(OracleGWTransient):
                                    antenna_time=self.antenna_time)
         lin = w["linear_coeffs"][det.name]
         d_inner_h = np.conj(np.dot(strain, lin))
-        h_inner_h = np.vdot(np.abs(strain) ** 2, w["quadratic_coeffs"][det.name]).real
+        if w.get("linear_interpolation", True):
+            h_inner_h = np.vdot(np.abs(strain) ** 2, w["quadratic_coeffs"][det.name]).real
+        else:
+            h_inner_h = self.ifft_fft_optimal_snr_squared(strain, det)
         arr = None
         if self.time_marginalization:
             fm = self._full_to_multiband
@@ -63,6 +70,33 @@ class OracleMultiband(OracleGWTransient):
                 self._full_d_h[fm[s:e + 1]] += strain[s:e + 1] * lin[s:e + 1]
             arr = np.fft.fft(self._full_d_h)
         return d_inner_h, h_inner_h, arr
+
+    def ifft_fft_optimal_snr_squared(self, strain, det):
+        """multiband.py:766-783: the first band summed directly on the
+        original grid (4/T mask w / S); band b >= 1: sqrt(w) h placed at bins
+        K_s .. K_e of an M_b / 2 + 1 half spectrum, its inverse real
+        transform written into the last M_b samples of N^_b zeros,
+        transformed back, |.|^2 weighted by Re Ibc_b, times 4 / T^_b."""
+        w = self.w
+        total = 0.0
+        for b in range(len(w["durations"])):
+            ks, ke = (int(v) for v in w["Ks_Ke"][b])
+            s, e = (int(v) for v in w["start_end_idxs"][b])
+            mb = int(w["Mbs"][b])
+            seg = strain[s:e + 1]
+            if b == 0:
+                total += (4. / self.event.duration) * np.vdot(
+                    np.abs(seg) ** 2,
+                    det.mask[ks:ke + 1] * w["windows"][s:e + 1] / det.psd[ks:ke + 1]).real
+            else:
+                wth = np.zeros(mb // 2 + 1, dtype=complex)
+                wth[ks:ke + 1] = w["square_root_windows"][s:e + 1] * seg
+                hbc = np.zeros(min(2 * mb, int(w["Nbs"][b])))
+                hbc[-mb:] = np.fft.irfft(wth)
+                thbc = np.fft.rfft(hbc)
+                ibc = w["Ibcs"][det.name][str(b)]
+                total += (4. / w["Tbhats"][b]) * np.vdot(np.abs(thbc) ** 2, ibc.real).real
+        return total
 
     def log_likelihood_ratio(self, parameters):
         params = copy.deepcopy(parameters)

@@ -201,3 +201,151 @@ def test_time_grid_index_follows_fancy_index_accumulation():
             assert (idx < 0).sum() == 3   # 20.0, 21.0, 21.0 dropped in band 0
         N_, j_first, n_window = obj._time_window
         assert N_ == N and 0 <= j_first and j_first + n_window <= N
+
+
+# ------------------------------------------------------------------ IFFT-FFT <h|h>
+
+def _band_strain(mb, det, params):
+    """The detector response at the banded points, as the oracle's
+    calculate_snrs_mb forms it."""
+    from oracle.likelihood import source_polarizations
+    w = mb.w
+    wa = dict(WF, maximum_frequency=mb.event.frequency_array[-1])
+    pols = source_polarizations("taylorf2", w["waveform_frequencies"], params, wa)
+    idx = w["unique_to_original_frequencies"]
+    pts = types.SimpleNamespace(tensor=det.tensor, vertex=det.vertex,
+                                frequency_array=w["banded_frequency_points"],
+                                mask=np.ones(len(idx), dtype=bool))
+    return detector_response(pts, {k: v[idx] for k, v in pols.items()}, params,
+                             mb.event.start_time)
+
+
+def test_ifft_fft_matches_full_likelihood():
+    """linear_interpolation=False on the reference test's configuration:
+    |dlnL| against the full likelihood below 5e-3, the reference's tolerance
+    for the IFFT-FFT algorithm (likelihood_test.py:1380-1386), and the
+    setup flags and arrays as multiband.py:613-646 leaves them."""
+    ev, ifos, inj = _event()
+    setup = _setup(ifos, inj, linear_interpolation=False)
+    assert setup.linear_interpolation is False and not hasattr(setup, "quadratic_coeffs")
+    nb = setup.number_of_bands
+    assert nb >= 3 and len(setup.Tbhats) == nb and len(setup.windows) == len(
+        setup.banded_frequency_points)
+    for b in range(nb):
+        nhat = min(2 * setup.Mbs[b], setup.Nbs[b])
+        assert setup.Tbhats[b] == T * nhat / setup.Nbs[b]
+        assert len(setup.Ibcs["H1"][b]) == nhat // 2 + 1
+    mb = OracleMultiband(ev, "taylorf2", setup, waveform_arguments=WF)
+    full = OracleGWTransient(ev, "taylorf2", waveform_arguments=WF)
+    rng = np.random.default_rng(3)
+    for k in range(6):
+        p = dict(inj)
+        if k:
+            p["ra"] += rng.normal(0, 0.05)
+            p["luminosity_distance"] *= rng.uniform(0.8, 1.2)
+            p["geocent_time"] += rng.normal(0, 1e-3)
+        ll, llmb = full.log_likelihood_ratio(p), mb.log_likelihood_ratio(p)
+        assert ll > 100
+        assert abs(ll - llmb) < 5e-3, (k, ll, llmb)
+
+
+def test_ifft_fft_band_sums_equal_the_lag_form():
+    """Each band's IFFT-FFT sum against the quadratic form it equals by
+    Parseval, written without transforms of the waveform:
+      sum_{k <= M_b} Re Ibc_k |thbc_k|^2
+        = (N^ sum_{n,m} x_n x_m I[|n - m|] + Ibc_0 t_0^2 + Ibc_M t_M^2) / 2
+    with x the band's time samples, I the inverse PSD's lags (the inverse
+    real transform of 1/S truncated at the band's Nyquist bin), t_0 = sum x,
+    t_M = sum (-1)^(n + M) x_n and Ibc_0, Ibc_M the sums of the kept lags
+    with signs 1 and (-1)^n.  The double sum runs over the autocorrelation of
+    x (np.correlate), so a wrong lag window, band placement or half-spectrum
+    weighting in the restatement shows as a mismatch."""
+    ev, ifos, inj = _event()
+    setup = _setup(ifos, inj, linear_interpolation=False)
+    mb = OracleMultiband(ev, "taylorf2", setup, waveform_arguments=WF)
+    for det, ifo in zip(ev.detectors, ifos):
+        strain = _band_strain(mb, det, inj)
+        inv = np.zeros(int(setup.Nbs[-1]) // 2 + 1)
+        inv[:len(det.psd)][det.mask] = 1 / det.psd[det.mask]
+        s0, e0 = setup.start_end_idxs[0]
+        k0, k1 = setup.Ks_Ke[0]
+        lag_form = 4 / T * np.sum(np.abs(strain[s0:e0 + 1]) ** 2 * det.mask[k0:k1 + 1]
+                                  * setup.windows[s0:e0 + 1] / det.psd[k0:k1 + 1])
+        for b in range(1, setup.number_of_bands):
+            s, e = setup.start_end_idxs[b]
+            ks, ke = setup.Ks_Ke[b]
+            m = int(setup.Mbs[b])
+            nhat = min(2 * m, int(setup.Nbs[b]))
+            wth = np.zeros(m // 2 + 1, dtype=complex)
+            wth[ks:ke + 1] = setup.square_root_windows[s:e + 1] * strain[s:e + 1]
+            x = np.fft.irfft(wth)
+            lag = np.fft.irfft(inv[:int(setup.Nbs[b]) // 2 + 1])
+            ac = np.correlate(x, x, mode="full")[m - 1:]          # sum_n x_n x_{n+d}, d >= 0
+            quad = lag[0] * ac[0] + 2 * np.dot(lag[1:m], ac[1:m])
+            kept = lag[np.minimum(np.arange(nhat), nhat - np.arange(nhat))]
+            sign = (-1.0) ** np.arange(nhat)
+            t0, tm = np.sum(x), (-1.0) ** m * np.dot((-1.0) ** np.arange(m), x)
+            S = (nhat * quad + np.sum(kept) * t0 ** 2 + np.dot(sign, kept) * tm ** 2) / 2
+            lag_form += 4 / setup.Tbhats[b] * S
+        oracle = mb.ifft_fft_optimal_snr_squared(strain, det)
+        assert oracle > 10
+        assert abs(oracle - lag_form) < 1e-10 * oracle, (det.name, oracle, lag_form)
+
+
+def test_ifft_fft_device_tables_reproduce_the_oracle():
+    """The device split of the IFFT-FFT <h|h> (MultibandSetup.ifft_fft_tables:
+    diagonal coefficients on the banded points, odd-bin weights per band)
+    evaluated with the GPU kernels' algebra in numpy -- packed half-length
+    inverse transform, half-bin modulation, forward transform, even / odd
+    separation -- equals the oracle's <h|h> per detector."""
+    ev, ifos, inj = _event()
+    setup = _setup(ifos, inj, linear_interpolation=False)
+    mb = OracleMultiband(ev, "taylorf2", setup, waveform_arguments=WF)
+    diag, bands = setup.ifft_fft_tables()
+    assert len(bands) == setup.number_of_bands - 1
+    for di, det in enumerate(ev.detectors):
+        strain = _band_strain(mb, det, inj)
+        hh = np.sum(diag[det.name] * np.abs(strain) ** 2)
+        for band in bands:
+            mh, p0, L, ks = (band[k] for k in ("half_length", "first_point", "n_points",
+                                              "first_bin"))
+            X = np.zeros(mh + 1, dtype=complex)
+            X[ks:ks + L] = band["sqrt_window"] * strain[p0:p0 + L]
+            k = np.arange(mh)
+            xm = np.conj(X[mh - k])
+            zc = (X[k] + xm) / 2 + 1j * np.exp(1j * np.pi * k / mh) * (X[k] - xm) / 2
+            u = np.fft.ifft(zc) * mh * np.exp(-1j * np.pi * k / mh)
+            U = np.fft.fft(u)
+            ur = np.conj(U[mh - 1 - k])
+            Y = ((U + ur) / 2 + np.exp(-1j * np.pi * (2 * k + 1) / (2 * mh)) * (U - ur) / 2j) / mh
+            hh += np.sum(band["odd_weights"][di] * np.abs(Y) ** 2)
+        oracle = mb.ifft_fft_optimal_snr_squared(strain, det)
+        assert abs(hh - oracle) < 1e-11 * oracle, (det.name, hh, oracle)
+
+
+def test_ifft_fft_weights_round_trip(tmp_path):
+    """IFFT-FFT weights saved and read back (multiband.py:648-712): the
+    algorithm flag, T^_b, the windows and each detector's Ibc per band."""
+    from bilby_amd.multiband import MultibandSetup
+    ev, ifos, inj = _event(seed=6)
+    setup = _setup(ifos, inj, linear_interpolation=False)
+    w = setup.weights
+    assert w["linear_interpolation"] is False and "quadratic_coeffs" not in w
+    assert set(w["Ibcs"]["L1"]) == {str(b) for b in range(setup.number_of_bands)}
+    fn = str(tmp_path / "mb_ifft.npz")
+    setup.save_weights(fn)
+    back = MultibandSetup.from_weights(fn, ifos)
+    assert back.linear_interpolation is False
+    assert np.array_equal(back.Tbhats, setup.Tbhats)
+    assert np.array_equal(back.square_root_windows, setup.square_root_windows)
+    for name in setup.Ibcs:
+        for b in range(setup.number_of_bands):
+            assert np.array_equal(back.Ibcs[name][b], setup.Ibcs[name][b])
+    d0, b0 = setup.ifft_fft_tables()
+    d1, b1 = back.ifft_fft_tables()
+    for name in d0:
+        assert np.array_equal(d0[name], d1[name])
+    for x, y in zip(b0, b1):
+        assert np.array_equal(x["odd_weights"], y["odd_weights"])
+    with pytest.raises(TypeError):
+        _setup(ifos, inj, linear_interpolation=1)
