@@ -513,6 +513,18 @@ def bench_multiband(steps, batch=4096, duration=128.0):
     out["mb_fp32_vs_mb_fp64_max"] = float(np.max(np.abs(mb32 - mb64)))
     out["mb_vs_full_fp64_max"] = float(np.max(np.abs(mb64 - f64)))
     out["lnL_max"] = float(np.max(f64))
+    # the IFFT-FFT <h|h> (linear_interpolation=False) on the same bands
+    t0 = time.perf_counter()
+    lki = MBGravitationalWaveTransient(ifos, wfg, reference_chirp_mass=mc, priors=priors.copy(),
+                                       distance_marginalization=True, phase_marginalization=True,
+                                       linear_interpolation=False, precision="fp32")
+    out["ifft_setup_s"] = time.perf_counter() - t0
+    out["ifft_half_lengths"] = [int(m) // 2 for m in lki.setup.Mbs[1:]]
+    for prec in ("fp32", "fp64"):
+        dt = timed(lambda: lki.log_likelihood_ratio_batch(rows, precision=prec), steps)
+        out["mb_ifft_%s_evals_per_s" % prec] = batch / dt
+    i64 = lki.log_likelihood_ratio_batch(rows, precision="fp64").cpu().numpy()
+    out["mb_ifft_vs_full_fp64_max"] = float(np.max(np.abs(i64 - f64)))
     return out
 
 

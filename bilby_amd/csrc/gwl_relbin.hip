@@ -538,11 +538,16 @@ __device__ __forceinline__ void fft_pass(double2* x, int P, int Ns, const double
 #pragma unroll
     for (int q = 0; q < R; ++q) v[g][q] = x[lpad(j + q * PR)];
     if (Ns > 1) {
-      const int step = (P / (Ns * R)) * tstep;  // W_P^{k q P / (Ns R)} = tab[k q step]
+      // W_{Ns R}^{k q} = w^q with w = tab[k (P / (Ns R)) tstep]: one table load
+      // per butterfly, the powers by multiplication (a few units in the last
+      // place after R - 1 products)
+      const double2 t = tab[k * (P / (Ns * R)) * tstep];
+      const double2 w = INV ? conj2(t) : t;
+      double2 wq = w;
 #pragma unroll
       for (int q = 1; q < R; ++q) {
-        const double2 w = tab[k * q * step];
-        v[g][q] = cmul(v[g][q], INV ? conj2(w) : w);
+        v[g][q] = cmul(v[g][q], wq);
+        if (q + 1 < R) wq = cmul(wq, w);
       }
     }
     dft_reg<R, INV>(v[g]);
