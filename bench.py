@@ -435,7 +435,11 @@ def bench_calibration(steps, batch=2048, n_curves=1000):
     out["time_tdp_fp32_evals_per_s"] = tbatch / dt
     out["time_tdp_ms_per_batch"] = dt * 1e3
     out["time_tdp_curve_transforms_per_s"] = tbatch * n_curves / dt
+    dt64 = timed(lambda: lk2.log_likelihood_ratio_batch(rows2, precision="fp64"), 1, warmup=1)
+    out["time_tdp_fp64_evals_per_s"] = tbatch / dt64
+    out["time_tdp_fp64_ms_per_batch"] = dt64 * 1e3
     r2 = lk2.log_likelihood_ratio_batch(rows2).cpu().numpy()
+    r64 = lk2.log_likelihood_ratio_batch(rows2, precision="fp64").cpu().numpy()
     ev2 = OracleEvent([OracleDetector(i.name, i.frequency_array, i.frequency_domain_strain,
                                       i.power_spectral_density_array, i.frequency_mask)
                        for i in ifos2], ifos2.duration, ifos2.sampling_frequency,
@@ -449,6 +453,7 @@ def bench_calibration(steps, batch=2048, n_curves=1000):
     ref2 = np.array([olk2.log_likelihood_ratio(p) for p in draws_as_dicts(draws2, range(2))])
     out["time_tdp_oracle_evals_per_s_1core"] = 2 / (time.perf_counter() - t0)
     out["time_tdp_fp32_vs_oracle_2rows"] = float(np.max(np.abs(r2[:2] - ref2)))
+    out["time_tdp_fp64_vs_oracle_2rows"] = float(np.max(np.abs(r64[:2] - ref2)))
     return out
 
 

@@ -74,6 +74,10 @@ struct gwl_calib {
   bool timem = false;
   float2* cal_t = nullptr;
   double* calps = nullptr;
+  // float64 evaluations: the curves' float32 remainders (same layout as
+  // cal_t, cal = hi + lo to 48 bits) and W_N^m = exp(-2 pi i m / N), m < N
+  float2* cal_lo = nullptr;
+  double2* wN = nullptr;
 };
 
 namespace gwl {
@@ -309,11 +313,14 @@ int calib_create_time(gwl_event* ev, int nc, const double* curves, int K, gwl_ca
   GWL_HIP_CHECK(hipSetDevice(ev->ctx->device));
   std::vector<double> hps((size_t)nd * (N + 2));
   GWL_HIP_CHECK(hipMemcpy(hps.data(), d.hps, hps.size() * 8, hipMemcpyDeviceToHost));
-  std::vector<float2> tab;
+  std::vector<float2> tab, tab_lo;
   std::vector<double> ps;
+  std::vector<double2> wn;
   try {
     tab.assign((size_t)nc * nd * N, make_float2(1.f, 0.f));
+    tab_lo.assign((size_t)nc * nd * N, make_float2(0.f, 0.f));
     ps.assign((size_t)nc * nd * (N + 2), 0.0);
+    wn.resize((size_t)N);
   } catch (...) {
     set_error("out of host memory for the calibration tables");
     return GWL_E_NOMEM;
@@ -332,7 +339,13 @@ int calib_create_time(gwl_event* ev, int nc, const double* curves, int K, gwl_ca
         if (n >= lo && n <= hi) {
           const double* v = curves + ((size_t)cv * K + k0 + (n - lo)) * 2;
           c2 = v[0] * v[0] + v[1] * v[1];
-          if (n < N) t[(size_t)(n % d.Q) * d.P + n / d.Q] = make_float2((float)v[0], (float)v[1]);
+          if (n < N) {
+            const size_t e = (size_t)(n % d.Q) * d.P + n / d.Q;
+            const float hr = (float)v[0], hi = (float)v[1];
+            t[e] = make_float2(hr, hi);
+            tab_lo[((size_t)cv * nd + k) * N + e] =
+                make_float2((float)(v[0] - (double)hr), (float)(v[1] - (double)hi));
+          }
         }
         acc += (h[n + 1] - h[n]) * c2;
         p[n + 1] = acc;
@@ -346,8 +359,16 @@ int calib_create_time(gwl_event* ev, int nc, const double* curves, int K, gwl_ca
   c->nc = nc;
   c->K = K;
   c->timem = true;
+  {
+    const long double step = 2.0L * 3.141592653589793238462643383279502884L / (long double)N;
+    for (int m = 0; m < N; ++m) {
+      const long double a = step * (long double)m;
+      wn[m] = make_double2((double)std::cos(a), (double)-std::sin(a));
+    }
+  }
   const size_t bt = up(tab.size() * sizeof(float2), 256), bp = up(ps.size() * 8, 256);
-  c->data_bytes = bt + bp;
+  const size_t bw = up(wn.size() * sizeof(double2), 256);
+  c->data_bytes = 2 * bt + bp + bw;
   if (hipMalloc(&c->data, c->data_bytes) != hipSuccess) {
     delete c;
     set_error("hipMalloc failed for the calibration curves");
@@ -355,8 +376,14 @@ int calib_create_time(gwl_event* ev, int nc, const double* curves, int K, gwl_ca
   }
   c->cal_t = static_cast<float2*>(c->data);
   c->calps = reinterpret_cast<double*>(static_cast<char*>(c->data) + bt);
+  c->cal_lo = reinterpret_cast<float2*>(static_cast<char*>(c->data) + bt + bp);
+  c->wN = reinterpret_cast<double2*>(static_cast<char*>(c->data) + 2 * bt + bp);
   hipError_t err = hipMemcpy(c->cal_t, tab.data(), tab.size() * sizeof(float2), hipMemcpyHostToDevice);
   if (err == hipSuccess) err = hipMemcpy(c->calps, ps.data(), ps.size() * 8, hipMemcpyHostToDevice);
+  if (err == hipSuccess)
+    err = hipMemcpy(c->cal_lo, tab_lo.data(), tab_lo.size() * sizeof(float2), hipMemcpyHostToDevice);
+  if (err == hipSuccess)
+    err = hipMemcpy(c->wN, wn.data(), wn.size() * sizeof(double2), hipMemcpyHostToDevice);
   if (err != hipSuccess) {
     (void)hipFree(c->data);
     delete c;
@@ -490,9 +517,9 @@ int gwl_calib_eval(gwl_calib* c, const double* params, int64_t n, int32_t ld, in
     set_error("gwl_calib_eval: too many rows in one call");
     return GWL_E_INVALID;
   }
-  if (c->timem && mode != gwl::CAL_FP32) {
+  if (c->timem && mode == gwl::CAL_BF16X3) {
     set_error("gwl_calib_eval: with time marginalisation the curves' transforms run in float32 "
-              "(mode GWL_ROQ_FP32)");
+              "or float64 (mode GWL_ROQ_FP32 or GWL_ROQ_FP64)");
     return GWL_E_UNSUPPORTED;
   }
   if (int rc = gwl::cal_reserve(c, n)) return rc;
@@ -502,7 +529,8 @@ int gwl_calib_eval(gwl_calib* c, const double* params, int64_t n, int32_t ld, in
     hipStream_t st = reinterpret_cast<hipStream_t>(stream);
     if (int rc = gwl::timing_begin(ev, st)) return rc;
     if (int rc = gwl::launch_cal_time(ev, params, n, ld, c->nc, c->cal_t, c->calps, c->ws,
-                                      per_curve, out, st))
+                                      per_curve, out, st, mode == gwl::CAL_FP64, c->cal_lo,
+                                      c->wN))
       return rc;
     return gwl::timing_end(ev, st);
   }

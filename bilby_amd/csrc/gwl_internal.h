@@ -150,9 +150,12 @@ int launch_time_series(gwl_event* ev, const double* params, int64_t n, int ld, i
 // Calibration with time marginalisation (gwl_time_wave.hip)
 bool cal_time_eligible(const gwl_event* ev);
 size_t cal_time_ws_bytes(int64_t rows, int nc);
+// fp64: the float64 kernel, with the curves' float32 remainders cal_lo and
+// the table wN[m] = W_N^m (m < N)
 int launch_cal_time(gwl_event* ev, const double* params, int64_t n, int ld, int nc,
                     const float2* cal, const double* calps, void* ws, double* per_curve,
-                    double* out, hipStream_t stream);
+                    double* out, hipStream_t stream, bool fp64 = false,
+                    const float2* cal_lo = nullptr, const double2* wN = nullptr);
 size_t multi_wave_lds_bytes();
 int multi_launch_wave(gwl_multi* m, hipStream_t stream);
 }  // namespace gwl

@@ -93,12 +93,17 @@ struct RowArgs {
   int ld;
   long long n;
   RowW* rows;
+  double2* c64 = nullptr;  // [n][GWL_MAX_DET] float64 detector amplitudes, or null
 };
 
 __device__ inline void rowconst_one(const EvDev& ev, const double* row, RowW* out,
-                                    const Tf2Part* pre = nullptr) {
+                                    const Tf2Part* pre = nullptr, double2* c64 = nullptr) {
   SampleConst sc;
   compute_sample_const(ev, row, sc, pre);
+  if (c64)
+#pragma unroll
+    for (int d = 0; d < GWL_MAX_DET; ++d)
+      c64[d] = d < ev.nd ? make_double2(sc.cre[d], sc.cim[d]) : make_double2(0.0, 0.0);
   RowW w;
 #pragma unroll
   for (int k = 0; k < 9; ++k) w.c[k] = sc.c[k];
@@ -167,7 +172,8 @@ __global__ __launch_bounds__(256) void k_rowconst(RowArgs a) {
   if (live && (threadIdx.x & 1)) tf2_part(a.ev, a.params + r * a.ld, t[k]);
   __syncthreads();
   if (live && !(threadIdx.x & 1))
-    rowconst_one(a.ev, a.params + r * a.ld, a.rows + r, a.ev.approx == GWL_TAYLORF2 ? &t[k] : nullptr);
+    rowconst_one(a.ev, a.params + r * a.ld, a.rows + r, a.ev.approx == GWL_TAYLORF2 ? &t[k] : nullptr,
+                 a.c64 ? a.c64 + r * GWL_MAX_DET : nullptr);
 }
 
 // The event fields the row loop reads.
@@ -604,7 +610,84 @@ struct CalTimeArgs {
   const float2* cal;    // [nc][nd][N] curve values, decimated order
   const double* calps;  // [nc][nd][N + 2] prefix sums of f^-7/3 sigma^2 / S |cal|^2
   double* lnl;          // [n][nc] time-marginalised lnL per curve
+  // float64 form: the curves' float32 remainders (cal = hi + lo to 48 bits),
+  // the rows' float64 detector amplitudes, W_N^m (m < N)
+  const float2* cal_lo;
+  const double2* c64;
+  const double2* wN;
 };
+
+// The float64 integrand of each detector at decimated element i (bin nbin):
+// C_d exp(-2 pi i (Psi(f) / 2 pi + f dt_d)) f^-7/6 conj(d_d / S_d) sigma.
+template <int ND>
+__device__ __forceinline__ void integrand_tf2_det64(const WaveEv& ev, const RowW& rw,
+                                                    const double2* c64, const double2* dw64,
+                                                    int i, int nbin, double2* out) {
+  const double f = (double)nbin * ev.df;
+  const double* tb = ev.tf2b + 3 * (size_t)i;
+  const double u = tb[0], v = tb[1], lf = tb[2];
+  const double v2 = v * v, v3 = v2 * v;
+  double s = rw.c[4];
+  s = fma(rw.c[0], v3 * v2, s);
+  s = fma(rw.c[1], v3, s);
+  s = fma(rw.c[2], v2, s);
+  s = fma(rw.c[3], v, s);
+  s = fma(rw.c[5], u, s);
+  s = fma(rw.c[6], u * u, s);
+  s = fma(rw.c[7], lf, s);
+  s = fma(rw.c[8], u * lf, s);
+  const double amp = v3 * sqrt(v);  // f^-7/6 (v = f^-1/3)
+#pragma unroll
+  for (int d = 0; d < ND; ++d) {
+    const double cyc = fma(f, rw.dt[d], s);
+    const double2 h = cmul(c64[d], expm2pi(cyc - floor(cyc)));
+    const double2 w = dw64[(size_t)d * ev.N + i];
+    const double wr = w.x * amp, wi = w.y * amp;
+    out[d] = make_double2(h.x * wr + h.y * wi, h.y * wr - h.x * wi);
+  }
+}
+
+// float64 form of wave_transform_accumulate: the three radix-8 passes in
+// float64 through the wave's exchange buffer, the window accumulation with
+// W_N^(n2 j mod N) from the event's float64 table.
+__device__ __forceinline__ void wave_transform_accumulate64(const WaveEv& ev, double2* x, int n2,
+                                                            const double2* tw, const double2* tw3,
+                                                            double2* xb, int lane,
+                                                            const double2* wN, double* accr,
+                                                            double* acci) {
+  const int k2 = 8 * (lane & 7);
+  const int base2 = 64 * (lane >> 3) + (lane & 7);
+  dft8(x);
+#pragma unroll
+  for (int s = 0; s < 8; ++s) xb[xpad(8 * lane + s)] = x[s];
+  wave_sync();
+#pragma unroll
+  for (int s = 0; s < 8; ++s) x[s] = xb[xpad(lane + 64 * s)];
+#pragma unroll
+  for (int s = 1; s < 8; ++s) x[s] = cmul(x[s], tw[(s * k2) & (kWP - 1)]);
+  dft8(x);
+  wave_sync();
+#pragma unroll
+  for (int s = 0; s < 8; ++s) xb[xpad(base2 + 8 * s)] = x[s];
+  wave_sync();
+#pragma unroll
+  for (int s = 0; s < 8; ++s) x[s] = xb[xpad(lane + 64 * s)];
+#pragma unroll
+  for (int s = 1; s < 8; ++s) x[s] = cmul(x[s], tw3[(s - 1) * 64 + lane]);
+  dft8(x);
+  wave_sync();
+  const unsigned nmask = (unsigned)ev.N - 1u;
+#pragma unroll
+  for (int s = 0; s < 8; ++s) {
+    const int jj = (lane + 64 * s - ev.jlo) & (kWP - 1);
+    if (jj < ev.L) {
+      const unsigned j = (unsigned)(ev.jlo + jj);
+      const double2 z = cmul(wN[((unsigned)n2 * j) & nmask], x[s]);
+      accr[s] += z.x;
+      acci[s] += z.y;
+    }
+  }
+}
 
 template <int ND>
 __global__ __launch_bounds__(64 * kWaves, 2) void k_cal_time_w(CalTimeArgs a) {
@@ -674,6 +757,132 @@ __global__ __launch_bounds__(64 * kWaves, 2) void k_cal_time_w(CalTimeArgs a) {
     }
   }
   // epilogue per curve: time-marginalised lnL (base.py:794-820 per curve)
+  const double norm = 4.0 / ev.T;
+  const double hs = rw.hscale, jit = rw.jitter;
+  const double b = time_lse_scale(ev, ev.tdelta);
+#pragma unroll
+  for (int cc = 0; cc < kCalCurves; ++cc) {
+    if (cc >= ncb) break;
+    const int c = c0 + cc;
+    double hh = 0.0;
+    if (kb <= ke) {
+#pragma unroll
+      for (int d = 0; d < ND; ++d) {
+        const double* ps = a.calps + ((size_t)c * ND + d) * (ev.N + 2);
+        hh += rw.cabs2[d] * (ps[ke + 1] - ps[kb]);
+      }
+    }
+    hh *= norm;
+    SplineY yy;
+    if (dist) yy = spline_y(ev, knots + ev.sp.nx + 4, hh * hs * hs);
+    double m = -INFINITY, sum = 0.0;
+#pragma unroll
+    for (int s = 0; s < 8; ++s) {
+      const int jj = (lane + 64 * s - ev.jlo) & (kWP - 1);
+      if (jj < ev.L) {
+        const int j = ev.jlo + jj;
+        const double t = (ev.start + (double)(j + 1) * ev.tstep) + jit;
+        if (t >= ev.tmin && t <= ev.tmax)
+          lse_push(m, sum, epi_value(ev, knots, yy, hh, hs, norm * accr[cc][s],
+                                     norm * acci[cc][s]) +
+                               time_log_weight(ev, t));
+      }
+    }
+    lse_wave(m, sum);
+    if (lane == 0) a.lnl[(size_t)r * a.nc + c] = (m > -INFINITY) ? log(b * sum) + m : -INFINITY;
+  }
+}
+
+// The float64 form of k_cal_time_w (base.py:305-324 in float64): the
+// integrand in float64 (float64 sin / cos of the phase, float64 amplitudes
+// and data), curve values hi + lo, float64 transforms and twiddles.  Tables
+// in LDS: W_512^m and the pass-3 twiddles in float64, the exchange buffers.
+// The float64 integrand of a curve pair needs up to 256 registers per lane:
+// one workgroup per CU is asked for so that nothing spills.
+template <int ND>
+__global__ __launch_bounds__(64 * kWaves, 1) void k_cal_time_w64(CalTimeArgs a) {
+  extern __shared__ double lds_d[];
+  const EvDev& ev = a.ev;
+  double2* tw = reinterpret_cast<double2*>(lds_d);
+  double2* tw3 = tw + kWP;
+  double2* xch = tw3 + kTw3;
+  double* knots = reinterpret_cast<double*>(xch + kWaves * kXch);
+  const bool dist = (ev.flags & GWL_DISTANCE_MARG) != 0;
+  for (int k = threadIdx.x; k < kWP; k += 64 * kWaves) {
+    double sn, cs;
+    sincospi(-2.0 * (double)k / (double)kWP, &sn, &cs);
+    tw[k] = make_double2(cs, sn);
+  }
+  for (int k = threadIdx.x; k < kTw3; k += 64 * kWaves) {
+    const int e = ((k / 64 + 1) * (k & 63)) & (kWP - 1);
+    double sn, cs;
+    sincospi(-2.0 * (double)e / (double)kWP, &sn, &cs);
+    tw3[k] = make_double2(cs, sn);
+  }
+  if (dist) stage_knots(ev, knots);
+  __syncthreads();
+  const int lane = threadIdx.x & 63;
+  const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const long long cb = blockIdx.x / a.rowblocks;
+  const long long r = (blockIdx.x - cb * a.rowblocks) * kWaves + wid;
+  if (r >= a.n) return;
+  const int c0 = (int)cb * kCalCurves;
+  const int ncb = min(kCalCurves, a.nc - c0);
+  const RowW& rw = a.rows[r];
+  if (!rw.valid) {
+    if (lane < ncb) a.lnl[(size_t)r * a.nc + c0 + lane] = row_valid_sentinel();
+    return;
+  }
+  double2 c64[ND];
+#pragma unroll
+  for (int d = 0; d < ND; ++d) c64[d] = a.c64[r * GWL_MAX_DET + d];
+  const WaveEv wv = wave_ev(ev);
+  const int Q = ev.Q;
+  const int kb = rw.kb, ke = rw.ke;
+  double2* xb = xch + wid * kXch;
+  double accr[kCalCurves][8], acci[kCalCurves][8];
+#pragma unroll
+  for (int cc = 0; cc < kCalCurves; ++cc)
+#pragma unroll
+    for (int s = 0; s < 8; ++s) accr[cc][s] = acci[cc][s] = 0.0;
+  for (int n2 = 0; n2 < Q; ++n2) {
+    const int ilo = (kb - n2 + Q - 1) / Q;
+    const int ihi = ke >= n2 ? (ke - n2) / Q : -1;
+    if (ilo > ihi) continue;
+    double2 xd[8][ND];
+#pragma unroll
+    for (int s = 0; s < 8; ++s) {
+      const int i = lane + 64 * s;
+      if (i >= ilo && i <= ihi) {
+        integrand_tf2_det64<ND>(wv, rw, c64, ev.dw64, n2 * kWP + i, Q * i + n2, xd[s]);
+      } else {
+#pragma unroll
+        for (int d = 0; d < ND; ++d) xd[s][d] = make_double2(0.0, 0.0);
+      }
+    }
+#pragma unroll
+    for (int cc = 0; cc < kCalCurves; ++cc) {
+      if (cc >= ncb) break;
+      const size_t off = (size_t)(c0 + cc) * ND * ev.N + (size_t)n2 * kWP;
+      const float2* hi = a.cal + off;
+      const float2* lo = a.cal_lo + off;
+      double2 x[8];
+#pragma unroll
+      for (int s = 0; s < 8; ++s) {
+        double xr = 0.0, xi = 0.0;
+#pragma unroll
+        for (int d = 0; d < ND; ++d) {
+          const size_t e = (size_t)d * ev.N + lane + 64 * s;
+          const float2 kh = hi[e], kl = lo[e];
+          const double kr = (double)kh.x + (double)kl.x, ki = (double)kh.y + (double)kl.y;
+          xr += kr * xd[s][d].x - ki * xd[s][d].y;
+          xi += kr * xd[s][d].y + ki * xd[s][d].x;
+        }
+        x[s] = make_double2(xr, xi);
+      }
+      wave_transform_accumulate64(wv, x, n2, tw, tw3, xb, lane, a.wN, accr[cc], acci[cc]);
+    }
+  }
   const double norm = 4.0 / ev.T;
   const double hs = rw.hscale, jit = rw.jitter;
   const double b = time_lse_scale(ev, ev.tdelta);
@@ -847,8 +1056,9 @@ size_t time_wave_lds_bytes(const EvDev& ev, bool fused) {
 }
 
 // k_cal_time_w: the 512-entry pass-2 table, tw3, the exchange buffers, knots
-static size_t cal_time_lds_bytes(const EvDev& ev) {
-  size_t b = (size_t)(kWP + kTw3 + kWaves * kXch) * sizeof(float2);
+// (float64 kernel: the same tables in float64)
+static size_t cal_time_lds_bytes(const EvDev& ev, bool fp64) {
+  size_t b = (size_t)(kWP + kTw3 + kWaves * kXch) * (fp64 ? sizeof(double2) : sizeof(float2));
   if (ev.flags & GWL_DISTANCE_MARG) b += (size_t)(ev.sp.nx + ev.sp.ny + 8) * sizeof(double);
   return b;
 }
@@ -905,19 +1115,32 @@ bool cal_time_eligible(const gwl_event* e) {
          ev.L <= kWP;
 }
 
+// rows: RowW, then the float64 detector amplitudes, then the per-curve lnL
+static size_t cal_time_c64_offset(int64_t rows) {
+  return ((size_t)rows * sizeof(RowW) + 255) / 256 * 256;
+}
+static size_t cal_time_lnl_offset(int64_t rows) {
+  return cal_time_c64_offset(rows) + (size_t)rows * GWL_MAX_DET * sizeof(double2);
+}
+
 size_t cal_time_ws_bytes(int64_t rows, int nc) {
-  return ((size_t)rows * sizeof(RowW) + 255) / 256 * 256 + (size_t)rows * nc * sizeof(double);
+  return cal_time_lnl_offset(rows) + (size_t)rows * nc * sizeof(double);
 }
 
 int launch_cal_time(gwl_event* e, const double* params, int64_t n, int ld, int nc,
                     const float2* cal, const double* calps, void* ws, double* per_curve,
-                    double* out, hipStream_t st) {
+                    double* out, hipStream_t st, bool fp64, const float2* cal_lo,
+                    const double2* wN) {
   const EvDev& ev = e->dev;
+  if (fp64 && (!cal_lo || !wN)) {
+    set_error("calibration time transform: float64 tables missing");
+    return GWL_E_INVALID;
+  }
   char* w = static_cast<char*>(ws);
   RowW* rows = reinterpret_cast<RowW*>(w);
-  double* lnl = per_curve ? per_curve
-                          : reinterpret_cast<double*>(w + ((size_t)n * sizeof(RowW) + 255) / 256 * 256);
-  RowArgs ra{ev, params, ld, n, rows};
+  double2* c64 = reinterpret_cast<double2*>(w + cal_time_c64_offset(n));
+  double* lnl = per_curve ? per_curve : reinterpret_cast<double*>(w + cal_time_lnl_offset(n));
+  RowArgs ra{ev, params, ld, n, rows, fp64 ? c64 : nullptr};
   hipLaunchKernelGGL(k_rowconst, dim3((unsigned)((n + kRowconstRows - 1) / kRowconstRows)),
                      dim3(256), 0, st, ra);
   GWL_HIP_CHECK(hipGetLastError());
@@ -930,7 +1153,10 @@ int launch_cal_time(gwl_event* e, const double* params, int64_t n, int ld, int n
   ca.cal = cal;
   ca.calps = calps;
   ca.lnl = lnl;
-  const size_t lds = cal_time_lds_bytes(ev);
+  ca.cal_lo = cal_lo;
+  ca.c64 = c64;
+  ca.wN = wN;
+  const size_t lds = cal_time_lds_bytes(ev, fp64);
   if (lds > e->ctx->lds_per_block) {
     set_error("calibration time transform exceeds the LDS of a workgroup");
     return GWL_E_UNSUPPORTED;
@@ -941,11 +1167,20 @@ int launch_cal_time(gwl_event* e, const double* params, int64_t n, int ld, int n
     return GWL_E_INVALID;
   }
   const dim3 grid((unsigned)blocks);
-  switch (ev.nd) {
-    case 1: hipLaunchKernelGGL(k_cal_time_w<1>, grid, dim3(64 * kWaves), lds, st, ca); break;
-    case 2: hipLaunchKernelGGL(k_cal_time_w<2>, grid, dim3(64 * kWaves), lds, st, ca); break;
-    case 3: hipLaunchKernelGGL(k_cal_time_w<3>, grid, dim3(64 * kWaves), lds, st, ca); break;
-    default: hipLaunchKernelGGL(k_cal_time_w<4>, grid, dim3(64 * kWaves), lds, st, ca); break;
+  if (fp64) {
+    switch (ev.nd) {
+      case 1: hipLaunchKernelGGL(k_cal_time_w64<1>, grid, dim3(64 * kWaves), lds, st, ca); break;
+      case 2: hipLaunchKernelGGL(k_cal_time_w64<2>, grid, dim3(64 * kWaves), lds, st, ca); break;
+      case 3: hipLaunchKernelGGL(k_cal_time_w64<3>, grid, dim3(64 * kWaves), lds, st, ca); break;
+      default: hipLaunchKernelGGL(k_cal_time_w64<4>, grid, dim3(64 * kWaves), lds, st, ca); break;
+    }
+  } else {
+    switch (ev.nd) {
+      case 1: hipLaunchKernelGGL(k_cal_time_w<1>, grid, dim3(64 * kWaves), lds, st, ca); break;
+      case 2: hipLaunchKernelGGL(k_cal_time_w<2>, grid, dim3(64 * kWaves), lds, st, ca); break;
+      case 3: hipLaunchKernelGGL(k_cal_time_w<3>, grid, dim3(64 * kWaves), lds, st, ca); break;
+      default: hipLaunchKernelGGL(k_cal_time_w<4>, grid, dim3(64 * kWaves), lds, st, ca); break;
+    }
   }
   GWL_HIP_CHECK(hipGetLastError());
   hipLaunchKernelGGL(k_cal_time_epi, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, st, rows,

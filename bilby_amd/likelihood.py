@@ -255,9 +255,9 @@ class GravitationalWaveTransient:
                  distance_marginalization_lookup_table=None, calibration_lookup_table=None,
                  number_of_response_curves=1000, starting_index=0, jitter_time=True,
                  reference_frame="sky", time_reference="geocenter", precision="fp32", device=0):
-        if calibration_marginalization and time_marginalization and precision != "fp32":
+        if calibration_marginalization and time_marginalization and precision == "bf16x3":
             raise NotImplementedError("calibration together with time marginalisation is "
-                                      "evaluated in float32 (precision='fp32')")
+                                      "evaluated in float32 or float64")
         from .frames import sky_frame
         # base.py:170-181: a detector time reference cannot be marginalised over
         self._frame = sky_frame(reference_frame, time_reference)
@@ -491,14 +491,15 @@ class GravitationalWaveTransient:
     def get_calibration_log_likelihoods(self, parameters):
         """base.py:822-858: ln L of every response curve for one parameter
         dict (float64 contraction; with time marginalisation each curve's
-        time-marginalised value, float32 transforms)."""
+        time-marginalised value, transforms in the likelihood's calibration
+        precision)."""
         rows = self.rows_from_parameters(dict(parameters))
         rows_t = torch.from_numpy(np.ascontiguousarray(rows[:1])).to("cuda:%d" % self.device)
         out = torch.empty(1, dtype=torch.float64, device=rows_t.device)
         per = torch.empty((1, self.number_of_response_curves), dtype=torch.float64,
                           device=rows_t.device)
         st = torch.cuda.current_stream(rows_t.device).cuda_stream
-        mode = _lib.FP32 if self.time_marginalization else _lib.FP64
+        mode = self.calibration_mode if self.time_marginalization else _lib.FP64
         check(LIB.gwl_calib_eval(self._calib, ctypes.c_void_p(rows_t.data_ptr()), 1, _lib.NPARAM,
                                  mode, ctypes.c_void_p(out.data_ptr()),
                                  ctypes.c_void_p(per.data_ptr()), ctypes.c_void_p(st)),
@@ -538,14 +539,12 @@ class GravitationalWaveTransient:
         the time / distance / phase marginalisation of this likelihood, as
         parameters['<ifo>_log_likelihood'] (the dict is updated, as there,
         and a copy returned).  Evaluated by one single-detector companion
-        likelihood per detector on the GPU (float64; float32 transforms with
-        calibration and time together); under calibration marginalisation
+        likelihood per detector on the GPU (float64); under calibration marginalisation
         each companion holds this likelihood's curves of its detector, so a
         detector's value is marginalised over its own curves
         (calculate_snrs per interferometer, base.py:296-346, 860-877)."""
         if getattr(self, "_per_detector", None) is None:
-            prec = "fp32" if (self.calibration_marginalization and self.time_marginalization) \
-                else "fp64"
+            prec = "fp64"
             self._per_detector = []
             for ifo in self.interferometers:
                 lk = GravitationalWaveTransient(
@@ -886,9 +885,9 @@ class GravitationalWaveTransient:
         marginalisation."""
         if precision is None:
             return self.calibration_mode if self._calib is not None else self.precision
-        if self._calib is not None and self.time_marginalization and precision != "fp32":
+        if self._calib is not None and self.time_marginalization and precision == "bf16x3":
             raise NotImplementedError("calibration together with time marginalisation is "
-                                      "evaluated in float32")
+                                      "evaluated in float32 or float64")
         if precision == "bf16x3":
             if self._calib is None:
                 raise ValueError("bf16x3 applies to calibration marginalisation only")

@@ -590,8 +590,10 @@ int gwl_mb_time_series(gwl_mb* mb, const double* params, int64_t n_rows, int32_t
  * precisions, (2 n_curves x 2K + n_curves x K) x 16 bytes; the workspace per
  * row ~ 36 K bytes.  With time marginalisation (base.py:305-324; a TaylorF2
  * event on the wave-per-row transform, time window <= 512 bins): the curves
- * in the transform's order and per-curve prefix sums for <h|h>,
- * n_curves x n_det x N x 16 bytes; per row (1 + n_curves) x 8 bytes. */
+ * in the transform's order as float32 pairs hi + lo (the float64 value to
+ * 48 bits) and per-curve prefix sums for <h|h>, n_curves x n_det x N x 24
+ * bytes, and the N twiddles W_N^m in float64; per row (1 + n_curves) x 8
+ * + 64 bytes. */
 typedef struct gwl_calib gwl_calib;
 int gwl_calib_create(gwl_event* ev, int32_t n_curves, const double* curves, gwl_calib** out);
 int gwl_calib_destroy(gwl_calib* cal);
@@ -601,8 +603,9 @@ int gwl_calib_device_bytes(gwl_calib* cal, int64_t* bytes);
  * are contracted with the curves on MFMA in `mode` (enum gwl_roq_mode:
  * GWL_ROQ_FP64, GWL_ROQ_FP32, GWL_ROQ_BF16X3; the uncalibrated totals are
  * float64 in every mode).  With time marginalisation every curve's window
- * transform runs (float32 transform, float64 window sums; mode must be
- * GWL_ROQ_FP32) and each curve's value is time-marginalised
+ * transform runs (GWL_ROQ_FP32: float32 integrand and transform, float64
+ * window sums; GWL_ROQ_FP64: all float64, the curves to 48 bits) and each
+ * curve's value is time-marginalised
  * (base.py:794-820) before the logsumexp over curves (:877).  per_curve (may
  * be NULL): device [n][n_curves], each curve's ln L
  * (get_calibration_log_likelihoods, base.py:822-858). */

@@ -93,8 +93,9 @@ def test_calibration_marginalised_matches_oracle(flags, tmp_path):
 def test_calibration_and_time_marginalised_matches_oracle(case, tmp_path):
     """Calibration together with time marginalisation (k_cal_time_w: one
     transform per curve, base.py:305-324, 862-877) against the oracle's
-    tiled per-curve FFTs: float32 transforms < 1e-3, per-curve values,
-    jittered rows, invalid rows; float64 is refused."""
+    tiled per-curve FFTs: float32 transforms < 1e-3, float64 (k_cal_time_w64,
+    curves to 48 bits) within 1e-8 max(1, |lnL|), per-curve values in both,
+    jittered rows, invalid rows; the split-bfloat16 mode is refused."""
     from bilby_amd import calibration as cal
     from bilby_amd.configs import make_taylorf2_marginalised
     from bilby_amd.likelihood import GravitationalWaveTransient
@@ -131,21 +132,33 @@ def test_calibration_and_time_marginalised_matches_oracle(case, tmp_path):
     draws["time_jitter"] = np.random.default_rng(3).uniform(-1, 1, n) / case["fs"]
     ref = np.array([olk.log_likelihood_ratio(p) for p in draws_as_dicts(draws, range(n))])
     got = lk.log_likelihood_ratio_batch(draws)
-    say("fp32", float(np.max(np.abs(got - ref))), "lnL range", ref.min(), ref.max())
+    got64 = lk.log_likelihood_ratio_batch(draws, precision="fp64")
+    scale = np.maximum(1.0, np.abs(ref))
+    say("fp32", float(np.max(np.abs(got - ref))), "fp64 relative",
+        float(np.max(np.abs(got64 - ref) / scale)), "lnL range", ref.min(), ref.max())
     assert np.all(np.isfinite(ref))
     assert np.max(np.abs(got - ref)) < 1e-3
+    assert np.max(np.abs(got64 - ref) / scale) < 1e-8
     p0 = draws_as_dicts(draws, [2])[0]
     per = lk.get_calibration_log_likelihoods(p0)
     assert per.shape == (case["curves"],)
-    assert np.max(np.abs(per - olk.per_curve_log_likelihoods(p0))) < 1e-3
+    oper = olk.per_curve_log_likelihoods(p0)
+    assert np.max(np.abs(per - oper)) < 1e-3
+    lk.calibration_mode = 0   # float64 per-curve values
+    per64 = lk.get_calibration_log_likelihoods(p0)
+    lk.calibration_mode = 1
+    assert np.max(np.abs(per64 - oper) / np.maximum(1.0, np.abs(oper))) < 1e-8
     bad = {k: v.copy() for k, v in draws.items()}
     bad["mass_1"][5] = -1.0
     out = lk.log_likelihood_ratio_batch(bad)
     assert out[5] == np.nan_to_num(-np.inf)
     keep = [i for i in range(n) if i != 5]
     assert np.array_equal(out[keep], got[keep])
+    out64 = lk.log_likelihood_ratio_batch(bad, precision="fp64")
+    assert out64[5] == np.nan_to_num(-np.inf)
+    assert np.array_equal(out64[keep], got64[keep])
     with pytest.raises(NotImplementedError):
-        lk.log_likelihood_ratio_batch(draws, precision="fp64")
+        lk.log_likelihood_ratio_batch(draws, precision="bf16x3")
 
 
 @pytest.mark.parametrize("time", [False, True])
