@@ -456,31 +456,45 @@ struct I0Series {
 };
 constexpr I0Series kI0Series{};
 
-// sum_{k < 60} c_k q^k in Horner form, unrolled (the coefficients become
+// The large-argument coefficients a_k = ((2k-1)!!)^2 / k!, k < 19: at
+// x >= 30 (y = 1/(8x) <= 1/240) the term a_18 y^18 is below 1e-17 and the
+// terms are still falling, so 19 terms give float64 round-off.
+struct I0Asymptotic {
+  double a[19];
+  constexpr I0Asymptotic() : a() {
+    a[0] = 1.0;
+    for (int k = 1; k < 19; ++k) a[k] = a[k - 1] * (double)((2 * k - 1) * (2 * k - 1)) / (double)k;
+  }
+};
+constexpr I0Asymptotic kI0Asymptotic{};
+
+// sum_{k < K} c_k q^k in Horner form, unrolled (the coefficients become
 // wave-uniform constants)
+template <int K>
 __host__ __device__ __forceinline__ double i0_series(double q) {
-  double s = kI0Series.c[59];
+  double s = kI0Series.c[K - 1];
 #pragma unroll
-  for (int k = 58; k >= 0; --k) s = fma(s, q, kI0Series.c[k]);
+  for (int k = K - 2; k >= 0; --k) s = fma(s, q, kI0Series.c[k]);
   return s;
 }
 
-// Not inlined: inlined into a loop, its 60 coefficients would be hoisted
-// into 120 vector registers of the caller.
+// Not inlined: inlined into a loop, its coefficients would be hoisted into
+// vector registers of the caller.  The series length follows the range (the
+// K of the comment above); lanes of one wavefront hold neighbouring window
+// bins, so they mostly take the same branch.
 __host__ __device__ inline __attribute__((noinline)) double ln_i0(double x) {
   x = fabs(x);
-  if (x < 30.0) return log(i0_series(0.25 * x * x));
-  double y = 1.0 / (8.0 * x);
-  double term = 1.0, sum = 1.0;
-  for (int k = 1; k < 40; ++k) {
-    double a = 2.0 * k - 1.0;
-    double next = term * a * a * y / (double)k;
-    if (next >= term) break;
-    term = next;
-    sum += term;
-    if (term < 1e-17 * sum) break;
+  if (x < 30.0) {
+    const double q = 0.25 * x * x;
+    if (x <= 8.0) return log(i0_series<24>(q));
+    if (x <= 16.0) return log(i0_series<40>(q));
+    return log(i0_series<60>(q));
   }
-  return x - 0.5 * log(kTwoPi * x) + log(sum);
+  const double y = 1.0 / (8.0 * x);
+  double s = kI0Asymptotic.a[18];
+#pragma unroll
+  for (int k = 17; k >= 0; --k) s = fma(s, y, kI0Asymptotic.a[k]);
+  return x - 0.5 * log(kTwoPi * x) + log(s);
 }
 
 // Largest l in [3, ncoef - 1] with t[l] <= x (FITPACK fpbisp interval rule).

@@ -57,12 +57,13 @@ def test_ln_i0_device_matches_scipy(ctx, torch_dev):
     """Device ln I0 vs log(scipy i0e(x)) + |x| (bilby/gw/utils.py:1006-1022);
     the reference's own bar is 1e-10 on linspace(-10, 10, 101)
     (test/gw/utils_test.py:349-352); here 1e-10 absolute or 1e-13 relative
-    over [-10, 10] and logspace(-6, 11)."""
+    over [-10, 10] and logspace(-6, 11), and on both sides of the series
+    lengths' switches (8, 16) and of the large-argument form (30)."""
     import torch
     from scipy.special import i0e
     from bilby_amd._lib import LIB, check
     x = np.concatenate([np.linspace(-10, 10, 101), np.logspace(-6, 11, 2000),
-                        [29.999999, 30.0, 30.000001]])
+                        [8.0, 8.000001, 16.0, 16.000001, 29.999999, 30.0, 30.000001]])
     xt = torch.from_numpy(x).to(torch_dev)
     yt = torch.empty_like(xt)
     check(LIB.gwl_ln_i0(ctx, ctypes.c_void_p(xt.data_ptr()), ctypes.c_void_p(yt.data_ptr()),
