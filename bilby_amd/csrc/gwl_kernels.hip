@@ -379,6 +379,7 @@ struct EpiArgs {
   double2* lse;
   int first, last;
   const double* rinfo;      // [n][4] hscale, jitter, valid from the transform (or null)
+  int gstride;              // groups per row in part / hhpart (0: G)
   const SampleConst* rows;  // [n] row constants, or null (then computed here)
   // live rows and output slots: with count set only rows r < *count exist
   // and row r's results go to out[scatter[r]] (the float64 pass over the
@@ -479,30 +480,100 @@ constexpr int kEpiRows = 4;
 // One wavefront finishes one row: sums the group partials, evaluates the
 // marginalised lnL at every window bin and reduces them with a running-max
 // logsumexp.  ``knots`` holds the spline knots in LDS (distance only).
+// Merge two (running max, scaled sum) logsumexp states.
+__device__ __forceinline__ void lse_merge(double& m, double& s, double mo, double so) {
+  const double mn = fmax(m, mo);
+  if (mn > -INFINITY)
+    s = (m > -INFINITY ? s * exp(m - mn) : 0.0) + (mo > -INFINITY ? so * exp(mo - mn) : 0.0);
+  else
+    s = 0.0;
+  m = mn;
+}
+
+// A row's epilogue inputs: <h|h> summed over its G groups (row stride Gs),
+// jitter, distance scale and validity (from the transform's rinfo, or
+// recomputed).
+__device__ __forceinline__ void epi_row_inputs(const EvDev& ev, const double* params, int ld,
+                                              long long r, int G, int Gs, const double* hhpart,
+                                              const double* rinfo, double& hh, double& jit,
+                                              double& hs, int& valid) {
+  hh = 0.0;
+  jit = 0.0;
+  hs = 1.0;
+  valid = 1;
+  if (rinfo) {  // written by the transform's first group
+    hs = rinfo[4 * r + 0];
+    jit = rinfo[4 * r + 1];
+    valid = rinfo[4 * r + 2] != 0.0;
+  } else {
+    SampleConst sc;
+    compute_sample_const(ev, params + r * ld, sc);
+    jit = sc.jitter;
+    hs = sc.hscale;
+    valid = sc.valid;
+  }
+#pragma unroll 8
+  for (int g = 0; g < G; ++g) hh += hhpart[(size_t)r * Gs + g];
+  hh *= 4.0 / ev.T;
+}
+
+// The time-conditioned lnL at window bin jj plus the log time-prior weight
+// (-inf outside the prior).
+__device__ __forceinline__ double epi_bin_value(const EvDev& ev, const double* knots,
+                                                const SplineY& yy, double hh, double hs,
+                                                double jit, const double2* prow, int G, int jj) {
+  const int j = ev.jlo + jj;
+  const double t = (ev.start + (double)(j + 1) * ev.tstep) + jit;
+  if (!(t >= ev.tmin && t <= ev.tmax)) return -INFINITY;
+  const double norm = 4.0 / ev.T;
+  double xr = 0.0, xi = 0.0;
+#pragma unroll 8
+  for (int g = 0; g < G; ++g) {
+    const double2 v = prow[(size_t)g * ev.L + jj];
+    xr += v.x;
+    xi += v.y;
+  }
+  xr *= norm;
+  xi *= norm;
+  double v;
+  if (ev.flags & GWL_DISTANCE_MARG) {
+    const double d = (ev.flags & GWL_PHASE_MARG) ? hypot(xr, xi) * hs : xr * hs;
+    v = spline_x(ev, knots, yy, d);
+  } else if (ev.flags & GWL_PHASE_MARG) {
+    v = ln_i0(hypot(xr, xi)) - hh / 2.0;
+  } else {
+    v = xr - hh / 2.0;
+  }
+  return v + time_log_weight(ev, t);
+}
+
+// The row's logsumexp state merged with the earlier slices' (lse), then
+// stored for the next slice or turned into the output.
+__device__ __forceinline__ void epi_finish(const EvDev& ev, long long r, double m, double s,
+                                           double* out, double2* lse, int first, int last) {
+  if (!first) {  // merge the slices before this one
+    const double2 st = lse[r];
+    lse_merge(m, s, st.x, st.y);
+  }
+  if (!last) {
+    lse[r] = make_double2(m, s);
+  } else {
+    const double b = time_lse_scale(ev, ev.tdelta);
+    out[r] = (m > -INFINITY) ? log(b * s) + m : -INFINITY;
+  }
+}
+
+// base.py:794-820 for one row, one wavefront: group sums, marginalised lnL
+// per window bin, logsumexp.
 __device__ inline void epi_time_row(const EvDev& ev, const double* params, int ld, long long r,
                                     int G, const double2* part, const double* hhpart, double* out,
                                     const double* knots, double2* lse = nullptr, int first = 1,
-                                    int last = 1, const double* rinfo = nullptr) {
+                                    int last = 1, const double* rinfo = nullptr, int Gs = 0) {
+  if (Gs <= 0) Gs = G;  // groups per row in the buffers (G of them summed here)
   const int lane = threadIdx.x & 63;
-  const bool dist = (ev.flags & GWL_DISTANCE_MARG) != 0;
   double hh = 0.0, jit = 0.0, hs = 1.0;
   int valid = 1;
-  if (lane == 0) {
-    if (rinfo) {  // written by the transform's first group
-      hs = rinfo[4 * r + 0];
-      jit = rinfo[4 * r + 1];
-      valid = rinfo[4 * r + 2] != 0.0;
-    } else {
-      SampleConst sc;
-      compute_sample_const(ev, params + r * ld, sc);
-      jit = sc.jitter;
-      hs = sc.hscale;
-      valid = sc.valid;
-    }
-#pragma unroll 8
-    for (int g = 0; g < G; ++g) hh += hhpart[(size_t)r * G + g];
-    hh *= 4.0 / ev.T;
-  }
+  if (lane == 0) epi_row_inputs(ev, params, ld, r, G, Gs, hhpart, rinfo, hh, jit, hs, valid);
   hh = __shfl(hh, 0, 64);
   jit = __shfl(jit, 0, 64);
   hs = __shfl(hs, 0, 64);
@@ -511,69 +582,62 @@ __device__ inline void epi_time_row(const EvDev& ev, const double* params, int l
     if (lane == 0) out[r] = row_valid_sentinel();
     return;
   }
-  const double norm = 4.0 / ev.T;
   SplineY yy;
-  if (dist) yy = spline_y(ev, knots + ev.sp.nx + 4, hh * hs * hs);
+  if (ev.flags & GWL_DISTANCE_MARG) yy = spline_y(ev, knots + ev.sp.nx + 4, hh * hs * hs);
   // single-pass (running max, scaled sum) logsumexp per lane, then merged
   double m = -INFINITY, s = 0.0;
-  const double2* prow = part + (size_t)r * G * ev.L;
-  for (int jj = lane; jj < ev.L; jj += 64) {
-    const int j = ev.jlo + jj;
-    const double t = (ev.start + (double)(j + 1) * ev.tstep) + jit;
-    if (!(t >= ev.tmin && t <= ev.tmax)) continue;
-    double xr = 0.0, xi = 0.0;
-#pragma unroll 8
-    for (int g = 0; g < G; ++g) {
-      const double2 v = prow[(size_t)g * ev.L + jj];
-      xr += v.x;
-      xi += v.y;
-    }
-    xr *= norm;
-    xi *= norm;
-    double v;
-    if (dist) {
-      const double d = (ev.flags & GWL_PHASE_MARG) ? hypot(xr, xi) * hs : xr * hs;
-      v = spline_x(ev, knots, yy, d);
-    } else if (ev.flags & GWL_PHASE_MARG) {
-      v = ln_i0(hypot(xr, xi)) - hh / 2.0;
-    } else {
-      v = xr - hh / 2.0;
-    }
-    v += time_log_weight(ev, t);
-    if (v > m) {
-      s = (m > -INFINITY ? s * exp(m - v) : 0.0) + 1.0;
-      m = v;
-    } else if (v > -INFINITY) {
-      s += exp(v - m);
-    }
-  }
+  const double2* prow = part + (size_t)r * Gs * ev.L;
+  for (int jj = lane; jj < ev.L; jj += 64) lse_push(m, s, epi_bin_value(ev, knots, yy, hh, hs, jit, prow, G, jj));
 #pragma unroll
-  for (int off = 32; off > 0; off >>= 1) {
-    const double mo = __shfl_xor(m, off, 64), so = __shfl_xor(s, off, 64);
-    const double mn = fmax(m, mo);
-    if (mn > -INFINITY) {
-      s = (m > -INFINITY ? s * exp(m - mn) : 0.0) + (mo > -INFINITY ? so * exp(mo - mn) : 0.0);
-    } else {
-      s = 0.0;
-    }
-    m = mn;
+  for (int off = 32; off > 0; off >>= 1) lse_merge(m, s, __shfl_xor(m, off, 64), __shfl_xor(s, off, 64));
+  if (lane == 0) epi_finish(ev, r, m, s, out, lse, first, last);
+}
+
+// The same for few rows: one workgroup per row, its threads over the window
+// bins (the per-bin spline / ln I0 work of one row in parallel instead of
+// serially in one wavefront), the waves' states merged through LDS.
+constexpr int kEpiWideThreads = 512;
+
+__global__ __launch_bounds__(kEpiWideThreads) void k_epi_time_wide(EpiArgs a) {
+  extern __shared__ double knots[];
+  __shared__ double s_in[4];
+  __shared__ int s_valid;
+  __shared__ double s_m[kEpiWideThreads / 64], s_s[kEpiWideThreads / 64];
+  const EvDev& ev = a.ev;
+  if (ev.flags & GWL_DISTANCE_MARG) stage_knots(ev, knots);
+  const long long r = blockIdx.x;
+  const int Gs = a.gstride > 0 ? a.gstride : a.G;
+  if (threadIdx.x == 0) {
+    double hh, jit, hs;
+    int valid;
+    epi_row_inputs(ev, a.params, a.ld, r, a.G, Gs, a.hhpart, a.rinfo, hh, jit, hs, valid);
+    s_in[0] = hh;
+    s_in[1] = jit;
+    s_in[2] = hs;
+    s_valid = valid;
   }
-  if (lane == 0) {
-    if (!first) {  // merge the slices before this one
-      const double2 st = lse[r];
-      const double mn = fmax(m, st.x);
-      if (mn > -INFINITY)
-        s = (m > -INFINITY ? s * exp(m - mn) : 0.0) + (st.x > -INFINITY ? st.y * exp(st.x - mn) : 0.0);
-      else
-        s = 0.0;
-      m = mn;
-    }
-    if (!last) {
-      lse[r] = make_double2(m, s);
-    } else {
-      const double b = time_lse_scale(ev, ev.tdelta);
-      out[r] = (m > -INFINITY) ? log(b * s) + m : -INFINITY;
-    }
+  __syncthreads();
+  if (!s_valid) {
+    if (threadIdx.x == 0) a.out[r] = row_valid_sentinel();
+    return;
+  }
+  const double hh = s_in[0], jit = s_in[1], hs = s_in[2];
+  SplineY yy;
+  if (ev.flags & GWL_DISTANCE_MARG) yy = spline_y(ev, knots + ev.sp.nx + 4, hh * hs * hs);
+  double m = -INFINITY, s = 0.0;
+  const double2* prow = a.part + (size_t)r * Gs * ev.L;
+  for (int jj = threadIdx.x; jj < ev.L; jj += kEpiWideThreads)
+    lse_push(m, s, epi_bin_value(ev, knots, yy, hh, hs, jit, prow, a.G, jj));
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) lse_merge(m, s, __shfl_xor(m, off, 64), __shfl_xor(s, off, 64));
+  if ((threadIdx.x & 63) == 0) {
+    s_m[threadIdx.x >> 6] = m;
+    s_s[threadIdx.x >> 6] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < kEpiWideThreads / 64; ++w) lse_merge(m, s, s_m[w], s_s[w]);
+    epi_finish(ev, r, m, s, a.out, a.lse, a.first, a.last);
   }
 }
 
@@ -584,7 +648,34 @@ __global__ __launch_bounds__(64 * kEpiRows) void k_epi_time(EpiArgs a) {
   const long long r = (long long)blockIdx.x * kEpiRows + (threadIdx.x >> 6);
   if (r >= a.n) return;
   epi_time_row(a.ev, a.params, a.ld, r, a.G, a.part, a.hhpart, a.out, knots, a.lse, a.first,
-               a.last, a.rinfo);
+               a.last, a.rinfo, a.gstride);
+}
+
+// Few rows with many groups: the group sums of each window bin (and of
+// <h|h>) by one thread per (row, bin), written over group 0, so that the
+// one-wavefront-per-row epilogue then reads one group instead of summing G
+// strided partials per bin serially.
+__global__ __launch_bounds__(256) void k_epi_gsum(double2* part, double* hhpart, long long n,
+                                                  int G, int L) {
+  const long long r = blockIdx.y;
+  const int jj = blockIdx.x * 256 + threadIdx.x;
+  if (r >= n) return;
+  if (jj < L) {
+    double2* p = part + (size_t)r * G * L + jj;
+    double xr = 0.0, xi = 0.0;
+#pragma unroll 8
+    for (int g = 0; g < G; ++g) {
+      const double2 v = p[(size_t)g * L];
+      xr += v.x;
+      xi += v.y;
+    }
+    p[0] = make_double2(xr, xi);
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    double hh = 0.0;
+    for (int g = 0; g < G; ++g) hh += hhpart[(size_t)r * G + g];
+    hhpart[(size_t)r * G] = hh;
+  }
 }
 
 // ------------------------------------------------------------------ multi-event
@@ -744,6 +835,9 @@ static int time_tile_groups(const EvDev& ev, long long n, int S, int n_cu, int w
   return G;
 }
 
+// rows up to which the epilogue sums the groups bin-parallel first (k_epi_gsum)
+constexpr int64_t kEpiGsumRows = 64;
+
 int launch_epi_time(gwl_event* e, const double* params, int64_t n, int ld, int G,
                     const double2* part, const double* hhpart, double* out, hipStream_t st,
                     const EvDev* slice, double2* lse, int first, int last, const double* rinfo) {
@@ -762,6 +856,20 @@ int launch_epi_time(gwl_event* e, const double* params, int64_t n, int ld, int G
   ea.last = last;
   ea.rinfo = rinfo;
   const size_t klds = (ev.flags & GWL_DISTANCE_MARG) ? (size_t)(ev.sp.nx + ev.sp.ny + 8) * sizeof(double) : 0;
+  // few rows: the groups summed bin-parallel first, then one workgroup per row
+  if (n <= kEpiGsumRows) {
+    if (G > 1) {
+      hipLaunchKernelGGL(k_epi_gsum, dim3((unsigned)((ev.L + 255) / 256), (unsigned)n), dim3(256),
+                         0, st, const_cast<double2*>(part), const_cast<double*>(hhpart),
+                         (long long)n, G, ev.L);
+      GWL_HIP_CHECK(hipGetLastError());
+      ea.G = 1;
+      ea.gstride = G;
+    }
+    hipLaunchKernelGGL(k_epi_time_wide, dim3((unsigned)n), dim3(kEpiWideThreads), klds, st, ea);
+    GWL_HIP_CHECK(hipGetLastError());
+    return GWL_OK;
+  }
   hipLaunchKernelGGL(k_epi_time, dim3((unsigned)((n + kEpiRows - 1) / kEpiRows)),
                      dim3(64 * kEpiRows), klds, st, ea);
   GWL_HIP_CHECK(hipGetLastError());
