@@ -268,11 +268,30 @@ def bench_uneven(steps):
         ref = np.array([olk.log_likelihood_ratio(p) for p in draws_as_dicts(d, idx)])
         omax = max(omax, float(np.max(np.abs(o[idx] - ref))))
     n_bins = [len(lk.interferometers.frequency_array) for lk in lks]
+    # roofline from the seeded event list: the algorithmic bytes of every
+    # row (float32 d/S and 1/S on each detector's masked bins, and the row),
+    # and the VALU issue of the launch from its counted instructions
+    abytes = sum(ev["n_rows"] * (sum(int(np.sum(np.asarray(i.frequency_mask)))
+                                     for i in lk.interferometers) * 12 + 72)
+                 for ev, lk in zip(events, lks))
+    valu = profile_figure("bench-valu-insts", "k_time_w_multi_uneven_64_events")
+    roof = {"kernel": "k_time_w_multi (persistent, all 64 events in one launch)",
+            "algorithmic_bytes_per_launch": abytes,
+            "hbm": {"achieved": abytes / dt / 1e9, "peak": 8000.0, "unit": "GB/s",
+                    "frac": abytes / dt / 1e9 / 8000.0,
+                    "note": "each event's data is L2/MALL-resident and re-read by its rows"}}
+    if valu:
+        roof.update(bound="valu", valu_insts_per_launch=valu,
+                    achieved=valu / dt / 1e9, peak=VALU_PEAK_GIPS,
+                    unit="G VALU wave-instructions/s", frac=valu / dt / 1e9 / VALU_PEAK_GIPS,
+                    valu_source="rocprofv3 SQ_INSTS_VALU per dispatch, profiles/ "
+                                "(bench-valu-insts k_time_w_multi_uneven_64_events); "
+                                "launch time of the graph replay in this run")
     return dict(events=len(events), rows=int(total_rows), ms_per_launch=dt * 1e3,
                 evals_per_s=total_rows / dt, bins_min=int(min(n_bins)), bins_max=int(max(n_bins)),
                 tile_steps_by_bins={str(k): int(v) for k, v in steps_of.items()},
                 tile_sweep_ms=sweep, fp32_vs_fp64_gpu_max_16_events=dmax,
-                fp32_vs_fp64_oracle_max_4_rows_x_64_events=omax, graph=True)
+                fp32_vs_fp64_oracle_max_4_rows_x_64_events=omax, graph=True, roofline=roof)
 
 
 def bench_relbin(steps):
