@@ -9,6 +9,7 @@ in libgwl's HIP kernels; there is no host fallback.
 """
 import copy
 import ctypes
+import threading
 import os
 
 import numpy as np
@@ -332,6 +333,8 @@ class GravitationalWaveTransient:
             self._setup_calibration_marginalization(calibration_lookup_table, priors)
             self._marginalized_parameters.append("recalib_index")
         self._one = None
+        self._stage = None                 # pinned host / device row buffers (_staged_eval)
+        self._stage_lock = threading.Lock()
 
     # ------------------------------------------------------------------ setup
     def _check_marginalized_prior_is_set(self, key):
@@ -992,6 +995,8 @@ class GravitationalWaveTransient:
                 rows = self.rows_from_parameters(parameters)
             else:
                 rows = np.ascontiguousarray(parameters, dtype=np.float64)
+            if stream is None and not return_inner_products:
+                return self._staged_eval(rows, prec)
             rows_t = torch.from_numpy(rows).to("cuda:%d" % self.device)
         out, aux = self._device_eval(rows_t, prec, aux=return_inner_products, stream=stream)
         if on_device:
@@ -1000,6 +1005,29 @@ class GravitationalWaveTransient:
         if return_inner_products:
             return out_h, aux.cpu().numpy()
         return out_h
+
+    def _staged_eval(self, rows, precision):
+        """Host rows -> host lnL through reused pinned buffers: one
+        asynchronous copy in, the evaluation, one copy out and one stream
+        synchronisation (the sampler pools' path: no pageable copies and no
+        allocations per call).  Calls from several threads take turns."""
+        n = rows.shape[0]
+        with self._stage_lock:
+            cap = 0 if self._stage is None else self._stage[0].shape[0]
+            if n > cap:
+                cap = max(n, 2 * cap, 1024)
+                dev = "cuda:%d" % self.device
+                self._stage = (torch.empty((cap, _lib.NPARAM), dtype=torch.float64).pin_memory(),
+                               torch.empty((cap, _lib.NPARAM), dtype=torch.float64, device=dev),
+                               torch.empty(cap, dtype=torch.float64).pin_memory())
+            h_in, d_in, h_out = self._stage
+            h_in.numpy()[:n] = rows
+            d_rows = d_in[:n]
+            d_rows.copy_(h_in[:n], non_blocking=True)
+            out, _ = self._device_eval(d_rows, precision)
+            h_out[:n].copy_(out, non_blocking=True)
+            torch.cuda.current_stream(d_in.device).synchronize()
+            return h_out.numpy()[:n].copy()
 
     def log_likelihood_ratio(self, parameters):
         """base.py:419-446: one parameter dict -> float (the caller's dict is
