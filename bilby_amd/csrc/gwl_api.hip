@@ -133,9 +133,93 @@ static int fft_launch(const gwl_ctx* ctx, const void* in, void* out, long long b
   return GWL_OK;
 }
 
+// Transforms longer than one LDS transform: P = Rr P2 (four-step).  The
+// radix-Rr step y[b][k1][n2] = W_P^(n2 k1) sum_n1 x[b][n2 + P2 n1] W_Rr^(n1 k1)
+// (float64 twiddles from the exact integer phase), P2-point LDS transforms
+// of the Rr sub-sequences, then X[b][k1 + Rr k2] = Y[b][k1][k2].
+template <typename R>
+__global__ __launch_bounds__(256) void k_fft_split_pre(const cplx<R>* in, cplx<R>* tmp,
+                                                       long long batch, int P, int Rr) {
+  using C = cplx<R>;
+  const int P2 = P / Rr;
+  const long long i = (long long)blockIdx.x * 256 + threadIdx.x;  // (b, n2)
+  if (i >= batch * P2) return;
+  const long long b = i / P2;
+  const int n2 = (int)(i - b * P2);
+  const C* x = in + b * P;
+  for (int k1 = 0; k1 < Rr; ++k1) {
+    double ar = 0.0, ai = 0.0;
+    for (int n1 = 0; n1 < Rr; ++n1) {
+      double sn, cs;
+      sincospi(-2.0 * (double)((n1 * k1) % Rr) / (double)Rr, &sn, &cs);
+      const C v = x[n2 + (long long)P2 * n1];
+      ar += (double)v.x * cs - (double)v.y * sn;
+      ai += (double)v.x * sn + (double)v.y * cs;
+    }
+    double sn, cs;
+    sincospi(-2.0 * (double)(((long long)n2 * k1) % P) / (double)P, &sn, &cs);
+    tmp[(b * Rr + k1) * P2 + n2] = cmk<C>((R)(ar * cs - ai * sn), (R)(ar * sn + ai * cs));
+  }
+}
+
+template <typename R>
+__global__ __launch_bounds__(256) void k_fft_split_post(const cplx<R>* tmp, cplx<R>* out,
+                                                        long long batch, int P, int Rr) {
+  const int P2 = P / Rr;
+  const long long i = (long long)blockIdx.x * 256 + threadIdx.x;  // output index
+  if (i >= batch * P) return;
+  const long long b = i / P;
+  const int k = (int)(i - b * P);
+  const int k1 = k % Rr, k2 = k / Rr;
+  out[i] = tmp[(b * Rr + k1) * P2 + k2];
+}
+
+template <typename R>
+static int fft_dispatch(const gwl_ctx* ctx, const void* in, void* out, long long batch, int P,
+                        hipStream_t st);
+
+// the longest power-of-two transform whose workgroup fits the LDS
+template <typename R>
+static int fft_lds_max(const gwl_ctx* ctx) {
+  int P = 16384;
+  while (P > 2 && (size_t)(P + P / 64 + 64) * sizeof(cplx<R>) > ctx->lds_per_block) P /= 2;
+  return P;
+}
+
+template <typename R>
+static int fft_split(const gwl_ctx* ctx, const void* in, void* out, long long batch, int P,
+                     hipStream_t st) {
+  const int P2 = fft_lds_max<R>(ctx);
+  const int Rr = P / P2;
+  if (P % P2 || Rr < 2 || Rr > 16) {
+    set_error("gwl_fft_batch: unsupported length");
+    return GWL_E_UNSUPPORTED;
+  }
+  void* tmp = nullptr;
+  const size_t bytes = (size_t)batch * P * sizeof(cplx<R>);
+  GWL_HIP_CHECK(hipMallocAsync(&tmp, bytes, st));
+  const unsigned g1 = (unsigned)((batch * P2 + 255) / 256), g2 = (unsigned)((batch * P + 255) / 256);
+  hipLaunchKernelGGL(k_fft_split_pre<R>, dim3(g1), dim3(256), 0, st,
+                     reinterpret_cast<const cplx<R>*>(in), reinterpret_cast<cplx<R>*>(tmp), batch,
+                     P, Rr);
+  hipError_t e = hipGetLastError();
+  int rc = e == hipSuccess ? GWL_OK : hip_fail(e, "k_fft_split_pre");
+  if (rc == GWL_OK) rc = fft_dispatch<R>(ctx, tmp, tmp, batch * Rr, P2, st);  // in place
+  if (rc == GWL_OK) {
+    hipLaunchKernelGGL(k_fft_split_post<R>, dim3(g2), dim3(256), 0, st,
+                       reinterpret_cast<const cplx<R>*>(tmp), reinterpret_cast<cplx<R>*>(out),
+                       batch, P, Rr);
+    e = hipGetLastError();
+    rc = e == hipSuccess ? GWL_OK : hip_fail(e, "k_fft_split_post");
+  }
+  (void)hipFreeAsync(tmp, st);
+  return rc;
+}
+
 template <typename R>
 static int fft_dispatch(const gwl_ctx* ctx, const void* in, void* out, long long batch, int P,
                         hipStream_t st) {
+  if (P > fft_lds_max<R>(ctx)) return fft_split<R>(ctx, in, out, batch, P, st);
   switch (P) {
     case 2: return fft_launch<R, 2>(ctx, in, out, batch, st);
     case 4: return fft_launch<R, 4>(ctx, in, out, batch, st);

@@ -297,10 +297,13 @@ int gwl_spline2d_eval_host(int32_t nx, int32_t ny, const double* tx, const doubl
                            const double* c, int64_t m, const double* x, const double* y,
                            double* out);
 
-/* Batched complex FFT of length P (power of two, 2..16384) with the LDS
+/* Batched complex FFT of length P (power of two, 2..16 x the longest LDS
+ * transform: 131072 in float64, 262144 in float32 on gfx950) with the LDS
  * radix kernel used by time marginalisation (numpy.fft.fft as called at
- * base.py:326).  in/out: device [batch][P][2]; precision selects float32
- * (GWL_FP32) or float64 data. */
+ * base.py:326); lengths past one workgroup's LDS run as a radix step over
+ * global memory and LDS transforms of the parts (a temporary of batch x P
+ * elements allocated on the stream).  in/out: device [batch][P][2];
+ * precision selects float32 (GWL_FP32) or float64 data. */
 int gwl_fft_batch(gwl_ctx* ctx, const void* in, void* out, int64_t batch, int32_t P,
                   int32_t precision, void* stream);
 
