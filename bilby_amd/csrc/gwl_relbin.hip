@@ -521,14 +521,14 @@ __device__ __forceinline__ void dft_reg(double2* a) {
   }
 }
 
-// One Stockham pass of radix R over P points in LDS (16 points per thread,
-// P / 16 threads): butterfly j reads x[j + q P / R], is twiddled by
+// One Stockham pass of radix R over P points in LDS (E points per thread,
+// P / E threads): butterfly j reads x[j + q P / R], is twiddled by
 // W_{Ns R}^{(j mod Ns) q}, transformed, and written to
 // x[(j - j mod Ns) R + j mod Ns + q Ns].  All reads finish before any write.
-template <int R, bool INV>
+template <int R, bool INV, int E>
 __device__ __forceinline__ void fft_pass(double2* x, int P, int Ns, const double2* tab, int tstep) {
-  constexpr int G = 16 / R;
-  const int T = P >> 4, PR = P / R;
+  constexpr int G = E / R;
+  const int T = P / E, PR = P / R;
   double2 v[G][R];
   int dst[G];
 #pragma unroll
@@ -562,14 +562,20 @@ __device__ __forceinline__ void fft_pass(double2* x, int P, int Ns, const double
 }
 
 // P-point transform in LDS, natural order in and out, unnormalised;
-// tab[k tstep] = exp(-2 pi i k / P).
-template <bool INV>
+// tab[k tstep] = exp(-2 pi i k / P).  E = 16: radix-16 passes; E = 8:
+// radix-8 passes (twice the threads for the same points); the remaining
+// bits in one pass of radix 2, 4 or 8.
+template <bool INV, int E = 16>
 __device__ void lds_fft(double2* x, int P, int logP, const double2* tab, int tstep) {
   int Ns = 1, bits = logP;
-  for (; bits >= 4; bits -= 4, Ns <<= 4) fft_pass<16, INV>(x, P, Ns, tab, tstep);
-  if (bits == 3) fft_pass<8, INV>(x, P, Ns, tab, tstep);
-  else if (bits == 2) fft_pass<4, INV>(x, P, Ns, tab, tstep);
-  else if (bits == 1) fft_pass<2, INV>(x, P, Ns, tab, tstep);
+  if constexpr (E == 16) {
+    for (; bits >= 4; bits -= 4, Ns <<= 4) fft_pass<16, INV, 16>(x, P, Ns, tab, tstep);
+  } else {
+    for (; bits >= 3; bits -= 3, Ns <<= 3) fft_pass<8, INV, 8>(x, P, Ns, tab, tstep);
+  }
+  if (bits == 3) fft_pass<8, INV, E>(x, P, Ns, tab, tstep);
+  else if (bits == 2) fft_pass<4, INV, E>(x, P, Ns, tab, tstep);
+  else if (bits == 1) fft_pass<2, INV, E>(x, P, Ns, tab, tstep);
 }
 
 struct MbFftArgs {
@@ -631,11 +637,13 @@ __device__ __forceinline__ void mb_block_sum(double s, double* red, double* dst)
   }
 }
 
-// One workgroup of M' / 16 threads per (row, detector): the whole band in
-// LDS (M' = P).
-__global__ __launch_bounds__(512) void k_mb_fft_lds(MbFftArgs a) {
+// One workgroup of M' / E threads per (row, detector): the whole band in
+// LDS (M' = P).  E = 8 points per thread (up to 1024 threads): four
+// wavefronts per SIMD where one 139 KB workgroup fills a CU.
+template <int E>
+__global__ __launch_bounds__(64 * 16 * 8 / E) void k_mb_fft_lds(MbFftArgs a) {
   extern __shared__ double2 xs[];
-  __shared__ double red[8];
+  __shared__ double red[16];
   const MbFftBand& bd = a.band;
   const long long prob = blockIdx.x;
   const long long r = prob / a.nd;
@@ -645,34 +653,34 @@ __global__ __launch_bounds__(512) void k_mb_fft_lds(MbFftArgs a) {
     if (threadIdx.x == 0) a.part[prob] = 0.0;
     return;
   }
-  const int P = bd.P, T = P >> 4;
+  const int P = bd.P, T = P / E;
   for (int i = threadIdx.x; i < P; i += T) xs[lpad(i)] = make_double2(0.0, 0.0);
   __syncthreads();
   for (int i = threadIdx.x; i < bd.L; i += T) xs[lpad(bd.ks + i)] = mb_point_strain(a, rr, d, i);
   __syncthreads();
-  double2 z[16];
+  double2 z[E];
 #pragma unroll
-  for (int g = 0; g < 16; ++g) {
+  for (int g = 0; g < E; ++g) {
     const int k = threadIdx.x + g * T;
     const double2 xm = k ? xs[lpad(P - k)] : make_double2(0.0, 0.0);  // X_{M'} = 0
     z[g] = mb_pack(xs[lpad(k)], xm, bd.tab[2 * k]);
   }
   __syncthreads();
 #pragma unroll
-  for (int g = 0; g < 16; ++g) xs[lpad(threadIdx.x + g * T)] = z[g];
+  for (int g = 0; g < E; ++g) xs[lpad(threadIdx.x + g * T)] = z[g];
   __syncthreads();
-  lds_fft<true>(xs, P, bd.logP, bd.tab, 4);
+  lds_fft<true, E>(xs, P, bd.logP, bd.tab, 4);
 #pragma unroll
-  for (int g = 0; g < 16; ++g) {
+  for (int g = 0; g < E; ++g) {
     const int p = threadIdx.x + g * T;
     xs[lpad(p)] = cmul(xs[lpad(p)], bd.tab[2 * p]);
   }
   __syncthreads();
-  lds_fft<false>(xs, P, bd.logP, bd.tab, 4);
+  lds_fft<false, E>(xs, P, bd.logP, bd.tab, 4);
   const double* odd = bd.odd + (size_t)d * bd.mh;
   double s = 0.0;
 #pragma unroll 4
-  for (int g = 0; g < 16; ++g) {
+  for (int g = 0; g < E; ++g) {
     const int j = threadIdx.x + g * T;
     s += mb_odd_term(xs[lpad(j)], xs[lpad(P - 1 - j)], bd.tab[2 * j + 1], odd[j]);
   }
@@ -869,6 +877,7 @@ static_assert(sizeof(RelRow) % sizeof(int) == 0, "RelRow holds whole ints");
 // their <h|h> in sums[3 r + 2]), added in band and detector order.
 constexpr long long kMbFftChunk = 1LL << 26;  // rows per launch of the LDS kernel
 constexpr int kMbFftSplitP = 4096;            // LDS transform of the split bands
+constexpr int kMbFftE = 8;                    // points per thread of the whole-band kernel
 
 size_t mb_fft_lds_bytes(int P) { return (size_t)(P + P / 16) * sizeof(double2); }
 
@@ -900,7 +909,7 @@ int mb_fft_launch(gwl_mb* mb, const RelRow* rows, long long n, double* sums, hip
         f.rows = rows + r0;
         f.n = std::min(kMbFftChunk, n - r0);
         f.part = pb + r0 * nd;
-        hipLaunchKernelGGL(k_mb_fft_lds, dim3((unsigned)(f.n * nd)), dim3(P / 16),
+        hipLaunchKernelGGL(k_mb_fft_lds<kMbFftE>, dim3((unsigned)(f.n * nd)), dim3(P / kMbFftE),
                            mb_fft_lds_bytes(P), st, f);
         GWL_HIP_CHECK(hipGetLastError());
       }
