@@ -32,6 +32,7 @@ struct RoqDev {
   int nd, L, Qn, NT, ntimes;
   int Kp, Np;  // GEMM K (2 L, multiple of 32) and N (2 nd NT padded to 128)
   double start, ts0, dts, tref, tcmin, dtc, tmin, tmax, sigma, fref, wf_fmin, dref;
+  double idts;  // 1 / dts
   unsigned flags;
   double tensor[GWL_MAX_DET][9];
   double vertex[GWL_MAX_DET][3];
@@ -448,8 +449,9 @@ __device__ inline void five_sample(const T* z, int base_c, double a, double& re,
     vr[k] = (double)v.x;
     vi[k] = (double)v.y;
   }
+  constexpr double kSixth = 1.0 / 6.0;
   const double b = 1.0 - a;
-  const double c = (a * a * a - a) / 6.0, d = (b * b * b - b) / 6.0;
+  const double c = (a * a * a - a) * kSixth, d = (b * b * b - b) * kSixth;
   const double r1r = (-vr[0] + 8.0 * vr[1] - 14.0 * vr[2] + 8.0 * vr[3] - vr[4]) / 4.0;
   const double r1i = (-vi[0] + 8.0 * vi[1] - 14.0 * vi[2] + 8.0 * vi[3] - vi[4]) / 4.0;
   const double r2r = vr[2] - 2.0 * vr[3] + vr[4], r2i = vi[2] - 2.0 * vi[3] + vi[4];
@@ -483,7 +485,7 @@ __device__ __forceinline__ bool roq_dh_at(const RoqDev& rd, const T* zr, const R
   for (int d = 0; d < GWL_MAX_DET; ++d) {
     if (d >= rd.nd) break;
     const double ifo_t = (toff + rr.dt[d]) + extra;
-    const double x = (ifo_t - rd.ts0) / rd.dts;
+    const double x = (ifo_t - rd.ts0) * rd.idts;
     const double cl = floor(x);
     const long long ci = (long long)cl;
     if (ci - 2 < 0 || ci + 2 >= rd.NT) return false;
@@ -737,6 +739,7 @@ int gwl_roq_create(gwl_ctx* ctx, const gwl_roq_desc* d, gwl_roq** out, int64_t* 
   rd.start = d->start_time;
   rd.ts0 = d->time_samples_start;
   rd.dts = d->time_samples_step;
+  rd.idts = 1.0 / rd.dts;
   rd.flags = d->flags;
   rd.fref = d->reference_frequency > 0 ? d->reference_frequency : 20.0;
   rd.wf_fmin = d->wf_minimum_frequency;
