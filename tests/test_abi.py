@@ -83,6 +83,7 @@ def test_new_entry_points_reject_null_handles_without_a_device():
                                                                                None),
         "gwl_window_time_series": lambda: LIB.gwl_window_time_series(None, None, 1 << 16, 0, 8,
                                                                      None, None),
+        "gwl_mb_set_fft": lambda: LIB.gwl_mb_set_fft(None, 0, None),
     }
     for name, call in calls.items():
         assert call() == -1, name
