@@ -169,22 +169,25 @@ class BatchedPool:
         kind = self._kind(func)
         if kind is None:
             return [func(x) for x in iterable]
-        points = [np.asarray(x, dtype=float).ravel() for x in iterable]
-        if not points:
+        # one array conversion of the whole queue: converting point by point
+        # costs more host time than the batched evaluation itself
+        thetas = np.asarray(iterable if isinstance(iterable, np.ndarray) else list(iterable),
+                            dtype=float)
+        if thetas.ndim == 0 or len(thetas) == 0:
             return []
+        thetas = thetas.reshape(len(thetas), -1)
         state = self._state_for(func, kind)
-        thetas = np.stack(points)
         self.batches += 1
-        self.points += len(points)
+        self.points += len(thetas)
         if kind == "logl":
             return evaluate_points(state, thetas).tolist()
         # LikePriorEvaluator.__call__ (base_sampler.py:1173-1185): the prior per
         # point; the likelihood only where the prior is finite (else 0)
-        lp = np.array([func.logp(x) for x in points], dtype=float)
+        lp = np.array([func.logp(x) for x in thetas], dtype=float)
         if np.isnan(lp).any():
             raise ValueError("Prior function returned NaN.")
         live = lp != -np.inf
-        ll = np.zeros(len(points))
+        ll = np.zeros(len(thetas))
         if live.any():
             ll[live] = evaluate_points(state, thetas[live])
             if np.isnan(ll).any():

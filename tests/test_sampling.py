@@ -68,9 +68,13 @@ def test_batched_pool_matches_per_point_wrapper(use_ratio):
     assert np.allclose(own.map(log_likelihood_wrapper, list(thetas)), expect, rtol=0, atol=1e-12)
     # other functions (a prior transform) are mapped point by point
     assert own.map(lambda t: priors.rescale(keys, t).tolist(), [[0.5, 0.0]]) == [[0.0, -5.0]]
+    assert np.allclose(own.map(log_likelihood_wrapper, (tuple(t) for t in thetas)), expect,
+                       rtol=0, atol=1e-12)
     assert own.map(log_likelihood_wrapper, []) == []
     with pytest.raises(ValueError):
         own.map(log_likelihood_wrapper, [[0.0, 1.0, 2.0]])
+    with pytest.raises(ValueError):  # a ragged queue
+        own.map(log_likelihood_wrapper, [[0.0, 1.0], [2.0]])
 
 
 def test_pool_reads_bilby_sampling_state():
