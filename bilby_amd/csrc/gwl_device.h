@@ -41,10 +41,58 @@ __device__ __forceinline__ float2 expm2pi(float r) {
   // v_sin_f32 / v_cos_f32 take their argument in revolutions.
   return make_float2(__builtin_amdgcn_cosf(r), -__builtin_amdgcn_sinf(r));
 }
+// float64: the nearest of the 64 points j / 64 of the circle from a table
+// (cos, sin of 2 pi j / 64, correctly rounded), times the Taylor series of
+// exp(-i theta) on |theta| <= pi / 64, whose truncation (theta^9 / 9! for
+// the sine, theta^10 / 10! for the cosine) stays below 5e-18: the table
+// method of P. T. P. Tang (ACM TOMS 15, 1989, 144-157) applied to the
+// circle.  r - j / 64 is exact, so the error is that of the few final
+// roundings (about 2 ulp).  Valid for |r| < 2^25.
+__constant__ __attribute__((aligned(16))) static const double kCis64[64][2] = {
+    {0x1.0000000000000p+0, 0x0.0p+0}, {0x1.fd88da3d12526p-1, 0x1.917a6bc29b42cp-4},
+    {0x1.f6297cff75cb0p-1, 0x1.8f8b83c69a60bp-3}, {0x1.e9f4156c62ddap-1, 0x1.294062ed59f06p-2},
+    {0x1.d906bcf328d46p-1, 0x1.87de2a6aea963p-2}, {0x1.c38b2f180bdb1p-1, 0x1.e2b5d3806f63bp-2},
+    {0x1.a9b66290ea1a3p-1, 0x1.1c73b39ae68c8p-1}, {0x1.8bc806b151741p-1, 0x1.44cf325091dd6p-1},
+    {0x1.6a09e667f3bcdp-1, 0x1.6a09e667f3bcdp-1}, {0x1.44cf325091dd6p-1, 0x1.8bc806b151741p-1},
+    {0x1.1c73b39ae68c8p-1, 0x1.a9b66290ea1a3p-1}, {0x1.e2b5d3806f63bp-2, 0x1.c38b2f180bdb1p-1},
+    {0x1.87de2a6aea963p-2, 0x1.d906bcf328d46p-1}, {0x1.294062ed59f06p-2, 0x1.e9f4156c62ddap-1},
+    {0x1.8f8b83c69a60bp-3, 0x1.f6297cff75cb0p-1}, {0x1.917a6bc29b42cp-4, 0x1.fd88da3d12526p-1},
+    {0x0.0p+0, 0x1.0000000000000p+0}, {-0x1.917a6bc29b42cp-4, 0x1.fd88da3d12526p-1},
+    {-0x1.8f8b83c69a60bp-3, 0x1.f6297cff75cb0p-1}, {-0x1.294062ed59f06p-2, 0x1.e9f4156c62ddap-1},
+    {-0x1.87de2a6aea963p-2, 0x1.d906bcf328d46p-1}, {-0x1.e2b5d3806f63bp-2, 0x1.c38b2f180bdb1p-1},
+    {-0x1.1c73b39ae68c8p-1, 0x1.a9b66290ea1a3p-1}, {-0x1.44cf325091dd6p-1, 0x1.8bc806b151741p-1},
+    {-0x1.6a09e667f3bcdp-1, 0x1.6a09e667f3bcdp-1}, {-0x1.8bc806b151741p-1, 0x1.44cf325091dd6p-1},
+    {-0x1.a9b66290ea1a3p-1, 0x1.1c73b39ae68c8p-1}, {-0x1.c38b2f180bdb1p-1, 0x1.e2b5d3806f63bp-2},
+    {-0x1.d906bcf328d46p-1, 0x1.87de2a6aea963p-2}, {-0x1.e9f4156c62ddap-1, 0x1.294062ed59f06p-2},
+    {-0x1.f6297cff75cb0p-1, 0x1.8f8b83c69a60bp-3}, {-0x1.fd88da3d12526p-1, 0x1.917a6bc29b42cp-4},
+    {-0x1.0000000000000p+0, 0x0.0p+0}, {-0x1.fd88da3d12526p-1, -0x1.917a6bc29b42cp-4},
+    {-0x1.f6297cff75cb0p-1, -0x1.8f8b83c69a60bp-3}, {-0x1.e9f4156c62ddap-1, -0x1.294062ed59f06p-2},
+    {-0x1.d906bcf328d46p-1, -0x1.87de2a6aea963p-2}, {-0x1.c38b2f180bdb1p-1, -0x1.e2b5d3806f63bp-2},
+    {-0x1.a9b66290ea1a3p-1, -0x1.1c73b39ae68c8p-1}, {-0x1.8bc806b151741p-1, -0x1.44cf325091dd6p-1},
+    {-0x1.6a09e667f3bcdp-1, -0x1.6a09e667f3bcdp-1}, {-0x1.44cf325091dd6p-1, -0x1.8bc806b151741p-1},
+    {-0x1.1c73b39ae68c8p-1, -0x1.a9b66290ea1a3p-1}, {-0x1.e2b5d3806f63bp-2, -0x1.c38b2f180bdb1p-1},
+    {-0x1.87de2a6aea963p-2, -0x1.d906bcf328d46p-1}, {-0x1.294062ed59f06p-2, -0x1.e9f4156c62ddap-1},
+    {-0x1.8f8b83c69a60bp-3, -0x1.f6297cff75cb0p-1}, {-0x1.917a6bc29b42cp-4, -0x1.fd88da3d12526p-1},
+    {0x0.0p+0, -0x1.0000000000000p+0}, {0x1.917a6bc29b42cp-4, -0x1.fd88da3d12526p-1},
+    {0x1.8f8b83c69a60bp-3, -0x1.f6297cff75cb0p-1}, {0x1.294062ed59f06p-2, -0x1.e9f4156c62ddap-1},
+    {0x1.87de2a6aea963p-2, -0x1.d906bcf328d46p-1}, {0x1.e2b5d3806f63bp-2, -0x1.c38b2f180bdb1p-1},
+    {0x1.1c73b39ae68c8p-1, -0x1.a9b66290ea1a3p-1}, {0x1.44cf325091dd6p-1, -0x1.8bc806b151741p-1},
+    {0x1.6a09e667f3bcdp-1, -0x1.6a09e667f3bcdp-1}, {0x1.8bc806b151741p-1, -0x1.44cf325091dd6p-1},
+    {0x1.a9b66290ea1a3p-1, -0x1.1c73b39ae68c8p-1}, {0x1.c38b2f180bdb1p-1, -0x1.e2b5d3806f63bp-2},
+    {0x1.d906bcf328d46p-1, -0x1.87de2a6aea963p-2}, {0x1.e9f4156c62ddap-1, -0x1.294062ed59f06p-2},
+    {0x1.f6297cff75cb0p-1, -0x1.8f8b83c69a60bp-3}, {0x1.fd88da3d12526p-1, -0x1.917a6bc29b42cp-4}
+};
 __device__ __forceinline__ double2 expm2pi(double r) {
-  double s, c;
-  sincospi(2.0 * r, &s, &c);
-  return make_double2(c, -s);
+  const double j = rint(r * 64.0);
+  const double t = fma(j, -0.015625, r);
+  const double th = t * kTwoPi;
+  const double t2 = th * th;
+  const double sn = fma(th * t2, fma(t2, fma(t2, -1.0 / 5040.0, 1.0 / 120.0), -1.0 / 6.0), th);
+  const double cs =
+      fma(t2, fma(t2, fma(t2, fma(t2, 1.0 / 40320.0, -1.0 / 720.0), 1.0 / 24.0), -0.5), 1.0);
+  const double2 w = *reinterpret_cast<const double2*>(kCis64[(int)j & 63]);
+  // (w.x - i w.y)(cs - i sn)
+  return make_double2(fma(w.x, cs, -w.y * sn), -fma(w.y, cs, w.x * sn));
 }
 
 // ------------------------------------------------------------ event (device view)
