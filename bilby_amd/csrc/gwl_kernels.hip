@@ -997,7 +997,11 @@ static int launch_reduce(gwl_event* e, const double* params, int64_t n, int ld, 
                          bool esc) {
   const EvDev& ev = e->dev;
   constexpr int NT = 256;
-  const int S = e->tile_rows >= 16 ? 16 : e->tile_rows >= 8 || e->tile_rows <= 0 ? 8 : 4;
+  // float64 tiles of more than 4 rows hold too many registers for more than
+  // one wavefront per SIMD; a row's sums do not depend on S
+  const int S = std::is_same<R, double>::value
+                    ? 4
+                    : e->tile_rows >= 16 ? 16 : e->tile_rows >= 8 || e->tile_rows <= 0 ? 8 : 4;
   const ReduceWs ws = reduce_ws(e, n);
   RedArgs ra;
   ra.ev = ev;
