@@ -64,6 +64,14 @@ struct RoqRow {
   int pad;
 };
 
+// TaylorF2 constants of a row for the node waveforms (k_roq_rows -> k_roq_prep)
+struct RoqTf2 {
+  double c[10];
+  double fisco;
+  int valid;
+  int pad;
+};
+
 }  // namespace gwl
 
 struct gwl_roq {
@@ -87,61 +95,69 @@ enum { ROQ_FP64 = 0, ROQ_FP32 = 1, ROQ_BF16X3 = 2 };
 // holds Re conj(h0(f_l)) = Re h0, column L + l holds Im conj(h0) = -Im h0,
 // where the detector response is h_d = C_d h0, h0 = f^-7/6 e^{-i Psi(f)} and
 // the amplitude prefactor lives in C_d.
+// One thread per row: the row's TaylorF2 constants, antenna patterns and
+// delays (what every node of the row shares), so that k_roq_prep's
+// workgroups start from them instead of one lane computing them while the
+// others wait.
+__global__ __launch_bounds__(128) void k_roq_rows(RoqDev rd, const double* params, int ld,
+                                                  long long n, RoqRow* rows, RoqTf2* tf2) {
+  const long long r = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  const double* row = params + r * (long long)ld;
+  RoqTf2 t;
+  double amp0;
+  const int valid = tf2_row_constants(row, rd.fref, rd.sigma, t.c, amp0, t.fisco);
+  t.valid = valid;
+  t.pad = 0;
+  // antenna and delay at the reference time when time-marginalised
+  // (roq.py:475-478), else at geocent_time
+  const bool timem = (rd.flags & GWL_TIME_MARG) != 0;
+  const bool jit = timem && (rd.flags & GWL_JITTER_TIME);
+  const double jitter = jit ? row[GWL_P_TIME_JITTER] : 0.0;
+  const double gt = row[GWL_P_GEOCENT_TIME] + jitter;  // base.py:427-428
+  const double tref = timem ? rd.tref : gt;
+  const double ci = cos(row[GWL_P_THETA_JN]);
+  const double ap = amp0 * (1.0 + ci * ci) / 2.0, ax = amp0 * ci;
+  RoqRow rr;
+  const SkyBasis sky = sky_basis(row[GWL_P_RA], row[GWL_P_DEC], tref, row[GWL_P_PSI]);
+  for (int d = 0; d < GWL_MAX_DET; ++d) rr.cre[d] = rr.cim[d] = rr.dt[d] = 0.0;
+  for (int d = 0; d < rd.nd; ++d) {
+    double fp, fc, delay;
+    detector_projection(rd.tensor[d], rd.vertex[d], sky, fp, fc, delay);
+    // C_d = F+ A+ - i Fx Ax; store conj(C_d)
+    rr.cre[d] = fp * ap;
+    rr.cim[d] = fc * ax;
+    rr.dt[d] = delay;
+  }
+  rr.dtg = gt - rd.start;
+  rr.jitter = jitter;
+  rr.hscale = (rd.flags & GWL_DISTANCE_MARG) ? row[GWL_P_LUMINOSITY_DISTANCE] / rd.dref : 1.0;
+  rr.valid = valid;
+  rr.pad = 0;
+  rr.hh = 0.0;
+  rows[r] = rr;
+  tf2[r] = t;
+}
+
 template <int MODE>
-__global__ __launch_bounds__(128) void k_roq_prep(RoqDev rd, const double* params, int ld,
-                                                   long long n, void* aout, void* aout_lo,
-                                                   RoqRow* rows) {
+__global__ __launch_bounds__(128) void k_roq_prep(RoqDev rd, const RoqTf2* tf2, long long n,
+                                                   void* aout, void* aout_lo, RoqRow* rows) {
   const long long r = blockIdx.x;
   __shared__ double c[10];
-  __shared__ double s_amp0, s_fisco;
+  __shared__ double s_fisco;
   __shared__ int s_valid;
   __shared__ double hpart[GWL_MAX_DET][2];
   const int tid = threadIdx.x;
   const bool live = r < n;
+  if (tid < 10) c[tid] = live ? tf2[r].c[tid] : 0.0;
   if (tid == 0) {
-    if (live) {
-      const double* row = params + r * (long long)ld;
-      double amp0, fisco;
-      int valid = tf2_row_constants(row, rd.fref, rd.sigma, c, amp0, fisco);
-      s_amp0 = amp0;
-      s_fisco = fisco;
-      s_valid = valid;
-      // antenna and delay at the reference time when time-marginalised
-      // (roq.py:475-478), else at geocent_time
-      const bool timem = (rd.flags & GWL_TIME_MARG) != 0;
-      const bool jit = timem && (rd.flags & GWL_JITTER_TIME);
-      const double jitter = jit ? row[GWL_P_TIME_JITTER] : 0.0;
-      const double gt = row[GWL_P_GEOCENT_TIME] + jitter;  // base.py:427-428
-      const double tref = timem ? rd.tref : gt;
-      const double ci = cos(row[GWL_P_THETA_JN]);
-      const double ap = amp0 * (1.0 + ci * ci) / 2.0, ax = amp0 * ci;
-      RoqRow rr;
-      const SkyBasis sky = sky_basis(row[GWL_P_RA], row[GWL_P_DEC], tref, row[GWL_P_PSI]);
-      for (int d = 0; d < rd.nd; ++d) {
-        double fp, fc, delay;
-        detector_projection(rd.tensor[d], rd.vertex[d], sky, fp, fc, delay);
-        // C_d = F+ A+ - i Fx Ax; store conj(C_d)
-        rr.cre[d] = fp * ap;
-        rr.cim[d] = fc * ax;
-        rr.dt[d] = delay;
-      }
-      rr.dtg = gt - rd.start;
-      rr.jitter = jitter;
-      rr.hscale = (rd.flags & GWL_DISTANCE_MARG) ? row[GWL_P_LUMINOSITY_DISTANCE] / rd.dref : 1.0;
-      rr.valid = valid;
-      rr.hh = 0.0;
-      rows[r] = rr;
-    } else {
-      s_valid = 0;
-      s_amp0 = 0.0;
-      s_fisco = 0.0;
-    }
+    s_fisco = live ? tf2[r].fisco : 0.0;
+    s_valid = live ? tf2[r].valid : 0;
     for (int d = 0; d < GWL_MAX_DET; ++d) hpart[d][0] = hpart[d][1] = 0.0;
   }
   __syncthreads();
   const bool valid = live && s_valid;
-  const double amp0 = s_amp0, fisco = s_fisco;
-  (void)amp0;
+  const double fisco = s_fisco;
   // lalsimulation evaluates a frequency-sequence TaylorF2 at every node
   // (source.py:1125-1128); the waveform band and the ISCO cut the nodes only
   // when the likelihood asks for ISCO termination
@@ -166,10 +182,9 @@ __global__ __launch_bounds__(128) void k_roq_prep(RoqDev rd, const double* param
         const double cyc = tf2_phase_cycles(c, b);
         const double amp = b[kNB];  // f^-7/6; amp0 sits in C_d
         if (MODE == ROQ_FP64) {
-          double sn, cs;
-          sincospi(2.0 * (cyc - floor(cyc)), &sn, &cs);
-          xr = amp * cs;   // Re conj(h0) = Re h0
-          xi = amp * sn;   // Im conj(h0) = -Im h0 = +amp sin
+          const double2 e = expm2pi(cyc - floor(cyc));
+          xr = amp * e.x;   // Re conj(h0) = Re h0
+          xi = -amp * e.y;  // Im conj(h0) = -Im h0 = +amp sin
         } else {
           // float32 operand: the rotation in float32 from the reduced phase
           const float2 e = expm2pi((float)(cyc - floor(cyc)));
@@ -640,7 +655,7 @@ static size_t roq_ws_bytes(const RoqDev& rd, int64_t n, int mode) {
   const size_t Mp = (size_t)((n + kBM - 1) / kBM) * kBM;
   const size_t a = mode == ROQ_FP64 ? Mp * rd.Kp * 8 : Mp * rd.Kp * 4;  // bf16: two halves
   const size_t c = mode == ROQ_FP64 ? Mp * rd.Np * 8 : Mp * rd.Np * 4;
-  return a + c + (size_t)n * sizeof(RoqRow) + 3 * 256;
+  return a + c + (size_t)n * sizeof(RoqRow) + (size_t)n * sizeof(RoqTf2) + 4 * 256;
 }
 
 // C[z][M][N] float32 partials of Ah/Al . Bh/Bl^T (split bfloat16 x3) over the
@@ -687,13 +702,17 @@ static int roq_prep_gemm(gwl_roq* e, const double* params, int64_t n, int32_t ld
   char* Cp = p + (abytes + 255) / 256 * 256;
   const size_t cbytes = (mode == ROQ_FP64 ? (size_t)Mp * rd.Np * 8 : (size_t)Mp * rd.Np * 4);
   RoqRow* rows = reinterpret_cast<RoqRow*>(Cp + (cbytes + 255) / 256 * 256);
+  RoqTf2* tf2 = reinterpret_cast<RoqTf2*>(reinterpret_cast<char*>(rows) +
+                                          ((size_t)n * sizeof(RoqRow) + 255) / 256 * 256);
+  hipLaunchKernelGGL(k_roq_rows, dim3((unsigned)((n + 127) / 128)), dim3(128), 0, st, rd, params,
+                     ld, (long long)n, rows, tf2);
   // one block per operand row: rows past n and padding columns are written as zeros
   if (mode == ROQ_FP64)
-    hipLaunchKernelGGL(k_roq_prep<ROQ_FP64>, dim3((unsigned)Mp), dim3(128), 0, st, rd, params, ld, n, A, nullptr, rows);
+    hipLaunchKernelGGL(k_roq_prep<ROQ_FP64>, dim3((unsigned)Mp), dim3(128), 0, st, rd, tf2, n, A, nullptr, rows);
   else if (mode == ROQ_FP32)
-    hipLaunchKernelGGL(k_roq_prep<ROQ_FP32>, dim3((unsigned)Mp), dim3(128), 0, st, rd, params, ld, n, A, nullptr, rows);
+    hipLaunchKernelGGL(k_roq_prep<ROQ_FP32>, dim3((unsigned)Mp), dim3(128), 0, st, rd, tf2, n, A, nullptr, rows);
   else
-    hipLaunchKernelGGL(k_roq_prep<ROQ_BF16X3>, dim3((unsigned)Mp), dim3(128), 0, st, rd, params, ld, n, A, Alo, rows);
+    hipLaunchKernelGGL(k_roq_prep<ROQ_BF16X3>, dim3((unsigned)Mp), dim3(128), 0, st, rd, tf2, n, A, Alo, rows);
   GWL_HIP_CHECK(hipGetLastError());
   if (mode == ROQ_FP64) {
     hipLaunchKernelGGL(k_gemm_f64, dim3(rd.Np / 64, Mp / 64), dim3(256), 0, st,
