@@ -65,6 +65,38 @@ def test_gemm_modes_match_numpy(mode):
     assert err < tol, err
 
 
+def test_gemm_bf16x3_wide_tiles_match_narrow_bit_for_bit():
+    """N a multiple of 256 runs the 128 x 256-tile split-bfloat16 kernel, other
+    N the 128 x 128 one; an output element's products and their order are the
+    same in both, so the first 384 columns of an N = 512 product equal the
+    N = 384 product bit for bit (and numpy within 3e-5)."""
+    import torch
+    from bilby_amd._lib import LIB, check
+    from bilby_amd.likelihood import get_context
+    ctx = get_context(0)
+    say("gemm bf16x3 wide vs narrow tiles")
+    rng = np.random.default_rng(11)
+    M, K = 384, 160
+    A = rng.normal(size=(M, K))
+    Bt = rng.normal(size=(512, K))
+    dev = torch.device("cuda:0")
+    ah, al = (t.to(dev) for t in _split_bf16(A))
+    outs = {}
+    for N in (512, 384):
+        bh, bl = (t.to(dev).contiguous() for t in _split_bf16(Bt[:N]))
+        c = torch.empty((M, N), dtype=torch.float32, device=dev)
+        check(LIB.gwl_gemm(ctx, 2, ctypes.c_void_p(ah.data_ptr()), ctypes.c_void_p(al.data_ptr()),
+                           ctypes.c_void_p(bh.data_ptr()), ctypes.c_void_p(bl.data_ptr()),
+                           ctypes.c_void_p(c.data_ptr()), M, N, K, None))
+        torch.cuda.synchronize()
+        outs[N] = c.cpu().numpy()
+    assert np.array_equal(outs[512][:, :384], outs[384])
+    ref = A @ Bt.T
+    err = np.linalg.norm(outs[512].astype(np.float64) - ref) / np.linalg.norm(ref)
+    say("gemm wide err", err)
+    assert err < 3e-5, err
+
+
 def _roq_setup():
     from bilby_amd.configs import make_roq
     return make_roq(n_draws=128, seed=77)
