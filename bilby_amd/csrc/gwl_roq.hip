@@ -405,7 +405,7 @@ __global__ __launch_bounds__(256) void k_gemm_bf16x3_w(const __bf16* __restrict_
                                                        const __bf16* __restrict__ Bh,
                                                        const __bf16* __restrict__ Bl,
                                                        float* __restrict__ C, int M, int N,
-                                                       int K) {
+                                                       int K, int kcol) {
   __shared__ __attribute__((aligned(16))) __bf16 sAh[kBM][kBK + kPadH];
   __shared__ __attribute__((aligned(16))) __bf16 sAl[kBM][kBK + kPadH];
   __shared__ __attribute__((aligned(16))) __bf16 sBh[kBNW][kBK + kPadH];
@@ -420,7 +420,10 @@ __global__ __launch_bounds__(256) void k_gemm_bf16x3_w(const __bf16* __restrict_
     for (int j = 0; j < 4; ++j)
 #pragma unroll
       for (int v = 0; v < 16; ++v) acc[i][j][v] = 0.f;
-  for (int k0 = 0; k0 < K; k0 += kBK) {
+  // K chunk blockIdx.z (columns [z kcol, (z + 1) kcol)) into partial plane z
+  const int kb = blockIdx.z * kcol, ke = min(K, kb + kcol);
+  C += (size_t)blockIdx.z * M * N;
+  for (int k0 = kb; k0 < ke; k0 += kBK) {
     // A: 128 rows x 32 bf16 = 512 chunks of 8 per operand (2 per thread);
     // B: 256 rows (4 per thread)
 #pragma unroll
@@ -754,8 +757,14 @@ int gwl::gemm_bf16x3_split(const __bf16* Ah, const __bf16* Al, const __bf16* Bh,
     return GWL_E_INVALID;
   }
   const int nz = (K + kcol - 1) / kcol;
-  hipLaunchKernelGGL(k_gemm_bf16x3, dim3((unsigned)(N / kBN), (unsigned)(M / kBM), (unsigned)nz),
-                     dim3(256), 0, st, Ah, Al, Bh, Bl, C, M, N, K, kcol);
+  if (N % kBNW == 0)
+    hipLaunchKernelGGL(k_gemm_bf16x3_w,
+                       dim3((unsigned)(N / kBNW), (unsigned)(M / kBM), (unsigned)nz), dim3(256), 0,
+                       st, Ah, Al, Bh, Bl, C, M, N, K, kcol);
+  else
+    hipLaunchKernelGGL(k_gemm_bf16x3,
+                       dim3((unsigned)(N / kBN), (unsigned)(M / kBM), (unsigned)nz), dim3(256), 0,
+                       st, Ah, Al, Bh, Bl, C, M, N, K, kcol);
   GWL_HIP_CHECK(hipGetLastError());
   return GWL_OK;
 }
@@ -810,7 +819,7 @@ static int roq_prep_gemm(gwl_roq* e, const double* params, int64_t n, int32_t ld
   } else if (rd.Np % kBNW == 0) {
     hipLaunchKernelGGL(k_gemm_bf16x3_w, dim3(rd.Np / kBNW, Mp / kBM), dim3(256), 0, st,
                        reinterpret_cast<const __bf16*>(A), reinterpret_cast<const __bf16*>(Alo),
-                       rd.bth, rd.btl, reinterpret_cast<float*>(Cp), Mp, rd.Np, rd.Kp);
+                       rd.bth, rd.btl, reinterpret_cast<float*>(Cp), Mp, rd.Np, rd.Kp, rd.Kp);
   } else {
     hipLaunchKernelGGL(k_gemm_bf16x3, dim3(rd.Np / kBN, Mp / kBM), dim3(256), 0, st,
                        reinterpret_cast<const __bf16*>(A), reinterpret_cast<const __bf16*>(Alo),
@@ -1125,7 +1134,7 @@ int gwl_gemm(gwl_ctx* ctx, int32_t mode, const void* A, const void* A_lo, const 
   else if (N % kBNW == 0)
     hipLaunchKernelGGL(k_gemm_bf16x3_w, dim3(N / kBNW, M / kBM), dim3(256), 0, st,
                        (const __bf16*)A, (const __bf16*)A_lo, (const __bf16*)Bt,
-                       (const __bf16*)Bt_lo, (float*)C, M, N, K);
+                       (const __bf16*)Bt_lo, (float*)C, M, N, K, K);
   else
     hipLaunchKernelGGL(k_gemm_bf16x3, dim3(N / kBN, M / kBM), dim3(256), 0, st,
                        (const __bf16*)A, (const __bf16*)A_lo, (const __bf16*)Bt,
