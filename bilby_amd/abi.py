@@ -49,13 +49,18 @@ class RowBuilder:
             p.setdefault(key, value)
         if self.frame is not None:
             p = self.frame(p)
+        # Python / numpy float64 scalars skip numpy's array dispatch: one
+        # sampler dict is converted per call on the per-sample path
         n = 1
         for v in p.values():
+            if isinstance(v, (float, int)):
+                continue
             if np.ndim(v) > 0:
                 n = len(v)
                 break
         rows = np.zeros((n, NPARAM))
         for key, col in PARAM_COLUMNS.items():
             if key in p:
-                rows[:, col] = np.asarray(p[key], dtype=float)
+                v = p[key]
+                rows[:, col] = v if isinstance(v, (float, int)) else np.asarray(v, dtype=float)
         return rows

@@ -236,3 +236,33 @@ def test_row_builder_columns_and_defaults():
     assert np.all(rows[:, abi.P_GEOCENT_TIME] == 7.0)
     assert np.all(rows[:, abi.P_PHASE] == [1.0, 2.0])
     assert np.all(rows[:, abi.P_RA] == 1.0) and np.all(rows[:, abi.P_CHI_2] == 0.0)
+
+
+def test_row_builder_scalar_and_array_inputs_agree():
+    """One dict of Python floats, numpy float64 / float32 scalars, ints, or
+    1-element arrays gives the same row, and a dict of arrays gives the rows of
+    its points one by one."""
+    from bilby_amd import abi
+    from bilby_amd.waveform_generator import convert_to_lal_binary_black_hole_parameters
+    rb = abi.RowBuilder(convert_to_lal_binary_black_hole_parameters, {"geocent_time": 5.0})
+    rng = np.random.default_rng(2)
+    n = 7
+    arrays = dict(mass_1=rng.uniform(20, 40, n), mass_2=rng.uniform(5, 20, n),
+                  chi_1=rng.uniform(-0.5, 0.5, n), chi_2=np.zeros(n),
+                  luminosity_distance=rng.uniform(100, 900, n), theta_jn=rng.uniform(0, 3, n),
+                  phase=rng.uniform(0, 6, n), ra=rng.uniform(0, 6, n), dec=rng.uniform(-1, 1, n),
+                  psi=rng.uniform(0, 3, n), geocent_time=np.full(n, 1.25))
+    rows = rb(arrays)
+    assert rows.shape == (n, abi.NPARAM)
+    for i in range(n):
+        point = {k: float(v[i]) for k, v in arrays.items()}
+        np.testing.assert_array_equal(rb(point), rows[i:i + 1])
+        np.testing.assert_array_equal(rb({k: np.float64(v) for k, v in point.items()}),
+                                      rows[i:i + 1])
+        np.testing.assert_array_equal(rb({k: np.array([v]) for k, v in point.items()}),
+                                      rows[i:i + 1])
+    f32 = {k: np.float32(v[0]) for k, v in arrays.items()}
+    np.testing.assert_array_equal(rb(f32), rb({k: float(v) for k, v in f32.items()}))
+    mixed = {k: float(v[0]) for k, v in arrays.items()}
+    mixed["geocent_time"] = 2
+    assert rb(mixed)[0, abi.PARAM_COLUMNS["geocent_time"]] == 2.0
