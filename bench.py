@@ -819,6 +819,13 @@ def bench_sampler(lk, draws, seconds, workers=16, queue=1024):
     out = {"batched_pool_queue": queue,
            "batched_pool_evals_per_s": n_calls * queue / t_pool,
            "batched_pool_ms_per_queue": 1e3 * t_pool / n_calls}
+    # the same queue size as device-resident rows (no host conversion or copies)
+    import torch
+    rows_t = torch.from_numpy(lk.rows_from_parameters(
+        {k: np.asarray(v)[:queue] for k, v in draws.items()})).cuda()
+    dt = timed(lambda: lk.log_likelihood_ratio_batch(rows_t), 50)
+    out["device_rows_same_queue_evals_per_s"] = queue / dt
+    out["batched_pool_over_device_rows"] = out["batched_pool_evals_per_s"] * dt / queue
     one = {k: np.asarray(v)[:1] for k, v in draws.items()}
     with LikelihoodService({"c2": lk}, hbm_budget=200e9, max_batch_rows=4096) as svc:
         client = svc.client("c2")
