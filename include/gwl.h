@@ -202,8 +202,9 @@ int gwl_event_time_window(gwl_event* ev, int64_t* j_first, int32_t* n_window, do
  * used to split the time-marginalised transform into groups.  With
  * rows_per_workgroup = 0 a TaylorF2 event whose time window fits a 512-point
  * transform uses the wave-per-row transform, and workgroups_per_cu is the
- * target wavefronts per CU (default 16).  Used by the autotuner; no
- * counterpart in the reference. */
+ * target wavefronts per CU (default 16).  Float64 evaluations without time
+ * marginalisation use 4 rows per workgroup whatever is set here.  Used by the
+ * autotuner; no counterpart in the reference. */
 int gwl_event_set_tile(gwl_event* ev, int32_t rows_per_workgroup, int32_t workgroups_per_cu);
 
 /* This event's tile inside multi-event launches: decimated sub-sequence
