@@ -7,7 +7,7 @@ import numpy as np
 
 from . import constants as C
 
-# bilby/gw/detector/detectors/{H1,L1,V1}.interferometer
+# bilby/gw/detector/detectors/{H1,L1,V1,K1}.interferometer
 DETECTOR_CONSTANTS = {
     "H1": dict(length=4, latitude=46 + 27. / 60 + 18.528 / 3600,
                longitude=-(119 + 24. / 60 + 27.5657 / 3600), elevation=142.554,
@@ -23,6 +23,11 @@ DETECTOR_CONSTANTS = {
                longitude=10 + 30. / 60 + 16.1878 / 3600, elevation=51.884,
                xarm_azimuth=70.5674, yarm_azimuth=160.5674,
                xarm_tilt=0.0, yarm_tilt=0.0,
+               minimum_frequency=20, maximum_frequency=2048),
+    "K1": dict(length=3, latitude=36 + 24 / 60 + 42.69722 / 3600,
+               longitude=137 + 18 / 60 + 21.44171 / 3600, elevation=414.181,
+               xarm_azimuth=90 - 60.39623489157727, yarm_azimuth=90 + 29.60357629670688,
+               xarm_tilt=0.0031414, yarm_tilt=-0.0036270,
                minimum_frequency=20, maximum_frequency=2048),
 }
 
@@ -240,12 +245,14 @@ class Interferometer:
 
 
 def get_empty_interferometer(name, power_spectral_density=None):
-    """bilby/gw/detector/networks.py:420-456 for the detectors H1, L1, V1."""
+    """bilby/gw/detector/networks.py:420-456 for the detectors H1, L1, V1, K1
+    (bilby reads K1's design PSD from a file; here, without a PSD given, K1
+    takes the analytic aLIGO curve scaled by 4 and V1 by 2)."""
     if name not in DETECTOR_CONSTANTS:
         raise ValueError("Interferometer {} not implemented".format(name))
     kw = dict(DETECTOR_CONSTANTS[name])
     if power_spectral_density is None:
-        power_spectral_density = aligo_analytic_psd(2.0 if name == "V1" else 1.0)
+        power_spectral_density = aligo_analytic_psd({"V1": 2.0, "K1": 4.0}.get(name, 1.0))
     return Interferometer(name=name, power_spectral_density=power_spectral_density, **kw)
 
 

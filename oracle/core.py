@@ -79,7 +79,7 @@ def frequency_mask(frequency_array, minimum_frequency, maximum_frequency, sampli
 
 
 # ---------------------------------------------------------------- geometry
-# bilby/gw/detector/detectors/{H1,L1,V1}.interferometer
+# bilby/gw/detector/detectors/{H1,L1,V1,K1}.interferometer
 DETECTORS = {
     "H1": dict(latitude=46 + 27. / 60 + 18.528 / 3600,
                longitude=-(119 + 24. / 60 + 27.5657 / 3600), elevation=142.554,
@@ -95,6 +95,11 @@ DETECTORS = {
                longitude=10 + 30. / 60 + 16.1878 / 3600, elevation=51.884,
                xarm_azimuth=70.5674, yarm_azimuth=160.5674,
                xarm_tilt=0.0, yarm_tilt=0.0,
+               minimum_frequency=20, maximum_frequency=2048),
+    "K1": dict(latitude=36 + 24 / 60 + 42.69722 / 3600,
+               longitude=137 + 18 / 60 + 21.44171 / 3600, elevation=414.181,
+               xarm_azimuth=90 - 60.39623489157727, yarm_azimuth=90 + 29.60357629670688,
+               xarm_tilt=0.0031414, yarm_tilt=-0.0036270,
                minimum_frequency=20, maximum_frequency=2048),
 }
 

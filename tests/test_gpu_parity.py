@@ -350,3 +350,30 @@ def test_alone_equals_in_batch():
         for i in (0, 17, 89):
             one = lk.log_likelihood_ratio_batch({k: v[i:i + 1] for k, v in d.items()})
             assert abs(one[0] - batch[i]) <= tol, (flags, i, one[0], batch[i])
+
+
+@pytest.mark.parametrize("time", [True, False])
+def test_four_detector_network_matches_oracle(time):
+    """H1+L1+V1+K1 (the kernels' largest detector count), TaylorF2 4 s at
+    2048 Hz, distance + phase with and without time: |dlnL| < 1e-6 (fp64)
+    and < 1e-3 (fp32) against the FP64 oracle."""
+    from bilby_amd.likelihood import GravitationalWaveTransient
+    from tests.helpers import oracle_from_setup, draws_as_dicts, fixed_marginalised
+    say("four-detector network, time", time)
+    setup = _tf2_setup(duration=4.0, sampling_frequency=2048.0,
+                       detectors=("H1", "L1", "V1", "K1"), time=time)
+    olk, _ = oracle_from_setup(setup)
+    priors = setup["priors"].copy()
+    lk = GravitationalWaveTransient(setup["interferometers"], setup["waveform_generator"],
+                                    time_marginalization=time, distance_marginalization=True,
+                                    phase_marginalization=True, priors=priors, precision="fp32")
+    draws = fixed_marginalised(setup["draws"], priors)
+    n = 24
+    ref = np.array([olk.log_likelihood_ratio(p) for p in draws_as_dicts(draws, range(n))])
+    d = {k: v[:n] for k, v in draws.items()}
+    out32 = lk.log_likelihood_ratio_batch(d, precision="fp32")
+    out64 = lk.log_likelihood_ratio_batch(d, precision="fp64")
+    say("fp64 max", np.max(np.abs(out64 - ref)), "fp32 max", np.max(np.abs(out32 - ref)))
+    assert np.all(np.isfinite(ref))
+    assert np.all(np.abs(out64 - ref) < 1e-6)
+    assert np.all(np.abs(out32 - ref) < 1e-3)

@@ -122,7 +122,7 @@ def test_detector_tensor_trace_free_and_product_agrees():
     """D is symmetric and trace-free; the product's host geometry equals the
     oracle's."""
     from bilby_amd.detector import get_empty_interferometer
-    for name in ("H1", "L1", "V1"):
+    for name in ("H1", "L1", "V1", "K1"):
         t, v = core.detector_geometry(name)
         assert abs(np.trace(t)) < 1e-12 and np.allclose(t, t.T)
         ifo = get_empty_interferometer(name)
@@ -541,7 +541,7 @@ def test_time_delay_sign_along_the_vertex():
     """A source in the direction of a detector's vertex reaches the detector
     |r|/c before the geocentre (time_delay_from_geocenter = -|r|/c); from the
     opposite direction, |r|/c after it."""
-    for name in ("H1", "L1", "V1"):
+    for name in ("H1", "L1", "V1", "K1"):
         _, vertex = core.detector_geometry(name)
         r = np.linalg.norm(vertex)
         for t in (1e9, 1.3e9):
